@@ -1,0 +1,225 @@
+// ORACLE — test infrastructure only. The C ABI through which tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg drive the CPU restatement of the reference renderer. The product
+// (repackshot_amd/) never loads this library.
+//
+// The film entry point follows the reference host's schedule (rt-cli/src/lib.rs:266-366): 32-pixel
+// tiles claimed from one atomic counter by `threads` workers; each pixel's samples are added to
+// its f32 film sums in sample-index order (image.rs:81-94), so the film does not depend on the
+// number of threads.
+#include <atomic>
+#include <cstring>
+#include <thread>
+#include "rt_render.hpp"
+
+using namespace orc;
+
+namespace {
+struct Handle {
+  Scene scene;
+  Prepared prep;
+};
+void set_err(char* err, size_t cap, const std::string& msg) {
+  if (!err || cap == 0) return;
+  size_t n = std::min(cap - 1, msg.size());
+  std::memcpy(err, msg.data(), n);
+  err[n] = 0;
+}
+}  // namespace
+
+extern "C" {
+
+void orc_set_energy_cache(const char* path) { energy::set_cache_path(path); }
+
+void* orc_scene_load(const uint8_t* bytes, size_t len, char* err, size_t err_cap) {
+  auto* h = new Handle();
+  std::string e;
+  if (!scene_from_rsb(bytes, len, h->scene, e) || !h->prep.build(h->scene, e)) {
+    set_err(err, err_cap, e);
+    delete h;
+    return nullptr;
+  }
+  return h;
+}
+
+void orc_scene_free(void* p) { delete static_cast<Handle*>(p); }
+
+// width, height, max_samples, seed, depth
+void orc_scene_info(void* p, uint32_t out[5]) {
+  const Scene& s = static_cast<Handle*>(p)->scene;
+  out[0] = s.render.width;
+  out[1] = s.render.height;
+  out[2] = s.render.max_samples;
+  out[3] = s.render.seed;
+  out[4] = s.render.depth();
+}
+
+// Geometry counts: meshes, instances, triangles (each mesh once), wide nodes, packets.
+void orc_geometry_info(void* p, uint64_t out[5]) {
+  const Geometry& g = static_cast<Handle*>(p)->scene.geometry;
+  uint64_t tris = 0, nodes = g.tlas.nodes.size(), packets = 0;
+  for (const Mesh& m : g.meshes) {
+    tris += m.triangle_count();
+    nodes += m.bvh.nodes.size();
+    packets += m.packets.size();
+  }
+  out[0] = g.meshes.size();
+  out[1] = g.instances.size();
+  out[2] = tris;
+  out[3] = nodes;
+  out[4] = packets;
+}
+
+// One sample (`sample_index`) of every pixel of the tile: w*h*3 radiance floats and w*h alphas,
+// row major from the tile's top-left (the value rt_render_tile_rgba splats).
+int orc_render_tile(void* p, uint32_t x, uint32_t y, uint32_t w, uint32_t h, uint32_t sample_index, float* rgb,
+                    float* alpha, uint64_t* rays) {
+  auto* hd = static_cast<Handle*>(p);
+  const Scene& s = hd->scene;
+  RenderStats st;
+  for (uint32_t j = 0; j < h; ++j)
+    for (uint32_t i = 0; i < w; ++i) {
+      uint32_t px = x + i, py = y + j;
+      size_t k = (size_t)j * w + i;
+      if (px >= s.render.width || py >= s.render.height) {
+        rgb[3 * k] = rgb[3 * k + 1] = rgb[3 * k + 2] = 0.0f;
+        if (alpha) alpha[k] = 0.0f;
+        continue;
+      }
+      PixelSampler sampler = PixelSampler::make(s.render.seed, px, py, sample_index);
+      Ray ray;
+      camera_sample(s, sampler, px, py, ray);
+      Rgb c;
+      float a;
+      radiance(s, hd->prep, sampler, ray, c, a, &st);
+      rgb[3 * k] = c.r;
+      rgb[3 * k + 1] = c.g;
+      rgb[3 * k + 2] = c.b;
+      if (alpha) alpha[k] = a;
+    }
+  if (rays) *rays = st.rays;
+  return 0;
+}
+
+// Accumulates samples [first, first + count) of every pixel into the whole-image film arrays
+// (rgb: 3*W*H, weight/alpha/lum2: W*H; caller-initialised), 32-pixel tiles over `threads` workers.
+// stats (may be null): rays, nearest queries, any queries, nearest nodes, nearest packets,
+// any nodes, any packets.
+int orc_render_film(void* p, uint32_t first, uint32_t count, int threads, float* rgb, float* weight, float* alpha,
+                    float* lum2, uint64_t* stats_out) {
+  auto* hd = static_cast<Handle*>(p);
+  const Scene& s = hd->scene;
+  uint32_t W = s.render.width, H = s.render.height;
+  std::vector<TileRect> tiles = tile_grid(W, H, 32);
+  std::atomic<size_t> next{0};
+  std::vector<RenderStats> per(std::max(threads, 1));
+  auto worker = [&](int tid) {
+    for (;;) {
+      size_t t = next.fetch_add(1);
+      if (t >= tiles.size()) return;
+      TileRect tile = tiles[t];
+      Film film;
+      film.init(tile.x, tile.y, tile.w, tile.h);
+      for (uint32_t j = 0; j < tile.h; ++j)
+        for (uint32_t i = 0; i < tile.w; ++i) {
+          size_t src = (size_t)(tile.y + j) * W + tile.x + i, dst = (size_t)j * tile.w + i;
+          for (int c = 0; c < 3; ++c) film.rgb[3 * dst + c] = rgb[3 * src + c];
+          film.weight[dst] = weight[src];
+          film.alpha[dst] = alpha[src];
+          film.lum2[dst] = lum2[src];
+        }
+      for (uint32_t k = 0; k < count; ++k) render_tile(s, hd->prep, tile, first + k, film, &per[tid]);
+      for (uint32_t j = 0; j < tile.h; ++j)
+        for (uint32_t i = 0; i < tile.w; ++i) {
+          size_t dst = (size_t)(tile.y + j) * W + tile.x + i, src = (size_t)j * tile.w + i;
+          for (int c = 0; c < 3; ++c) rgb[3 * dst + c] = film.rgb[3 * src + c];
+          weight[dst] = film.weight[src];
+          alpha[dst] = film.alpha[src];
+          lum2[dst] = film.lum2[src];
+        }
+    }
+  };
+  int n = std::max(threads, 1);
+  std::vector<std::thread> pool;
+  for (int t = 0; t < n; ++t) pool.emplace_back(worker, t);
+  for (auto& t : pool) t.join();
+  if (stats_out) {
+    uint64_t acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (const RenderStats& r : per) {
+      acc[0] += r.rays;
+      acc[1] += r.nearest;
+      acc[2] += r.any;
+      acc[3] += r.nearest_trav.nodes;
+      acc[4] += r.nearest_trav.packets;
+      acc[5] += r.any_trav.nodes;
+      acc[6] += r.any_trav.packets;
+    }
+    std::memcpy(stats_out, acc, sizeof(acc));
+  }
+  return 0;
+}
+
+// Nearest and any hit of `n` rays given as 8 floats each (origin, direction, t_min, t_max).
+// Outputs per ray: instance, triangle (0xffffffff on a miss), t, any (0/1). `brute` selects the
+// brute-force loop instead of the tree.
+int orc_trace_rays(void* p, const float* rays, size_t n, int brute, uint32_t* inst, uint32_t* tri, float* t,
+                   uint8_t* any) {
+  const Geometry& g = static_cast<Handle*>(p)->scene.geometry;
+  for (size_t i = 0; i < n; ++i) {
+    const float* r = rays + 8 * i;
+    Ray ray(Vec3(r[0], r[1], r[2]), Vec3(r[3], r[4], r[5]));
+    ray.t_min = r[6];
+    ray.t_max = r[7];
+    RayHit h;
+    bool hit, a;
+    if (brute) {
+      hit = g.brute_force(ray, h, a);
+    } else {
+      hit = g.trace(ray, false, h);
+      a = g.occluded(ray);
+    }
+    inst[i] = hit ? h.instance : 0xffffffffu;
+    tri[i] = hit ? h.triangle : 0xffffffffu;
+    t[i] = hit ? h.t : INF;
+    any[i] = a ? 1 : 0;
+  }
+  return 0;
+}
+
+// Energy tables (regenerated by the restated recipe, or read from the cache file).
+void orc_energy_tables(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_in, uint16_t* clear_out,
+                       float* albedo_avg, float* schlick_avg) {
+  const energy::Tables& t = energy::tables();
+  std::memcpy(albedo, t.albedo.data(), t.albedo.size() * 2);
+  std::memcpy(schlick, t.schlick.data(), t.schlick.size() * 2);
+  std::memcpy(clear_in, t.clear_in.data(), t.clear_in.size() * 2);
+  std::memcpy(clear_out, t.clear_out.data(), t.clear_out.size() * 2);
+  std::memcpy(albedo_avg, t.albedo_avg.data(), t.albedo_avg.size() * 4);
+  std::memcpy(schlick_avg, t.schlick_avg.data(), t.schlick_avg.size() * 4);
+}
+
+// Scalar numerics for the golden-vector tests: 0 exp2, 1 log2, 2 powf, 3 sin, 4 cos, 5 atan2,
+// 6 acos.
+float orc_fmath(int which, float x, float y) {
+  switch (which) {
+    case 0: return exp2f_(x);
+    case 1: return log2f_(x);
+    case 2: return powf_(x, y);
+    case 3: return fsin(x);
+    case 4: return fcos(x);
+    case 5: return atan2f_(x, y);
+    default: return acosf_(x);
+  }
+}
+
+void orc_pcg32(uint64_t seed, uint64_t stream, uint32_t* out, size_t n) {
+  Pcg32 r = Pcg32::seeded(seed, stream);
+  for (size_t i = 0; i < n; ++i) out[i] = r.next_u32();
+}
+uint64_t orc_mix64(uint64_t z) { return mix64(z); }
+uint32_t orc_sobol(uint32_t dim, uint32_t index) { return sobol(dim, index); }
+uint32_t orc_owen(uint32_t x, uint32_t seed) { return owen_scramble(x, seed); }
+float orc_sampler_get1(uint32_t seed, uint32_t x, uint32_t y, uint32_t index, uint32_t dim) {
+  return PixelSampler::make(seed, x, y, index).get1(dim);
+}
+
+}  // extern "C"

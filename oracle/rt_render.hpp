@@ -1,0 +1,190 @@
+// ORACLE — test infrastructure only (see rt_core.hpp).
+// Restates the reference's scene model, lights, environment, integrator, tile renderer, film and
+// RSB subset reader (paths inside engine/crates/rt-core/src):
+//   scene/mod.rs:92-357      Camera (basis, generate_ray, sample_ray, lens_radius)
+//   scene/mod.rs:377-934     Material, Light, Environment, Floor, RenderSettings defaults
+//   scene/rsb.rs:1197-1610   the RSB v1 reader: CAMR MESH MATL LGHT ENVR FLOR RNDR INST
+//   light.rs:120-1551        Lights::new, select, pick_at, sample_selected, crossed_from, escaped
+//   environment.rs:275-431   the studio and flat ramps, their sampling and density
+//   integrator/mod.rs:237-1891  Prepared, shadow, add, radiance_weighted (the branches the
+//                            supported scenes reach)
+//   render.rs:44-497         tile_grid, camera_sample_weighted, render_tile_traced
+//   image.rs:81-169          Film::splat_alpha, resolve
+//   pixel_filter.rs:103-120  offset (box, centre, tent)
+// Scenes using anything outside that subset are refused at load with a message.
+#pragma once
+#include <memory>
+#include <string>
+#include <vector>
+#include "rt_bsdf.hpp"
+#include "rt_geometry.hpp"
+
+namespace orc {
+
+struct Camera {
+  Vec3 position{0, -1000, 0}, target{0, 0, 0}, up{0, 0, 1};
+  float focal_mm = 50, sensor_mm = 36, fstop = 0, focus_dist = 1000;
+  bool ortho = false;
+  float ortho_width = 1000;
+  u32 projection = 0;
+  float fov_x_deg = 360, fov_y_deg = 180, exterior = 1;
+  void basis(Vec3& forward, Vec3& right, Vec3& upv) const;
+  float lens_radius() const;
+  Ray generate_ray(float aspect, float u, float v, float lx, float ly) const;
+};
+
+enum LightKind : u32 { LIGHT_RECT = 0, LIGHT_DISK = 1, LIGHT_SUN = 2, LIGHT_POINT = 3, LIGHT_SPOT = 4, LIGHT_IES = 5 };
+constexpr u32 LIGHT_FLAG_GLOSSY = 1, LIGHT_FLAG_CAMERA = 2, LIGHT_FLAG_NO_SHADOW = 4;
+
+struct Light {
+  u32 kind = 0;
+  Vec3 position{0, 0, 1000}, direction{0, 0, -1}, up{0, 1, 0};
+  float size_x = 500, size_y = 500;
+  Rgb color{1, 1, 1};
+  float power = 1, spread_deg = 180, angle_deg = 0.5f;
+  u32 flags = LIGHT_FLAG_GLOSSY | LIGHT_FLAG_CAMERA;
+};
+
+struct Environment {
+  u32 kind = 0;  // 0 studio, 1 flat
+  float floor = 0.1f, horizon = 0.3f, zenith = 0.6f, strength = 1, rotation_deg = 0;
+  Rgb background_color{0.05f, 0.05f, 0.05f};
+  u32 background_mode = 0;
+};
+
+struct Floor {
+  u32 mode = 0;  // 0 none, 1 visible
+  float z = 0;
+  Rgb color{0.5f, 0.5f, 0.5f};
+};
+
+struct RenderSettings {
+  u32 width = 640, height = 360, max_samples = 64, seed = 0;
+  float max_seconds = 0, exposure_ev = 0;
+  u32 tonemap = 1, max_depth = 8;
+  float clamp_indirect = 0;
+  u32 pixel_filter = 0, edge_supersample = 1;
+  u32 depth() const { return max_depth == 0 ? 8 : std::min<u32>(max_depth, 1024); }
+};
+
+struct Scene {
+  Camera camera;
+  std::vector<Material> materials;
+  Geometry geometry;
+  std::vector<Light> lights;
+  Environment environment;
+  Floor floor;
+  RenderSettings render;
+};
+
+bool scene_from_rsb(const uint8_t* bytes, size_t len, Scene& out, std::string& err);
+
+// ---------------------------------------------------------------- lights (light.rs)
+struct AreaLight {
+  bool rect;
+  float half_x, half_y, radius;
+  Vec3 centre;
+  Frame frame;
+  float area;
+  Rgb radiance;
+  float exponent;
+  bool glossy, camera, shadow;
+  u32 entry;
+  float base, bound;
+  Rgb radiance_at(float cos) const { return exponent == 0.0f ? radiance : radiance * powf_(cos, exponent); }
+  bool intersect(const Ray& ray, float t_max, float& t_out, float& cos_out) const;
+  float weight(Vec3 x) const;
+};
+
+struct EnvRamp {
+  float floor, horizon, zenith, lower_mass, total_mass;
+  float value(float z) const;
+  float pdf(Vec3 dir) const;
+  bool sample(float u0, float u1, Vec3& dir, float& pdf_out) const;
+};
+
+constexpr int KEPT_WEIGHTS = 8;
+struct Selection {
+  Vec3 point;
+  float total = 0;
+  bool by_flux = true;
+  float weights[KEPT_WEIGHTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+struct LightSample {
+  Vec3 wi;
+  float distance;
+  Rgb radiance;
+  float pdf;
+  bool delta, glossy, photon, shadow;
+};
+
+struct Lights {
+  std::vector<AreaLight> areas;
+  EnvRamp env;
+  bool has_env_entry = false;
+  u32 env_entry = 0;
+  float env_base = 0;
+  // entry -> (is_env, area index)
+  std::vector<int> entries;  // area index, or -1 for the environment
+  std::vector<float> cdf, flux_pmf;
+  bool per_point = false;
+
+  bool build(const Scene& scene, std::string& err);
+  float weight(size_t entry, Vec3 x) const;
+  Selection select(Vec3 x) const;
+  Selection select_by_flux(Vec3 x) const { Selection s; s.point = x; s.by_flux = true; return s; }
+  float weight_at(size_t entry, const Selection& s) const;
+  float pmf(u32 entry, const Selection& from) const;
+  bool sample_selected(const Selection& sel, float u_select, float u0, float u1, LightSample& out) const;
+  Rgb crossed_from(const Ray& ray, float t_max, bool has_bsdf, float bsdf_pdf, const Selection* from, float offset) const;
+  Rgb escaped(Vec3 dir, bool has_bsdf, float bsdf_pdf, const Selection* from) const;
+};
+
+// ---------------------------------------------------------------- prepared scene + film
+struct Prepared {
+  Lights lights;
+  u32 background;  // 0 environment, 1 colour, 2 transparent
+  Rgb background_color;
+  u32 floor_mode;
+  float floor_z;
+  Bsdf floor_bsdf;
+  u32 max_depth;
+  float clamp_indirect;
+  Material fallback;
+  std::vector<bool> has_medium;
+  std::vector<Rgb> media;  // per material: absorption coefficients per mm
+
+  bool build(const Scene& scene, std::string& err);
+  float floor_t(const Ray& ray, bool camera_ray) const;
+  const Material& material(const Scene& scene, u32 instance) const;
+  void add(Rgb& sum, Rgb c, bool indirect) const;
+};
+
+// Counters gathered while rendering.
+struct RenderStats {
+  u64 rays = 0;           // camera + extension + shadow rays, as the reference counts them
+  u64 nearest = 0, any = 0;  // queries
+  TraversalStats nearest_trav, any_trav;
+};
+
+// One sample of one pixel: radiance and alpha. `stats` may be null.
+void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& sampler, Ray camera_ray,
+              Rgb& out_sum, float& out_alpha, RenderStats* stats);
+void camera_sample(const Scene& scene, const PixelSampler& sampler, u32 x, u32 y, Ray& ray);
+
+struct TileRect { u32 x, y, w, h; };
+std::vector<TileRect> tile_grid(u32 width, u32 height, u32 size);
+
+struct Film {
+  u32 width = 0, height = 0, origin_x = 0, origin_y = 0;
+  std::vector<float> rgb, weight, alpha, lum2;
+  void init(u32 ox, u32 oy, u32 w, u32 h);
+  void splat_alpha(u32 x, u32 y, Rgb c, float a);
+  std::vector<float> resolve() const;
+};
+
+void render_tile(const Scene& scene, const Prepared& prep, TileRect tile, u32 sample_index, Film& film,
+                 RenderStats* stats);
+
+}  // namespace orc
