@@ -1,0 +1,91 @@
+/*
+ * rpx — the C ABI of the MI355X tracer (librpx.so), a drop-in for the reference renderer's
+ * render call.
+ *
+ * The first group mirrors the reference's hand-written wasm C ABI one for one (same names,
+ * argument meaning, status codes and ownership rules); paths are inside the reference's
+ * engine/crates/rt-wasm/src:
+ *   rt_version            ffi.rs:12-15
+ *   rt_features           ffi.rs:16-25
+ *   rt_scene_load         ffi.rs:52-75   (RSB v1 bytes, copied; 0 on error)
+ *   rt_scene_update       ffi.rs:77-88   (keeps the meshes when the bytes carry no MESH)
+ *   rt_scene_error_len    ffi.rs:90-93
+ *   rt_scene_error_copy   ffi.rs:95-111
+ *   rt_scene_free         ffi.rs:113-124
+ *   rt_render_tile        ffi.rs:118-158 (one sample of a tile: w*h*3 linear RGB floats)
+ *   rt_render_tile_rgba   ffi.rs:160-196 (w*h*4 floats, RGB premultiplied, alpha)
+ * Status codes (rt-wasm/src/lib.rs:36-51): 0 OK, 1 bad handle, 2 empty or oversized tile
+ * (more than 2048 x 2048 pixels), 3 null pointer, 4 bad scene update; codes from 16 on are the
+ * GPU's (16 no device or device memory exhausted, 17 kernel launch or execution failure).
+ * Output buffers belong to the caller and need no alignment.
+ *
+ * The second group (rpx_*) has no counterpart in the reference: it is the GPU-native form of the
+ * hosts' progressive loop (rt-cli/src/lib.rs:266-366), with the film resident in HBM. A film
+ * accumulates the samples of each call in sample-index order per pixel (image.rs:81-94), so any
+ * split of [0, n) into consecutive calls gives the same film. Calls on one handle are
+ * serialised; different handles run on their own streams.
+ */
+#ifndef RPX_H
+#define RPX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RPX_OK 0u
+#define RPX_BAD_HANDLE 1u
+#define RPX_BAD_TILE 2u
+#define RPX_NULL_POINTER 3u
+#define RPX_BAD_SCENE 4u
+#define RPX_GPU_MEMORY 16u
+#define RPX_GPU_LAUNCH 17u
+
+uint32_t rt_version(void);
+uint32_t rt_features(void);
+uint32_t rt_scene_load(const uint8_t* ptr, uint32_t len);
+uint32_t rt_scene_update(uint32_t handle, const uint8_t* ptr, uint32_t len);
+uint32_t rt_scene_error_len(void);
+void rt_scene_error_copy(uint8_t* ptr);
+uint32_t rt_scene_free(uint32_t handle);
+uint32_t rt_render_tile(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t h, uint32_t sample_index,
+                        float* out);
+uint32_t rt_render_tile_rgba(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t h,
+                             uint32_t sample_index, float* out);
+
+/* Renders samples [first, first + count) of every pixel into the handle's resident film. */
+uint32_t rpx_render_samples(uint32_t handle, uint32_t first, uint32_t count);
+/* Clears the resident film. */
+uint32_t rpx_film_reset(uint32_t handle);
+/* Copies the film's sums to host buffers (3*W*H, W*H, W*H, W*H floats; any may be null). */
+uint32_t rpx_film_read(uint32_t handle, float* rgb, float* weight, float* alpha, float* lum2);
+/* The film's device pointers (rgb, weight, alpha, lum2), valid until the handle is freed. */
+uint32_t rpx_film_device(uint32_t handle, void** ptrs4);
+/* Nearest and any hit of n rays (8 floats each: origin, direction, t_min, t_max) in host
+ * memory: instance, mesh triangle (0xffffffff on a miss), t (inf on a miss), any (0/1). */
+uint32_t rpx_trace_rays(uint32_t handle, const float* rays, uint32_t n, uint32_t* inst, uint32_t* tri, float* t,
+                        uint8_t* any);
+/* width, height, max_samples, seed, max_depth, triangles, BVH nodes, instances. */
+uint32_t rpx_scene_info(uint32_t handle, uint64_t* out8);
+/* Rays cast since the handle was loaded (camera + extension + shadow, as the reference counts
+ * them) and samples rendered. */
+uint32_t rpx_scene_stats(uint32_t handle, uint64_t* out2);
+/* Timing of the last rpx_render_samples call on the handle's stream, milliseconds: the whole
+ * call and the sum of the closest-hit kernels. */
+uint32_t rpx_last_timing(uint32_t handle, float* out2);
+/* The GPU-generated energy-compensation tables of the current device (1024, 1024, 17408,
+ * 17408 u16 and 32, 32 f32). */
+uint32_t rpx_energy_tables(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_in, uint16_t* clear_out,
+                           float* albedo_avg, float* schlick_avg);
+/* Scene generators (this library's workloads), written as RSB v1 bytes into buf (cap bytes);
+ * returns the length, or 0 when cap is too small (call with cap 0 to size the buffer).
+ * kind 0: the reference's example scene (example.rs:113-230); 1: icosphere on a floor
+ * (256x256); 2: eight displaced level-7 icospheres (1024x1024). */
+uint64_t rpx_write_scene(uint32_t kind, uint32_t seed, uint8_t* buf, uint64_t cap);
+/* Number of HIP devices visible (0 without a GPU). */
+uint32_t rpx_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

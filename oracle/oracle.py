@@ -1,0 +1,155 @@
+"""ORACLE — test infrastructure only.
+
+Python binding of oracle/_build/liboracle.so, the CPU restatement of the reference renderer
+(rt_core.hpp, rt_geometry.cpp, rt_bsdf.cpp, rt_render.cpp cite the reference lines each part
+follows). Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it, as the
+checker; the product (repackshot_amd/) never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+CACHE = os.path.join(HERE, "_build", "energy_tables.bin")
+
+_lib = None
+
+
+def build(force=False):
+    """Compiles the oracle with its Makefile (g++, no FMA contraction)."""
+    args = ["make", "-C", HERE]
+    if force:
+        args.append("-B")
+    subprocess.run(args, check=True, capture_output=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        vp, u32, u64, f32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_float
+        L.orc_set_energy_cache.argtypes = [ctypes.c_char_p]
+        L.orc_scene_load.restype = vp
+        L.orc_scene_load.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t]
+        L.orc_scene_free.argtypes = [vp]
+        L.orc_scene_info.argtypes = [vp, vp]
+        L.orc_geometry_info.argtypes = [vp, vp]
+        L.orc_render_tile.argtypes = [vp, u32, u32, u32, u32, u32, vp, vp, vp]
+        L.orc_render_film.argtypes = [vp, u32, u32, ctypes.c_int, vp, vp, vp, vp, vp]
+        L.orc_trace_rays.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int, vp, vp, vp, vp]
+        L.orc_energy_tables.argtypes = [vp] * 6
+        L.orc_fmath.restype = f32
+        L.orc_fmath.argtypes = [ctypes.c_int, f32, f32]
+        L.orc_pcg32.argtypes = [u64, u64, vp, ctypes.c_size_t]
+        L.orc_mix64.restype = u64
+        L.orc_mix64.argtypes = [u64]
+        L.orc_sobol.restype = u32
+        L.orc_sobol.argtypes = [u32, u32]
+        L.orc_owen.restype = u32
+        L.orc_owen.argtypes = [u32, u32]
+        L.orc_sampler_get1.restype = f32
+        L.orc_sampler_get1.argtypes = [u32, u32, u32, u32, u32]
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        L.orc_set_energy_cache(CACHE.encode())
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class OracleScene:
+    def __init__(self, rsb: bytes):
+        err = ctypes.create_string_buffer(512)
+        h = lib().orc_scene_load(rsb, len(rsb), err, 512)
+        if not h:
+            raise ValueError(err.value.decode())
+        self.h = ctypes.c_void_p(h)
+        info = np.zeros(5, np.uint32)
+        lib().orc_scene_info(self.h, _p(info))
+        self.width, self.height, self.max_samples, self.seed, self.depth = (int(v) for v in info)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_free(self.h)
+            self.h = None
+
+    def geometry_info(self):
+        out = np.zeros(5, np.uint64)
+        lib().orc_geometry_info(self.h, _p(out))
+        return dict(zip(["meshes", "instances", "triangles", "nodes", "packets"], (int(v) for v in out)))
+
+    def render_tile(self, x, y, w, h, sample_index):
+        rgb = np.zeros((h, w, 3), np.float32)
+        alpha = np.zeros((h, w), np.float32)
+        rays = ctypes.c_uint64()
+        lib().orc_render_tile(self.h, x, y, w, h, sample_index, _p(rgb), _p(alpha), ctypes.byref(rays))
+        return rgb, alpha
+
+    def render_film(self, first, count, threads=None, film=None):
+        """Accumulates samples [first, first + count) into `film` (rgb, weight, alpha, lum2), or a new
+        one; returns the film and the traversal counters."""
+        if threads is None:
+            threads = os.cpu_count() or 1
+        H, W = self.height, self.width
+        if film is None:
+            film = (np.zeros((H, W, 3), np.float32), np.zeros((H, W), np.float32),
+                    np.zeros((H, W), np.float32), np.zeros((H, W), np.float32))
+        stats = np.zeros(7, np.uint64)
+        lib().orc_render_film(self.h, first, count, threads, *(_p(a) for a in film), _p(stats))
+        keys = ["rays", "nearest", "any", "nearest_nodes", "nearest_packets", "any_nodes", "any_packets"]
+        return film, {k: int(v) for k, v in zip(keys, stats)}
+
+    def trace_rays(self, rays, brute=False):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        inst = np.zeros(n, np.uint32)
+        tri = np.zeros(n, np.uint32)
+        t = np.zeros(n, np.float32)
+        anyhit = np.zeros(n, np.uint8)
+        lib().orc_trace_rays(self.h, _p(rays), n, 1 if brute else 0, _p(inst), _p(tri), _p(t), _p(anyhit))
+        return inst, tri, t, anyhit
+
+
+def energy_tables():
+    a = np.zeros(1024, np.uint16)
+    s = np.zeros(1024, np.uint16)
+    ci = np.zeros(17408, np.uint16)
+    co = np.zeros(17408, np.uint16)
+    aa = np.zeros(32, np.float32)
+    sa = np.zeros(32, np.float32)
+    lib().orc_energy_tables(_p(a), _p(s), _p(ci), _p(co), _p(aa), _p(sa))
+    return {"albedo": a, "schlick": s, "clear_in": ci, "clear_out": co, "albedo_avg": aa, "schlick_avg": sa}
+
+
+def fmath(which, x, y=0.0):
+    return np.float32(lib().orc_fmath({"exp2": 0, "log2": 1, "powf": 2, "sin": 3, "cos": 4, "atan2": 5, "acos": 6}[which], x, y))
+
+
+def pcg32(seed, stream, n):
+    out = np.zeros(n, np.uint32)
+    lib().orc_pcg32(seed, stream, _p(out), n)
+    return out
+
+
+def mix64(z):
+    return int(lib().orc_mix64(z))
+
+
+def sobol(dim, index):
+    return int(lib().orc_sobol(dim, index))
+
+
+def owen(x, seed):
+    return int(lib().orc_owen(x, seed))
+
+
+def sampler_get1(seed, x, y, index, dim):
+    return np.float32(lib().orc_sampler_get1(seed, x, y, index, dim))

@@ -1,0 +1,490 @@
+// Scene generators of this library's workloads, written as RSB v1 bytes (the reference's own scene
+// interchange format, scene/rsb.rs:1-130 and the writer src/io/rsb.ts), so the same bytes feed the
+// GPU tracer (rt_scene_load) and any other reader.
+//
+//   kind 0  the reference's example scene (example.rs:20-230): a level-2 icosphere on a floor quad,
+//           a half-size second instance, one 5 W rect light, the studio dome, an 85 mm camera
+//           at f/11, 512 x 384
+//   kind 1  icosphere of 5,120 triangles (level 4, radius 1000 mm) as a rough GGX conductor
+//           (F0 0.95/0.64/0.54, roughness 0.3) on a 2-triangle Lambert floor (albedo 0.8), a
+//           1 x 1 m rect light of radiance 10 at 6500 K, 3 m up, a uniform dome of 0.5; pinhole
+//           camera, fov 40 degrees; 256 x 256, 16 spp, depth 8, seed 0
+//   kind 2  eight level-7 icospheres (327,680 triangles each, 2,621,440 in all) radially displaced
+//           by seeded value noise (amplitude 0.05 r) at seeded places in a 6 x 6 m box: four rough
+//           conductors (roughness 0.05..0.6), two smooth glass spheres (IOR 1.5), two Lambert
+//           (albedo 0.2..0.9); three rect lights of seeded size 0.5..2 m and the studio dome;
+//           1024 x 1024, 64 spp, depth 12
+// All arithmetic is deterministic f32 (the sampler's fmath functions for angles), so a seed gives
+// the same bytes on every host.
+#include <cstring>
+#include <mutex>
+#include <unordered_map>
+#include "../../include/rpx.h"
+#include "rpx_math.h"
+
+namespace rpx {
+namespace {
+
+struct Writer {
+  std::vector<uint8_t> b;
+  void u(u32 v) {
+    uint8_t t[4];
+    std::memcpy(t, &v, 4);
+    b.insert(b.end(), t, t + 4);
+  }
+  void f(float v) { u(f2u(v)); }
+  void vec(V3 v) {
+    f(v.x);
+    f(v.y);
+    f(v.z);
+  }
+  void rgb(C3 c) {
+    f(c.r);
+    f(c.g);
+    f(c.b);
+  }
+  size_t begin(const char* tag) {
+    b.insert(b.end(), tag, tag + 4);
+    u(0);
+    return b.size();
+  }
+  void end(size_t start) {
+    u32 len = (u32)(b.size() - start);
+    std::memcpy(&b[start - 4], &len, 4);
+  }
+};
+
+struct Mat {
+  u32 kind = 0;
+  C3 base{0.8f, 0.8f, 0.8f};
+  float metallic = 0, roughness = 0.5f, ior = 1.5f, transmission = 0;
+};
+struct MeshData {
+  std::vector<V3> pos, nrm;
+  std::vector<u32> idx;
+  u32 material = 0;
+};
+struct Cam {
+  V3 position, target, up{0, 0, 1};
+  float focal = 50, sensor = 36, fstop = 0, focus = 1000, ortho_width = 1000;
+};
+struct Lgt {
+  V3 position, direction, up{0, 0, 1};
+  float sx, sy;
+  C3 color{1, 1, 1};
+  float power, spread = 180, angle = 0;
+  u32 flags = 3;
+};
+struct Inst {
+  u32 mesh;
+  float rows[12];
+  u32 material, flags;
+};
+
+void write_camr(Writer& w, const Cam& c) {
+  size_t s = w.begin("CAMR");
+  w.vec(c.position);
+  w.vec(c.target);
+  w.vec(c.up);
+  w.f(c.focal);
+  w.f(c.sensor);
+  w.f(c.fstop);
+  w.f(c.focus);
+  w.u(0);
+  w.f(c.ortho_width);
+  w.end(s);
+}
+void write_matl(Writer& w, const Mat& m) {
+  size_t s = w.begin("MATL");
+  w.u(m.kind);
+  w.rgb(m.base);
+  w.f(m.metallic);
+  w.f(m.roughness);
+  w.f(m.ior);
+  w.f(m.transmission);
+  w.f(0);  // coat
+  w.f(0);  // coat roughness
+  w.rgb(c3(0, 0, 0));
+  w.f(0);  // emission power
+  w.u(0);  // flags
+  w.f(1);  // opacity
+  w.f(0);  // anisotropy
+  w.f(0);  // absorption
+  w.f(0);  // bevel
+  w.f(0);  // scatter
+  w.rgb(c3(1, 1, 1));
+  w.f(0);  // g
+  w.f(0);  // abbe
+  w.end(s);
+}
+void write_mesh(Writer& w, const MeshData& m) {
+  size_t s = w.begin("MESH");
+  w.u((u32)m.pos.size());
+  w.u((u32)m.idx.size());
+  for (V3 p : m.pos) w.vec(p);
+  for (size_t i = 0; i < m.pos.size(); ++i) w.vec(m.nrm.empty() ? v3(0, 0, 0) : m.nrm[i]);
+  for (u32 i : m.idx) w.u(i);
+  w.u(m.material);
+  w.end(s);
+}
+void write_inst(Writer& w, const Inst& in) {
+  size_t s = w.begin("INST");
+  w.u(in.mesh);
+  for (float v : in.rows) w.f(v);
+  w.u(in.material);
+  w.u(in.flags);
+  w.end(s);
+}
+void write_lght(Writer& w, const Lgt& l) {
+  size_t s = w.begin("LGHT");
+  w.u(0);
+  w.vec(l.position);
+  w.vec(l.direction);
+  w.vec(l.up);
+  w.f(l.sx);
+  w.f(l.sy);
+  w.rgb(l.color);
+  w.f(l.power);
+  w.f(l.spread);
+  w.f(l.angle);
+  w.u(l.flags);
+  w.end(s);
+}
+void write_envr(Writer& w, u32 kind, float fl, float hz, float zn) {
+  size_t s = w.begin("ENVR");
+  w.u(kind);
+  w.f(fl);
+  w.f(hz);
+  w.f(zn);
+  w.f(1);  // strength
+  w.f(0);  // rotation
+  w.rgb(c3(0.05f, 0.05f, 0.05f));
+  w.u(0);            // background: the environment
+  w.u(0xffffffffu);  // hdri map
+  w.u(0xffffffffu);  // backplate map
+  w.u(0);            // fit
+  w.f(3);            // turbidity
+  w.f(0);            // sun azimuth
+  w.f(45);           // sun elevation
+  w.f(0);
+  w.f(0);
+  w.f(1);
+  w.f(1);
+  w.u(0);  // sky model
+  w.end(s);
+}
+void write_flor(Writer& w) {
+  size_t s = w.begin("FLOR");
+  w.u(0);
+  w.f(0);
+  w.rgb(c3(0.5f, 0.5f, 0.5f));
+  w.end(s);
+}
+void write_rndr(Writer& w, u32 width, u32 height, u32 spp, u32 seed, u32 depth) {
+  size_t s = w.begin("RNDR");
+  w.u(width);
+  w.u(height);
+  w.u(spp);
+  w.u(seed);
+  w.f(0);  // max seconds
+  w.f(0);  // exposure
+  w.u(1);  // tonemap
+  w.u(depth);
+  w.f(0);  // clamp
+  w.u(0);  // passes
+  w.u(0);  // denoise
+  w.u(0);  // caustic photons
+  w.f(0);  // caustic radius
+  w.f(1);  // denoise strength
+  w.u(0);  // pixel filter
+  w.u(1);  // edge supersample
+  w.end(s);
+}
+
+// example.rs:20-111 — the icosahedron subdivided `levels` times, vertices on the unit sphere
+// (midpoints in order of first appearance).
+void icosphere_dirs(u32 levels, std::vector<V3>& dirs, std::vector<u32>& faces) {
+  float t = (1.0f + sqrt_(5.0f)) / 2.0f;
+  const float raw[12][3] = {{-1, t, 0}, {1, t, 0}, {-1, -t, 0}, {1, -t, 0}, {0, -1, t}, {0, 1, t},
+                            {0, -1, -t}, {0, 1, -t}, {t, 0, -1}, {t, 0, 1}, {-t, 0, -1}, {-t, 0, 1}};
+  dirs.clear();
+  for (auto& r : raw) dirs.push_back(normalize(v3(r[0], r[1], r[2])));
+  const u32 f0[20][3] = {{0, 11, 5}, {0, 5, 1},  {0, 1, 7},   {0, 7, 10}, {0, 10, 11}, {1, 5, 9}, {5, 11, 4},
+                         {11, 10, 2}, {10, 7, 6}, {7, 1, 8},   {3, 9, 4},  {3, 4, 2},   {3, 2, 6}, {3, 6, 8},
+                         {3, 8, 9},  {4, 9, 5},  {2, 4, 11},  {6, 2, 10}, {8, 6, 7},   {9, 8, 1}};
+  faces.assign(&f0[0][0], &f0[0][0] + 60);
+  for (u32 l = 0; l < levels; ++l) {
+    std::unordered_map<u64, u32> mid;
+    mid.reserve(faces.size() * 2);
+    auto midpoint = [&](u32 a, u32 b) -> u32 {
+      u64 key = ((u64)std::min(a, b) << 32) | std::max(a, b);
+      auto it = mid.find(key);
+      if (it != mid.end()) return it->second;
+      dirs.push_back(normalize(dirs[a] + dirs[b]));
+      u32 id = (u32)dirs.size() - 1;
+      mid.emplace(key, id);
+      return id;
+    };
+    std::vector<u32> next;
+    next.reserve(faces.size() * 4);
+    for (size_t f = 0; f < faces.size(); f += 3) {
+      u32 a = faces[f], b = faces[f + 1], c = faces[f + 2];
+      u32 ab = midpoint(a, b);
+      u32 bc = midpoint(b, c);
+      u32 ca = midpoint(c, a);
+      u32 q[12] = {a, ab, ca, b, bc, ab, c, ca, bc, ab, bc, ca};
+      next.insert(next.end(), q, q + 12);
+    }
+    faces.swap(next);
+  }
+}
+
+MeshData icosphere(V3 center, float radius, u32 levels, u32 material) {
+  MeshData m;
+  icosphere_dirs(levels, m.nrm, m.idx);
+  for (V3 d : m.nrm) m.pos.push_back(center + d * radius);
+  m.material = material;
+  return m;
+}
+
+MeshData floor_quad(float z, float half, u32 material) {
+  MeshData m;
+  m.pos = {v3(-half, -half, z), v3(half, -half, z), v3(half, half, z), v3(-half, half, z)};
+  m.nrm.assign(4, v3(0, 0, 1));
+  m.idx = {0, 1, 2, 0, 2, 3};
+  m.material = material;
+  return m;
+}
+
+void identity_rows(float* r) {
+  const float id[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+  std::memcpy(r, id, 48);
+}
+
+// The colour of a black body (rt-cli/src/scenes.rs kelvin_rgb, src/core/light-math.ts:31-53),
+// normalised to its largest channel.
+C3 kelvin_rgb(float kelvin) {
+  float t = clampf(kelvin, 1000.0f, 12000.0f) / 100.0f;
+  float r = t <= 66.0f ? 255.0f : 329.69873f * fpow(t - 60.0f, -0.13320476f);
+  float g = t <= 66.0f ? 99.4708f * fln(t) - 161.11957f : 288.12217f * fpow(t - 60.0f, -0.07551485f);
+  float b = t >= 66.0f ? 255.0f : (t <= 19.0f ? 0.0f : 138.51773f * fln(t - 10.0f) - 305.0448f);
+  C3 c = c3(clampf(r, 0.0f, 255.0f), clampf(g, 0.0f, 255.0f), clampf(b, 0.0f, 255.0f));
+  float largest = fmax_r(fmax_r(fmax_r(c.r, c.g), c.b), 1.0f);
+  return c / largest;
+}
+
+// The focal length (mm) of a 36 mm sensor for a horizontal field of view of `fov_deg`.
+float focal_for_fov(float fov_deg) {
+  float s, c;
+  sin_cos(0.5f * fov_deg * deg_to_rad(), s, c);
+  return 18.0f / (s / c);
+}
+
+// Seeded 3-D value noise in [-1, 1]: lattice values hashed from integer coordinates, smoothstep
+// weights, trilinear interpolation.
+float lattice(i32 x, i32 y, i32 z, u64 seed) {
+  u64 h = mix64(seed ^ mix64(((u64)(u32)x << 42) ^ ((u64)(u32)y << 21) ^ (u64)(u32)z));
+  return (float)(h >> 40) * (2.0f / 16777216.0f) - 1.0f;
+}
+float value_noise(V3 p, u64 seed) {
+  float fx = floor_(p.x), fy = floor_(p.y), fz = floor_(p.z);
+  i32 ix = f_to_i32(fx), iy = f_to_i32(fy), iz = f_to_i32(fz);
+  float tx = p.x - fx, ty = p.y - fy, tz = p.z - fz;
+  float sx = tx * tx * (3.0f - 2.0f * tx), sy = ty * ty * (3.0f - 2.0f * ty), sz = tz * tz * (3.0f - 2.0f * tz);
+  float c[2][2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int d = 0; d < 2; ++d) c[a][b][d] = lattice(ix + a, iy + b, iz + d, seed);
+  float x00 = c[0][0][0] + (c[1][0][0] - c[0][0][0]) * sx, x10 = c[0][1][0] + (c[1][1][0] - c[0][1][0]) * sx;
+  float x01 = c[0][0][1] + (c[1][0][1] - c[0][0][1]) * sx, x11 = c[0][1][1] + (c[1][1][1] - c[0][1][1]) * sx;
+  float y0 = x00 + (x10 - x00) * sy, y1 = x01 + (x11 - x01) * sy;
+  return y0 + (y1 - y0) * sz;
+}
+
+// Smooth vertex normals: the normalised sum of the (area-weighted) normals of the faces around.
+std::vector<V3> vertex_normals(const std::vector<V3>& pos, const std::vector<u32>& idx) {
+  std::vector<V3> n(pos.size(), v3(0, 0, 0));
+  for (size_t f = 0; f < idx.size(); f += 3) {
+    V3 a = pos[idx[f]], b = pos[idx[f + 1]], c = pos[idx[f + 2]];
+    V3 w = cross(b - a, c - a);
+    for (int k = 0; k < 3; ++k) n[idx[f + k]] = n[idx[f + k]] + w;
+  }
+  for (V3& v : n) v = normalize(v);
+  return n;
+}
+
+std::vector<uint8_t> example_scene() {
+  Writer w;
+  w.b = {'R', 'S', 'B', '1', 0, 0, 0, 0};
+  V3 light_position = v3(250, -450, 650), camera_position = v3(560, -900, 400), target = v3(0, 0, 90);
+  Cam cam;
+  cam.position = camera_position;
+  cam.target = target;
+  cam.focal = 85;
+  cam.sensor = 36;
+  cam.fstop = 11;
+  cam.focus = length(target - camera_position);
+  cam.ortho_width = 600;
+  write_camr(w, cam);
+  Mat m0, m1, m2;
+  m0.base = c3(0.92f, 0.38f, 0.22f);
+  m0.roughness = 0.35f;
+  m1.base = c3(0.3f, 0.3f, 0.32f);
+  m1.roughness = 0.8f;
+  m2.base = c3(0.2f, 0.42f, 0.85f);
+  m2.roughness = 0.25f;
+  write_matl(w, m0);
+  write_matl(w, m1);
+  write_matl(w, m2);
+  write_mesh(w, icosphere(v3(0, 0, 100), 100, 2, 0));
+  write_mesh(w, floor_quad(0, 900, 1));
+  Inst i0{0, {}, 0, 0}, i1{1, {}, 1, 0}, i2{0, {}, 2, 0};
+  identity_rows(i0.rows);
+  identity_rows(i1.rows);
+  const float small[12] = {0.5f, 0, 0, -330, 0, 0.5f, 0, 120, 0, 0, 0.5f, 0};
+  std::memcpy(i2.rows, small, 48);
+  write_inst(w, i0);
+  write_inst(w, i1);
+  write_inst(w, i2);
+  Lgt l;
+  l.position = light_position;
+  l.direction = normalize(target - light_position);
+  l.sx = 400;
+  l.sy = 400;
+  l.power = 5;
+  write_lght(w, l);
+  write_envr(w, 0, 0.05f, 0.12f, 0.3f);
+  write_flor(w);
+  write_rndr(w, 512, 384, 16, 1, 8);
+  return w.b;
+}
+
+std::vector<uint8_t> config_plain(u32 seed) {
+  Writer w;
+  w.b = {'R', 'S', 'B', '1', 0, 0, 0, 0};
+  Cam cam;
+  cam.position = v3(0, -5200, 2400);
+  cam.target = v3(0, 0, 800);
+  cam.focal = focal_for_fov(40.0f);
+  write_camr(w, cam);
+  Mat metal;
+  metal.base = c3(0.95f, 0.64f, 0.54f);
+  metal.metallic = 1;
+  metal.roughness = 0.3f;
+  Mat lambert;
+  lambert.base = c3(0.8f, 0.8f, 0.8f);
+  lambert.ior = 1.0f;
+  write_matl(w, metal);
+  write_matl(w, lambert);
+  write_mesh(w, icosphere(v3(0, 0, 1000), 1000, 4, 0));
+  write_mesh(w, floor_quad(0, 5000, 1));
+  Lgt l;
+  l.position = v3(0, 0, 3000);
+  l.direction = v3(0, 0, -1);
+  l.up = v3(0, 1, 0);
+  l.sx = 1000;
+  l.sy = 1000;
+  l.color = kelvin_rgb(6500.0f);
+  // radiance 10 over 1 m^2: power = L * pi * A
+  l.power = 10.0f * PI * 1.0f;
+  write_lght(w, l);
+  write_envr(w, 1, 0.5f, 0.5f, 0.5f);
+  write_flor(w);
+  write_rndr(w, 256, 256, 16, seed, 8);
+  return w.b;
+}
+
+std::vector<uint8_t> config_large(u32 seed) {
+  Writer w;
+  w.b = {'R', 'S', 'B', '1', 0, 0, 0, 0};
+  Pcg32 rng = Pcg32::seeded((u64)seed, 2);
+  Cam cam;
+  cam.position = v3(0, -7600, 3400);
+  cam.target = v3(0, 0, 500);
+  cam.focal = focal_for_fov(40.0f);
+  write_camr(w, cam);
+  // Materials 0..3 rough conductors, 4..5 smooth glass, 6..7 Lambert.
+  for (int k = 0; k < 8; ++k) {
+    Mat m;
+    if (k < 4) {
+      float a = rng.next_f32(), b = rng.next_f32(), c = rng.next_f32();
+      m.base = c3(0.55f + 0.45f * a, 0.55f + 0.45f * b, 0.55f + 0.45f * c);
+      m.metallic = 1;
+      m.roughness = 0.05f + 0.55f * rng.next_f32();
+    } else if (k < 6) {
+      m.base = c3(1, 1, 1);
+      m.transmission = 1;
+      m.roughness = 0;
+      m.ior = 1.5f;
+    } else {
+      float a = 0.2f + 0.7f * rng.next_f32();
+      m.base = c3(a, a, a);
+      m.ior = 1.0f;
+      m.roughness = 1.0f;
+    }
+    write_matl(w, m);
+  }
+  std::vector<V3> dirs;
+  std::vector<u32> faces;
+  icosphere_dirs(7, dirs, faces);
+  for (int k = 0; k < 8; ++k) {
+    float radius = 450.0f + 350.0f * rng.next_f32();
+    float cx = (-3000.0f + radius) + (6000.0f - 2.0f * radius) * rng.next_f32();
+    float cy = (-3000.0f + radius) + (6000.0f - 2.0f * radius) * rng.next_f32();
+    V3 centre = v3(cx, cy, radius * 1.05f);
+    u64 nseed = mix64(((u64)seed << 8) | (u64)k);
+    MeshData m;
+    m.pos.resize(dirs.size());
+    for (size_t i = 0; i < dirs.size(); ++i) {
+      V3 d = dirs[i];
+      float h = value_noise(d * 4.0f, nseed);
+      m.pos[i] = centre + d * (radius * (1.0f + 0.05f * h));
+    }
+    m.idx = faces;
+    m.nrm = vertex_normals(m.pos, m.idx);
+    m.material = (u32)k;
+    write_mesh(w, m);
+  }
+  for (int k = 0; k < 3; ++k) {
+    Lgt l;
+    float sx = 500.0f + 1500.0f * rng.next_f32(), sy = 500.0f + 1500.0f * rng.next_f32();
+    float s, c;
+    sin_cos(TAU * ((float)k / 3.0f + 0.15f * rng.next_f32()), s, c);
+    float dist = 3500.0f + 1500.0f * rng.next_f32();
+    l.position = v3(dist * c, dist * s, 2500.0f + 1500.0f * rng.next_f32());
+    l.direction = normalize(v3(0, 0, 300) - l.position);
+    l.sx = sx;
+    l.sy = sy;
+    l.color = kelvin_rgb(4500.0f + 3000.0f * rng.next_f32());
+    // about radiance 8..16 over the panel's area in m^2
+    l.power = (8.0f + 8.0f * rng.next_f32()) * PI * (sx * sy * 1.0e-6f);
+    write_lght(w, l);
+  }
+  write_envr(w, 0, 0.1f, 0.3f, 0.6f);
+  write_flor(w);
+  write_rndr(w, 1024, 1024, 64, seed, 12);
+  return w.b;
+}
+}  // namespace
+}  // namespace rpx
+
+extern "C" uint64_t rpx_write_scene(uint32_t kind, uint32_t seed, uint8_t* buf, uint64_t cap) {
+  // The last scene is kept: callers size the buffer with cap 0 and then fetch it.
+  static std::mutex mutex;
+  static uint32_t last_kind = 0xffffffffu, last_seed = 0;
+  static std::vector<uint8_t> bytes;
+  std::lock_guard<std::mutex> lock(mutex);
+  if (kind != last_kind || seed != last_seed) {
+    switch (kind) {
+      case 0: bytes = rpx::example_scene(); break;
+      case 1: bytes = rpx::config_plain(seed); break;
+      case 2: bytes = rpx::config_large(seed); break;
+      default: return 0;
+    }
+    last_kind = kind;
+    last_seed = seed;
+  }
+  if (!buf || cap < bytes.size()) return cap == 0 ? (uint64_t)bytes.size() : 0;
+  std::memcpy(buf, bytes.data(), bytes.size());
+  return bytes.size();
+}

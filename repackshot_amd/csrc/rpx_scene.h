@@ -1,0 +1,187 @@
+// Device-resident scene layout of the GPU tracer and the host-side scene (parsed from the
+// reference's RSB v1 bytes, BVHs built, lights prepared, BSDFs precomputed).
+//
+// HBM layout (all arrays 16-byte aligned, one allocation per array):
+//   nodes      Node4[]       4-wide BVH nodes of every mesh (BLAS) followed by the instance BVH
+//                            (TLAS): six planes x 4 children of f32 boxes + 4 child references,
+//                            128 B per node (two 64-B cache lines, one load of 8 x 16 B)
+//   tris       float4[3 n]   triangles in BLAS leaf order: v0 (w = mesh triangle id), v1, v2;
+//                            full f32 so the watertight test sees the reference's bits
+//   positions  float[3 v], normals float[3 v], indices u32[3 t]   per-mesh vertex data for
+//                            hit details (barycentrics, shading normal, terminator lift)
+//   instances  DevInstance[] world<->object transforms, BLAS root, material, flags
+//   bsdfs      Bsdf[2 m]     per material: seen from outside / inside
+//   materials  DevMaterial[] emission, medium absorption
+//   areas      DevArea[]     rect and disk lights
+// Child reference encoding: bit 31 set = leaf, then (first << 3) | (count - 1); all ones = empty.
+#pragma once
+#include <string>
+#include <vector>
+#include "rpx_bsdf.h"
+
+namespace rpx {
+
+constexpr u32 REF_EMPTY = 0xffffffffu;
+constexpr u32 REF_LEAF = 1u << 31;
+constexpr u32 LEAF_COUNT_BITS = 3;
+RPX_HD u32 make_leaf(u32 first, u32 count) { return REF_LEAF | (first << LEAF_COUNT_BITS) | (count - 1); }
+
+struct alignas(16) Node4 {
+  float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+  u32 child[4];
+  u32 pad[4];
+};
+static_assert(sizeof(Node4) == 128, "Node4 is two cache lines");
+
+constexpr u32 INST_IDENTITY = 1, INST_ONE_SIDED = 2, INST_CAMERA_HIDDEN = 4, INST_ACTIVE = 8;
+
+struct alignas(16) DevInstance {
+  float to_local[12];  // world -> object rows
+  float to_world[12];  // object -> world rows
+  u32 mesh, blas_root, material, flags;
+};
+
+struct DevMesh {
+  u32 vertex_offset, index_offset, has_normals, inside_out;
+};
+
+struct DevMaterial {
+  u32 emissive, has_medium;
+  C3 radiance;
+  C3 sigma;
+};
+
+struct DevArea {
+  u32 rect, glossy, camera, shadow;
+  float half_x, half_y, radius, area;
+  V3 centre;
+  Frame frame;
+  C3 radiance;
+  float exponent, base, bound;
+  u32 entry;
+};
+
+constexpr int MAX_ENTRIES = 64;
+constexpr int KEPT_WEIGHTS = 8;
+
+// Everything the kernels read, passed by value as a kernel argument.
+struct DevScene {
+  const Node4* nodes;
+  const float4* tris;
+  const DevInstance* instances;
+  const DevMesh* meshes;
+  const float* positions;
+  const float* normals;
+  const u32* indices;
+  const Bsdf* bsdfs;
+  const DevMaterial* materials;
+  const DevArea* areas;
+  const float* cdf;       // [n_entries]
+  const float* flux_pmf;  // [n_entries]
+  const i32* entries;     // area index, -1 = environment
+  EnergyTables tables;
+  u32 tlas_root, n_instances, n_materials, n_areas, n_entries, per_point, has_env_entry, env_entry;
+  // environment ramp
+  float env_floor, env_horizon, env_zenith, env_lower_mass, env_total_mass, env_base;
+  // camera
+  V3 cam_position, cam_forward, cam_right, cam_up;
+  float focal_mm, sensor_w, sensor_h, lens_radius, focus_dist, ortho_width, aspect, exterior;
+  u32 ortho;
+  // render settings
+  u32 width, height, seed, max_depth, pixel_filter, background;
+  float clamp_indirect;
+  C3 background_color;
+  u32 floor_mode;
+  float floor_z;
+  Bsdf floor_bsdf;
+  Bsdf fallback_bsdf[2];
+};
+
+// ---------------------------------------------------------------- host side
+struct HostMaterial {
+  u32 kind = 0;
+  C3 base_color{0.8f, 0.8f, 0.8f};
+  float metallic = 0, roughness = 0.5f, ior = 1.5f, transmission = 0, coat = 0, coat_roughness = 0;
+  C3 emission{0, 0, 0};
+  float emission_power = 0;
+  u32 flags = 0;
+  float opacity = 1, anisotropy = 0, absorption_mm = 0, bevel_mm = 0, scatter_mm = 0;
+  C3 scatter_color{1, 1, 1};
+  float scatter_g = 0, abbe = 0;
+};
+
+struct HostLight {
+  u32 kind = 0;
+  V3 position{0, 0, 1000}, direction{0, 0, -1}, up{0, 1, 0};
+  float size_x = 500, size_y = 500;
+  C3 color{1, 1, 1};
+  float power = 1, spread_deg = 180, angle_deg = 0.5f;
+  u32 flags = 3;
+};
+
+struct HostMesh {
+  std::vector<float> positions;  // xyz
+  std::vector<float> normals;    // xyz, empty when none
+  std::vector<u32> indices;
+  u32 material = 0;
+  bool inside_out = false;
+  float bmin[3], bmax[3];
+};
+
+struct HostInstance {
+  u32 mesh = 0;
+  float rows[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+  u32 material = 0, flags = 0;
+};
+
+struct HostCamera {
+  V3 position{0, -1000, 0}, target{0, 0, 0}, up{0, 0, 1};
+  float focal_mm = 50, sensor_mm = 36, fstop = 0, focus_dist = 1000;
+  u32 ortho = 0;
+  float ortho_width = 1000;
+  u32 projection = 0;
+  float exterior = 1;
+};
+
+struct HostScene {
+  HostCamera camera;
+  std::vector<HostMaterial> materials;
+  std::vector<HostMesh> meshes;
+  std::vector<HostInstance> instances;
+  std::vector<HostLight> lights;
+  u32 env_kind = 0;
+  float env_floor = 0.1f, env_horizon = 0.3f, env_zenith = 0.6f, env_strength = 1;
+  C3 background_color{0.05f, 0.05f, 0.05f};
+  u32 background_mode = 0;
+  u32 floor_mode = 0;
+  float floor_z = 0;
+  C3 floor_color{0.5f, 0.5f, 0.5f};
+  u32 width = 640, height = 360, max_samples = 64, seed = 0, max_depth = 8, pixel_filter = 0;
+  float clamp_indirect = 0;
+};
+
+// Parses the RSB subset the tracer follows; false with a message for anything else.
+bool parse_rsb(const uint8_t* bytes, size_t len, HostScene& out, std::string& err);
+
+// The flattened scene ready for upload (host arrays) plus its DevScene constants.
+struct FlatScene {
+  std::vector<Node4> nodes;
+  std::vector<float4> tris;
+  std::vector<DevInstance> instances;
+  std::vector<DevMesh> meshes;
+  std::vector<float> positions, normals;
+  std::vector<u32> indices;
+  std::vector<Bsdf> bsdfs;
+  std::vector<DevMaterial> materials;
+  std::vector<DevArea> areas;
+  std::vector<float> cdf, flux_pmf;
+  std::vector<i32> entries;
+  DevScene consts;  // pointers filled at upload
+  u64 triangles = 0;
+  u32 max_blas_depth = 0, max_tlas_depth = 0;
+};
+
+// Builds BVHs, lights and BSDFs (host tables of the energy compensation in `tables`).
+bool flatten_scene(const HostScene& s, const EnergyTables& tables, FlatScene& out, std::string& err, int threads);
+
+}  // namespace rpx

@@ -1,0 +1,261 @@
+"""ctypes binding of librpx.so, the GPU tracer's C ABI (include/rpx.h), and a Python mirror of the
+reference renderer's host-facing render API.
+
+The reference's hosts drive its engine through the wasm C ABI (engine/crates/rt-wasm/src/ffi.rs:
+rt_scene_load / rt_render_tile / rt_render_tile_rgba / rt_scene_update / rt_scene_free, with the
+last load error as text) and accumulate each pixel's samples in f32 in sample order
+(src/render/tiles.ts:154-219, rt-cli/src/lib.rs:266-366). `Scene` keeps those names, argument
+meanings, status codes and error behaviour; `render_samples` / `film` are the GPU-native progressive
+loop with the film resident in HBM.
+
+There is no CPU fallback: without the built library, or without a GPU, the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librpx.so")
+
+STATUS = {
+    0: "ok",
+    1: "bad handle",
+    2: "empty or oversized tile",
+    3: "null pointer",
+    4: "bad scene",
+    16: "no GPU or GPU memory exhausted",
+    17: "GPU kernel launch or execution failure",
+}
+
+# Every exported symbol of include/rpx.h, with its ctypes signature.
+_SIGS = {
+    "rt_version": (ctypes.c_uint32, []),
+    "rt_features": (ctypes.c_uint32, []),
+    "rt_scene_load": (ctypes.c_uint32, [ctypes.c_void_p, ctypes.c_uint32]),
+    "rt_scene_update": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32]),
+    "rt_scene_error_len": (ctypes.c_uint32, []),
+    "rt_scene_error_copy": (None, [ctypes.c_void_p]),
+    "rt_scene_free": (ctypes.c_uint32, [ctypes.c_uint32]),
+    "rt_render_tile": (ctypes.c_uint32, [ctypes.c_uint32] * 6 + [ctypes.c_void_p]),
+    "rt_render_tile_rgba": (ctypes.c_uint32, [ctypes.c_uint32] * 6 + [ctypes.c_void_p]),
+    "rpx_render_samples": (ctypes.c_uint32, [ctypes.c_uint32] * 3),
+    "rpx_film_reset": (ctypes.c_uint32, [ctypes.c_uint32]),
+    "rpx_film_read": (ctypes.c_uint32, [ctypes.c_uint32] + [ctypes.c_void_p] * 4),
+    "rpx_film_device": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
+    "rpx_trace_rays": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 4),
+    "rpx_scene_info": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
+    "rpx_scene_stats": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
+    "rpx_last_timing": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
+    "rpx_energy_tables": (ctypes.c_uint32, [ctypes.c_void_p] * 6),
+    "rpx_write_scene": (ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64]),
+    "rpx_device_count": (ctypes.c_uint32, []),
+}
+
+_lib = None
+
+
+class RsbError(RuntimeError):
+    """The scene bytes could not be loaded (the reference's RsbError text)."""
+
+
+class RenderError(RuntimeError):
+    """A render call returned a non-zero status."""
+
+
+def library():
+    """The loaded librpx.so; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "librpx.so is not built (run repackshot_amd/build.py or __graft_entry__.build())")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def device_count():
+    return int(library().rpx_device_count())
+
+
+def last_error():
+    lib = library()
+    n = lib.rt_scene_error_len()
+    buf = ctypes.create_string_buffer(max(int(n), 1))
+    lib.rt_scene_error_copy(buf)
+    return buf.raw[:n].decode("utf-8", "replace")
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _check(status, what):
+    if status != 0:
+        raise RenderError("%s: status %d (%s) %s" % (what, status, STATUS.get(status, "?"), last_error()))
+
+
+def write_scene(kind, seed=0):
+    """RSB v1 bytes of one of this library's scenes (0 example, 1 icosphere on a floor, 2 eight
+    displaced level-7 icospheres)."""
+    lib = library()
+    n = lib.rpx_write_scene(kind, seed, None, 0)
+    if n == 0:
+        raise ValueError("unknown scene kind %d" % kind)
+    buf = np.zeros(n, np.uint8)
+    got = lib.rpx_write_scene(kind, seed, _ptr(buf), n)
+    if got != n:
+        raise RuntimeError("scene generator returned %d bytes, expected %d" % (got, n))
+    return buf.tobytes()
+
+
+def energy_tables():
+    """The energy-compensation tables generated on the current GPU."""
+    lib = library()
+    a = np.zeros(1024, np.uint16)
+    s = np.zeros(1024, np.uint16)
+    ci = np.zeros(17408, np.uint16)
+    co = np.zeros(17408, np.uint16)
+    aa = np.zeros(32, np.float32)
+    sa = np.zeros(32, np.float32)
+    _check(lib.rpx_energy_tables(_ptr(a), _ptr(s), _ptr(ci), _ptr(co), _ptr(aa), _ptr(sa)), "energy tables")
+    return {"albedo": a, "schlick": s, "clear_in": ci, "clear_out": co, "albedo_avg": aa, "schlick_avg": sa}
+
+
+class Scene:
+    """A scene loaded on the GPU from RSB v1 bytes (rt_scene_load)."""
+
+    def __init__(self, rsb: bytes):
+        lib = library()
+        data = np.frombuffer(rsb, np.uint8)
+        h = lib.rt_scene_load(_ptr(data), len(rsb))
+        if h == 0:
+            raise RsbError(last_error())
+        self.handle = int(h)
+        info = self.info()
+        self.width, self.height = info["width"], info["height"]
+
+    def close(self):
+        if getattr(self, "handle", 0):
+            library().rt_scene_free(self.handle)
+            self.handle = 0
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def update(self, rsb: bytes):
+        """rt_scene_update: replaces the scene, keeping the meshes when the bytes carry none."""
+        data = np.frombuffer(rsb, np.uint8)
+        st = library().rt_scene_update(self.handle, _ptr(data), len(rsb))
+        if st == 4:
+            raise RsbError(last_error())
+        _check(st, "rt_scene_update")
+        info = self.info()
+        self.width, self.height = info["width"], info["height"]
+
+    def info(self):
+        out = np.zeros(8, np.uint64)
+        _check(library().rpx_scene_info(self.handle, _ptr(out)), "rpx_scene_info")
+        keys = ["width", "height", "max_samples", "seed", "max_depth", "triangles", "nodes", "instances"]
+        return {k: int(v) for k, v in zip(keys, out)}
+
+    def render_tile(self, x, y, w, h, sample_index):
+        """One sample of every pixel of the tile: (h, w, 3) float32 linear RGB."""
+        out = np.zeros((h, w, 3), np.float32)
+        _check(library().rt_render_tile(self.handle, x, y, w, h, sample_index, _ptr(out)), "rt_render_tile")
+        return out
+
+    def render_tile_rgba(self, x, y, w, h, sample_index):
+        out = np.zeros((h, w, 4), np.float32)
+        _check(library().rt_render_tile_rgba(self.handle, x, y, w, h, sample_index, _ptr(out)), "rt_render_tile_rgba")
+        return out
+
+    def render_samples(self, first, count):
+        """Accumulates samples [first, first + count) of every pixel into the resident film."""
+        _check(library().rpx_render_samples(self.handle, first, count), "rpx_render_samples")
+
+    def reset_film(self):
+        _check(library().rpx_film_reset(self.handle), "rpx_film_reset")
+
+    def film(self):
+        """The film's sums: rgb (H, W, 3), weight, alpha, lum2 (H, W), float32."""
+        H, W = self.height, self.width
+        rgb = np.zeros((H, W, 3), np.float32)
+        weight = np.zeros((H, W), np.float32)
+        alpha = np.zeros((H, W), np.float32)
+        lum2 = np.zeros((H, W), np.float32)
+        _check(library().rpx_film_read(self.handle, _ptr(rgb), _ptr(weight), _ptr(alpha), _ptr(lum2)), "rpx_film_read")
+        return rgb, weight, alpha, lum2
+
+    def film_device_pointers(self):
+        ptrs = (ctypes.c_void_p * 4)()
+        _check(library().rpx_film_device(self.handle, ptrs), "rpx_film_device")
+        return [int(p or 0) for p in ptrs]
+
+    def resolve(self):
+        """Film::resolve (image.rs:158-169): sums times the f32 reciprocal of the count."""
+        rgb, weight, _, _ = self.film()
+        return resolve(rgb, weight)
+
+    def trace_rays(self, rays):
+        """Nearest and any hit of rays (n, 8) float32: origin, direction, t_min, t_max."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        inst = np.zeros(n, np.uint32)
+        tri = np.zeros(n, np.uint32)
+        t = np.zeros(n, np.float32)
+        anyhit = np.zeros(n, np.uint8)
+        _check(library().rpx_trace_rays(self.handle, _ptr(rays), n, _ptr(inst), _ptr(tri), _ptr(t), _ptr(anyhit)),
+               "rpx_trace_rays")
+        return inst, tri, t, anyhit
+
+    def stats(self):
+        out = np.zeros(2, np.uint64)
+        _check(library().rpx_scene_stats(self.handle, _ptr(out)), "rpx_scene_stats")
+        return {"rays": int(out[0]), "samples": int(out[1])}
+
+    def last_timing_ms(self):
+        out = np.zeros(2, np.float32)
+        _check(library().rpx_last_timing(self.handle, _ptr(out)), "rpx_last_timing")
+        return float(out[0])
+
+
+def resolve(rgb, weight):
+    """Average radiance per pixel: rgb * f32(1 / weight), zero where no sample landed."""
+    rgb = np.asarray(rgb, np.float32)
+    weight = np.asarray(weight, np.float32)
+    out = np.zeros_like(rgb)
+    m = weight > 0
+    inv = np.zeros_like(weight)
+    inv[m] = np.float32(1.0) / weight[m]
+    out[m] = rgb[m] * inv[m][..., None]
+    return out
+
+
+def fnv1a(values):
+    """FNV-1a over the little-endian bytes of f32 values (render.rs:981-989)."""
+    data = np.ascontiguousarray(values, dtype="<f4").view(np.uint8)
+    h = np.uint32(0x811C9DC5)
+    # vectorised in blocks is not possible (sequential); use Python ints over bytes
+    hv = int(h)
+    for b in data.tobytes():
+        hv = ((hv ^ b) * 0x01000193) & 0xFFFFFFFF
+    return hv

@@ -1,0 +1,98 @@
+"""A writer of RSB v1 scene bytes (the reference's scene interchange format, scene/rsb.rs:1-130;
+the interface's writer is src/io/rsb.ts) for the sections the GPU tracer reads: CAMR, MATL, MESH,
+INST, LGHT, ENVR, FLOR, RNDR. Values are packed as given (little-endian f32/u32); no arithmetic is
+done on them, so the bytes carry exactly the caller's floats.
+"""
+import struct
+
+import numpy as np
+
+
+def _f(*v):
+    return struct.pack("<%df" % len(v), *[float(np.float32(x)) for x in v])
+
+
+def _u(*v):
+    return struct.pack("<%dI" % len(v), *[int(x) for x in v])
+
+
+def _section(tag: bytes, payload: bytes) -> bytes:
+    return tag + _u(len(payload)) + payload
+
+
+class SceneWriter:
+    def __init__(self):
+        self.sections = []
+
+    def camera(self, position, target, up=(0, 0, 1), focal_mm=50.0, sensor_mm=36.0, fstop=0.0,
+               focus_dist=1000.0, ortho=False, ortho_width=1000.0):
+        self.sections.append(_section(b"CAMR", _f(*position, *target, *up, focal_mm, sensor_mm, fstop, focus_dist)
+                                      + _u(1 if ortho else 0) + _f(ortho_width)))
+        return self
+
+    def material(self, kind=0, base_color=(0.8, 0.8, 0.8), metallic=0.0, roughness=0.5, ior=1.5, transmission=0.0,
+                 coat=0.0, coat_roughness=0.0, emission=(0, 0, 0), emission_power=0.0, flags=0, opacity=1.0,
+                 anisotropy=0.0, absorption_mm=0.0):
+        payload = (_u(kind) + _f(*base_color, metallic, roughness, ior, transmission, coat, coat_roughness,
+                                 *emission, emission_power) + _u(flags)
+                   + _f(opacity, anisotropy, absorption_mm, 0.0, 0.0, 1.0, 1.0, 1.0, 0.0, 0.0))
+        self.sections.append(_section(b"MATL", payload))
+        return self
+
+    def mesh(self, positions, indices, normals=None, material=0):
+        p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        n = np.zeros_like(p) if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        i = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+        payload = _u(p.shape[0], i.size) + p.astype("<f4").tobytes() + n.astype("<f4").tobytes() \
+            + i.astype("<u4").tobytes() + _u(material)
+        self.sections.append(_section(b"MESH", payload))
+        return self
+
+    def instance(self, mesh, rows=((1, 0, 0, 0), (0, 1, 0, 0), (0, 0, 1, 0)), material=0, flags=0):
+        flat = [v for r in rows for v in r]
+        self.sections.append(_section(b"INST", _u(mesh) + _f(*flat) + _u(material, flags)))
+        return self
+
+    def light(self, kind=0, position=(0, 0, 1000), direction=(0, 0, -1), up=(0, 1, 0), size_x=500.0, size_y=500.0,
+              color=(1, 1, 1), power=1.0, spread_deg=180.0, angle_deg=0.0, flags=3):
+        self.sections.append(_section(b"LGHT", _u(kind) + _f(*position, *direction, *up, size_x, size_y, *color, power,
+                                                           spread_deg, angle_deg) + _u(flags)))
+        return self
+
+    def environment(self, kind=0, floor=0.1, horizon=0.3, zenith=0.6, strength=1.0, background_color=(0.05,) * 3,
+                    background_mode=0):
+        payload = (_u(kind) + _f(floor, horizon, zenith, strength, 0.0, *background_color) + _u(background_mode)
+                   + _u(0xFFFFFFFF, 0xFFFFFFFF, 0) + _f(3.0, 0.0, 45.0, 0.0, 0.0, 1.0, 1.0) + _u(0))
+        self.sections.append(_section(b"ENVR", payload))
+        return self
+
+    def floor(self, mode=0, z=0.0, color=(0.5, 0.5, 0.5)):
+        self.sections.append(_section(b"FLOR", _u(mode) + _f(z, *color)))
+        return self
+
+    def render(self, width, height, max_samples=16, seed=0, max_depth=8, clamp_indirect=0.0, pixel_filter=0):
+        payload = (_u(width, height, max_samples, seed) + _f(0.0, 0.0) + _u(1, max_depth) + _f(clamp_indirect)
+                   + _u(0, 0, 0) + _f(0.0, 1.0) + _u(pixel_filter, 1))
+        self.sections.append(_section(b"RNDR", payload))
+        return self
+
+    def bytes(self) -> bytes:
+        return b"RSB1" + _u(0) + b"".join(self.sections)
+
+
+def uv_sphere(center, radius, lat=16, lon=32):
+    """A UV sphere (positions, normals, indices) in float32."""
+    th = np.linspace(0.0, np.pi, lat + 1)
+    ph = np.linspace(0.0, 2 * np.pi, lon, endpoint=False)
+    tt, pp = np.meshgrid(th, ph, indexing="ij")
+    n = np.stack([np.sin(tt) * np.cos(pp), np.sin(tt) * np.sin(pp), np.cos(tt)], -1).reshape(-1, 3)
+    p = np.asarray(center, np.float64) + radius * n
+    idx = []
+    for i in range(lat):
+        for j in range(lon):
+            a = i * lon + j
+            b = i * lon + (j + 1) % lon
+            c = a + lon
+            d = b + lon
+            idx += [a, c, b, b, c, d]
+    return p.astype(np.float32), n.astype(np.float32), np.array(idx, np.uint32)

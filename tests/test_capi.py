@@ -1,0 +1,55 @@
+"""The C ABI library without a GPU: it loads, exports every entry point of include/rpx.h, refuses
+work it cannot do with the reference's status codes, and its scene generators write RSB bytes that
+the oracle reads."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "rpx.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:uint32_t|uint64_t|void)\s+(r[tp]x?_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol(rpx):
+    lib = rpx.library()
+    names = declared_functions()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(lib, name), name
+    assert sorted(rpx.exported_symbols()) == names
+
+
+def test_status_codes_without_a_scene(rpx):
+    lib = rpx.library()
+    out = np.zeros(12, np.float32)
+    assert lib.rt_render_tile(12345, 0, 0, 2, 2, 0, out.ctypes.data) == 1       # bad handle
+    assert lib.rt_render_tile(12345, 0, 0, 2, 2, 0, None) == 3                  # null pointer
+    assert lib.rt_scene_free(12345) == 1
+    assert lib.rt_scene_load(None, 10) == 0
+    assert "null pointer" in rpx.last_error()
+
+
+def test_load_without_a_gpu_fails_loudly(rpx, example_rsb):
+    if rpx.device_count() != 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(rpx.RsbError, match="no HIP device"):
+        rpx.Scene(example_rsb)
+
+
+def test_generated_plain_scene_is_what_it_says(rpx):
+    s = orc.OracleScene(rpx.write_scene(1, 0))
+    g = s.geometry_info()
+    assert (s.width, s.height, s.max_samples, s.depth) == (256, 256, 16, 8)
+    assert g["triangles"] == 5120 + 2 and g["instances"] == 2
+
+
+def test_scene_generators_are_deterministic(rpx):
+    assert rpx.write_scene(1, 3) == rpx.write_scene(1, 3)
+    assert rpx.write_scene(0) == rpx.write_scene(0)
