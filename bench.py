@@ -1,0 +1,194 @@
+"""Benchmark of the GPU tracer on the single large render: 1024 x 1024, 64 spp, depth 12, eight
+displaced level-7 icospheres (2,621,440 triangles), four rough conductors, two smooth glass, two
+Lambert, three rect lights and the studio dome (repackshot_amd/csrc/rpx_generators.hip, kind 2).
+
+A step renders the whole image at 64 samples per pixel into the film resident in HBM. Prints one
+JSON line: Mpaths/s of the timed steps, Mrays/s, the roofline of the closest-hit kernel (the
+dominant one), the CPU baseline (the oracle on this host's cores over a bounded sample of the same
+workload) and the parity of the GPU film against the oracle on that sample's tiles.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpaths/s at 1024² 64spp & mixed queue; adjoint grads/s vs FD at fixed rRMSE"
+HBM_PEAK_GBS = 8000.0  # MI355X datasheet: 8 TB/s HBM3E
+# Algorithmic bytes of one node and one triangle packet of the reference's BVH8 layout
+# (geometry/bvh.rs:95-102: bounds [3][2][8] f32 + child [8] u32 = 224 B, align 64 -> 256 B;
+# geometry/triangle.rs:160-167: v [3][3][4] f32 + id [4] u32 = 160 B).
+NODE_BYTES = 256
+PACKET_BYTES = 160
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(rsb, spp, budget_s=15.0):
+    """The oracle (the CPU restatement of the reference renderer, its host schedule: 32-px tiles
+    over an atomic tile queue on every hardware thread) timed on a bounded sample of the same
+    workload: whole 32 x 32 tiles at all `spp` samples, batches of one tile per thread until about
+    `budget_s` of wall time. Returns the baseline record, the rendered tiles (for the parity check
+    of the GPU film) and the reference traversal's counters over them."""
+    from oracle import oracle as orc
+
+    threads = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    o = orc.OracleScene(rsb)
+    log("bench: oracle scene ready in %.1f s (%d threads)" % (time.perf_counter() - t0, threads))
+    W, H = o.width, o.height
+    order = [(tx * 32, ty * 32) for ty in range((H + 31) // 32) for tx in range((W + 31) // 32)]
+    rng = np.random.default_rng(12345)
+    rng.shuffle(order)
+    film = None
+    counts = {}
+    rects = []
+    elapsed = 0.0
+    while elapsed < budget_s and len(rects) < len(order):
+        batch = [(x, y, min(32, W - x), min(32, H - y)) for (x, y) in order[len(rects):len(rects) + threads]]
+        t1 = time.perf_counter()
+        film, st = o.render_tiles(batch, 0, spp, threads=threads, film=film)
+        elapsed += time.perf_counter() - t1
+        rects += batch
+        for k, v in st.items():
+            counts[k] = counts.get(k, 0) + v
+        log("bench: cpu baseline %d tiles, %.1f s" % (len(rects), elapsed))
+    paths = sum(w * h for (_, _, w, h) in rects) * spp
+    record = {"value": round(paths / elapsed / 1e6, 4), "unit": "Mpaths/s", "cores": threads, "kind": "port",
+              "sample": "%d random 32x32 tiles of the same image at all %d spp (%d paths, %d rays), %.1f s"
+                        % (len(rects), spp, paths, counts["rays"], elapsed),
+              "mrays_per_s": round(counts["rays"] / elapsed / 1e6, 4)}
+    return record, rects, film, counts
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this tracer runs on one GPU; --gpus %d is not supported (one process, one GPU)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from repackshot_amd import native
+
+    if native.device_count() == 0:
+        raise SystemExit("bench.py needs a GPU")
+    t0 = time.perf_counter()
+    rsb = native.write_scene(2, args.seed)
+    log("bench: scene generated (%.1f MB) in %.1f s" % (len(rsb) / 1e6, time.perf_counter() - t0))
+    t0 = time.perf_counter()
+    scene = native.Scene(rsb)
+    info = scene.info()
+    log("bench: scene loaded on the GPU in %.1f s: %s" % (time.perf_counter() - t0, info))
+    spp = info["max_samples"]
+    W, H = info["width"], info["height"]
+    paths_per_step = W * H * spp
+
+    scene.set_profiling(False)
+    for i in range(args.warmup):
+        scene.reset_film()
+        scene.render_samples(0, spp)
+        log("bench: warmup step %d done" % i)
+    # Timed steps: the library call is synchronous (it waits for its stream), so the wall clock
+    # brackets the GPU work exactly.
+    stats0 = scene.stats()
+    times = []
+    for i in range(args.steps):
+        scene.reset_film()
+        t1 = time.perf_counter()
+        scene.render_samples(0, spp)
+        times.append(time.perf_counter() - t1)
+        log("bench: step %d: %.3f s" % (i, times[-1]))
+    stats1 = scene.stats()
+    total = sum(times)
+    mpaths = paths_per_step * args.steps / total / 1e6
+    rays = stats1["rays"] - stats0["rays"]
+    nearest = stats1["nearest"] - stats0["nearest"]
+    mrays = rays / total / 1e6
+
+    # Per-kernel timing on the tracer's stream (HIP events) for one more step.
+    scene.set_profiling(True)
+    scene.reset_film()
+    s_before = scene.stats()
+    scene.render_samples(0, spp)
+    s_after = scene.stats()
+    timing = scene.last_timing()
+    scene.set_profiling(False)
+    nearest_prof = s_after["nearest"] - s_before["nearest"]
+    extend_avg_ms = timing["extend_ms"] / max(timing["extend_launches"], 1)
+
+    result = {
+        "metric": METRIC,
+        "value": round(mpaths, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * total / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene generator; no external assets)",
+        "config": {"workload": "single large render: 1024x1024, 64 spp, max depth 12, 8 displaced level-7 "
+                               "icospheres (2,621,440 tris), 4 rough conductors, 2 smooth glass, 2 Lambert, "
+                               "3 rect lights + studio dome, MIS NEE, f32 radiance",
+                   "width": W, "height": H, "spp": spp, "max_depth": info["max_depth"],
+                   "triangles": info["triangles"], "seed": args.seed},
+        "mrays_per_s": round(mrays, 3),
+        "rays_per_path": round(rays / (paths_per_step * args.steps), 4),
+        "stage_ms_per_step": {k: round(timing[k], 2) for k in ("extend_ms", "shade_ms", "shadow_ms", "total_ms")},
+    }
+
+    if not args.no_cpu_baseline:
+        cpu, rects, ofilm, counts = cpu_baseline(rsb, spp)
+        result["cpu_baseline"] = cpu
+        # Parity of the GPU film (samples 0..spp-1 of the profiled step) on the baseline's tiles.
+        rgb, weight, alpha, lum2 = scene.film()
+        exact = True
+        max_rel = 0.0
+        for (x, y, w, h) in rects:
+            for g, c in ((rgb, ofilm[0]), (weight, ofilm[1]), (alpha, ofilm[2]), (lum2, ofilm[3])):
+                gs, cs = g[y:y + h, x:x + w], c[y:y + h, x:x + w]
+                if not np.array_equal(gs.view(np.uint32), cs.view(np.uint32)):
+                    exact = False
+                    max_rel = max(max_rel, float(np.max(np.abs(gs - cs) / np.maximum(np.abs(cs), 1e-6))))
+        result["parity"] = {"tiles": len(rects), "pixels": int(sum(w * h for (_, _, w, h) in rects)),
+                            "film_bit_exact_vs_oracle": exact, "max_rel_err": max_rel}
+        # Algorithmic bytes of one closest-hit ray: the reference traversal's BVH8 nodes and packets
+        # visited per query, counted by the oracle over the baseline's rays.
+        b_ray = (NODE_BYTES * counts["nearest_nodes"] + PACKET_BYTES * counts["nearest_packets"]) / max(
+            counts["nearest"], 1)
+        per_launch = b_ray * nearest_prof / max(timing["extend_launches"], 1)
+        achieved = per_launch / (extend_avg_ms * 1e-3) / 1e9
+        result["roofline"] = {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "k_extend (closest hit)",
+            "bytes_per_ray": round(b_ray, 1),
+            "extend_avg_ms": round(extend_avg_ms, 4),
+            "extend_launches_per_step": timing["extend_launches"],
+        }
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

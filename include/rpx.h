@@ -67,12 +67,16 @@ uint32_t rpx_trace_rays(uint32_t handle, const float* rays, uint32_t n, uint32_t
                         uint8_t* any);
 /* width, height, max_samples, seed, max_depth, triangles, BVH nodes, instances. */
 uint32_t rpx_scene_info(uint32_t handle, uint64_t* out8);
-/* Rays cast since the handle was loaded (camera + extension + shadow, as the reference counts
- * them) and samples rendered. */
-uint32_t rpx_scene_stats(uint32_t handle, uint64_t* out2);
-/* Timing of the last rpx_render_samples call on the handle's stream, milliseconds: the whole
- * call and the sum of the closest-hit kernels. */
-uint32_t rpx_last_timing(uint32_t handle, float* out2);
+/* Since the handle was loaded: rays cast (camera + extension + shadow, as the reference counts
+ * them), of them closest-hit and shadow rays, and samples rendered. */
+uint32_t rpx_scene_stats(uint32_t handle, uint64_t* out4);
+/* Records per-stage GPU timing of the following rpx_render_samples calls (events on the
+ * handle's stream). */
+uint32_t rpx_set_profiling(uint32_t handle, uint32_t on);
+/* Timing of the last rpx_render_samples call, from events on the handle's stream: the whole
+ * call, the closest-hit, shade and shadow kernels (ms, summed), and the number of closest-hit and
+ * shadow launches. */
+uint32_t rpx_last_timing(uint32_t handle, float* out6);
 /* The GPU-generated energy-compensation tables of the current device (1024, 1024, 17408,
  * 17408 u16 and 32, 32 f32). */
 uint32_t rpx_energy_tables(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_in, uint16_t* clear_out,

@@ -42,6 +42,7 @@ def lib():
         L.orc_geometry_info.argtypes = [vp, vp]
         L.orc_render_tile.argtypes = [vp, u32, u32, u32, u32, u32, vp, vp, vp]
         L.orc_render_film.argtypes = [vp, u32, u32, ctypes.c_int, vp, vp, vp, vp, vp]
+        L.orc_render_tiles.argtypes = [vp, vp, ctypes.c_size_t, u32, u32, ctypes.c_int, vp, vp, vp, vp, vp]
         L.orc_trace_rays.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int, vp, vp, vp, vp]
         L.orc_energy_tables.argtypes = [vp] * 6
         L.orc_fmath.restype = f32
@@ -104,6 +105,20 @@ class OracleScene:
                     np.zeros((H, W), np.float32), np.zeros((H, W), np.float32))
         stats = np.zeros(7, np.uint64)
         lib().orc_render_film(self.h, first, count, threads, *(_p(a) for a in film), _p(stats))
+        keys = ["rays", "nearest", "any", "nearest_nodes", "nearest_packets", "any_nodes", "any_packets"]
+        return film, {k: int(v) for k, v in zip(keys, stats)}
+
+    def render_tiles(self, rects, first, count, threads=None, film=None):
+        """render_film over the listed (x, y, w, h) tiles only."""
+        if threads is None:
+            threads = os.cpu_count() or 1
+        H, W = self.height, self.width
+        if film is None:
+            film = (np.zeros((H, W, 3), np.float32), np.zeros((H, W), np.float32),
+                    np.zeros((H, W), np.float32), np.zeros((H, W), np.float32))
+        r = np.ascontiguousarray(rects, np.uint32).reshape(-1, 4)
+        stats = np.zeros(7, np.uint64)
+        lib().orc_render_tiles(self.h, _p(r), r.shape[0], first, count, threads, *(_p(a) for a in film), _p(stats))
         keys = ["rays", "nearest", "any", "nearest_nodes", "nearest_packets", "any_nodes", "any_packets"]
         return film, {k: int(v) for k, v in zip(keys, stats)}
 

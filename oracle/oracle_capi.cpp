@@ -104,12 +104,31 @@ int orc_render_tile(void* p, uint32_t x, uint32_t y, uint32_t w, uint32_t h, uin
 // (rgb: 3*W*H, weight/alpha/lum2: W*H; caller-initialised), 32-pixel tiles over `threads` workers.
 // stats (may be null): rays, nearest queries, any queries, nearest nodes, nearest packets,
 // any nodes, any packets.
+static int render_tiles(Handle* hd, const std::vector<TileRect>& tiles, uint32_t first, uint32_t count, int threads,
+                        float* rgb, float* weight, float* alpha, float* lum2, uint64_t* stats_out);
+
 int orc_render_film(void* p, uint32_t first, uint32_t count, int threads, float* rgb, float* weight, float* alpha,
                     float* lum2, uint64_t* stats_out) {
   auto* hd = static_cast<Handle*>(p);
   const Scene& s = hd->scene;
-  uint32_t W = s.render.width, H = s.render.height;
-  std::vector<TileRect> tiles = tile_grid(W, H, 32);
+  return render_tiles(hd, tile_grid(s.render.width, s.render.height, 32), first, count, threads, rgb, weight, alpha,
+                      lum2, stats_out);
+}
+
+// orc_render_film restricted to the listed tiles (x, y, w, h quadruples, inside the image).
+int orc_render_tiles(void* p, const uint32_t* rects, size_t n, uint32_t first, uint32_t count, int threads, float* rgb,
+                     float* weight, float* alpha, float* lum2, uint64_t* stats_out) {
+  std::vector<TileRect> tiles(n);
+  for (size_t i = 0; i < n; ++i) tiles[i] = TileRect{rects[4 * i], rects[4 * i + 1], rects[4 * i + 2], rects[4 * i + 3]};
+  return render_tiles(static_cast<Handle*>(p), tiles, first, count, threads, rgb, weight, alpha, lum2, stats_out);
+}
+
+}  // extern "C"
+
+static int render_tiles(Handle* hd, const std::vector<TileRect>& tiles, uint32_t first, uint32_t count, int threads,
+                        float* rgb, float* weight, float* alpha, float* lum2, uint64_t* stats_out) {
+  const Scene& s = hd->scene;
+  uint32_t W = s.render.width;
   std::atomic<size_t> next{0};
   std::vector<RenderStats> per(std::max(threads, 1));
   auto worker = [&](int tid) {
@@ -157,6 +176,8 @@ int orc_render_film(void* p, uint32_t first, uint32_t count, int threads, float*
   }
   return 0;
 }
+
+extern "C" {
 
 // Nearest and any hit of `n` rays given as 8 floats each (origin, direction, t_min, t_max).
 // Outputs per ray: instance, triangle (0xffffffff on a miss), t, any (0/1). `brute` selects the

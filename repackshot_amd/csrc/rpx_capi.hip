@@ -92,8 +92,24 @@ struct Handle {
   Film film{};
   std::vector<DevBuf> film_bufs;
   u64 samples = 0;
-  float last_ms = 0, last_extend_ms = 0;
+  float last_ms = 0;
+  // Per-stage timing of the last render call (profiling on): extend, shade, shadow ms and the
+  // number of extend and shadow launches.
+  bool profiling = false;
+  float stage_ms[3] = {0, 0, 0};
+  u32 stage_launches[2] = {0, 0};
+  std::vector<hipEvent_t> pool;
+  size_t pool_used = 0;
+  std::vector<std::pair<int, size_t>> marks;  // (stage, first event index)
   hipEvent_t ev[2] = {nullptr, nullptr};
+  hipEvent_t next_event() {
+    if (pool_used == pool.size()) {
+      hipEvent_t e;
+      hipEventCreate(&e);
+      pool.push_back(e);
+    }
+    return pool[pool_used++];
+  }
 
   ~Handle() {
     if (stream) hipStreamSynchronize(stream);
@@ -102,6 +118,7 @@ struct Handle {
         if (b.p) hipFree(b.p);
     for (hipEvent_t e : ev)
       if (e) hipEventDestroy(e);
+    for (hipEvent_t e : pool) hipEventDestroy(e);
     if (stream) hipStreamDestroy(stream);
   }
 };
@@ -237,17 +254,31 @@ hipError_t run_pass(Handle* h, u32 x0, u32 y0, u32 w, u32 hh, u32 sample_index) 
   A.sample_index = sample_index;
   for (u32 k = 0; k < 4; ++k) A.sobol_cam[k] = sobol_point(k, sample_index);
   hipError_t e = launch_camera(S, h->paths, A, h->stream);
+  auto mark = [&](int stage) {
+    if (!h->profiling) return;
+    h->marks.push_back({stage, h->pool_used});
+    hipEventRecord(h->next_event(), h->stream);
+  };
+  auto done = [&]() {
+    if (h->profiling) hipEventRecord(h->next_event(), h->stream);
+  };
   for (u32 round = 0; round <= S.max_depth && e == hipSuccess; ++round) {
+    mark(0);
     e = launch_extend(S, h->paths, round, n, h->stream);
+    done();
     if (e != hipSuccess) break;
     RoundArgs R;
     R.round = round;
     R.sample_index = sample_index;
     u32 dims = bounce_dim(round);
     for (u32 k = 0; k < 6; ++k) R.sobol[k] = sobol_point(dims + k, sample_index);
+    mark(1);
     e = launch_shade(S, h->paths, R, n, h->stream);
+    done();
     if (e != hipSuccess) break;
+    mark(2);
     e = launch_shadow(S, h->paths, n, h->stream);
+    done();
   }
   return e;
 }
@@ -256,7 +287,7 @@ u32 status_of(hipError_t e) { return e == hipSuccess ? RPX_OK : RPX_GPU_LAUNCH; 
 
 u32 load_into(Handle* h, const uint8_t* ptr, u32 len, bool keep_meshes, std::string& err) {
   HostScene s;
-  if (!parse_rsb(ptr, len, s, err)) return RPX_BAD_SCENE;
+  if (!parse_rsb(ptr, len, s, err, keep_meshes)) return RPX_BAD_SCENE;
   if (keep_meshes && s.meshes.empty()) {
     // rt_scene_update: a file without MESH sections keeps the loaded meshes (its INST sections,
     // or the implicit instances of the kept meshes, refer to them).
@@ -268,6 +299,11 @@ u32 load_into(Handle* h, const uint8_t* ptr, u32 len, bool keep_meshes, std::str
         in.mesh = (u32)i;
         in.material = s.meshes[i].material;
         s.instances.push_back(in);
+      }
+    for (const HostMesh& m : s.meshes)
+      if (m.material >= s.materials.size()) {
+        err = "a kept mesh refers to a material that does not exist";
+        return RPX_BAD_SCENE;
       }
     for (const HostInstance& in : s.instances)
       if (in.mesh >= s.meshes.size() || in.material >= s.materials.size()) {
@@ -425,6 +461,8 @@ uint32_t rpx_render_samples(uint32_t handle, uint32_t first, uint32_t count) {
     set_error(err);
     return RPX_GPU_MEMORY;
   }
+  h->pool_used = 0;
+  h->marks.clear();
   hipEventRecord(h->ev[0], h->stream);
   hipError_t e = hipSuccess;
   for (u32 k = 0; k < count && e == hipSuccess; ++k) {
@@ -438,6 +476,15 @@ uint32_t rpx_render_samples(uint32_t handle, uint32_t first, uint32_t count) {
     return RPX_GPU_LAUNCH;
   }
   hipEventElapsedTime(&h->last_ms, h->ev[0], h->ev[1]);
+  h->stage_ms[0] = h->stage_ms[1] = h->stage_ms[2] = 0;
+  h->stage_launches[0] = h->stage_launches[1] = 0;
+  for (auto& m : h->marks) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, h->pool[m.second], h->pool[m.second + 1]);
+    h->stage_ms[m.first] += ms;
+    if (m.first == 0) h->stage_launches[0]++;
+    if (m.first == 2) h->stage_launches[1]++;
+  }
   h->samples += count;
   return RPX_OK;
 }
@@ -542,28 +589,42 @@ uint32_t rpx_scene_info(uint32_t handle, uint64_t* out8) {
   return RPX_OK;
 }
 
-uint32_t rpx_scene_stats(uint32_t handle, uint64_t* out2) {
-  if (!out2) return RPX_NULL_POINTER;
+uint32_t rpx_scene_stats(uint32_t handle, uint64_t* out4) {
+  if (!out4) return RPX_NULL_POINTER;
   auto h = find(handle);
   if (!h) return RPX_BAD_HANDLE;
   std::lock_guard<std::mutex> lock(h->mutex);
   hipSetDevice(h->device);
-  u64 r[2] = {0, 0};
+  u64 r[3] = {0, 0, 0};
   if (h->paths.rays) {
-    hipMemcpyAsync(r, h->paths.rays, 16, hipMemcpyDeviceToHost, h->stream);
+    hipMemcpyAsync(r, h->paths.rays, 24, hipMemcpyDeviceToHost, h->stream);
     hipStreamSynchronize(h->stream);
   }
-  out2[0] = r[0] + r[1];
-  out2[1] = h->samples;
+  out4[0] = r[0] + r[1] + r[2];
+  out4[1] = r[0];
+  out4[2] = r[1] + r[2];
+  out4[3] = h->samples;
   return RPX_OK;
 }
 
-uint32_t rpx_last_timing(uint32_t handle, float* out2) {
-  if (!out2) return RPX_NULL_POINTER;
+uint32_t rpx_set_profiling(uint32_t handle, uint32_t on) {
   auto h = find(handle);
   if (!h) return RPX_BAD_HANDLE;
-  out2[0] = h->last_ms;
-  out2[1] = h->last_extend_ms;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  h->profiling = on != 0;
+  return RPX_OK;
+}
+
+uint32_t rpx_last_timing(uint32_t handle, float* out6) {
+  if (!out6) return RPX_NULL_POINTER;
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  out6[0] = h->last_ms;
+  out6[1] = h->stage_ms[0];
+  out6[2] = h->stage_ms[1];
+  out6[3] = h->stage_ms[2];
+  out6[4] = (float)h->stage_launches[0];
+  out6[5] = (float)h->stage_launches[1];
   return RPX_OK;
 }
 

@@ -24,7 +24,7 @@ struct PathState {
   u32* queue[2];   // extension queues (ping-pong by round)
   u32* shadow_queue;
   u32* counts;     // [0], [1] queue lengths, [2] shadow queue length
-  u64* rays;       // [0] rays traced (extension + shadow), [1] shadow rays counted without a trace
+  u64* rays;       // [0] closest-hit rays traced, [1] shadow rays counted without a trace, [2] shadow rays traced
 };
 
 struct Film {

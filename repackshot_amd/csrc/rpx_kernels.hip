@@ -419,7 +419,7 @@ __global__ void __launch_bounds__(BLOCK) k_shadow(DevScene S, PathState P) {
   __shared__ u32 lds[LDS_STACK * BLOCK];
   u32* col = lds + threadIdx.x;
   u32 n = P.counts[2];
-  if (blockIdx.x == 0 && threadIdx.x == 0) P.rays[0] += n;
+  if (blockIdx.x == 0 && threadIdx.x == 0) P.rays[2] += n;
   u32 stride = gridDim.x * blockDim.x;
   for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
     u32 i = P.shadow_queue[j];

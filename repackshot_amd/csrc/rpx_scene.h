@@ -160,8 +160,10 @@ struct HostScene {
   float clamp_indirect = 0;
 };
 
-// Parses the RSB subset the tracer follows; false with a message for anything else.
-bool parse_rsb(const uint8_t* bytes, size_t len, HostScene& out, std::string& err);
+// Parses the RSB subset the tracer follows; false with a message for anything else. With
+// `meshes_may_follow`, a file without MESH sections keeps its instances unchecked (an update that
+// reuses the loaded meshes checks them).
+bool parse_rsb(const uint8_t* bytes, size_t len, HostScene& out, std::string& err, bool meshes_may_follow = false);
 
 // The flattened scene ready for upload (host arrays) plus its DevScene constants.
 struct FlatScene {

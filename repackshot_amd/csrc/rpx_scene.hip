@@ -48,7 +48,7 @@ struct Reader {
 };
 }  // namespace
 
-bool parse_rsb(const uint8_t* bytes, size_t len, HostScene& s, std::string& err) {
+bool parse_rsb(const uint8_t* bytes, size_t len, HostScene& s, std::string& err, bool meshes_may_follow) {
   if (!bytes) {
     err = "null pointer";
     return false;
@@ -254,6 +254,7 @@ bool parse_rsb(const uint8_t* bytes, size_t len, HostScene& s, std::string& err)
   }
   for (const HostMesh& m : s.meshes)
     if (m.material >= s.materials.size()) { err = "material_id does not refer to a MATL section"; return false; }
+  if (meshes_may_follow && s.meshes.empty()) return true;
   if (s.instances.empty()) {
     for (size_t i = 0; i < s.meshes.size(); ++i) {
       HostInstance in;

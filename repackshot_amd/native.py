@@ -46,6 +46,7 @@ _SIGS = {
     "rpx_trace_rays": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 4),
     "rpx_scene_info": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_scene_stats": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
+    "rpx_set_profiling": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
     "rpx_last_timing": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_energy_tables": (ctypes.c_uint32, [ctypes.c_void_p] * 6),
     "rpx_write_scene": (ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64]),
@@ -228,14 +229,19 @@ class Scene:
         return inst, tri, t, anyhit
 
     def stats(self):
-        out = np.zeros(2, np.uint64)
+        out = np.zeros(4, np.uint64)
         _check(library().rpx_scene_stats(self.handle, _ptr(out)), "rpx_scene_stats")
-        return {"rays": int(out[0]), "samples": int(out[1])}
+        return {"rays": int(out[0]), "nearest": int(out[1]), "shadow": int(out[2]), "samples": int(out[3])}
 
-    def last_timing_ms(self):
-        out = np.zeros(2, np.float32)
+    def set_profiling(self, on=True):
+        _check(library().rpx_set_profiling(self.handle, 1 if on else 0), "rpx_set_profiling")
+
+    def last_timing(self):
+        """ms of the last render call: total, closest-hit, shade and shadow kernels; launch counts."""
+        out = np.zeros(6, np.float32)
         _check(library().rpx_last_timing(self.handle, _ptr(out)), "rpx_last_timing")
-        return float(out[0])
+        return {"total_ms": float(out[0]), "extend_ms": float(out[1]), "shade_ms": float(out[2]),
+                "shadow_ms": float(out[3]), "extend_launches": int(out[4]), "shadow_launches": int(out[5])}
 
 
 def resolve(rgb, weight):
