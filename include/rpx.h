@@ -70,13 +70,18 @@ uint32_t rpx_scene_info(uint32_t handle, uint64_t* out8);
 /* Since the handle was loaded: rays cast (camera + extension + shadow, as the reference counts
  * them), of them closest-hit and shadow rays, and samples rendered. */
 uint32_t rpx_scene_stats(uint32_t handle, uint64_t* out4);
-/* Records per-stage GPU timing of the following rpx_render_samples calls (events on the
- * handle's stream). */
+/* Pool options of a handle's renders: path slots (0 keeps the current), traversal launch shape
+ * (index; 0xffffffff keeps), iterations per completion check (0 keeps), most path samples per
+ * chunk (0 keeps; at most 2^27), flags: bit 0 one sample pass at a time (the pool drains after
+ * each pass), bit 1 each group of iterations launched as a captured HIP graph. Films do not
+ * depend on any of them. */
+uint32_t rpx_set_options(uint32_t handle, uint32_t pool, uint32_t shape, uint32_t group, uint32_t chunk,
+                         uint32_t flags);
+/* Records per-stage GPU timing of the following renders (events on the handle's stream). */
 uint32_t rpx_set_profiling(uint32_t handle, uint32_t on);
-/* Timing of the last rpx_render_samples call, from events on the handle's stream: the whole
- * call, the closest-hit, shade and shadow kernels (ms, summed), and the number of closest-hit and
- * shadow launches. */
-uint32_t rpx_last_timing(uint32_t handle, float* out6);
+/* Timing of the last render: total ms, then ms of the closest-hit, shade, shadow and regen
+ * kernels (summed; with profiling on), their launch counts, pool iterations and chunks. */
+uint32_t rpx_last_timing(uint32_t handle, float* out12);
 /* The GPU-generated energy-compensation tables of the current device (1024, 1024, 17408,
  * 17408 u16 and 32, 32 f32). */
 uint32_t rpx_energy_tables(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_in, uint16_t* clear_out,
@@ -86,6 +91,19 @@ uint32_t rpx_energy_tables(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_
  * kind 0: the reference's example scene (example.rs:113-230); 1: icosphere on a floor
  * (256x256); 2: eight displaced level-7 icospheres (1024x1024). */
 uint64_t rpx_write_scene(uint32_t kind, uint32_t seed, uint8_t* buf, uint64_t cap);
+/* Render queues: jobs (scene handle, samples [first, first + count) of its whole image) rendered
+ * together by one persistent path pool, each into its own resident film. Films equal those of
+ * rpx_render_samples on the same scene and samples. */
+uint32_t rpx_queue_create(void); /* 0 on error */
+uint32_t rpx_queue_add(uint32_t queue, uint32_t scene, uint32_t first, uint32_t count, uint32_t* job);
+uint32_t rpx_queue_run(uint32_t queue); /* renders every job once more (films accumulate) */
+uint32_t rpx_queue_film_read(uint32_t queue, uint32_t job, float* rgb, float* weight, float* alpha, float* lum2);
+uint32_t rpx_queue_set_options(uint32_t queue, uint32_t pool, uint32_t shape, uint32_t group, uint32_t chunk,
+                               uint32_t flags);
+uint32_t rpx_queue_set_profiling(uint32_t queue, uint32_t on);
+uint32_t rpx_queue_last_timing(uint32_t queue, float* out12);
+uint32_t rpx_queue_stats(uint32_t queue, uint64_t* out4);
+uint32_t rpx_queue_free(uint32_t queue);
 /* Number of HIP devices visible (0 without a GPU). */
 uint32_t rpx_device_count(void);
 

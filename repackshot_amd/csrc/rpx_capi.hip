@@ -1,13 +1,14 @@
 // The C ABI of the GPU tracer (include/rpx.h): a registry of scene handles, each owning its
-// device scene, wavefront path buffers, resident film and HIP stream.
-#include <atomic>
+// device scene, path pool, resident film and HIP stream, and of render queues (several scenes'
+// jobs rendered by one pool).
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
 #include "../../include/rpx.h"
-#include "rpx_kernels.h"
+#include "rpx_engine.h"
 
 namespace rpx {
 namespace {
@@ -15,110 +16,26 @@ namespace {
 thread_local std::string g_error;
 void set_error(const std::string& e) { g_error = e; }
 
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-};
-
-// Energy tables of one device: generated there by the recipe kernel on first use; the host copy
-// prepares the BSDFs.
-struct DeviceTables {
-  bool ready = false;
-  uint16_t *albedo = nullptr, *schlick = nullptr, *clear_in = nullptr, *clear_out = nullptr;
-  float *albedo_avg = nullptr, *schlick_avg = nullptr;
-  std::vector<uint16_t> h_albedo, h_schlick, h_clear_in, h_clear_out;
-  std::vector<float> h_albedo_avg, h_schlick_avg;
-  EnergyTables host_view() const {
-    return EnergyTables{h_albedo.data(), h_schlick.data(), h_clear_in.data(), h_clear_out.data(),
-                        h_albedo_avg.data(), h_schlick_avg.data()};
-  }
-  EnergyTables device_view() const { return EnergyTables{albedo, schlick, clear_in, clear_out, albedo_avg, schlick_avg}; }
-};
-std::mutex g_tables_mutex;
-std::map<int, DeviceTables> g_tables;
-
-bool device_tables(int dev, DeviceTables*& out, std::string& err) {
-  std::lock_guard<std::mutex> lock(g_tables_mutex);
-  DeviceTables& t = g_tables[dev];
-  if (!t.ready) {
-    const size_t SL = 1024, CL = 17408;
-    if (hipMalloc(&t.albedo, SL * 2) != hipSuccess || hipMalloc(&t.schlick, SL * 2) != hipSuccess ||
-        hipMalloc(&t.clear_in, CL * 2) != hipSuccess || hipMalloc(&t.clear_out, CL * 2) != hipSuccess ||
-        hipMalloc(&t.albedo_avg, 32 * 4) != hipSuccess || hipMalloc(&t.schlick_avg, 32 * 4) != hipSuccess) {
-      err = "device memory for the energy tables";
-      return false;
-    }
-    hipError_t e = launch_energy(t.albedo, t.schlick, t.clear_in, t.clear_out, t.albedo_avg, t.schlick_avg, 0);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e != hipSuccess) {
-      err = std::string("energy table kernel: ") + hipGetErrorString(e);
-      return false;
-    }
-    t.h_albedo.resize(SL);
-    t.h_schlick.resize(SL);
-    t.h_clear_in.resize(CL);
-    t.h_clear_out.resize(CL);
-    t.h_albedo_avg.resize(32);
-    t.h_schlick_avg.resize(32);
-    hipMemcpy(t.h_albedo.data(), t.albedo, SL * 2, hipMemcpyDeviceToHost);
-    hipMemcpy(t.h_schlick.data(), t.schlick, SL * 2, hipMemcpyDeviceToHost);
-    hipMemcpy(t.h_clear_in.data(), t.clear_in, CL * 2, hipMemcpyDeviceToHost);
-    hipMemcpy(t.h_clear_out.data(), t.clear_out, CL * 2, hipMemcpyDeviceToHost);
-    hipMemcpy(t.h_albedo_avg.data(), t.albedo_avg, 32 * 4, hipMemcpyDeviceToHost);
-    e = hipMemcpy(t.h_schlick_avg.data(), t.schlick_avg, 32 * 4, hipMemcpyDeviceToHost);
-    if (e != hipSuccess) {
-      err = std::string("energy table copy: ") + hipGetErrorString(e);
-      return false;
-    }
-    t.ready = true;
-  }
-  out = &t;
-  return true;
-}
-
 struct Handle {
   int device = 0;
   hipStream_t stream = nullptr;
   std::mutex mutex;
   HostScene host;
   DevScene dev{};
+  DevScene* d_scene = nullptr;  // device copy of `dev` (a one-entry scene table)
   std::vector<DevBuf> scene_bufs;
   u64 triangles = 0, nodes = 0;
-  // wavefront buffers
-  u32 capacity = 0;
-  PathState paths{};
-  std::vector<DevBuf> path_bufs;
+  std::unique_ptr<Engine> engine;
   // resident film
   Film film{};
   std::vector<DevBuf> film_bufs;
   u64 samples = 0;
-  float last_ms = 0;
-  // Per-stage timing of the last render call (profiling on): extend, shade, shadow ms and the
-  // number of extend and shadow launches.
-  bool profiling = false;
-  float stage_ms[3] = {0, 0, 0};
-  u32 stage_launches[2] = {0, 0};
-  std::vector<hipEvent_t> pool;
-  size_t pool_used = 0;
-  std::vector<std::pair<int, size_t>> marks;  // (stage, first event index)
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  hipEvent_t next_event() {
-    if (pool_used == pool.size()) {
-      hipEvent_t e;
-      hipEventCreate(&e);
-      pool.push_back(e);
-    }
-    return pool[pool_used++];
-  }
 
   ~Handle() {
     if (stream) hipStreamSynchronize(stream);
-    for (auto* v : {&scene_bufs, &path_bufs, &film_bufs})
-      for (DevBuf& b : *v)
-        if (b.p) hipFree(b.p);
-    for (hipEvent_t e : ev)
-      if (e) hipEventDestroy(e);
-    for (hipEvent_t e : pool) hipEventDestroy(e);
+    engine.reset();
+    dev_free_all(scene_bufs);
+    dev_free_all(film_bufs);
     if (stream) hipStreamDestroy(stream);
   }
 };
@@ -135,28 +52,16 @@ std::shared_ptr<Handle> find(u32 h) {
 
 template <class T>
 bool upload(std::vector<DevBuf>& bufs, const std::vector<T>& v, const T*& out, hipStream_t st, std::string& err) {
-  size_t bytes = std::max<size_t>(v.size() * sizeof(T), 16);
-  DevBuf b;
-  if (hipMalloc(&b.p, bytes) != hipSuccess) {
+  void* p = nullptr;
+  if (!dev_alloc(bufs, std::max<size_t>(v.size() * sizeof(T), 16), &p)) {
     err = "device memory exhausted while uploading the scene";
     return false;
   }
-  b.bytes = bytes;
-  bufs.push_back(b);
-  if (!v.empty() && hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess) {
+  if (!v.empty() && hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess) {
     err = "scene upload failed";
     return false;
   }
-  out = static_cast<const T*>(b.p);
-  return true;
-}
-
-bool alloc(std::vector<DevBuf>& bufs, size_t bytes, void** out) {
-  DevBuf b;
-  if (hipMalloc(&b.p, std::max<size_t>(bytes, 16)) != hipSuccess) return false;
-  b.bytes = bytes;
-  bufs.push_back(b);
-  *out = b.p;
+  out = static_cast<const T*>(p);
   return true;
 }
 
@@ -167,9 +72,18 @@ bool build_device_scene(Handle* h, std::string& err) {
   FlatScene flat;
   unsigned hw = std::thread::hardware_concurrency();
   if (!flatten_scene(h->host, tables->host_view(), flat, err, hw ? (int)std::min(hw, 16u) : 4)) return false;
-  for (DevBuf& b : h->scene_bufs)
-    if (b.p) hipFree(b.p);
-  h->scene_bufs.clear();
+  // Traversal stack bound: at most three entries per 4-wide level.
+  u32 bound = 3 * (flat.max_tlas_depth + flat.max_blas_depth) + 2;
+  if (bound > 64) {
+    err = "the BVH is too deep for the traversal stack";
+    return false;
+  }
+  if (flat.consts.width > 65535 || flat.consts.height > 65535) {
+    err = "the image is wider or taller than 65535 pixels";
+    return false;
+  }
+  hipStreamSynchronize(h->stream);
+  dev_free_all(h->scene_bufs);
   DevScene d = flat.consts;
   auto& B = h->scene_bufs;
   hipStream_t st = h->stream;
@@ -182,45 +96,17 @@ bool build_device_scene(Handle* h, std::string& err) {
       !upload(B, flat.entries, d.entries, st, err))
     return false;
   d.tables = tables->device_view();
+  std::vector<DevScene> one(1, d);
+  const DevScene* ds = nullptr;
+  if (!upload(B, one, ds, st, err)) return false;
   if (hipStreamSynchronize(st) != hipSuccess) {
     err = "scene upload failed";
     return false;
   }
   h->dev = d;
+  h->d_scene = const_cast<DevScene*>(ds);
   h->triangles = flat.triangles;
   h->nodes = flat.nodes.size();
-  // Traversal stack bound: at most three entries per 4-wide level (LDS 32 + private 32).
-  u32 bound = 3 * (flat.max_tlas_depth + flat.max_blas_depth) + 2;
-  if (bound > 64) {
-    err = "the BVH is too deep for the traversal stack";
-    return false;
-  }
-  return true;
-}
-
-bool ensure_paths(Handle* h, u32 n, std::string& err) {
-  if (n <= h->capacity) return true;
-  for (DevBuf& b : h->path_bufs)
-    if (b.p) hipFree(b.p);
-  h->path_bufs.clear();
-  h->capacity = 0;
-  PathState& P = h->paths;
-  auto& B = h->path_bufs;
-  size_t f4 = (size_t)n * sizeof(float4);
-  bool ok = alloc(B, f4, (void**)&P.ray_o) && alloc(B, f4, (void**)&P.ray_d) && alloc(B, f4, (void**)&P.hit) &&
-            alloc(B, f4, (void**)&P.thr) && alloc(B, f4, (void**)&P.sum) && alloc(B, f4, (void**)&P.med) &&
-            alloc(B, f4, (void**)&P.sel_p) && alloc(B, f4, (void**)&P.sel_w0) && alloc(B, f4, (void**)&P.sel_w1) &&
-            alloc(B, f4, (void**)&P.sh_o) && alloc(B, f4, (void**)&P.sh_d) && alloc(B, f4, (void**)&P.sh_c) &&
-            alloc(B, (size_t)n * 8, (void**)&P.key) && alloc(B, (size_t)n * 4, (void**)&P.queue[0]) &&
-            alloc(B, (size_t)n * 4, (void**)&P.queue[1]) && alloc(B, (size_t)n * 4, (void**)&P.shadow_queue) &&
-            alloc(B, 64, (void**)&P.counts) && alloc(B, 64, (void**)&P.rays);
-  if (!ok) {
-    err = "device memory exhausted for the path buffers";
-    return false;
-  }
-  hipMemsetAsync(P.counts, 0, 64, h->stream);
-  hipMemsetAsync(P.rays, 0, 64, h->stream);
-  h->capacity = n;
   return true;
 }
 
@@ -229,8 +115,8 @@ bool ensure_film(Handle* h, std::string& err) {
   size_t px = (size_t)h->dev.width * h->dev.height;
   auto& B = h->film_bufs;
   Film& F = h->film;
-  if (!alloc(B, px * 12, (void**)&F.rgb) || !alloc(B, px * 4, (void**)&F.weight) || !alloc(B, px * 4, (void**)&F.alpha) ||
-      !alloc(B, px * 4, (void**)&F.lum2)) {
+  if (!dev_alloc(B, px * 12, (void**)&F.rgb) || !dev_alloc(B, px * 4, (void**)&F.weight) ||
+      !dev_alloc(B, px * 4, (void**)&F.alpha) || !dev_alloc(B, px * 4, (void**)&F.lum2)) {
     err = "device memory exhausted for the film";
     return false;
   }
@@ -241,46 +127,18 @@ bool ensure_film(Handle* h, std::string& err) {
   return true;
 }
 
-// One sample pass over the pixel rectangle (x0, y0, w, h): camera, then max_depth + 1 rounds of
-// extend / shade / shadow. Path i is pixel (x0 + i % w, y0 + i / w).
-hipError_t run_pass(Handle* h, u32 x0, u32 y0, u32 w, u32 hh, u32 sample_index) {
-  const DevScene& S = h->dev;
-  u32 n = w * hh;
-  PassArgs A;
-  A.n_paths = n;
-  A.x0 = x0;
-  A.y0 = y0;
-  A.w = w;
-  A.sample_index = sample_index;
-  for (u32 k = 0; k < 4; ++k) A.sobol_cam[k] = sobol_point(k, sample_index);
-  hipError_t e = launch_camera(S, h->paths, A, h->stream);
-  auto mark = [&](int stage) {
-    if (!h->profiling) return;
-    h->marks.push_back({stage, h->pool_used});
-    hipEventRecord(h->next_event(), h->stream);
-  };
-  auto done = [&]() {
-    if (h->profiling) hipEventRecord(h->next_event(), h->stream);
-  };
-  for (u32 round = 0; round <= S.max_depth && e == hipSuccess; ++round) {
-    mark(0);
-    e = launch_extend(S, h->paths, round, n, h->stream);
-    done();
-    if (e != hipSuccess) break;
-    RoundArgs R;
-    R.round = round;
-    R.sample_index = sample_index;
-    u32 dims = bounce_dim(round);
-    for (u32 k = 0; k < 6; ++k) R.sobol[k] = sobol_point(dims + k, sample_index);
-    mark(1);
-    e = launch_shade(S, h->paths, R, n, h->stream);
-    done();
-    if (e != hipSuccess) break;
-    mark(2);
-    e = launch_shadow(S, h->paths, n, h->stream);
-    done();
-  }
-  return e;
+DevSegment whole_image(const Handle* h, u32 scene, u32 first, u32 count) {
+  DevSegment s{};
+  s.scene = scene;
+  s.x0 = 0;
+  s.y0 = 0;
+  s.w = h->dev.width;
+  s.h = h->dev.height;
+  s.first = first;
+  s.count = count;
+  s.film = h->film;
+  s.film_w = h->dev.width;
+  return s;
 }
 
 u32 status_of(hipError_t e) { return e == hipSuccess ? RPX_OK : RPX_GPU_LAUNCH; }
@@ -319,6 +177,62 @@ u32 load_into(Handle* h, const uint8_t* ptr, u32 len, bool keep_meshes, std::str
   }
   return RPX_OK;
 }
+
+// A render queue: jobs (scene handle, sample range) rendered by one path pool, each into its own
+// resident film.
+struct Job {
+  std::shared_ptr<Handle> scene;
+  u32 first, count;
+  u32 width, height;
+  Film film;
+};
+
+struct Queue {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mutex;
+  std::vector<Job> jobs;
+  std::vector<DevBuf> film_bufs;
+  DevBuf scenes;  // device scene table of the last run
+  std::unique_ptr<Engine> engine;
+  ~Queue() {
+    if (stream) hipStreamSynchronize(stream);
+    engine.reset();
+    dev_free_all(film_bufs);
+    if (scenes.p) hipFree(scenes.p);
+    if (stream) hipStreamDestroy(stream);
+  }
+};
+
+std::map<u32, std::shared_ptr<Queue>> g_queues;
+u32 g_next_queue = 1;
+
+std::shared_ptr<Queue> find_queue(u32 q) {
+  std::lock_guard<std::mutex> lock(g_registry_mutex);
+  auto it = g_queues.find(q);
+  return it == g_queues.end() ? nullptr : it->second;
+}
+
+void apply_options(Engine& e, u32 pool, u32 shape, u32 group, u32 chunk, u32 flags) {
+  if (pool) e.opt.pool = std::max<u32>(pool, 64);
+  if (chunk) e.opt.chunk = std::min(chunk, RESULTS_MAX);
+  if (shape < (u32)N_SHAPES) e.opt.shape = (int)shape;
+  if (group) e.opt.group = std::max<u32>(2, group & ~1u);
+  e.opt.per_pass = (flags & 1u) != 0;
+  e.opt.graph = (flags & 2u) != 0;
+}
+
+void fill_timing(const Engine& e, float* out12) {
+  const Timing& t = e.timing;
+  out12[0] = t.total_ms;
+  for (int k = 0; k < 4; ++k) {
+    out12[1 + k] = t.stage_ms[k];
+    out12[5 + k] = (float)t.launches[k];
+  }
+  out12[9] = (float)t.iterations;
+  out12[10] = (float)t.chunks;
+  out12[11] = 0.0f;
+}
 }  // namespace
 }  // namespace rpx
 
@@ -346,12 +260,16 @@ uint32_t rt_scene_load(const uint8_t* ptr, uint32_t len) {
   }
   auto h = std::make_shared<Handle>();
   hipGetDevice(&h->device);
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&h->ev[0]) != hipSuccess || hipEventCreate(&h->ev[1]) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("rt_scene_load: cannot create a HIP stream");
     return 0;
   }
   std::string err;
+  h->engine = std::make_unique<Engine>();
+  if (!h->engine->init(h->device, h->stream, err)) {
+    set_error("rt_scene_load: " + err);
+    return 0;
+  }
   if (load_into(h.get(), ptr, len, false, err) != RPX_OK) {
     set_error(err);
     return 0;
@@ -376,12 +294,11 @@ uint32_t rt_scene_update(uint32_t handle, const uint8_t* ptr, uint32_t len) {
   u32 s = load_into(h.get(), ptr, len, true, err);
   if (s != RPX_OK) {
     set_error(err);
-    return s == RPX_BAD_SCENE ? RPX_BAD_SCENE : s;
+    return s;
   }
   // A new scene starts a new film.
-  for (DevBuf& b : h->film_bufs)
-    if (b.p) hipFree(b.p);
-  h->film_bufs.clear();
+  hipStreamSynchronize(h->stream);
+  dev_free_all(h->film_bufs);
   h->film = Film{};
   h->samples = 0;
   return RPX_OK;
@@ -414,27 +331,35 @@ static uint32_t render_tile_impl(uint32_t handle, uint32_t x, uint32_t y, uint32
   if (w == 0 || hh == 0 || (u64)w * hh > (1ull << 22)) return RPX_BAD_TILE;
   std::lock_guard<std::mutex> lock(h->mutex);
   hipSetDevice(h->device);
-  std::string err;
   // Pixels outside the image are written as zero (they are not rendered).
   u32 W = h->dev.width, H = h->dev.height;
   std::memset(out, 0, (size_t)w * hh * channels * 4);
   if (x >= W || y >= H) return RPX_OK;
   u32 cw = std::min(w, W - x), ch = std::min(hh, H - y);
-  if (!ensure_paths(h.get(), cw * ch, err)) {
-    set_error(err);
+  DevSegment s{};
+  s.scene = 0;
+  s.x0 = x;
+  s.y0 = y;
+  s.w = cw;
+  s.h = ch;
+  s.first = sample_index;
+  s.count = 1;
+  s.film_w = W;
+  std::string err;
+  float* d_out = nullptr;
+  if (hipMalloc(&d_out, (size_t)cw * ch * channels * 4) != hipSuccess) {
+    set_error("device memory exhausted for the tile");
     return RPX_GPU_MEMORY;
   }
-  float* d_out = nullptr;
-  if (hipMalloc(&d_out, (size_t)cw * ch * channels * 4) != hipSuccess) return RPX_GPU_MEMORY;
-  hipError_t e = run_pass(h.get(), x, y, cw, ch, sample_index);
-  if (e == hipSuccess) e = launch_tile_out(h->paths, d_out, cw * ch, channels, h->stream);
+  hipError_t e = h->engine->run(h->d_scene, std::vector<DevSegment>(1, s), false, err);
+  if (e == hipSuccess) e = launch_tile_out(h->engine->P, d_out, cw * ch, channels, h->stream);
   std::vector<float> tmp((size_t)cw * ch * channels);
   if (e == hipSuccess) e = hipMemcpyAsync(tmp.data(), d_out, tmp.size() * 4, hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   hipFree(d_out);
   if (e != hipSuccess) {
-    set_error(std::string("render: ") + hipGetErrorString(e));
-    return RPX_GPU_LAUNCH;
+    set_error(err.empty() ? std::string("render: ") + hipGetErrorString(e) : err);
+    return e == hipErrorOutOfMemory ? RPX_GPU_MEMORY : RPX_GPU_LAUNCH;
   }
   for (u32 j = 0; j < ch; ++j)
     std::memcpy(out + (size_t)j * w * channels, tmp.data() + (size_t)j * cw * channels, (size_t)cw * channels * 4);
@@ -453,37 +378,18 @@ uint32_t rt_render_tile_rgba(uint32_t handle, uint32_t x, uint32_t y, uint32_t w
 uint32_t rpx_render_samples(uint32_t handle, uint32_t first, uint32_t count) {
   auto h = find(handle);
   if (!h) return RPX_BAD_HANDLE;
+  if (count == 0) return RPX_OK;
   std::lock_guard<std::mutex> lock(h->mutex);
   hipSetDevice(h->device);
   std::string err;
-  u32 n = h->dev.width * h->dev.height;
-  if (!ensure_paths(h.get(), n, err) || !ensure_film(h.get(), err)) {
+  if (!ensure_film(h.get(), err)) {
     set_error(err);
     return RPX_GPU_MEMORY;
   }
-  h->pool_used = 0;
-  h->marks.clear();
-  hipEventRecord(h->ev[0], h->stream);
-  hipError_t e = hipSuccess;
-  for (u32 k = 0; k < count && e == hipSuccess; ++k) {
-    e = run_pass(h.get(), 0, 0, h->dev.width, h->dev.height, first + k);
-    if (e == hipSuccess) e = launch_splat(h->paths, h->film, n, h->stream);
-  }
-  hipEventRecord(h->ev[1], h->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  hipError_t e = h->engine->run(h->d_scene, std::vector<DevSegment>(1, whole_image(h.get(), 0, first, count)), true, err);
   if (e != hipSuccess) {
-    set_error(std::string("render: ") + hipGetErrorString(e));
-    return RPX_GPU_LAUNCH;
-  }
-  hipEventElapsedTime(&h->last_ms, h->ev[0], h->ev[1]);
-  h->stage_ms[0] = h->stage_ms[1] = h->stage_ms[2] = 0;
-  h->stage_launches[0] = h->stage_launches[1] = 0;
-  for (auto& m : h->marks) {
-    float ms = 0;
-    hipEventElapsedTime(&ms, h->pool[m.second], h->pool[m.second + 1]);
-    h->stage_ms[m.first] += ms;
-    if (m.first == 0) h->stage_launches[0]++;
-    if (m.first == 2) h->stage_launches[1]++;
+    set_error(err);
+    return e == hipErrorOutOfMemory ? RPX_GPU_MEMORY : RPX_GPU_LAUNCH;
   }
   h->samples += count;
   return RPX_OK;
@@ -507,6 +413,17 @@ uint32_t rpx_film_reset(uint32_t handle) {
   return hipStreamSynchronize(h->stream) == hipSuccess ? RPX_OK : RPX_GPU_LAUNCH;
 }
 
+static uint32_t read_film(const Film& F, size_t px, hipStream_t st, float* rgb, float* weight, float* alpha,
+                          float* lum2) {
+  hipError_t e = hipSuccess;
+  if (rgb && e == hipSuccess) e = hipMemcpyAsync(rgb, F.rgb, px * 12, hipMemcpyDeviceToHost, st);
+  if (weight && e == hipSuccess) e = hipMemcpyAsync(weight, F.weight, px * 4, hipMemcpyDeviceToHost, st);
+  if (alpha && e == hipSuccess) e = hipMemcpyAsync(alpha, F.alpha, px * 4, hipMemcpyDeviceToHost, st);
+  if (lum2 && e == hipSuccess) e = hipMemcpyAsync(lum2, F.lum2, px * 4, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e == hipSuccess ? RPX_OK : RPX_GPU_LAUNCH;
+}
+
 uint32_t rpx_film_read(uint32_t handle, float* rgb, float* weight, float* alpha, float* lum2) {
   auto h = find(handle);
   if (!h) return RPX_BAD_HANDLE;
@@ -517,14 +434,7 @@ uint32_t rpx_film_read(uint32_t handle, float* rgb, float* weight, float* alpha,
     set_error(err);
     return RPX_GPU_MEMORY;
   }
-  size_t px = (size_t)h->dev.width * h->dev.height;
-  hipError_t e = hipSuccess;
-  if (rgb && e == hipSuccess) e = hipMemcpyAsync(rgb, h->film.rgb, px * 12, hipMemcpyDeviceToHost, h->stream);
-  if (weight && e == hipSuccess) e = hipMemcpyAsync(weight, h->film.weight, px * 4, hipMemcpyDeviceToHost, h->stream);
-  if (alpha && e == hipSuccess) e = hipMemcpyAsync(alpha, h->film.alpha, px * 4, hipMemcpyDeviceToHost, h->stream);
-  if (lum2 && e == hipSuccess) e = hipMemcpyAsync(lum2, h->film.lum2, px * 4, hipMemcpyDeviceToHost, h->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  return e == hipSuccess ? RPX_OK : RPX_GPU_LAUNCH;
+  return read_film(h->film, (size_t)h->dev.width * h->dev.height, h->stream, rgb, weight, alpha, lum2);
 }
 
 uint32_t rpx_film_device(uint32_t handle, void** ptrs4) {
@@ -556,21 +466,21 @@ uint32_t rpx_trace_rays(uint32_t handle, const float* rays, uint32_t n, uint32_t
   hipSetDevice(h->device);
   void *d_rays = nullptr, *d_inst = nullptr, *d_tri = nullptr, *d_t = nullptr, *d_any = nullptr;
   std::vector<DevBuf> tmp;
-  if (!alloc(tmp, (size_t)n * 32, &d_rays) || !alloc(tmp, (size_t)n * 4, &d_inst) || !alloc(tmp, (size_t)n * 4, &d_tri) ||
-      !alloc(tmp, (size_t)n * 4, &d_t) || !alloc(tmp, (size_t)n, &d_any)) {
-    for (DevBuf& b : tmp) hipFree(b.p);
+  if (!dev_alloc(tmp, (size_t)n * 32, &d_rays) || !dev_alloc(tmp, (size_t)n * 4, &d_inst) ||
+      !dev_alloc(tmp, (size_t)n * 4, &d_tri) || !dev_alloc(tmp, (size_t)n * 4, &d_t) || !dev_alloc(tmp, (size_t)n, &d_any)) {
+    dev_free_all(tmp);
     return RPX_GPU_MEMORY;
   }
   hipError_t e = hipMemcpyAsync(d_rays, rays, (size_t)n * 32, hipMemcpyHostToDevice, h->stream);
   if (e == hipSuccess)
-    e = launch_trace_rays(h->dev, (const float*)d_rays, n, (u32*)d_inst, (u32*)d_tri, (float*)d_t, (uint8_t*)d_any,
+    e = launch_trace_rays(h->d_scene, (const float*)d_rays, n, (u32*)d_inst, (u32*)d_tri, (float*)d_t, (uint8_t*)d_any,
                           h->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(inst, d_inst, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(tri, d_tri, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(t, d_t, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(any, d_any, (size_t)n, hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-  for (DevBuf& b : tmp) hipFree(b.p);
+  dev_free_all(tmp);
   return status_of(e);
 }
 
@@ -589,21 +499,31 @@ uint32_t rpx_scene_info(uint32_t handle, uint64_t* out8) {
   return RPX_OK;
 }
 
+static void stats_of(Engine& e, u64 samples, uint64_t* out4) {
+  u64 r[3] = {0, 0, 0};
+  e.ray_counts(r);
+  out4[0] = r[0] + r[1] + r[2];
+  out4[1] = r[0];
+  out4[2] = r[1] + r[2];
+  out4[3] = samples;
+}
+
 uint32_t rpx_scene_stats(uint32_t handle, uint64_t* out4) {
   if (!out4) return RPX_NULL_POINTER;
   auto h = find(handle);
   if (!h) return RPX_BAD_HANDLE;
   std::lock_guard<std::mutex> lock(h->mutex);
   hipSetDevice(h->device);
-  u64 r[3] = {0, 0, 0};
-  if (h->paths.rays) {
-    hipMemcpyAsync(r, h->paths.rays, 24, hipMemcpyDeviceToHost, h->stream);
-    hipStreamSynchronize(h->stream);
-  }
-  out4[0] = r[0] + r[1] + r[2];
-  out4[1] = r[0];
-  out4[2] = r[1] + r[2];
-  out4[3] = h->samples;
+  stats_of(*h->engine, h->samples, out4);
+  return RPX_OK;
+}
+
+uint32_t rpx_set_options(uint32_t handle, uint32_t pool, uint32_t shape, uint32_t group, uint32_t chunk,
+                         uint32_t flags) {
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  apply_options(*h->engine, pool, shape, group, chunk, flags);
   return RPX_OK;
 }
 
@@ -611,20 +531,16 @@ uint32_t rpx_set_profiling(uint32_t handle, uint32_t on) {
   auto h = find(handle);
   if (!h) return RPX_BAD_HANDLE;
   std::lock_guard<std::mutex> lock(h->mutex);
-  h->profiling = on != 0;
+  h->engine->profiling = on != 0;
   return RPX_OK;
 }
 
-uint32_t rpx_last_timing(uint32_t handle, float* out6) {
-  if (!out6) return RPX_NULL_POINTER;
+uint32_t rpx_last_timing(uint32_t handle, float* out12) {
+  if (!out12) return RPX_NULL_POINTER;
   auto h = find(handle);
   if (!h) return RPX_BAD_HANDLE;
-  out6[0] = h->last_ms;
-  out6[1] = h->stage_ms[0];
-  out6[2] = h->stage_ms[1];
-  out6[3] = h->stage_ms[2];
-  out6[4] = (float)h->stage_launches[0];
-  out6[5] = (float)h->stage_launches[1];
+  std::lock_guard<std::mutex> lock(h->mutex);
+  fill_timing(*h->engine, out12);
   return RPX_OK;
 }
 
@@ -645,6 +561,189 @@ uint32_t rpx_energy_tables(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_
   if (clear_out) std::memcpy(clear_out, t->h_clear_out.data(), 17408 * 2);
   if (albedo_avg) std::memcpy(albedo_avg, t->h_albedo_avg.data(), 32 * 4);
   if (schlick_avg) std::memcpy(schlick_avg, t->h_schlick_avg.data(), 32 * 4);
+  return RPX_OK;
+}
+
+// ------------------------------------------------------------------ render queues
+uint32_t rpx_queue_create(void) {
+  if (rpx_device_count() == 0) {
+    set_error("rpx_queue_create: no HIP device is available");
+    return 0;
+  }
+  auto q = std::make_shared<Queue>();
+  hipGetDevice(&q->device);
+  if (hipStreamCreateWithFlags(&q->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("rpx_queue_create: cannot create a HIP stream");
+    return 0;
+  }
+  std::string err;
+  q->engine = std::make_unique<Engine>();
+  if (!q->engine->init(q->device, q->stream, err)) {
+    set_error("rpx_queue_create: " + err);
+    return 0;
+  }
+  std::lock_guard<std::mutex> lock(g_registry_mutex);
+  u32 id = g_next_queue++;
+  if (g_next_queue == 0) g_next_queue = 1;
+  g_queues[id] = q;
+  return id;
+}
+
+uint32_t rpx_queue_add(uint32_t queue, uint32_t scene, uint32_t first, uint32_t count, uint32_t* job) {
+  if (!job) return RPX_NULL_POINTER;
+  auto q = find_queue(queue);
+  auto h = find(scene);
+  if (!q || !h) return RPX_BAD_HANDLE;
+  if (h->device != q->device) {
+    set_error("rpx_queue_add: the scene lives on another device");
+    return RPX_BAD_SCENE;
+  }
+  std::lock_guard<std::mutex> lock(q->mutex);
+  hipSetDevice(q->device);
+  Job j;
+  j.scene = h;
+  j.first = first;
+  j.count = count;
+  {
+    std::lock_guard<std::mutex> hl(h->mutex);
+    j.width = h->dev.width;
+    j.height = h->dev.height;
+  }
+  size_t px = (size_t)j.width * j.height;
+  auto& B = q->film_bufs;
+  Film& F = j.film;
+  if (!dev_alloc(B, px * 12, (void**)&F.rgb) || !dev_alloc(B, px * 4, (void**)&F.weight) ||
+      !dev_alloc(B, px * 4, (void**)&F.alpha) || !dev_alloc(B, px * 4, (void**)&F.lum2)) {
+    set_error("device memory exhausted for a job's film");
+    return RPX_GPU_MEMORY;
+  }
+  hipMemsetAsync(F.rgb, 0, px * 12, q->stream);
+  hipMemsetAsync(F.weight, 0, px * 4, q->stream);
+  hipMemsetAsync(F.alpha, 0, px * 4, q->stream);
+  hipMemsetAsync(F.lum2, 0, px * 4, q->stream);
+  if (hipStreamSynchronize(q->stream) != hipSuccess) return RPX_GPU_LAUNCH;
+  *job = (u32)q->jobs.size();
+  q->jobs.push_back(j);
+  return RPX_OK;
+}
+
+uint32_t rpx_queue_run(uint32_t queue) {
+  auto q = find_queue(queue);
+  if (!q) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(q->mutex);
+  hipSetDevice(q->device);
+  if (q->jobs.empty()) return RPX_OK;
+  // The scenes stay locked (in handle order) while the pool reads them.
+  std::vector<Handle*> scenes;
+  for (const Job& j : q->jobs) scenes.push_back(j.scene.get());
+  std::sort(scenes.begin(), scenes.end());
+  scenes.erase(std::unique(scenes.begin(), scenes.end()), scenes.end());
+  std::vector<std::unique_lock<std::mutex>> locks;
+  for (Handle* h : scenes) locks.emplace_back(h->mutex);
+  std::vector<DevScene> table;
+  std::map<Handle*, u32> index;
+  for (Handle* h : scenes) {
+    hipStreamSynchronize(h->stream);
+    index[h] = (u32)table.size();
+    table.push_back(h->dev);
+  }
+  std::vector<DevSegment> segs;
+  for (const Job& j : q->jobs) {
+    if (j.scene->dev.width != j.width || j.scene->dev.height != j.height) {
+      set_error("rpx_queue_run: a job's scene changed its image size since it was queued");
+      return RPX_BAD_SCENE;
+    }
+    DevSegment s{};
+    s.scene = index[j.scene.get()];
+    s.w = j.width;
+    s.h = j.height;
+    s.first = j.first;
+    s.count = j.count;
+    s.film = j.film;
+    s.film_w = j.width;
+    if (s.count) segs.push_back(s);
+  }
+  size_t bytes = table.size() * sizeof(DevScene);
+  if (!q->scenes.p || q->scenes.bytes < bytes) {
+    if (q->scenes.p) hipFree(q->scenes.p);
+    q->scenes = DevBuf{};
+    if (hipMalloc(&q->scenes.p, bytes) != hipSuccess) {
+      q->scenes.p = nullptr;
+      set_error("device memory exhausted for the scene table");
+      return RPX_GPU_MEMORY;
+    }
+    q->scenes.bytes = bytes;
+  }
+  if (hipMemcpyAsync(q->scenes.p, table.data(), bytes, hipMemcpyHostToDevice, q->stream) != hipSuccess)
+    return RPX_GPU_LAUNCH;
+  std::string err;
+  hipError_t e = q->engine->run((const DevScene*)q->scenes.p, segs, true, err);
+  if (e != hipSuccess) {
+    set_error(err);
+    return e == hipErrorOutOfMemory ? RPX_GPU_MEMORY : RPX_GPU_LAUNCH;
+  }
+  return RPX_OK;
+}
+
+uint32_t rpx_queue_film_read(uint32_t queue, uint32_t job, float* rgb, float* weight, float* alpha, float* lum2) {
+  auto q = find_queue(queue);
+  if (!q) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(q->mutex);
+  if (job >= q->jobs.size()) return RPX_BAD_HANDLE;
+  hipSetDevice(q->device);
+  const Job& j = q->jobs[job];
+  return read_film(j.film, (size_t)j.width * j.height, q->stream, rgb, weight, alpha, lum2);
+}
+
+uint32_t rpx_queue_set_options(uint32_t queue, uint32_t pool, uint32_t shape, uint32_t group, uint32_t chunk,
+                               uint32_t flags) {
+  auto q = find_queue(queue);
+  if (!q) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(q->mutex);
+  apply_options(*q->engine, pool, shape, group, chunk, flags);
+  return RPX_OK;
+}
+
+uint32_t rpx_queue_set_profiling(uint32_t queue, uint32_t on) {
+  auto q = find_queue(queue);
+  if (!q) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(q->mutex);
+  q->engine->profiling = on != 0;
+  return RPX_OK;
+}
+
+uint32_t rpx_queue_last_timing(uint32_t queue, float* out12) {
+  if (!out12) return RPX_NULL_POINTER;
+  auto q = find_queue(queue);
+  if (!q) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(q->mutex);
+  fill_timing(*q->engine, out12);
+  return RPX_OK;
+}
+
+uint32_t rpx_queue_stats(uint32_t queue, uint64_t* out4) {
+  if (!out4) return RPX_NULL_POINTER;
+  auto q = find_queue(queue);
+  if (!q) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(q->mutex);
+  hipSetDevice(q->device);
+  u64 samples = 0;
+  for (const Job& j : q->jobs) samples += j.count;
+  stats_of(*q->engine, samples, out4);
+  return RPX_OK;
+}
+
+uint32_t rpx_queue_free(uint32_t queue) {
+  std::shared_ptr<Queue> q;
+  {
+    std::lock_guard<std::mutex> lock(g_registry_mutex);
+    auto it = g_queues.find(queue);
+    if (it == g_queues.end()) return RPX_BAD_HANDLE;
+    q = it->second;
+    g_queues.erase(it);
+  }
+  std::lock_guard<std::mutex> lock(q->mutex);
+  hipSetDevice(q->device);
   return RPX_OK;
 }
 

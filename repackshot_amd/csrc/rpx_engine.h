@@ -1,0 +1,92 @@
+// Host side of the GPU tracer's path pool: device tables, the pool's buffers, and the loop that
+// runs a list of segments (pixel rectangles x sample ranges of one or more scenes) to completion.
+#pragma once
+#include <string>
+#include <vector>
+#include "rpx_kernels.h"
+
+namespace rpx {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+bool dev_alloc(std::vector<DevBuf>& bufs, size_t bytes, void** out);
+void dev_free_all(std::vector<DevBuf>& bufs);
+
+// Constant tables of one device: the BSDF energy tables (generated there by the recipe kernel on
+// first use; the host copy prepares the BSDFs) and the Sobol generator columns.
+struct DeviceTables {
+  bool ready = false;
+  uint16_t *albedo = nullptr, *schlick = nullptr, *clear_in = nullptr, *clear_out = nullptr;
+  float *albedo_avg = nullptr, *schlick_avg = nullptr;
+  u32* sobol = nullptr;
+  std::vector<uint16_t> h_albedo, h_schlick, h_clear_in, h_clear_out;
+  std::vector<float> h_albedo_avg, h_schlick_avg;
+  EnergyTables host_view() const {
+    return EnergyTables{h_albedo.data(), h_schlick.data(), h_clear_in.data(), h_clear_out.data(),
+                        h_albedo_avg.data(), h_schlick_avg.data()};
+  }
+  EnergyTables device_view() const { return EnergyTables{albedo, schlick, clear_in, clear_out, albedo_avg, schlick_avg}; }
+};
+bool device_tables(int dev, DeviceTables*& out, std::string& err);
+
+// How the pool runs.
+struct EngineOptions {
+  u32 pool = 1u << 21;  // path slots (capped by the work of a chunk)
+  int shape = 0;        // traversal launch shape (index into SHAPES)
+  u32 group = 8;        // iterations launched between checks for completion (even)
+  u32 chunk = 1u << 27;  // most work ids (path samples) per chunk, at most RESULTS_MAX
+  bool per_pass = false;  // one sample pass at a time, the pool drained after each (no regeneration across samples)
+  bool graph = false;   // launch each group of iterations as one captured HIP graph
+};
+
+// GPU time of the last run, from events on the engine's stream (profiling on).
+struct Timing {
+  float total_ms = 0;
+  float stage_ms[4] = {0, 0, 0, 0};  // extend, shade, shadow, regen
+  u32 launches[4] = {0, 0, 0, 0};
+  u32 iterations = 0, chunks = 0;
+};
+
+// Largest number of work ids (path samples) in one chunk: the results buffer holds one float4 each.
+constexpr u32 RESULTS_MAX = 1u << 27;
+
+class Engine {
+ public:
+  Engine() = default;
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+  ~Engine();
+  bool init(int device, hipStream_t stream, std::string& err);
+  // Renders every segment. With `splat`, each segment's samples are added to its film in sample
+  // order; without, the results of a single chunk stay in `P.results` (work id order).
+  hipError_t run(const DevScene* d_scenes, const std::vector<DevSegment>& segs, bool splat, std::string& err);
+  // Closest-hit rays, shadow rays counted without a trace, shadow rays traced (since init).
+  bool ray_counts(u64 out[3]);
+
+  EngineOptions opt;
+  bool profiling = false;
+  Timing timing;
+  PathState P{};
+
+ private:
+  bool ensure_pool(u32 n, std::string& err);
+  bool ensure_results(u32 n, std::string& err);
+  bool ensure_segs(u32 n, std::string& err);
+  hipError_t run_chunk(const DevScene* d_scenes, const std::vector<DevSegment>& chunk, u32 total, bool splat);
+  hipEvent_t next_event();
+
+  int device_ = 0;
+  hipStream_t stream_ = nullptr;
+  u32 capacity_ = 0, results_cap_ = 0, segs_cap_ = 0;
+  std::vector<DevBuf> pool_bufs_, fixed_bufs_;
+  DevBuf results_, segs_;
+  u32* idle_host_ = nullptr;  // pinned
+  std::vector<hipEvent_t> events_;
+  size_t events_used_ = 0;
+  std::vector<std::pair<int, size_t>> marks_;
+  hipEvent_t ev_[2] = {nullptr, nullptr};
+};
+
+}  // namespace rpx

@@ -1,0 +1,329 @@
+// Host side of the path pool: per-device tables, buffers, and the iteration loop.
+//
+// A run splits its segments into chunks of at most RESULTS_MAX work ids (whole sample passes of a
+// segment), and for each chunk resets the pool, then launches groups of iterations
+// (regen, extend, shade, shadow) until the extend kernel finds no live path (a flag read back
+// after each group), then splats the chunk's samples into the films. Each chunk's films receive
+// whole samples in increasing sample order, so the films do not depend on the chunking, the pool
+// size or the launch shape.
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include "rpx_engine.h"
+
+namespace rpx {
+
+bool dev_alloc(std::vector<DevBuf>& bufs, size_t bytes, void** out) {
+  DevBuf b;
+  if (hipMalloc(&b.p, std::max<size_t>(bytes, 16)) != hipSuccess) return false;
+  b.bytes = bytes;
+  bufs.push_back(b);
+  *out = b.p;
+  return true;
+}
+
+void dev_free_all(std::vector<DevBuf>& bufs) {
+  for (DevBuf& b : bufs)
+    if (b.p) hipFree(b.p);
+  bufs.clear();
+}
+
+namespace {
+std::mutex g_tables_mutex;
+std::map<int, DeviceTables> g_tables;
+}  // namespace
+
+bool device_tables(int dev, DeviceTables*& out, std::string& err) {
+  std::lock_guard<std::mutex> lock(g_tables_mutex);
+  DeviceTables& t = g_tables[dev];
+  if (!t.ready) {
+    const size_t SL = 1024, CL = 17408;
+    if (hipMalloc(&t.albedo, SL * 2) != hipSuccess || hipMalloc(&t.schlick, SL * 2) != hipSuccess ||
+        hipMalloc(&t.clear_in, CL * 2) != hipSuccess || hipMalloc(&t.clear_out, CL * 2) != hipSuccess ||
+        hipMalloc(&t.albedo_avg, 32 * 4) != hipSuccess || hipMalloc(&t.schlick_avg, 32 * 4) != hipSuccess ||
+        hipMalloc(&t.sobol, 64 * 32 * 4) != hipSuccess) {
+      err = "device memory for the energy and Sobol tables";
+      return false;
+    }
+    std::vector<u32> cols(64 * 32);
+    sobol_columns(cols.data());
+    hipError_t e = hipMemcpy(t.sobol, cols.data(), cols.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_energy(t.albedo, t.schlick, t.clear_in, t.clear_out, t.albedo_avg, t.schlick_avg, 0);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+      err = std::string("energy table kernel: ") + hipGetErrorString(e);
+      return false;
+    }
+    t.h_albedo.resize(SL);
+    t.h_schlick.resize(SL);
+    t.h_clear_in.resize(CL);
+    t.h_clear_out.resize(CL);
+    t.h_albedo_avg.resize(32);
+    t.h_schlick_avg.resize(32);
+    hipMemcpy(t.h_albedo.data(), t.albedo, SL * 2, hipMemcpyDeviceToHost);
+    hipMemcpy(t.h_schlick.data(), t.schlick, SL * 2, hipMemcpyDeviceToHost);
+    hipMemcpy(t.h_clear_in.data(), t.clear_in, CL * 2, hipMemcpyDeviceToHost);
+    hipMemcpy(t.h_clear_out.data(), t.clear_out, CL * 2, hipMemcpyDeviceToHost);
+    hipMemcpy(t.h_albedo_avg.data(), t.albedo_avg, 32 * 4, hipMemcpyDeviceToHost);
+    e = hipMemcpy(t.h_schlick_avg.data(), t.schlick_avg, 32 * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      err = std::string("energy table copy: ") + hipGetErrorString(e);
+      return false;
+    }
+    t.ready = true;
+  }
+  out = &t;
+  return true;
+}
+
+Engine::~Engine() {
+  if (stream_) hipStreamSynchronize(stream_);
+  dev_free_all(pool_bufs_);
+  dev_free_all(fixed_bufs_);
+  if (results_.p) hipFree(results_.p);
+  if (segs_.p) hipFree(segs_.p);
+  if (idle_host_) hipHostFree(idle_host_);
+  for (hipEvent_t e : ev_)
+    if (e) hipEventDestroy(e);
+  for (hipEvent_t e : events_) hipEventDestroy(e);
+}
+
+bool Engine::init(int device, hipStream_t stream, std::string& err) {
+  device_ = device;
+  stream_ = stream;
+  DeviceTables* tables;
+  if (!device_tables(device, tables, err)) return false;
+  P.sobol = tables->sobol;
+  if (!dev_alloc(fixed_bufs_, N_COUNTS * 4, (void**)&P.counts) || !dev_alloc(fixed_bufs_, 64, (void**)&P.rays)) {
+    err = "device memory exhausted for the pool counters";
+    return false;
+  }
+  if (hipHostMalloc((void**)&idle_host_, 64, hipHostMallocDefault) != hipSuccess || hipEventCreate(&ev_[0]) != hipSuccess ||
+      hipEventCreate(&ev_[1]) != hipSuccess) {
+    err = "cannot create the pool's host flag or events";
+    return false;
+  }
+  hipMemsetAsync(P.counts, 0, N_COUNTS * 4, stream_);
+  hipMemsetAsync(P.rays, 0, 64, stream_);
+  return hipStreamSynchronize(stream_) == hipSuccess;
+}
+
+bool Engine::ensure_pool(u32 n, std::string& err) {
+  if (n <= capacity_) return true;
+  hipStreamSynchronize(stream_);
+  dev_free_all(pool_bufs_);
+  capacity_ = 0;
+  auto& B = pool_bufs_;
+  size_t f4 = (size_t)n * sizeof(float4);
+  bool ok = dev_alloc(B, f4, (void**)&P.ray_o) && dev_alloc(B, f4, (void**)&P.ray_d) && dev_alloc(B, f4, (void**)&P.hit) &&
+            dev_alloc(B, f4, (void**)&P.thr) && dev_alloc(B, f4, (void**)&P.sum) && dev_alloc(B, f4, (void**)&P.med) &&
+            dev_alloc(B, f4, (void**)&P.sel_p) && dev_alloc(B, f4, (void**)&P.sel_w0) &&
+            dev_alloc(B, f4, (void**)&P.sel_w1) && dev_alloc(B, f4, (void**)&P.sh_o) &&
+            dev_alloc(B, f4, (void**)&P.sh_d) && dev_alloc(B, f4, (void**)&P.sh_c) &&
+            dev_alloc(B, f4, (void**)&P.ident) && dev_alloc(B, (size_t)n * 8, (void**)&P.key) &&
+            dev_alloc(B, (size_t)n * 4, (void**)&P.live[0]) && dev_alloc(B, (size_t)n * 4, (void**)&P.live[1]) &&
+            dev_alloc(B, (size_t)n * 4, (void**)&P.shadow_queue) && dev_alloc(B, (size_t)n * 4, (void**)&P.done_queue);
+  if (!ok) {
+    dev_free_all(pool_bufs_);
+    err = "device memory exhausted for the path pool";
+    return false;
+  }
+  capacity_ = n;
+  return true;
+}
+
+bool Engine::ensure_results(u32 n, std::string& err) {
+  if (n <= results_cap_) return true;
+  hipStreamSynchronize(stream_);
+  if (results_.p) hipFree(results_.p);
+  results_ = DevBuf{};
+  results_cap_ = 0;
+  if (hipMalloc(&results_.p, (size_t)n * sizeof(float4)) != hipSuccess) {
+    results_.p = nullptr;
+    err = "device memory exhausted for the sample results";
+    return false;
+  }
+  results_cap_ = n;
+  P.results = (float4*)results_.p;
+  return true;
+}
+
+bool Engine::ensure_segs(u32 n, std::string& err) {
+  if (n <= segs_cap_) return true;
+  hipStreamSynchronize(stream_);
+  if (segs_.p) hipFree(segs_.p);
+  segs_ = DevBuf{};
+  segs_cap_ = 0;
+  if (hipMalloc(&segs_.p, (size_t)n * sizeof(DevSegment)) != hipSuccess) {
+    segs_.p = nullptr;
+    err = "device memory exhausted for the segment table";
+    return false;
+  }
+  segs_cap_ = n;
+  return true;
+}
+
+hipEvent_t Engine::next_event() {
+  if (events_used_ == events_.size()) {
+    hipEvent_t e;
+    hipEventCreate(&e);
+    events_.push_back(e);
+  }
+  return events_[events_used_++];
+}
+
+bool Engine::ray_counts(u64 out[3]) {
+  return hipMemcpyAsync(out, P.rays, 24, hipMemcpyDeviceToHost, stream_) == hipSuccess &&
+         hipStreamSynchronize(stream_) == hipSuccess;
+}
+
+hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegment>& chunk, u32 total, bool splat) {
+  hipError_t e = hipMemcpyAsync(segs_.p, chunk.data(), chunk.size() * sizeof(DevSegment), hipMemcpyHostToDevice, stream_);
+  if (e != hipSuccess) return e;
+  Launch L;
+  L.scenes = d_scenes;
+  L.segs = (const DevSegment*)segs_.p;
+  L.n_segs = (u32)chunk.size();
+  L.pool = std::min(opt.pool, total);
+  L.shape = opt.shape;
+  L.stream = stream_;
+  e = launch_pool_init(P, L, total);
+  if (e != hipSuccess) return e;
+  u32 group = std::max<u32>(2, opt.group & ~1u);
+  auto mark = [&](int stage) {
+    if (profiling && !opt.graph) {
+      marks_.push_back({stage, events_used_});
+      hipEventRecord(next_event(), stream_);
+    }
+  };
+  auto done = [&]() {
+    if (profiling && !opt.graph) hipEventRecord(next_event(), stream_);
+  };
+  auto iterations = [&](u32 count) -> hipError_t {
+    hipError_t r = hipSuccess;
+    for (u32 k = 0; k < count && r == hipSuccess; ++k) {
+      u32 parity = k & 1u;
+      mark(3);
+      r = launch_regen(P, L, parity);
+      done();
+      if (r != hipSuccess) break;
+      mark(0);
+      r = launch_extend(P, L, parity);
+      done();
+      if (r != hipSuccess) break;
+      mark(1);
+      r = launch_shade(P, L, parity);
+      done();
+      if (r != hipSuccess) break;
+      mark(2);
+      r = launch_shadow(P, L);
+      done();
+    }
+    return r;
+  };
+  hipGraphExec_t exec = nullptr;
+  if (opt.graph) {
+    hipGraph_t graph = nullptr;
+    e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    hipError_t ce = iterations(group);
+    e = hipStreamEndCapture(stream_, &graph);
+    if (ce != hipSuccess) e = ce;
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (graph) hipGraphDestroy(graph);
+    if (e != hipSuccess) return e;
+  }
+  for (;;) {
+    e = opt.graph ? hipGraphLaunch(exec, stream_) : iterations(group);
+    if (e != hipSuccess) break;
+    timing.iterations += group;
+    e = hipMemcpyAsync(idle_host_, P.counts + C_IDLE, 4, hipMemcpyDeviceToHost, stream_);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+    if (e != hipSuccess || *idle_host_) break;
+  }
+  if (exec) hipGraphExecDestroy(exec);
+  if (e != hipSuccess) return e;
+  if (splat)
+    for (const DevSegment& s : chunk) {
+      e = launch_splat(P, s, stream_);
+      if (e != hipSuccess) return e;
+    }
+  timing.chunks++;
+  return hipSuccess;
+}
+
+hipError_t Engine::run(const DevScene* d_scenes, const std::vector<DevSegment>& segs, bool splat, std::string& err) {
+  timing = Timing{};
+  events_used_ = 0;
+  marks_.clear();
+  // Chunks of whole sample passes, at most `limit` work ids each.
+  const u32 limit = std::min(std::max<u32>(opt.chunk, 1), RESULTS_MAX);
+  std::vector<std::vector<DevSegment>> chunks;
+  std::vector<u32> totals;
+  std::vector<DevSegment> cur;
+  u32 base = 0;
+  auto flush = [&]() {
+    if (!cur.empty()) {
+      chunks.push_back(cur);
+      totals.push_back(base);
+      cur.clear();
+      base = 0;
+    }
+  };
+  for (const DevSegment& s : segs) {
+    u32 npix = s.w * s.h;
+    if (npix == 0 || npix > limit) {
+      err = "a segment's pixel rectangle is empty or larger than the results buffer";
+      return hipErrorInvalidValue;
+    }
+    u32 k0 = 0;
+    while (k0 < s.count) {
+      u32 room = (limit - base) / npix;
+      if (opt.per_pass) room = std::min<u32>(room, 1);
+      if (room == 0) {
+        flush();
+        continue;
+      }
+      u32 take = std::min(room, s.count - k0);
+      DevSegment piece = s;
+      piece.first = s.first + k0;
+      piece.count = take;
+      piece.base = base;
+      cur.push_back(piece);
+      base += take * npix;
+      k0 += take;
+      if (opt.per_pass) flush();
+    }
+  }
+  flush();
+  if (!splat && chunks.size() > 1) {
+    err = "unsplatted results must fit one chunk";
+    return hipErrorInvalidValue;
+  }
+  u32 max_total = 0, max_segs = 0;
+  for (size_t c = 0; c < chunks.size(); ++c) {
+    max_total = std::max(max_total, totals[c]);
+    max_segs = std::max<u32>(max_segs, (u32)chunks[c].size());
+  }
+  if (!ensure_pool(std::min(opt.pool, max_total), err) || !ensure_results(max_total, err) || !ensure_segs(max_segs, err))
+    return hipErrorOutOfMemory;
+  hipEventRecord(ev_[0], stream_);
+  hipError_t e = hipSuccess;
+  for (size_t c = 0; c < chunks.size() && e == hipSuccess; ++c) e = run_chunk(d_scenes, chunks[c], totals[c], splat);
+  hipEventRecord(ev_[1], stream_);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  if (e != hipSuccess) {
+    err = std::string("render: ") + hipGetErrorString(e);
+    return e;
+  }
+  hipEventElapsedTime(&timing.total_ms, ev_[0], ev_[1]);
+  for (auto& m : marks_) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, events_[m.second], events_[m.second + 1]);
+    timing.stage_ms[m.first] += ms;
+    timing.launches[m.first]++;
+  }
+  return hipSuccess;
+}
+
+}  // namespace rpx

@@ -1,12 +1,48 @@
 // Wavefront path state (structure of arrays in HBM) and the kernel launchers of the GPU tracer.
+//
+// The tracer keeps a pool of path slots resident on the device. Work is a list of segments
+// (samples [first, first + count) of a pixel rectangle of one scene); every (segment, sample,
+// pixel) is one work id, numbered segment by segment, then sample-major. Each iteration runs
+//   regen   finished slots write their radiance to `results[work id]` and take the next work ids
+//           from a device cursor (camera ray of the new path), appending to the live queue
+//   extend  closest hit of every live path
+//   shade   surface, emission, NEE, BSDF sample, roulette: survivors to the other live queue,
+//           finished paths to the done queue, shadow rays to the shadow queue
+//   shadow  any hit of the shadow rays; unblocked ones add their contribution
+// until a regen leaves no live path. A splat then adds each pixel's samples to its film in
+// sample-index order, so the film does not depend on the pool size or on the order paths finish.
 #pragma once
 #include "rpx_scene.h"
 
 namespace rpx {
 
 constexpr u32 FLAG_PRIMARY = 1, FLAG_HAS_PDF = 2, FLAG_MEDIUM = 4, FLAG_BY_FLUX = 8;
+constexpr u32 BOUNCE_SHIFT = 16;  // flags bits 16..31: the bounce of the path's next vertex
+constexpr u32 NO_WORK = 0xffffffffu;
 
-// One entry per path slot (a pixel of the current sample pass).
+struct Film {
+  float* rgb;
+  float* weight;
+  float* alpha;
+  float* lum2;
+};
+
+// Samples [first, first + count) of the pixels (x0 .. x0 + w, y0 .. y0 + h) of scene `scene`;
+// work ids base .. base + count * w * h.
+struct DevSegment {
+  u32 scene;
+  u32 x0, y0, w, h;
+  u32 first, count;
+  u32 base;
+  Film film;
+  u32 film_w;  // film row length in pixels
+};
+
+// Counter slots (u32) of PathState::counts.
+constexpr u32 C_LIVE0 = 0, C_LIVE1 = 1, C_SHADOW = 2, C_DONE = 3, C_CURSOR = 4, C_END = 5, C_IDLE = 6;
+constexpr u32 N_COUNTS = 16;
+
+// One entry per path slot.
 struct PathState {
   float4* ray_o;   // origin
   float4* ray_d;   // direction
@@ -21,42 +57,50 @@ struct PathState {
   float4* sh_d;    // shadow ray direction
   float4* sh_c;    // contribution the shadow ray adds when unblocked
   u64* key;        // pixel key of the sampler
-  u32* queue[2];   // extension queues (ping-pong by round)
+  uint4* ident;    // work id (NO_WORK: empty slot), scene, x | y << 16, sample index
+  u32* live[2];    // live queues (ping-pong by iteration)
   u32* shadow_queue;
-  u32* counts;     // [0], [1] queue lengths, [2] shadow queue length
+  u32* done_queue;
+  u32* counts;     // see C_*
   u64* rays;       // [0] closest-hit rays traced, [1] shadow rays counted without a trace, [2] shadow rays traced
+  float4* results; // per work id of the current chunk: radiance rgb, alpha
+  const u32* sobol;  // 64 x 32 generator columns in logical-dimension order
 };
 
-struct Film {
-  float* rgb;
-  float* weight;
-  float* alpha;
-  float* lum2;
+// Launch shape of the traversal kernels (chosen per device by the tuner).
+struct TraversalShape {
+  u32 block;       // lanes per block
+  u32 lds_stack;   // stack entries per lane held in LDS (the rest of 64 in private memory)
+  u32 min_waves;   // waves per SIMD the register allocation must allow (1: no constraint)
+};
+constexpr int N_SHAPES = 5;
+extern const TraversalShape SHAPES[N_SHAPES];
+
+struct Launch {
+  const DevScene* scenes;
+  const DevSegment* segs;
+  u32 n_segs;
+  u32 pool;   // path slots in use (grid bound of the pool-sized kernels)
+  int shape;  // index into SHAPES
+  hipStream_t stream;
 };
 
-struct PassArgs {
-  u32 n_paths, x0, y0, w, sample_index;
-  u32 sobol_cam[4];  // unscrambled Sobol points of logical dimensions 0..3 at this sample index
-};
-
-struct RoundArgs {
-  u32 round, sample_index;
-  u32 sobol[6];  // unscrambled Sobol points of the round's six vertex dimensions
-};
-
-hipError_t launch_camera(const DevScene& S, const PathState& P, const PassArgs& A, hipStream_t st);
-hipError_t launch_extend(const DevScene& S, const PathState& P, u32 round, u32 max_paths, hipStream_t st);
-hipError_t launch_shade(const DevScene& S, const PathState& P, const RoundArgs& R, u32 max_paths, hipStream_t st);
-hipError_t launch_shadow(const DevScene& S, const PathState& P, u32 max_paths, hipStream_t st);
-hipError_t launch_splat(const PathState& P, const Film& F, u32 n, hipStream_t st);
+hipError_t launch_pool_init(const PathState& P, const Launch& L, u32 work_end);
+hipError_t launch_regen(const PathState& P, const Launch& L, u32 parity);
+hipError_t launch_extend(const PathState& P, const Launch& L, u32 parity);
+hipError_t launch_shade(const PathState& P, const Launch& L, u32 parity);
+hipError_t launch_shadow(const PathState& P, const Launch& L);
+// Adds the chunk's samples of segment `seg` (host copy `s`) to its film, in sample order.
+hipError_t launch_splat(const PathState& P, const DevSegment& s, hipStream_t st);
+// Copies the one-sample results of a w x h tile (work ids 0 ..) out as rgb or rgba floats.
 hipError_t launch_tile_out(const PathState& P, float* out, u32 n, u32 channels, hipStream_t st);
-hipError_t launch_trace_rays(const DevScene& S, const float* rays, u32 n, u32* inst, u32* tri, float* t, uint8_t* any,
-                             hipStream_t st);
+hipError_t launch_trace_rays(const DevScene* scene, const float* rays, u32 n, u32* inst, u32* tri, float* t,
+                             uint8_t* any, hipStream_t st);
 hipError_t launch_energy(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_in, uint16_t* clear_out, float* albedo_avg,
                          float* schlick_avg, hipStream_t st);
 
-// The Sobol generator matrices (Joe–Kuo new-joe-kuo-6.21201, 64 dimensions) and the logical-to-
-// Sobol dimension layout of the reference sampler.
-u32 sobol_point(u32 logical_dim, u32 index);
+// The Sobol generator matrices (Joe–Kuo new-joe-kuo-6.21201, 64 dimensions) in the reference
+// sampler's logical-dimension layout: column k of logical dimension d at [d * 32 + k].
+void sobol_columns(u32 out[64 * 32]);
 
 }  // namespace rpx

@@ -1,18 +1,21 @@
-// The GPU tracer's kernels (gfx950, wave64), in wavefront form: one kernel per stage, each
-// grid-striding over a compacted queue of live paths held in HBM.
+// The GPU tracer's kernels (gfx950, wave64), in wavefront form over a resident pool of path
+// slots: one kernel per stage, each grid-striding over a compacted queue whose length lives in
+// device memory, so the host never waits between stages.
 //
-//   camera   sampler key + camera ray per pixel (or tile pixel) of one sample index
+//   regen    finished slots hand their radiance to the results buffer and take new work ids
+//            (camera ray: sampler key, filter offset, pinhole / thin lens / orthographic)
 //   extend   closest hit over the 4-wide BVHs (instance BVH, then the mesh's), LDS stack
 //   shade    surface details, emission, MIS-weighted light crossings, next-event estimation
 //            (light selection + sample + BSDF eval, a shadow ray queued), BSDF sample, Russian
-//            roulette; survivors are appended to the next extension queue
+//            roulette; survivors go to the next live queue, finished paths to the done queue
 //   shadow   any hit of the queued shadow rays; unblocked ones add their contribution
 //   splat    per-pixel film sums, samples added in sample-index order
 //
 // Queues are compacted per wave: a ballot of the lanes that append, one atomic per wave, and each
-// lane's slot from the popcount of the lanes below it (mbcnt). Every path of a launch is at the
-// same bounce and sample index, so the Sobol points of that bounce's dimensions are launch
-// constants (host-computed); only the per-pixel Owen scramble runs per lane.
+// lane's slot from the popcount of the lanes below it (mbcnt). Paths of different samples, bounces
+// and scenes share a launch: every lane computes its own Sobol points from the generator columns
+// and its sample index, and scene constants are read per distinct scene of a wave (a waterfall
+// over the wave's scene ids, one pass when the wave holds one scene).
 //
 // Reference (paths inside engine/crates/rt-core/src): integrator/mod.rs:1140-1891 (the path
 // loop), light.rs:1026-1551 (select, pick_at, sample_selected, crossed_from, escaped),
@@ -24,123 +27,187 @@
 
 namespace rpx {
 
+const TraversalShape SHAPES[N_SHAPES] = {{128, 24, 1}, {128, 32, 1}, {128, 16, 1}, {256, 16, 1}, {128, 16, 8}};
+
 // ================================================================== traversal
 namespace {
-constexpr int BLOCK = 128;
-constexpr int LDS_STACK = 32;
-constexpr int TAIL_STACK = 32;
+constexpr int STACK_TOTAL = 64;  // entries per lane; the host refuses deeper trees
 
+// The geometry a lane traverses (its scene's arrays and instance-BVH root).
+struct Geo {
+  const Node4* nodes;
+  const float4* tris;
+  const DevInstance* instances;
+  u32 root;
+};
+__device__ __forceinline__ Geo geo_of(const DevScene* scenes, u32 sid) {
+  const DevScene& S = scenes[sid];
+  return Geo{S.nodes, S.tris, S.instances, S.tlas_root};
+}
+
+// PreparedRay (triangle.rs:41-91): origin, reciprocal direction, octant, and the shear of the
+// watertight test, whose axis permutation is held as lane masks (k*0: axis is x, k*1: axis is y).
+// Scalars only, so the whole ray stays in registers.
 struct PRay {
-  float org[3], inv[3];
-  u32 neg[3];
-  int kx, ky, kz;
+  float ox, oy, oz;
+  float ix, iy, iz;
   float sx, sy, sz;
   float t_min, t_max;
+  bool nx, ny, nz;
+  bool kx0, kx1, ky0, ky1, kz0, kz1;
 };
+
+__device__ __forceinline__ float sel3(float a, float b, float c, int k) { return k == 0 ? a : (k == 1 ? b : c); }
+__device__ __forceinline__ float pick(float a, float b, float c, bool k0, bool k1) { return k0 ? a : (k1 ? b : c); }
 
 __device__ __forceinline__ PRay prepare(V3 o, V3 d, float t_min, float t_max) {
   PRay r;
-  float dd[3] = {d.x, d.y, d.z};
-  r.inv[0] = 1.0f / d.x;
-  r.inv[1] = 1.0f / d.y;
-  r.inv[2] = 1.0f / d.z;
+  r.ix = 1.0f / d.x;
+  r.iy = 1.0f / d.y;
+  r.iz = 1.0f / d.z;
   float a0 = fabs_(d.x), a1 = fabs_(d.y), a2 = fabs_(d.z);
   int kz = (a0 >= a1 && a0 >= a2) ? 0 : (a1 >= a2 ? 1 : 2);
   // (kz + 1) % 3 and (kz + 2) % 3, swapped for a negative d[kz] (keeps the winding's sign)
-  int kx = (kz + 1) % 3, ky = (kz + 2) % 3;
-  float dkz = dd[kz];
-  if (dkz < 0.0f) {
+  int kx = kz == 2 ? 0 : kz + 1, ky = kz == 0 ? 2 : kz - 1;
+  if (sel3(d.x, d.y, d.z, kz) < 0.0f) {
     int t = kx;
     kx = ky;
     ky = t;
   }
-  float szz = r.inv[kz];
-  r.org[0] = o.x;
-  r.org[1] = o.y;
-  r.org[2] = o.z;
-  r.neg[0] = sign_negative(r.inv[0]);
-  r.neg[1] = sign_negative(r.inv[1]);
-  r.neg[2] = sign_negative(r.inv[2]);
-  r.kx = kx;
-  r.ky = ky;
-  r.kz = kz;
-  r.sx = dd[kx] * szz;
-  r.sy = dd[ky] * szz;
+  float szz = sel3(r.ix, r.iy, r.iz, kz);
+  r.ox = o.x;
+  r.oy = o.y;
+  r.oz = o.z;
+  r.nx = sign_negative(r.ix);
+  r.ny = sign_negative(r.iy);
+  r.nz = sign_negative(r.iz);
+  r.kx0 = kx == 0;
+  r.kx1 = kx == 1;
+  r.ky0 = ky == 0;
+  r.ky1 = ky == 1;
+  r.kz0 = kz == 0;
+  r.kz1 = kz == 1;
+  r.sx = sel3(d.x, d.y, d.z, kx) * szz;
+  r.sy = sel3(d.x, d.y, d.z, ky) * szz;
   r.sz = szz;
   r.t_min = t_min;
   r.t_max = fmin_r(t_max, F32_MAX);
   return r;
 }
 
-__device__ __forceinline__ float sel3(float a, float b, float c, int k) { return k == 0 ? a : (k == 1 ? b : c); }
-
-// The watertight test (triangle.rs:113-154): true with t when the triangle is hit in
-// [t_min, t_max].
-__device__ __forceinline__ bool watertight(const PRay& r, float4 va, float4 vb, float4 vc, float t_max, bool cull_back,
-                                           float& t_out) {
-  float ox = r.org[r.kx], oy = r.org[r.ky], oz = r.org[r.kz];
-  float a_x = sel3(va.x, va.y, va.z, r.kx) - ox, a_y = sel3(va.x, va.y, va.z, r.ky) - oy, a_z = sel3(va.x, va.y, va.z, r.kz) - oz;
-  float b_x = sel3(vb.x, vb.y, vb.z, r.kx) - ox, b_y = sel3(vb.x, vb.y, vb.z, r.ky) - oy, b_z = sel3(vb.x, vb.y, vb.z, r.kz) - oz;
-  float c_x = sel3(vc.x, vc.y, vc.z, r.kx) - ox, c_y = sel3(vc.x, vc.y, vc.z, r.ky) - oy, c_z = sel3(vc.x, vc.y, vc.z, r.kz) - oz;
+// Edge functions of the watertight test (triangle.rs:113-154) for vertices given relative to the
+// ray origin (the same differences as permuting first, then subtracting): u, v, w and the sheared
+// z of each vertex.
+__device__ __forceinline__ void shear(const PRay& r, V3 pa, V3 pb, V3 pc, float& u, float& v, float& w, float& az,
+                                      float& bz, float& cz) {
+  float a_x = pick(pa.x, pa.y, pa.z, r.kx0, r.kx1), a_y = pick(pa.x, pa.y, pa.z, r.ky0, r.ky1),
+        a_z = pick(pa.x, pa.y, pa.z, r.kz0, r.kz1);
+  float b_x = pick(pb.x, pb.y, pb.z, r.kx0, r.kx1), b_y = pick(pb.x, pb.y, pb.z, r.ky0, r.ky1),
+        b_z = pick(pb.x, pb.y, pb.z, r.kz0, r.kz1);
+  float c_x = pick(pc.x, pc.y, pc.z, r.kx0, r.kx1), c_y = pick(pc.x, pc.y, pc.z, r.ky0, r.ky1),
+        c_z = pick(pc.x, pc.y, pc.z, r.kz0, r.kz1);
   float ax = a_x - r.sx * a_z, ay = a_y - r.sy * a_z;
   float bx = b_x - r.sx * b_z, by = b_y - r.sy * b_z;
   float cx = c_x - r.sx * c_z, cy = c_y - r.sy * c_z;
-  float u = cx * by - cy * bx;
-  float v = ax * cy - ay * cx;
-  float w = bx * ay - by * ax;
+  u = cx * by - cy * bx;
+  v = ax * cy - ay * cx;
+  w = bx * ay - by * ax;
+  az = r.sz * a_z;
+  bz = r.sz * b_z;
+  cz = r.sz * c_z;
+}
+
+// The watertight test: true with t when the triangle is hit in [t_min, t_max].
+__device__ __forceinline__ bool watertight(const PRay& r, float4 va, float4 vb, float4 vc, float t_max, bool cull_back,
+                                           float& t_out) {
+  V3 pa = v3(va.x - r.ox, va.y - r.oy, va.z - r.oz);
+  V3 pb = v3(vb.x - r.ox, vb.y - r.oy, vb.z - r.oz);
+  V3 pc = v3(vc.x - r.ox, vc.y - r.oy, vc.z - r.oz);
+  float u, v, w, az, bz, cz;
+  shear(r, pa, pb, pc, u, v, w, az, bz, cz);
   bool negative = u < 0.0f || v < 0.0f || w < 0.0f;
   bool positive = u > 0.0f || v > 0.0f || w > 0.0f;
   bool rejected = cull_back ? negative : (negative && positive);
   float det = u + v + w;
   if (rejected || det == 0.0f) return false;
-  float az = r.sz * a_z, bz = r.sz * b_z, cz = r.sz * c_z;
   float t = (u * az + v * bz + w * cz) / det;
   t_out = t;
   return t >= r.t_min && t <= t_max;
 }
 
-// Slab test of the four children of a node (traverse.rs:184-222): near distances and hit mask.
-__device__ __forceinline__ u32 node_test(const Node4* __restrict__ node, const PRay& r, float t_max, float near[4]) {
+// Slab test of one child (traverse.rs:184-222): the near distance, and whether near <= far.
+__device__ __forceinline__ bool slab(const PRay& r, float lx, float hx, float ly, float hy, float lz, float hz,
+                                     float t_max, float& near) {
+  const float ROBUST = 1.0f + 4.0f * F32_EPSILON;
+  float x0 = r.nx ? hx : lx, x1 = r.nx ? lx : hx;
+  float y0 = r.ny ? hy : ly, y1 = r.ny ? ly : hy;
+  float z0 = r.nz ? hz : lz, z1 = r.nz ? lz : hz;
+  float tx0 = (x0 - r.ox) * r.ix, ty0 = (y0 - r.oy) * r.iy, tz0 = (z0 - r.oz) * r.iz;
+  float tx1 = (x1 - r.ox) * r.ix, ty1 = (y1 - r.oy) * r.iy, tz1 = (z1 - r.oz) * r.iz;
+  float nr = max_sel(tz0, max_sel(ty0, max_sel(tx0, r.t_min)));
+  float fr = min_sel(tz1, min_sel(ty1, min_sel(tx1, t_max))) * ROBUST;
+  near = nr;
+  return nr <= fr;
+}
+
+// The four children of a node: hit mask, near distances, child references.
+__device__ __forceinline__ u32 node_test(const Node4* __restrict__ node, const PRay& r, float t_max, float4& near,
+                                         uint4& child) {
   const float4* n = reinterpret_cast<const float4*>(node);
   float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5];
-  float4 x0 = r.neg[0] ? hx : lx, x1 = r.neg[0] ? lx : hx;
-  float4 y0 = r.neg[1] ? hy : ly, y1 = r.neg[1] ? ly : hy;
-  float4 z0 = r.neg[2] ? hz : lz, z1 = r.neg[2] ? lz : hz;
-  const float ROBUST = 1.0f + 4.0f * F32_EPSILON;
-  float bx0[4] = {x0.x, x0.y, x0.z, x0.w}, bx1[4] = {x1.x, x1.y, x1.z, x1.w};
-  float by0[4] = {y0.x, y0.y, y0.z, y0.w}, by1[4] = {y1.x, y1.y, y1.z, y1.w};
-  float bz0[4] = {z0.x, z0.y, z0.z, z0.w}, bz1[4] = {z1.x, z1.y, z1.z, z1.w};
+  child = reinterpret_cast<const uint4*>(node)[6];
   u32 bits = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    float tx0 = (bx0[k] - r.org[0]) * r.inv[0];
-    float ty0 = (by0[k] - r.org[1]) * r.inv[1];
-    float tz0 = (bz0[k] - r.org[2]) * r.inv[2];
-    float tx1 = (bx1[k] - r.org[0]) * r.inv[0];
-    float ty1 = (by1[k] - r.org[1]) * r.inv[1];
-    float tz1 = (bz1[k] - r.org[2]) * r.inv[2];
-    float nr = max_sel(tz0, max_sel(ty0, max_sel(tx0, r.t_min)));
-    float fr = min_sel(tz1, min_sel(ty1, min_sel(tx1, t_max))) * ROBUST;
-    near[k] = nr;
-    bits |= (nr <= fr) ? (1u << k) : 0u;
-  }
+  bits |= slab(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, t_max, near.x) ? 1u : 0u;
+  bits |= slab(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, t_max, near.y) ? 2u : 0u;
+  bits |= slab(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, t_max, near.z) ? 4u : 0u;
+  bits |= slab(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, t_max, near.w) ? 8u : 0u;
   return bits;
 }
 
+// Per-lane traversal stack: the first LDSN entries in this lane's LDS column (entry k at
+// lds[k * BLK]), the rest in private memory (reached only by deep trees).
+template <int BLK, int LDSN>
 struct Stack {
-  u32* lds;  // this lane's column: entry k at lds[k * BLOCK]
-  u32 tail[TAIL_STACK];
+  u32* lds;
+  u32 tail[STACK_TOTAL - LDSN];
   int sp;
   __device__ __forceinline__ void push(u32 v) {
-    if (sp < LDS_STACK) lds[sp * BLOCK] = v;
-    else tail[sp - LDS_STACK] = v;
+    if (sp < LDSN) lds[sp * BLK] = v;
+    else tail[sp - LDSN] = v;
     ++sp;
   }
   __device__ __forceinline__ u32 pop() {
     --sp;
-    return sp < LDS_STACK ? lds[sp * BLOCK] : tail[sp - LDS_STACK];
+    return sp < LDSN ? lds[sp * BLK] : tail[sp - LDSN];
   }
 };
+
+__device__ __forceinline__ void order2(u64& a, u64& b) {
+  u64 lo = a < b ? a : b, hi = a < b ? b : a;
+  a = lo;
+  b = hi;
+}
+
+// Pushes the hit children of a node, farthest first, and returns the nearest (a five-exchange
+// sorting network over (near bits, child) keys; missed children sort last).
+template <class St>
+__device__ __forceinline__ u32 push_ordered(u32 bits, float4 near, uint4 ch, St& st) {
+  const u64 MISS = ~0ull;
+  u64 k0 = (bits & 1u) ? (((u64)f2u(near.x) << 32) | ch.x) : MISS;
+  u64 k1 = (bits & 2u) ? (((u64)f2u(near.y) << 32) | ch.y) : MISS;
+  u64 k2 = (bits & 4u) ? (((u64)f2u(near.z) << 32) | ch.z) : MISS;
+  u64 k3 = (bits & 8u) ? (((u64)f2u(near.w) << 32) | ch.w) : MISS;
+  order2(k0, k1);
+  order2(k2, k3);
+  order2(k0, k2);
+  order2(k1, k3);
+  order2(k1, k2);
+  u32 n = (u32)__popc(bits);
+  if (n > 3) st.push((u32)k3);
+  if (n > 2) st.push((u32)k2);
+  if (n > 1) st.push((u32)k1);
+  return (u32)k0;
+}
 
 __device__ __forceinline__ V3 xf_point(const float* m, V3 p) {
   return v3(m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
@@ -159,72 +226,47 @@ __device__ __forceinline__ bool usable_ray(V3 o, V3 d) {
   return finite3(o) && finite3(d) && !(d.x == 0.0f && d.y == 0.0f && d.z == 0.0f);
 }
 
-// Pushes the hit children of a node, farthest first (closest hit) and returns the nearest.
-__device__ __forceinline__ u32 push_children(const Node4* node, u32 bits, const float near[4], Stack& st) {
-  const u32* ch = node->child;
-  u32 c0 = ch[0], c1 = ch[1], c2 = ch[2], c3v = ch[3];
-  u32 cs[4] = {c0, c1, c2, c3v};
-  // Keys (distance bits, child) sorted descending.
-  u64 keys[4];
-  int n = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k)
-    if (bits & (1u << k)) keys[n++] = ((u64)f2u(near[k]) << 32) | cs[k];
-  // insertion sort, at most 4
-  for (int i = 1; i < n; ++i) {
-    u64 key = keys[i];
-    int j = i;
-    while (j > 0 && keys[j - 1] < key) {
-      keys[j] = keys[j - 1];
-      --j;
-    }
-    keys[j] = key;
-  }
-  for (int i = 0; i < n - 1; ++i) st.push((u32)keys[i]);
-  return (u32)keys[n - 1];
-}
-}  // namespace
-
-// Nearest hit: instance, mesh triangle id and t (traverse.rs:665-740). `camera` skips the
+// Nearest hit: instance, mesh triangle id and t (traverse.rs:665-740), the lexicographic minimum
+// of (t, instance, triangle) over the triangles hit in [t_min, t_max]. `camera` skips the
 // instances hidden to the camera. Returns false on a miss.
-__device__ bool trace_nearest(const DevScene& S, u32* lds_col, V3 o, V3 d, float t_min, float t_max, bool camera,
+template <int BLK, int LDSN>
+__device__ bool trace_nearest(const Geo& g, u32* lds_col, V3 o, V3 d, float t_min, float t_max, bool camera,
                               u32& hit_inst, u32& hit_tri, float& hit_t) {
   if (!usable_ray(o, d)) return false;
-  PRay world = prepare(o, d, t_min, t_max);
-  if (!(world.t_min <= world.t_max)) return false;
-  float best_t = world.t_max;
+  PRay ray = prepare(o, d, t_min, t_max);
+  if (!(ray.t_min <= ray.t_max)) return false;
+  float best_t = ray.t_max;
   u32 best_inst = 0xffffffffu, best_tri = 0xffffffffu;
-  Stack st;
+  Stack<BLK, LDSN> st;
   st.lds = lds_col;
   st.sp = 0;
-  PRay ray = world;
+  bool local = false;  // `ray` is in a transformed instance's object space
   int tlas_sp = -1;  // stack height where the current instance's walk began (-1: in the TLAS)
   u32 inst = 0;
   bool cull = false;
-  u32 cur = S.tlas_root;
+  u32 cur = g.root;
   for (;;) {
     if (cur != REF_EMPTY) {
       if (!(cur & REF_LEAF)) {
-        const Node4* node = S.nodes + cur;
-        float near[4];
-        u32 bits = node_test(node, ray, best_t, near);
+        float4 near;
+        uint4 ch;
+        u32 bits = node_test(g.nodes + cur, ray, best_t, near, ch);
         if (bits) {
-          cur = push_children(node, bits, near, st);
+          cur = push_ordered(bits, near, ch, st);
           continue;
         }
       } else if (tlas_sp < 0) {
         // An instance leaf: walk its mesh in object space.
         u32 i = (cur & ~REF_LEAF) >> LEAF_COUNT_BITS;
-        const DevInstance& in = S.instances[i];
+        const DevInstance& in = g.instances[i];
         u32 fl = in.flags;
         if ((fl & INST_ACTIVE) && !(camera && (fl & INST_CAMERA_HIDDEN))) {
           inst = i;
           cull = (fl & INST_ONE_SIDED) != 0;
-          if (fl & INST_IDENTITY) {
-            ray = world;
-          } else {
+          if (!(fl & INST_IDENTITY)) {
             V3 lo = xf_point(in.to_local, o), ld = xf_vector(in.to_local, d);
             ray = prepare(lo, ld, t_min, t_max);
+            local = true;
           }
           tlas_sp = st.sp;
           cur = in.blas_root;
@@ -232,8 +274,8 @@ __device__ bool trace_nearest(const DevScene& S, u32* lds_col, V3 o, V3 d, float
         }
       } else {
         u32 first = (cur & ~REF_LEAF) >> LEAF_COUNT_BITS, count = (cur & 7u) + 1;
-        for (u32 k = 0; k < count; ++k) {
-          const float4* tp = S.tris + 3 * (size_t)(first + k);
+        const float4* tp = g.tris + 3 * (size_t)first;
+        for (u32 k = 0; k < count; ++k, tp += 3) {
           float4 va = tp[0], vb = tp[1], vc = tp[2];
           float t;
           if (watertight(ray, va, vb, vc, best_t, cull, t)) {
@@ -250,7 +292,8 @@ __device__ bool trace_nearest(const DevScene& S, u32* lds_col, V3 o, V3 d, float
     // Next entry; leaving a mesh walk goes back to world space.
     if (tlas_sp >= 0 && st.sp == tlas_sp) {
       tlas_sp = -1;
-      ray = world;
+      if (local) ray = prepare(o, d, t_min, t_max);
+      local = false;
     }
     if (st.sp == 0) break;
     cur = st.pop();
@@ -263,45 +306,44 @@ __device__ bool trace_nearest(const DevScene& S, u32* lds_col, V3 o, V3 d, float
 }
 
 // Any hit in [t_min, t_max] (traverse.rs:742-802).
-__device__ bool trace_any(const DevScene& S, u32* lds_col, V3 o, V3 d, float t_min, float t_max) {
+template <int BLK, int LDSN>
+__device__ bool trace_any(const Geo& g, u32* lds_col, V3 o, V3 d, float t_min, float t_max) {
   if (!usable_ray(o, d)) return false;
-  PRay world = prepare(o, d, t_min, t_max);
-  if (!(world.t_min <= world.t_max)) return false;
-  Stack st;
+  PRay ray = prepare(o, d, t_min, t_max);
+  if (!(ray.t_min <= ray.t_max)) return false;
+  Stack<BLK, LDSN> st;
   st.lds = lds_col;
   st.sp = 0;
-  PRay ray = world;
+  bool local = false;  // `ray` is in a transformed instance's object space
   int tlas_sp = -1;
   bool cull = false;
-  u32 cur = S.tlas_root;
+  u32 cur = g.root;
   for (;;) {
     if (cur != REF_EMPTY) {
       if (!(cur & REF_LEAF)) {
-        const Node4* node = S.nodes + cur;
-        float near[4];
-        u32 bits = node_test(node, ray, ray.t_max, near);
+        float4 near;
+        uint4 ch;
+        u32 bits = node_test(g.nodes + cur, ray, ray.t_max, near, ch);
         if (bits) {
-          int first = __builtin_ctz(bits);
-          bits &= bits - 1;
-          while (bits) {
-            int k = __builtin_ctz(bits);
-            bits &= bits - 1;
-            st.push(node->child[k]);
-          }
-          cur = node->child[first];
+          // Slot order: the first hit child next, the others pushed.
+          u32 next = REF_EMPTY;
+          if (bits & 1u) next = ch.x;
+          if (bits & 2u) { if (next == REF_EMPTY) next = ch.y; else st.push(ch.y); }
+          if (bits & 4u) { if (next == REF_EMPTY) next = ch.z; else st.push(ch.z); }
+          if (bits & 8u) { if (next == REF_EMPTY) next = ch.w; else st.push(ch.w); }
+          cur = next;
           continue;
         }
       } else if (tlas_sp < 0) {
         u32 i = (cur & ~REF_LEAF) >> LEAF_COUNT_BITS;
-        const DevInstance& in = S.instances[i];
+        const DevInstance& in = g.instances[i];
         u32 fl = in.flags;
         if (fl & INST_ACTIVE) {
           cull = (fl & INST_ONE_SIDED) != 0;
-          if (fl & INST_IDENTITY) {
-            ray = world;
-          } else {
+          if (!(fl & INST_IDENTITY)) {
             V3 lo = xf_point(in.to_local, o), ld = xf_vector(in.to_local, d);
             ray = prepare(lo, ld, t_min, t_max);
+            local = true;
           }
           tlas_sp = st.sp;
           cur = in.blas_root;
@@ -309,8 +351,8 @@ __device__ bool trace_any(const DevScene& S, u32* lds_col, V3 o, V3 d, float t_m
         }
       } else {
         u32 first = (cur & ~REF_LEAF) >> LEAF_COUNT_BITS, count = (cur & 7u) + 1;
-        for (u32 k = 0; k < count; ++k) {
-          const float4* tp = S.tris + 3 * (size_t)(first + k);
+        const float4* tp = g.tris + 3 * (size_t)first;
+        for (u32 k = 0; k < count; ++k, tp += 3) {
           float t;
           if (watertight(ray, tp[0], tp[1], tp[2], ray.t_max, cull, t)) return true;
         }
@@ -318,7 +360,8 @@ __device__ bool trace_any(const DevScene& S, u32* lds_col, V3 o, V3 d, float t_m
     }
     if (tlas_sp >= 0 && st.sp == tlas_sp) {
       tlas_sp = -1;
-      ray = world;
+      if (local) ray = prepare(o, d, t_min, t_max);
+      local = false;
     }
     if (st.sp == 0) break;
     cur = st.pop();
@@ -329,102 +372,203 @@ __device__ bool trace_any(const DevScene& S, u32* lds_col, V3 o, V3 d, float t_m
 // ================================================================== queue compaction
 __device__ __forceinline__ u32 wave_append(u32* counter, bool want) {
   u64 mask = __ballot(want);
-  u32 lane = __lane_id();
   u32 below = __builtin_amdgcn_mbcnt_hi((u32)(mask >> 32), __builtin_amdgcn_mbcnt_lo((u32)mask, 0u));
   u32 total = (u32)__popcll(mask);
   u32 base = 0;
   int leader = mask ? (int)__ffsll((long long)mask) - 1 : 0;
-  if (want && (int)lane == leader) base = atomicAdd(counter, total);
+  if (want && (int)__lane_id() == leader) base = atomicAdd(counter, total);
   base = __shfl(base, leader);
   return base + below;
 }
 
-// ================================================================== camera (render.rs:79-97)
+// The wave-uniform value of `v` in the lowest lane of `mask` (mask must be non-zero).
+__device__ __forceinline__ u32 lowest_lane_value(u64 mask, u32 v) {
+  int leader = (int)__ffsll((long long)mask) - 1;
+  return (u32)__builtin_amdgcn_readlane((int)v, leader);
+}
+
+// ================================================================== sampler
+// One number of logical dimension `dim` at sample `index` of the pixel with sampler key `key`
+// (sampler.rs:329-395): the Sobol point from the generator columns, Owen-scrambled per pixel.
+__device__ __forceinline__ float sample_at(const u32* __restrict__ cols, u64 key, u32 index, u32 dim) {
+  u32 x = 0;
+  if (dim < 64) {
+    const u32* c = cols + dim * 32;
+    u32 bits = index;
+    while (bits) {
+      x ^= c[__builtin_ctz(bits)];
+      bits &= bits - 1;
+    }
+  }
+  return sample1(key, index, dim, x);
+}
+}  // namespace
+
+// ================================================================== regen: results + camera (render.rs:79-97)
+namespace {
 __device__ __forceinline__ float filter_offset(u32 filter, float u) {
   if (filter == 1) return 0.5f;
   if (filter == 2) return 0.5f + (u < 0.5f ? sqrt_(2.0f * u) - 1.0f : 1.0f - sqrt_(2.0f - 2.0f * u));
   return u;
 }
 
-__global__ void __launch_bounds__(256) k_camera(DevScene S, PathState P, PassArgs A) {
-  u32 n = A.n_paths;
-  u32 stride = gridDim.x * blockDim.x;
-  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    u32 px = A.x0 + i % A.w, py = A.y0 + i / A.w;
-    u64 key = pixel_key(S.seed, px, py);
-    float du = sample1(key, A.sample_index, DIM_CAMERA, A.sobol_cam[0]);
-    float dv = sample1(key, A.sample_index, DIM_CAMERA + 1, A.sobol_cam[1]);
-    du = filter_offset(S.pixel_filter, du);
-    dv = filter_offset(S.pixel_filter, dv);
-    float u = ((float)px + du) / (float)(S.width > 1 ? S.width : 1);
-    float v = ((float)py + dv) / (float)(S.height > 1 ? S.height : 1);
-    V3 o, d;
-    if (S.ortho) {
-      float wdt = S.ortho_width;
-      o = S.cam_position + S.cam_right * ((u - 0.5f) * wdt) + S.cam_up * ((0.5f - v) * wdt / S.aspect);
-      d = S.cam_forward;
+// The camera ray of pixel (px, py) at sample `index` (scene/mod.rs:294-357).
+__device__ __forceinline__ void camera_ray(const DevScene& S, const u32* cols, u64 key, u32 px, u32 py, u32 index,
+                                           V3& o, V3& d) {
+  float du = sample_at(cols, key, index, DIM_CAMERA);
+  float dv = sample_at(cols, key, index, DIM_CAMERA + 1);
+  du = filter_offset(S.pixel_filter, du);
+  dv = filter_offset(S.pixel_filter, dv);
+  float u = ((float)px + du) / (float)(S.width > 1 ? S.width : 1);
+  float v = ((float)py + dv) / (float)(S.height > 1 ? S.height : 1);
+  if (S.ortho) {
+    float wdt = S.ortho_width;
+    o = S.cam_position + S.cam_right * ((u - 0.5f) * wdt) + S.cam_up * ((0.5f - v) * wdt / S.aspect);
+    d = S.cam_forward;
+  } else {
+    float x = (u - 0.5f) * S.sensor_w;
+    float y = (0.5f - v) * S.sensor_h;
+    V3 dir = normalize(S.cam_forward * S.focal_mm + S.cam_right * x + S.cam_up * y);
+    if (S.lens_radius == 0.0f) {
+      o = S.cam_position;
+      d = dir;
     } else {
-      float x = (u - 0.5f) * S.sensor_w;
-      float y = (0.5f - v) * S.sensor_h;
-      V3 dir = normalize(S.cam_forward * S.focal_mm + S.cam_right * x + S.cam_up * y);
-      if (S.lens_radius == 0.0f) {
-        o = S.cam_position;
-        d = dir;
-      } else {
-        float l0 = sample1(key, A.sample_index, DIM_LENS, A.sobol_cam[2]);
-        float l1 = sample1(key, A.sample_index, DIM_LENS + 1, A.sobol_cam[3]);
-        float lx, ly;
-        concentric_disk(l0, l1, lx, ly);
-        V3 focus_point = S.cam_position + dir * (S.focus_dist / dot(dir, S.cam_forward));
-        o = S.cam_position + (S.cam_right * lx + S.cam_up * ly) * S.lens_radius;
-        d = normalize(focus_point - o);
-      }
+      float l0 = sample_at(cols, key, index, DIM_LENS);
+      float l1 = sample_at(cols, key, index, DIM_LENS + 1);
+      float lx, ly;
+      concentric_disk(l0, l1, lx, ly);
+      V3 focus_point = S.cam_position + dir * (S.focus_dist / dot(dir, S.cam_forward));
+      o = S.cam_position + (S.cam_right * lx + S.cam_up * ly) * S.lens_radius;
+      d = normalize(focus_point - o);
     }
-    P.ray_o[i] = make_float4(o.x, o.y, o.z, 0.0f);
-    P.ray_d[i] = make_float4(d.x, d.y, d.z, 0.0f);
-    P.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    P.sum[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-    P.med[i] = make_float4(0.0f, 0.0f, 0.0f, u2f(FLAG_PRIMARY));
-    P.key[i] = key;
-    P.queue[0][i] = i;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) P.counts[0] = n;
+}
+
+// The segment holding work id `wid` (segments sorted by base).
+__device__ __forceinline__ u32 find_segment(const DevSegment* __restrict__ segs, u32 n_segs, u32 wid) {
+  u32 lo = 0, hi = n_segs - 1;
+  while (lo < hi) {
+    u32 mid = (lo + hi + 1) >> 1;
+    if (segs[mid].base <= wid) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+}  // namespace
+
+// Resets the pool: every slot empty and queued as finished, the work cursor at 0.
+__global__ void __launch_bounds__(256) k_pool_init(PathState P, u32 pool, u32 work_end) {
+  u32 stride = gridDim.x * blockDim.x;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < pool; i += stride) {
+    P.done_queue[i] = i;
+    P.ident[i] = make_uint4(NO_WORK, 0u, 0u, 0u);
+  }
+  if (blockIdx.x == 0 && threadIdx.x < N_COUNTS) {
+    u32 c = threadIdx.x;
+    P.counts[c] = c == C_DONE ? pool : (c == C_END ? work_end : 0u);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_regen(const DevScene* __restrict__ scenes, const DevSegment* __restrict__ segs,
+                                               u32 n_segs, PathState P, u32 parity) {
+  u32 n = P.counts[C_DONE];
+  u32 end = P.counts[C_END];
+  u32* out_q = P.live[parity];
+  u32* out_n = &P.counts[C_LIVE0 + parity];
+  u32 stride = gridDim.x * blockDim.x;
+  // Whole waves iterate together so that the ballots below see every lane.
+  u32 n_iter = (n + stride - 1) / stride;
+  u32 gid = blockIdx.x * blockDim.x + threadIdx.x;
+  for (u32 it = 0; it < n_iter; ++it) {
+    u32 j = it * stride + gid;
+    bool active = j < n;
+    u32 slot = 0;
+    if (active) {
+      slot = P.done_queue[j];
+      u32 done_work = P.ident[slot].x;
+      if (done_work != NO_WORK) P.results[done_work] = P.sum[slot];
+    }
+    u32 wid = wave_append(&P.counts[C_CURSOR], active);
+    bool got = active && wid < end;
+    u32 sid = 0, px = 0, py = 0, index = 0;
+    if (got) {
+      const DevSegment& sg = segs[find_segment(segs, n_segs, wid)];
+      u32 local = wid - sg.base, npix = sg.w * sg.h;
+      u32 k = local / npix, p = local - k * npix;
+      sid = sg.scene;
+      px = sg.x0 + p % sg.w;
+      py = sg.y0 + p / sg.w;
+      index = sg.first + k;
+    } else if (active) {
+      P.ident[slot] = make_uint4(NO_WORK, 0u, 0u, 0u);
+    }
+    // One pass per distinct scene of the wave.
+    bool todo = got;
+    u64 pending = __ballot(todo);
+    while (pending) {
+      u32 cur = lowest_lane_value(pending, sid);
+      if (todo && sid == cur) {
+        const DevScene& S = scenes[cur];
+        u64 key = pixel_key(S.seed, px, py);
+        V3 o, d;
+        camera_ray(S, P.sobol, key, px, py, index, o, d);
+        P.ray_o[slot] = make_float4(o.x, o.y, o.z, 0.0f);
+        P.ray_d[slot] = make_float4(d.x, d.y, d.z, 0.0f);
+        P.thr[slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        P.sum[slot] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        P.med[slot] = make_float4(0.0f, 0.0f, 0.0f, u2f(FLAG_PRIMARY));
+        P.key[slot] = key;
+        P.ident[slot] = make_uint4(wid, sid, px | (py << 16), index);
+        todo = false;
+      }
+      pending = __ballot(todo);
+    }
+    u32 q = wave_append(out_n, got);
+    if (got) out_q[q] = slot;
+  }
 }
 
 // ================================================================== extend
-__global__ void __launch_bounds__(BLOCK) k_extend(DevScene S, PathState P, u32 round) {
-  __shared__ u32 lds[LDS_STACK * BLOCK];
+template <int BLK, int LDSN, int WPE>
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) k_extend(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
+  __shared__ u32 lds[LDSN * BLK];
   u32* col = lds + threadIdx.x;
-  const u32* q = P.queue[round & 1];
-  u32 n = P.counts[round & 1];
+  const u32* q = P.live[parity];
+  u32 n = P.counts[C_LIVE0 + parity];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    P.counts[(round + 1) & 1] = 0;
-    P.counts[2] = 0;
+    // The queues this iteration's shade and shadow fill (their previous contents are consumed).
+    P.counts[C_LIVE0 + (parity ^ 1)] = 0;
+    P.counts[C_SHADOW] = 0;
+    P.counts[C_DONE] = 0;
     P.rays[0] += n;
+    if (n == 0) P.counts[C_IDLE] = 1;
   }
   u32 stride = gridDim.x * blockDim.x;
   for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
     u32 i = q[j];
     float4 o = P.ray_o[i], d = P.ray_d[i];
     bool primary = (f2u(P.med[i].w) & FLAG_PRIMARY) != 0;
+    Geo g = geo_of(scenes, P.ident[i].y);
     u32 inst = 0xffffffffu, tri = 0xffffffffu;
     float t = RPX_INF;
-    trace_nearest(S, col, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), RAY_EPSILON, RPX_INF, primary, inst, tri, t);
+    trace_nearest<BLK, LDSN>(g, col, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), RAY_EPSILON, RPX_INF, primary, inst, tri, t);
     P.hit[i] = make_uint4(inst, tri, f2u(t), 0u);
   }
 }
 
 // ================================================================== shadow
-__global__ void __launch_bounds__(BLOCK) k_shadow(DevScene S, PathState P) {
-  __shared__ u32 lds[LDS_STACK * BLOCK];
+template <int BLK, int LDSN, int WPE>
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) k_shadow(const DevScene* __restrict__ scenes, PathState P) {
+  __shared__ u32 lds[LDSN * BLK];
   u32* col = lds + threadIdx.x;
-  u32 n = P.counts[2];
+  u32 n = P.counts[C_SHADOW];
   if (blockIdx.x == 0 && threadIdx.x == 0) P.rays[2] += n;
   u32 stride = gridDim.x * blockDim.x;
   for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
     u32 i = P.shadow_queue[j];
     float4 o = P.sh_o[i], d = P.sh_d[i];
-    if (!trace_any(S, col, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), RAY_EPSILON, o.w)) {
+    Geo g = geo_of(scenes, P.ident[i].y);
+    if (!trace_any<BLK, LDSN>(g, col, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), RAY_EPSILON, o.w)) {
       float4 c = P.sh_c[i];
       float4 s = P.sum[i];
       P.sum[i] = make_float4(s.x + c.x, s.y + c.y, s.z + c.z, s.w);
@@ -521,16 +665,408 @@ __device__ __forceinline__ void add_to(C3& sum, C3 c, bool indirect, float clamp
   if (indirect && clamp_ind > 0.0f && pk > clamp_ind) sum = sum + c * (clamp_ind / pk);
   else sum = sum + c;
 }
+
+// One vertex of the path in slot i (integrator/mod.rs:1140-1891, one loop iteration): `cont` when
+// the path goes on with a new ray, `want_shadow` when a shadow ray was queued behind it.
+__device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P, u32 i, bool& cont, bool& want_shadow) {
+  const EnergyTables& T = S.tables;
+  {
+  float4 ro = P.ray_o[i], rd = P.ray_d[i];
+  V3 o = v3(ro.x, ro.y, ro.z), d = v3(rd.x, rd.y, rd.z);
+  uint4 h = P.hit[i];
+  float4 th = P.thr[i];
+  float4 sm = P.sum[i];
+  float4 md = P.med[i];
+  u32 flags = f2u(md.w);
+  bool primary = (flags & FLAG_PRIMARY) != 0;
+  bool has_pdf = (flags & FLAG_HAS_PDF) != 0;
+  bool in_medium = (flags & FLAG_MEDIUM) != 0;
+  C3 thr = c3(th.x, th.y, th.z);
+  float prev_pdf = th.w;
+  C3 sum = c3(sm.x, sm.y, sm.z);
+  float alpha = sm.w;
+  u64 key = P.key[i];
+  const u32 bounce = flags >> BOUNCE_SHIFT;
+  const u32 dims = bounce_dim(bounce);
+  const u32 sample_index = P.ident[i].w;
+  // The selection of the vertex the ray left (only read once a BSDF sample made the ray).
+  Sel from;
+  if (has_pdf) {
+    float4 sp = P.sel_p[i];
+    from.point = v3(sp.x, sp.y, sp.z);
+    from.total = sp.w;
+    from.by_flux = (flags & FLAG_BY_FLUX) != 0;
+    float4 w0 = P.sel_w0[i], w1 = P.sel_w1[i];
+    from.w[0] = w0.x; from.w[1] = w0.y; from.w[2] = w0.z; from.w[3] = w0.w;
+    from.w[4] = w1.x; from.w[5] = w1.y; from.w[6] = w1.z; from.w[7] = w1.w;
+  }
+  bool hit_ok = h.x != 0xffffffffu;
+  float t_geometry = hit_ok ? u2f(h.z) : RPX_INF;
+  float t_floor = RPX_INF;
+  if ((S.floor_mode == 1) && d.z != 0.0f) {
+    float tf = (S.floor_z - o.z) / d.z;
+    if (tf > RAY_EPSILON && tf < RPX_INF) t_floor = tf;
+  }
+  float t_nearest = fmin_r(t_geometry, t_floor);
+  if (in_medium && is_finite(t_nearest)) {
+    C3 sg = c3(md.x, md.y, md.z);
+    thr = thr * c3(fexp(-sg.r * t_nearest), fexp(-sg.g * t_nearest), fexp(-sg.b * t_nearest));
+  }
+  // Area lights crossed before the surface (light.rs:1354-1384).
+  C3 crossed = splat(0.0f);
+  for (u32 a = 0; a < S.n_areas; ++a) {
+    const DevArea& L = S.areas[a];
+    bool visible = has_pdf ? L.glossy : L.camera;
+    if (!visible) continue;
+    float t, cos;
+    if (!area_intersect(L, o, d, t_nearest, t, cos)) continue;
+    float weight = 1.0f;
+    if (has_pdf) {
+      float dd = 0.0f + t;
+      float light_pdf = sel_pmf(S, from, L.entry) * dd * dd / (L.area * cos);
+      weight = power_heuristic(prev_pdf, light_pdf);
+    }
+    crossed = crossed + radiance_at(L, cos) * weight;
+  }
+  add_to(sum, thr * crossed, bounce >= 2, S.clamp_indirect);
+  bool done = false;
+  if (t_nearest == RPX_INF) {
+    if (primary) {
+      if (S.background == 0) {
+        C3 seen = splat(env_value(S, d.z));
+        if (S.exterior != 1.0f) seen = seen * S.exterior;
+        sum = sum + thr * seen;
+      } else if (S.background == 1) {
+        sum = sum + thr * S.background_color;
+      } else {
+        alpha = peak(crossed) > 0.0f ? 1.0f : 0.0f;
+      }
+    } else {
+      C3 sky = splat(env_value(S, d.z));
+      if (has_pdf && S.has_env_entry)
+        sky = sky * power_heuristic(prev_pdf, sel_pmf(S, from, S.env_entry) * env_pdf(S, d.z));
+      add_to(sum, thr * sky, bounce >= 2, S.clamp_indirect);
+    }
+    done = true;
+  }
+  V3 s_point, s_hit_point, s_normal, s_geometric;
+  bool s_instance = false, s_front = true, s_outside = true;
+  u32 inst = 0;
+  if (!done) {
+    if (t_floor < t_geometry) {
+      V3 p = o + d * t_floor;
+      V3 point = v3(p.x, p.y, S.floor_z);
+      V3 nn = v3(0.0f, 0.0f, d.z < 0.0f ? 1.0f : -1.0f);
+      s_point = point;
+      s_hit_point = point;
+      s_normal = nn;
+      s_geometric = nn;
+    } else {
+      // Hit details (geometry/mod.rs:731-808) and the terminator lift (:682-728).
+      inst = h.x;
+      u32 tri = h.y;
+      const DevInstance& in = S.instances[inst];
+      const DevMesh& mesh = S.meshes[in.mesh];
+      bool ident = (in.flags & INST_IDENTITY) != 0;
+      const u32* idx = S.indices + mesh.index_offset + 3 * (size_t)tri;
+      u32 i0 = mesh.vertex_offset + idx[0], i1 = mesh.vertex_offset + idx[1], i2 = mesh.vertex_offset + idx[2];
+      const float* ps = S.positions;
+      V3 p0 = v3(ps[3 * (size_t)i0], ps[3 * (size_t)i0 + 1], ps[3 * (size_t)i0 + 2]);
+      V3 p1 = v3(ps[3 * (size_t)i1], ps[3 * (size_t)i1 + 1], ps[3 * (size_t)i1 + 2]);
+      V3 p2 = v3(ps[3 * (size_t)i2], ps[3 * (size_t)i2 + 1], ps[3 * (size_t)i2 + 2]);
+      V3 lo = ident ? o : xf_point(in.to_local, o), ld = ident ? d : xf_vector(in.to_local, d);
+      PRay r = prepare(lo, ld, RAY_EPSILON, RPX_INF);
+      float b0, b1, b2;
+      {
+        float u, v, w, az, bz, cz;
+        shear(r, p0 - lo, p1 - lo, p2 - lo, u, v, w, az, bz, cz);
+        bool negative = u < 0.0f || v < 0.0f || w < 0.0f;
+        bool positive = u > 0.0f || v > 0.0f || w > 0.0f;
+        float det = u + v + w;
+        bool ok = !(negative && positive) && det != 0.0f;
+        if (ok) {
+          float t = (u * az + v * bz + w * cz) / det;
+          ok = t >= r.t_min && t <= F32_MAX;
+        }
+        if (ok) {
+          b0 = u / det;
+          b1 = v / det;
+          b2 = w / det;
+        } else {
+          b0 = b1 = b2 = 1.0f / 3.0f;
+        }
+      }
+      V3 point_local = p0 * b0 + p1 * b1 + p2 * b2;
+      V3 wind = cross(p1 - p0, p2 - p0);
+      V3 winding = normalize(ident ? wind : xf_transpose(in.to_local, wind));
+      V3 incoming = normalize(d);
+      V3 geometric = length_sq(winding) > 0.0f ? winding : -incoming;
+      bool front_face = dot(winding, d) < 0.0f;
+      V3 smooth = v3(0.0f, 0.0f, 0.0f);
+      V3 n0, n1, n2;
+      if (mesh.has_normals) {
+        const float* ns = S.normals;
+        n0 = v3(ns[3 * (size_t)i0], ns[3 * (size_t)i0 + 1], ns[3 * (size_t)i0 + 2]);
+        n1 = v3(ns[3 * (size_t)i1], ns[3 * (size_t)i1 + 1], ns[3 * (size_t)i1 + 2]);
+        n2 = v3(ns[3 * (size_t)i2], ns[3 * (size_t)i2 + 1], ns[3 * (size_t)i2 + 2]);
+        V3 interp = n0 * b0 + n1 * b1 + n2 * b2;
+        smooth = ident ? interp : xf_transpose(in.to_local, interp);
+      }
+      V3 normal = length_sq(smooth) > 1.0e-12f ? normalize(smooth) : geometric;
+      V3 f_normal = dot(normal, d) > 0.0f ? -normal : normal;
+      V3 ng = dot(geometric, d) > 0.0f ? -geometric : geometric;
+      V3 hit_point = ident ? point_local : xf_point(in.to_world, point_local);
+      // bevel::shading_point without a bevel
+      V3 sn = f_normal;
+      if (dot(sn, ng) < 0.0f) sn = -sn;
+      if (dot(sn, d) >= -1.0e-4f) sn = ng;
+      // terminator_point
+      V3 tp = hit_point;
+      if (mesh.has_normals) {
+        V3 pt = p0 * b0 + p1 * b1 + p2 * b2;
+        V3 interp = n0 * b0 + n1 * b1 + n2 * b2;
+        V3 world_normal = ident ? interp : xf_transpose(in.to_local, interp);
+        float side = dot(world_normal, ng) < 0.0f ? -1.0f : 1.0f;
+        V3 lifted = pt;
+        V3 cs[3] = {p0, p1, p2};
+        V3 nsv[3] = {n0, n1, n2};
+        float ws[3] = {b0, b1, b2};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          V3 nk = normalize(nsv[k]) * side;
+          float below = dot(pt - cs[k], nk);
+          if (below < 0.0f) lifted = lifted - nk * (below * ws[k]);
+        }
+        if (finite3(lifted)) tp = ident ? lifted : xf_point(in.to_world, lifted);
+      }
+      s_point = tp;
+      s_hit_point = hit_point;
+      s_normal = sn;
+      s_geometric = ng;
+      s_instance = true;
+      s_front = front_face;
+      s_outside = front_face != (mesh.inside_out != 0);
+    }
+  }
+  u32 mat = 0xffffffffu;
+  if (!done && s_instance) {
+    mat = S.instances[inst].material;
+    if (mat < S.n_materials) {
+      const DevMaterial& M = S.materials[mat];
+      if (M.emissive) {
+        bool one_sided = (S.instances[inst].flags & INST_ONE_SIDED) != 0;
+        if (s_front || !one_sided) add_to(sum, thr * M.radiance * 1.0f, bounce >= 2, S.clamp_indirect);
+        done = true;
+      }
+    }
+  }
+  if (!done && bounce >= S.max_depth) done = true;
+  if (!done) {
+    const Bsdf& B = s_instance ? (mat < S.n_materials ? S.bsdfs[2 * mat + (s_outside ? 0 : 1)]
+                                                      : S.fallback_bsdf[s_outside ? 0 : 1])
+                               : S.floor_bsdf;
+    Frame frame = frame_from_normal(s_normal);
+    V3 wo = to_local(frame, -d);
+    V3 ng = s_geometric;
+    V3 origin = offset_origin(s_point, ng);
+    // Light selection at this vertex (light.rs:1026-1046).
+    Sel sel;
+    sel.point = s_point;
+    sel.total = 0.0f;
+    sel.by_flux = 1;
+    for (int k = 0; k < KEPT_WEIGHTS; ++k) sel.w[k] = 0.0f;
+    u32 ne = S.n_entries;
+    if (S.per_point && ne > 0) {
+      sel.by_flux = 0;
+      for (u32 e = 0; e < ne; ++e) {
+        float w = entry_weight(S, e, s_point);
+        if (e < (u32)KEPT_WEIGHTS) sel.w[e] = w;
+        sel.total += w;
+      }
+    }
+    // Next-event estimation (light.rs:1150-1315).
+    float u_sel = sample_at(P.sobol, key, sample_index, dims + OFFSET_LIGHT_SELECT);
+    float ul0 = sample_at(P.sobol, key, sample_index, dims + OFFSET_LIGHT);
+    float ul1 = sample_at(P.sobol, key, sample_index, dims + OFFSET_LIGHT + 1);
+    bool have_ls = false;
+    V3 ls_wi = v3(0, 0, 0);
+    float ls_dist = 0.0f, ls_pdf = 0.0f;
+    C3 ls_rad = splat(0.0f);
+    bool ls_glossy = true, ls_shadow = true;
+    if (ne > 0) {
+      u32 index = 0;
+      float p = 0.0f;
+      bool ok = true;
+      if (sel.by_flux) {
+        u32 k = 0;
+        while (k < ne && S.cdf[k] <= u_sel) ++k;
+        index = k < ne - 1 ? k : ne - 1;
+        p = sel_pmf(S, sel, index);
+      } else if (!(sel.total > 0.0f && is_finite(sel.total))) {
+        ok = false;
+      } else {
+        float acc = 0.0f, lo_c = 0.0f;
+        u32 last = ne - 1;
+        for (u32 e = 0; e <= last; ++e) {
+          float w = sel_weight(S, sel, e);
+          acc += w;
+          float hi = e == last ? 1.0f : acc / sel.total;
+          if (hi > u_sel || e == last) {
+            index = e;
+            p = w / sel.total;
+            break;
+          }
+          lo_c = hi;
+        }
+        (void)lo_c;
+      }
+      if (ok && p > 0.0f) {
+        i32 a = S.entries[index];
+        if (a >= 0) {
+          const DevArea& L = S.areas[a];
+          float x0, y0;
+          if (L.rect) {
+            x0 = (2.0f * ul0 - 1.0f) * L.half_x;
+            y0 = (2.0f * ul1 - 1.0f) * L.half_y;
+          } else {
+            float dx, dy;
+            concentric_disk(ul0, ul1, dx, dy);
+            x0 = dx * L.radius;
+            y0 = dy * L.radius;
+          }
+          V3 pt = L.centre + L.frame.x * x0 + L.frame.y * y0;
+          V3 dv = pt - s_point;
+          float dist2 = length_sq(dv);
+          float distance = sqrt_(dist2);
+          V3 wi = dv / distance;
+          float cos = -dot(wi, L.frame.z);
+          if (cos > 0.0f && distance > 0.0f) {
+            ls_wi = wi;
+            ls_dist = distance;
+            ls_rad = radiance_at(L, cos);
+            ls_pdf = p * dist2 / (L.area * cos);
+            ls_glossy = L.glossy != 0;
+            ls_shadow = L.shadow != 0;
+            have_ls = true;
+          }
+        } else if (S.env_total_mass > 0.0f) {
+          float target = ul0 * S.env_total_mass;
+          float z;
+          if (target < S.env_lower_mass) {
+            z = -1.0f + invert_linear(S.env_floor, S.env_horizon, target / S.env_lower_mass);
+          } else {
+            float upper = S.env_total_mass - S.env_lower_mass;
+            float fraction = clampf((target - S.env_lower_mass) / upper, 0.0f, 1.0f);
+            z = invert_linear(S.env_horizon, S.env_zenith, fraction);
+          }
+          z = clampf(z, -1.0f, 1.0f);
+          float rr = sqrt_(fmax_r((1.0f - z) * (1.0f + z), 0.0f));
+          float sn, cs;
+          sin_cos(TAU * ul1, sn, cs);
+          V3 dir = v3(rr * cs, rr * sn, z);
+          float pdf = env_pdf(S, dir.z);
+          if (pdf > 0.0f) {
+            ls_wi = dir;
+            ls_dist = RPX_INF;
+            ls_rad = splat(env_value(S, dir.z));
+            ls_pdf = p * pdf;
+            ls_glossy = true;
+            ls_shadow = true;
+            have_ls = true;
+          }
+        }
+        if (have_ls && !(ls_pdf > 0.0f && is_finite(ls_pdf) && luminance(ls_rad) > 0.0f)) have_ls = false;
+      }
+    }
+    if (have_ls) {
+      V3 wi = to_local(frame, ls_wi);
+      C3 f = bsdf_eval(T, B, wo, wi, !ls_glossy);
+      if (peak(f) > 0.0f) {
+        V3 from_p = wi.z < 0.0f ? offset_origin(s_hit_point, -ng) : origin;
+        float weight = !ls_glossy ? 1.0f : power_heuristic(ls_pdf, bsdf_pdf(T, B, wo, wi));
+        C3 c = thr * f * ls_rad * (fabs_(wi.z) * weight / ls_pdf);
+        // What `add` would add (finite, clamped), queued behind the shadow ray.
+        C3 add_c = splat(0.0f);
+        bool adds = finite_c(c);
+        if (adds) {
+          float pk = fmax_r(fmax_r(c.r, c.g), c.b);
+          add_c = (bounce >= 1 && S.clamp_indirect > 0.0f && pk > S.clamp_indirect) ? c * (S.clamp_indirect / pk) : c;
+        }
+        if (!ls_shadow) {
+          if (adds) sum = sum + add_c;
+        } else {
+          float t_max = is_finite(ls_dist) ? ls_dist * (1.0f - 1.0e-4f) : RPX_INF;
+          bool floor_blocks = false;
+          if (S.floor_mode == 1 && ls_wi.z != 0.0f) {
+            float tf = (S.floor_z - from_p.z) / ls_wi.z;
+            floor_blocks = tf > RAY_EPSILON && tf < t_max;
+          }
+          if (floor_blocks) {
+            atomicAdd((unsigned long long*)&P.rays[1], 1ull);
+          } else if (adds) {
+            want_shadow = true;
+            P.sh_o[i] = make_float4(from_p.x, from_p.y, from_p.z, t_max);
+            P.sh_d[i] = make_float4(ls_wi.x, ls_wi.y, ls_wi.z, 0.0f);
+            P.sh_c[i] = make_float4(add_c.r, add_c.g, add_c.b, 0.0f);
+          } else {
+            // A shadow ray is still cast (and counted) for a contribution that adds nothing.
+            atomicAdd((unsigned long long*)&P.rays[1], 1ull);
+          }
+        }
+      }
+    }
+    // The next direction (integrator/mod.rs:1772-1888).
+    float ub0 = sample_at(P.sobol, key, sample_index, dims + OFFSET_BSDF);
+    float ub1 = sample_at(P.sobol, key, sample_index, dims + OFFSET_BSDF + 1);
+    BsdfSample bs;
+    if (bsdf_sample(T, B, wo, ub0, ub1, bs)) {
+      bool transmitted = bs.wi.z < 0.0f;
+      V3 wi = normalize(to_world(frame, bs.wi));
+      bool ok = transmitted ? !(dot(wi, ng) >= 0.0f) : !(dot(wi, ng) <= 0.0f);
+      if (ok) {
+        thr = thr * bs.weight;
+        if (bounce >= 2) {
+          float survive = clampf(peak(thr), 0.05f, 1.0f);
+          float u_rr = sample_at(P.sobol, key, sample_index, dims + OFFSET_ROULETTE);
+          if (u_rr >= survive) ok = false;
+          else thr = thr / survive;
+        }
+        if (ok && !(peak(thr) > 0.0f && is_finite(peak(thr)))) ok = false;
+      }
+      if (ok) {
+        V3 next_o = transmitted ? offset_origin(s_hit_point, -ng) : origin;
+        u32 nflags = FLAG_HAS_PDF | (sel.by_flux ? FLAG_BY_FLUX : 0u) | ((bounce + 1) << BOUNCE_SHIFT);
+        C3 sg = c3(md.x, md.y, md.z);
+        bool med = in_medium;
+        if (transmitted) {
+          med = false;
+          if (s_outside && s_instance && mat < S.n_materials && S.materials[mat].has_medium) {
+            med = true;
+            sg = S.materials[mat].sigma;
+          }
+        }
+        if (med) nflags |= FLAG_MEDIUM;
+        P.ray_o[i] = make_float4(next_o.x, next_o.y, next_o.z, 0.0f);
+        P.ray_d[i] = make_float4(wi.x, wi.y, wi.z, 0.0f);
+        P.med[i] = make_float4(sg.r, sg.g, sg.b, u2f(nflags));
+        P.thr[i] = make_float4(thr.r, thr.g, thr.b, bs.pdf);
+        P.sel_p[i] = make_float4(sel.point.x, sel.point.y, sel.point.z, sel.total);
+        P.sel_w0[i] = make_float4(sel.w[0], sel.w[1], sel.w[2], sel.w[3]);
+        P.sel_w1[i] = make_float4(sel.w[4], sel.w[5], sel.w[6], sel.w[7]);
+        cont = true;
+      }
+    }
+  }
+  P.sum[i] = make_float4(sum.r, sum.g, sum.b, alpha);
+  }
+}
 }  // namespace
 
-__global__ void __launch_bounds__(256) k_shade(DevScene S, PathState P, RoundArgs R) {
-  const EnergyTables& T = S.tables;
-  const u32* q = P.queue[R.round & 1];
-  u32 n = P.counts[R.round & 1];
-  u32* out_q = P.queue[(R.round + 1) & 1];
-  u32* out_n = &P.counts[(R.round + 1) & 1];
-  const u32 bounce = R.round;
-  const u32 dims = bounce_dim(bounce);
+__global__ void __launch_bounds__(256) k_shade(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
+  const u32* q = P.live[parity];
+  u32 n = P.counts[C_LIVE0 + parity];
+  u32* out_q = P.live[parity ^ 1];
+  u32* out_n = &P.counts[C_LIVE0 + (parity ^ 1)];
   u32 stride = gridDim.x * blockDim.x;
   // Whole waves iterate together so that the ballots below see every lane.
   u32 n_iter = (n + stride - 1) / stride;
@@ -539,429 +1075,70 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, PathState P, RoundArg
     u32 j = it * stride + gid;
     bool active = j < n;
     bool cont = false, want_shadow = false;
-    u32 i = 0;
+    u32 i = 0, sid = 0;
     if (active) {
       i = q[j];
-      float4 ro = P.ray_o[i], rd = P.ray_d[i];
-      V3 o = v3(ro.x, ro.y, ro.z), d = v3(rd.x, rd.y, rd.z);
-      uint4 h = P.hit[i];
-      float4 th = P.thr[i];
-      float4 sm = P.sum[i];
-      float4 md = P.med[i];
-      u32 flags = f2u(md.w);
-      bool primary = (flags & FLAG_PRIMARY) != 0;
-      bool has_pdf = (flags & FLAG_HAS_PDF) != 0;
-      bool in_medium = (flags & FLAG_MEDIUM) != 0;
-      C3 thr = c3(th.x, th.y, th.z);
-      float prev_pdf = th.w;
-      C3 sum = c3(sm.x, sm.y, sm.z);
-      float alpha = sm.w;
-      u64 key = P.key[i];
-      // The selection of the vertex the ray left (only read once a BSDF sample made the ray).
-      Sel from;
-      if (has_pdf) {
-        float4 sp = P.sel_p[i];
-        from.point = v3(sp.x, sp.y, sp.z);
-        from.total = sp.w;
-        from.by_flux = (flags & FLAG_BY_FLUX) != 0;
-        float4 w0 = P.sel_w0[i], w1 = P.sel_w1[i];
-        from.w[0] = w0.x; from.w[1] = w0.y; from.w[2] = w0.z; from.w[3] = w0.w;
-        from.w[4] = w1.x; from.w[5] = w1.y; from.w[6] = w1.z; from.w[7] = w1.w;
-      }
-      bool hit_ok = h.x != 0xffffffffu;
-      float t_geometry = hit_ok ? u2f(h.z) : RPX_INF;
-      float t_floor = RPX_INF;
-      if ((S.floor_mode == 1) && d.z != 0.0f) {
-        float tf = (S.floor_z - o.z) / d.z;
-        if (tf > RAY_EPSILON && tf < RPX_INF) t_floor = tf;
-      }
-      float t_nearest = fmin_r(t_geometry, t_floor);
-      if (in_medium && is_finite(t_nearest)) {
-        C3 sg = c3(md.x, md.y, md.z);
-        thr = thr * c3(fexp(-sg.r * t_nearest), fexp(-sg.g * t_nearest), fexp(-sg.b * t_nearest));
-      }
-      // Area lights crossed before the surface (light.rs:1354-1384).
-      C3 crossed = splat(0.0f);
-      for (u32 a = 0; a < S.n_areas; ++a) {
-        const DevArea& L = S.areas[a];
-        bool visible = has_pdf ? L.glossy : L.camera;
-        if (!visible) continue;
-        float t, cos;
-        if (!area_intersect(L, o, d, t_nearest, t, cos)) continue;
-        float weight = 1.0f;
-        if (has_pdf) {
-          float dd = 0.0f + t;
-          float light_pdf = sel_pmf(S, from, L.entry) * dd * dd / (L.area * cos);
-          weight = power_heuristic(prev_pdf, light_pdf);
-        }
-        crossed = crossed + radiance_at(L, cos) * weight;
-      }
-      add_to(sum, thr * crossed, bounce >= 2, S.clamp_indirect);
-      bool done = false;
-      if (t_nearest == RPX_INF) {
-        if (primary) {
-          if (S.background == 0) {
-            C3 seen = splat(env_value(S, d.z));
-            if (S.exterior != 1.0f) seen = seen * S.exterior;
-            sum = sum + thr * seen;
-          } else if (S.background == 1) {
-            sum = sum + thr * S.background_color;
-          } else {
-            alpha = peak(crossed) > 0.0f ? 1.0f : 0.0f;
-          }
-        } else {
-          C3 sky = splat(env_value(S, d.z));
-          if (has_pdf && S.has_env_entry)
-            sky = sky * power_heuristic(prev_pdf, sel_pmf(S, from, S.env_entry) * env_pdf(S, d.z));
-          add_to(sum, thr * sky, bounce >= 2, S.clamp_indirect);
-        }
-        done = true;
-      }
-      V3 s_point, s_hit_point, s_normal, s_geometric;
-      bool s_instance = false, s_front = true, s_outside = true;
-      u32 inst = 0;
-      if (!done) {
-        if (t_floor < t_geometry) {
-          V3 p = o + d * t_floor;
-          V3 point = v3(p.x, p.y, S.floor_z);
-          V3 nn = v3(0.0f, 0.0f, d.z < 0.0f ? 1.0f : -1.0f);
-          s_point = point;
-          s_hit_point = point;
-          s_normal = nn;
-          s_geometric = nn;
-        } else {
-          // Hit details (geometry/mod.rs:731-808) and the terminator lift (:682-728).
-          inst = h.x;
-          u32 tri = h.y;
-          const DevInstance& in = S.instances[inst];
-          const DevMesh& mesh = S.meshes[in.mesh];
-          bool ident = (in.flags & INST_IDENTITY) != 0;
-          const u32* idx = S.indices + mesh.index_offset + 3 * (size_t)tri;
-          u32 i0 = mesh.vertex_offset + idx[0], i1 = mesh.vertex_offset + idx[1], i2 = mesh.vertex_offset + idx[2];
-          const float* ps = S.positions;
-          V3 p0 = v3(ps[3 * (size_t)i0], ps[3 * (size_t)i0 + 1], ps[3 * (size_t)i0 + 2]);
-          V3 p1 = v3(ps[3 * (size_t)i1], ps[3 * (size_t)i1 + 1], ps[3 * (size_t)i1 + 2]);
-          V3 p2 = v3(ps[3 * (size_t)i2], ps[3 * (size_t)i2 + 1], ps[3 * (size_t)i2 + 2]);
-          V3 lo = ident ? o : xf_point(in.to_local, o), ld = ident ? d : xf_vector(in.to_local, d);
-          PRay r = prepare(lo, ld, RAY_EPSILON, RPX_INF);
-          float b0, b1, b2;
-          {
-            float ox = r.org[r.kx], oy = r.org[r.ky], oz = r.org[r.kz];
-            float a_x = comp(p0, r.kx) - ox, a_y = comp(p0, r.ky) - oy, a_z = comp(p0, r.kz) - oz;
-            float b_x = comp(p1, r.kx) - ox, b_y = comp(p1, r.ky) - oy, b_z = comp(p1, r.kz) - oz;
-            float c_x = comp(p2, r.kx) - ox, c_y = comp(p2, r.ky) - oy, c_z = comp(p2, r.kz) - oz;
-            float ax = a_x - r.sx * a_z, ay = a_y - r.sy * a_z;
-            float bx = b_x - r.sx * b_z, by = b_y - r.sy * b_z;
-            float cx = c_x - r.sx * c_z, cy = c_y - r.sy * c_z;
-            float u = cx * by - cy * bx, v = ax * cy - ay * cx, w = bx * ay - by * ax;
-            bool negative = u < 0.0f || v < 0.0f || w < 0.0f;
-            bool positive = u > 0.0f || v > 0.0f || w > 0.0f;
-            float det = u + v + w;
-            bool ok = !(negative && positive) && det != 0.0f;
-            if (ok) {
-              float az = r.sz * a_z, bz = r.sz * b_z, cz = r.sz * c_z;
-              float t = (u * az + v * bz + w * cz) / det;
-              ok = t >= r.t_min && t <= F32_MAX;
-            }
-            if (ok) {
-              b0 = u / det;
-              b1 = v / det;
-              b2 = w / det;
-            } else {
-              b0 = b1 = b2 = 1.0f / 3.0f;
-            }
-          }
-          V3 point_local = p0 * b0 + p1 * b1 + p2 * b2;
-          V3 wind = cross(p1 - p0, p2 - p0);
-          V3 winding = normalize(ident ? wind : xf_transpose(in.to_local, wind));
-          V3 incoming = normalize(d);
-          V3 geometric = length_sq(winding) > 0.0f ? winding : -incoming;
-          bool front_face = dot(winding, d) < 0.0f;
-          V3 smooth = v3(0.0f, 0.0f, 0.0f);
-          V3 n0, n1, n2;
-          if (mesh.has_normals) {
-            const float* ns = S.normals;
-            n0 = v3(ns[3 * (size_t)i0], ns[3 * (size_t)i0 + 1], ns[3 * (size_t)i0 + 2]);
-            n1 = v3(ns[3 * (size_t)i1], ns[3 * (size_t)i1 + 1], ns[3 * (size_t)i1 + 2]);
-            n2 = v3(ns[3 * (size_t)i2], ns[3 * (size_t)i2 + 1], ns[3 * (size_t)i2 + 2]);
-            V3 interp = n0 * b0 + n1 * b1 + n2 * b2;
-            smooth = ident ? interp : xf_transpose(in.to_local, interp);
-          }
-          V3 normal = length_sq(smooth) > 1.0e-12f ? normalize(smooth) : geometric;
-          V3 f_normal = dot(normal, d) > 0.0f ? -normal : normal;
-          V3 ng = dot(geometric, d) > 0.0f ? -geometric : geometric;
-          V3 hit_point = ident ? point_local : xf_point(in.to_world, point_local);
-          // bevel::shading_point without a bevel
-          V3 sn = f_normal;
-          if (dot(sn, ng) < 0.0f) sn = -sn;
-          if (dot(sn, d) >= -1.0e-4f) sn = ng;
-          // terminator_point
-          V3 tp = hit_point;
-          if (mesh.has_normals) {
-            V3 pt = p0 * b0 + p1 * b1 + p2 * b2;
-            V3 interp = n0 * b0 + n1 * b1 + n2 * b2;
-            V3 world_normal = ident ? interp : xf_transpose(in.to_local, interp);
-            float side = dot(world_normal, ng) < 0.0f ? -1.0f : 1.0f;
-            V3 lifted = pt;
-            V3 cs[3] = {p0, p1, p2};
-            V3 nsv[3] = {n0, n1, n2};
-            float ws[3] = {b0, b1, b2};
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-              V3 nk = normalize(nsv[k]) * side;
-              float below = dot(pt - cs[k], nk);
-              if (below < 0.0f) lifted = lifted - nk * (below * ws[k]);
-            }
-            if (finite3(lifted)) tp = ident ? lifted : xf_point(in.to_world, lifted);
-          }
-          s_point = tp;
-          s_hit_point = hit_point;
-          s_normal = sn;
-          s_geometric = ng;
-          s_instance = true;
-          s_front = front_face;
-          s_outside = front_face != (mesh.inside_out != 0);
-        }
-      }
-      u32 mat = 0xffffffffu;
-      if (!done && s_instance) {
-        mat = S.instances[inst].material;
-        if (mat < S.n_materials) {
-          const DevMaterial& M = S.materials[mat];
-          if (M.emissive) {
-            bool one_sided = (S.instances[inst].flags & INST_ONE_SIDED) != 0;
-            if (s_front || !one_sided) add_to(sum, thr * M.radiance * 1.0f, bounce >= 2, S.clamp_indirect);
-            done = true;
-          }
-        }
-      }
-      if (!done && bounce >= S.max_depth) done = true;
-      if (!done) {
-        const Bsdf& B = s_instance ? (mat < S.n_materials ? S.bsdfs[2 * mat + (s_outside ? 0 : 1)]
-                                                          : S.fallback_bsdf[s_outside ? 0 : 1])
-                                   : S.floor_bsdf;
-        Frame frame = frame_from_normal(s_normal);
-        V3 wo = to_local(frame, -d);
-        V3 ng = s_geometric;
-        V3 origin = offset_origin(s_point, ng);
-        // Light selection at this vertex (light.rs:1026-1046).
-        Sel sel;
-        sel.point = s_point;
-        sel.total = 0.0f;
-        sel.by_flux = 1;
-        for (int k = 0; k < KEPT_WEIGHTS; ++k) sel.w[k] = 0.0f;
-        u32 ne = S.n_entries;
-        if (S.per_point && ne > 0) {
-          sel.by_flux = 0;
-          for (u32 e = 0; e < ne; ++e) {
-            float w = entry_weight(S, e, s_point);
-            if (e < (u32)KEPT_WEIGHTS) sel.w[e] = w;
-            sel.total += w;
-          }
-        }
-        // Next-event estimation (light.rs:1150-1315).
-        float u_sel = sample1(key, R.sample_index, dims + OFFSET_LIGHT_SELECT, R.sobol[OFFSET_LIGHT_SELECT]);
-        float ul0 = sample1(key, R.sample_index, dims + OFFSET_LIGHT, R.sobol[OFFSET_LIGHT]);
-        float ul1 = sample1(key, R.sample_index, dims + OFFSET_LIGHT + 1, R.sobol[OFFSET_LIGHT + 1]);
-        bool have_ls = false;
-        V3 ls_wi = v3(0, 0, 0);
-        float ls_dist = 0.0f, ls_pdf = 0.0f;
-        C3 ls_rad = splat(0.0f);
-        bool ls_glossy = true, ls_shadow = true;
-        if (ne > 0) {
-          u32 index = 0;
-          float p = 0.0f;
-          bool ok = true;
-          if (sel.by_flux) {
-            u32 k = 0;
-            while (k < ne && S.cdf[k] <= u_sel) ++k;
-            index = k < ne - 1 ? k : ne - 1;
-            p = sel_pmf(S, sel, index);
-          } else if (!(sel.total > 0.0f && is_finite(sel.total))) {
-            ok = false;
-          } else {
-            float acc = 0.0f, lo_c = 0.0f;
-            u32 last = ne - 1;
-            for (u32 e = 0; e <= last; ++e) {
-              float w = sel_weight(S, sel, e);
-              acc += w;
-              float hi = e == last ? 1.0f : acc / sel.total;
-              if (hi > u_sel || e == last) {
-                index = e;
-                p = w / sel.total;
-                break;
-              }
-              lo_c = hi;
-            }
-            (void)lo_c;
-          }
-          if (ok && p > 0.0f) {
-            i32 a = S.entries[index];
-            if (a >= 0) {
-              const DevArea& L = S.areas[a];
-              float x0, y0;
-              if (L.rect) {
-                x0 = (2.0f * ul0 - 1.0f) * L.half_x;
-                y0 = (2.0f * ul1 - 1.0f) * L.half_y;
-              } else {
-                float dx, dy;
-                concentric_disk(ul0, ul1, dx, dy);
-                x0 = dx * L.radius;
-                y0 = dy * L.radius;
-              }
-              V3 pt = L.centre + L.frame.x * x0 + L.frame.y * y0;
-              V3 dv = pt - s_point;
-              float dist2 = length_sq(dv);
-              float distance = sqrt_(dist2);
-              V3 wi = dv / distance;
-              float cos = -dot(wi, L.frame.z);
-              if (cos > 0.0f && distance > 0.0f) {
-                ls_wi = wi;
-                ls_dist = distance;
-                ls_rad = radiance_at(L, cos);
-                ls_pdf = p * dist2 / (L.area * cos);
-                ls_glossy = L.glossy != 0;
-                ls_shadow = L.shadow != 0;
-                have_ls = true;
-              }
-            } else if (S.env_total_mass > 0.0f) {
-              float target = ul0 * S.env_total_mass;
-              float z;
-              if (target < S.env_lower_mass) {
-                z = -1.0f + invert_linear(S.env_floor, S.env_horizon, target / S.env_lower_mass);
-              } else {
-                float upper = S.env_total_mass - S.env_lower_mass;
-                float fraction = clampf((target - S.env_lower_mass) / upper, 0.0f, 1.0f);
-                z = invert_linear(S.env_horizon, S.env_zenith, fraction);
-              }
-              z = clampf(z, -1.0f, 1.0f);
-              float rr = sqrt_(fmax_r((1.0f - z) * (1.0f + z), 0.0f));
-              float sn, cs;
-              sin_cos(TAU * ul1, sn, cs);
-              V3 dir = v3(rr * cs, rr * sn, z);
-              float pdf = env_pdf(S, dir.z);
-              if (pdf > 0.0f) {
-                ls_wi = dir;
-                ls_dist = RPX_INF;
-                ls_rad = splat(env_value(S, dir.z));
-                ls_pdf = p * pdf;
-                ls_glossy = true;
-                ls_shadow = true;
-                have_ls = true;
-              }
-            }
-            if (have_ls && !(ls_pdf > 0.0f && is_finite(ls_pdf) && luminance(ls_rad) > 0.0f)) have_ls = false;
-          }
-        }
-        if (have_ls) {
-          V3 wi = to_local(frame, ls_wi);
-          C3 f = bsdf_eval(T, B, wo, wi, !ls_glossy);
-          if (peak(f) > 0.0f) {
-            V3 from_p = wi.z < 0.0f ? offset_origin(s_hit_point, -ng) : origin;
-            float weight = !ls_glossy ? 1.0f : power_heuristic(ls_pdf, bsdf_pdf(T, B, wo, wi));
-            C3 c = thr * f * ls_rad * (fabs_(wi.z) * weight / ls_pdf);
-            // What `add` would add (finite, clamped), queued behind the shadow ray.
-            C3 add_c = splat(0.0f);
-            bool adds = finite_c(c);
-            if (adds) {
-              float pk = fmax_r(fmax_r(c.r, c.g), c.b);
-              add_c = (bounce >= 1 && S.clamp_indirect > 0.0f && pk > S.clamp_indirect) ? c * (S.clamp_indirect / pk) : c;
-            }
-            if (!ls_shadow) {
-              if (adds) sum = sum + add_c;
-            } else {
-              float t_max = is_finite(ls_dist) ? ls_dist * (1.0f - 1.0e-4f) : RPX_INF;
-              bool floor_blocks = false;
-              if (S.floor_mode == 1 && ls_wi.z != 0.0f) {
-                float tf = (S.floor_z - from_p.z) / ls_wi.z;
-                floor_blocks = tf > RAY_EPSILON && tf < t_max;
-              }
-              if (floor_blocks) {
-                atomicAdd((unsigned long long*)&P.rays[1], 1ull);
-              } else if (adds) {
-                want_shadow = true;
-                P.sh_o[i] = make_float4(from_p.x, from_p.y, from_p.z, t_max);
-                P.sh_d[i] = make_float4(ls_wi.x, ls_wi.y, ls_wi.z, 0.0f);
-                P.sh_c[i] = make_float4(add_c.r, add_c.g, add_c.b, 0.0f);
-              } else {
-                // A shadow ray is still cast (and counted) for a contribution that adds nothing.
-                atomicAdd((unsigned long long*)&P.rays[1], 1ull);
-              }
-            }
-          }
-        }
-        // The next direction (integrator/mod.rs:1772-1888).
-        float ub0 = sample1(key, R.sample_index, dims + OFFSET_BSDF, R.sobol[OFFSET_BSDF]);
-        float ub1 = sample1(key, R.sample_index, dims + OFFSET_BSDF + 1, R.sobol[OFFSET_BSDF + 1]);
-        BsdfSample bs;
-        if (bsdf_sample(T, B, wo, ub0, ub1, bs)) {
-          bool transmitted = bs.wi.z < 0.0f;
-          V3 wi = normalize(to_world(frame, bs.wi));
-          bool ok = transmitted ? !(dot(wi, ng) >= 0.0f) : !(dot(wi, ng) <= 0.0f);
-          if (ok) {
-            thr = thr * bs.weight;
-            if (bounce >= 2) {
-              float survive = clampf(peak(thr), 0.05f, 1.0f);
-              float u_rr = sample1(key, R.sample_index, dims + OFFSET_ROULETTE, R.sobol[OFFSET_ROULETTE]);
-              if (u_rr >= survive) ok = false;
-              else thr = thr / survive;
-            }
-            if (ok && !(peak(thr) > 0.0f && is_finite(peak(thr)))) ok = false;
-          }
-          if (ok) {
-            V3 next_o = transmitted ? offset_origin(s_hit_point, -ng) : origin;
-            u32 nflags = FLAG_HAS_PDF | (sel.by_flux ? FLAG_BY_FLUX : 0u);
-            C3 sg = c3(md.x, md.y, md.z);
-            bool med = in_medium;
-            if (transmitted) {
-              med = false;
-              if (s_outside && s_instance && mat < S.n_materials && S.materials[mat].has_medium) {
-                med = true;
-                sg = S.materials[mat].sigma;
-              }
-            }
-            if (med) nflags |= FLAG_MEDIUM;
-            P.ray_o[i] = make_float4(next_o.x, next_o.y, next_o.z, 0.0f);
-            P.ray_d[i] = make_float4(wi.x, wi.y, wi.z, 0.0f);
-            P.med[i] = make_float4(sg.r, sg.g, sg.b, u2f(nflags));
-            P.thr[i] = make_float4(thr.r, thr.g, thr.b, bs.pdf);
-            P.sel_p[i] = make_float4(sel.point.x, sel.point.y, sel.point.z, sel.total);
-            P.sel_w0[i] = make_float4(sel.w[0], sel.w[1], sel.w[2], sel.w[3]);
-            P.sel_w1[i] = make_float4(sel.w[4], sel.w[5], sel.w[6], sel.w[7]);
-            cont = true;
-          }
-        }
-      }
-      P.sum[i] = make_float4(sum.r, sum.g, sum.b, alpha);
+      sid = P.ident[i].y;
     }
-    // Compaction of the survivors and of the shadow rays (whole waves take part).
+    // One pass per distinct scene of the wave.
+    bool todo = active;
+    u64 pending = __ballot(todo);
+    while (pending) {
+      u32 cur = lowest_lane_value(pending, sid);
+      if (todo && sid == cur) {
+        shade_path(scenes[cur], P, i, cont, want_shadow);
+        todo = false;
+      }
+      pending = __ballot(todo);
+    }
+    // Compaction of the survivors, the finished paths and the shadow rays.
     u32 slot = wave_append(out_n, cont);
     if (cont) out_q[slot] = i;
-    u32 sslot = wave_append(&P.counts[2], want_shadow);
+    bool done = active && !cont;
+    u32 dslot = wave_append(&P.counts[C_DONE], done);
+    if (done) P.done_queue[dslot] = i;
+    u32 sslot = wave_append(&P.counts[C_SHADOW], want_shadow);
     if (want_shadow) P.shadow_queue[sslot] = i;
   }
 }
 
 // ================================================================== splat / tile output
-__global__ void __launch_bounds__(256) k_splat(PathState P, Film F, u32 n) {
+// Adds samples first .. first + count of each pixel of the segment to its film, in sample order
+// (image.rs:81-94): the sums are carried in registers and stored once, which performs the same f32
+// additions in the same order as adding sample by sample.
+__global__ void __launch_bounds__(256) k_splat(PathState P, DevSegment s) {
+  u32 n = s.w * s.h;
   u32 stride = gridDim.x * blockDim.x;
-  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    float4 s = P.sum[i];
-    C3 c = c3(s.x, s.y, s.z) * splat(1.0f);
-    F.rgb[3 * (size_t)i] += c.r;
-    F.rgb[3 * (size_t)i + 1] += c.g;
-    F.rgb[3 * (size_t)i + 2] += c.b;
-    F.weight[i] += 1.0f;
-    F.alpha[i] += s.w;
-    float l = luminance(c);
-    F.lum2[i] += l * l;
+  for (u32 p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+    u32 px = s.x0 + p % s.w, py = s.y0 + p / s.w;
+    size_t f = (size_t)py * s.film_w + px;
+    float r = s.film.rgb[3 * f], g = s.film.rgb[3 * f + 1], b = s.film.rgb[3 * f + 2];
+    float wt = s.film.weight[f], al = s.film.alpha[f], l2 = s.film.lum2[f];
+    const float4* res = P.results + s.base + p;
+    for (u32 k = 0; k < s.count; ++k) {
+      float4 v = res[(size_t)k * n];
+      C3 c = c3(v.x, v.y, v.z) * splat(1.0f);
+      r += c.r;
+      g += c.g;
+      b += c.b;
+      wt += 1.0f;
+      al += v.w;
+      float l = luminance(c);
+      l2 += l * l;
+    }
+    s.film.rgb[3 * f] = r;
+    s.film.rgb[3 * f + 1] = g;
+    s.film.rgb[3 * f + 2] = b;
+    s.film.weight[f] = wt;
+    s.film.alpha[f] = al;
+    s.film.lum2[f] = l2;
   }
 }
 
 __global__ void __launch_bounds__(256) k_tile_out(PathState P, float* out, u32 n, u32 channels) {
   u32 stride = gridDim.x * blockDim.x;
   for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    float4 s = P.sum[i];
+    float4 s = P.results[i];
     // A one-sample film resolves to the sample times 1/1.
     float inv = 1.0f / 1.0f;
     out[channels * (size_t)i] = s.x * inv;
@@ -972,21 +1149,22 @@ __global__ void __launch_bounds__(256) k_tile_out(PathState P, float* out, u32 n
 }
 
 // ================================================================== ray queries for parity tests
-__global__ void __launch_bounds__(BLOCK) k_trace_rays(DevScene S, const float* rays, u32 n, u32* inst, u32* tri,
-                                                       float* t, uint8_t* any) {
-  __shared__ u32 lds[LDS_STACK * BLOCK];
+__global__ void __launch_bounds__(128) k_trace_rays(const DevScene* __restrict__ scene, const float* rays, u32 n,
+                                                     u32* inst, u32* tri, float* t, uint8_t* any) {
+  __shared__ u32 lds[32 * 128];
   u32* col = lds + threadIdx.x;
+  Geo g = geo_of(scene, 0);
   u32 stride = gridDim.x * blockDim.x;
   for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const float* r = rays + 8 * (size_t)i;
     V3 o = v3(r[0], r[1], r[2]), d = v3(r[3], r[4], r[5]);
     u32 hi = 0xffffffffu, ht = 0xffffffffu;
     float tt = RPX_INF;
-    trace_nearest(S, col, o, d, r[6], r[7], false, hi, ht, tt);
+    trace_nearest<128, 32>(g, col, o, d, r[6], r[7], false, hi, ht, tt);
     inst[i] = hi;
     tri[i] = ht;
     t[i] = tt;
-    any[i] = trace_any(S, col, o, d, r[6], r[7]) ? 1 : 0;
+    any[i] = trace_any<128, 32>(g, col, o, d, r[6], r[7]) ? 1 : 0;
   }
 }
 
@@ -1117,33 +1295,60 @@ static inline u32 grid_for(u32 n, u32 block, u32 cap) {
   return g < cap ? g : cap;
 }
 
-hipError_t launch_camera(const DevScene& S, const PathState& P, const PassArgs& A, hipStream_t st) {
-  hipLaunchKernelGGL(k_camera, dim3(grid_for(A.n_paths, 256, 8192)), dim3(256), 0, st, S, P, A);
+hipError_t launch_pool_init(const PathState& P, const Launch& L, u32 work_end) {
+  hipLaunchKernelGGL(k_pool_init, dim3(grid_for(L.pool, 256, 4096)), dim3(256), 0, L.stream, P, L.pool, work_end);
   return hipGetLastError();
 }
-hipError_t launch_extend(const DevScene& S, const PathState& P, u32 round, u32 max_paths, hipStream_t st) {
-  hipLaunchKernelGGL(k_extend, dim3(grid_for(max_paths, BLOCK, 8192)), dim3(BLOCK), 0, st, S, P, round);
+hipError_t launch_regen(const PathState& P, const Launch& L, u32 parity) {
+  hipLaunchKernelGGL(k_regen, dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, L.segs, L.n_segs, P,
+                     parity);
   return hipGetLastError();
 }
-hipError_t launch_shade(const DevScene& S, const PathState& P, const RoundArgs& R, u32 max_paths, hipStream_t st) {
-  hipLaunchKernelGGL(k_shade, dim3(grid_for(max_paths, 256, 8192)), dim3(256), 0, st, S, P, R);
+
+#define RPX_TRAVERSAL_SHAPES(X) X(0, 128, 24, 1) X(1, 128, 32, 1) X(2, 128, 16, 1) X(3, 256, 16, 1) X(4, 128, 16, 8)
+
+hipError_t launch_extend(const PathState& P, const Launch& L, u32 parity) {
+  switch (L.shape) {
+#define X(I, B, N, W)                                                                                            \
+  case I:                                                                                                       \
+    hipLaunchKernelGGL((k_extend<B, N, W>), dim3(grid_for(L.pool, B, 1u << 20)), dim3(B), 0, L.stream, L.scenes, P, \
+                       parity);                                                                                 \
+    break;
+    RPX_TRAVERSAL_SHAPES(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
-hipError_t launch_shadow(const DevScene& S, const PathState& P, u32 max_paths, hipStream_t st) {
-  hipLaunchKernelGGL(k_shadow, dim3(grid_for(max_paths, BLOCK, 8192)), dim3(BLOCK), 0, st, S, P);
+hipError_t launch_shade(const PathState& P, const Launch& L, u32 parity) {
+  hipLaunchKernelGGL(k_shade, dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, P, parity);
   return hipGetLastError();
 }
-hipError_t launch_splat(const PathState& P, const Film& F, u32 n, hipStream_t st) {
-  hipLaunchKernelGGL(k_splat, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, P, F, n);
+hipError_t launch_shadow(const PathState& P, const Launch& L) {
+  switch (L.shape) {
+#define X(I, B, N, W)                                                                                             \
+  case I:                                                                                                        \
+    hipLaunchKernelGGL((k_shadow<B, N, W>), dim3(grid_for(L.pool, B, 1u << 20)), dim3(B), 0, L.stream, L.scenes, P); \
+    break;
+    RPX_TRAVERSAL_SHAPES(X)
+#undef X
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+hipError_t launch_splat(const PathState& P, const DevSegment& s, hipStream_t st) {
+  hipLaunchKernelGGL(k_splat, dim3(grid_for(s.w * s.h, 256, 8192)), dim3(256), 0, st, P, s);
   return hipGetLastError();
 }
 hipError_t launch_tile_out(const PathState& P, float* out, u32 n, u32 channels, hipStream_t st) {
   hipLaunchKernelGGL(k_tile_out, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, P, out, n, channels);
   return hipGetLastError();
 }
-hipError_t launch_trace_rays(const DevScene& S, const float* rays, u32 n, u32* inst, u32* tri, float* t, uint8_t* any,
-                             hipStream_t st) {
-  hipLaunchKernelGGL(k_trace_rays, dim3(grid_for(n, BLOCK, 8192)), dim3(BLOCK), 0, st, S, rays, n, inst, tri, t, any);
+hipError_t launch_trace_rays(const DevScene* scene, const float* rays, u32 n, u32* inst, u32* tri, float* t,
+                             uint8_t* any, hipStream_t st) {
+  hipLaunchKernelGGL(k_trace_rays, dim3(grid_for(n, 128, 8192)), dim3(128), 0, st, scene, rays, n, inst, tri, t, any);
   return hipGetLastError();
 }
 hipError_t launch_energy(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_in, uint16_t* clear_out, float* albedo_avg,

@@ -46,12 +46,26 @@ _SIGS = {
     "rpx_trace_rays": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 4),
     "rpx_scene_info": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_scene_stats": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
+    "rpx_set_options": (ctypes.c_uint32, [ctypes.c_uint32] * 6),
     "rpx_set_profiling": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
     "rpx_last_timing": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_energy_tables": (ctypes.c_uint32, [ctypes.c_void_p] * 6),
     "rpx_write_scene": (ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64]),
+    "rpx_queue_create": (ctypes.c_uint32, []),
+    "rpx_queue_add": (ctypes.c_uint32, [ctypes.c_uint32] * 4 + [ctypes.c_void_p]),
+    "rpx_queue_run": (ctypes.c_uint32, [ctypes.c_uint32]),
+    "rpx_queue_film_read": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32] + [ctypes.c_void_p] * 4),
+    "rpx_queue_set_options": (ctypes.c_uint32, [ctypes.c_uint32] * 6),
+    "rpx_queue_set_profiling": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
+    "rpx_queue_last_timing": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
+    "rpx_queue_stats": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
+    "rpx_queue_free": (ctypes.c_uint32, [ctypes.c_uint32]),
     "rpx_device_count": (ctypes.c_uint32, []),
 }
+
+KEEP = 0xFFFFFFFF
+FLAG_PER_PASS = 1  # one sample pass at a time (the pool drains after each pass)
+FLAG_GRAPH = 2     # groups of iterations launched as captured HIP graphs
 
 _lib = None
 
@@ -233,15 +247,86 @@ class Scene:
         _check(library().rpx_scene_stats(self.handle, _ptr(out)), "rpx_scene_stats")
         return {"rays": int(out[0]), "nearest": int(out[1]), "shadow": int(out[2]), "samples": int(out[3])}
 
+    def set_options(self, pool=0, shape=KEEP, group=0, chunk=0, per_pass=False, graph=False):
+        """Path-pool options (0 / KEEP keep the current value); the film does not depend on them."""
+        flags = (FLAG_PER_PASS if per_pass else 0) | (FLAG_GRAPH if graph else 0)
+        _check(library().rpx_set_options(self.handle, pool, shape, group, chunk, flags), "rpx_set_options")
+
     def set_profiling(self, on=True):
         _check(library().rpx_set_profiling(self.handle, 1 if on else 0), "rpx_set_profiling")
 
     def last_timing(self):
-        """ms of the last render call: total, closest-hit, shade and shadow kernels; launch counts."""
-        out = np.zeros(6, np.float32)
+        """GPU time of the last render: total, per-kernel sums (ms) and launch counts."""
+        out = np.zeros(12, np.float32)
         _check(library().rpx_last_timing(self.handle, _ptr(out)), "rpx_last_timing")
-        return {"total_ms": float(out[0]), "extend_ms": float(out[1]), "shade_ms": float(out[2]),
-                "shadow_ms": float(out[3]), "extend_launches": int(out[4]), "shadow_launches": int(out[5])}
+        return _timing(out)
+
+
+def _timing(out):
+    return {"total_ms": float(out[0]), "extend_ms": float(out[1]), "shade_ms": float(out[2]),
+            "shadow_ms": float(out[3]), "regen_ms": float(out[4]), "extend_launches": int(out[5]),
+            "shade_launches": int(out[6]), "shadow_launches": int(out[7]), "regen_launches": int(out[8]),
+            "iterations": int(out[9]), "chunks": int(out[10])}
+
+
+class Queue:
+    """A render queue (rpx_queue_*): jobs of several scenes rendered by one persistent path pool,
+    each into its own resident film."""
+
+    def __init__(self):
+        q = library().rpx_queue_create()
+        if q == 0:
+            raise RenderError(last_error())
+        self.handle = int(q)
+        self.jobs = []
+
+    def add(self, scene, first, count):
+        job = ctypes.c_uint32()
+        _check(library().rpx_queue_add(self.handle, scene.handle, first, count, ctypes.byref(job)), "rpx_queue_add")
+        self.jobs.append((scene.width, scene.height))
+        return int(job.value)
+
+    def run(self):
+        _check(library().rpx_queue_run(self.handle), "rpx_queue_run")
+
+    def film(self, job):
+        W, H = self.jobs[job]
+        rgb = np.zeros((H, W, 3), np.float32)
+        weight = np.zeros((H, W), np.float32)
+        alpha = np.zeros((H, W), np.float32)
+        lum2 = np.zeros((H, W), np.float32)
+        _check(library().rpx_queue_film_read(self.handle, job, _ptr(rgb), _ptr(weight), _ptr(alpha), _ptr(lum2)),
+               "rpx_queue_film_read")
+        return rgb, weight, alpha, lum2
+
+    def set_options(self, pool=0, shape=KEEP, group=0, chunk=0, per_pass=False, graph=False):
+        flags = (FLAG_PER_PASS if per_pass else 0) | (FLAG_GRAPH if graph else 0)
+        _check(library().rpx_queue_set_options(self.handle, pool, shape, group, chunk, flags),
+               "rpx_queue_set_options")
+
+    def set_profiling(self, on=True):
+        _check(library().rpx_queue_set_profiling(self.handle, 1 if on else 0), "rpx_queue_set_profiling")
+
+    def last_timing(self):
+        out = np.zeros(12, np.float32)
+        _check(library().rpx_queue_last_timing(self.handle, _ptr(out)), "rpx_queue_last_timing")
+        return _timing(out)
+
+    def stats(self):
+        out = np.zeros(4, np.uint64)
+        _check(library().rpx_queue_stats(self.handle, _ptr(out)), "rpx_queue_stats")
+        return {"rays": int(out[0]), "nearest": int(out[1]), "shadow": int(out[2]), "samples": int(out[3])}
+
+    def close(self):
+        if getattr(self, "handle", 0):
+            library().rpx_queue_free(self.handle)
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def resolve(rgb, weight):
