@@ -2,12 +2,16 @@
 displaced level-7 icospheres (2,621,440 triangles), four rough conductors, two smooth glass, two
 Lambert, three rect lights and the studio dome (repackshot_amd/csrc/rpx_generators.hip, kind 2).
 
-A step renders the whole image at 64 samples per pixel into the film resident in HBM. Prints one
-JSON line: Mpaths/s of the timed steps, Mrays/s, the roofline of the closest-hit kernel (the
-dominant one), the CPU baseline (the oracle on this host's cores over a bounded sample of the same
-workload) and the parity of the GPU film against the oracle on that sample's tiles.
+A step renders the whole image at 64 samples per pixel into the film resident in HBM (the path
+pool's launch shape and size chosen first by the library's tuner, outside the timed region).
+Prints one JSON line: Mpaths/s of the timed steps, Mrays/s, the roofline of the closest-hit kernel
+(the dominant one), the CPU baseline (the oracle on this host's cores over a bounded sample of the
+same workload), the parity of the GPU film against the oracle on that sample's tiles, and the
+mixed render queue (64 seeded jobs of 256..1024 px, 10k..500k triangles, 16..64 spp) rendered by
+one persistent pool beside the same jobs rendered one at a time in sample passes.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-queue]
+                       [--pool P --shape S]   (fixed launch options instead of the tuner)
 """
 import argparse
 import json
@@ -27,6 +31,9 @@ HBM_PEAK_GBS = 8000.0  # MI355X datasheet: 8 TB/s HBM3E
 # geometry/triangle.rs:160-167: v [3][3][4] f32 + id [4] u32 = 160 B).
 NODE_BYTES = 256
 PACKET_BYTES = 160
+# HBM bytes per k_extend launch measured by rocprofv3 counters (FETCH_SIZE + WRITE_SIZE), when a
+# measurement for the current kernels is recorded; None otherwise.
+TRAFFIC_BYTES_PER_LAUNCH = None
 
 
 def log(*a):
@@ -70,13 +77,66 @@ def cpu_baseline(rsb, spp, budget_s=15.0):
     return record, rects, film, counts
 
 
+def mixed_queue(native, seed=0, jobs=64):
+    """The mixed render queue: every job loaded as its own scene; the persistent pool (one queue,
+    tuned) against the plain per-job tracer (each scene alone, one sample pass at a time)."""
+    t0 = time.perf_counter()
+    scenes, spps, paths = [], [], 0
+    for j in range(jobs):
+        sc = native.Scene(native.write_queue_job(seed, j))
+        info = sc.info()
+        scenes.append(sc)
+        spps.append(info["max_samples"])
+        paths += info["width"] * info["height"] * info["max_samples"]
+    log("bench: queue of %d jobs loaded in %.1f s (%d paths)" % (jobs, time.perf_counter() - t0, paths))
+    q = native.Queue()
+    ids = [q.add(sc, 0, spp) for sc, spp in zip(scenes, spps)]
+    tune = q.autotune()
+    log("bench: queue tuner chose pool %d shape %d" % (tune["pool"], tune["shape"]))
+    q.run()  # warm
+    q.reset()
+    st0 = q.stats()
+    t1 = time.perf_counter()
+    q.run()
+    t_persistent = time.perf_counter() - t1
+    rays = q.stats()["rays"] - st0["rays"]
+    log("bench: persistent queue %.3f s" % t_persistent)
+    # The plain per-job tracer: each scene alone, sample passes drained one by one.
+    for sc in scenes:
+        sc.set_options(per_pass=True)
+    scenes[0].reset_film()
+    scenes[0].render_samples(0, 1)  # warm
+    t_plain = 0.0
+    for sc, spp in zip(scenes, spps):
+        sc.reset_film()
+        t2 = time.perf_counter()
+        sc.render_samples(0, spp)
+        t_plain += time.perf_counter() - t2
+    log("bench: plain per-job %.3f s" % t_plain)
+    # Every job's film from the queue equals the same scene rendered alone, bit for bit.
+    identical = all(all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(q.film(i), sc.film()))
+                    for i, sc in zip(ids, scenes))
+    q.close()
+    for sc in scenes:
+        sc.close()
+    return {"jobs": jobs, "paths": paths, "persistent_mpaths_per_s": round(paths / t_persistent / 1e6, 3),
+            "persistent_mrays_per_s": round(rays / t_persistent / 1e6, 3),
+            "plain_per_job_mpaths_per_s": round(paths / t_plain / 1e6, 3),
+            "speedup": round(t_plain / t_persistent, 3), "persistent_s": round(t_persistent, 3),
+            "plain_s": round(t_plain, 3), "tuned": {"pool": tune["pool"], "shape": tune["shape"]},
+            "films_bit_exact_vs_plain": identical}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-queue", action="store_true")
+    ap.add_argument("--pool", type=int, default=0)
+    ap.add_argument("--shape", type=int, default=-1)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this tracer runs on one GPU; --gpus %d is not supported (one process, one GPU)"
@@ -98,6 +158,13 @@ def main():
     W, H = info["width"], info["height"]
     paths_per_step = W * H * spp
 
+    if args.shape >= 0:
+        scene.set_options(pool=args.pool or (1 << 21), shape=args.shape)
+        tuned = {"pool": args.pool or (1 << 21), "shape": args.shape, "by": "fixed"}
+    else:
+        t = scene.autotune(4)
+        tuned = {"pool": t["pool"], "shape": t["shape"], "by": "tuner", "candidates_ms": t["ms"]}
+        log("bench: tuner chose pool %d shape %d (%s)" % (t["pool"], t["shape"], t["ms"]))
     scene.set_profiling(False)
     for i in range(args.warmup):
         scene.reset_film()
@@ -112,12 +179,11 @@ def main():
         t1 = time.perf_counter()
         scene.render_samples(0, spp)
         times.append(time.perf_counter() - t1)
-        log("bench: step %d: %.3f s" % (i, times[-1]))
+        log("bench: step %d: %.4f s" % (i, times[-1]))
     stats1 = scene.stats()
     total = sum(times)
     mpaths = paths_per_step * args.steps / total / 1e6
     rays = stats1["rays"] - stats0["rays"]
-    nearest = stats1["nearest"] - stats0["nearest"]
     mrays = rays / total / 1e6
 
     # Per-kernel timing on the tracer's stream (HIP events) for one more step.
@@ -151,7 +217,10 @@ def main():
                    "triangles": info["triangles"], "seed": args.seed},
         "mrays_per_s": round(mrays, 3),
         "rays_per_path": round(rays / (paths_per_step * args.steps), 4),
-        "stage_ms_per_step": {k: round(timing[k], 2) for k in ("extend_ms", "shade_ms", "shadow_ms", "total_ms")},
+        "launch": tuned,
+        "stage_ms_per_step": {k: round(timing[k], 2) for k in ("extend_ms", "shade_ms", "shadow_ms", "regen_ms",
+                                                                "total_ms")},
+        "iterations_per_step": timing["iterations"],
     }
 
     if not args.no_cpu_baseline:
@@ -181,12 +250,15 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "k_extend (closest hit)",
+            "traffic": TRAFFIC_BYTES_PER_LAUNCH,
+            "kernel": "k_extend (closest hit), launch shape %d" % tuned["shape"],
             "bytes_per_ray": round(b_ray, 1),
+            "closest_hit_rays_per_launch": round(nearest_prof / max(timing["extend_launches"], 1), 1),
             "extend_avg_ms": round(extend_avg_ms, 4),
             "extend_launches_per_step": timing["extend_launches"],
         }
+    if not args.no_queue:
+        result["mixed_queue"] = mixed_queue(native, args.seed)
     print(json.dumps(result), flush=True)
 
 

@@ -77,6 +77,10 @@ uint32_t rpx_scene_stats(uint32_t handle, uint64_t* out4);
  * depend on any of them. */
 uint32_t rpx_set_options(uint32_t handle, uint32_t pool, uint32_t shape, uint32_t group, uint32_t chunk,
                          uint32_t flags);
+/* The launch tuner: renders samples [0, samples) of the image (default 4; the film is untouched)
+ * with every traversal launch shape at pool sizes 2^20, 2^21, 2^22, keeps the fastest as the
+ * handle's options, and reports: pool, shape, best ms, then the 15 candidates' ms (shape-major). */
+uint32_t rpx_autotune(uint32_t handle, uint32_t samples, float* out18);
 /* Records per-stage GPU timing of the following renders (events on the handle's stream). */
 uint32_t rpx_set_profiling(uint32_t handle, uint32_t on);
 /* Timing of the last render: total ms, then ms of the closest-hit, shade, shadow and regen
@@ -91,15 +95,21 @@ uint32_t rpx_energy_tables(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_
  * kind 0: the reference's example scene (example.rs:113-230); 1: icosphere on a floor
  * (256x256); 2: eight displaced level-7 icospheres (1024x1024). */
 uint64_t rpx_write_scene(uint32_t kind, uint32_t seed, uint8_t* buf, uint64_t cap);
+/* Job `job` of the seeded mixed render queue (this library's batched workload), as RSB v1 bytes
+ * (same buffer protocol): a square image of 256..1024 px, 10k..500k triangles, 16/32/64 spp. */
+uint64_t rpx_write_queue_job(uint32_t seed, uint32_t job, uint8_t* buf, uint64_t cap);
 /* Render queues: jobs (scene handle, samples [first, first + count) of its whole image) rendered
  * together by one persistent path pool, each into its own resident film. Films equal those of
  * rpx_render_samples on the same scene and samples. */
 uint32_t rpx_queue_create(void); /* 0 on error */
 uint32_t rpx_queue_add(uint32_t queue, uint32_t scene, uint32_t first, uint32_t count, uint32_t* job);
 uint32_t rpx_queue_run(uint32_t queue); /* renders every job once more (films accumulate) */
+uint32_t rpx_queue_reset(uint32_t queue); /* clears every job's film */
 uint32_t rpx_queue_film_read(uint32_t queue, uint32_t job, float* rgb, float* weight, float* alpha, float* lum2);
 uint32_t rpx_queue_set_options(uint32_t queue, uint32_t pool, uint32_t shape, uint32_t group, uint32_t chunk,
                                uint32_t flags);
+/* The tuner over the queue's jobs' first samples (up to `paths` path samples; 0: 2^23). */
+uint32_t rpx_queue_autotune(uint32_t queue, uint32_t paths, float* out18);
 uint32_t rpx_queue_set_profiling(uint32_t queue, uint32_t on);
 uint32_t rpx_queue_last_timing(uint32_t queue, float* out12);
 uint32_t rpx_queue_stats(uint32_t queue, uint64_t* out4);

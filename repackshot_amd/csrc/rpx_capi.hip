@@ -527,6 +527,36 @@ uint32_t rpx_set_options(uint32_t handle, uint32_t pool, uint32_t shape, uint32_
   return RPX_OK;
 }
 
+static uint32_t tune_report(Engine& e, hipError_t r, const std::vector<float>& ms, const std::string& err, float* out) {
+  if (r != hipSuccess) {
+    set_error(err);
+    return r == hipErrorOutOfMemory ? RPX_GPU_MEMORY : RPX_GPU_LAUNCH;
+  }
+  out[0] = (float)e.opt.pool;
+  out[1] = (float)e.opt.shape;
+  out[2] = 0.0f;
+  for (size_t k = 0; k < ms.size(); ++k) {
+    out[3 + k] = ms[k];
+    if (out[2] == 0.0f || (ms[k] > 0.0f && ms[k] < out[2])) out[2] = ms[k];
+  }
+  return RPX_OK;
+}
+
+uint32_t rpx_autotune(uint32_t handle, uint32_t samples, float* out18) {
+  if (!out18) return RPX_NULL_POINTER;
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  hipSetDevice(h->device);
+  u32 npix = h->dev.width * h->dev.height;
+  u32 count = std::max<u32>(1, std::min<u32>(samples ? samples : 4, RESULTS_MAX / std::max<u32>(npix, 1)));
+  DevSegment s = whole_image(h.get(), 0, 0, count);
+  std::vector<float> ms;
+  std::string err;
+  hipError_t r = h->engine->autotune(h->d_scene, std::vector<DevSegment>(1, s), ms, err);
+  return tune_report(*h->engine, r, ms, err, out18);
+}
+
 uint32_t rpx_set_profiling(uint32_t handle, uint32_t on) {
   auto h = find(handle);
   if (!h) return RPX_BAD_HANDLE;
@@ -685,6 +715,21 @@ uint32_t rpx_queue_run(uint32_t queue) {
   return RPX_OK;
 }
 
+uint32_t rpx_queue_reset(uint32_t queue) {
+  auto q = find_queue(queue);
+  if (!q) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(q->mutex);
+  hipSetDevice(q->device);
+  for (const Job& j : q->jobs) {
+    size_t px = (size_t)j.width * j.height;
+    hipMemsetAsync(j.film.rgb, 0, px * 12, q->stream);
+    hipMemsetAsync(j.film.weight, 0, px * 4, q->stream);
+    hipMemsetAsync(j.film.alpha, 0, px * 4, q->stream);
+    hipMemsetAsync(j.film.lum2, 0, px * 4, q->stream);
+  }
+  return hipStreamSynchronize(q->stream) == hipSuccess ? RPX_OK : RPX_GPU_LAUNCH;
+}
+
 uint32_t rpx_queue_film_read(uint32_t queue, uint32_t job, float* rgb, float* weight, float* alpha, float* lum2) {
   auto q = find_queue(queue);
   if (!q) return RPX_BAD_HANDLE;
@@ -702,6 +747,55 @@ uint32_t rpx_queue_set_options(uint32_t queue, uint32_t pool, uint32_t shape, ui
   std::lock_guard<std::mutex> lock(q->mutex);
   apply_options(*q->engine, pool, shape, group, chunk, flags);
   return RPX_OK;
+}
+
+uint32_t rpx_queue_autotune(uint32_t queue, uint32_t paths, float* out18) {
+  if (!out18) return RPX_NULL_POINTER;
+  auto q = find_queue(queue);
+  if (!q) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(q->mutex);
+  hipSetDevice(q->device);
+  if (q->jobs.empty()) return RPX_OK;
+  std::vector<Handle*> scenes;
+  for (const Job& j : q->jobs) scenes.push_back(j.scene.get());
+  std::sort(scenes.begin(), scenes.end());
+  scenes.erase(std::unique(scenes.begin(), scenes.end()), scenes.end());
+  std::vector<std::unique_lock<std::mutex>> locks;
+  for (Handle* h : scenes) locks.emplace_back(h->mutex);
+  std::vector<DevScene> table;
+  std::map<Handle*, u32> index;
+  for (Handle* h : scenes) {
+    hipStreamSynchronize(h->stream);
+    index[h] = (u32)table.size();
+    table.push_back(h->dev);
+  }
+  // The probe: the jobs' first samples, in queue order, up to `paths` path samples.
+  u32 budget = std::min<u32>(paths ? paths : (1u << 23), RESULTS_MAX);
+  std::vector<DevSegment> probe;
+  u32 used = 0;
+  for (const Job& j : q->jobs) {
+    u32 npix = j.width * j.height;
+    if (j.count == 0 || used + npix > budget) continue;
+    DevSegment s{};
+    s.scene = index[j.scene.get()];
+    s.w = j.width;
+    s.h = j.height;
+    s.first = j.first;
+    s.count = std::min(j.count, (budget - used) / npix);
+    s.film = j.film;
+    s.film_w = j.width;
+    used += s.count * npix;
+    probe.push_back(s);
+  }
+  if (probe.empty()) return RPX_OK;
+  void* d = nullptr;
+  if (hipMalloc(&d, table.size() * sizeof(DevScene)) != hipSuccess) return RPX_GPU_MEMORY;
+  hipMemcpy(d, table.data(), table.size() * sizeof(DevScene), hipMemcpyHostToDevice);
+  std::vector<float> ms;
+  std::string err;
+  hipError_t r = q->engine->autotune((const DevScene*)d, probe, ms, err);
+  hipFree(d);
+  return tune_report(*q->engine, r, ms, err, out18);
 }
 
 uint32_t rpx_queue_set_profiling(uint32_t queue, uint32_t on) {

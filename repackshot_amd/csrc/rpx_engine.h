@@ -51,6 +51,8 @@ struct Timing {
 
 // Largest number of work ids (path samples) in one chunk: the results buffer holds one float4 each.
 constexpr u32 RESULTS_MAX = 1u << 27;
+constexpr int N_TUNE_POOLS = 3;
+constexpr u32 TUNE_POOLS[N_TUNE_POOLS] = {1u << 20, 1u << 21, 1u << 22};
 
 class Engine {
  public:
@@ -62,6 +64,11 @@ class Engine {
   // Renders every segment. With `splat`, each segment's samples are added to its film in sample
   // order; without, the results of a single chunk stay in `P.results` (work id order).
   hipError_t run(const DevScene* d_scenes, const std::vector<DevSegment>& segs, bool splat, std::string& err);
+  // The tuner: renders `probe` (whose results are discarded; one chunk) with every launch shape at
+  // each of TUNE_POOLS pool sizes, keeps the fastest in `opt`, and returns every candidate's GPU
+  // time (ms; shape-major).
+  hipError_t autotune(const DevScene* d_scenes, const std::vector<DevSegment>& probe, std::vector<float>& ms,
+                      std::string& err);
   // Closest-hit rays, shadow rays counted without a trace, shadow rays traced (since init).
   bool ray_counts(u64 out[3]);
 

@@ -326,4 +326,38 @@ hipError_t Engine::run(const DevScene* d_scenes, const std::vector<DevSegment>& 
   return hipSuccess;
 }
 
+hipError_t Engine::autotune(const DevScene* d_scenes, const std::vector<DevSegment>& probe, std::vector<float>& ms,
+                            std::string& err) {
+  EngineOptions keep = opt;
+  bool keep_profiling = profiling;
+  profiling = false;
+  opt.per_pass = false;
+  opt.graph = false;
+  ms.assign(N_SHAPES * N_TUNE_POOLS, 0.0f);
+  float best = 0.0f;
+  EngineOptions chosen = keep;
+  hipError_t e = hipSuccess;
+  for (int sh = 0; sh < N_SHAPES && e == hipSuccess; ++sh)
+    for (int p = 0; p < N_TUNE_POOLS && e == hipSuccess; ++p) {
+      opt.shape = sh;
+      opt.pool = TUNE_POOLS[p];
+      e = run(d_scenes, probe, false, err);  // warm (allocations, caches)
+      if (e == hipSuccess) e = run(d_scenes, probe, false, err);
+      if (e != hipSuccess) break;
+      float t = timing.total_ms;
+      ms[sh * N_TUNE_POOLS + p] = t;
+      if (best == 0.0f || t < best) {
+        best = t;
+        chosen.shape = sh;
+        chosen.pool = TUNE_POOLS[p];
+      }
+    }
+  profiling = keep_profiling;
+  opt = keep;
+  if (e != hipSuccess) return e;
+  opt.shape = chosen.shape;
+  opt.pool = chosen.pool;
+  return hipSuccess;
+}
+
 }  // namespace rpx

@@ -14,8 +14,14 @@
 //           conductors (roughness 0.05..0.6), two smooth glass spheres (IOR 1.5), two Lambert
 //           (albedo 0.2..0.9); three rect lights of seeded size 0.5..2 m and the studio dome;
 //           1024 x 1024, 64 spp, depth 12
+//   queue   job j of a seeded render queue (rpx_write_queue_job): image 256, 512, 768 or 1024 px
+//           square; 10k..500k triangles (log-uniform) in four value-noise displaced spheres
+//           (latitude / longitude grids) on a floor; materials drawn as in kind 2; one of 12
+//           cameras on a circle round the set; 1..4 rect lights and the studio dome; 16, 32 or
+//           64 spp; depth 12
 // All arithmetic is deterministic f32 (the sampler's fmath functions for angles), so a seed gives
 // the same bytes on every host.
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <unordered_map>
@@ -465,8 +471,128 @@ std::vector<uint8_t> config_large(u32 seed) {
   write_rndr(w, 1024, 1024, 64, seed, 12);
   return w.b;
 }
+
+// The parameters of job `job` of the render queue with seed `seed`: every job draws from one
+// PCG32 stream in job order.
+struct QueueJob {
+  u32 res, tris, spp, camera, lights;
+  u64 geo_seed;
+};
+QueueJob queue_job(u32 seed, u32 job) {
+  Pcg32 rng = Pcg32::seeded((u64)seed, 3);
+  QueueJob q{};
+  for (u32 j = 0; j <= job; ++j) {
+    q.res = 256u * (1u + std::min<u32>(3u, (u32)(rng.next_f32() * 4.0f)));
+    q.tris = (u32)(10000.0f * fpow(50.0f, rng.next_f32()));
+    q.spp = 16u << std::min<u32>(2u, (u32)(rng.next_f32() * 3.0f));
+    q.camera = std::min<u32>(11u, (u32)(rng.next_f32() * 12.0f));
+    q.lights = 1u + std::min<u32>(3u, (u32)(rng.next_f32() * 4.0f));
+    q.geo_seed = mix64(((u64)seed << 32) | (u64)j);
+  }
+  return q;
+}
+
+// A sphere of about `tris` triangles as a latitude / longitude grid (lat rows, 2 lat columns),
+// its radius displaced by value noise of amplitude 0.05.
+MeshData displaced_grid_sphere(V3 centre, float radius, u32 tris, u64 seed, u32 material) {
+  u32 lat = std::max<u32>(2u, (u32)sqrt_((float)tris / 4.0f));
+  u32 lon = 2 * lat;
+  MeshData m;
+  for (u32 i = 0; i <= lat; ++i) {
+    float st, ct;
+    sin_cos(PI * (float)i / (float)lat, st, ct);
+    for (u32 j = 0; j < lon; ++j) {
+      float sp, cp;
+      sin_cos(TAU * (float)j / (float)lon, sp, cp);
+      V3 d = v3(st * cp, st * sp, ct);
+      float h = value_noise(d * 4.0f, seed);
+      m.pos.push_back(centre + d * (radius * (1.0f + 0.05f * h)));
+    }
+  }
+  for (u32 i = 0; i < lat; ++i)
+    for (u32 j = 0; j < lon; ++j) {
+      u32 a = i * lon + j, b = i * lon + (j + 1) % lon, c = a + lon, d = b + lon;
+      u32 q[6] = {a, c, b, b, c, d};
+      m.idx.insert(m.idx.end(), q, q + 6);
+    }
+  m.nrm = vertex_normals(m.pos, m.idx);
+  m.material = material;
+  return m;
+}
+
+std::vector<uint8_t> config_queue_job(u32 seed, u32 job) {
+  QueueJob q = queue_job(seed, job);
+  Writer w;
+  w.b = {'R', 'S', 'B', '1', 0, 0, 0, 0};
+  Pcg32 rng = Pcg32::seeded(q.geo_seed, 4);
+  Cam cam;
+  float s, c;
+  sin_cos(TAU * (float)q.camera / 12.0f, s, c);
+  cam.position = v3(6000.0f * c, 6000.0f * s, 1500.0f + 300.0f * (float)(q.camera % 3));
+  cam.target = v3(0, 0, 400);
+  cam.focal = focal_for_fov(40.0f);
+  write_camr(w, cam);
+  // Materials 0..3 drawn as the large render's (conductor, glass, Lambert in 4 : 2 : 2), 4 the floor.
+  for (int k = 0; k < 4; ++k) {
+    Mat m;
+    float pick = rng.next_f32();
+    if (pick < 0.5f) {
+      float a = rng.next_f32(), b = rng.next_f32(), cc = rng.next_f32();
+      m.base = c3(0.55f + 0.45f * a, 0.55f + 0.45f * b, 0.55f + 0.45f * cc);
+      m.metallic = 1;
+      m.roughness = 0.05f + 0.55f * rng.next_f32();
+    } else if (pick < 0.75f) {
+      m.base = c3(1, 1, 1);
+      m.transmission = 1;
+      m.roughness = 0;
+      m.ior = 1.5f;
+    } else {
+      float a = 0.2f + 0.7f * rng.next_f32();
+      m.base = c3(a, a, a);
+      m.ior = 1.0f;
+      m.roughness = 1.0f;
+    }
+    write_matl(w, m);
+  }
+  Mat floor_m;
+  floor_m.base = c3(0.8f, 0.8f, 0.8f);
+  floor_m.ior = 1.0f;
+  floor_m.roughness = 1.0f;
+  write_matl(w, floor_m);
+  for (u32 k = 0; k < 4; ++k) {
+    float radius = 300.0f + 400.0f * rng.next_f32();
+    float cx = (-2000.0f + radius) + (4000.0f - 2.0f * radius) * rng.next_f32();
+    float cy = (-2000.0f + radius) + (4000.0f - 2.0f * radius) * rng.next_f32();
+    write_mesh(w, displaced_grid_sphere(v3(cx, cy, radius * 1.05f), radius, q.tris / 4, mix64(q.geo_seed ^ k), k));
+  }
+  write_mesh(w, floor_quad(0, 6000, 4));
+  for (u32 k = 0; k < q.lights; ++k) {
+    Lgt l;
+    float sx = 500.0f + 1500.0f * rng.next_f32(), sy = 500.0f + 1500.0f * rng.next_f32();
+    float ls, lc;
+    sin_cos(TAU * ((float)k / (float)q.lights + 0.2f * rng.next_f32()), ls, lc);
+    float dist = 3000.0f + 1500.0f * rng.next_f32();
+    l.position = v3(dist * lc, dist * ls, 2500.0f + 1500.0f * rng.next_f32());
+    l.direction = normalize(v3(0, 0, 300) - l.position);
+    l.sx = sx;
+    l.sy = sy;
+    l.color = kelvin_rgb(4500.0f + 3000.0f * rng.next_f32());
+    l.power = (8.0f + 8.0f * rng.next_f32()) * PI * (sx * sy * 1.0e-6f);
+    write_lght(w, l);
+  }
+  write_envr(w, 0, 0.1f, 0.3f, 0.6f);
+  write_flor(w);
+  write_rndr(w, q.res, q.res, q.spp, job, 12);
+  return w.b;
+}
 }  // namespace
 }  // namespace rpx
+
+static uint64_t hand_out(std::vector<uint8_t>& bytes, uint8_t* buf, uint64_t cap) {
+  if (!buf || cap < bytes.size()) return cap == 0 ? (uint64_t)bytes.size() : 0;
+  std::memcpy(buf, bytes.data(), bytes.size());
+  return bytes.size();
+}
 
 extern "C" uint64_t rpx_write_scene(uint32_t kind, uint32_t seed, uint8_t* buf, uint64_t cap) {
   // The last scene is kept: callers size the buffer with cap 0 and then fetch it.
@@ -484,7 +610,18 @@ extern "C" uint64_t rpx_write_scene(uint32_t kind, uint32_t seed, uint8_t* buf, 
     last_kind = kind;
     last_seed = seed;
   }
-  if (!buf || cap < bytes.size()) return cap == 0 ? (uint64_t)bytes.size() : 0;
-  std::memcpy(buf, bytes.data(), bytes.size());
-  return bytes.size();
+  return hand_out(bytes, buf, cap);
+}
+
+extern "C" uint64_t rpx_write_queue_job(uint32_t seed, uint32_t job, uint8_t* buf, uint64_t cap) {
+  static std::mutex mutex;
+  static uint32_t last_seed = 0, last_job = 0xffffffffu;
+  static std::vector<uint8_t> bytes;
+  std::lock_guard<std::mutex> lock(mutex);
+  if (seed != last_seed || job != last_job) {
+    bytes = rpx::config_queue_job(seed, job);
+    last_seed = seed;
+    last_job = job;
+  }
+  return hand_out(bytes, buf, cap);
 }

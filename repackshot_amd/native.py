@@ -47,15 +47,19 @@ _SIGS = {
     "rpx_scene_info": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_scene_stats": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_set_options": (ctypes.c_uint32, [ctypes.c_uint32] * 6),
+    "rpx_autotune": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_set_profiling": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
     "rpx_last_timing": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_energy_tables": (ctypes.c_uint32, [ctypes.c_void_p] * 6),
     "rpx_write_scene": (ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64]),
+    "rpx_write_queue_job": (ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64]),
     "rpx_queue_create": (ctypes.c_uint32, []),
     "rpx_queue_add": (ctypes.c_uint32, [ctypes.c_uint32] * 4 + [ctypes.c_void_p]),
     "rpx_queue_run": (ctypes.c_uint32, [ctypes.c_uint32]),
+    "rpx_queue_reset": (ctypes.c_uint32, [ctypes.c_uint32]),
     "rpx_queue_film_read": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32] + [ctypes.c_void_p] * 4),
     "rpx_queue_set_options": (ctypes.c_uint32, [ctypes.c_uint32] * 6),
+    "rpx_queue_autotune": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_queue_set_profiling": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
     "rpx_queue_last_timing": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_queue_stats": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
@@ -130,6 +134,17 @@ def write_scene(kind, seed=0):
     got = lib.rpx_write_scene(kind, seed, _ptr(buf), n)
     if got != n:
         raise RuntimeError("scene generator returned %d bytes, expected %d" % (got, n))
+    return buf.tobytes()
+
+
+def write_queue_job(seed, job):
+    """RSB v1 bytes of job `job` of the seeded mixed render queue."""
+    lib = library()
+    n = lib.rpx_write_queue_job(seed, job, None, 0)
+    buf = np.zeros(n, np.uint8)
+    got = lib.rpx_write_queue_job(seed, job, _ptr(buf), n)
+    if got != n:
+        raise RuntimeError("queue generator returned %d bytes, expected %d" % (got, n))
     return buf.tobytes()
 
 
@@ -252,6 +267,12 @@ class Scene:
         flags = (FLAG_PER_PASS if per_pass else 0) | (FLAG_GRAPH if graph else 0)
         _check(library().rpx_set_options(self.handle, pool, shape, group, chunk, flags), "rpx_set_options")
 
+    def autotune(self, samples=4):
+        """Runs the launch tuner; returns the chosen pool, shape and every candidate's ms."""
+        out = np.zeros(18, np.float32)
+        _check(library().rpx_autotune(self.handle, samples, _ptr(out)), "rpx_autotune")
+        return _tune(out)
+
     def set_profiling(self, on=True):
         _check(library().rpx_set_profiling(self.handle, 1 if on else 0), "rpx_set_profiling")
 
@@ -260,6 +281,11 @@ class Scene:
         out = np.zeros(12, np.float32)
         _check(library().rpx_last_timing(self.handle, _ptr(out)), "rpx_last_timing")
         return _timing(out)
+
+
+def _tune(out):
+    return {"pool": int(out[0]), "shape": int(out[1]), "best_ms": float(out[2]),
+            "ms": [[round(float(v), 3) for v in out[3 + 3 * s:6 + 3 * s]] for s in range(5)]}
 
 
 def _timing(out):
@@ -289,6 +315,10 @@ class Queue:
     def run(self):
         _check(library().rpx_queue_run(self.handle), "rpx_queue_run")
 
+    def reset(self):
+        """Clears every job's film."""
+        _check(library().rpx_queue_reset(self.handle), "rpx_queue_reset")
+
     def film(self, job):
         W, H = self.jobs[job]
         rgb = np.zeros((H, W, 3), np.float32)
@@ -303,6 +333,11 @@ class Queue:
         flags = (FLAG_PER_PASS if per_pass else 0) | (FLAG_GRAPH if graph else 0)
         _check(library().rpx_queue_set_options(self.handle, pool, shape, group, chunk, flags),
                "rpx_queue_set_options")
+
+    def autotune(self, paths=0):
+        out = np.zeros(18, np.float32)
+        _check(library().rpx_queue_autotune(self.handle, paths, _ptr(out)), "rpx_queue_autotune")
+        return _tune(out)
 
     def set_profiling(self, on=True):
         _check(library().rpx_queue_set_profiling(self.handle, 1 if on else 0), "rpx_queue_set_profiling")
