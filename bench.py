@@ -31,9 +31,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X datasheet: 8 TB/s HBM3E
 # geometry/triangle.rs:160-167: v [3][3][4] f32 + id [4] u32 = 160 B).
 NODE_BYTES = 256
 PACKET_BYTES = 160
-# HBM bytes per k_extend launch measured by rocprofv3 counters (FETCH_SIZE + WRITE_SIZE), when a
-# measurement for the current kernels is recorded; None otherwise.
-TRAFFIC_BYTES_PER_LAUNCH = None
+# Memory-side bytes of k_extend per closest-hit ray, from rocprofv3 FETCH_SIZE and WRITE_SIZE runs
+# of this workload (pool 4,194,304, launch shape 0; FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0,
+# the factors a 256 MiB device copy gives): 1.668e9 bytes per dispatch over 2,278,155 rays.
+TRAFFIC_BYTES_PER_RAY = 732.3
 
 
 def log(*a):
@@ -250,7 +251,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": TRAFFIC_BYTES_PER_LAUNCH,
+            "traffic": round(TRAFFIC_BYTES_PER_RAY * nearest_prof / max(timing["extend_launches"], 1)),
             "kernel": "k_extend (closest hit), launch shape %d" % tuned["shape"],
             "bytes_per_ray": round(b_ray, 1),
             "closest_hit_rays_per_launch": round(nearest_prof / max(timing["extend_launches"], 1), 1),
