@@ -10,7 +10,10 @@ same workload), the parity of the GPU film against the oracle on that sample's t
 mixed render queue (64 seeded jobs of 256..1024 px, 10k..500k triangles, 16..64 spp) rendered by
 one persistent pool beside the same jobs rendered one at a time in sample passes.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-queue]
+The adjoint record times the path-replay gradients of the inverse-material fit's scene against
+central finite differences of the same forward renderer.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-queue] [--no-adjoint]
                        [--pool P --shape S]   (fixed launch options instead of the tuner)
 """
 import argparse
@@ -128,6 +131,83 @@ def mixed_queue(native, seed=0, jobs=64):
             "films_bit_exact_vs_plain": identical}
 
 
+FIT_PARAMS = [(m, 0) for m in range(6)] + [(m, 1) for m in range(8)] + [(4, 4), (5, 4)]
+
+
+def adjoint_bench(native, seed=0, ranges=8, step=1.0e-2):
+    """The inverse-material fit's workload (512x512, 32 spp, depth 6, 200,000 triangles, 8 materials x
+    (roughness, rgb albedo, IOR) + a 256x256 rgb floor texture = 196,648 parameters): gradients per
+    second of the path-replay adjoint against central finite differences of the same GPU forward
+    renderer, and their agreement at 16 parameters over independent sample ranges (a fixed linear
+    loss, so both estimate the same derivative)."""
+    s = native.Scene(native.write_scene(4, seed))
+    info = s.info()
+    spp = info["max_samples"]
+    rng = np.random.default_rng(3)
+    tex = rng.uniform(0.2, 0.8, size=(256, 256, 3)).astype(np.float32)
+    origin, scale = (-4000.0, -4000.0), (1.0 / 8000.0, 1.0 / 8000.0)
+    s.set_texture(8, tex, origin, scale)
+    params = s.material_params()
+    n_params = params.size + tex.size
+    weights = np.random.default_rng(5).uniform(-1.0, 1.0, size=(s.height, s.width, 3))
+    w32 = weights.astype(np.float32)
+
+    def image(first):
+        s.reset_film()
+        s.render_samples(first, spp)
+        rgb, wt, _, _ = s.film()
+        return rgb.astype(np.float64) * (1.0 / wt.astype(np.float64))[..., None]
+
+    s.adjoint(0, spp, w32)  # warm
+    image(0)
+    t0 = time.perf_counter()
+    s.adjoint(0, spp, w32)
+    t_adj = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    s.reset_film()
+    s.render_samples(0, spp)
+    t_fwd = time.perf_counter() - t0
+    A, F = [], []
+    for k in range(ranges):
+        first = 1000 * k
+        gm, _ = s.adjoint(first, spp, w32)
+        A.append([gm[m, j] for m, j in FIT_PARAMS])
+        row = []
+        for m, j in FIT_PARAMS:
+            up, dn = params.copy(), params.copy()
+            up[m, j] += step
+            dn[m, j] -= step
+            s.set_material_params(up)
+            lu = float(np.sum(weights * image(first)))
+            s.set_material_params(dn)
+            ld = float(np.sum(weights * image(first)))
+            row.append((lu - ld) / (2 * step))
+        s.set_material_params(params)
+        F.append(row)
+    A, F = np.array(A), np.array(F)
+    am, fm = A.mean(0), F.mean(0)
+    sa, sf = A.std(0, ddof=1) / np.sqrt(ranges), F.std(0, ddof=1) / np.sqrt(ranges)
+    sigma = np.hypot(sa, sf)
+    resolved = sigma <= 0.01 * np.abs(fm)
+    rel = np.abs(am - fm) / np.abs(fm)
+    s.close()
+    adj_gps = n_params / t_adj
+    fd_gps = 1.0 / (2.0 * t_fwd)
+    return {"workload": "inverse-material fit: 512x512, 32 spp, depth 6, 199,714 tris, 8 materials x 5 + "
+                        "256x256x3 texels", "parameters": int(n_params),
+            "adjoint_s": round(t_adj, 4), "forward_s": round(t_fwd, 4),
+            "adjoint_grads_per_s": round(adj_gps, 1), "fd_grads_per_s": round(fd_gps, 2),
+            "grads_per_s_speedup": round(adj_gps / fd_gps, 1),
+            "time_for_16_gradients": {"adjoint_s": round(t_adj, 4), "fd_s": round(32 * t_fwd, 4),
+                                       "speedup": round(32 * t_fwd / t_adj, 2)},
+            "fd_check": {"params": len(FIT_PARAMS), "ranges": ranges, "fd_step": step,
+                         "aggregate_rel_err": round(float(np.linalg.norm(am - fm) / np.linalg.norm(fm)), 4),
+                         "max_abs_diff_over_sigma": round(float(np.max(np.abs(am - fm) / sigma)), 2),
+                         "resolved_params": int(resolved.sum()),
+                         "max_rel_err_resolved": round(float(np.max(rel[resolved])) if resolved.any() else -1.0, 4),
+                         "adjoint_stderr_over_fd_stderr": round(float(np.median(sa / sf)), 3)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +216,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-queue", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--pool", type=int, default=0)
     ap.add_argument("--shape", type=int, default=-1)
     args = ap.parse_args()
@@ -260,6 +341,8 @@ def main():
         }
     if not args.no_queue:
         result["mixed_queue"] = mixed_queue(native, args.seed)
+    if not args.no_adjoint:
+        result["adjoint"] = adjoint_bench(native, args.seed)
     print(json.dumps(result), flush=True)
 
 

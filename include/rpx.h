@@ -93,11 +93,27 @@ uint32_t rpx_energy_tables(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_
 /* Scene generators (this library's workloads), written as RSB v1 bytes into buf (cap bytes);
  * returns the length, or 0 when cap is too small (call with cap 0 to size the buffer).
  * kind 0: the reference's example scene (example.rs:113-230); 1: icosphere on a floor
- * (256x256); 2: eight displaced level-7 icospheres (1024x1024). */
+ * (256x256); 2: eight displaced level-7 icospheres (1024x1024); 4: the inverse-material fit's scene
+ * (200,000 triangles, 512x512, floor material 8 takes the base-colour texture). */
 uint64_t rpx_write_scene(uint32_t kind, uint32_t seed, uint8_t* buf, uint64_t cap);
 /* Job `job` of the seeded mixed render queue (this library's batched workload), as RSB v1 bytes
  * (same buffer protocol): a square image of 256..1024 px, 10k..500k triangles, 16/32/64 spp. */
 uint64_t rpx_write_queue_job(uint32_t seed, uint32_t job, uint8_t* buf, uint64_t cap);
+/* Differentiable rendering (no counterpart in the reference). Five parameters per material, in
+ * this order: GGX roughness, base colour r, g, b, IOR. rpx_material_params returns the number of
+ * materials and writes their parameters (up to cap floats); rpx_set_material_params replaces all
+ * of them (n = number of materials). rpx_set_texture gives one material an RGB f32 base-colour
+ * texture (w x h, row-major, bilinear, repeating), mapped planar in world x, y:
+ * u = (x - origin_x) * scale_x, v = (y - origin_y) * scale_y; material 0xffffffff removes it.
+ * rpx_adjoint: gradients of a loss of the image (the mean of samples [first, first + count))
+ * given d loss / d image (3*W*H floats), by path replay backpropagation: d loss / d parameter
+ * (5 per material) and, with a texture, d loss / d texel (3*w*h floats; may be null). */
+uint32_t rpx_material_params(uint32_t handle, float* out, uint32_t cap);
+uint32_t rpx_set_material_params(uint32_t handle, const float* params, uint32_t n);
+uint32_t rpx_set_texture(uint32_t handle, uint32_t material, uint32_t w, uint32_t h, const float* rgb, float origin_x,
+                         float origin_y, float scale_x, float scale_y);
+uint32_t rpx_adjoint(uint32_t handle, uint32_t first, uint32_t count, const float* d_image, float* grad_materials,
+                     float* grad_texture);
 /* Render queues: jobs (scene handle, samples [first, first + count) of its whole image) rendered
  * together by one persistent path pool, each into its own resident film. Films equal those of
  * rpx_render_samples on the same scene and samples. */

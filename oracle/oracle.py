@@ -56,6 +56,11 @@ def lib():
         L.orc_owen.argtypes = [u32, u32]
         L.orc_sampler_get1.restype = f32
         L.orc_sampler_get1.argtypes = [u32, u32, u32, u32, u32]
+        L.orc_material_params.restype = u32
+        L.orc_material_params.argtypes = [vp, vp, u32]
+        L.orc_set_material_params.argtypes = [vp, vp, u32]
+        L.orc_set_texture.argtypes = [vp, u32, u32, u32, vp, f32, f32, f32, f32]
+        L.orc_adjoint.argtypes = [vp, u32, u32, ctypes.c_int, vp, vp, vp]
         os.makedirs(os.path.dirname(CACHE), exist_ok=True)
         L.orc_set_energy_cache(CACHE.encode())
         _lib = L
@@ -121,6 +126,40 @@ class OracleScene:
         lib().orc_render_tiles(self.h, _p(r), r.shape[0], first, count, threads, *(_p(a) for a in film), _p(stats))
         keys = ["rays", "nearest", "any", "nearest_nodes", "nearest_packets", "any_nodes", "any_packets"]
         return film, {k: int(v) for k, v in zip(keys, stats)}
+
+    def material_params(self):
+        n = lib().orc_material_params(self.h, None, 0)
+        out = np.zeros(5 * n, np.float32)
+        lib().orc_material_params(self.h, _p(out), out.size)
+        return out.reshape(n, 5)
+
+    def set_material_params(self, params):
+        params = np.ascontiguousarray(params, np.float32).reshape(-1, 5)
+        if lib().orc_set_material_params(self.h, _p(params), params.shape[0]) != 0:
+            raise ValueError("one parameter set per material expected")
+
+    def set_texture(self, material, rgb, origin=(0.0, 0.0), scale=(1.0, 1.0)):
+        self.texture = None
+        if material is None:
+            lib().orc_set_texture(self.h, 0xFFFFFFFF, 0, 0, None, 0, 0, 1, 1)
+            return
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        h, w = rgb.shape[:2]
+        if lib().orc_set_texture(self.h, material, w, h, _p(rgb), origin[0], origin[1], scale[0], scale[1]) != 0:
+            raise ValueError("bad texture")
+        self.texture = (w, h)
+
+    def adjoint(self, first, count, d_image, threads=None):
+        """d loss / d parameter (materials x 5) and d loss / d texel (h, w, 3), in float64."""
+        if threads is None:
+            threads = os.cpu_count() or 1
+        d_image = np.ascontiguousarray(d_image, np.float32)
+        n = lib().orc_material_params(self.h, None, 0)
+        gm = np.zeros(5 * n, np.float64)
+        tex = getattr(self, "texture", None)
+        gt = np.zeros((tex[1], tex[0], 3) if tex else (0,), np.float64)
+        lib().orc_adjoint(self.h, first, count, threads, _p(d_image), _p(gm), _p(gt) if tex else None)
+        return gm.reshape(n, 5), gt
 
     def trace_rays(self, rays, brute=False):
         rays = np.ascontiguousarray(rays, np.float32)

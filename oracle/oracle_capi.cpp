@@ -243,4 +243,105 @@ float orc_sampler_get1(uint32_t seed, uint32_t x, uint32_t y, uint32_t index, ui
   return PixelSampler::make(seed, x, y, index).get1(dim);
 }
 
+// ---------------------------------------------------------------- differentiable-render extension
+// Parameters per material: roughness, base r, g, b, IOR. Returns the number of materials.
+uint32_t orc_material_params(void* p, float* out, uint32_t cap) {
+  const Scene& s = static_cast<Handle*>(p)->scene;
+  for (size_t m = 0; m < s.materials.size() && out && (m + 1) * 5 <= cap; ++m) {
+    const Material& mm = s.materials[m];
+    float* o = out + 5 * m;
+    o[0] = mm.roughness;
+    o[1] = mm.base_color.r;
+    o[2] = mm.base_color.g;
+    o[3] = mm.base_color.b;
+    o[4] = mm.ior;
+  }
+  return (uint32_t)s.materials.size();
+}
+
+int orc_set_material_params(void* p, const float* params, uint32_t n) {
+  Scene& s = static_cast<Handle*>(p)->scene;
+  if (n != s.materials.size()) return 1;
+  for (uint32_t m = 0; m < n; ++m) {
+    Material& mm = s.materials[m];
+    mm.roughness = params[5 * m];
+    mm.base_color = Rgb(params[5 * m + 1], params[5 * m + 2], params[5 * m + 3]);
+    mm.ior = params[5 * m + 4];
+  }
+  // Media absorb by the base colour (integrator/mod.rs:300-311).
+  std::string err;
+  return static_cast<Handle*>(p)->prep.build(s, err) ? 0 : 2;
+}
+
+int orc_set_texture(void* p, uint32_t material, uint32_t w, uint32_t h, const float* rgb, float ox, float oy, float sx,
+                    float sy) {
+  Scene& s = static_cast<Handle*>(p)->scene;
+  s.texture = Texture{};
+  if (material == 0xffffffffu) return 0;
+  if (material >= s.materials.size() || w == 0 || h == 0) return 1;
+  s.texture.material = material;
+  s.texture.w = w;
+  s.texture.h = h;
+  s.texture.ox = ox;
+  s.texture.oy = oy;
+  s.texture.sx = sx;
+  s.texture.sy = sy;
+  s.texture.rgb.assign(rgb, rgb + (size_t)w * h * 3);
+  return 0;
+}
+
+// Path replay backpropagation on the CPU (the GPU adjoint's reference): for samples
+// [first, first + count) of every pixel, the forward radiance, then the replay adding
+// d loss / d parameter (d_image: d loss / d image, 3*W*H; the image is the samples' mean) into
+// grad_mat (5 per material) and grad_tex (3 per texel), summed in double.
+int orc_adjoint(void* p, uint32_t first, uint32_t count, int threads, const float* d_image, double* grad_mat,
+                double* grad_tex) {
+  auto* hd = static_cast<Handle*>(p);
+  const Scene& s = hd->scene;
+  u32 W = s.render.width, H = s.render.height;
+  size_t nm = s.materials.size() * 5, nt = (size_t)s.texture.w * s.texture.h * 3;
+  int n = std::max(threads, 1);
+  std::vector<std::vector<double>> gm(n, std::vector<double>(nm, 0.0)), gt(n, std::vector<double>(nt, 0.0));
+  std::atomic<u32> next_row{0};
+  float scale = 1.0f / (float)count;
+  auto worker = [&](int tid) {
+    for (;;) {
+      u32 y = next_row.fetch_add(1);
+      if (y >= H) return;
+      for (u32 x = 0; x < W; ++x) {
+        const float* d = d_image + 3 * ((size_t)y * W + x);
+        for (u32 k = 0; k < count; ++k) {
+          PixelSampler sampler = PixelSampler::make(s.render.seed, x, y, first + k);
+          Ray ray;
+          camera_sample(s, sampler, x, y, ray);
+          Rgb total;
+          float a;
+          radiance(s, hd->prep, sampler, ray, total, a, nullptr);
+          AdjointPath ap;
+          ap.adj = Rgb(d[0] * scale, d[1] * scale, d[2] * scale);
+          ap.total = total;
+          ap.grad_mat = gm[tid].data();
+          ap.grad_tex = gt[tid].empty() ? nullptr : gt[tid].data();
+          Rgb again;
+          radiance(s, hd->prep, sampler, ray, again, a, nullptr, &ap);
+        }
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 0; t < n; ++t) pool.emplace_back(worker, t);
+  for (auto& t : pool) t.join();
+  for (size_t i = 0; i < nm; ++i) {
+    double acc = 0.0;
+    for (int t = 0; t < n; ++t) acc += gm[t][i];
+    grad_mat[i] = acc;
+  }
+  for (size_t i = 0; i < nt && grad_tex; ++i) {
+    double acc = 0.0;
+    for (int t = 0; t < n; ++t) acc += gt[t][i];
+    grad_tex[i] = acc;
+  }
+  return 0;
+}
+
 }  // extern "C"

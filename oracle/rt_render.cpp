@@ -683,8 +683,95 @@ static Ray shadow_ray(Vec3 origin, Vec3 dir, float distance) {
 }
 
 // ------------------------------------------------------------------ integrator/mod.rs:1140-1891
+void Texture::tap(float x, float y, u32 tx[4], u32 ty[4], float wt[4], float& ax, float& ay) const {
+  float u = (x - ox) * sx, v = (y - oy) * sy;
+  if (!std::isfinite(u)) u = 0.0f;
+  if (!std::isfinite(v)) v = 0.0f;
+  u = u - std::floor(u);
+  v = v - std::floor(v);
+  float fx = u * (float)w - 0.5f, fy = v * (float)h - 0.5f;
+  float x0f = std::floor(fx), y0f = std::floor(fy);
+  ax = fx - x0f;
+  ay = fy - y0f;
+  int ix = (int)x0f, iy = (int)y0f;
+  u32 x0 = (u32)(ix < 0 ? ix + (int)w : (ix >= (int)w ? ix - (int)w : ix));
+  u32 y0 = (u32)(iy < 0 ? iy + (int)h : (iy >= (int)h ? iy - (int)h : iy));
+  u32 x1 = x0 + 1 == w ? 0 : x0 + 1, y1 = y0 + 1 == h ? 0 : y0 + 1;
+  tx[0] = x0; ty[0] = y0;
+  tx[1] = x1; ty[1] = y0;
+  tx[2] = x0; ty[2] = y1;
+  tx[3] = x1; ty[3] = y1;
+  wt[0] = (1.0f - ax) * (1.0f - ay);
+  wt[1] = ax * (1.0f - ay);
+  wt[2] = (1.0f - ax) * ay;
+  wt[3] = ax * ay;
+}
+
+Rgb Texture::value(float x, float y) const {
+  u32 tx[4], ty[4];
+  float wt[4], ax, ay;
+  tap(x, y, tx, ty, wt, ax, ay);
+  float c[3];
+  for (int ch = 0; ch < 3; ++ch) {
+    float a = rgb[3 * ((size_t)ty[0] * w + tx[0]) + ch], b = rgb[3 * ((size_t)ty[1] * w + tx[1]) + ch];
+    float d = rgb[3 * ((size_t)ty[2] * w + tx[2]) + ch], e = rgb[3 * ((size_t)ty[3] * w + tx[3]) + ch];
+    c[ch] = ((a * wt[0] + b * wt[1]) + d * wt[2]) + e * wt[3];
+  }
+  return Rgb(c[0], c[1], c[2]);
+}
+
+namespace {
+// d f / d parameter at fixed directions by central differences of the BSDF rebuilt with each of
+// the five parameters moved by +-PARAM_STEP.
+void bsdf_param_derivs(const Material& m, bool entering, Vec3 wo, Vec3 wi, Lobes lobes, Rgb out[5]) {
+  for (int k = 0; k < 5; ++k) {
+    Rgb d(0.0f, 0.0f, 0.0f);
+    for (int side = 0; side < 2; ++side) {
+      float step = side == 0 ? PARAM_STEP : -PARAM_STEP;
+      Material p = m;
+      if (k == 0) p.roughness += step;
+      else if (k == 1) p.base_color.r += step;
+      else if (k == 2) p.base_color.g += step;
+      else if (k == 3) p.base_color.b += step;
+      else p.ior += step;
+      Rgb f = Bsdf::surface(p, entering).eval(wo, wi, lobes);
+      d = d + f * (side == 0 ? 1.0f : -1.0f);
+    }
+    out[k] = d * (0.5f / PARAM_STEP);
+  }
+}
+
+float dot3(Rgb a, Rgb b) { return a.r * b.r + a.g * b.g + a.b * b.b; }
+
+// Where a vertex's gradient goes: material `mat` (or none) and, for a textured one, its texels.
+struct GradTarget {
+  bool on = false;
+  u32 mat = 0;
+  bool tex = false;
+  u32 tx[4], ty[4];
+  float wt[4];
+};
+
+void commit(const Scene& scene, const AdjointPath& a, const GradTarget& t, const float g[5]) {
+  if (!t.on) return;
+  for (int k = 0; k < 5; ++k) {
+    if (t.tex && k >= 1 && k <= 3) continue;
+    a.grad_mat[5 * (size_t)t.mat + k] += (double)g[k];
+  }
+  if (t.tex) {
+    const Texture& tx = scene.texture;
+    for (int q = 0; q < 4; ++q)
+      for (int c = 0; c < 3; ++c)
+        a.grad_tex[3 * ((size_t)t.ty[q] * tx.w + t.tx[q]) + c] += (double)(g[1 + c] * t.wt[q]);
+  }
+}
+}  // namespace
+
 void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& sampler, Ray camera_ray, Rgb& out_sum,
-              float& out_alpha, RenderStats* stats) {
+              float& out_alpha, RenderStats* stats, const AdjointPath* adjoint) {
+  // The adjoint's pending term: d log f / d parameter of the last BSDF sample, and its target.
+  GradTarget pending_target;
+  Rgb pending_dl[5];
   const Lights& lights = prep.lights;
   const Geometry& geo = scene.geometry;
   Ray ray = camera_ray;
@@ -706,6 +793,14 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
     float t_geometry = has_hit ? hit.t : INF;
     float t_floor = prep.floor_t(ray, primary);
     float t_nearest = fmin_r(t_geometry, t_floor);
+    if (adjoint && pending_target.on) {
+      // The radiance gathered after the last vertex: the forward's total minus what is summed.
+      Rgb rest(adjoint->total.r - sum.r, adjoint->total.g - sum.g, adjoint->total.b - sum.b);
+      float g[5];
+      for (int k = 0; k < 5; ++k) g[k] = dot3(adjoint->adj, rest * pending_dl[k]);
+      commit(scene, *adjoint, pending_target, g);
+    }
+    pending_target = GradTarget{};
     if (in_medium && std::isfinite(t_nearest)) {
       Rgb bl(expf_(-sigma.r * t_nearest), expf_(-sigma.g * t_nearest), expf_(-sigma.b * t_nearest));
       throughput = throughput * bl;
@@ -771,7 +866,22 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
       break;
     }
     if (bounce >= prep.max_depth) break;
-    Bsdf bsdf = material ? Bsdf::surface(*material, s_outside) : prep.floor_bsdf;
+    // A textured material's base colour comes from its texture at the hit point.
+    u32 mat_id = s_has_instance ? geo.instances[s_instance].material_id : 0xffffffffu;
+    bool has_mat = s_has_instance && mat_id < scene.materials.size();
+    Material shaded = material ? *material : Material{};
+    GradTarget target;
+    if (has_mat) {
+      target.on = true;
+      target.mat = mat_id;
+      if (scene.texture.on() && mat_id == scene.texture.material) {
+        float ax, ay;
+        scene.texture.tap(s_hit_point.x, s_hit_point.y, target.tx, target.ty, target.wt, ax, ay);
+        shaded.base_color = scene.texture.value(s_hit_point.x, s_hit_point.y);
+        target.tex = true;
+      }
+    }
+    Bsdf bsdf = material ? Bsdf::surface(shaded, s_outside) : prep.floor_bsdf;
     Frame frame = Frame::from_normal(s_normal);
     Vec3 wo = frame.to_local(-ray.dir);
     Vec3 ng = s_geometric;
@@ -798,6 +908,17 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
         if (arrives) {
           float weight = (ls.delta || !ls.glossy) ? 1.0f : power_heuristic(ls.pdf, bsdf.pdf(wo, wi));
           Rgb c = throughput * f * ls.radiance * (std::fabs(wi.z) * weight / ls.pdf);
+          if (adjoint && has_mat && std::isfinite(c.r) && std::isfinite(c.g) && std::isfinite(c.b)) {
+            Rgb d[5];
+            bsdf_param_derivs(shaded, s_outside, wo, wi, ls.glossy ? Lobes::All : Lobes::Diffuse, d);
+            float pk = fmax_r(fmax_r(c.r, c.g), c.b);
+            float scale = (bounce >= 1 && prep.clamp_indirect > 0.0f && pk > prep.clamp_indirect)
+                              ? prep.clamp_indirect / pk : 1.0f;
+            float g[5];
+            for (int k = 0; k < 5; ++k)
+              g[k] = dot3(adjoint->adj, throughput * d[k] * ls.radiance * (std::fabs(wi.z) * weight / ls.pdf * scale));
+            commit(scene, *adjoint, target, g);
+          }
           prep.add(sum, c, bounce >= 1);
         }
       }
@@ -820,6 +941,15 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
       throughput = throughput / survive;
     }
     if (!(peak(throughput) > 0.0f && std::isfinite(peak(throughput)))) break;
+    if (adjoint && has_mat) {
+      Rgb f_s = bsdf.eval(wo, bs.wi, Lobes::All);
+      Rgb d[5];
+      bsdf_param_derivs(shaded, s_outside, wo, bs.wi, Lobes::All, d);
+      for (int k = 0; k < 5; ++k)
+        pending_dl[k] = Rgb(f_s.r > 0.0f ? d[k].r / f_s.r : 0.0f, f_s.g > 0.0f ? d[k].g / f_s.g : 0.0f,
+                            f_s.b > 0.0f ? d[k].b / f_s.b : 0.0f);
+      pending_target = target;
+    }
     Vec3 next_origin = transmitted ? offset_origin(s_hit_point, -ng) : origin;
     if (bs.straight) {
       offset += (next_origin - ray.origin).dot(ray.dir);

@@ -67,9 +67,23 @@ struct RenderSettings {
   u32 depth() const { return max_depth == 0 ? 8 : std::min<u32>(max_depth, 1024); }
 };
 
+// The differentiable-render extension (no reference counterpart): an RGB f32 base-colour texture
+// of one material, bilinear and repeating, mapped planar in world x, y (u = (x - ox) * sx,
+// v = (y - oy) * sy), texel centres at (i + 1/2) / w.
+struct Texture {
+  u32 material = 0xffffffffu, w = 0, h = 0;
+  float ox = 0, oy = 0, sx = 1, sy = 1;
+  std::vector<float> rgb;
+  bool on() const { return material != 0xffffffffu; }
+  // Texels (x0, y0), (x1, y0), (x0, y1), (x1, y1) of world point (x, y) and their weights.
+  void tap(float x, float y, u32 tx[4], u32 ty[4], float wt[4], float& ax, float& ay) const;
+  Rgb value(float x, float y) const;
+};
+
 struct Scene {
   Camera camera;
   std::vector<Material> materials;
+  Texture texture;
   Geometry geometry;
   std::vector<Light> lights;
   Environment environment;
@@ -168,9 +182,19 @@ struct RenderStats {
   TraversalStats nearest_trav, any_trav;
 };
 
-// One sample of one pixel: radiance and alpha. `stats` may be null.
+// Path replay backpropagation of one path (the differentiable-render extension): `adj` is d loss /
+// d radiance of the path, `total` its radiance from the forward render; d loss / d parameter is
+// added to grad_mat (5 per material: roughness, base r, g, b, IOR) and grad_tex (rgb per texel).
+struct AdjointPath {
+  Rgb adj, total;
+  double* grad_mat;
+  double* grad_tex;
+};
+constexpr float PARAM_STEP = 1.0e-3f;
+
+// One sample of one pixel: radiance and alpha. `stats` and `adjoint` may be null.
 void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& sampler, Ray camera_ray,
-              Rgb& out_sum, float& out_alpha, RenderStats* stats);
+              Rgb& out_sum, float& out_alpha, RenderStats* stats, const AdjointPath* adjoint = nullptr);
 void camera_sample(const Scene& scene, const PixelSampler& sampler, u32 x, u32 y, Ray& ray);
 
 struct TileRect { u32 x, y, w, h; };

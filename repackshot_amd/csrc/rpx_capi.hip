@@ -30,12 +30,15 @@ struct Handle {
   Film film{};
   std::vector<DevBuf> film_bufs;
   u64 samples = 0;
+  // base-colour texture of one material
+  std::vector<DevBuf> tex_bufs;
 
   ~Handle() {
     if (stream) hipStreamSynchronize(stream);
     engine.reset();
     dev_free_all(scene_bufs);
     dev_free_all(film_bufs);
+    dev_free_all(tex_bufs);
     if (stream) hipStreamDestroy(stream);
   }
 };
@@ -105,9 +108,16 @@ bool build_device_scene(Handle* h, std::string& err) {
   }
   h->dev = d;
   h->d_scene = const_cast<DevScene*>(ds);
+  dev_free_all(h->tex_bufs);
   h->triangles = flat.triangles;
   h->nodes = flat.nodes.size();
   return true;
+}
+
+// Copies the handle's scene constants to its one-entry device scene table.
+bool upload_scene_table(Handle* h) {
+  return hipMemcpyAsync(h->d_scene, &h->dev, sizeof(DevScene), hipMemcpyHostToDevice, h->stream) == hipSuccess &&
+         hipStreamSynchronize(h->stream) == hipSuccess;
 }
 
 bool ensure_film(Handle* h, std::string& err) {
@@ -351,6 +361,7 @@ static uint32_t render_tile_impl(uint32_t handle, uint32_t x, uint32_t y, uint32
     set_error("device memory exhausted for the tile");
     return RPX_GPU_MEMORY;
   }
+  h->engine->textured = h->dev.tex_material != NO_TEXTURE;
   hipError_t e = h->engine->run(h->d_scene, std::vector<DevSegment>(1, s), false, err);
   if (e == hipSuccess) e = launch_tile_out(h->engine->P, d_out, cw * ch, channels, h->stream);
   std::vector<float> tmp((size_t)cw * ch * channels);
@@ -386,6 +397,7 @@ uint32_t rpx_render_samples(uint32_t handle, uint32_t first, uint32_t count) {
     set_error(err);
     return RPX_GPU_MEMORY;
   }
+  h->engine->textured = h->dev.tex_material != NO_TEXTURE;
   hipError_t e = h->engine->run(h->d_scene, std::vector<DevSegment>(1, whole_image(h.get(), 0, first, count)), true, err);
   if (e != hipSuccess) {
     set_error(err);
@@ -553,6 +565,7 @@ uint32_t rpx_autotune(uint32_t handle, uint32_t samples, float* out18) {
   DevSegment s = whole_image(h.get(), 0, 0, count);
   std::vector<float> ms;
   std::string err;
+  h->engine->textured = h->dev.tex_material != NO_TEXTURE;
   hipError_t r = h->engine->autotune(h->d_scene, std::vector<DevSegment>(1, s), ms, err);
   return tune_report(*h->engine, r, ms, err, out18);
 }
@@ -591,6 +604,140 @@ uint32_t rpx_energy_tables(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_
   if (clear_out) std::memcpy(clear_out, t->h_clear_out.data(), 17408 * 2);
   if (albedo_avg) std::memcpy(albedo_avg, t->h_albedo_avg.data(), 32 * 4);
   if (schlick_avg) std::memcpy(schlick_avg, t->h_schlick_avg.data(), 32 * 4);
+  return RPX_OK;
+}
+
+// ------------------------------------------------------------------ differentiable parameters
+uint32_t rpx_material_params(uint32_t handle, float* out, uint32_t cap) {
+  auto h = find(handle);
+  if (!h) return 0;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  u32 n = (u32)h->host.materials.size();
+  for (u32 m = 0; m < n && out && (m + 1) * N_MAT_PARAMS <= cap; ++m) {
+    const HostMaterial& hm = h->host.materials[m];
+    float* o = out + N_MAT_PARAMS * m;
+    o[0] = hm.roughness;
+    o[1] = hm.base_color.r;
+    o[2] = hm.base_color.g;
+    o[3] = hm.base_color.b;
+    o[4] = hm.ior;
+  }
+  return n;
+}
+
+uint32_t rpx_set_material_params(uint32_t handle, const float* params, uint32_t n) {
+  if (!params) return RPX_NULL_POINTER;
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  hipSetDevice(h->device);
+  if (n != h->host.materials.size()) {
+    set_error("rpx_set_material_params: expected one parameter set per material");
+    return RPX_BAD_SCENE;
+  }
+  DeviceTables* tables;
+  std::string err;
+  if (!device_tables(h->device, tables, err)) {
+    set_error(err);
+    return RPX_GPU_LAUNCH;
+  }
+  std::vector<Bsdf> bsdfs;
+  std::vector<DevMaterial> mats;
+  for (u32 m = 0; m < n; ++m) {
+    HostMaterial& hm = h->host.materials[m];
+    const float* p = params + N_MAT_PARAMS * m;
+    hm.roughness = p[0];
+    hm.base_color = c3(p[1], p[2], p[3]);
+    hm.ior = p[4];
+    Bsdf sides[2];
+    material_sides(tables->host_view(), hm, sides);
+    bsdfs.push_back(sides[0]);
+    bsdfs.push_back(sides[1]);
+    mats.push_back(dev_material(hm));
+  }
+  hipStreamSynchronize(h->stream);
+  hipError_t e = hipMemcpy(const_cast<Bsdf*>(h->dev.bsdfs), bsdfs.data(), bsdfs.size() * sizeof(Bsdf),
+                           hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(const_cast<DevMaterial*>(h->dev.materials), mats.data(), mats.size() * sizeof(DevMaterial),
+                  hipMemcpyHostToDevice);
+  return e == hipSuccess ? RPX_OK : RPX_GPU_LAUNCH;
+}
+
+uint32_t rpx_set_texture(uint32_t handle, uint32_t material, uint32_t w, uint32_t hh, const float* rgb, float origin_x,
+                         float origin_y, float scale_x, float scale_y) {
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  hipSetDevice(h->device);
+  hipStreamSynchronize(h->stream);
+  dev_free_all(h->tex_bufs);
+  h->dev.texture = nullptr;
+  h->dev.tex_material = NO_TEXTURE;
+  h->dev.tex_w = h->dev.tex_h = 0;
+  if (material != NO_TEXTURE) {
+    if (!rgb) return RPX_NULL_POINTER;
+    if (material >= h->host.materials.size() || w == 0 || hh == 0 || w > 65535 || hh > 65535) {
+      set_error("rpx_set_texture: no such material, or an empty or oversized texture");
+      return RPX_BAD_SCENE;
+    }
+    void* p = nullptr;
+    size_t bytes = (size_t)w * hh * 3 * sizeof(float);
+    if (!dev_alloc(h->tex_bufs, bytes, &p)) return RPX_GPU_MEMORY;
+    if (hipMemcpy(p, rgb, bytes, hipMemcpyHostToDevice) != hipSuccess) return RPX_GPU_LAUNCH;
+    h->dev.texture = (const float*)p;
+    h->dev.tex_material = material;
+    h->dev.tex_w = w;
+    h->dev.tex_h = hh;
+    h->dev.tex_origin_x = origin_x;
+    h->dev.tex_origin_y = origin_y;
+    h->dev.tex_scale_x = scale_x;
+    h->dev.tex_scale_y = scale_y;
+  }
+  return upload_scene_table(h.get()) ? RPX_OK : RPX_GPU_LAUNCH;
+}
+
+uint32_t rpx_adjoint(uint32_t handle, uint32_t first, uint32_t count, const float* d_image, float* grad_materials,
+                     float* grad_texture) {
+  if (!d_image || !grad_materials) return RPX_NULL_POINTER;
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  if (count == 0) return RPX_OK;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  hipSetDevice(h->device);
+  size_t px = (size_t)h->dev.width * h->dev.height;
+  size_t n_mat = h->host.materials.size() * N_MAT_PARAMS;
+  size_t n_tex = (size_t)h->dev.tex_w * h->dev.tex_h * 3;
+  std::vector<DevBuf> tmp;
+  void *d_adj = nullptr, *d_gm = nullptr, *d_gt = nullptr;
+  if (!dev_alloc(tmp, px * 12, &d_adj) || !dev_alloc(tmp, std::max<size_t>(n_mat, 1) * 4, &d_gm) ||
+      !dev_alloc(tmp, std::max<size_t>(n_tex, 1) * 4, &d_gt)) {
+    dev_free_all(tmp);
+    set_error("device memory exhausted for the adjoint");
+    return RPX_GPU_MEMORY;
+  }
+  hipError_t e = hipMemcpyAsync(d_adj, d_image, px * 12, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(d_gm, 0, std::max<size_t>(n_mat, 1) * 4, h->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(d_gt, 0, std::max<size_t>(n_tex, 1) * 4, h->stream);
+  std::string err;
+  if (e == hipSuccess) {
+    DevSegment s = whole_image(h.get(), 0, first, count);
+    s.adjoint = (const float*)d_adj;
+    // The image is the mean of the samples: d image / d sample = f32(1 / count) (image.rs:158-169).
+    s.adjoint_scale = 1.0f / (float)count;
+    h->engine->textured = h->dev.tex_material != NO_TEXTURE;
+    e = h->engine->adjoint(h->d_scene, std::vector<DevSegment>(1, s), (float*)d_gm, (float*)d_gt, h->dev.tex_w,
+                           h->dev.tex_h, err);
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(grad_materials, d_gm, n_mat * 4, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess && grad_texture && n_tex)
+    e = hipMemcpyAsync(grad_texture, d_gt, n_tex * 4, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  dev_free_all(tmp);
+  if (e != hipSuccess) {
+    set_error(err.empty() ? std::string("adjoint: ") + hipGetErrorString(e) : err);
+    return e == hipErrorOutOfMemory ? RPX_GPU_MEMORY : RPX_GPU_LAUNCH;
+  }
   return RPX_OK;
 }
 
@@ -707,6 +854,8 @@ uint32_t rpx_queue_run(uint32_t queue) {
   if (hipMemcpyAsync(q->scenes.p, table.data(), bytes, hipMemcpyHostToDevice, q->stream) != hipSuccess)
     return RPX_GPU_LAUNCH;
   std::string err;
+  q->engine->textured = false;
+  for (const DevScene& d : table) q->engine->textured = q->engine->textured || d.tex_material != NO_TEXTURE;
   hipError_t e = q->engine->run((const DevScene*)q->scenes.p, segs, true, err);
   if (e != hipSuccess) {
     set_error(err);
@@ -793,6 +942,8 @@ uint32_t rpx_queue_autotune(uint32_t queue, uint32_t paths, float* out18) {
   hipMemcpy(d, table.data(), table.size() * sizeof(DevScene), hipMemcpyHostToDevice);
   std::vector<float> ms;
   std::string err;
+  q->engine->textured = false;
+  for (const DevScene& ds : table) q->engine->textured = q->engine->textured || ds.tex_material != NO_TEXTURE;
   hipError_t r = q->engine->autotune((const DevScene*)d, probe, ms, err);
   hipFree(d);
   return tune_report(*q->engine, r, ms, err, out18);

@@ -64,6 +64,12 @@ class Engine {
   // Renders every segment. With `splat`, each segment's samples are added to its film in sample
   // order; without, the results of a single chunk stay in `P.results` (work id order).
   hipError_t run(const DevScene* d_scenes, const std::vector<DevSegment>& segs, bool splat, std::string& err);
+  // Path replay backpropagation: for each chunk, the forward pass (radiance per path sample into
+  // the results buffer), then the replay that accumulates d loss / d parameter into
+  // grad_mat (n_mat x 5) and grad_tex (tex_w x tex_h x 3), device buffers the caller zeroes. The
+  // segments carry the adjoint image and its scale.
+  hipError_t adjoint(const DevScene* d_scenes, const std::vector<DevSegment>& segs, float* grad_mat, float* grad_tex,
+                     u32 tex_w, u32 tex_h, std::string& err);
   // The tuner: renders `probe` (whose results are discarded; one chunk) with every launch shape at
   // each of TUNE_POOLS pool sizes, keeps the fastest in `opt`, and returns every candidate's GPU
   // time (ms; shape-major).
@@ -74,20 +80,25 @@ class Engine {
 
   EngineOptions opt;
   bool profiling = false;
+  bool textured = false;  // a scene of the next runs has a texture (slots get a BSDF record)
   Timing timing;
   PathState P{};
 
  private:
   bool ensure_pool(u32 n, std::string& err);
+  bool ensure_adjoint(u32 n, std::string& err);
+  bool plan(const std::vector<DevSegment>& segs, std::vector<std::vector<DevSegment>>& chunks, std::vector<u32>& totals,
+            std::string& err);
   bool ensure_results(u32 n, std::string& err);
   bool ensure_segs(u32 n, std::string& err);
-  hipError_t run_chunk(const DevScene* d_scenes, const std::vector<DevSegment>& chunk, u32 total, bool splat);
+  hipError_t run_chunk(const DevScene* d_scenes, const std::vector<DevSegment>& chunk, u32 total, bool splat,
+                       bool adj = false);
   hipEvent_t next_event();
 
   int device_ = 0;
   hipStream_t stream_ = nullptr;
-  u32 capacity_ = 0, results_cap_ = 0, segs_cap_ = 0;
-  std::vector<DevBuf> pool_bufs_, fixed_bufs_;
+  u32 capacity_ = 0, results_cap_ = 0, segs_cap_ = 0, adjoint_cap_ = 0, bsdf_cap_ = 0;
+  std::vector<DevBuf> pool_bufs_, fixed_bufs_, adjoint_bufs_, bsdf_bufs_;
   DevBuf results_, segs_;
   u32* idle_host_ = nullptr;  // pinned
   std::vector<hipEvent_t> events_;

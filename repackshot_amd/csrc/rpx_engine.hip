@@ -79,6 +79,8 @@ bool device_tables(int dev, DeviceTables*& out, std::string& err) {
 Engine::~Engine() {
   if (stream_) hipStreamSynchronize(stream_);
   dev_free_all(pool_bufs_);
+  dev_free_all(adjoint_bufs_);
+  dev_free_all(bsdf_bufs_);
   dev_free_all(fixed_bufs_);
   if (results_.p) hipFree(results_.p);
   if (segs_.p) hipFree(segs_.p);
@@ -109,6 +111,16 @@ bool Engine::init(int device, hipStream_t stream, std::string& err) {
 }
 
 bool Engine::ensure_pool(u32 n, std::string& err) {
+  if (textured && n > bsdf_cap_) {
+    hipStreamSynchronize(stream_);
+    dev_free_all(bsdf_bufs_);
+    bsdf_cap_ = 0;
+    if (!dev_alloc(bsdf_bufs_, (size_t)n * sizeof(Bsdf), (void**)&P.bsdf_tmp)) {
+      err = "device memory exhausted for the textured BSDF records";
+      return false;
+    }
+    bsdf_cap_ = n;
+  }
   if (n <= capacity_) return true;
   hipStreamSynchronize(stream_);
   dev_free_all(pool_bufs_);
@@ -177,7 +189,8 @@ bool Engine::ray_counts(u64 out[3]) {
          hipStreamSynchronize(stream_) == hipSuccess;
 }
 
-hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegment>& chunk, u32 total, bool splat) {
+hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegment>& chunk, u32 total, bool splat,
+                             bool adj) {
   hipError_t e = hipMemcpyAsync(segs_.p, chunk.data(), chunk.size() * sizeof(DevSegment), hipMemcpyHostToDevice, stream_);
   if (e != hipSuccess) return e;
   Launch L;
@@ -204,7 +217,7 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegm
     for (u32 k = 0; k < count && r == hipSuccess; ++k) {
       u32 parity = k & 1u;
       mark(3);
-      r = launch_regen(P, L, parity);
+      r = launch_regen(P, L, parity, adj);
       done();
       if (r != hipSuccess) break;
       mark(0);
@@ -212,17 +225,17 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegm
       done();
       if (r != hipSuccess) break;
       mark(1);
-      r = launch_shade(P, L, parity);
+      r = launch_shade(P, L, parity, adj);
       done();
       if (r != hipSuccess) break;
       mark(2);
-      r = launch_shadow(P, L);
+      r = launch_shadow(P, L, adj);
       done();
     }
     return r;
   };
   hipGraphExec_t exec = nullptr;
-  if (opt.graph) {
+  if (opt.graph && !adj) {
     hipGraph_t graph = nullptr;
     e = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
@@ -234,7 +247,7 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegm
     if (e != hipSuccess) return e;
   }
   for (;;) {
-    e = opt.graph ? hipGraphLaunch(exec, stream_) : iterations(group);
+    e = exec ? hipGraphLaunch(exec, stream_) : iterations(group);
     if (e != hipSuccess) break;
     timing.iterations += group;
     e = hipMemcpyAsync(idle_host_, P.counts + C_IDLE, 4, hipMemcpyDeviceToHost, stream_);
@@ -252,14 +265,12 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegm
   return hipSuccess;
 }
 
-hipError_t Engine::run(const DevScene* d_scenes, const std::vector<DevSegment>& segs, bool splat, std::string& err) {
-  timing = Timing{};
-  events_used_ = 0;
-  marks_.clear();
+bool Engine::plan(const std::vector<DevSegment>& segs, std::vector<std::vector<DevSegment>>& chunks,
+                  std::vector<u32>& totals, std::string& err) {
   // Chunks of whole sample passes, at most `limit` work ids each.
   const u32 limit = std::min(std::max<u32>(opt.chunk, 1), RESULTS_MAX);
-  std::vector<std::vector<DevSegment>> chunks;
-  std::vector<u32> totals;
+  chunks.clear();
+  totals.clear();
   std::vector<DevSegment> cur;
   u32 base = 0;
   auto flush = [&]() {
@@ -274,7 +285,7 @@ hipError_t Engine::run(const DevScene* d_scenes, const std::vector<DevSegment>& 
     u32 npix = s.w * s.h;
     if (npix == 0 || npix > limit) {
       err = "a segment's pixel rectangle is empty or larger than the results buffer";
-      return hipErrorInvalidValue;
+      return false;
     }
     u32 k0 = 0;
     while (k0 < s.count) {
@@ -296,6 +307,16 @@ hipError_t Engine::run(const DevScene* d_scenes, const std::vector<DevSegment>& 
     }
   }
   flush();
+  return true;
+}
+
+hipError_t Engine::run(const DevScene* d_scenes, const std::vector<DevSegment>& segs, bool splat, std::string& err) {
+  timing = Timing{};
+  events_used_ = 0;
+  marks_.clear();
+  std::vector<std::vector<DevSegment>> chunks;
+  std::vector<u32> totals;
+  if (!plan(segs, chunks, totals, err)) return hipErrorInvalidValue;
   if (!splat && chunks.size() > 1) {
     err = "unsplatted results must fit one chunk";
     return hipErrorInvalidValue;
@@ -314,6 +335,78 @@ hipError_t Engine::run(const DevScene* d_scenes, const std::vector<DevSegment>& 
   if (e == hipSuccess) e = hipStreamSynchronize(stream_);
   if (e != hipSuccess) {
     err = std::string("render: ") + hipGetErrorString(e);
+    return e;
+  }
+  hipEventElapsedTime(&timing.total_ms, ev_[0], ev_[1]);
+  for (auto& m : marks_) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, events_[m.second], events_[m.second + 1]);
+    timing.stage_ms[m.first] += ms;
+    timing.launches[m.first]++;
+  }
+  return hipSuccess;
+}
+
+bool Engine::ensure_adjoint(u32 n, std::string& err) {
+  if (n <= adjoint_cap_) return true;
+  hipStreamSynchronize(stream_);
+  dev_free_all(adjoint_bufs_);
+  adjoint_cap_ = 0;
+  auto& B = adjoint_bufs_;
+  size_t f4 = (size_t)n * sizeof(float4);
+  bool ok = dev_alloc(B, f4, (void**)&P.adj) && dev_alloc(B, f4, (void**)&P.ltot) &&
+            dev_alloc(B, f4 * N_MAT_PARAMS, (void**)&P.dl) && dev_alloc(B, f4, (void**)&P.tgt) &&
+            dev_alloc(B, 2 * f4, (void**)&P.gn);
+  if (!ok) {
+    dev_free_all(adjoint_bufs_);
+    err = "device memory exhausted for the adjoint's path state";
+    return false;
+  }
+  adjoint_cap_ = n;
+  return true;
+}
+
+hipError_t Engine::adjoint(const DevScene* d_scenes, const std::vector<DevSegment>& segs, float* grad_mat,
+                           float* grad_tex, u32 tex_w, u32 tex_h, std::string& err) {
+  timing = Timing{};
+  events_used_ = 0;
+  marks_.clear();
+  EngineOptions keep = opt;
+  opt.per_pass = false;
+  opt.graph = false;
+  std::vector<std::vector<DevSegment>> chunks;
+  std::vector<u32> totals;
+  bool planned = plan(segs, chunks, totals, err);
+  opt = keep;
+  if (!planned) return hipErrorInvalidValue;
+  u32 max_total = 0, max_segs = 0;
+  for (size_t c = 0; c < chunks.size(); ++c) {
+    max_total = std::max(max_total, totals[c]);
+    max_segs = std::max<u32>(max_segs, (u32)chunks[c].size());
+  }
+  u32 pool = std::min(opt.pool, max_total);
+  if (!ensure_pool(pool, err) || !ensure_adjoint(pool, err) || !ensure_results(max_total, err) ||
+      !ensure_segs(max_segs, err))
+    return hipErrorOutOfMemory;
+  P.grad_mat = grad_mat;
+  P.grad_tex = grad_tex;
+  P.tex_w = tex_w;
+  P.tex_h = tex_h;
+  bool keep_graph = opt.graph;
+  opt.graph = false;
+  hipEventRecord(ev_[0], stream_);
+  hipError_t e = hipSuccess;
+  for (size_t c = 0; c < chunks.size() && e == hipSuccess; ++c) {
+    e = run_chunk(d_scenes, chunks[c], totals[c], false, false);  // the forward: radiance per path sample
+    if (e == hipSuccess) e = run_chunk(d_scenes, chunks[c], totals[c], false, true);  // the replay
+  }
+  hipEventRecord(ev_[1], stream_);
+  opt.graph = keep_graph;
+  if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  P.grad_mat = nullptr;
+  P.grad_tex = nullptr;
+  if (e != hipSuccess) {
+    err = std::string("adjoint: ") + hipGetErrorString(e);
     return e;
   }
   hipEventElapsedTime(&timing.total_ms, ev_[0], ev_[1]);

@@ -14,6 +14,11 @@
 //           conductors (roughness 0.05..0.6), two smooth glass spheres (IOR 1.5), two Lambert
 //           (albedo 0.2..0.9); three rect lights of seeded size 0.5..2 m and the studio dome;
 //           1024 x 1024, 64 spp, depth 12
+//   kind 4  the inverse-material fit's scene: kind 2's layout reduced to 200,000 triangles (eight
+//           value-noise displaced latitude / longitude spheres of 25,000), every GGX material rough
+//           (four conductors, two rough glass of IOR 1.5, two Lambert), a Lambert floor quad
+//           (material 8, the one a base-colour texture goes on), three rect lights and the studio
+//           dome; 512 x 512, 32 spp, depth 6
 //   queue   job j of a seeded render queue (rpx_write_queue_job): image 256, 512, 768 or 1024 px
 //           square; 10k..500k triangles (log-uniform) in four value-noise displaced spheres
 //           (latitude / longitude grids) on a floor; materials drawn as in kind 2; one of 12
@@ -520,6 +525,68 @@ MeshData displaced_grid_sphere(V3 centre, float radius, u32 tris, u64 seed, u32 
   return m;
 }
 
+std::vector<uint8_t> config_fit(u32 seed) {
+  Writer w;
+  w.b = {'R', 'S', 'B', '1', 0, 0, 0, 0};
+  Pcg32 rng = Pcg32::seeded((u64)seed, 5);
+  Cam cam;
+  cam.position = v3(0, -7600, 3400);
+  cam.target = v3(0, 0, 500);
+  cam.focal = focal_for_fov(40.0f);
+  write_camr(w, cam);
+  // 0..3 rough conductors, 4..5 rough glass, 6..7 Lambert, 8 the floor.
+  for (int k = 0; k < 8; ++k) {
+    Mat m;
+    if (k < 4) {
+      float a = rng.next_f32(), b = rng.next_f32(), c = rng.next_f32();
+      m.base = c3(0.55f + 0.35f * a, 0.55f + 0.35f * b, 0.55f + 0.35f * c);
+      m.metallic = 1;
+      m.roughness = 0.05f + 0.55f * rng.next_f32();
+    } else if (k < 6) {
+      m.base = c3(0.9f, 0.92f, 0.95f);
+      m.transmission = 1;
+      m.roughness = 0.05f + 0.25f * rng.next_f32();
+      m.ior = 1.5f;
+    } else {
+      float a = 0.2f + 0.7f * rng.next_f32();
+      m.base = c3(a, a * 0.9f, a * 0.8f);
+      m.ior = 1.0f;
+      m.roughness = 1.0f;
+    }
+    write_matl(w, m);
+  }
+  Mat floor_m;
+  floor_m.base = c3(0.5f, 0.5f, 0.5f);
+  floor_m.ior = 1.0f;
+  floor_m.roughness = 1.0f;
+  write_matl(w, floor_m);
+  for (u32 k = 0; k < 8; ++k) {
+    float radius = 450.0f + 350.0f * rng.next_f32();
+    float cx = (-3000.0f + radius) + (6000.0f - 2.0f * radius) * rng.next_f32();
+    float cy = (-3000.0f + radius) + (6000.0f - 2.0f * radius) * rng.next_f32();
+    write_mesh(w, displaced_grid_sphere(v3(cx, cy, radius * 1.05f), radius, 25000, mix64(((u64)seed << 8) | k), k));
+  }
+  write_mesh(w, floor_quad(0, 4000, 8));
+  for (int k = 0; k < 3; ++k) {
+    Lgt l;
+    float sx = 500.0f + 1500.0f * rng.next_f32(), sy = 500.0f + 1500.0f * rng.next_f32();
+    float s, c;
+    sin_cos(TAU * ((float)k / 3.0f + 0.15f * rng.next_f32()), s, c);
+    float dist = 3500.0f + 1500.0f * rng.next_f32();
+    l.position = v3(dist * c, dist * s, 2500.0f + 1500.0f * rng.next_f32());
+    l.direction = normalize(v3(0, 0, 300) - l.position);
+    l.sx = sx;
+    l.sy = sy;
+    l.color = kelvin_rgb(4500.0f + 3000.0f * rng.next_f32());
+    l.power = (8.0f + 8.0f * rng.next_f32()) * PI * (sx * sy * 1.0e-6f);
+    write_lght(w, l);
+  }
+  write_envr(w, 0, 0.1f, 0.3f, 0.6f);
+  write_flor(w);
+  write_rndr(w, 512, 512, 32, seed, 6);
+  return w.b;
+}
+
 std::vector<uint8_t> config_queue_job(u32 seed, u32 job) {
   QueueJob q = queue_job(seed, job);
   Writer w;
@@ -605,6 +672,7 @@ extern "C" uint64_t rpx_write_scene(uint32_t kind, uint32_t seed, uint8_t* buf, 
       case 0: bytes = rpx::example_scene(); break;
       case 1: bytes = rpx::config_plain(seed); break;
       case 2: bytes = rpx::config_large(seed); break;
+      case 4: bytes = rpx::config_fit(seed); break;
       default: return 0;
     }
     last_kind = kind;

@@ -36,6 +36,9 @@ struct DevSegment {
   u32 base;
   Film film;
   u32 film_w;  // film row length in pixels
+  // Adjoint runs: d loss / d image (rgb per film pixel) and the image's d image / d sample.
+  const float* adjoint;
+  float adjoint_scale;
 };
 
 // Counter slots (u32) of PathState::counts.
@@ -64,7 +67,21 @@ struct PathState {
   u32* counts;     // see C_*
   u64* rays;       // [0] closest-hit rays traced, [1] shadow rays counted without a trace, [2] shadow rays traced
   float4* results; // per work id of the current chunk: radiance rgb, alpha
+  Bsdf* bsdf_tmp;  // per slot: the BSDF of a textured hit (allocated for scenes with a texture)
   const u32* sobol;  // 64 x 32 generator columns in logical-dimension order
+  // Adjoint (path replay) state per slot: d loss / d radiance of the path, the forward's radiance of
+  // the path, the pending d log f / d parameter (5 per slot) of the last BSDF sample, its target
+  // (material, texel x0 | y0 << 16, tap fractions), and the next-event gradient waiting for the
+  // shadow ray (5 scalars in 2 float4).
+  float4* adj;
+  float4* ltot;
+  float4* dl;
+  uint4* tgt;
+  float4* gn;
+  // Gradient sums: per material 5 parameters; per texel of the base-colour texture rgb.
+  float* grad_mat;
+  float* grad_tex;
+  u32 tex_w, tex_h;
 };
 
 // Launch shape of the traversal kernels (chosen per device by the tuner).
@@ -86,10 +103,11 @@ struct Launch {
 };
 
 hipError_t launch_pool_init(const PathState& P, const Launch& L, u32 work_end);
-hipError_t launch_regen(const PathState& P, const Launch& L, u32 parity);
+// `adj`: the adjoint (path replay) variants of the kernels.
+hipError_t launch_regen(const PathState& P, const Launch& L, u32 parity, bool adj = false);
 hipError_t launch_extend(const PathState& P, const Launch& L, u32 parity);
-hipError_t launch_shade(const PathState& P, const Launch& L, u32 parity);
-hipError_t launch_shadow(const PathState& P, const Launch& L);
+hipError_t launch_shade(const PathState& P, const Launch& L, u32 parity, bool adj = false);
+hipError_t launch_shadow(const PathState& P, const Launch& L, bool adj = false);
 // Adds the chunk's samples of segment `seg` (host copy `s`) to its film, in sample order.
 hipError_t launch_splat(const PathState& P, const DevSegment& s, hipStream_t st);
 // Copies the one-sample results of a w x h tile (work ids 0 ..) out as rgb or rgba floats.

@@ -404,6 +404,27 @@ __device__ __forceinline__ float sample_at(const u32* __restrict__ cols, u64 key
 }
 }  // namespace
 
+// ================================================================== adjoint records
+namespace {
+// The adjoint's gradient of one vertex: its target (a material's five parameters, and for a
+// textured material the texels of its base colour instead of parameters 1..3) and the scalar
+// d loss / d parameter contributions.
+struct VertexGrad {
+  u32 mat, tex;  // NO_TARGET: none; tex packs the tap's texel (x0 | y0 << 16)
+  float ax, ay;  // the tap's fractions
+  float g[N_MAT_PARAMS];
+};
+constexpr u32 NO_TARGET = 0xffffffffu;
+
+__device__ __forceinline__ void no_grad(VertexGrad& v) {
+  v.mat = NO_TARGET;
+  v.tex = NO_TARGET;
+  v.ax = v.ay = 0.0f;
+  for (int k = 0; k < N_MAT_PARAMS; ++k) v.g[k] = 0.0f;
+}
+
+}  // namespace
+
 // ================================================================== regen: results + camera (render.rs:79-97)
 namespace {
 __device__ __forceinline__ float filter_offset(u32 filter, float u) {
@@ -469,6 +490,9 @@ __global__ void __launch_bounds__(256) k_pool_init(PathState P, u32 pool, u32 wo
   }
 }
 
+// With ADJ, finished paths keep the forward's results and new paths load their adjoint and the
+// forward's radiance of the same work id.
+template <bool ADJ>
 __global__ void __launch_bounds__(256) k_regen(const DevScene* __restrict__ scenes, const DevSegment* __restrict__ segs,
                                                u32 n_segs, PathState P, u32 parity) {
   u32 n = P.counts[C_DONE];
@@ -486,7 +510,7 @@ __global__ void __launch_bounds__(256) k_regen(const DevScene* __restrict__ scen
     if (active) {
       slot = P.done_queue[j];
       u32 done_work = P.ident[slot].x;
-      if (done_work != NO_WORK) P.results[done_work] = P.sum[slot];
+      if (!ADJ && done_work != NO_WORK) P.results[done_work] = P.sum[slot];
     }
     u32 wid = wave_append(&P.counts[C_CURSOR], active);
     bool got = active && wid < end;
@@ -499,6 +523,13 @@ __global__ void __launch_bounds__(256) k_regen(const DevScene* __restrict__ scen
       px = sg.x0 + p % sg.w;
       py = sg.y0 + p / sg.w;
       index = sg.first + k;
+      if constexpr (ADJ) {
+        const float* a = sg.adjoint + 3 * ((size_t)py * sg.film_w + px);
+        float sc = sg.adjoint_scale;
+        P.adj[slot] = make_float4(a[0] * sc, a[1] * sc, a[2] * sc, 0.0f);
+        P.ltot[slot] = P.results[wid];
+        P.tgt[slot] = make_uint4(NO_TARGET, NO_TARGET, 0u, 0u);
+      }
     } else if (active) {
       P.ident[slot] = make_uint4(NO_WORK, 0u, 0u, 0u);
     }
@@ -557,21 +588,94 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
 }
 
 // ================================================================== shadow
-template <int BLK, int LDSN, int WPE>
+// The adjoint's gradient sums: texels per lane; material parameters summed over the lanes of a
+// wave that share a material, then one atomic per parameter (the whole wave must call this).
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+__device__ void commit_grad(const PathState& P, bool has, const VertexGrad& v) {
+  bool tex = has && v.tex != NO_TARGET;
+  if (tex) {
+    TexTap t;
+    t.x0 = v.tex & 0xffffu;
+    t.y0 = v.tex >> 16;
+    t.x1 = t.x0 + 1 == P.tex_w ? 0 : t.x0 + 1;
+    t.y1 = t.y0 + 1 == P.tex_h ? 0 : t.y0 + 1;
+    t.ax = v.ax;
+    t.ay = v.ay;
+    float w[4];
+    texture_weights(t, w);
+    u32 xs[4] = {t.x0, t.x1, t.x0, t.x1}, ys[4] = {t.y0, t.y0, t.y1, t.y1};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      float* dst = P.grad_tex + 3 * ((size_t)ys[q] * P.tex_w + xs[q]);
+      atomicAdd(dst, v.g[1] * w[q]);
+      atomicAdd(dst + 1, v.g[2] * w[q]);
+      atomicAdd(dst + 2, v.g[3] * w[q]);
+    }
+  }
+  bool todo = has && v.mat != NO_TARGET;
+  u64 pending = __ballot(todo);
+  while (pending) {
+    u32 cur = lowest_lane_value(pending, v.mat);
+    int leader = (int)__ffsll((long long)pending) - 1;
+    bool mine = todo && v.mat == cur;
+#pragma unroll
+    for (int k = 0; k < N_MAT_PARAMS; ++k) {
+      // A textured material's base colour is its texture: those terms went to the texels.
+      bool to_texels = tex && k >= 1 && k <= 3;
+      float x = wave_sum(mine && !to_texels ? v.g[k] : 0.0f);
+      if ((int)__lane_id() == leader) atomicAdd(P.grad_mat + (size_t)N_MAT_PARAMS * cur + k, x);
+    }
+    todo = todo && !mine;
+    pending = __ballot(todo);
+  }
+}
+
+template <int BLK, int LDSN, int WPE, bool ADJ>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) k_shadow(const DevScene* __restrict__ scenes, PathState P) {
   __shared__ u32 lds[LDSN * BLK];
   u32* col = lds + threadIdx.x;
   u32 n = P.counts[C_SHADOW];
   if (blockIdx.x == 0 && threadIdx.x == 0) P.rays[2] += n;
   u32 stride = gridDim.x * blockDim.x;
-  for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
-    u32 i = P.shadow_queue[j];
-    float4 o = P.sh_o[i], d = P.sh_d[i];
-    Geo g = geo_of(scenes, P.ident[i].y);
-    if (!trace_any<BLK, LDSN>(g, col, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), RAY_EPSILON, o.w)) {
-      float4 c = P.sh_c[i];
-      float4 s = P.sum[i];
-      P.sum[i] = make_float4(s.x + c.x, s.y + c.y, s.z + c.z, s.w);
+  // Whole waves iterate together (the adjoint's gradient sums are wave operations).
+  u32 n_iter = (n + stride - 1) / stride;
+  u32 gid = blockIdx.x * blockDim.x + threadIdx.x;
+  for (u32 it = 0; it < n_iter; ++it) {
+    u32 j = it * stride + gid;
+    bool visible = false;
+    u32 i = 0;
+    if (j < n) {
+      i = P.shadow_queue[j];
+      float4 o = P.sh_o[i], d = P.sh_d[i];
+      Geo g = geo_of(scenes, P.ident[i].y);
+      if (!trace_any<BLK, LDSN>(g, col, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), RAY_EPSILON, o.w)) {
+        visible = true;
+        float4 c = P.sh_c[i];
+        float4 s = P.sum[i];
+        P.sum[i] = make_float4(s.x + c.x, s.y + c.y, s.z + c.z, s.w);
+      }
+    }
+    if constexpr (ADJ) {
+      VertexGrad v;
+      no_grad(v);
+      if (visible) {
+        uint4 tg = P.tgt[i];
+        float4 a = P.gn[2 * (size_t)i], b = P.gn[2 * (size_t)i + 1];
+        v.mat = tg.x;
+        v.tex = tg.y;
+        v.ax = u2f(tg.z);
+        v.ay = u2f(tg.w);
+        v.g[0] = a.x;
+        v.g[1] = a.y;
+        v.g[2] = a.z;
+        v.g[3] = a.w;
+        v.g[4] = b.x;
+      }
+      commit_grad(P, visible, v);
     }
   }
 }
@@ -666,9 +770,48 @@ __device__ __forceinline__ void add_to(C3& sum, C3 c, bool indirect, float clamp
   else sum = sum + c;
 }
 
+// d f / d parameter of the BSDF at (wo, wi_a) (diffuse-only flag `diffuse_a`) and at (wo, wi_b), by
+// central differences of the BSDF rebuilt with each parameter moved by +-PARAM_STEP (the
+// directions stay fixed: the adjoint differentiates the integrand in path space, its sampling
+// densities held constant).
+constexpr float PARAM_STEP = 1.0e-3f;
+__device__ void bsdf_param_derivs(const EnergyTables& T, const DevMaterial& M, C3 base, bool entering, V3 wo,
+                                  bool want_a, V3 wi_a, bool diffuse_a, bool want_b, V3 wi_b, C3 da[N_MAT_PARAMS],
+                                  C3 db[N_MAT_PARAMS]) {
+#pragma unroll 1
+  for (int k = 0; k < N_MAT_PARAMS; ++k) {
+    C3 d_a = splat(0.0f), d_b = splat(0.0f);
+#pragma unroll 1
+    for (int side = 0; side < 2; ++side) {
+      float step = side == 0 ? PARAM_STEP : -PARAM_STEP;
+      float r = M.roughness, ior = M.ior;
+      C3 bc = base;
+      if (k == 0) r += step;
+      else if (k == 1) bc.r += step;
+      else if (k == 2) bc.g += step;
+      else if (k == 3) bc.b += step;
+      else ior += step;
+      Bsdf b = material_bsdf(T, M, bc, r, ior, entering);
+      float sgn = side == 0 ? 1.0f : -1.0f;
+      if (want_a) d_a = d_a + bsdf_eval(T, b, wo, wi_a, diffuse_a) * sgn;
+      if (want_b) d_b = d_b + bsdf_eval(T, b, wo, wi_b, false) * sgn;
+    }
+    da[k] = d_a * (0.5f / PARAM_STEP);
+    db[k] = d_b * (0.5f / PARAM_STEP);
+  }
+}
+
+__device__ __forceinline__ float dot3(C3 a, C3 b) { return a.r * b.r + a.g * b.g + a.b * b.b; }
+
 // One vertex of the path in slot i (integrator/mod.rs:1140-1891, one loop iteration): `cont` when
 // the path goes on with a new ray, `want_shadow` when a shadow ray was queued behind it.
-__device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P, u32 i, bool& cont, bool& want_shadow) {
+// With ADJ (path replay backpropagation), the same vertex is replayed and its gradient found:
+// `g_prev` the term of the previous vertex's BSDF sample (it needs the radiance the path gathered
+// after it, the forward's total minus what is summed so far), `g_now` the next-event term of this
+// vertex when its light casts no shadow; a shadowed next-event term waits for the shadow kernel.
+template <bool ADJ>
+__device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P, u32 i, bool& cont, bool& want_shadow,
+                                           VertexGrad& g_prev, VertexGrad& g_now) {
   const EnergyTables& T = S.tables;
   {
   float4 ro = P.ray_o[i], rd = P.ray_d[i];
@@ -689,6 +832,25 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
   const u32 bounce = flags >> BOUNCE_SHIFT;
   const u32 dims = bounce_dim(bounce);
   const u32 sample_index = P.ident[i].w;
+  C3 adj = splat(0.0f);
+  if constexpr (ADJ) {
+    float4 a = P.adj[i];
+    adj = c3(a.x, a.y, a.z);
+    uint4 tg = P.tgt[i];
+    if (tg.x != NO_TARGET || tg.y != NO_TARGET) {
+      float4 lt = P.ltot[i];
+      C3 rest = c3(lt.x - sum.r, lt.y - sum.g, lt.z - sum.b);
+      g_prev.mat = tg.x;
+      g_prev.tex = tg.y;
+      g_prev.ax = u2f(tg.z);
+      g_prev.ay = u2f(tg.w);
+      for (int k = 0; k < N_MAT_PARAMS; ++k) {
+        float4 dl = P.dl[(size_t)N_MAT_PARAMS * i + k];
+        g_prev.g[k] = dot3(adj, rest * c3(dl.x, dl.y, dl.z));
+      }
+    }
+    P.tgt[i] = make_uint4(NO_TARGET, NO_TARGET, 0u, 0u);
+  }
   // The selection of the vertex the ray left (only read once a BSDF sample made the ray).
   Sel from;
   if (has_pdf) {
@@ -862,9 +1024,36 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
   }
   if (!done && bounce >= S.max_depth) done = true;
   if (!done) {
-    const Bsdf& B = s_instance ? (mat < S.n_materials ? S.bsdfs[2 * mat + (s_outside ? 0 : 1)]
-                                                      : S.fallback_bsdf[s_outside ? 0 : 1])
-                               : S.floor_bsdf;
+    const bool has_mat = s_instance && mat < S.n_materials;
+    // A textured material's BSDF is built at the hit from the texture's base colour.
+    const bool textured = has_mat && mat == S.tex_material;
+    C3 base = has_mat ? S.materials[mat].base : splat(0.0f);
+    TexTap tap{};
+    if (textured) {
+      // Built into this slot's BSDF record, so that B is a global-memory reference either way.
+      tap = texture_tap(S.tex_w, S.tex_h, S.tex_origin_x, S.tex_origin_y, S.tex_scale_x, S.tex_scale_y, s_hit_point.x,
+                        s_hit_point.y);
+      base = texture_value(S.texture, S.tex_w, tap);
+      const DevMaterial& M = S.materials[mat];
+      P.bsdf_tmp[i] = material_bsdf(T, M, base, M.roughness, M.ior, s_outside);
+    }
+    const Bsdf& B = textured ? P.bsdf_tmp[i]
+                             : (s_instance ? (mat < S.n_materials ? S.bsdfs[2 * mat + (s_outside ? 0 : 1)]
+                                                                  : S.fallback_bsdf[s_outside ? 0 : 1])
+                                           : S.floor_bsdf);
+    VertexGrad target;
+    no_grad(target);
+    if constexpr (ADJ) {
+      if (has_mat) {
+        target.mat = mat;
+        if (textured) {
+          target.tex = tap.x0 | (tap.y0 << 16);
+          target.ax = tap.ax;
+          target.ay = tap.ay;
+        }
+      }
+    }
+    C3 d_nee[N_MAT_PARAMS];
     Frame frame = frame_from_normal(s_normal);
     V3 wo = to_local(frame, -d);
     V3 ng = s_geometric;
@@ -992,8 +1181,22 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
           float pk = fmax_r(fmax_r(c.r, c.g), c.b);
           add_c = (bounce >= 1 && S.clamp_indirect > 0.0f && pk > S.clamp_indirect) ? c * (S.clamp_indirect / pk) : c;
         }
+        if constexpr (ADJ) {
+          // d(added contribution)/d parameter with the light sample and the MIS weight held.
+          if (adds && has_mat) {
+            C3 unused[N_MAT_PARAMS];
+            bsdf_param_derivs(T, S.materials[mat], base, s_outside, wo, true, wi, !ls_glossy, false, wi, d_nee, unused);
+            float pk = fmax_r(fmax_r(c.r, c.g), c.b);
+            float scale = (bounce >= 1 && S.clamp_indirect > 0.0f && pk > S.clamp_indirect) ? S.clamp_indirect / pk : 1.0f;
+            for (int k = 0; k < N_MAT_PARAMS; ++k)
+              target.g[k] = dot3(adj, thr * d_nee[k] * ls_rad * (fabs_(wi.z) * weight / ls_pdf * scale));
+          }
+        }
         if (!ls_shadow) {
           if (adds) sum = sum + add_c;
+          if constexpr (ADJ) {
+            if (adds && has_mat) g_now = target;
+          }
         } else {
           float t_max = is_finite(ls_dist) ? ls_dist * (1.0f - 1.0e-4f) : RPX_INF;
           bool floor_blocks = false;
@@ -1008,6 +1211,11 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
             P.sh_o[i] = make_float4(from_p.x, from_p.y, from_p.z, t_max);
             P.sh_d[i] = make_float4(ls_wi.x, ls_wi.y, ls_wi.z, 0.0f);
             P.sh_c[i] = make_float4(add_c.r, add_c.g, add_c.b, 0.0f);
+            if constexpr (ADJ) {
+              P.gn[2 * (size_t)i] = make_float4(target.g[0], target.g[1], target.g[2], target.g[3]);
+              P.gn[2 * (size_t)i + 1] = make_float4(target.g[4], 0.0f, 0.0f, 0.0f);
+              P.tgt[i] = make_uint4(target.mat, target.tex, f2u(target.ax), f2u(target.ay));
+            }
           } else {
             // A shadow ray is still cast (and counted) for a contribution that adds nothing.
             atomicAdd((unsigned long long*)&P.rays[1], 1ull);
@@ -1054,6 +1262,21 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
         P.sel_w0[i] = make_float4(sel.w[0], sel.w[1], sel.w[2], sel.w[3]);
         P.sel_w1[i] = make_float4(sel.w[4], sel.w[5], sel.w[6], sel.w[7]);
         cont = true;
+        if constexpr (ADJ) {
+          if (has_mat) {
+            // d log f / d parameter along the sampled direction, applied once the radiance
+            // gathered after this vertex is known (at the next vertex).
+            C3 f_s = bsdf_eval(T, B, wo, bs.wi, false);
+            C3 unused[N_MAT_PARAMS], d_s[N_MAT_PARAMS];
+            bsdf_param_derivs(T, S.materials[mat], base, s_outside, wo, false, bs.wi, false, true, bs.wi, unused, d_s);
+            for (int k = 0; k < N_MAT_PARAMS; ++k) {
+              C3 q = c3(f_s.r > 0.0f ? d_s[k].r / f_s.r : 0.0f, f_s.g > 0.0f ? d_s[k].g / f_s.g : 0.0f,
+                        f_s.b > 0.0f ? d_s[k].b / f_s.b : 0.0f);
+              P.dl[(size_t)N_MAT_PARAMS * i + k] = make_float4(q.r, q.g, q.b, 0.0f);
+            }
+            P.tgt[i] = make_uint4(target.mat, target.tex, f2u(target.ax), f2u(target.ay));
+          }
+        }
       }
     }
   }
@@ -1062,6 +1285,7 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
 }
 }  // namespace
 
+template <bool ADJ>
 __global__ void __launch_bounds__(256) k_shade(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
   const u32* q = P.live[parity];
   u32 n = P.counts[C_LIVE0 + parity];
@@ -1080,16 +1304,23 @@ __global__ void __launch_bounds__(256) k_shade(const DevScene* __restrict__ scen
       i = q[j];
       sid = P.ident[i].y;
     }
+    VertexGrad g_prev, g_now;
+    no_grad(g_prev);
+    no_grad(g_now);
     // One pass per distinct scene of the wave.
     bool todo = active;
     u64 pending = __ballot(todo);
     while (pending) {
       u32 cur = lowest_lane_value(pending, sid);
       if (todo && sid == cur) {
-        shade_path(scenes[cur], P, i, cont, want_shadow);
+        shade_path<ADJ>(scenes[cur], P, i, cont, want_shadow, g_prev, g_now);
         todo = false;
       }
       pending = __ballot(todo);
+    }
+    if constexpr (ADJ) {
+      commit_grad(P, active, g_prev);
+      commit_grad(P, active, g_now);
     }
     // Compaction of the survivors, the finished paths and the shadow rays.
     u32 slot = wave_append(out_n, cont);
@@ -1299,9 +1530,13 @@ hipError_t launch_pool_init(const PathState& P, const Launch& L, u32 work_end) {
   hipLaunchKernelGGL(k_pool_init, dim3(grid_for(L.pool, 256, 4096)), dim3(256), 0, L.stream, P, L.pool, work_end);
   return hipGetLastError();
 }
-hipError_t launch_regen(const PathState& P, const Launch& L, u32 parity) {
-  hipLaunchKernelGGL(k_regen, dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, L.segs, L.n_segs, P,
-                     parity);
+hipError_t launch_regen(const PathState& P, const Launch& L, u32 parity, bool adj) {
+  if (adj)
+    hipLaunchKernelGGL(k_regen<true>, dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, L.segs,
+                       L.n_segs, P, parity);
+  else
+    hipLaunchKernelGGL(k_regen<false>, dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, L.segs,
+                       L.n_segs, P, parity);
   return hipGetLastError();
 }
 
@@ -1321,15 +1556,23 @@ hipError_t launch_extend(const PathState& P, const Launch& L, u32 parity) {
   }
   return hipGetLastError();
 }
-hipError_t launch_shade(const PathState& P, const Launch& L, u32 parity) {
-  hipLaunchKernelGGL(k_shade, dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, P, parity);
+hipError_t launch_shade(const PathState& P, const Launch& L, u32 parity, bool adj) {
+  if (adj)
+    hipLaunchKernelGGL(k_shade<true>, dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, P, parity);
+  else
+    hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, P, parity);
   return hipGetLastError();
 }
-hipError_t launch_shadow(const PathState& P, const Launch& L) {
+hipError_t launch_shadow(const PathState& P, const Launch& L, bool adj) {
+  if (adj) {
+    hipLaunchKernelGGL((k_shadow<128, 24, 1, true>), dim3(grid_for(L.pool, 128, 1u << 20)), dim3(128), 0, L.stream,
+                       L.scenes, P);
+    return hipGetLastError();
+  }
   switch (L.shape) {
 #define X(I, B, N, W)                                                                                             \
   case I:                                                                                                        \
-    hipLaunchKernelGGL((k_shadow<B, N, W>), dim3(grid_for(L.pool, B, 1u << 20)), dim3(B), 0, L.stream, L.scenes, P); \
+    hipLaunchKernelGGL((k_shadow<B, N, W, false>), dim3(grid_for(L.pool, B, 1u << 20)), dim3(B), 0, L.stream, L.scenes, P); \
     break;
     RPX_TRAVERSAL_SHAPES(X)
 #undef X

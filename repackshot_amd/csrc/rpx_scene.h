@@ -45,11 +45,19 @@ struct DevMesh {
   u32 vertex_offset, index_offset, has_normals, inside_out;
 };
 
+// What the integrator reads of a material, and the parameters its BSDF is built from (for
+// textured materials, whose BSDF is built per hit, and for the adjoint's parameter derivatives).
 struct DevMaterial {
   u32 emissive, has_medium;
   C3 radiance;
   C3 sigma;
+  u32 kind, flags;
+  C3 base;
+  float metallic, roughness, ior, transmission, coat, coat_roughness, absorption_mm;
 };
+
+// Differentiable parameters per material, in this order (the adjoint's gradient layout).
+constexpr int N_MAT_PARAMS = 5;  // roughness, base colour r, g, b, ior
 
 struct DevArea {
   u32 rect, glossy, camera, shadow;
@@ -95,7 +103,64 @@ struct DevScene {
   float floor_z;
   Bsdf floor_bsdf;
   Bsdf fallback_bsdf[2];
+  // An RGB albedo texture (f32, row-major, bilinear, repeating) replacing the base colour of
+  // material `tex_material` (NO_TEXTURE: none), mapped planar in world x, y:
+  // u = (x - tex_origin_x) * tex_scale_x, v = (y - tex_origin_y) * tex_scale_y.
+  const float* texture;
+  u32 tex_material, tex_w, tex_h;
+  float tex_origin_x, tex_origin_y, tex_scale_x, tex_scale_y;
 };
+constexpr u32 NO_TEXTURE = 0xffffffffu;
+
+// Bilinear texture coordinates of world point (x, y): the four texels (x0, y0), (x1, y0),
+// (x0, y1), (x1, y1) and the weights of x1 and y1.
+struct TexTap {
+  u32 x0, y0, x1, y1;
+  float ax, ay;
+};
+RPX_HD TexTap texture_tap(u32 w, u32 h, float ox, float oy, float sx, float sy, float x, float y) {
+  float u = (x - ox) * sx, v = (y - oy) * sy;
+  if (!is_finite(u)) u = 0.0f;
+  if (!is_finite(v)) v = 0.0f;
+  u = u - floor_(u);
+  v = v - floor_(v);
+  float fx = u * (float)w - 0.5f, fy = v * (float)h - 0.5f;
+  float x0f = floor_(fx), y0f = floor_(fy);
+  TexTap t;
+  t.ax = fx - x0f;
+  t.ay = fy - y0f;
+  i32 ix = f_to_i32(x0f), iy = f_to_i32(y0f);
+  t.x0 = (u32)(ix < 0 ? ix + (i32)w : (ix >= (i32)w ? ix - (i32)w : ix));
+  t.y0 = (u32)(iy < 0 ? iy + (i32)h : (iy >= (i32)h ? iy - (i32)h : iy));
+  t.x1 = t.x0 + 1 == w ? 0 : t.x0 + 1;
+  t.y1 = t.y0 + 1 == h ? 0 : t.y0 + 1;
+  return t;
+}
+// The texel weights of a tap, in the order (x0, y0), (x1, y0), (x0, y1), (x1, y1).
+RPX_HD void texture_weights(const TexTap& t, float w[4]) {
+  w[0] = (1.0f - t.ax) * (1.0f - t.ay);
+  w[1] = t.ax * (1.0f - t.ay);
+  w[2] = (1.0f - t.ax) * t.ay;
+  w[3] = t.ax * t.ay;
+}
+RPX_HD C3 texture_value(const float* tex, u32 w, const TexTap& t) {
+  float k[4];
+  texture_weights(t, k);
+  const float* a = tex + 3 * ((size_t)t.y0 * w + t.x0);
+  const float* b = tex + 3 * ((size_t)t.y0 * w + t.x1);
+  const float* c = tex + 3 * ((size_t)t.y1 * w + t.x0);
+  const float* d = tex + 3 * ((size_t)t.y1 * w + t.x1);
+  return c3(((a[0] * k[0] + b[0] * k[1]) + c[0] * k[2]) + d[0] * k[3],
+            ((a[1] * k[0] + b[1] * k[1]) + c[1] * k[2]) + d[1] * k[3],
+            ((a[2] * k[0] + b[2] * k[1]) + c[2] * k[2]) + d[2] * k[3]);
+}
+
+// The BSDF of one side of a material whose base colour is `base` (bsdf.rs:384-438).
+RPX_HD Bsdf material_bsdf(const EnergyTables& t, const DevMaterial& m, C3 base, float roughness, float ior,
+                          bool entering) {
+  return bsdf_surface(t, m.kind, base, m.metallic, roughness, ior, m.transmission, m.coat, m.coat_roughness, m.flags,
+                      m.absorption_mm, entering);
+}
 
 // ---------------------------------------------------------------- host side
 struct HostMaterial {
@@ -185,5 +250,9 @@ struct FlatScene {
 
 // Builds BVHs, lights and BSDFs (host tables of the energy compensation in `tables`).
 bool flatten_scene(const HostScene& s, const EnergyTables& tables, FlatScene& out, std::string& err, int threads);
+
+// The device record and the two BSDF sides (outside, inside) of a material.
+DevMaterial dev_material(const HostMaterial& m);
+void material_sides(const EnergyTables& tables, const HostMaterial& m, Bsdf out[2]);
 
 }  // namespace rpx

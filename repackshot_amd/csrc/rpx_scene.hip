@@ -578,6 +578,43 @@ Frame light_frame(V3 normal, V3 up) {
 }
 }  // namespace
 
+DevMaterial dev_material(const HostMaterial& m) {
+  DevMaterial dm;
+  dm.emissive = m.kind == 2 ? 1u : 0u;
+  C3 e = m.emission * m.emission_power;
+  dm.radiance = dm.emissive ? c3(fmax_r(e.r, 0.0f), fmax_r(e.g, 0.0f), fmax_r(e.b, 0.0f)) : splat(0.0f);
+  // integrator/mod.rs:300-311 (medium_of)
+  bool clear = m.transmission > 0.0f && m.metallic < 1.0f && (m.flags & MATERIAL_FLAG_THIN) == 0;
+  dm.has_medium = 0;
+  dm.sigma = splat(0.0f);
+  if (clear && m.absorption_mm > 0.0f && m.kind != 2) {
+    C3 c = m.base_color;
+    C3 sg = c3(-fln(clampf(c.r, 1.0e-4f, 1.0f)) / m.absorption_mm, -fln(clampf(c.g, 1.0e-4f, 1.0f)) / m.absorption_mm,
+               -fln(clampf(c.b, 1.0e-4f, 1.0f)) / m.absorption_mm);
+    if (sg.r > 0.0f || sg.g > 0.0f || sg.b > 0.0f) {
+      dm.has_medium = 1;
+      dm.sigma = sg;
+    }
+  }
+  dm.kind = m.kind;
+  dm.flags = m.flags;
+  dm.base = m.base_color;
+  dm.metallic = m.metallic;
+  dm.roughness = m.roughness;
+  dm.ior = m.ior;
+  dm.transmission = m.transmission;
+  dm.coat = m.coat;
+  dm.coat_roughness = m.coat_roughness;
+  dm.absorption_mm = m.absorption_mm;
+  return dm;
+}
+
+void material_sides(const EnergyTables& tables, const HostMaterial& m, Bsdf out[2]) {
+  for (int side = 0; side < 2; ++side)
+    out[side] = bsdf_surface(tables, m.kind, m.base_color, m.metallic, m.roughness, m.ior, m.transmission, m.coat,
+                             m.coat_roughness, m.flags, m.absorption_mm, side == 0);
+}
+
 bool flatten_scene(const HostScene& s, const EnergyTables& tables, FlatScene& out, std::string& err, int threads) {
   out = FlatScene();
   const size_t nm = s.meshes.size();
@@ -754,27 +791,11 @@ bool flatten_scene(const HostScene& s, const EnergyTables& tables, FlatScene& ou
   if (out.tris.empty()) out.tris.push_back(make_float4(0, 0, 0, 0));
   // Materials: the BSDF of each side and what the integrator reads of them.
   for (const HostMaterial& m : s.materials) {
-    for (int side = 0; side < 2; ++side)
-      out.bsdfs.push_back(bsdf_surface(tables, m.kind, m.base_color, m.metallic, m.roughness, m.ior, m.transmission,
-                                       m.coat, m.coat_roughness, m.flags, m.absorption_mm, side == 0));
-    DevMaterial dm;
-    dm.emissive = m.kind == 2 ? 1u : 0u;
-    C3 e = m.emission * m.emission_power;
-    dm.radiance = dm.emissive ? c3(fmax_r(e.r, 0.0f), fmax_r(e.g, 0.0f), fmax_r(e.b, 0.0f)) : splat(0.0f);
-    // integrator/mod.rs:300-311 (medium_of)
-    bool clear = m.transmission > 0.0f && m.metallic < 1.0f && (m.flags & MATERIAL_FLAG_THIN) == 0;
-    dm.has_medium = 0;
-    dm.sigma = splat(0.0f);
-    if (clear && m.absorption_mm > 0.0f && m.kind != 2) {
-      C3 c = m.base_color;
-      C3 sg = c3(-fln(clampf(c.r, 1.0e-4f, 1.0f)) / m.absorption_mm, -fln(clampf(c.g, 1.0e-4f, 1.0f)) / m.absorption_mm,
-                 -fln(clampf(c.b, 1.0e-4f, 1.0f)) / m.absorption_mm);
-      if (sg.r > 0.0f || sg.g > 0.0f || sg.b > 0.0f) {
-        dm.has_medium = 1;
-        dm.sigma = sg;
-      }
-    }
-    out.materials.push_back(dm);
+    Bsdf sides[2];
+    material_sides(tables, m, sides);
+    out.bsdfs.push_back(sides[0]);
+    out.bsdfs.push_back(sides[1]);
+    out.materials.push_back(dev_material(m));
   }
   // Lights (light.rs:656-994).
   V3 centre;
@@ -916,6 +937,11 @@ bool flatten_scene(const HostScene& s, const EnergyTables& tables, FlatScene& ou
   k.floor_mode = s.floor_mode;
   k.floor_z = s.floor_z;
   k.floor_bsdf = bsdf_lambert(s.floor_color);
+  k.texture = nullptr;
+  k.tex_material = NO_TEXTURE;
+  k.tex_w = k.tex_h = 0;
+  k.tex_origin_x = k.tex_origin_y = 0.0f;
+  k.tex_scale_x = k.tex_scale_y = 1.0f;
   HostMaterial fallback;
   for (int side = 0; side < 2; ++side)
     k.fallback_bsdf[side] = bsdf_surface(tables, fallback.kind, fallback.base_color, fallback.metallic, fallback.roughness,

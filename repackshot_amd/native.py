@@ -53,6 +53,10 @@ _SIGS = {
     "rpx_energy_tables": (ctypes.c_uint32, [ctypes.c_void_p] * 6),
     "rpx_write_scene": (ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64]),
     "rpx_write_queue_job": (ctypes.c_uint64, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64]),
+    "rpx_material_params": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32]),
+    "rpx_set_material_params": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32]),
+    "rpx_set_texture": (ctypes.c_uint32, [ctypes.c_uint32] * 4 + [ctypes.c_void_p] + [ctypes.c_float] * 4),
+    "rpx_adjoint": (ctypes.c_uint32, [ctypes.c_uint32] * 3 + [ctypes.c_void_p] * 3),
     "rpx_queue_create": (ctypes.c_uint32, []),
     "rpx_queue_add": (ctypes.c_uint32, [ctypes.c_uint32] * 4 + [ctypes.c_void_p]),
     "rpx_queue_run": (ctypes.c_uint32, [ctypes.c_uint32]),
@@ -266,6 +270,44 @@ class Scene:
         """Path-pool options (0 / KEEP keep the current value); the film does not depend on them."""
         flags = (FLAG_PER_PASS if per_pass else 0) | (FLAG_GRAPH if graph else 0)
         _check(library().rpx_set_options(self.handle, pool, shape, group, chunk, flags), "rpx_set_options")
+
+    def material_params(self):
+        """The differentiable parameters: (materials, 5) float32 — roughness, base r, g, b, IOR."""
+        lib = library()
+        n = lib.rpx_material_params(self.handle, None, 0)
+        out = np.zeros(5 * n, np.float32)
+        lib.rpx_material_params(self.handle, _ptr(out), out.size)
+        return out.reshape(n, 5)
+
+    def set_material_params(self, params):
+        params = np.ascontiguousarray(params, np.float32).reshape(-1, 5)
+        _check(library().rpx_set_material_params(self.handle, _ptr(params), params.shape[0]),
+               "rpx_set_material_params")
+
+    def set_texture(self, material, rgb, origin=(0.0, 0.0), scale=(1.0, 1.0)):
+        """Gives `material` an RGB f32 base-colour texture (h, w, 3), planar in world x, y; None removes it."""
+        if material is None:
+            _check(library().rpx_set_texture(self.handle, KEEP, 0, 0, None, 0.0, 0.0, 1.0, 1.0), "rpx_set_texture")
+            self.texture_shape = None
+            return
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        h, w = rgb.shape[:2]
+        _check(library().rpx_set_texture(self.handle, material, w, h, _ptr(rgb), origin[0], origin[1], scale[0],
+                                         scale[1]), "rpx_set_texture")
+        self.texture_shape = (h, w)
+
+    def adjoint(self, first, count, d_image):
+        """d loss / d parameter ((materials, 5)) and d loss / d texel ((h, w, 3) or None) of a loss of
+        the mean image of samples [first, first + count), given d loss / d image (H, W, 3)."""
+        d_image = np.ascontiguousarray(d_image, np.float32)
+        if d_image.shape != (self.height, self.width, 3):
+            raise ValueError("d_image must be (height, width, 3)")
+        n = library().rpx_material_params(self.handle, None, 0)
+        gm = np.zeros((n, 5), np.float32)
+        shape = getattr(self, "texture_shape", None)
+        gt = np.zeros(shape + (3,), np.float32) if shape else None
+        _check(library().rpx_adjoint(self.handle, first, count, _ptr(d_image), _ptr(gm), _ptr(gt)), "rpx_adjoint")
+        return gm, gt
 
     def autotune(self, samples=4):
         """Runs the launch tuner; returns the chosen pool, shape and every candidate's ms."""

@@ -1,0 +1,125 @@
+"""The oracle's path-replay adjoint (the CPU reference gradient of the differentiable render)
+against central finite differences of the oracle's own forward render.
+
+For Lambert surfaces at a depth that Russian roulette never reaches, the sampled directions do not
+depend on the albedo, so with the same random numbers the render is a smooth (polynomial)
+function of the base colours and texels: the adjoint must then equal the finite difference of a
+linear loss up to f32 rounding. This pins the replay bookkeeping (remaining radiance, next-event
+terms, pending BSDF-sample terms), the per-vertex parameter derivatives and the texture's
+bilinear distribution, independently of any statistical argument.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from repackshot_amd.rsb import SceneWriter, uv_sphere
+
+W, H, SPP = 24, 18, 3
+
+
+def lambert_scene(depth=2):
+    w = SceneWriter()
+    w.camera((250, -700, 330), (0, 0, 90), focal_mm=35.0)
+    w.material(base_color=(0.6, 0.5, 0.4), ior=1.0, roughness=1.0)
+    w.material(base_color=(0.3, 0.7, 0.5), ior=1.0, roughness=1.0)
+    w.material(base_color=(0.5, 0.5, 0.5), ior=1.0, roughness=1.0)
+    p, n, i = uv_sphere((0, 0, 100), 100.0, 12, 24)
+    w.mesh(p, i, n, 0)
+    quad = np.array([[-900, -900, 0], [900, -900, 0], [900, 900, 0], [-900, 900, 0]], np.float32)
+    w.mesh(quad, [0, 1, 2, 0, 2, 3], np.tile([0, 0, 1], (4, 1)), 2)
+    w.instance(0, material=0)
+    w.instance(0, rows=((0.5, 0, 0, -230), (0, 0.5, 0, 90), (0, 0, 0.5, 0)), material=1)
+    w.instance(1, material=2)
+    w.light(position=(200, -300, 600), direction=(-0.2, 0.3, -1), size_x=300, size_y=300, power=6.0)
+    w.environment(kind=0, floor=0.05, horizon=0.2, zenith=0.4)
+    w.render(W, H, SPP, 7, depth)
+    return w.bytes()
+
+
+def texture(seed=5, size=8):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(0.2, 0.8, size=(size, size, 3)).astype(np.float32)
+
+
+TEX_ORIGIN, TEX_SCALE = (-900.0, -900.0), (1.0 / 600.0, 1.0 / 600.0)
+
+
+def loss_and_image(o, weights):
+    film, _ = o.render_film(0, SPP, threads=8)
+    rgb, wgt = film[0], film[1]
+    img = rgb * (np.float32(1.0) / wgt)[..., None]
+    return float(np.sum(weights.astype(np.float64) * img.astype(np.float64))), img
+
+
+@pytest.fixture(scope="module")
+def setup():
+    rsb = lambert_scene()
+    o = orc.OracleScene(rsb)
+    tex = texture()
+    o.set_texture(2, tex, TEX_ORIGIN, TEX_SCALE)
+    weights = np.random.default_rng(11).uniform(-1.0, 1.0, size=(H, W, 3)).astype(np.float32)
+    gm, gt = o.adjoint(0, SPP, weights, threads=8)
+    return rsb, o, tex, weights, gm, gt
+
+
+def test_adjoint_albedo_equals_finite_differences(setup):
+    rsb, o, tex, weights, gm, gt = setup
+    base = o.material_params()
+    h = 1.0e-2
+    for m in (0, 1):
+        for k in (1, 2, 3):
+            up, dn = base.copy(), base.copy()
+            up[m, k] += h
+            dn[m, k] -= h
+            o.set_material_params(up)
+            lu, _ = loss_and_image(o, weights)
+            o.set_material_params(dn)
+            ld, _ = loss_and_image(o, weights)
+            fd = (lu - ld) / (2 * h)
+            assert abs(gm[m, k] - fd) <= 2e-3 * abs(fd) + 1e-4, (m, k, gm[m, k], fd)
+    o.set_material_params(base)
+    # Lambert surfaces ignore roughness; the textured floor's base colour lives in its texels.
+    assert np.all(gm[:, 0] == 0.0)
+    assert np.all(gm[2, 1:4] == 0.0)
+
+
+def test_adjoint_texels_equal_finite_differences(setup):
+    rsb, o, tex, weights, gm, gt = setup
+    assert gt.shape == tex.shape and np.any(gt != 0.0)
+    # The texels the image leans on most, and one it never sees at this depth if there is one.
+    order = np.argsort(-np.abs(gt).reshape(-1))
+    h = 1.0e-2
+    for flat in order[:6]:
+        y, x, c = np.unravel_index(flat, gt.shape)
+        up, dn = tex.copy(), tex.copy()
+        up[y, x, c] += h
+        dn[y, x, c] -= h
+        o.set_texture(2, up, TEX_ORIGIN, TEX_SCALE)
+        lu, _ = loss_and_image(o, weights)
+        o.set_texture(2, dn, TEX_ORIGIN, TEX_SCALE)
+        ld, _ = loss_and_image(o, weights)
+        fd = (lu - ld) / (2 * h)
+        assert abs(gt[y, x, c] - fd) <= 2e-3 * abs(fd) + 1e-4, ((y, x, c), gt[y, x, c], fd)
+    o.set_texture(2, tex, TEX_ORIGIN, TEX_SCALE)
+
+
+def test_adjoint_is_linear_in_the_image_gradient(setup):
+    rsb, o, tex, weights, gm, gt = setup
+    gm2, gt2 = o.adjoint(0, SPP, 2.0 * weights, threads=8)
+    np.testing.assert_allclose(gm2, 2.0 * gm, rtol=1e-6, atol=1e-9)
+    gm0, gt0 = o.adjoint(0, SPP, np.zeros_like(weights), threads=8)
+    assert np.all(gm0 == 0.0) and np.all(gt0 == 0.0)
+
+
+def test_textured_forward_render_reads_the_texture(setup):
+    rsb, o, tex, weights, gm, gt = setup
+    _, img_tex = loss_and_image(o, weights)
+    o.set_texture(2, np.full_like(tex, 0.5), TEX_ORIGIN, TEX_SCALE)
+    _, img_flat = loss_and_image(o, weights)
+    o.set_texture(None, None)
+    _, img_plain = loss_and_image(o, weights)
+    o.set_texture(2, tex, TEX_ORIGIN, TEX_SCALE)
+    # A constant texture equal to the material's own base colour renders the same image (up to
+    # the rounding of the bilinear weights' sum).
+    np.testing.assert_allclose(img_flat, img_plain, rtol=1e-5, atol=1e-7)
+    assert np.max(np.abs(img_tex - img_plain)) > 1e-3
