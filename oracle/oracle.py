@@ -61,6 +61,7 @@ def lib():
         L.orc_set_material_params.argtypes = [vp, vp, u32]
         L.orc_set_texture.argtypes = [vp, u32, u32, u32, vp, f32, f32, f32, f32]
         L.orc_adjoint.argtypes = [vp, u32, u32, ctypes.c_int, vp, vp, vp]
+        L.orc_render_film_f64.argtypes = [vp, u32, u32, ctypes.c_int, vp, vp]
         os.makedirs(os.path.dirname(CACHE), exist_ok=True)
         L.orc_set_energy_cache(CACHE.encode())
         _lib = L
@@ -126,6 +127,15 @@ class OracleScene:
         lib().orc_render_tiles(self.h, _p(r), r.shape[0], first, count, threads, *(_p(a) for a in film), _p(stats))
         keys = ["rays", "nearest", "any", "nearest_nodes", "nearest_packets", "any_nodes", "any_packets"]
         return film, {k: int(v) for k, v in zip(keys, stats)}
+
+    def render_f64(self, first, count, threads=None):
+        """The mean image of samples [first, first + count), radiance summed in float64: (H, W, 3)."""
+        if threads is None:
+            threads = os.cpu_count() or 1
+        rgb = np.zeros((self.height, self.width, 3), np.float64)
+        alpha = np.zeros((self.height, self.width), np.float64)
+        lib().orc_render_film_f64(self.h, first, count, threads, _p(rgb), _p(alpha))
+        return rgb / count
 
     def material_params(self):
         n = lib().orc_material_params(self.h, None, 0)

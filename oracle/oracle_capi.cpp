@@ -344,4 +344,43 @@ int orc_adjoint(void* p, uint32_t first, uint32_t count, int threads, const floa
   return 0;
 }
 
+// The convergence reference: samples [first, first + count) of every pixel with the radiance summed
+// in double (rgb: 3*W*H, alpha: W*H, caller-zeroed), rows over `threads` workers.
+int orc_render_film_f64(void* p, uint32_t first, uint32_t count, int threads, double* rgb, double* alpha) {
+  auto* hd = static_cast<Handle*>(p);
+  const Scene& s = hd->scene;
+  u32 W = s.render.width, H = s.render.height;
+  std::atomic<u32> next_row{0};
+  auto worker = [&]() {
+    for (;;) {
+      u32 y = next_row.fetch_add(1);
+      if (y >= H) return;
+      for (u32 x = 0; x < W; ++x) {
+        double r = 0, g = 0, b = 0, a = 0;
+        for (u32 k = 0; k < count; ++k) {
+          PixelSampler sampler = PixelSampler::make(s.render.seed, x, y, first + k);
+          Ray ray;
+          camera_sample(s, sampler, x, y, ray);
+          Rgb c;
+          float al;
+          radiance(s, hd->prep, sampler, ray, c, al, nullptr);
+          r += c.r;
+          g += c.g;
+          b += c.b;
+          a += al;
+        }
+        size_t i = (size_t)y * W + x;
+        rgb[3 * i] += r;
+        rgb[3 * i + 1] += g;
+        rgb[3 * i + 2] += b;
+        alpha[i] += a;
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 0; t < std::max(threads, 1); ++t) pool.emplace_back(worker);
+  for (auto& t : pool) t.join();
+  return 0;
+}
+
 }  // extern "C"
