@@ -11,8 +11,10 @@
 //   shadow   any hit of the queued shadow rays; unblocked ones add their contribution
 //   splat    per-pixel film sums, samples added in sample-index order
 //
-// Queues are compacted per wave: a ballot of the lanes that append, one atomic per wave, and each
-// lane's slot from the popcount of the lanes below it (mbcnt). Paths of different samples, bounces
+// Queues are compacted per block: a ballot per wave, each lane's slot from the popcount of the lanes
+// below it (mbcnt), a prefix over the block's waves in LDS and one atomic per block. New paths take
+// consecutive work ids and queue entries without atomics (k_regen, then k_bookkeep moves the
+// counters). Paths of different samples, bounces
 // and scenes share a launch: every lane computes its own Sobol points from the generator columns
 // and its sample index, and scene constants are read per distinct scene of a wave (a waterfall
 // over the wave's scene ids, one pass when the wave holds one scene).
@@ -370,15 +372,36 @@ __device__ bool trace_any(const Geo& g, u32* lds_col, V3 o, V3 d, float t_min, f
 }
 
 // ================================================================== queue compaction
-__device__ __forceinline__ u32 wave_append(u32* counter, bool want) {
-  u64 mask = __ballot(want);
-  u32 below = __builtin_amdgcn_mbcnt_hi((u32)(mask >> 32), __builtin_amdgcn_mbcnt_lo((u32)mask, 0u));
-  u32 total = (u32)__popcll(mask);
-  u32 base = 0;
-  int leader = mask ? (int)__ffsll((long long)mask) - 1 : 0;
-  if (want && (int)__lane_id() == leader) base = atomicAdd(counter, total);
-  base = __shfl(base, leader);
-  return base + below;
+
+// Appends for a whole block to NC queues at once: per-wave ballots, a prefix over the block's waves
+// in LDS, and one atomic per queue per block (every thread of the block must call this).
+constexpr int MAX_BLOCK_WAVES = 16;
+template <int NC>
+__device__ __forceinline__ void block_append(u32* const counters[NC], const bool want[NC], u32 slot[NC]) {
+  __shared__ u32 wave_count[NC][MAX_BLOCK_WAVES];
+  __shared__ u32 base[NC];
+  u32 lane = __lane_id(), wave = threadIdx.x / 64u, waves = (blockDim.x + 63u) / 64u;
+  u32 below[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    u64 mask = __ballot(want[c]);
+    below[c] = __builtin_amdgcn_mbcnt_hi((u32)(mask >> 32), __builtin_amdgcn_mbcnt_lo((u32)mask, 0u));
+    if (lane == 0) wave_count[c][wave] = (u32)__popcll(mask);
+  }
+  __syncthreads();
+  if (threadIdx.x < (u32)NC) {
+    u32 c = threadIdx.x, total = 0;
+    for (u32 w = 0; w < waves; ++w) {
+      u32 t = wave_count[c][w];
+      wave_count[c][w] = total;
+      total += t;
+    }
+    base[c] = total ? atomicAdd(counters[c], total) : 0u;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NC; ++c) slot[c] = base[c] + wave_count[c][wave] + below[c];
+  __syncthreads();
 }
 
 // The wave-uniform value of `v` in the lowest lane of `mask` (mask must be non-zero).
@@ -492,13 +515,22 @@ __global__ void __launch_bounds__(256) k_pool_init(PathState P, u32 pool, u32 wo
 
 // With ADJ, finished paths keep the forward's results and new paths load their adjoint and the
 // forward's radiance of the same work id.
+//
+// The j-th finished slot takes work id cursor + j and live-queue entry survivors + j (no atomics);
+// k_bookkeep then advances the counters.
+__device__ __forceinline__ u32 new_paths(u32 done, u32 cursor, u32 end) {
+  return cursor < end ? (done < end - cursor ? done : end - cursor) : 0u;
+}
+
 template <bool ADJ>
 __global__ void __launch_bounds__(256) k_regen(const DevScene* __restrict__ scenes, const DevSegment* __restrict__ segs,
                                                u32 n_segs, PathState P, u32 parity) {
   u32 n = P.counts[C_DONE];
   u32 end = P.counts[C_END];
+  u32 cursor = P.counts[C_CURSOR];
+  u32 survivors = P.counts[C_LIVE0 + parity];
+  u32 fresh = new_paths(n, cursor, end);
   u32* out_q = P.live[parity];
-  u32* out_n = &P.counts[C_LIVE0 + parity];
   u32 stride = gridDim.x * blockDim.x;
   // Whole waves iterate together so that the ballots below see every lane.
   u32 n_iter = (n + stride - 1) / stride;
@@ -512,8 +544,8 @@ __global__ void __launch_bounds__(256) k_regen(const DevScene* __restrict__ scen
       u32 done_work = P.ident[slot].x;
       if (!ADJ && done_work != NO_WORK) P.results[done_work] = P.sum[slot];
     }
-    u32 wid = wave_append(&P.counts[C_CURSOR], active);
-    bool got = active && wid < end;
+    u32 wid = cursor + j;
+    bool got = active && j < fresh;
     u32 sid = 0, px = 0, py = 0, index = 0;
     if (got) {
       const DevSegment& sg = segs[find_segment(segs, n_segs, wid)];
@@ -554,9 +586,25 @@ __global__ void __launch_bounds__(256) k_regen(const DevScene* __restrict__ scen
       }
       pending = __ballot(todo);
     }
-    u32 q = wave_append(out_n, got);
-    if (got) out_q[q] = slot;
+    if (got) out_q[survivors + j] = slot;
   }
+}
+
+// The counters after a regen: the live queue holds survivors + new paths, the cursor moves past
+// the new paths, and the queues this iteration's shade and shadow fill start empty. An empty live
+// queue marks the pool idle (every path of the chunk done).
+__global__ void k_bookkeep(PathState P, u32 parity) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  u32 n = P.counts[C_DONE], end = P.counts[C_END], cursor = P.counts[C_CURSOR];
+  u32 fresh = new_paths(n, cursor, end);
+  u32 live = P.counts[C_LIVE0 + parity] + fresh;
+  P.counts[C_LIVE0 + parity] = live;
+  P.counts[C_CURSOR] = cursor + fresh;
+  P.counts[C_DONE] = 0;
+  P.counts[C_LIVE0 + (parity ^ 1)] = 0;
+  P.counts[C_SHADOW] = 0;
+  P.rays[0] += live;
+  if (live == 0) P.counts[C_IDLE] = 1;
 }
 
 // ================================================================== extend
@@ -566,14 +614,6 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
   u32* col = lds + threadIdx.x;
   const u32* q = P.live[parity];
   u32 n = P.counts[C_LIVE0 + parity];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    // The queues this iteration's shade and shadow fill (their previous contents are consumed).
-    P.counts[C_LIVE0 + (parity ^ 1)] = 0;
-    P.counts[C_SHADOW] = 0;
-    P.counts[C_DONE] = 0;
-    P.rays[0] += n;
-    if (n == 0) P.counts[C_IDLE] = 1;
-  }
   u32 stride = gridDim.x * blockDim.x;
   for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
     u32 i = q[j];
@@ -1322,14 +1362,16 @@ __global__ void __launch_bounds__(256) k_shade(const DevScene* __restrict__ scen
       commit_grad(P, active, g_prev);
       commit_grad(P, active, g_now);
     }
-    // Compaction of the survivors, the finished paths and the shadow rays.
-    u32 slot = wave_append(out_n, cont);
-    if (cont) out_q[slot] = i;
+    // Compaction of the survivors, the finished paths and the shadow rays (one atomic per block
+    // and queue).
     bool done = active && !cont;
-    u32 dslot = wave_append(&P.counts[C_DONE], done);
-    if (done) P.done_queue[dslot] = i;
-    u32 sslot = wave_append(&P.counts[C_SHADOW], want_shadow);
-    if (want_shadow) P.shadow_queue[sslot] = i;
+    u32* const counters[3] = {out_n, &P.counts[C_DONE], &P.counts[C_SHADOW]};
+    const bool want[3] = {cont, done, want_shadow};
+    u32 slots[3];
+    block_append<3>(counters, want, slots);
+    if (cont) out_q[slots[0]] = i;
+    if (done) P.done_queue[slots[1]] = i;
+    if (want_shadow) P.shadow_queue[slots[2]] = i;
   }
 }
 
@@ -1537,6 +1579,9 @@ hipError_t launch_regen(const PathState& P, const Launch& L, u32 parity, bool ad
   else
     hipLaunchKernelGGL(k_regen<false>, dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, L.segs,
                        L.n_segs, P, parity);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, L.stream, P, parity);
   return hipGetLastError();
 }
 
