@@ -14,7 +14,7 @@ The adjoint record times the path-replay gradients of the inverse-material fit's
 central finite differences of the same forward renderer.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-queue] [--no-adjoint]
-                       [--pool P --shape S]   (fixed launch options instead of the tuner)
+                       [--pool P --shape S --shadow-shape T]   (fixed launch options instead of the tuner)
 """
 import argparse
 import json
@@ -96,7 +96,7 @@ def mixed_queue(native, seed=0, jobs=64):
     q = native.Queue()
     ids = [q.add(sc, 0, spp) for sc, spp in zip(scenes, spps)]
     tune = q.autotune()
-    log("bench: queue tuner chose pool %d shape %d" % (tune["pool"], tune["shape"]))
+    log("bench: queue tuner chose pool %d shapes %d / %d" % (tune["pool"], tune["shape"], tune["shadow_shape"]))
     q.run()  # warm
     q.reset()
     st0 = q.stats()
@@ -127,7 +127,8 @@ def mixed_queue(native, seed=0, jobs=64):
             "persistent_mrays_per_s": round(rays / t_persistent / 1e6, 3),
             "plain_per_job_mpaths_per_s": round(paths / t_plain / 1e6, 3),
             "speedup": round(t_plain / t_persistent, 3), "persistent_s": round(t_persistent, 3),
-            "plain_s": round(t_plain, 3), "tuned": {"pool": tune["pool"], "shape": tune["shape"]},
+            "plain_s": round(t_plain, 3), "tuned": {"pool": tune["pool"], "shape": tune["shape"],
+                                                      "shadow_shape": tune["shadow_shape"]},
             "films_bit_exact_vs_plain": identical}
 
 
@@ -219,6 +220,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--pool", type=int, default=0)
     ap.add_argument("--shape", type=int, default=-1)
+    ap.add_argument("--shadow-shape", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this tracer runs on one GPU; --gpus %d is not supported (one process, one GPU)"
@@ -241,12 +243,15 @@ def main():
     paths_per_step = W * H * spp
 
     if args.shape >= 0:
-        scene.set_options(pool=args.pool or (1 << 21), shape=args.shape)
-        tuned = {"pool": args.pool or (1 << 21), "shape": args.shape, "by": "fixed"}
+        scene.set_options(pool=args.pool or (1 << 21), shape=args.shape, shadow_shape=args.shadow_shape)
+        tuned = {"pool": args.pool or (1 << 21), "shape": args.shape, "shadow_shape": args.shadow_shape,
+                 "by": "fixed"}
     else:
-        t = scene.autotune(4)
-        tuned = {"pool": t["pool"], "shape": t["shape"], "by": "tuner", "candidates_ms": t["ms"]}
-        log("bench: tuner chose pool %d shape %d (%s)" % (t["pool"], t["shape"], t["ms"]))
+        t = scene.autotune(32)
+        tuned = {"pool": t["pool"], "shape": t["shape"], "shadow_shape": t["shadow_shape"], "by": "tuner",
+                 "candidates_ms": t["ms"], "shadow_candidates_ms": t["shadow_ms"]}
+        log("bench: tuner chose pool %d shapes %d / %d (%s %s)" % (t["pool"], t["shape"], t["shadow_shape"], t["ms"],
+                                                                  t["shadow_ms"]))
     scene.set_profiling(False)
     for i in range(args.warmup):
         scene.reset_film()

@@ -70,17 +70,18 @@ uint32_t rpx_scene_info(uint32_t handle, uint64_t* out8);
 /* Since the handle was loaded: rays cast (camera + extension + shadow, as the reference counts
  * them), of them closest-hit and shadow rays, and samples rendered. */
 uint32_t rpx_scene_stats(uint32_t handle, uint64_t* out4);
-/* Pool options of a handle's renders: path slots (0 keeps the current), traversal launch shape
- * (index; 0xffffffff keeps), iterations per completion check (0 keeps), most path samples per
- * chunk (0 keeps; at most 2^27), flags: bit 0 one sample pass at a time (the pool drains after
- * each pass), bit 1 each group of iterations launched as a captured HIP graph. Films do not
- * depend on any of them. */
-uint32_t rpx_set_options(uint32_t handle, uint32_t pool, uint32_t shape, uint32_t group, uint32_t chunk,
-                         uint32_t flags);
-/* The launch tuner: renders samples [0, samples) of the image (default 4; the film is untouched)
- * with every traversal launch shape at pool sizes 2^20, 2^21, 2^22, keeps the fastest as the
- * handle's options, and reports: pool, shape, best ms, then the 15 candidates' ms (shape-major). */
-uint32_t rpx_autotune(uint32_t handle, uint32_t samples, float* out18);
+/* Pool options of a handle's renders: path slots (0 keeps the current), closest-hit and shadow
+ * launch shapes (index 0..7; 0xffffffff keeps), iterations per completion check (0 keeps), most
+ * path samples per chunk (0 keeps; at most 2^27), flags: bit 0 one sample pass at a time (the
+ * pool drains after each pass), bit 1 each group of iterations launched as a captured HIP graph.
+ * Films do not depend on any of them. */
+uint32_t rpx_set_options(uint32_t handle, uint32_t pool, uint32_t shape, uint32_t shadow_shape, uint32_t group,
+                         uint32_t chunk, uint32_t flags);
+/* The launch tuner: renders samples [0, samples) of the image (default 32; the film is untouched)
+ * with every closest-hit launch shape (8) at pool sizes 2^20 .. 2^25, then every shadow launch
+ * shape at the best of those, keeps the fastest as the handle's options, and reports: pool,
+ * closest-hit shape, shadow shape, best ms, then the 48 + 8 candidates' ms. */
+uint32_t rpx_autotune(uint32_t handle, uint32_t samples, float* out60);
 /* Records per-stage GPU timing of the following renders (events on the handle's stream). */
 uint32_t rpx_set_profiling(uint32_t handle, uint32_t on);
 /* Timing of the last render: total ms, then ms of the closest-hit, shade, shadow and regen
@@ -122,10 +123,10 @@ uint32_t rpx_queue_add(uint32_t queue, uint32_t scene, uint32_t first, uint32_t 
 uint32_t rpx_queue_run(uint32_t queue); /* renders every job once more (films accumulate) */
 uint32_t rpx_queue_reset(uint32_t queue); /* clears every job's film */
 uint32_t rpx_queue_film_read(uint32_t queue, uint32_t job, float* rgb, float* weight, float* alpha, float* lum2);
-uint32_t rpx_queue_set_options(uint32_t queue, uint32_t pool, uint32_t shape, uint32_t group, uint32_t chunk,
-                               uint32_t flags);
-/* The tuner over the queue's jobs' first samples (up to `paths` path samples; 0: 2^23). */
-uint32_t rpx_queue_autotune(uint32_t queue, uint32_t paths, float* out18);
+uint32_t rpx_queue_set_options(uint32_t queue, uint32_t pool, uint32_t shape, uint32_t shadow_shape,
+                               uint32_t group, uint32_t chunk, uint32_t flags);
+/* The tuner over the queue's jobs' first samples (up to `paths` path samples; 0: 2^25). */
+uint32_t rpx_queue_autotune(uint32_t queue, uint32_t paths, float* out60);
 uint32_t rpx_queue_set_profiling(uint32_t queue, uint32_t on);
 uint32_t rpx_queue_last_timing(uint32_t queue, float* out12);
 uint32_t rpx_queue_stats(uint32_t queue, uint64_t* out4);

@@ -223,8 +223,9 @@ std::shared_ptr<Queue> find_queue(u32 q) {
   return it == g_queues.end() ? nullptr : it->second;
 }
 
-void apply_options(Engine& e, u32 pool, u32 shape, u32 group, u32 chunk, u32 flags) {
+void apply_options(Engine& e, u32 pool, u32 shape, u32 shadow_shape, u32 group, u32 chunk, u32 flags) {
   if (pool) e.opt.pool = std::max<u32>(pool, 64);
+  if (shadow_shape < (u32)N_SHAPES) e.opt.shadow_shape = (int)shadow_shape;
   if (chunk) e.opt.chunk = std::min(chunk, RESULTS_MAX);
   if (shape < (u32)N_SHAPES) e.opt.shape = (int)shape;
   if (group) e.opt.group = std::max<u32>(2, group & ~1u);
@@ -530,12 +531,12 @@ uint32_t rpx_scene_stats(uint32_t handle, uint64_t* out4) {
   return RPX_OK;
 }
 
-uint32_t rpx_set_options(uint32_t handle, uint32_t pool, uint32_t shape, uint32_t group, uint32_t chunk,
-                         uint32_t flags) {
+uint32_t rpx_set_options(uint32_t handle, uint32_t pool, uint32_t shape, uint32_t shadow_shape, uint32_t group,
+                         uint32_t chunk, uint32_t flags) {
   auto h = find(handle);
   if (!h) return RPX_BAD_HANDLE;
   std::lock_guard<std::mutex> lock(h->mutex);
-  apply_options(*h->engine, pool, shape, group, chunk, flags);
+  apply_options(*h->engine, pool, shape, shadow_shape, group, chunk, flags);
   return RPX_OK;
 }
 
@@ -546,28 +547,29 @@ static uint32_t tune_report(Engine& e, hipError_t r, const std::vector<float>& m
   }
   out[0] = (float)e.opt.pool;
   out[1] = (float)e.opt.shape;
-  out[2] = 0.0f;
+  out[2] = (float)e.opt.shadow_shape;
+  out[3] = 0.0f;
   for (size_t k = 0; k < ms.size(); ++k) {
-    out[3 + k] = ms[k];
-    if (out[2] == 0.0f || (ms[k] > 0.0f && ms[k] < out[2])) out[2] = ms[k];
+    out[4 + k] = ms[k];
+    if (out[3] == 0.0f || (ms[k] > 0.0f && ms[k] < out[3])) out[3] = ms[k];
   }
   return RPX_OK;
 }
 
-uint32_t rpx_autotune(uint32_t handle, uint32_t samples, float* out18) {
-  if (!out18) return RPX_NULL_POINTER;
+uint32_t rpx_autotune(uint32_t handle, uint32_t samples, float* out60) {
+  if (!out60) return RPX_NULL_POINTER;
   auto h = find(handle);
   if (!h) return RPX_BAD_HANDLE;
   std::lock_guard<std::mutex> lock(h->mutex);
   hipSetDevice(h->device);
   u32 npix = h->dev.width * h->dev.height;
-  u32 count = std::max<u32>(1, std::min<u32>(samples ? samples : 4, RESULTS_MAX / std::max<u32>(npix, 1)));
+  u32 count = std::max<u32>(1, std::min<u32>(samples ? samples : 32, RESULTS_MAX / std::max<u32>(npix, 1)));
   DevSegment s = whole_image(h.get(), 0, 0, count);
   std::vector<float> ms;
   std::string err;
   h->engine->textured = h->dev.tex_material != NO_TEXTURE;
   hipError_t r = h->engine->autotune(h->d_scene, std::vector<DevSegment>(1, s), ms, err);
-  return tune_report(*h->engine, r, ms, err, out18);
+  return tune_report(*h->engine, r, ms, err, out60);
 }
 
 uint32_t rpx_set_profiling(uint32_t handle, uint32_t on) {
@@ -889,17 +891,17 @@ uint32_t rpx_queue_film_read(uint32_t queue, uint32_t job, float* rgb, float* we
   return read_film(j.film, (size_t)j.width * j.height, q->stream, rgb, weight, alpha, lum2);
 }
 
-uint32_t rpx_queue_set_options(uint32_t queue, uint32_t pool, uint32_t shape, uint32_t group, uint32_t chunk,
-                               uint32_t flags) {
+uint32_t rpx_queue_set_options(uint32_t queue, uint32_t pool, uint32_t shape, uint32_t shadow_shape,
+                               uint32_t group, uint32_t chunk, uint32_t flags) {
   auto q = find_queue(queue);
   if (!q) return RPX_BAD_HANDLE;
   std::lock_guard<std::mutex> lock(q->mutex);
-  apply_options(*q->engine, pool, shape, group, chunk, flags);
+  apply_options(*q->engine, pool, shape, shadow_shape, group, chunk, flags);
   return RPX_OK;
 }
 
-uint32_t rpx_queue_autotune(uint32_t queue, uint32_t paths, float* out18) {
-  if (!out18) return RPX_NULL_POINTER;
+uint32_t rpx_queue_autotune(uint32_t queue, uint32_t paths, float* out60) {
+  if (!out60) return RPX_NULL_POINTER;
   auto q = find_queue(queue);
   if (!q) return RPX_BAD_HANDLE;
   std::lock_guard<std::mutex> lock(q->mutex);
@@ -919,7 +921,7 @@ uint32_t rpx_queue_autotune(uint32_t queue, uint32_t paths, float* out18) {
     table.push_back(h->dev);
   }
   // The probe: the jobs' first samples, in queue order, up to `paths` path samples.
-  u32 budget = std::min<u32>(paths ? paths : (1u << 23), RESULTS_MAX);
+  u32 budget = std::min<u32>(paths ? paths : (1u << 25), RESULTS_MAX);
   std::vector<DevSegment> probe;
   u32 used = 0;
   for (const Job& j : q->jobs) {
@@ -946,7 +948,7 @@ uint32_t rpx_queue_autotune(uint32_t queue, uint32_t paths, float* out18) {
   for (const DevScene& ds : table) q->engine->textured = q->engine->textured || ds.tex_material != NO_TEXTURE;
   hipError_t r = q->engine->autotune((const DevScene*)d, probe, ms, err);
   hipFree(d);
-  return tune_report(*q->engine, r, ms, err, out18);
+  return tune_report(*q->engine, r, ms, err, out60);
 }
 
 uint32_t rpx_queue_set_profiling(uint32_t queue, uint32_t on) {

@@ -34,7 +34,8 @@ bool device_tables(int dev, DeviceTables*& out, std::string& err);
 // How the pool runs.
 struct EngineOptions {
   u32 pool = 1u << 21;  // path slots (capped by the work of a chunk)
-  int shape = 0;        // traversal launch shape (index into SHAPES)
+  int shape = 0;        // closest-hit launch shape (index into SHAPES)
+  int shadow_shape = 0;  // any-hit (shadow) launch shape
   u32 group = 8;        // iterations launched between checks for completion (even)
   u32 chunk = 1u << 27;  // most work ids (path samples) per chunk, at most RESULTS_MAX
   bool per_pass = false;  // one sample pass at a time, the pool drained after each (no regeneration across samples)
@@ -51,8 +52,9 @@ struct Timing {
 
 // Largest number of work ids (path samples) in one chunk: the results buffer holds one float4 each.
 constexpr u32 RESULTS_MAX = 1u << 27;
-constexpr int N_TUNE_POOLS = 3;
-constexpr u32 TUNE_POOLS[N_TUNE_POOLS] = {1u << 20, 1u << 21, 1u << 22};
+constexpr int N_TUNE_POOLS = 6;
+constexpr int N_TUNE_CANDIDATES = N_SHAPES * N_TUNE_POOLS + N_SHAPES;
+constexpr u32 TUNE_POOLS[N_TUNE_POOLS] = {1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25};
 
 class Engine {
  public:
@@ -70,9 +72,10 @@ class Engine {
   // segments carry the adjoint image and its scale.
   hipError_t adjoint(const DevScene* d_scenes, const std::vector<DevSegment>& segs, float* grad_mat, float* grad_tex,
                      u32 tex_w, u32 tex_h, std::string& err);
-  // The tuner: renders `probe` (whose results are discarded; one chunk) with every launch shape at
-  // each of TUNE_POOLS pool sizes, keeps the fastest in `opt`, and returns every candidate's GPU
-  // time (ms; shape-major).
+  // The tuner: renders `probe` (whose results are discarded; one chunk) with every closest-hit
+  // launch shape at each of TUNE_POOLS pool sizes, then every shadow launch shape at the best of
+  // those, keeps the fastest in `opt`, and returns every candidate's GPU time (ms: shape-major
+  // closest-hit candidates, then the shadow candidates).
   hipError_t autotune(const DevScene* d_scenes, const std::vector<DevSegment>& probe, std::vector<float>& ms,
                       std::string& err);
   // Closest-hit rays, shadow rays counted without a trace, shadow rays traced (since init).

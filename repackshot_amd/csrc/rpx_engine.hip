@@ -199,6 +199,7 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegm
   L.n_segs = (u32)chunk.size();
   L.pool = std::min(opt.pool, total);
   L.shape = opt.shape;
+  L.shadow_shape = opt.shadow_shape;
   L.stream = stream_;
   e = launch_pool_init(P, L, total);
   if (e != hipSuccess) return e;
@@ -426,29 +427,49 @@ hipError_t Engine::autotune(const DevScene* d_scenes, const std::vector<DevSegme
   profiling = false;
   opt.per_pass = false;
   opt.graph = false;
-  ms.assign(N_SHAPES * N_TUNE_POOLS, 0.0f);
+  ms.assign(N_TUNE_CANDIDATES, 0.0f);
   float best = 0.0f;
   EngineOptions chosen = keep;
   hipError_t e = hipSuccess;
+  auto time_it = [&](float& t) {
+    e = run(d_scenes, probe, false, err);  // warm (allocations, caches)
+    if (e == hipSuccess) e = run(d_scenes, probe, false, err);
+    t = timing.total_ms;
+  };
+  opt.shadow_shape = 0;
   for (int sh = 0; sh < N_SHAPES && e == hipSuccess; ++sh)
     for (int p = 0; p < N_TUNE_POOLS && e == hipSuccess; ++p) {
       opt.shape = sh;
       opt.pool = TUNE_POOLS[p];
-      e = run(d_scenes, probe, false, err);  // warm (allocations, caches)
-      if (e == hipSuccess) e = run(d_scenes, probe, false, err);
+      float t = 0.0f;
+      time_it(t);
       if (e != hipSuccess) break;
-      float t = timing.total_ms;
       ms[sh * N_TUNE_POOLS + p] = t;
       if (best == 0.0f || t < best) {
         best = t;
         chosen.shape = sh;
         chosen.pool = TUNE_POOLS[p];
+        chosen.shadow_shape = 0;
       }
     }
+  opt.shape = chosen.shape;
+  opt.pool = chosen.pool;
+  for (int sh = 0; sh < N_SHAPES && e == hipSuccess; ++sh) {
+    opt.shadow_shape = sh;
+    float t = 0.0f;
+    time_it(t);
+    if (e != hipSuccess) break;
+    ms[N_SHAPES * N_TUNE_POOLS + sh] = t;
+    if (t < best) {
+      best = t;
+      chosen.shadow_shape = sh;
+    }
+  }
   profiling = keep_profiling;
   opt = keep;
   if (e != hipSuccess) return e;
   opt.shape = chosen.shape;
+  opt.shadow_shape = chosen.shadow_shape;
   opt.pool = chosen.pool;
   return hipSuccess;
 }

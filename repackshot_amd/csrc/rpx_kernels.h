@@ -89,8 +89,9 @@ struct TraversalShape {
   u32 block;       // lanes per block
   u32 lds_stack;   // stack entries per lane held in LDS (the rest of 64 in private memory)
   u32 min_waves;   // waves per SIMD the register allocation must allow (1: no constraint)
+  u32 resident;    // 1: resident lanes (each lane walks ray after ray; grid = what the GPU holds)
 };
-constexpr int N_SHAPES = 5;
+constexpr int N_SHAPES = 8;
 extern const TraversalShape SHAPES[N_SHAPES];
 
 struct Launch {
@@ -98,7 +99,8 @@ struct Launch {
   const DevSegment* segs;
   u32 n_segs;
   u32 pool;   // path slots in use (grid bound of the pool-sized kernels)
-  int shape;  // index into SHAPES
+  int shape;         // closest-hit launch shape (index into SHAPES)
+  int shadow_shape;  // any-hit launch shape
   hipStream_t stream;
 };
 

@@ -29,7 +29,8 @@
 
 namespace rpx {
 
-const TraversalShape SHAPES[N_SHAPES] = {{128, 24, 1}, {128, 32, 1}, {128, 16, 1}, {256, 16, 1}, {128, 16, 8}};
+const TraversalShape SHAPES[N_SHAPES] = {{128, 24, 1, 0}, {128, 32, 1, 0}, {128, 16, 1, 0}, {256, 16, 1, 0},
+                                         {128, 16, 8, 0}, {128, 24, 1, 1}, {128, 16, 1, 1}, {256, 16, 1, 1}};
 
 // ================================================================== traversal
 namespace {
@@ -716,6 +717,221 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
         v.g[4] = b.x;
       }
       commit_grad(P, visible, v);
+    }
+  }
+}
+
+
+// ================================================================== traversal with resident lanes
+// The same closest-hit and any-hit walks as above, one node or leaf per loop trip, each lane taking
+// its next ray (queue entries gid, gid + stride, ...) as soon as its current one ends, so a wave is
+// not held by its slowest ray. Launched with about as many lanes as the GPU keeps resident.
+template <int BLK, int LDSN, int WPE>
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_extend_flat(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
+  __shared__ u32 lds[LDSN * BLK];
+  const u32* q = P.live[parity];
+  const u32 n = P.counts[C_LIVE0 + parity];
+  const u32 stride = gridDim.x * blockDim.x;
+  u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+  Stack<BLK, LDSN> st;
+  st.lds = lds + threadIdx.x;
+  st.sp = 0;
+  bool busy = false, local = false, cull = false, camera = false;
+  int tlas_sp = -1;
+  u32 slot = 0, inst = 0, cur = REF_EMPTY, best_inst = 0, best_tri = 0;
+  float best_t = 0.0f;
+  V3 o = v3(0, 0, 0), d = v3(0, 0, 1);
+  Geo g{};
+  PRay ray = prepare(o, d, RAY_EPSILON, RPX_INF);
+  for (;;) {
+    while (!busy && j < n) {
+      slot = q[j];
+      float4 ro = P.ray_o[slot], rd = P.ray_d[slot];
+      o = v3(ro.x, ro.y, ro.z);
+      d = v3(rd.x, rd.y, rd.z);
+      camera = (f2u(P.med[slot].w) & FLAG_PRIMARY) != 0;
+      g = geo_of(scenes, P.ident[slot].y);
+      if (usable_ray(o, d)) {
+        ray = prepare(o, d, RAY_EPSILON, RPX_INF);
+        busy = ray.t_min <= ray.t_max;
+      }
+      if (busy) {
+        best_t = ray.t_max;
+        best_inst = best_tri = 0xffffffffu;
+        st.sp = 0;
+        local = false;
+        cull = false;
+        tlas_sp = -1;
+        inst = 0;
+        cur = g.root;
+      } else {
+        P.hit[slot] = make_uint4(0xffffffffu, 0xffffffffu, f2u(RPX_INF), 0u);
+        j += stride;
+      }
+    }
+    if (!busy) break;
+    bool moved = false;
+    if (cur != REF_EMPTY) {
+      if (!(cur & REF_LEAF)) {
+        float4 near;
+        uint4 ch;
+        u32 bits = node_test(g.nodes + cur, ray, best_t, near, ch);
+        if (bits) {
+          cur = push_ordered(bits, near, ch, st);
+          moved = true;
+        }
+      } else if (tlas_sp < 0) {
+        u32 i = (cur & ~REF_LEAF) >> LEAF_COUNT_BITS;
+        const DevInstance& in = g.instances[i];
+        u32 fl = in.flags;
+        if ((fl & INST_ACTIVE) && !(camera && (fl & INST_CAMERA_HIDDEN))) {
+          inst = i;
+          cull = (fl & INST_ONE_SIDED) != 0;
+          if (!(fl & INST_IDENTITY)) {
+            V3 lo = xf_point(in.to_local, o), ld = xf_vector(in.to_local, d);
+            ray = prepare(lo, ld, RAY_EPSILON, RPX_INF);
+            local = true;
+          }
+          tlas_sp = st.sp;
+          cur = in.blas_root;
+          moved = true;
+        }
+      } else {
+        u32 first = (cur & ~REF_LEAF) >> LEAF_COUNT_BITS, count = (cur & 7u) + 1;
+        const float4* tp = g.tris + 3 * (size_t)first;
+        for (u32 k = 0; k < count; ++k, tp += 3) {
+          float4 va = tp[0], vb = tp[1], vc = tp[2];
+          float t;
+          if (watertight(ray, va, vb, vc, best_t, cull, t)) {
+            u32 tri = f2u(va.w);
+            if (t < best_t || (t == best_t && (inst < best_inst || (inst == best_inst && tri < best_tri)))) {
+              best_t = t;
+              best_inst = inst;
+              best_tri = tri;
+            }
+          }
+        }
+      }
+    }
+    if (!moved) {
+      if (tlas_sp >= 0 && st.sp == tlas_sp) {
+        tlas_sp = -1;
+        if (local) ray = prepare(o, d, RAY_EPSILON, RPX_INF);
+        local = false;
+      }
+      if (st.sp == 0) {
+        P.hit[slot] = best_inst == 0xffffffffu ? make_uint4(0xffffffffu, 0xffffffffu, f2u(RPX_INF), 0u)
+                                               : make_uint4(best_inst, best_tri, f2u(best_t), 0u);
+        busy = false;
+        j += stride;
+      } else {
+        cur = st.pop();
+      }
+    }
+  }
+}
+
+template <int BLK, int LDSN, int WPE>
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_shadow_flat(const DevScene* __restrict__ scenes, PathState P) {
+  __shared__ u32 lds[LDSN * BLK];
+  const u32 n = P.counts[C_SHADOW];
+  if (blockIdx.x == 0 && threadIdx.x == 0) P.rays[2] += n;
+  const u32 stride = gridDim.x * blockDim.x;
+  u32 j = blockIdx.x * blockDim.x + threadIdx.x;
+  Stack<BLK, LDSN> st;
+  st.lds = lds + threadIdx.x;
+  st.sp = 0;
+  bool busy = false, local = false, cull = false;
+  int tlas_sp = -1;
+  u32 slot = 0, cur = REF_EMPTY;
+  float t_max = 0.0f;
+  V3 o = v3(0, 0, 0), d = v3(0, 0, 1);
+  Geo g{};
+  PRay ray = prepare(o, d, RAY_EPSILON, RPX_INF);
+  for (;;) {
+    while (!busy && j < n) {
+      slot = P.shadow_queue[j];
+      float4 so = P.sh_o[slot], sd = P.sh_d[slot];
+      o = v3(so.x, so.y, so.z);
+      d = v3(sd.x, sd.y, sd.z);
+      t_max = so.w;
+      g = geo_of(scenes, P.ident[slot].y);
+      if (usable_ray(o, d)) {
+        ray = prepare(o, d, RAY_EPSILON, t_max);
+        busy = ray.t_min <= ray.t_max;
+      }
+      if (busy) {
+        st.sp = 0;
+        local = false;
+        cull = false;
+        tlas_sp = -1;
+        cur = g.root;
+      } else {
+        // No walk: nothing can block the ray.
+        float4 c = P.sh_c[slot], s = P.sum[slot];
+        P.sum[slot] = make_float4(s.x + c.x, s.y + c.y, s.z + c.z, s.w);
+        j += stride;
+      }
+    }
+    if (!busy) break;
+    bool moved = false, blocked = false;
+    if (cur != REF_EMPTY) {
+      if (!(cur & REF_LEAF)) {
+        float4 near;
+        uint4 ch;
+        u32 bits = node_test(g.nodes + cur, ray, ray.t_max, near, ch);
+        if (bits) {
+          u32 next = REF_EMPTY;
+          if (bits & 1u) next = ch.x;
+          if (bits & 2u) { if (next == REF_EMPTY) next = ch.y; else st.push(ch.y); }
+          if (bits & 4u) { if (next == REF_EMPTY) next = ch.z; else st.push(ch.z); }
+          if (bits & 8u) { if (next == REF_EMPTY) next = ch.w; else st.push(ch.w); }
+          cur = next;
+          moved = true;
+        }
+      } else if (tlas_sp < 0) {
+        u32 i = (cur & ~REF_LEAF) >> LEAF_COUNT_BITS;
+        const DevInstance& in = g.instances[i];
+        u32 fl = in.flags;
+        if (fl & INST_ACTIVE) {
+          cull = (fl & INST_ONE_SIDED) != 0;
+          if (!(fl & INST_IDENTITY)) {
+            V3 lo = xf_point(in.to_local, o), ld = xf_vector(in.to_local, d);
+            ray = prepare(lo, ld, RAY_EPSILON, t_max);
+            local = true;
+          }
+          tlas_sp = st.sp;
+          cur = in.blas_root;
+          moved = true;
+        }
+      } else {
+        u32 first = (cur & ~REF_LEAF) >> LEAF_COUNT_BITS, count = (cur & 7u) + 1;
+        const float4* tp = g.tris + 3 * (size_t)first;
+        for (u32 k = 0; k < count && !blocked; ++k, tp += 3) {
+          float t;
+          blocked = watertight(ray, tp[0], tp[1], tp[2], ray.t_max, cull, t);
+        }
+      }
+    }
+    if (blocked) {
+      busy = false;
+      j += stride;
+    } else if (!moved) {
+      if (tlas_sp >= 0 && st.sp == tlas_sp) {
+        tlas_sp = -1;
+        if (local) ray = prepare(o, d, RAY_EPSILON, t_max);
+        local = false;
+      }
+      if (st.sp == 0) {
+        float4 c = P.sh_c[slot], s = P.sum[slot];
+        P.sum[slot] = make_float4(s.x + c.x, s.y + c.y, s.z + c.z, s.w);
+        busy = false;
+        j += stride;
+      } else {
+        cur = st.pop();
+      }
     }
   }
 }
@@ -1586,6 +1802,18 @@ hipError_t launch_regen(const PathState& P, const Launch& L, u32 parity, bool ad
 }
 
 #define RPX_TRAVERSAL_SHAPES(X) X(0, 128, 24, 1) X(1, 128, 32, 1) X(2, 128, 16, 1) X(3, 256, 16, 1) X(4, 128, 16, 8)
+#define RPX_FLAT_SHAPES(X) X(5, 128, 24, 1) X(6, 128, 16, 1) X(7, 256, 16, 1)
+
+// Blocks of `kernel` the GPU keeps resident at once (all CUs), for the resident-lane kernels.
+template <class K>
+static u32 resident_blocks(K kernel, int block) {
+  int dev = 0, cus = 0, per_cu = 0;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0);
+  long long n = (long long)cus * (per_cu > 0 ? per_cu : 1);
+  return n > 0 ? (u32)n : 1u;
+}
 
 hipError_t launch_extend(const PathState& P, const Launch& L, u32 parity) {
   switch (L.shape) {
@@ -1595,6 +1823,15 @@ hipError_t launch_extend(const PathState& P, const Launch& L, u32 parity) {
                        parity);                                                                                 \
     break;
     RPX_TRAVERSAL_SHAPES(X)
+#undef X
+#define X(I, B, N, W)                                                                                       \
+  case I: {                                                                                                 \
+    static const u32 resident = resident_blocks(k_extend_flat<B, N, W>, B);                                \
+    u32 grid = grid_for(L.pool, B, resident);                                                               \
+    hipLaunchKernelGGL((k_extend_flat<B, N, W>), dim3(grid), dim3(B), 0, L.stream, L.scenes, P, parity);    \
+    break;                                                                                                  \
+  }
+    RPX_FLAT_SHAPES(X)
 #undef X
     default:
       return hipErrorInvalidValue;
@@ -1614,12 +1851,21 @@ hipError_t launch_shadow(const PathState& P, const Launch& L, bool adj) {
                        L.scenes, P);
     return hipGetLastError();
   }
-  switch (L.shape) {
+  switch (L.shadow_shape) {
 #define X(I, B, N, W)                                                                                             \
   case I:                                                                                                        \
     hipLaunchKernelGGL((k_shadow<B, N, W, false>), dim3(grid_for(L.pool, B, 1u << 20)), dim3(B), 0, L.stream, L.scenes, P); \
     break;
     RPX_TRAVERSAL_SHAPES(X)
+#undef X
+#define X(I, B, N, W)                                                                                       \
+  case I: {                                                                                                 \
+    static const u32 resident = resident_blocks(k_shadow_flat<B, N, W>, B);                                \
+    u32 grid = grid_for(L.pool, B, resident);                                                               \
+    hipLaunchKernelGGL((k_shadow_flat<B, N, W>), dim3(grid), dim3(B), 0, L.stream, L.scenes, P);            \
+    break;                                                                                                  \
+  }
+    RPX_FLAT_SHAPES(X)
 #undef X
     default:
       return hipErrorInvalidValue;

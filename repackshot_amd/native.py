@@ -46,7 +46,7 @@ _SIGS = {
     "rpx_trace_rays": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 4),
     "rpx_scene_info": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_scene_stats": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
-    "rpx_set_options": (ctypes.c_uint32, [ctypes.c_uint32] * 6),
+    "rpx_set_options": (ctypes.c_uint32, [ctypes.c_uint32] * 7),
     "rpx_autotune": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_set_profiling": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
     "rpx_last_timing": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
@@ -62,7 +62,7 @@ _SIGS = {
     "rpx_queue_run": (ctypes.c_uint32, [ctypes.c_uint32]),
     "rpx_queue_reset": (ctypes.c_uint32, [ctypes.c_uint32]),
     "rpx_queue_film_read": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32] + [ctypes.c_void_p] * 4),
-    "rpx_queue_set_options": (ctypes.c_uint32, [ctypes.c_uint32] * 6),
+    "rpx_queue_set_options": (ctypes.c_uint32, [ctypes.c_uint32] * 7),
     "rpx_queue_autotune": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_queue_set_profiling": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
     "rpx_queue_last_timing": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
@@ -266,10 +266,11 @@ class Scene:
         _check(library().rpx_scene_stats(self.handle, _ptr(out)), "rpx_scene_stats")
         return {"rays": int(out[0]), "nearest": int(out[1]), "shadow": int(out[2]), "samples": int(out[3])}
 
-    def set_options(self, pool=0, shape=KEEP, group=0, chunk=0, per_pass=False, graph=False):
+    def set_options(self, pool=0, shape=KEEP, shadow_shape=KEEP, group=0, chunk=0, per_pass=False, graph=False):
         """Path-pool options (0 / KEEP keep the current value); the film does not depend on them."""
         flags = (FLAG_PER_PASS if per_pass else 0) | (FLAG_GRAPH if graph else 0)
-        _check(library().rpx_set_options(self.handle, pool, shape, group, chunk, flags), "rpx_set_options")
+        _check(library().rpx_set_options(self.handle, pool, shape, shadow_shape, group, chunk, flags),
+               "rpx_set_options")
 
     def material_params(self):
         """The differentiable parameters: (materials, 5) float32 — roughness, base r, g, b, IOR."""
@@ -309,9 +310,9 @@ class Scene:
         _check(library().rpx_adjoint(self.handle, first, count, _ptr(d_image), _ptr(gm), _ptr(gt)), "rpx_adjoint")
         return gm, gt
 
-    def autotune(self, samples=4):
+    def autotune(self, samples=32):
         """Runs the launch tuner; returns the chosen pool, shape and every candidate's ms."""
-        out = np.zeros(18, np.float32)
+        out = np.zeros(60, np.float32)
         _check(library().rpx_autotune(self.handle, samples, _ptr(out)), "rpx_autotune")
         return _tune(out)
 
@@ -326,8 +327,9 @@ class Scene:
 
 
 def _tune(out):
-    return {"pool": int(out[0]), "shape": int(out[1]), "best_ms": float(out[2]),
-            "ms": [[round(float(v), 3) for v in out[3 + 3 * s:6 + 3 * s]] for s in range(5)]}
+    return {"pool": int(out[0]), "shape": int(out[1]), "shadow_shape": int(out[2]), "best_ms": float(out[3]),
+            "ms": [[round(float(v), 3) for v in out[4 + 6 * s:10 + 6 * s]] for s in range(8)],
+            "shadow_ms": [round(float(v), 3) for v in out[52:60]]}
 
 
 def _timing(out):
@@ -371,13 +373,13 @@ class Queue:
                "rpx_queue_film_read")
         return rgb, weight, alpha, lum2
 
-    def set_options(self, pool=0, shape=KEEP, group=0, chunk=0, per_pass=False, graph=False):
+    def set_options(self, pool=0, shape=KEEP, shadow_shape=KEEP, group=0, chunk=0, per_pass=False, graph=False):
         flags = (FLAG_PER_PASS if per_pass else 0) | (FLAG_GRAPH if graph else 0)
-        _check(library().rpx_queue_set_options(self.handle, pool, shape, group, chunk, flags),
+        _check(library().rpx_queue_set_options(self.handle, pool, shape, shadow_shape, group, chunk, flags),
                "rpx_queue_set_options")
 
     def autotune(self, paths=0):
-        out = np.zeros(18, np.float32)
+        out = np.zeros(60, np.float32)
         _check(library().rpx_queue_autotune(self.handle, paths, _ptr(out)), "rpx_queue_autotune")
         return _tune(out)
 

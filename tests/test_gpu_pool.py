@@ -31,6 +31,10 @@ def plain(gpu):
     dict(pool=1 << 16, shape=2),
     dict(pool=1 << 16, shape=3),
     dict(pool=1 << 16, shape=4),
+    dict(pool=1 << 16, shape=5, shadow_shape=6),        # resident lanes
+    dict(pool=3000, shape=6, shadow_shape=7),
+    dict(pool=1 << 16, shape=7, shadow_shape=5),
+    dict(pool=1 << 16, shape=1, shadow_shape=4),
     dict(per_pass=True),
     dict(graph=True, group=4),
     dict(pool=3000, graph=True),
