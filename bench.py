@@ -35,9 +35,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X datasheet: 8 TB/s HBM3E
 NODE_BYTES = 256
 PACKET_BYTES = 160
 # Memory-side bytes of k_extend per closest-hit ray, from rocprofv3 FETCH_SIZE and WRITE_SIZE runs
-# of this workload (pool 4,194,304, launch shape 0; FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0,
-# the factors a 256 MiB device copy gives): 1.668e9 bytes per dispatch over 2,278,155 rays.
-TRAFFIC_BYTES_PER_RAY = 732.3
+# of this workload (pool 33,554,432, closest-hit shape 3, shadow shape 4; FETCH_SIZE scaled by 2.0,
+# WRITE_SIZE by 1.0, the factors a 256 MiB device copy gives): 2.775e9 bytes per dispatch over
+# 5,695,387 rays.
+TRAFFIC_BYTES_PER_RAY = 487.3
 
 
 def log(*a):
