@@ -62,6 +62,21 @@ def lib():
         L.orc_set_texture.argtypes = [vp, u32, u32, u32, vp, f32, f32, f32, f32]
         L.orc_adjoint.argtypes = [vp, u32, u32, ctypes.c_int, vp, vp, vp]
         L.orc_render_film_f64.argtypes = [vp, u32, u32, ctypes.c_int, vp, vp]
+        L.orc_adaptive_new.restype = vp
+        L.orc_adaptive_new.argtypes = [u32, u32, f32, f32, u32]
+        L.orc_adaptive_free.argtypes = [vp]
+        L.orc_adaptive_left.restype = u32
+        L.orc_adaptive_left.argtypes = [vp]
+        L.orc_adaptive_advance.argtypes = [vp, u32]
+        L.orc_adaptive_samples_of_pixel.restype = u32
+        L.orc_adaptive_samples_of_pixel.argtypes = [vp, u32, u32]
+        L.orc_adaptive_blocks.restype = u32
+        L.orc_adaptive_blocks.argtypes = [vp]
+        L.orc_adaptive_decide.argtypes = [vp, vp, u32, vp, vp, vp]
+        L.orc_adaptive_pixel_error.restype = f32
+        L.orc_adaptive_pixel_error.argtypes = [f32, f32, f32, f32, u32, f32]
+        L.orc_render_adaptive.restype = u32
+        L.orc_render_adaptive.argtypes = [vp, u32, ctypes.c_int, vp, vp, vp, vp, vp]
         os.makedirs(os.path.dirname(CACHE), exist_ok=True)
         L.orc_set_energy_cache(CACHE.encode())
         _lib = L
@@ -137,6 +152,19 @@ class OracleScene:
         lib().orc_render_film_f64(self.h, first, count, threads, _p(rgb), _p(alpha))
         return rgb / count
 
+    def render_adaptive(self, cap=0, threads=None):
+        """The adaptive render to `cap` samples (0: the scene's): film (rgb, weight, alpha, lum2),
+        samples per block, number of decisions."""
+        if threads is None:
+            threads = os.cpu_count() or 1
+        H, W = self.height, self.width
+        film = (np.zeros((H, W, 3), np.float32), np.zeros((H, W), np.float32), np.zeros((H, W), np.float32),
+                np.zeros((H, W), np.float32))
+        blocks = ((W + 7) // 8) * ((H + 7) // 8)
+        counts = np.zeros(blocks, np.uint32)
+        n = lib().orc_render_adaptive(self.h, cap, threads, *(_p(a) for a in film), _p(counts))
+        return film, counts, int(n)
+
     def material_params(self):
         n = lib().orc_material_params(self.h, None, 0)
         out = np.zeros(5 * n, np.float32)
@@ -180,6 +208,41 @@ class OracleScene:
         anyhit = np.zeros(n, np.uint8)
         lib().orc_trace_rays(self.h, _p(rays), n, 1 if brute else 0, _p(inst), _p(tri), _p(t), _p(anyhit))
         return inst, tri, t, anyhit
+
+
+class Adaptive:
+    """The state of an adaptive render in rounds (adaptive.rs Adaptive)."""
+
+    def __init__(self, width, height, target, ev, cap):
+        self.h = ctypes.c_void_p(lib().orc_adaptive_new(width, height, target, ev, cap))
+        self.width, self.height = width, height
+        self.n_blocks = int(lib().orc_adaptive_blocks(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_adaptive_free(self.h)
+            self.h = None
+
+    def left(self):
+        return int(lib().orc_adaptive_left(self.h))
+
+    def advance(self, n):
+        lib().orc_adaptive_advance(self.h, n)
+
+    def samples_of_pixel(self, x, y):
+        return int(lib().orc_adaptive_samples_of_pixel(self.h, x, y))
+
+    def decide(self, sums, stride, lum2):
+        sums = np.ascontiguousarray(sums, np.float32)
+        lum2 = np.ascontiguousarray(lum2, np.float32)
+        out = np.zeros(6, np.float32)
+        active = np.zeros(self.n_blocks, np.uint8)
+        lib().orc_adaptive_decide(self.h, _p(sums), stride, _p(lum2), _p(out), _p(active))
+        return out, active
+
+
+def adaptive_pixel_error(r, g, b, lum2, n, floor):
+    return np.float32(lib().orc_adaptive_pixel_error(r, g, b, lum2, n, floor))
 
 
 def energy_tables():

@@ -9,6 +9,7 @@
 #include <atomic>
 #include <cstring>
 #include <thread>
+#include "rt_adaptive.hpp"
 #include "rt_render.hpp"
 
 using namespace orc;
@@ -381,6 +382,89 @@ int orc_render_film_f64(void* p, uint32_t first, uint32_t count, int threads, do
   for (int t = 0; t < std::max(threads, 1); ++t) pool.emplace_back(worker);
   for (auto& t : pool) t.join();
   return 0;
+}
+
+// ---------------------------------------------------------------- adaptive sampling (adaptive.rs)
+void* orc_adaptive_new(uint32_t width, uint32_t height, float target, float ev, uint32_t cap) {
+  return new adaptive::Adaptive(width, height, target, ev, cap);
+}
+void orc_adaptive_free(void* a) { delete static_cast<adaptive::Adaptive*>(a); }
+uint32_t orc_adaptive_left(void* a) { return static_cast<adaptive::Adaptive*>(a)->left(); }
+void orc_adaptive_advance(void* a, uint32_t n) { static_cast<adaptive::Adaptive*>(a)->advance(n); }
+uint32_t orc_adaptive_samples_of_pixel(void* a, uint32_t x, uint32_t y) {
+  auto* ad = static_cast<adaptive::Adaptive*>(a);
+  return ad->samples_of(ad->block_of(x, y));
+}
+uint32_t orc_adaptive_blocks(void* a) { return static_cast<adaptive::Adaptive*>(a)->blocks(); }
+// The round's decision from whole-image planes: out6 = round, active, blocks (as floats), threshold,
+// max error, remaining; active_out one byte per block.
+void orc_adaptive_decide(void* a, const float* sums, uint32_t stride, const float* lum2, float* out6,
+                         uint8_t* active_out) {
+  auto* ad = static_cast<adaptive::Adaptive*>(a);
+  adaptive::Decision d = ad->decide(sums, stride, lum2);
+  out6[0] = (float)d.round;
+  out6[1] = (float)d.active;
+  out6[2] = (float)d.blocks;
+  out6[3] = d.threshold;
+  out6[4] = d.max_error;
+  out6[5] = d.remaining;
+  std::memcpy(active_out, ad->active().data(), ad->active().size());
+}
+float orc_adaptive_pixel_error(float r, float g, float b, float lum2, uint32_t n, float floor) {
+  return adaptive::pixel_error(r, g, b, lum2, n, floor);
+}
+
+// The scene rendered adaptively to the cap (0: the scene's max_samples) with the host loop of the
+// reference's tests (adaptive/tests.rs:294-356): whole rounds, the first with the plain renderer,
+// later ones block by block, each pixel's samples in index order. Film arrays as orc_render_film
+// (caller-zeroed); block_counts gets each block's samples; returns the number of decisions.
+uint32_t orc_render_adaptive(void* p, uint32_t cap, int threads, float* rgb, float* weight, float* alpha, float* lum2,
+                             uint32_t* block_counts) {
+  auto* hd = static_cast<Handle*>(p);
+  const Scene& s = hd->scene;
+  u32 W = s.render.width, H = s.render.height;
+  if (cap == 0) cap = s.render.max_samples;
+  adaptive::Adaptive ad(W, H, s.render.noise_threshold, s.render.exposure_ev, cap);
+  Film film;
+  film.init(0, 0, W, H);
+  u32 decisions = 0;
+  int n = std::max(threads, 1);
+  for (;;) {
+    if (ad.left() == 0) {
+      ad.decide(film.rgb.data(), 3, film.lum2.data());
+      ++decisions;
+      if (ad.converged()) break;
+    }
+    u32 count = ad.left();
+    // The round's blocks (all of them in the first), each its quota from its own count.
+    std::vector<std::pair<u32, u32>> work;  // (block, samples)
+    for (u32 b = 0; b < ad.blocks(); ++b) {
+      u32 q = ad.quota(b);
+      if (q > 0) work.push_back({b, std::min(q, count)});
+    }
+    std::atomic<size_t> next{0};
+    auto worker = [&]() {
+      for (;;) {
+        size_t k = next.fetch_add(1);
+        if (k >= work.size()) return;
+        u32 b = work[k].first, bx = b % ad.cols(), by = b / ad.cols();
+        TileRect t{bx * adaptive::BLOCK, by * adaptive::BLOCK, std::min(adaptive::BLOCK, W - bx * adaptive::BLOCK),
+                   std::min(adaptive::BLOCK, H - by * adaptive::BLOCK)};
+        u32 first = ad.counts()[b];
+        for (u32 j = 0; j < work[k].second; ++j) render_tile(s, hd->prep, t, first + j, film, nullptr);
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 0; t < n; ++t) pool.emplace_back(worker);
+    for (auto& t : pool) t.join();
+    ad.advance(count);
+  }
+  std::memcpy(rgb, film.rgb.data(), film.rgb.size() * 4);
+  std::memcpy(weight, film.weight.data(), film.weight.size() * 4);
+  std::memcpy(alpha, film.alpha.data(), film.alpha.size() * 4);
+  std::memcpy(lum2, film.lum2.data(), film.lum2.size() * 4);
+  std::memcpy(block_counts, ad.counts().data(), ad.counts().size() * 4);
+  return decisions;
 }
 
 }  // extern "C"

@@ -226,9 +226,14 @@ bool scene_from_rsb(const uint8_t* bytes, size_t len, Scene& scene, std::string&
       (void)r.opt_f32(1);   // denoise strength
       s.pixel_filter = r.opt_u32(0);
       s.edge_supersample = r.opt_u32(1);
-      u32 adaptive = r.opt_u32(0);
+      // E10 (rsb.rs:1105-1110): the adaptive bit asks the host for rounds; the renderer reads it not.
+      s.adaptive = r.opt_u32(0);
+      s.noise_threshold = r.opt_f32(0);
       if (caustic_photons != 0) { err = "caustics are not supported"; return false; }
-      if (adaptive != 0) { err = "adaptive sampling is not supported"; return false; }
+      if (!(s.noise_threshold >= 0.0f && s.noise_threshold <= 1.0f)) {
+        err = "noise_threshold must be between 0 and 1";
+        return false;
+      }
       if (s.pixel_filter == 3) { err = "the Blackman-Harris filter is not supported"; return false; }
       if (s.edge_supersample > 1) { err = "edge supersampling is not supported"; return false; }
       if (s.width < 1 || s.width > 16384 || s.height < 1 || s.height > 16384) {

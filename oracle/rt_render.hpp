@@ -64,6 +64,8 @@ struct RenderSettings {
   u32 tonemap = 1, max_depth = 8;
   float clamp_indirect = 0;
   u32 pixel_filter = 0, edge_supersample = 1;
+  u32 adaptive = 0;
+  float noise_threshold = 0;
   u32 depth() const { return max_depth == 0 ? 8 : std::min<u32>(max_depth, 1024); }
 };
 
