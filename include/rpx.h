@@ -14,6 +14,16 @@
  *   rt_scene_free         ffi.rs:113-124
  *   rt_render_tile        ffi.rs:118-158 (one sample of a tile: w*h*3 linear RGB floats)
  *   rt_render_tile_rgba   ffi.rs:160-196 (w*h*4 floats, RGB premultiplied, alpha)
+ *   rt_render_block_rgba  ffi.rs:196-234 (a block of an adaptive round: one sample of the
+ *                         rectangle, as rt_render_tile_rgba; `iteration` picks a photon map,
+ *                         and scenes with photon maps or edge supersampling are refused at load)
+ *   rt_adaptive_decide    ffi.rs:237-296 (the adaptive decision of a width x height image from its
+ *                         blocks' sample counts, RGBA sums (4 floats per pixel) and squared-
+ *                         luminance sums: one byte per block (1: in the next round) and six words:
+ *                         the next round's samples, active blocks, blocks, then the f32 bits of the
+ *                         threshold, the largest error and the remaining work; status 2 for an
+ *                         empty image). The per-pixel errors, their 3 x 3 smoothing and the
+ *                         per-block maxima run on the GPU.
  * Status codes (rt-wasm/src/lib.rs:36-51): 0 OK, 1 bad handle, 2 empty or oversized tile
  * (more than 2048 x 2048 pixels), 3 null pointer, 4 bad scene update; codes from 16 on are the
  * GPU's (16 no device or device memory exhausted, 17 kernel launch or execution failure).
@@ -52,6 +62,11 @@ uint32_t rt_render_tile(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uin
                         float* out);
 uint32_t rt_render_tile_rgba(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t h,
                              uint32_t sample_index, float* out);
+uint32_t rt_render_block_rgba(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t h,
+                              uint32_t sample_index, uint32_t iteration, float* out);
+uint32_t rt_adaptive_decide(uint32_t width, uint32_t height, float ev, float target, uint32_t cap,
+                            const uint8_t* counts_ptr, const uint8_t* sums_ptr, const uint8_t* lum2_ptr,
+                            uint8_t* active_ptr, uint8_t* decision_ptr);
 
 /* Renders samples [first, first + count) of every pixel into the handle's resident film. */
 uint32_t rpx_render_samples(uint32_t handle, uint32_t first, uint32_t count);
@@ -131,6 +146,17 @@ uint32_t rpx_queue_set_profiling(uint32_t queue, uint32_t on);
 uint32_t rpx_queue_last_timing(uint32_t queue, float* out12);
 uint32_t rpx_queue_stats(uint32_t queue, uint64_t* out4);
 uint32_t rpx_queue_free(uint32_t queue);
+/* Adaptive sampling in rounds (the reference hosts' loop over rt_render_block_rgba and
+ * rt_adaptive_decide, rt-core/src/adaptive.rs:319-554, run with the film resident in HBM): clears
+ * the handle's film, renders a warm-up round of every 8 x 8 block, then rounds of the blocks the
+ * decisions choose, each block's samples in index order, until no block is chosen; `cap` bounds
+ * every block's samples (0: the scene's max_samples), the scene's noise_threshold is the target and
+ * its exposure the floor of the mean. out8: decisions, rounds, path samples, fewest and most
+ * samples of a block, the f32 bits of the last decision's largest error and remaining work, and the
+ * GPU time in ns. rpx_adaptive_counts copies the last adaptive render's samples per block (row by
+ * row; status 2 when `cap` words do not hold them). */
+uint32_t rpx_render_adaptive(uint32_t handle, uint32_t cap, uint64_t* out8);
+uint32_t rpx_adaptive_counts(uint32_t handle, uint32_t* counts, uint32_t cap);
 /* Number of HIP devices visible (0 without a GPU). */
 uint32_t rpx_device_count(void);
 

@@ -8,6 +8,7 @@
 #include <mutex>
 #include <thread>
 #include "../../include/rpx.h"
+#include "rpx_adaptive.h"
 #include "rpx_engine.h"
 
 namespace rpx {
@@ -32,6 +33,12 @@ struct Handle {
   u64 samples = 0;
   // base-colour texture of one material
   std::vector<DevBuf> tex_bufs;
+  // adaptive sampling: samples per block of the last adaptive render, device scratch
+  std::vector<u32> block_counts;
+  std::vector<DevBuf> adaptive_bufs;
+  size_t adaptive_px = 0, adaptive_blocks = 0;
+  u32* d_block_counts = nullptr;
+  float *d_pixel_error = nullptr, *d_block_error = nullptr;
 
   ~Handle() {
     if (stream) hipStreamSynchronize(stream);
@@ -39,6 +46,7 @@ struct Handle {
     dev_free_all(scene_bufs);
     dev_free_all(film_bufs);
     dev_free_all(tex_bufs);
+    dev_free_all(adaptive_bufs);
     if (stream) hipStreamDestroy(stream);
   }
 };
@@ -385,6 +393,204 @@ uint32_t rt_render_tile(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uin
 uint32_t rt_render_tile_rgba(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t h, uint32_t sample_index,
                              float* out) {
   return render_tile_impl(handle, x, y, w, h, sample_index, out, 4);
+}
+uint32_t rt_render_block_rgba(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t h, uint32_t sample_index,
+                              uint32_t iteration, float* out) {
+  // Scenes with photon maps or edge supersampling are refused at load, so a block of a round is a
+  // tile and `iteration` (the photon map's) selects nothing.
+  (void)iteration;
+  return render_tile_impl(handle, x, y, w, h, sample_index, out, 4);
+}
+
+// ---------------------------------------------------------------- adaptive sampling
+static bool ensure_adaptive(Handle* h, size_t px, size_t blocks, std::string& err) {
+  if (px <= h->adaptive_px && blocks <= h->adaptive_blocks) return true;
+  hipStreamSynchronize(h->stream);
+  dev_free_all(h->adaptive_bufs);
+  h->adaptive_px = h->adaptive_blocks = 0;
+  auto& B = h->adaptive_bufs;
+  if (!dev_alloc(B, blocks * 4, (void**)&h->d_block_counts) || !dev_alloc(B, px * 4, (void**)&h->d_pixel_error) ||
+      !dev_alloc(B, blocks * 4, (void**)&h->d_block_error)) {
+    dev_free_all(h->adaptive_bufs);
+    err = "device memory exhausted for the adaptive sampling's error planes";
+    return false;
+  }
+  h->adaptive_px = px;
+  h->adaptive_blocks = blocks;
+  return true;
+}
+
+uint32_t rpx_render_adaptive(uint32_t handle, uint32_t cap, uint64_t* out8) {
+  if (!out8) return RPX_NULL_POINTER;
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  hipSetDevice(h->device);
+  if (cap == 0) cap = h->host.max_samples;
+  if (cap == 0) {
+    set_error("rpx_render_adaptive: neither a cap nor the scene's max_samples bounds the rounds");
+    return RPX_BAD_TILE;
+  }
+  std::string err;
+  u32 W = h->dev.width, H = h->dev.height;
+  adaptive::State ad;
+  ad.init(W, H, h->host.noise_threshold, h->host.exposure_ev, cap);
+  size_t px = (size_t)W * H, blocks = ad.counts.size();
+  if (!ensure_film(h.get(), err) || !ensure_adaptive(h.get(), px, blocks, err)) {
+    set_error(err);
+    return RPX_GPU_MEMORY;
+  }
+  hipStream_t st = h->stream;
+  const Film& F = h->film;
+  hipMemsetAsync(F.rgb, 0, px * 12, st);
+  hipMemsetAsync(F.weight, 0, px * 4, st);
+  hipMemsetAsync(F.alpha, 0, px * 4, st);
+  hipMemsetAsync(F.lum2, 0, px * 4, st);
+  h->engine->textured = h->dev.tex_material != NO_TEXTURE;
+  hipEvent_t ev[2];
+  hipEventCreate(&ev[0]);
+  hipEventCreate(&ev[1]);
+  hipEventRecord(ev[0], st);
+  u64 decisions = 0, rounds = 0, paths = 0;
+  adaptive::Decision last{};
+  std::vector<float> block_error(blocks);
+  std::vector<DevSegment> segs;
+  hipError_t e = hipSuccess;
+  for (;;) {
+    if (ad.left() == 0) {
+      // adaptive.rs:517-539: the round's samples are counted, then the next round is chosen.
+      for (u32 b = 0; b < blocks; ++b) ad.counts[b] += ad.quota(b);
+      ad.iteration += ad.round;
+      ad.done = 0;
+      e = hipMemcpyAsync(h->d_block_counts, ad.counts.data(), blocks * 4, hipMemcpyHostToDevice, st);
+      if (e == hipSuccess)
+        e = adaptive::block_errors(W, H, F.rgb, 3, F.lum2, h->d_block_counts, ad.floor, h->d_pixel_error,
+                                   h->d_block_error, st);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(block_error.data(), h->d_block_error, blocks * 4, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) break;
+      last = adaptive::choose(W, H, ad.counts, block_error, ad.target, ad.cap, ad.active);
+      ad.round = last.round;
+      ++decisions;
+      if (ad.round == 0) break;
+    }
+    u32 count = ad.left();
+    // Every block with a quota renders its next samples; the blocks' pixels are disjoint.
+    segs.clear();
+    for (u32 b = 0; b < blocks; ++b) {
+      u32 q = ad.quota(b);
+      if (q == 0) continue;
+      u32 bx = b % ad.cols, by = b / ad.cols;
+      DevSegment s{};
+      s.scene = 0;
+      s.x0 = bx * adaptive::BLOCK;
+      s.y0 = by * adaptive::BLOCK;
+      s.w = std::min(adaptive::BLOCK, W - s.x0);
+      s.h = std::min(adaptive::BLOCK, H - s.y0);
+      s.first = ad.counts[b];
+      s.count = std::min(q, count);
+      s.film = F;
+      s.film_w = W;
+      segs.push_back(s);
+      paths += (u64)s.w * s.h * s.count;
+    }
+    e = h->engine->run(h->d_scene, segs, true, err, true);
+    if (e != hipSuccess) break;
+    ad.done = std::min(ad.done + count, ad.round);
+    ++rounds;
+  }
+  hipEventRecord(ev[1], st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  float ms = 0;
+  if (e == hipSuccess) hipEventElapsedTime(&ms, ev[0], ev[1]);
+  hipEventDestroy(ev[0]);
+  hipEventDestroy(ev[1]);
+  if (e != hipSuccess) {
+    set_error(err.empty() ? std::string("rpx_render_adaptive: ") + hipGetErrorString(e) : err);
+    return e == hipErrorOutOfMemory ? RPX_GPU_MEMORY : RPX_GPU_LAUNCH;
+  }
+  h->block_counts = ad.counts;
+  h->samples = 0;
+  u32 lo = ~0u, hi = 0;
+  for (u32 c : ad.counts) {
+    lo = std::min(lo, c);
+    hi = std::max(hi, c);
+  }
+  u32 bits[2];
+  std::memcpy(&bits[0], &last.max_error, 4);
+  std::memcpy(&bits[1], &last.remaining, 4);
+  out8[0] = decisions;
+  out8[1] = rounds;
+  out8[2] = paths;
+  out8[3] = blocks ? lo : 0;
+  out8[4] = hi;
+  out8[5] = bits[0];
+  out8[6] = bits[1];
+  out8[7] = (u64)((double)ms * 1.0e6);
+  return RPX_OK;
+}
+
+uint32_t rpx_adaptive_counts(uint32_t handle, uint32_t* counts, uint32_t cap) {
+  if (!counts) return RPX_NULL_POINTER;
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  if (cap < h->block_counts.size()) return RPX_BAD_TILE;
+  std::memcpy(counts, h->block_counts.data(), h->block_counts.size() * 4);
+  return RPX_OK;
+}
+
+uint32_t rt_adaptive_decide(uint32_t width, uint32_t height, float ev, float target, uint32_t cap,
+                            const uint8_t* counts_ptr, const uint8_t* sums_ptr, const uint8_t* lum2_ptr,
+                            uint8_t* active_ptr, uint8_t* decision_ptr) {
+  if (!counts_ptr || !sums_ptr || !lum2_ptr || !active_ptr || !decision_ptr) return RPX_NULL_POINTER;
+  if (width == 0 || height == 0 || (u64)width * height > (1ull << 28)) return RPX_BAD_TILE;
+  if (rpx_device_count() == 0) {
+    set_error("rt_adaptive_decide: no HIP device is available");
+    return RPX_GPU_MEMORY;
+  }
+  u32 cols = (width + adaptive::BLOCK - 1) / adaptive::BLOCK, rows = (height + adaptive::BLOCK - 1) / adaptive::BLOCK;
+  size_t blocks = (size_t)cols * rows, px = (size_t)width * height;
+  // The planes are little-endian words with no alignment promised.
+  std::vector<u32> counts(blocks);
+  std::memcpy(counts.data(), counts_ptr, blocks * 4);
+  std::vector<DevBuf> bufs;
+  float *d_sums = nullptr, *d_lum2 = nullptr, *d_error = nullptr, *d_block_error = nullptr;
+  u32* d_counts = nullptr;
+  if (!dev_alloc(bufs, px * 16, (void**)&d_sums) || !dev_alloc(bufs, px * 4, (void**)&d_lum2) ||
+      !dev_alloc(bufs, px * 4, (void**)&d_error) || !dev_alloc(bufs, blocks * 4, (void**)&d_block_error) ||
+      !dev_alloc(bufs, blocks * 4, (void**)&d_counts)) {
+    dev_free_all(bufs);
+    set_error("rt_adaptive_decide: device memory exhausted");
+    return RPX_GPU_MEMORY;
+  }
+  hipStream_t st = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_sums, sums_ptr, px * 16, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_lum2, lum2_ptr, px * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_counts, counts.data(), blocks * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = adaptive::block_errors(width, height, d_sums, 4, d_lum2, d_counts, adaptive::floor_for(ev), d_error,
+                               d_block_error, st);
+  std::vector<float> block_error(blocks);
+  if (e == hipSuccess) e = hipMemcpyAsync(block_error.data(), d_block_error, blocks * 4, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (st) hipStreamDestroy(st);
+  dev_free_all(bufs);
+  if (e != hipSuccess) {
+    set_error(std::string("rt_adaptive_decide: ") + hipGetErrorString(e));
+    return RPX_GPU_LAUNCH;
+  }
+  std::vector<uint8_t> active;
+  adaptive::Decision d = adaptive::choose(width, height, counts, block_error, target, cap, active);
+  u32 words[6] = {d.round, d.active, d.blocks, 0, 0, 0};
+  std::memcpy(&words[3], &d.threshold, 4);
+  std::memcpy(&words[4], &d.max_error, 4);
+  std::memcpy(&words[5], &d.remaining, 4);
+  std::memcpy(active_ptr, active.data(), blocks);
+  std::memcpy(decision_ptr, words, sizeof(words));
+  return RPX_OK;
 }
 
 uint32_t rpx_render_samples(uint32_t handle, uint32_t first, uint32_t count) {

@@ -64,8 +64,10 @@ class Engine {
   ~Engine();
   bool init(int device, hipStream_t stream, std::string& err);
   // Renders every segment. With `splat`, each segment's samples are added to its film in sample
-  // order; without, the results of a single chunk stay in `P.results` (work id order).
-  hipError_t run(const DevScene* d_scenes, const std::vector<DevSegment>& segs, bool splat, std::string& err);
+  // order; without, the results of a single chunk stay in `P.results` (work id order). `disjoint`
+  // promises that no two segments share a film pixel: their splats then run in one launch.
+  hipError_t run(const DevScene* d_scenes, const std::vector<DevSegment>& segs, bool splat, std::string& err,
+                 bool disjoint = false);
   // Path replay backpropagation: for each chunk, the forward pass (radiance per path sample into
   // the results buffer), then the replay that accumulates d loss / d parameter into
   // grad_mat (n_mat x 5) and grad_tex (tex_w x tex_h x 3), device buffers the caller zeroes. The
@@ -95,7 +97,7 @@ class Engine {
   bool ensure_results(u32 n, std::string& err);
   bool ensure_segs(u32 n, std::string& err);
   hipError_t run_chunk(const DevScene* d_scenes, const std::vector<DevSegment>& chunk, u32 total, bool splat,
-                       bool adj = false);
+                       bool adj = false, bool disjoint = false);
   hipEvent_t next_event();
 
   int device_ = 0;

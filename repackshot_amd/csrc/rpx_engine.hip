@@ -190,7 +190,7 @@ bool Engine::ray_counts(u64 out[3]) {
 }
 
 hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegment>& chunk, u32 total, bool splat,
-                             bool adj) {
+                             bool adj, bool disjoint) {
   hipError_t e = hipMemcpyAsync(segs_.p, chunk.data(), chunk.size() * sizeof(DevSegment), hipMemcpyHostToDevice, stream_);
   if (e != hipSuccess) return e;
   Launch L;
@@ -257,7 +257,10 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegm
   }
   if (exec) hipGraphExecDestroy(exec);
   if (e != hipSuccess) return e;
-  if (splat)
+  if (splat && disjoint && chunk.size() > 1) {
+    e = launch_splat_segments(P, L.segs, L.n_segs, stream_);
+    if (e != hipSuccess) return e;
+  } else if (splat)
     for (const DevSegment& s : chunk) {
       e = launch_splat(P, s, stream_);
       if (e != hipSuccess) return e;
@@ -311,7 +314,8 @@ bool Engine::plan(const std::vector<DevSegment>& segs, std::vector<std::vector<D
   return true;
 }
 
-hipError_t Engine::run(const DevScene* d_scenes, const std::vector<DevSegment>& segs, bool splat, std::string& err) {
+hipError_t Engine::run(const DevScene* d_scenes, const std::vector<DevSegment>& segs, bool splat, std::string& err,
+                       bool disjoint) {
   timing = Timing{};
   events_used_ = 0;
   marks_.clear();
@@ -331,7 +335,7 @@ hipError_t Engine::run(const DevScene* d_scenes, const std::vector<DevSegment>& 
     return hipErrorOutOfMemory;
   hipEventRecord(ev_[0], stream_);
   hipError_t e = hipSuccess;
-  for (size_t c = 0; c < chunks.size() && e == hipSuccess; ++c) e = run_chunk(d_scenes, chunks[c], totals[c], splat);
+  for (size_t c = 0; c < chunks.size() && e == hipSuccess; ++c) e = run_chunk(d_scenes, chunks[c], totals[c], splat, false, disjoint);
   hipEventRecord(ev_[1], stream_);
   if (e == hipSuccess) e = hipStreamSynchronize(stream_);
   if (e != hipSuccess) {

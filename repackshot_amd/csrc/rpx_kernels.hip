@@ -25,6 +25,7 @@
 // scale, tie-break), geometry/triangle.rs:113-154 (watertight test), geometry/mod.rs:682-808
 // (hit details, terminator lift), bevel.rs:232-252 (shading normal), render.rs:79-97 and
 // scene/mod.rs:294-357 (camera), image.rs:81-94 (film).
+#include <algorithm>
 #include "rpx_kernels.h"
 
 namespace rpx {
@@ -1595,32 +1596,46 @@ __global__ void __launch_bounds__(256) k_shade(const DevScene* __restrict__ scen
 // Adds samples first .. first + count of each pixel of the segment to its film, in sample order
 // (image.rs:81-94): the sums are carried in registers and stored once, which performs the same f32
 // additions in the same order as adding sample by sample.
+// Adds samples 0..count of pixel p of segment s to its film pixel in sample order.
+__device__ __forceinline__ void splat_pixel(const PathState& P, const DevSegment& s, u32 p) {
+  u32 n = s.w * s.h;
+  u32 px = s.x0 + p % s.w, py = s.y0 + p / s.w;
+  size_t f = (size_t)py * s.film_w + px;
+  float r = s.film.rgb[3 * f], g = s.film.rgb[3 * f + 1], b = s.film.rgb[3 * f + 2];
+  float wt = s.film.weight[f], al = s.film.alpha[f], l2 = s.film.lum2[f];
+  const float4* res = P.results + s.base + p;
+  for (u32 k = 0; k < s.count; ++k) {
+    float4 v = res[(size_t)k * n];
+    C3 c = c3(v.x, v.y, v.z) * splat(1.0f);
+    r += c.r;
+    g += c.g;
+    b += c.b;
+    wt += 1.0f;
+    al += v.w;
+    float l = luminance(c);
+    l2 += l * l;
+  }
+  s.film.rgb[3 * f] = r;
+  s.film.rgb[3 * f + 1] = g;
+  s.film.rgb[3 * f + 2] = b;
+  s.film.weight[f] = wt;
+  s.film.alpha[f] = al;
+  s.film.lum2[f] = l2;
+}
+
 __global__ void __launch_bounds__(256) k_splat(PathState P, DevSegment s) {
   u32 n = s.w * s.h;
   u32 stride = gridDim.x * blockDim.x;
-  for (u32 p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
-    u32 px = s.x0 + p % s.w, py = s.y0 + p / s.w;
-    size_t f = (size_t)py * s.film_w + px;
-    float r = s.film.rgb[3 * f], g = s.film.rgb[3 * f + 1], b = s.film.rgb[3 * f + 2];
-    float wt = s.film.weight[f], al = s.film.alpha[f], l2 = s.film.lum2[f];
-    const float4* res = P.results + s.base + p;
-    for (u32 k = 0; k < s.count; ++k) {
-      float4 v = res[(size_t)k * n];
-      C3 c = c3(v.x, v.y, v.z) * splat(1.0f);
-      r += c.r;
-      g += c.g;
-      b += c.b;
-      wt += 1.0f;
-      al += v.w;
-      float l = luminance(c);
-      l2 += l * l;
-    }
-    s.film.rgb[3 * f] = r;
-    s.film.rgb[3 * f + 1] = g;
-    s.film.rgb[3 * f + 2] = b;
-    s.film.weight[f] = wt;
-    s.film.alpha[f] = al;
-    s.film.lum2[f] = l2;
+  for (u32 p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) splat_pixel(P, s, p);
+}
+
+// Many small segments whose pixels are disjoint (the blocks of an adaptive round): one wave per
+// segment, in one launch.
+__global__ void __launch_bounds__(64) k_splat_segments(PathState P, const DevSegment* __restrict__ segs, u32 n_segs) {
+  for (u32 i = blockIdx.x; i < n_segs; i += gridDim.x) {
+    DevSegment s = segs[i];
+    u32 n = s.w * s.h;
+    for (u32 p = threadIdx.x; p < n; p += 64) splat_pixel(P, s, p);
   }
 }
 
@@ -1874,6 +1889,10 @@ hipError_t launch_shadow(const PathState& P, const Launch& L, bool adj) {
 }
 hipError_t launch_splat(const PathState& P, const DevSegment& s, hipStream_t st) {
   hipLaunchKernelGGL(k_splat, dim3(grid_for(s.w * s.h, 256, 8192)), dim3(256), 0, st, P, s);
+  return hipGetLastError();
+}
+hipError_t launch_splat_segments(const PathState& P, const DevSegment* segs, u32 n_segs, hipStream_t st) {
+  hipLaunchKernelGGL(k_splat_segments, dim3(std::min<u32>(n_segs, 1u << 16)), dim3(64), 0, st, P, segs, n_segs);
   return hipGetLastError();
 }
 hipError_t launch_tile_out(const PathState& P, float* out, u32 n, u32 channels, hipStream_t st) {

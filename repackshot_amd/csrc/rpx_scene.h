@@ -222,6 +222,9 @@ struct HostScene {
   float floor_z = 0;
   C3 floor_color{0.5f, 0.5f, 0.5f};
   u32 width = 640, height = 360, max_samples = 64, seed = 0, max_depth = 8, pixel_filter = 0;
+  float exposure_ev = 0;
+  u32 adaptive = 0;
+  float noise_threshold = 0;
   float clamp_indirect = 0;
 };
 

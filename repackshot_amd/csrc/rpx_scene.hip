@@ -214,9 +214,9 @@ bool parse_rsb(const uint8_t* bytes, size_t len, HostScene& s, std::string& err,
       s.height = r.u();
       s.max_samples = r.u();
       s.seed = r.u();
-      (void)r.f();
-      (void)r.f();
-      (void)r.u();
+      (void)r.f();  // max seconds
+      s.exposure_ev = r.f();
+      (void)r.u();  // tonemap
       if (!r.ok) { err = "RNDR section too short"; return false; }
       s.max_depth = r.opt_u(8);
       s.clamp_indirect = r.opt_f(0);
@@ -227,9 +227,14 @@ bool parse_rsb(const uint8_t* bytes, size_t len, HostScene& s, std::string& err,
       (void)r.opt_f(1);
       s.pixel_filter = r.opt_u(0);
       u32 edge = r.opt_u(1);
-      u32 adaptive = r.opt_u(0);
+      // E10 (rsb.rs:1105-1110): the adaptive bit asks the host for rounds (rpx_render_adaptive).
+      s.adaptive = r.opt_u(0);
+      s.noise_threshold = r.opt_f(0);
       if (photons) { err = "caustics are not supported by the GPU tracer"; return false; }
-      if (adaptive) { err = "adaptive sampling is not supported by the GPU tracer"; return false; }
+      if (!(s.noise_threshold >= 0.0f && s.noise_threshold <= 1.0f)) {
+        err = "noise_threshold must be between 0 and 1";
+        return false;
+      }
       if (s.pixel_filter == 3) { err = "the Blackman-Harris pixel filter is not supported by the GPU tracer"; return false; }
       if (edge > 1) { err = "edge supersampling is not supported by the GPU tracer"; return false; }
       if (s.width < 1 || s.width > 16384 || s.height < 1 || s.height > 16384) {

@@ -39,6 +39,9 @@ _SIGS = {
     "rt_scene_free": (ctypes.c_uint32, [ctypes.c_uint32]),
     "rt_render_tile": (ctypes.c_uint32, [ctypes.c_uint32] * 6 + [ctypes.c_void_p]),
     "rt_render_tile_rgba": (ctypes.c_uint32, [ctypes.c_uint32] * 6 + [ctypes.c_void_p]),
+    "rt_render_block_rgba": (ctypes.c_uint32, [ctypes.c_uint32] * 7 + [ctypes.c_void_p]),
+    "rt_adaptive_decide": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
+                                             ctypes.c_uint32] + [ctypes.c_void_p] * 5),
     "rpx_render_samples": (ctypes.c_uint32, [ctypes.c_uint32] * 3),
     "rpx_film_reset": (ctypes.c_uint32, [ctypes.c_uint32]),
     "rpx_film_read": (ctypes.c_uint32, [ctypes.c_uint32] + [ctypes.c_void_p] * 4),
@@ -68,6 +71,8 @@ _SIGS = {
     "rpx_queue_last_timing": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_queue_stats": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_queue_free": (ctypes.c_uint32, [ctypes.c_uint32]),
+    "rpx_render_adaptive": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
+    "rpx_adaptive_counts": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32]),
     "rpx_device_count": (ctypes.c_uint32, []),
 }
 
@@ -222,6 +227,32 @@ class Scene:
         _check(library().rt_render_tile_rgba(self.handle, x, y, w, h, sample_index, _ptr(out)), "rt_render_tile_rgba")
         return out
 
+    def render_block_rgba(self, x, y, w, h, sample_index, iteration=0):
+        """rt_render_block_rgba: one sample of a block of an adaptive round, as render_tile_rgba."""
+        out = np.zeros((h, w, 4), np.float32)
+        _check(library().rt_render_block_rgba(self.handle, x, y, w, h, sample_index, iteration, _ptr(out)),
+               "rt_render_block_rgba")
+        return out
+
+    def render_adaptive(self, cap=0):
+        """Adaptive sampling in rounds into the resident film (cleared first): 8 x 8 blocks sampled
+        until the scene's noise target or `cap` samples (0: the scene's max_samples). Returns the
+        decisions, rounds, path samples, fewest / most samples of a block, the last decision's
+        largest error and remaining work, and the GPU time (ms)."""
+        out = np.zeros(8, np.uint64)
+        _check(library().rpx_render_adaptive(self.handle, cap, _ptr(out)), "rpx_render_adaptive")
+        bits = out[5:7].astype(np.uint32).view(np.float32)
+        return {"decisions": int(out[0]), "rounds": int(out[1]), "paths": int(out[2]), "min_samples": int(out[3]),
+                "max_samples": int(out[4]), "max_error": float(bits[0]), "remaining": float(bits[1]),
+                "gpu_ms": int(out[7]) / 1.0e6}
+
+    def adaptive_counts(self):
+        """Samples of every 8 x 8 block of the last adaptive render, (rows, cols) uint32."""
+        cols, rows = (self.width + 7) // 8, (self.height + 7) // 8
+        out = np.zeros((rows, cols), np.uint32)
+        _check(library().rpx_adaptive_counts(self.handle, _ptr(out), out.size), "rpx_adaptive_counts")
+        return out
+
     def render_samples(self, first, count):
         """Accumulates samples [first, first + count) of every pixel into the resident film."""
         _check(library().rpx_render_samples(self.handle, first, count), "rpx_render_samples")
@@ -324,6 +355,86 @@ class Scene:
         out = np.zeros(12, np.float32)
         _check(library().rpx_last_timing(self.handle, _ptr(out)), "rpx_last_timing")
         return _timing(out)
+
+
+ADAPTIVE_BLOCK = 8
+ADAPTIVE_WARMUP = 16
+DECISION_WORDS = 6
+
+
+def adaptive_decide(width, height, ev, target, cap, counts, sums, lum2):
+    """rt_adaptive_decide (rt-wasm/src/ffi.rs:237-296): the adaptive decision of a width x height
+    image from its blocks' sample counts, RGBA sums (4 floats per pixel) and squared-luminance sums.
+    Returns one uint8 per block (1: in the next round) and the six decision words (round, active
+    blocks, blocks, then the f32 bits of the threshold, largest error and remaining work)."""
+    cols, rows = (width + ADAPTIVE_BLOCK - 1) // ADAPTIVE_BLOCK, (height + ADAPTIVE_BLOCK - 1) // ADAPTIVE_BLOCK
+    counts = np.ascontiguousarray(counts, np.uint32).reshape(-1)
+    sums = np.ascontiguousarray(sums, np.float32).reshape(-1)
+    lum2 = np.ascontiguousarray(lum2, np.float32).reshape(-1)
+    if counts.size != cols * rows or sums.size != 4 * width * height or lum2.size != width * height:
+        raise ValueError("rt_adaptive_decide: plane sizes do not fit the image")
+    active = np.zeros(cols * rows, np.uint8)
+    words = np.zeros(DECISION_WORDS, np.uint32)
+    _check(library().rt_adaptive_decide(width, height, ev, target, cap, _ptr(counts), _ptr(sums), _ptr(lum2),
+                                        _ptr(active), _ptr(words)), "rt_adaptive_decide")
+    return active, words
+
+
+class Adaptive:
+    """The state of an adaptive render in rounds for a host that renders the blocks itself
+    (rt-core/src/adaptive.rs:319-554): the samples of every 8 x 8 block, the active blocks and the
+    current round, with each decision taken by rt_adaptive_decide on the GPU."""
+
+    def __init__(self, width, height, target, ev, cap):
+        self.width, self.height = width, height
+        self.cols = (width + ADAPTIVE_BLOCK - 1) // ADAPTIVE_BLOCK
+        self.rows = (height + ADAPTIVE_BLOCK - 1) // ADAPTIVE_BLOCK
+        self.counts = np.zeros(self.cols * self.rows, np.uint32)
+        self.active = np.ones(self.cols * self.rows, np.uint8)
+        self.round = min(ADAPTIVE_WARMUP, cap) if cap > 0 else ADAPTIVE_WARMUP
+        self.done = 0
+        self.iteration = 0
+        self.target, self.ev, self.cap = max(float(target), 0.0), float(ev), int(cap)
+
+    def left(self):
+        return self.round - self.done
+
+    def converged(self):
+        return self.round == 0
+
+    def advance(self, count):
+        self.done = min(self.done + count, self.round)
+
+    def quotas(self):
+        """adaptive.rs:421-430: each block's samples in the current round."""
+        q = np.where(self.active != 0, self.round, 0).astype(np.int64)
+        if self.cap > 0:
+            q = np.minimum(q, np.maximum(self.cap - self.counts.astype(np.int64), 0))
+        return q.astype(np.uint32)
+
+    def samples_of_pixel(self, x, y):
+        b = (y // ADAPTIVE_BLOCK) * self.cols + x // ADAPTIVE_BLOCK
+        return int(self.counts[b]) + min(int(self.quotas()[b]), self.done)
+
+    def decide(self, sums, stride, lum2):
+        """Counts the round's samples and chooses the next round (adaptive.rs:517-539) from the
+        image's sums (`stride` floats per pixel, RGB first). Returns the decision (round, active,
+        blocks, threshold, largest error, remaining, as float32) and the chosen blocks."""
+        self.counts += self.quotas()
+        self.iteration += self.round
+        self.done = 0
+        n = self.width * self.height
+        planes = np.asarray(sums, np.float32).reshape(n, stride)
+        rgba = np.zeros((n, 4), np.float32)
+        rgba[:, :min(stride, 4)] = planes[:, :4]
+        active, words = adaptive_decide(self.width, self.height, self.ev, self.target, self.cap, self.counts, rgba,
+                                        lum2)
+        self.active = active
+        self.round = int(words[0])
+        out = np.zeros(6, np.float32)
+        out[:3] = words[:3]
+        out[3:] = words[3:].view(np.float32)
+        return out, active
 
 
 def _tune(out):

@@ -49,17 +49,18 @@ def synthetic(ad, w, h, rnd):
     return sums, lum2
 
 
-def synthetic_decisions(rounds, decide):
+def synthetic_decisions(rounds, make):
     """Six rounds of decisions on a 45 x 37 synthetic image (target 2 %, exposure +0.5, cap 200),
     hashed: each round's size, active blocks, block count, threshold, largest error and remaining
-    work, and the chosen blocks. `decide(ad, sums, stride, lum2)` returns (out6, active)."""
+    work, and the chosen blocks. `make(w, h, target, ev, cap)` gives the adaptive state, whose
+    `decide(sums, stride, lum2)` returns (out6, active)."""
     w, h = 45, 37
-    ad = orc.Adaptive(w, h, 0.02, 0.5, 200)
+    ad = make(w, h, 0.02, 0.5, 200)
     data = bytearray()
     for rnd in range(rounds):
         ad.advance(ad.left())
         sums, lum2 = synthetic(ad, w, h, rnd)
-        out, active = decide(ad, sums, 4, lum2)
+        out, active = ad.decide(sums, 4, lum2)
         data += struct.pack("<3I", int(out[0]), int(out[1]), int(out[2]))
         data += np.asarray(out[3:6], "<f4").tobytes()
         data += bytes(active.astype(np.uint8))
@@ -69,7 +70,7 @@ def synthetic_decisions(rounds, decide):
 
 
 def test_synthetic_decisions_have_the_pinned_hash():
-    assert synthetic_decisions(6, lambda ad, s, st, l: ad.decide(s, st, l)) == PINNED_E10_DECISIONS
+    assert synthetic_decisions(6, orc.Adaptive) == PINNED_E10_DECISIONS
 
 
 def test_pixel_error_of_known_sums():
