@@ -13,7 +13,11 @@ one persistent pool beside the same jobs rendered one at a time in sample passes
 The adjoint record times the path-replay gradients of the inverse-material fit's scene against
 central finite differences of the same forward renderer.
 
+The adaptive record renders the same scene adaptively (8 x 8 blocks in rounds to a 3 % noise target
+or 64 samples) with the decisions on the GPU between rounds.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-queue] [--no-adjoint]
+                       [--no-adaptive]
                        [--pool P --shape S --shadow-shape T]   (fixed launch options instead of the tuner)
 """
 import argparse
@@ -210,6 +214,33 @@ def adjoint_bench(native, seed=0, ranges=8, step=1.0e-2):
                          "adjoint_stderr_over_fd_stderr": round(float(np.median(sa / sf)), 3)}}
 
 
+def adaptive_bench(native, rsb, tuned, target=0.03, cap=64):
+    """Adaptive sampling on the same render: 8 x 8 blocks in rounds to a 3 % noise target (the
+    reference's E10 example target) or 64 samples, the film resident in HBM and the decisions
+    (per-pixel errors, smoothing, block maxima) on the GPU between rounds."""
+    from repackshot_amd.rsb import with_adaptive
+
+    scene = native.Scene(with_adaptive(rsb, target))
+    scene.set_options(pool=tuned["pool"], shape=tuned["shape"], shadow_shape=tuned["shadow_shape"])
+    W, H = scene.width, scene.height
+    scene.render_adaptive(cap)  # warm-up (pool and planes allocated)
+    t0 = time.perf_counter()
+    st = scene.render_adaptive(cap)
+    wall = time.perf_counter() - t0
+    counts = scene.adaptive_counts()
+    scene.close()
+    uniform = W * H * cap
+    rec = {"noise_target": target, "cap": cap, "decisions": st["decisions"], "rounds": st["rounds"],
+           "path_samples": st["paths"], "uniform_path_samples": uniform,
+           "fraction_of_uniform": round(st["paths"] / uniform, 4),
+           "block_samples_min": st["min_samples"], "block_samples_max": st["max_samples"],
+           "blocks_at_cap": int((counts >= cap).sum()), "blocks": int(counts.size),
+           "last_max_error": round(st["max_error"], 5), "wall_s": round(wall, 4), "gpu_ms": round(st["gpu_ms"], 2),
+           "mpaths_per_s": round(st["paths"] / wall / 1e6, 3)}
+    log("bench: adaptive %s" % rec)
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +250,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-queue", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-adaptive", action="store_true")
     ap.add_argument("--pool", type=int, default=0)
     ap.add_argument("--shape", type=int, default=-1)
     ap.add_argument("--shadow-shape", type=int, default=0)
@@ -353,6 +385,8 @@ def main():
             "extend_avg_ms": round(extend_avg_ms, 4),
             "extend_launches_per_step": timing["extend_launches"],
         }
+    if not args.no_adaptive:
+        result["adaptive"] = adaptive_bench(native, rsb, tuned)
     if not args.no_queue:
         result["mixed_queue"] = mixed_queue(native, args.seed)
     if not args.no_adjoint:

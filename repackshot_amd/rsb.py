@@ -80,6 +80,33 @@ class SceneWriter:
         return b"RSB1" + _u(0) + b"".join(self.sections)
 
 
+# The RNDR fields up to pixel_filter and edge_supersample with their defaults (scene/rsb.rs:46-57):
+# width, height, max_samples, seed, max_seconds, exposure_ev, tonemap, max_depth, clamp_indirect,
+# three u32 and a f32 off, denoise_strength 1, pixel_filter 0, edge_supersample 1.
+_RNDR_DEFAULTS = (_u(0, 0, 0, 0) + _f(0.0, 0.0) + _u(1, 8) + _f(0.0) + _u(0, 0, 0) + _f(0.0, 1.0) + _u(0, 1))
+
+
+def with_adaptive(rsb: bytes, noise_threshold: float, width=None, height=None) -> bytes:
+    """The scene bytes with adaptive sampling asked for in the RNDR section (the E10 form: the
+    section grows to 72 bytes with the adaptive bit and the noise target, scene/rsb.rs:57), and
+    optionally another image size. Fields the section did not carry take their defaults."""
+    out = bytearray(rsb)
+    i = 8
+    while i + 8 <= len(out):
+        tag, n = bytes(out[i:i + 4]), struct.unpack("<I", out[i + 4:i + 8])[0]
+        if tag == b"RNDR":
+            body = bytearray(_RNDR_DEFAULTS)
+            old = out[i + 8:i + 8 + n]
+            body[:min(len(old), 64)] = old[:64]
+            if width is not None:
+                body[0:8] = _u(width, height)
+            body += _u(1) + _f(noise_threshold)
+            out[i:i + 8 + n] = _section(b"RNDR", bytes(body))
+            return bytes(out)
+        i += 8 + n
+    raise ValueError("the scene bytes have no RNDR section")
+
+
 def uv_sphere(center, radius, lat=16, lon=32):
     """A UV sphere (positions, normals, indices) in float32."""
     th = np.linspace(0.0, np.pi, lat + 1)

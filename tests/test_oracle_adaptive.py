@@ -91,19 +91,8 @@ def adaptive_scene(target=0.05, width=64, height=48):
     """The convergence tests' glossy scene with adaptive sampling asked for (RNDR adaptive bit and a
     noise target)."""
     import test_gpu_convergence as conv
-    rsb = bytearray(conv.scene())
-    i = 8
-    while i < len(rsb):
-        tag, n = rsb[i:i + 4], struct.unpack("<I", rsb[i + 4:i + 8])[0]
-        if tag == b"RNDR":
-            body = bytearray(rsb[i + 8:i + 8 + n])
-            body[0:8] = struct.pack("<II", width, height)
-            body = bytes(body) + b"\x00" * (64 - len(body)) if len(body) < 64 else bytes(body[:64])
-            body += struct.pack("<If", 1, target)
-            rsb[i:i + 8 + n] = b"RNDR" + struct.pack("<I", len(body)) + body
-            break
-        i += 8 + n
-    return bytes(rsb)
+    from repackshot_amd.rsb import with_adaptive
+    return with_adaptive(conv.scene(), target, width, height)
 
 
 def test_adaptive_pixels_are_plain_means_of_their_own_samples():
