@@ -339,7 +339,7 @@ def main():
         "rays_per_path": round(rays / (paths_per_step * args.steps), 4),
         "launch": tuned,
         "stage_ms_per_step": {k: round(timing[k], 2) for k in ("extend_ms", "shade_ms", "shadow_ms", "regen_ms",
-                                                                "total_ms")},
+                                                                "drain_ms", "total_ms")},
         "iterations_per_step": timing["iterations"],
     }
 

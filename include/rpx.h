@@ -88,10 +88,12 @@ uint32_t rpx_scene_stats(uint32_t handle, uint64_t* out4);
 /* Pool options of a handle's renders: path slots (0 keeps the current), closest-hit and shadow
  * launch shapes (index 0..7; 0xffffffff keeps), iterations per completion check (0 keeps), most
  * path samples per chunk (0 keeps; at most 2^27), flags: bit 0 one sample pass at a time (the
- * pool drains after each pass), bit 1 each group of iterations launched as a captured HIP graph.
- * Films do not depend on any of them. */
+ * pool drains after each pass), bit 1 each group of iterations launched as a captured HIP graph,
+ * bit 2 no drain; tail: once no new path is left, at most this many live paths are finished in
+ * one drain launch, each lane taking its path to its end (0 keeps; default 2^16). Films do not
+ * depend on any of them. */
 uint32_t rpx_set_options(uint32_t handle, uint32_t pool, uint32_t shape, uint32_t shadow_shape, uint32_t group,
-                         uint32_t chunk, uint32_t flags);
+                         uint32_t chunk, uint32_t flags, uint32_t tail);
 /* The launch tuner: renders samples [0, samples) of the image (default 32; the film is untouched)
  * with every closest-hit launch shape (8) at pool sizes 2^20 .. 2^25, then every shadow launch
  * shape at the best of those, keeps the fastest as the handle's options, and reports: pool,
@@ -139,7 +141,7 @@ uint32_t rpx_queue_run(uint32_t queue); /* renders every job once more (films ac
 uint32_t rpx_queue_reset(uint32_t queue); /* clears every job's film */
 uint32_t rpx_queue_film_read(uint32_t queue, uint32_t job, float* rgb, float* weight, float* alpha, float* lum2);
 uint32_t rpx_queue_set_options(uint32_t queue, uint32_t pool, uint32_t shape, uint32_t shadow_shape,
-                               uint32_t group, uint32_t chunk, uint32_t flags);
+                               uint32_t group, uint32_t chunk, uint32_t flags, uint32_t tail);
 /* The tuner over the queue's jobs' first samples (up to `paths` path samples; 0: 2^25). */
 uint32_t rpx_queue_autotune(uint32_t queue, uint32_t paths, float* out60);
 uint32_t rpx_queue_set_profiling(uint32_t queue, uint32_t on);

@@ -231,7 +231,9 @@ std::shared_ptr<Queue> find_queue(u32 q) {
   return it == g_queues.end() ? nullptr : it->second;
 }
 
-void apply_options(Engine& e, u32 pool, u32 shape, u32 shadow_shape, u32 group, u32 chunk, u32 flags) {
+void apply_options(Engine& e, u32 pool, u32 shape, u32 shadow_shape, u32 group, u32 chunk, u32 flags, u32 tail) {
+  if (tail) e.opt.tail = tail;
+  if (flags & 4u) e.opt.tail = 0;
   if (pool) e.opt.pool = std::max<u32>(pool, 64);
   if (shadow_shape < (u32)N_SHAPES) e.opt.shadow_shape = (int)shadow_shape;
   if (chunk) e.opt.chunk = std::min(chunk, RESULTS_MAX);
@@ -250,7 +252,7 @@ void fill_timing(const Engine& e, float* out12) {
   }
   out12[9] = (float)t.iterations;
   out12[10] = (float)t.chunks;
-  out12[11] = 0.0f;
+  out12[11] = t.stage_ms[4];
 }
 }  // namespace
 }  // namespace rpx
@@ -738,11 +740,11 @@ uint32_t rpx_scene_stats(uint32_t handle, uint64_t* out4) {
 }
 
 uint32_t rpx_set_options(uint32_t handle, uint32_t pool, uint32_t shape, uint32_t shadow_shape, uint32_t group,
-                         uint32_t chunk, uint32_t flags) {
+                         uint32_t chunk, uint32_t flags, uint32_t tail) {
   auto h = find(handle);
   if (!h) return RPX_BAD_HANDLE;
   std::lock_guard<std::mutex> lock(h->mutex);
-  apply_options(*h->engine, pool, shape, shadow_shape, group, chunk, flags);
+  apply_options(*h->engine, pool, shape, shadow_shape, group, chunk, flags, tail);
   return RPX_OK;
 }
 
@@ -1098,11 +1100,11 @@ uint32_t rpx_queue_film_read(uint32_t queue, uint32_t job, float* rgb, float* we
 }
 
 uint32_t rpx_queue_set_options(uint32_t queue, uint32_t pool, uint32_t shape, uint32_t shadow_shape,
-                               uint32_t group, uint32_t chunk, uint32_t flags) {
+                               uint32_t group, uint32_t chunk, uint32_t flags, uint32_t tail) {
   auto q = find_queue(queue);
   if (!q) return RPX_BAD_HANDLE;
   std::lock_guard<std::mutex> lock(q->mutex);
-  apply_options(*q->engine, pool, shape, shadow_shape, group, chunk, flags);
+  apply_options(*q->engine, pool, shape, shadow_shape, group, chunk, flags, tail);
   return RPX_OK;
 }
 

@@ -40,13 +40,16 @@ struct EngineOptions {
   u32 chunk = 1u << 27;  // most work ids (path samples) per chunk, at most RESULTS_MAX
   bool per_pass = false;  // one sample pass at a time, the pool drained after each (no regeneration across samples)
   bool graph = false;   // launch each group of iterations as one captured HIP graph
+  // Once no new path is left, at most this many live paths finish in one drain launch (0: the
+  // iteration loop runs to the end).
+  u32 tail = 1u << 16;
 };
 
 // GPU time of the last run, from events on the engine's stream (profiling on).
 struct Timing {
   float total_ms = 0;
-  float stage_ms[4] = {0, 0, 0, 0};  // extend, shade, shadow, regen
-  u32 launches[4] = {0, 0, 0, 0};
+  float stage_ms[5] = {0, 0, 0, 0, 0};  // extend, shade, shadow, regen, drain
+  u32 launches[5] = {0, 0, 0, 0, 0};
   u32 iterations = 0, chunks = 0;
 };
 

@@ -200,6 +200,8 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegm
   L.pool = std::min(opt.pool, total);
   L.shape = opt.shape;
   L.shadow_shape = opt.shadow_shape;
+  // The drain runs the forward vertices only; the replay keeps the loop.
+  L.tail = adj ? 0u : opt.tail;
   L.stream = stream_;
   e = launch_pool_init(P, L, total);
   if (e != hipSuccess) return e;
@@ -221,6 +223,12 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegm
       r = launch_regen(P, L, parity, adj);
       done();
       if (r != hipSuccess) break;
+      if (L.tail) {
+        mark(4);
+        r = launch_finish(P, L, parity);
+        done();
+        if (r != hipSuccess) break;
+      }
       mark(0);
       r = launch_extend(P, L, parity);
       done();

@@ -42,7 +42,8 @@ struct DevSegment {
 };
 
 // Counter slots (u32) of PathState::counts.
-constexpr u32 C_LIVE0 = 0, C_LIVE1 = 1, C_SHADOW = 2, C_DONE = 3, C_CURSOR = 4, C_END = 5, C_IDLE = 6;
+// C_TAIL: paths handed to the drain kernel by the last bookkeeping (0: none).
+constexpr u32 C_LIVE0 = 0, C_LIVE1 = 1, C_SHADOW = 2, C_DONE = 3, C_CURSOR = 4, C_END = 5, C_IDLE = 6, C_TAIL = 7;
 constexpr u32 N_COUNTS = 16;
 
 // One entry per path slot.
@@ -101,6 +102,7 @@ struct Launch {
   u32 pool;   // path slots in use (grid bound of the pool-sized kernels)
   int shape;         // closest-hit launch shape (index into SHAPES)
   int shadow_shape;  // any-hit launch shape
+  u32 tail;          // once no new path is left, at most this many live paths are drained in one launch (0: never)
   hipStream_t stream;
 };
 
@@ -110,6 +112,8 @@ hipError_t launch_regen(const PathState& P, const Launch& L, u32 parity, bool ad
 hipError_t launch_extend(const PathState& P, const Launch& L, u32 parity);
 hipError_t launch_shade(const PathState& P, const Launch& L, u32 parity, bool adj = false);
 hipError_t launch_shadow(const PathState& P, const Launch& L, bool adj = false);
+// The drain: when the last bookkeeping handed paths over (C_TAIL), takes each to its end.
+hipError_t launch_finish(const PathState& P, const Launch& L, u32 parity);
 // Adds the chunk's samples of segment `seg` (host copy `s`) to its film, in sample order.
 hipError_t launch_splat(const PathState& P, const DevSegment& s, hipStream_t st);
 // Splats every segment of a device table in one launch; the segments' pixels must not overlap.

@@ -594,8 +594,10 @@ __global__ void __launch_bounds__(256) k_regen(const DevScene* __restrict__ scen
 
 // The counters after a regen: the live queue holds survivors + new paths, the cursor moves past
 // the new paths, and the queues this iteration's shade and shadow fill start empty. An empty live
-// queue marks the pool idle (every path of the chunk done).
-__global__ void k_bookkeep(PathState P, u32 parity) {
+// queue marks the pool idle (every path of the chunk done). Once no new path is left and at most
+// `tail` paths are live, they go to the drain kernel instead (C_TAIL) and this iteration's extend,
+// shade and shadow find empty queues.
+__global__ void k_bookkeep(PathState P, u32 parity, u32 tail) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   u32 n = P.counts[C_DONE], end = P.counts[C_END], cursor = P.counts[C_CURSOR];
   u32 fresh = new_paths(n, cursor, end);
@@ -605,8 +607,14 @@ __global__ void k_bookkeep(PathState P, u32 parity) {
   P.counts[C_DONE] = 0;
   P.counts[C_LIVE0 + (parity ^ 1)] = 0;
   P.counts[C_SHADOW] = 0;
+  P.counts[C_TAIL] = 0;
   P.rays[0] += live;
-  if (live == 0) P.counts[C_IDLE] = 1;
+  if (live == 0) {
+    P.counts[C_IDLE] = 1;
+  } else if (live <= tail && cursor + fresh >= end) {
+    P.counts[C_TAIL] = live;
+    P.counts[C_LIVE0 + parity] = 0;
+  }
 }
 
 // ================================================================== extend
@@ -1592,11 +1600,96 @@ __global__ void __launch_bounds__(256) k_shade(const DevScene* __restrict__ scen
   }
 }
 
+// ================================================================== drain
+// The last paths of a chunk, once no new path is left: launched iteration by iteration, every
+// bounce would wait for the slowest ray of the whole pool (a few rays in thousands walk hundreds
+// of nodes). Here each lane takes its path to its end in one launch (closest hit, shade, shadow
+// ray, again), so a path waits only for its own rays and those of its wave. The vertices run in
+// the loop's order on the same per-slot state, so every path's radiance is the loop's, bit for
+// bit; the radiance goes to the results as the regen would put it there.
+template <int BLK, int LDSN>
+__global__ void __launch_bounds__(BLK) k_finish(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
+  const u32 n = P.counts[C_TAIL];
+  if (n == 0) return;
+  __shared__ u32 lds[LDSN * BLK];
+  u32* col = lds + threadIdx.x;
+  const u32* q = P.live[parity];
+  u32 stride = gridDim.x * blockDim.x;
+  // Whole waves iterate together (the shade's scene passes are wave operations).
+  u32 n_iter = (n + stride - 1) / stride;
+  u32 gid = blockIdx.x * blockDim.x + threadIdx.x;
+  u32 nearest = 0, shadow = 0;
+  for (u32 it = 0; it < n_iter; ++it) {
+    u32 j = it * stride + gid;
+    bool active = j < n;
+    u32 i = 0, sid = 0;
+    if (active) {
+      i = q[j];
+      sid = P.ident[i].y;
+    }
+    // The first closest-hit ray was counted by the bookkeeping that handed the path over.
+    bool counted = true;
+    while (__ballot(active)) {
+      if (active) {
+        float4 o = P.ray_o[i], d = P.ray_d[i];
+        bool primary = (f2u(P.med[i].w) & FLAG_PRIMARY) != 0;
+        Geo g = geo_of(scenes, sid);
+        u32 inst = 0xffffffffu, tri = 0xffffffffu;
+        float t = RPX_INF;
+        trace_nearest<BLK, LDSN>(g, col, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), RAY_EPSILON, RPX_INF, primary, inst,
+                                 tri, t);
+        P.hit[i] = make_uint4(inst, tri, f2u(t), 0u);
+        if (!counted) ++nearest;
+        counted = false;
+      }
+      bool cont = false, want_shadow = false;
+      VertexGrad g_prev, g_now;
+      no_grad(g_prev);
+      no_grad(g_now);
+      bool todo = active;
+      u64 pending = __ballot(todo);
+      while (pending) {
+        u32 cur = lowest_lane_value(pending, sid);
+        if (todo && sid == cur) {
+          shade_path<false>(scenes[cur], P, i, cont, want_shadow, g_prev, g_now);
+          todo = false;
+        }
+        pending = __ballot(todo);
+      }
+      if (want_shadow) {
+        ++shadow;
+        float4 o = P.sh_o[i], d = P.sh_d[i];
+        Geo g = geo_of(scenes, sid);
+        if (!trace_any<BLK, LDSN>(g, col, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), RAY_EPSILON, o.w)) {
+          float4 c = P.sh_c[i];
+          float4 s = P.sum[i];
+          P.sum[i] = make_float4(s.x + c.x, s.y + c.y, s.z + c.z, s.w);
+        }
+      }
+      if (active && !cont) {
+        P.results[P.ident[i].x] = P.sum[i];
+        P.ident[i] = make_uint4(NO_WORK, 0u, 0u, 0u);
+        active = false;
+      }
+    }
+  }
+  // One atomic per wave and counter.
+  u32 wn = nearest, ws = shadow;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    wn += __shfl_xor(wn, off);
+    ws += __shfl_xor(ws, off);
+  }
+  if (__lane_id() == 0) {
+    if (wn) atomicAdd((unsigned long long*)&P.rays[0], (unsigned long long)wn);
+    if (ws) atomicAdd((unsigned long long*)&P.rays[2], (unsigned long long)ws);
+  }
+}
+
 // ================================================================== splat / tile output
-// Adds samples first .. first + count of each pixel of the segment to its film, in sample order
+// Adds samples first .. first + count of pixel p of segment s to its film, in sample order
 // (image.rs:81-94): the sums are carried in registers and stored once, which performs the same f32
 // additions in the same order as adding sample by sample.
-// Adds samples 0..count of pixel p of segment s to its film pixel in sample order.
 __device__ __forceinline__ void splat_pixel(const PathState& P, const DevSegment& s, u32 p) {
   u32 n = s.w * s.h;
   u32 px = s.x0 + p % s.w, py = s.y0 + p / s.w;
@@ -1812,7 +1905,13 @@ hipError_t launch_regen(const PathState& P, const Launch& L, u32 parity, bool ad
                        L.n_segs, P, parity);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, L.stream, P, parity);
+  hipLaunchKernelGGL(k_bookkeep, dim3(1), dim3(64), 0, L.stream, P, parity, L.tail);
+  return hipGetLastError();
+}
+hipError_t launch_finish(const PathState& P, const Launch& L, u32 parity) {
+  u32 lanes = L.pool < L.tail ? L.pool : L.tail;
+  hipLaunchKernelGGL((k_finish<64, 32>), dim3(grid_for(lanes, 64, 1u << 16)), dim3(64), 0, L.stream, L.scenes, P,
+                     parity);
   return hipGetLastError();
 }
 

@@ -49,7 +49,7 @@ _SIGS = {
     "rpx_trace_rays": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 4),
     "rpx_scene_info": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_scene_stats": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
-    "rpx_set_options": (ctypes.c_uint32, [ctypes.c_uint32] * 7),
+    "rpx_set_options": (ctypes.c_uint32, [ctypes.c_uint32] * 8),
     "rpx_autotune": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_set_profiling": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
     "rpx_last_timing": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
@@ -65,7 +65,7 @@ _SIGS = {
     "rpx_queue_run": (ctypes.c_uint32, [ctypes.c_uint32]),
     "rpx_queue_reset": (ctypes.c_uint32, [ctypes.c_uint32]),
     "rpx_queue_film_read": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32] + [ctypes.c_void_p] * 4),
-    "rpx_queue_set_options": (ctypes.c_uint32, [ctypes.c_uint32] * 7),
+    "rpx_queue_set_options": (ctypes.c_uint32, [ctypes.c_uint32] * 8),
     "rpx_queue_autotune": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_queue_set_profiling": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
     "rpx_queue_last_timing": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
@@ -79,6 +79,7 @@ _SIGS = {
 KEEP = 0xFFFFFFFF
 FLAG_PER_PASS = 1  # one sample pass at a time (the pool drains after each pass)
 FLAG_GRAPH = 2     # groups of iterations launched as captured HIP graphs
+FLAG_NO_DRAIN = 4  # the iteration loop runs to the end (no drain launch for the last paths)
 
 _lib = None
 
@@ -297,10 +298,12 @@ class Scene:
         _check(library().rpx_scene_stats(self.handle, _ptr(out)), "rpx_scene_stats")
         return {"rays": int(out[0]), "nearest": int(out[1]), "shadow": int(out[2]), "samples": int(out[3])}
 
-    def set_options(self, pool=0, shape=KEEP, shadow_shape=KEEP, group=0, chunk=0, per_pass=False, graph=False):
-        """Path-pool options (0 / KEEP keep the current value); the film does not depend on them."""
-        flags = (FLAG_PER_PASS if per_pass else 0) | (FLAG_GRAPH if graph else 0)
-        _check(library().rpx_set_options(self.handle, pool, shape, shadow_shape, group, chunk, flags),
+    def set_options(self, pool=0, shape=KEEP, shadow_shape=KEEP, group=0, chunk=0, per_pass=False, graph=False,
+                    drain=True, tail=0):
+        """Path-pool options (0 / KEEP keep the current value; the flags per_pass, graph and drain
+        are set by every call); the film does not depend on them."""
+        flags = _flags(per_pass, graph, drain)
+        _check(library().rpx_set_options(self.handle, pool, shape, shadow_shape, group, chunk, flags, tail),
                "rpx_set_options")
 
     def material_params(self):
@@ -437,6 +440,10 @@ class Adaptive:
         return out, active
 
 
+def _flags(per_pass, graph, drain):
+    return (FLAG_PER_PASS if per_pass else 0) | (FLAG_GRAPH if graph else 0) | (0 if drain else FLAG_NO_DRAIN)
+
+
 def _tune(out):
     return {"pool": int(out[0]), "shape": int(out[1]), "shadow_shape": int(out[2]), "best_ms": float(out[3]),
             "ms": [[round(float(v), 3) for v in out[4 + 6 * s:10 + 6 * s]] for s in range(8)],
@@ -447,7 +454,7 @@ def _timing(out):
     return {"total_ms": float(out[0]), "extend_ms": float(out[1]), "shade_ms": float(out[2]),
             "shadow_ms": float(out[3]), "regen_ms": float(out[4]), "extend_launches": int(out[5]),
             "shade_launches": int(out[6]), "shadow_launches": int(out[7]), "regen_launches": int(out[8]),
-            "iterations": int(out[9]), "chunks": int(out[10])}
+            "iterations": int(out[9]), "chunks": int(out[10]), "drain_ms": float(out[11])}
 
 
 class Queue:
@@ -484,9 +491,10 @@ class Queue:
                "rpx_queue_film_read")
         return rgb, weight, alpha, lum2
 
-    def set_options(self, pool=0, shape=KEEP, shadow_shape=KEEP, group=0, chunk=0, per_pass=False, graph=False):
-        flags = (FLAG_PER_PASS if per_pass else 0) | (FLAG_GRAPH if graph else 0)
-        _check(library().rpx_queue_set_options(self.handle, pool, shape, shadow_shape, group, chunk, flags),
+    def set_options(self, pool=0, shape=KEEP, shadow_shape=KEEP, group=0, chunk=0, per_pass=False, graph=False,
+                    drain=True, tail=0):
+        flags = _flags(per_pass, graph, drain)
+        _check(library().rpx_queue_set_options(self.handle, pool, shape, shadow_shape, group, chunk, flags, tail),
                "rpx_queue_set_options")
 
     def autotune(self, paths=0):

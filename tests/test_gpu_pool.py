@@ -1,6 +1,7 @@
 """The path pool and the render queue, through the C ABI: films must not depend on the pool size,
 the traversal launch shape, the chunking of the sample range, per-pass or persistent scheduling,
-or graph launches, and a queue's films must equal each scene rendered on its own and the oracle.
+graph launches or the drain of the last paths, and a queue's films must equal each scene rendered
+on its own and the oracle.
 
 Bar: bit-exact (raw f32 bits of every film sum)."""
 import numpy as np
@@ -39,6 +40,11 @@ def plain(gpu):
     dict(graph=True, group=4),
     dict(pool=3000, graph=True),
     dict(chunk=2 * 256 * 256 + 17),     # chunks of two sample passes
+    dict(drain=False),                  # the iteration loop to the end
+    dict(pool=5000, tail=3000),         # drain after the last regeneration
+    dict(pool=1 << 16, tail=1 << 30),   # the whole chunk drained at once
+    dict(tail=64, shape=5),
+    dict(pool=3000, graph=True, group=2, tail=2000),
 ])
 def test_pool_options_do_not_change_the_film(gpu, plain, opts):
     rsb, want = plain
