@@ -1550,8 +1550,11 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
 }
 }  // namespace
 
+// Three waves per SIMD: the register budget (168) the shade fits without spilling; two waves leave
+// the loads of a path's records unhidden, four spill.
 template <bool ADJ>
-__global__ void __launch_bounds__(256) k_shade(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_shade(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
   const u32* q = P.live[parity];
   u32 n = P.counts[C_LIVE0 + parity];
   u32* out_q = P.live[parity ^ 1];
