@@ -13,11 +13,14 @@ one persistent pool beside the same jobs rendered one at a time in sample passes
 The adjoint record times the path-replay gradients of the inverse-material fit's scene against
 central finite differences of the same forward renderer.
 
+The denoise record times the guide passes and the reference's full denoising filter on the film of
+the timed render.
+
 The adaptive record renders the same scene adaptively (8 x 8 blocks in rounds to a 3 % noise target
 or 64 samples) with the decisions on the GPU between rounds.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-queue] [--no-adjoint]
-                       [--no-adaptive]
+                       [--no-adaptive] [--no-denoise]
                        [--pool P --shape S --shadow-shape T]   (fixed launch options instead of the tuner)
 """
 import argparse
@@ -214,6 +217,49 @@ def adjoint_bench(native, seed=0, ranges=8, step=1.0e-2):
                          "adjoint_stderr_over_fd_stderr": round(float(np.median(sa / sf)), 3)}}
 
 
+# Algorithmic bytes of one a-trous pass per pixel: the guides (ids 8 B, normal + depth 16 B, albedo 16 B,
+# depth gradient 8 B) and the illumination + variance (16 B) read once, the next illumination +
+# variance (16 B) written; the 25 taps' re-reads come from the caches.
+ATROUS_BYTES_PER_PIXEL = 80
+
+
+def denoise_bench(scene, spp, repeats=5):
+    """The passes and the denoiser (the reference's full filter: five a-trous passes guided by the
+    albedo, normal, depth and ids) on the film of the timed render: GPU time of the pass rays (the
+    centre ray and every sample's camera ray of each pixel) and of the filter, and the a-trous
+    kernel's algorithmic bytes per second."""
+    W, H = scene.width, scene.height
+    scene.reset_passes()
+    scene.render_passes(0, spp)  # warm
+    scene.reset_passes()
+    t0 = time.perf_counter()
+    scene.render_passes(0, spp)
+    t_passes = time.perf_counter() - t0
+    scene.denoise_on_device(0, 1.0, spp)  # warm (work buffers)
+    walls, gpu = [], []
+    for _ in range(repeats):
+        t1 = time.perf_counter()
+        scene.denoise_on_device(0, 1.0, spp)
+        walls.append(time.perf_counter() - t1)
+        gpu.append(scene.denoise_timing())
+    best = min(range(repeats), key=lambda i: gpu[i]["total_ms"])
+    t = gpu[best]
+    atrous_ms = t["atrous_ms"] / max(t["levels"], 1)
+    achieved = ATROUS_BYTES_PER_PIXEL * W * H / (atrous_ms * 1e-3) / 1e9
+    rec = {"workload": "full filter on the %dx%d film of %d spp" % (W, H, spp),
+           "passes_ms": round(1000.0 * t_passes, 3),
+           "pass_rays_per_s_m": round(W * H * (spp + 1) / t_passes / 1e6, 1),
+           "denoise_ms": round(t["total_ms"], 4), "denoise_wall_ms": round(1000.0 * min(walls), 4),
+           "mpixels_per_s": round(W * H / (t["total_ms"] * 1e-3) / 1e6, 1),
+           "stage_ms": {k: round(t[k], 4) for k in ("setup_ms", "firefly_ms", "variance_ms", "atrous_ms", "final_ms")},
+           "atrous_pass_ms": round(atrous_ms, 4),
+           "atrous_roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(achieved / HBM_PEAK_GBS, 4),
+                               "bytes_per_pixel": ATROUS_BYTES_PER_PIXEL}}
+    log("bench: denoise %s" % rec)
+    return rec
+
+
 def adaptive_bench(native, rsb, tuned, target=0.03, cap=64):
     """Adaptive sampling on the same render: 8 x 8 blocks in rounds to a 3 % noise target (the
     reference's E10 example target) or 64 samples, the film resident in HBM and the decisions
@@ -251,6 +297,7 @@ def main():
     ap.add_argument("--no-queue", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-adaptive", action="store_true")
+    ap.add_argument("--no-denoise", action="store_true")
     ap.add_argument("--pool", type=int, default=0)
     ap.add_argument("--shape", type=int, default=-1)
     ap.add_argument("--shadow-shape", type=int, default=0)
@@ -385,6 +432,8 @@ def main():
             "extend_avg_ms": round(extend_avg_ms, 4),
             "extend_launches_per_step": timing["extend_launches"],
         }
+    if not args.no_denoise:
+        result["denoise"] = denoise_bench(scene, spp)
     if not args.no_adaptive:
         result["adaptive"] = adaptive_bench(native, rsb, tuned)
     if not args.no_queue:

@@ -24,8 +24,25 @@
  *                         threshold, the largest error and the remaining work; status 2 for an
  *                         empty image). The per-pixel errors, their 3 x 3 smoothing and the
  *                         per-block maxima run on the GPU.
+ *   rt_render_tile_passes ffi.rs:303-338 (the exact passes of `mask` of a tile, from one ray
+ *                         through each pixel's centre: normal 2 (3 f32), depth 4, part id 8 (u32),
+ *                         material id 16 (u32), circle of confusion 32, planes one after another
+ *                         in bit order, little-endian words; pixels outside the image background)
+ *   rt_render_tile_albedo ffi.rs:340-380 (w*h*3 floats: the base colour of what sample
+ *                         `sample_index`'s camera ray meets first; 0 for the background)
+ *   rt_render_tile_alpha  ffi.rs:382-420 (w*h floats: 1 where that ray meets an instance)
+ *   rt_denoise            ffi.rs:600-636 (the denoiser, full filter, on a band of `rows` rows from
+ *                         `origin_y` of a width x height image of `samples` samples: in_ptr holds
+ *                         width*rows*14 little-endian words, the planes sums (RGBA), lum2, albedo
+ *                         (3), normal (3), depth, part id, material id; writes the denoised RGBA
+ *                         sums of rows y0..y1; status 5 when the sizes, the band or its halo do
+ *                         not fit)
+ *   rt_denoise_with       ffi.rs:638-687 (preset 0 full, 1 viewport; strength 0..1 blends the
+ *                         noisy rows with the denoised ones, 0 runs no filter)
+ *   rt_denoise_halo       ffi.rs:689-692 (rows a band holds around its rows: 72 full, 24 viewport)
  * Status codes (rt-wasm/src/lib.rs:36-51): 0 OK, 1 bad handle, 2 empty or oversized tile
- * (more than 2048 x 2048 pixels), 3 null pointer, 4 bad scene update; codes from 16 on are the
+ * (more than 2048 x 2048 pixels), 3 null pointer, 4 bad scene update, 5 bad denoise input; codes
+ * from 16 on are the
  * GPU's (16 no device or device memory exhausted, 17 kernel launch or execution failure).
  * Output buffers belong to the caller and need no alignment.
  *
@@ -48,6 +65,7 @@ extern "C" {
 #define RPX_BAD_TILE 2u
 #define RPX_NULL_POINTER 3u
 #define RPX_BAD_SCENE 4u
+#define RPX_BAD_DENOISE 5u
 #define RPX_GPU_MEMORY 16u
 #define RPX_GPU_LAUNCH 17u
 
@@ -67,6 +85,18 @@ uint32_t rt_render_block_rgba(uint32_t handle, uint32_t x, uint32_t y, uint32_t 
 uint32_t rt_adaptive_decide(uint32_t width, uint32_t height, float ev, float target, uint32_t cap,
                             const uint8_t* counts_ptr, const uint8_t* sums_ptr, const uint8_t* lum2_ptr,
                             uint8_t* active_ptr, uint8_t* decision_ptr);
+uint32_t rt_render_tile_passes(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t h, uint32_t mask,
+                               uint8_t* out_ptr);
+uint32_t rt_render_tile_albedo(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t h, uint32_t sample_index,
+                               float* out_ptr);
+uint32_t rt_render_tile_alpha(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t h, uint32_t sample_index,
+                              float* out_ptr);
+uint32_t rt_denoise(uint32_t width, uint32_t height, uint32_t origin_y, uint32_t rows, uint32_t samples, uint32_t y0,
+                    uint32_t y1, const uint8_t* in_ptr, float* out_ptr);
+uint32_t rt_denoise_with(uint32_t preset, float strength, uint32_t width, uint32_t height, uint32_t origin_y,
+                         uint32_t rows, uint32_t samples, uint32_t y0, uint32_t y1, const uint8_t* in_ptr,
+                         float* out_ptr);
+uint32_t rt_denoise_halo(uint32_t preset);
 
 /* Renders samples [first, first + count) of every pixel into the handle's resident film. */
 uint32_t rpx_render_samples(uint32_t handle, uint32_t first, uint32_t count);
@@ -159,6 +189,25 @@ uint32_t rpx_queue_free(uint32_t queue);
  * row; status 2 when `cap` words do not hold them). */
 uint32_t rpx_render_adaptive(uint32_t handle, uint32_t cap, uint64_t* out8);
 uint32_t rpx_adaptive_counts(uint32_t handle, uint32_t* counts, uint32_t cap);
+/* Passes and the denoiser with everything resident in HBM. rpx_render_passes writes the exact
+ * passes of every pixel (centre rays) into the handle's pass film and adds the albedo and coverage
+ * of samples [first, first + count) to its sums in sample order (rt-core/src/render.rs:500-640);
+ * rpx_passes_reset clears the sums; rpx_passes_read copies the resolved planes of `mask` (bits as
+ * rt_render_tile_passes, plus albedo 1 and alpha 64) one after another in bit order.
+ * rpx_denoise_film denoises the resident film (its sums of `samples` samples; 0: the samples
+ * rendered since the last reset) with the resident passes, preset and strength as rt_denoise_with,
+ * into W*H*4 RGBA sums (out may be null: the result stays on the device); rpx_denoise_timing: total
+ * ms, then setup, firefly, variance, a-trous and final ms, levels, 0. rpx_denoise_counted is
+ * rt_denoise_with over 15 words per pixel: the 14 planes and each pixel's own sample count (f32),
+ * the input of an adaptive image (rt-wasm/src/lib.rs:369-475, DENOISE_WORDS_COUNTED). */
+uint32_t rpx_render_passes(uint32_t handle, uint32_t first, uint32_t count);
+uint32_t rpx_passes_reset(uint32_t handle);
+uint32_t rpx_passes_read(uint32_t handle, uint32_t mask, uint8_t* out);
+uint32_t rpx_denoise_film(uint32_t handle, uint32_t preset, float strength, uint32_t samples, float* out);
+uint32_t rpx_denoise_timing(uint32_t handle, float* out8);
+uint32_t rpx_denoise_counted(uint32_t preset, float strength, uint32_t width, uint32_t height, uint32_t origin_y,
+                             uint32_t rows, uint32_t samples, uint32_t y0, uint32_t y1, const uint8_t* in_ptr,
+                             float* out_ptr);
 /* Number of HIP devices visible (0 without a GPU). */
 uint32_t rpx_device_count(void);
 

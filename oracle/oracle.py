@@ -48,6 +48,8 @@ def lib():
         L.orc_fmath.restype = f32
         L.orc_fmath.argtypes = [ctypes.c_int, f32, f32]
         L.orc_pcg32.argtypes = [u64, u64, vp, ctypes.c_size_t]
+        L.orc_fnv1a.restype = u32
+        L.orc_fnv1a.argtypes = [vp, ctypes.c_size_t]
         L.orc_mix64.restype = u64
         L.orc_mix64.argtypes = [u64]
         L.orc_sobol.restype = u32
@@ -77,6 +79,11 @@ def lib():
         L.orc_adaptive_pixel_error.argtypes = [f32, f32, f32, f32, u32, f32]
         L.orc_render_adaptive.restype = u32
         L.orc_render_adaptive.argtypes = [vp, u32, ctypes.c_int, vp, vp, vp, vp, vp]
+        L.orc_render_passes.restype = u64
+        L.orc_render_passes.argtypes = [vp, u32, u32, ctypes.c_int] + [vp] * 7
+        L.orc_denoise.restype = ctypes.c_int
+        L.orc_denoise.argtypes = [u32] * 9 + [f32] + [vp] * 9
+        L.orc_denoise_synthetic.argtypes = [u32, u32, u32, f32, f32, f32] + [vp] * 7
         os.makedirs(os.path.dirname(CACHE), exist_ok=True)
         L.orc_set_energy_cache(CACHE.encode())
         _lib = L
@@ -164,6 +171,16 @@ class OracleScene:
         counts = np.zeros(blocks, np.uint32)
         n = lib().orc_render_adaptive(self.h, cap, threads, *(_p(a) for a in film), _p(counts))
         return film, counts, int(n)
+
+    def render_passes(self, first, count, threads=None):
+        """The resolved passes of the whole image (exact planes from the centre rays, albedo and
+        alpha over samples [first, first + count)): a dict of planes in the reference's key order."""
+        if threads is None:
+            threads = os.cpu_count() or 1
+        H, W = self.height, self.width
+        planes = new_passes(W, H)
+        lib().orc_render_passes(self.h, first, count, threads, *(_p(planes[k]) for k in PASS_KEYS))
+        return planes
 
     def material_params(self):
         n = lib().orc_material_params(self.h, None, 0)
@@ -266,6 +283,12 @@ def pcg32(seed, stream, n):
     return out
 
 
+def fnv1a(values):
+    """FNV-1a over the little-endian bytes of an array (the reference's image and pass hashes)."""
+    a = np.ascontiguousarray(values)
+    return int(lib().orc_fnv1a(_p(a), a.nbytes))
+
+
 def mix64(z):
     return int(lib().orc_mix64(z))
 
@@ -280,3 +303,63 @@ def owen(x, seed):
 
 def sampler_get1(seed, x, y, index, dim):
     return np.float32(lib().orc_sampler_get1(seed, x, y, index, dim))
+
+
+# The pass planes (rt-core/src/passes.rs:76-87), in the order of their bits.
+PASS_KEYS = ["albedo", "normal", "depth", "partId", "materialId", "coc", "alpha"]
+PASS_CHANNELS = [3, 3, 1, 1, 1, 1, 1]
+FLOOR_ID = 0xFFFFFFFF
+
+
+def new_passes(width, height):
+    """Background passes of a width x height image (depth +inf, the rest zero)."""
+    out = {}
+    for k, c in zip(PASS_KEYS, PASS_CHANNELS):
+        dt = np.uint32 if k in ("partId", "materialId") else np.float32
+        shape = (height, width, c) if c > 1 else (height, width)
+        out[k] = np.zeros(shape, dt)
+    out["depth"][:] = np.inf
+    return out
+
+
+# Presets of the denoiser (rt-post/src/denoise.rs:76-90): (levels, variance radius).
+FULL = (5, 3)
+VIEWPORT = (3, 1)
+
+
+def denoise(width, height, origin_y, rows, samples, y0, y1, planes, preset=FULL, strength=1.0, counts=None):
+    """denoise_rows_at (rt-post/src/denoise.rs:386-407) over a band: `planes` holds sums (RGBA, 4
+    per pixel), lum2, albedo, normal, depth, partId, materialId of `rows` rows. Returns (status,
+    RGBA sums of rows y0..y1); status 0 ok, 1 shape, 2 rows."""
+    def arr(k, dt):
+        return np.ascontiguousarray(planes[k], dt)
+    sums, lum2 = arr("sums", np.float32), arr("lum2", np.float32)
+    albedo, normal, depth = arr("albedo", np.float32), arr("normal", np.float32), arr("depth", np.float32)
+    part, material = arr("partId", np.uint32), arr("materialId", np.uint32)
+    n = width * rows
+    for a, c in ((sums, 4), (lum2, 1), (albedo, 3), (normal, 3), (depth, 1), (part, 1), (material, 1)):
+        if a.size != n * c:
+            return 1, None
+    cnt = None if counts is None else np.ascontiguousarray(counts, np.float32)
+    if cnt is not None and cnt.size != n:
+        return 1, None
+    out = np.zeros(max(width * max(y1 - y0, 0) * 4, 1), np.float32)
+    st = lib().orc_denoise(width, height, origin_y, rows, samples, y0, y1, preset[0], preset[1], strength, _p(sums),
+                           _p(lum2), _p(albedo), _p(normal), _p(depth), _p(part), _p(material), _p(cnt), _p(out))
+    if st != 0:
+        return int(st), None
+    return 0, out[:width * (y1 - y0) * 4].reshape(y1 - y0, width, 4)
+
+
+def denoise_synthetic(width, height, samples, left, right, spread):
+    """The reference tests' synthetic film (rt-post/src/denoise/tests.rs:22-73) as denoiser planes."""
+    n = width * height
+    planes = {"sums": np.zeros((height, width, 4), np.float32), "lum2": np.zeros((height, width), np.float32),
+              "albedo": np.zeros((height, width, 3), np.float32), "normal": np.zeros((height, width, 3), np.float32),
+              "depth": np.zeros((height, width), np.float32), "partId": np.zeros((height, width), np.uint32),
+              "materialId": np.zeros((height, width), np.uint32)}
+    assert n > 0
+    lib().orc_denoise_synthetic(width, height, samples, left, right, spread,
+                                *(_p(planes[k]) for k in ("sums", "lum2", "albedo", "normal", "depth", "partId",
+                                                          "materialId")))
+    return planes

@@ -10,6 +10,8 @@
 #include <cstring>
 #include <thread>
 #include "rt_adaptive.hpp"
+#include "rt_denoise.hpp"
+#include "rt_passes.hpp"
 #include "rt_render.hpp"
 
 using namespace orc;
@@ -238,6 +240,12 @@ void orc_pcg32(uint64_t seed, uint64_t stream, uint32_t* out, size_t n) {
   for (size_t i = 0; i < n; ++i) out[i] = r.next_u32();
 }
 uint64_t orc_mix64(uint64_t z) { return mix64(z); }
+// FNV-1a over bytes (render.rs:981-989 hashes the little-endian bytes of f32 planes this way).
+uint32_t orc_fnv1a(const uint8_t* bytes, size_t n) {
+  uint32_t h = 0x811c9dc5u;
+  for (size_t i = 0; i < n; ++i) h = (h ^ bytes[i]) * 0x01000193u;
+  return h;
+}
 uint32_t orc_sobol(uint32_t dim, uint32_t index) { return sobol(dim, index); }
 uint32_t orc_owen(uint32_t x, uint32_t seed) { return owen_scramble(x, seed); }
 float orc_sampler_get1(uint32_t seed, uint32_t x, uint32_t y, uint32_t index, uint32_t dim) {
@@ -465,6 +473,122 @@ uint32_t orc_render_adaptive(void* p, uint32_t cap, int threads, float* rgb, flo
   std::memcpy(lum2, film.lum2.data(), film.lum2.size() * 4);
   std::memcpy(block_counts, ad.counts().data(), ad.counts().size() * 4);
   return decisions;
+}
+
+// The passes of the whole image as the reference's tests assemble them
+// (rt-core/src/render/passes_tests.rs:18-37): per 32-pixel tile, the exact passes from the centre
+// rays, then the albedo and alpha of samples [first, first + count) in index order, resolved.
+// Planes (caller-sized, any may be null): albedo 3N, normal 3N, depth N, part N, material N, coc N,
+// alpha N. `threads` workers claim tiles from one counter. Returns the rays cast.
+uint64_t orc_render_passes(void* p, uint32_t first, uint32_t count, int threads, float* albedo, float* normal,
+                           float* depth, uint32_t* part, uint32_t* material, float* coc, float* alpha) {
+  auto* hd = static_cast<Handle*>(p);
+  const Scene& s = hd->scene;
+  u32 W = s.render.width;
+  std::vector<TileRect> tiles = tile_grid(s.render.width, s.render.height, 32);
+  std::atomic<size_t> next{0};
+  std::atomic<uint64_t> rays{0};
+  auto worker = [&]() {
+    for (;;) {
+      size_t t = next.fetch_add(1);
+      if (t >= tiles.size()) return;
+      TileRect tile = tiles[t];
+      PassFilm film;
+      film.init(tile.x, tile.y, tile.w, tile.h, PASS_ALL);
+      uint64_t r = render_tile_exact(s, hd->prep, tile, film);
+      for (u32 k = 0; k < count; ++k) r += render_tile_albedo(s, hd->prep, tile, first + k, film);
+      Passes ps = film.resolve();
+      for (u32 j = 0; j < tile.h; ++j)
+        for (u32 i = 0; i < tile.w; ++i) {
+          size_t src = (size_t)j * tile.w + i, dst = (size_t)(tile.y + j) * W + tile.x + i;
+          for (int c = 0; c < 3; ++c) {
+            if (albedo) albedo[3 * dst + c] = ps.albedo[3 * src + c];
+            if (normal) normal[3 * dst + c] = ps.normal[3 * src + c];
+          }
+          if (depth) depth[dst] = ps.depth[src];
+          if (part) part[dst] = ps.part_id[src];
+          if (material) material[dst] = ps.material_id[src];
+          if (coc) coc[dst] = ps.coc[src];
+          if (alpha) alpha[dst] = ps.alpha[src];
+        }
+      rays += r;
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 0; t < std::max(threads, 1); ++t) pool.emplace_back(worker);
+  for (auto& t : pool) t.join();
+  return rays.load();
+}
+
+// The denoiser (rt-post/src/denoise.rs:386-407, denoise_rows_at) on a band of `rows` rows from
+// `origin_y` of a width x height image: rows y0..y1 of RGBA sums into `out` (width * (y1 - y0) * 4
+// floats). `levels` and `radius` give the preset (clamped as the reference clamps); `counts` may be
+// null. Returns 0, 1 (shape) or 2 (rows).
+int orc_denoise(uint32_t width, uint32_t height, uint32_t origin_y, uint32_t rows, uint32_t samples, uint32_t y0,
+                uint32_t y1, uint32_t levels, uint32_t radius, float strength, const float* sums, const float* lum2,
+                const float* albedo, const float* normal, const float* depth, const uint32_t* part,
+                const uint32_t* material, const float* counts, float* out) {
+  denoise::Input in;
+  in.width = width;
+  in.height = height;
+  in.origin_y = origin_y;
+  in.rows = rows;
+  in.samples = samples;
+  in.counts = counts;
+  in.sums = sums;
+  in.lum2 = lum2;
+  in.albedo = albedo;
+  in.normal = normal;
+  in.depth = depth;
+  in.part_id = part;
+  in.material_id = material;
+  std::vector<float> v;
+  denoise::Status st = denoise::denoise_rows_at(in, y0, y1, denoise::Preset{levels, radius}, strength, v);
+  if (st != denoise::OK) return (int)st;
+  std::memcpy(out, v.data(), v.size() * 4);
+  return 0;
+}
+
+// The reference tests' synthetic film (rt-post/src/denoise/tests.rs:22-73): the top row
+// background (a gradient), the left half part 1 around `left`, the right half part 2 around
+// `right`, exponential-like noise of relative `spread` from Pcg32(7, 11). Planes as orc_denoise.
+void orc_denoise_synthetic(uint32_t width, uint32_t height, uint32_t samples, float left, float right, float spread,
+                           float* sums, float* lum2, float* albedo, float* normal, float* depth, uint32_t* part,
+                           uint32_t* material) {
+  size_t n = (size_t)width * height;
+  std::memset(sums, 0, n * 16);
+  std::memset(lum2, 0, n * 4);
+  std::memset(albedo, 0, n * 12);
+  std::memset(normal, 0, n * 12);
+  std::memset(part, 0, n * 4);
+  std::memset(material, 0, n * 4);
+  for (size_t p = 0; p < n; ++p) depth[p] = INF;
+  Pcg32 rng = Pcg32::seeded(7, 11);
+  for (u32 y = 0; y < height; ++y)
+    for (u32 x = 0; x < width; ++x) {
+      size_t p = (size_t)y * width + x;
+      if (y == 0) {
+        for (int k = 0; k < 3; ++k) sums[p * 4 + k] = ((float)x * 0.01f + (float)k) * (float)samples;
+        sums[p * 4 + 3] = (float)samples;
+        continue;
+      }
+      bool left_side = x < width / 2;
+      u32 id = left_side ? 1 : 2;
+      float base = left_side ? left : right;
+      part[p] = id;
+      material[p] = id;
+      for (int k = 0; k < 3; ++k) albedo[p * 3 + k] = 0.8f;
+      normal[p * 3 + 2] = 1.0f;
+      depth[p] = 500.0f;
+      for (u32 s = 0; s < samples; ++s) {
+        float u = fmax_r(rng.next_f32(), 1.0e-6f);
+        float v = base * fmax_r(1.0f + spread * (-lnf_(u) - 1.0f), 0.0f);
+        for (int k = 0; k < 3; ++k) sums[p * 4 + k] += v;
+        sums[p * 4 + 3] += 1.0f;
+        float l = 0.2126f * v + 0.7152f * v + 0.0722f * v;
+        lum2[p] += l * l;
+      }
+    }
 }
 
 }  // extern "C"

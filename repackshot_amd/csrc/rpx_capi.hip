@@ -9,6 +9,7 @@
 #include <thread>
 #include "../../include/rpx.h"
 #include "rpx_adaptive.h"
+#include "rpx_denoise.h"
 #include "rpx_engine.h"
 
 namespace rpx {
@@ -31,6 +32,7 @@ struct Handle {
   Film film{};
   std::vector<DevBuf> film_bufs;
   u64 samples = 0;
+  u64 film_samples = 0;  // samples per pixel in the film since it was last cleared
   // base-colour texture of one material
   std::vector<DevBuf> tex_bufs;
   // adaptive sampling: samples per block of the last adaptive render, device scratch
@@ -39,9 +41,18 @@ struct Handle {
   size_t adaptive_px = 0, adaptive_blocks = 0;
   u32* d_block_counts = nullptr;
   float *d_pixel_error = nullptr, *d_block_error = nullptr;
+  // resident pass film (sums of the sampled passes, exact planes), the resolved albedo the
+  // denoiser reads, the denoiser's work buffers and its output (RGBA sums)
+  PassPlanes passes{};
+  float* pass_albedo = nullptr;
+  float4* denoised = nullptr;
+  std::vector<DevBuf> pass_bufs;
+  std::unique_ptr<denoise::Denoiser> denoiser;
 
   ~Handle() {
     if (stream) hipStreamSynchronize(stream);
+    denoiser.reset();
+    dev_free_all(pass_bufs);
     engine.reset();
     dev_free_all(scene_bufs);
     dev_free_all(film_bufs);
@@ -317,11 +328,16 @@ uint32_t rt_scene_update(uint32_t handle, const uint8_t* ptr, uint32_t len) {
     set_error(err);
     return s;
   }
-  // A new scene starts a new film.
+  // A new scene starts a new film and new passes.
   hipStreamSynchronize(h->stream);
   dev_free_all(h->film_bufs);
   h->film = Film{};
   h->samples = 0;
+  h->film_samples = 0;
+  dev_free_all(h->pass_bufs);
+  h->passes = PassPlanes{};
+  h->pass_albedo = nullptr;
+  h->denoised = nullptr;
   return RPX_OK;
 }
 
@@ -514,6 +530,7 @@ uint32_t rpx_render_adaptive(uint32_t handle, uint32_t cap, uint64_t* out8) {
   }
   h->block_counts = ad.counts;
   h->samples = 0;
+  h->film_samples = 0;
   u32 lo = ~0u, hi = 0;
   for (u32 c : ad.counts) {
     lo = std::min(lo, c);
@@ -613,6 +630,7 @@ uint32_t rpx_render_samples(uint32_t handle, uint32_t first, uint32_t count) {
     return e == hipErrorOutOfMemory ? RPX_GPU_MEMORY : RPX_GPU_LAUNCH;
   }
   h->samples += count;
+  h->film_samples += count;
   return RPX_OK;
 }
 
@@ -631,6 +649,7 @@ uint32_t rpx_film_reset(uint32_t handle) {
   hipMemsetAsync(h->film.weight, 0, px * 4, h->stream);
   hipMemsetAsync(h->film.alpha, 0, px * 4, h->stream);
   hipMemsetAsync(h->film.lum2, 0, px * 4, h->stream);
+  h->film_samples = 0;
   return hipStreamSynchronize(h->stream) == hipSuccess ? RPX_OK : RPX_GPU_LAUNCH;
 }
 
@@ -1199,6 +1218,393 @@ uint32_t rpx_queue_free(uint32_t queue) {
   }
   std::lock_guard<std::mutex> lock(q->mutex);
   hipSetDevice(q->device);
+  return RPX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- passes and the denoiser
+namespace rpx {
+namespace {
+
+constexpr u32 PASS_NORMAL = 2, PASS_DEPTH = 4, PASS_PART_ID = 8, PASS_MATERIAL_ID = 16, PASS_COC = 32,
+              PASS_ALBEDO = 1, PASS_ALPHA = 64, PASS_ALL = 127;
+constexpr u32 PASS_EXACT = PASS_ALL & ~(PASS_ALBEDO | PASS_ALPHA);
+
+const u32* sobol_of(int device) {
+  DeviceTables* t = nullptr;
+  std::string err;
+  return device_tables(device, t, err) ? t->sobol : nullptr;
+}
+
+// Device planes of n pixels: sums (albedo, alpha, weight) zero, the exact planes background
+// (normal 0, depth +inf, ids 0, coc 0).
+bool alloc_passes(std::vector<DevBuf>& B, size_t n, PassPlanes& Q, hipStream_t st) {
+  if (!dev_alloc(B, n * 12, (void**)&Q.albedo) || !dev_alloc(B, n * 4, (void**)&Q.alpha) ||
+      !dev_alloc(B, n * 4, (void**)&Q.weight) || !dev_alloc(B, n * 12, (void**)&Q.normal) ||
+      !dev_alloc(B, n * 4, (void**)&Q.depth) || !dev_alloc(B, n * 4, (void**)&Q.part) ||
+      !dev_alloc(B, n * 4, (void**)&Q.material) || !dev_alloc(B, n * 4, (void**)&Q.coc))
+    return false;
+  hipMemsetAsync(Q.albedo, 0, n * 12, st);
+  hipMemsetAsync(Q.alpha, 0, n * 4, st);
+  hipMemsetAsync(Q.weight, 0, n * 4, st);
+  hipMemsetAsync(Q.normal, 0, n * 12, st);
+  hipMemsetD32Async((hipDeviceptr_t)Q.depth, 0x7f800000, n, st);
+  hipMemsetAsync(Q.part, 0, n * 4, st);
+  hipMemsetAsync(Q.material, 0, n * 4, st);
+  hipMemsetAsync(Q.coc, 0, n * 4, st);
+  return true;
+}
+
+bool ensure_passes(Handle* h, std::string& err) {
+  if (h->passes.albedo) return true;
+  size_t n = (size_t)h->dev.width * h->dev.height;
+  auto& B = h->pass_bufs;
+  if (!alloc_passes(B, n, h->passes, h->stream) || !dev_alloc(B, n * 12, (void**)&h->pass_albedo) ||
+      !dev_alloc(B, n * 16, (void**)&h->denoised)) {
+    dev_free_all(B);
+    h->passes = PassPlanes{};
+    err = "device memory exhausted for the passes";
+    return false;
+  }
+  return true;
+}
+
+// The resolved planes of `mask` (known bits, in the order of their bits) of an n-pixel pass film,
+// one after another as little-endian words, to host memory.
+hipError_t read_planes(const PassPlanes& Q, size_t n, u32 mask, uint8_t* out, hipStream_t st) {
+  std::vector<DevBuf> tmp;
+  float *albedo = nullptr, *alpha = nullptr;
+  if ((mask & (PASS_ALBEDO | PASS_ALPHA)) &&
+      (!dev_alloc(tmp, n * 12, (void**)&albedo) || !dev_alloc(tmp, n * 4, (void**)&alpha))) {
+    dev_free_all(tmp);
+    return hipErrorOutOfMemory;
+  }
+  hipError_t e = hipSuccess;
+  if (albedo) e = launch_passes_resolve(Q, (u32)n, albedo, alpha, st);
+  const void* planes[7] = {albedo, Q.normal, Q.depth, Q.part, Q.material, Q.coc, alpha};
+  const size_t channels[7] = {3, 3, 1, 1, 1, 1, 1};
+  size_t at = 0;
+  for (int k = 0; k < 7 && e == hipSuccess; ++k) {
+    if (!(mask & (1u << k))) continue;
+    e = hipMemcpyAsync(out + at, planes[k], n * channels[k] * 4, hipMemcpyDeviceToHost, st);
+    at += n * channels[k] * 4;
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  dev_free_all(tmp);
+  return e;
+}
+
+// The one-sample albedo (3 per pixel) or coverage of a tile (render.rs:558-640 for one sample).
+uint32_t tile_sampled(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t hh, uint32_t sample_index, float* out,
+                      bool albedo) {
+  if (!out) return RPX_NULL_POINTER;
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  if (w == 0 || hh == 0 || (u64)w * hh > (1ull << 22)) return RPX_BAD_TILE;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  hipSetDevice(h->device);
+  size_t n = (size_t)w * hh;
+  std::vector<DevBuf> tmp;
+  PassPlanes Q{};
+  const u32* sobol = sobol_of(h->device);
+  if (!sobol || !dev_alloc(tmp, n * 12, (void**)&Q.albedo) || !dev_alloc(tmp, n * 4, (void**)&Q.alpha) ||
+      !dev_alloc(tmp, n * 4, (void**)&Q.weight)) {
+    dev_free_all(tmp);
+    set_error("device memory exhausted for the tile's passes");
+    return RPX_GPU_MEMORY;
+  }
+  hipStream_t st = h->stream;
+  hipMemsetAsync(Q.albedo, 0, n * 12, st);
+  hipMemsetAsync(Q.alpha, 0, n * 4, st);
+  hipMemsetAsync(Q.weight, 0, n * 4, st);
+  hipError_t e = launch_passes(h->d_scene, sobol, Q, x, y, w, hh, x, y, w, sample_index, 1, PASSES_SAMPLED, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, albedo ? (void*)Q.albedo : (void*)Q.alpha, n * (albedo ? 12 : 4), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  dev_free_all(tmp);
+  if (e != hipSuccess) {
+    set_error(std::string("passes: ") + hipGetErrorString(e));
+    return RPX_GPU_LAUNCH;
+  }
+  return RPX_OK;
+}
+
+// The device state of the handle-less denoiser calls: one per device, calls serialised.
+struct DenoiseContext {
+  std::mutex mutex;
+  hipStream_t stream = nullptr;
+  denoise::Denoiser dn;
+  DevBuf in, out;
+};
+std::mutex g_dn_mutex;
+std::map<int, std::unique_ptr<DenoiseContext>> g_dn;
+
+DenoiseContext* denoise_context(int dev) {
+  std::lock_guard<std::mutex> lock(g_dn_mutex);
+  auto& c = g_dn[dev];
+  if (!c) {
+    auto fresh = std::make_unique<DenoiseContext>();
+    if (hipStreamCreateWithFlags(&fresh->stream, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    c = std::move(fresh);
+  }
+  return c.get();
+}
+
+bool grow(DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes) return true;
+  if (b.p) hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+  if (hipMalloc(&b.p, std::max<size_t>(bytes, 256)) != hipSuccess) return false;
+  b.bytes = std::max<size_t>(bytes, 256);
+  return true;
+}
+
+bool preset_of(u32 id, denoise::Preset& p) {
+  if (id == 0) p = denoise::FULL;
+  else if (id == 1) p = denoise::VIEWPORT;
+  else return false;
+  return true;
+}
+
+// rt-wasm/src/lib.rs:420-475 and rt-post/src/denoise.rs:322-407: the band's planes (`words` per
+// pixel: 14, or 15 with each pixel's own sample count) in host memory, rows y0..y1 of RGBA sums out.
+uint32_t denoise_band(u32 preset_id, float strength, u32 width, u32 height, u32 origin_y, u32 rows, u32 samples, u32 y0,
+                      u32 y1, const uint8_t* in_ptr, float* out_ptr, u32 words) {
+  if (!in_ptr || !out_ptr) return RPX_NULL_POINTER;
+  denoise::Preset preset;
+  if (!preset_of(preset_id, preset)) return RPX_BAD_DENOISE;
+  preset = preset.clamped();
+  // The band (denoise.rs:264-287) and the rows with their halo (denoise.rs:338-356).
+  if (width == 0 || (u64)origin_y + rows > height) return RPX_BAD_DENOISE;
+  u32 halo = preset.halo(), band_end = origin_y + rows;
+  u32 lo = y0 > halo ? y0 - halo : 0;
+  u64 hi = std::min<u64>((u64)y1 + halo, height);
+  if (y0 > y1 || y0 < origin_y || y1 > band_end || origin_y > lo || band_end < hi) return RPX_BAD_DENOISE;
+  size_t n = (size_t)width * rows, w4 = (size_t)width * 4;
+  if (y1 == y0) return RPX_OK;
+  if (rpx_device_count() == 0) {
+    set_error("rt_denoise: no HIP device is available");
+    return RPX_GPU_MEMORY;
+  }
+  int dev = 0;
+  hipGetDevice(&dev);
+  DenoiseContext* c = denoise_context(dev);
+  if (!c) {
+    set_error("rt_denoise: cannot create a HIP stream");
+    return RPX_GPU_MEMORY;
+  }
+  std::lock_guard<std::mutex> lock(c->mutex);
+  if (!grow(c->in, n * words * 4) || !grow(c->out, n * 16)) {
+    set_error("rt_denoise: device memory exhausted for the band");
+    return RPX_GPU_MEMORY;
+  }
+  hipStream_t st = c->stream;
+  hipError_t e = hipMemcpyAsync(c->in.p, in_ptr, n * words * 4, hipMemcpyHostToDevice, st);
+  const float* f = static_cast<const float*>(c->in.p);
+  denoise::Planes P{};
+  P.rgb = f;
+  P.rgb_stride = 4;
+  P.alpha = f + 3;
+  P.alpha_stride = 4;
+  P.lum2 = f + n * 4;
+  P.albedo = f + n * 5;
+  P.normal = f + n * 8;
+  P.depth = f + n * 11;
+  P.part = reinterpret_cast<const u32*>(f + n * 12);
+  P.material = reinterpret_cast<const u32*>(f + n * 13);
+  P.counts = words > 14 ? f + n * 14 : nullptr;
+  float4* d_out = static_cast<float4*>(c->out.p);
+  if (e == hipSuccess) e = c->dn.run(P, width, rows, samples, preset, strength, d_out, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out_ptr, d_out + (size_t)(y0 - origin_y) * width, (size_t)(y1 - y0) * w4 * 4,
+                       hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    set_error(std::string("rt_denoise: ") + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? RPX_GPU_MEMORY : RPX_GPU_LAUNCH;
+  }
+  return RPX_OK;
+}
+
+}  // namespace
+}  // namespace rpx
+
+extern "C" {
+
+uint32_t rt_render_tile_passes(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t hh, uint32_t mask,
+                               uint8_t* out_ptr) {
+  if (!out_ptr) return RPX_NULL_POINTER;
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  if (w == 0 || hh == 0 || (u64)w * hh > (1ull << 22)) return RPX_BAD_TILE;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  hipSetDevice(h->device);
+  u32 m = mask & PASS_EXACT;
+  if (m == 0) return RPX_OK;
+  size_t n = (size_t)w * hh;
+  std::vector<DevBuf> tmp;
+  PassPlanes Q{};
+  const u32* sobol = sobol_of(h->device);
+  if (!sobol || !alloc_passes(tmp, n, Q, h->stream)) {
+    dev_free_all(tmp);
+    set_error("device memory exhausted for the tile's passes");
+    return RPX_GPU_MEMORY;
+  }
+  hipError_t e = launch_passes(h->d_scene, sobol, Q, x, y, w, hh, x, y, w, 0, 0, PASSES_EXACT, h->stream);
+  if (e == hipSuccess) e = read_planes(Q, n, m, out_ptr, h->stream);
+  dev_free_all(tmp);
+  if (e != hipSuccess) {
+    set_error(std::string("rt_render_tile_passes: ") + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? RPX_GPU_MEMORY : RPX_GPU_LAUNCH;
+  }
+  return RPX_OK;
+}
+
+uint32_t rt_render_tile_albedo(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t h, uint32_t sample_index,
+                               float* out_ptr) {
+  return tile_sampled(handle, x, y, w, h, sample_index, out_ptr, true);
+}
+
+uint32_t rt_render_tile_alpha(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t h, uint32_t sample_index,
+                              float* out_ptr) {
+  return tile_sampled(handle, x, y, w, h, sample_index, out_ptr, false);
+}
+
+uint32_t rt_denoise(uint32_t width, uint32_t height, uint32_t origin_y, uint32_t rows, uint32_t samples, uint32_t y0,
+                    uint32_t y1, const uint8_t* in_ptr, float* out_ptr) {
+  return denoise_band(0, 1.0f, width, height, origin_y, rows, samples, y0, y1, in_ptr, out_ptr, 14);
+}
+
+uint32_t rt_denoise_with(uint32_t preset, float strength, uint32_t width, uint32_t height, uint32_t origin_y,
+                         uint32_t rows, uint32_t samples, uint32_t y0, uint32_t y1, const uint8_t* in_ptr,
+                         float* out_ptr) {
+  return denoise_band(preset, strength, width, height, origin_y, rows, samples, y0, y1, in_ptr, out_ptr, 14);
+}
+
+uint32_t rt_denoise_halo(uint32_t preset) {
+  denoise::Preset p;
+  return preset_of(preset, p) ? p.halo() : 0;
+}
+
+uint32_t rpx_denoise_counted(uint32_t preset, float strength, uint32_t width, uint32_t height, uint32_t origin_y,
+                             uint32_t rows, uint32_t samples, uint32_t y0, uint32_t y1, const uint8_t* in_ptr,
+                             float* out_ptr) {
+  return denoise_band(preset, strength, width, height, origin_y, rows, samples, y0, y1, in_ptr, out_ptr, 15);
+}
+
+uint32_t rpx_render_passes(uint32_t handle, uint32_t first, uint32_t count) {
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  hipSetDevice(h->device);
+  std::string err;
+  const u32* sobol = sobol_of(h->device);
+  if (!sobol || !ensure_passes(h.get(), err)) {
+    set_error(err);
+    return RPX_GPU_MEMORY;
+  }
+  u32 W = h->dev.width, H = h->dev.height;
+  hipError_t e = launch_passes(h->d_scene, sobol, h->passes, 0, 0, W, H, 0, 0, W, first, count,
+                               PASSES_EXACT | (count ? PASSES_SAMPLED : 0u), h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) {
+    set_error(std::string("rpx_render_passes: ") + hipGetErrorString(e));
+    return RPX_GPU_LAUNCH;
+  }
+  return RPX_OK;
+}
+
+uint32_t rpx_passes_reset(uint32_t handle) {
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  hipSetDevice(h->device);
+  std::string err;
+  if (!ensure_passes(h.get(), err)) {
+    set_error(err);
+    return RPX_GPU_MEMORY;
+  }
+  size_t n = (size_t)h->dev.width * h->dev.height;
+  hipMemsetAsync(h->passes.albedo, 0, n * 12, h->stream);
+  hipMemsetAsync(h->passes.alpha, 0, n * 4, h->stream);
+  hipMemsetAsync(h->passes.weight, 0, n * 4, h->stream);
+  return hipStreamSynchronize(h->stream) == hipSuccess ? RPX_OK : RPX_GPU_LAUNCH;
+}
+
+uint32_t rpx_passes_read(uint32_t handle, uint32_t mask, uint8_t* out) {
+  if (!out) return RPX_NULL_POINTER;
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  hipSetDevice(h->device);
+  std::string err;
+  if (!ensure_passes(h.get(), err)) {
+    set_error(err);
+    return RPX_GPU_MEMORY;
+  }
+  hipError_t e = read_planes(h->passes, (size_t)h->dev.width * h->dev.height, mask & PASS_ALL, out, h->stream);
+  return e == hipSuccess ? RPX_OK : (e == hipErrorOutOfMemory ? RPX_GPU_MEMORY : RPX_GPU_LAUNCH);
+}
+
+uint32_t rpx_denoise_film(uint32_t handle, uint32_t preset, float strength, uint32_t samples, float* out) {
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  denoise::Preset p;
+  if (!preset_of(preset, p)) return RPX_BAD_DENOISE;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  hipSetDevice(h->device);
+  std::string err;
+  if (!ensure_film(h.get(), err) || !ensure_passes(h.get(), err)) {
+    set_error(err);
+    return RPX_GPU_MEMORY;
+  }
+  if (!h->denoiser) {
+    h->denoiser = std::make_unique<denoise::Denoiser>();
+    h->denoiser->profiling = true;
+  }
+  u32 W = h->dev.width, H = h->dev.height;
+  size_t n = (size_t)W * H;
+  if (samples == 0) samples = (u32)std::min<u64>(h->film_samples, 0xffffffffu);
+  hipStream_t st = h->stream;
+  hipError_t e = launch_passes_resolve(h->passes, (u32)n, h->pass_albedo, nullptr, st);
+  denoise::Planes P{};
+  P.rgb = h->film.rgb;
+  P.rgb_stride = 3;
+  P.alpha = h->film.alpha;
+  P.alpha_stride = 1;
+  P.lum2 = h->film.lum2;
+  P.albedo = h->pass_albedo;
+  P.normal = h->passes.normal;
+  P.depth = h->passes.depth;
+  P.part = h->passes.part;
+  P.material = h->passes.material;
+  P.counts = nullptr;
+  if (e == hipSuccess) e = h->denoiser->run(P, W, H, samples, p, strength, h->denoised, st);
+  if (e == hipSuccess && out) e = hipMemcpyAsync(out, h->denoised, n * 16, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    set_error(std::string("rpx_denoise_film: ") + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? RPX_GPU_MEMORY : RPX_GPU_LAUNCH;
+  }
+  return RPX_OK;
+}
+
+uint32_t rpx_denoise_timing(uint32_t handle, float* out8) {
+  if (!out8) return RPX_NULL_POINTER;
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  denoise::DenoiseTiming t = h->denoiser ? h->denoiser->timing : denoise::DenoiseTiming{};
+  out8[0] = t.total_ms;
+  out8[1] = t.setup_ms;
+  out8[2] = t.firefly_ms;
+  out8[3] = t.variance_ms;
+  out8[4] = t.atrous_ms;
+  out8[5] = t.final_ms;
+  out8[6] = (float)t.levels;
+  out8[7] = 0.0f;
   return RPX_OK;
 }
 

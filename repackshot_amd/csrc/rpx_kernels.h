@@ -122,6 +122,28 @@ hipError_t launch_splat_segments(const PathState& P, const DevSegment* segs, u32
 hipError_t launch_tile_out(const PathState& P, float* out, u32 n, u32 channels, hipStream_t st);
 hipError_t launch_trace_rays(const DevScene* scene, const float* rays, u32 n, u32* inst, u32* tri, float* t,
                              uint8_t* any, hipStream_t st);
+// Planes of a pass film in device memory (rt-core/src/passes.rs): sums of the sampled passes (albedo
+// rgb, coverage) with their sample count, and the exact passes of the pixels' centre rays. Planes
+// a launch does not write may be null.
+struct PassPlanes {
+  float* albedo;  // 3 per pixel, sums
+  float* alpha;   // coverage sums
+  float* weight;  // samples summed
+  float* normal;  // 3 per pixel
+  float* depth;
+  u32* part;
+  u32* material;
+  float* coc;
+};
+constexpr u32 PASSES_EXACT = 1, PASSES_SAMPLED = 2;
+constexpr u32 FLOOR_ID = 0xffffffffu;  // part and material id of the studio floor
+// The passes of the pixels of rectangle (x0, y0, w, h) inside the image: `what` PASSES_EXACT
+// writes the exact planes, PASSES_SAMPLED adds samples [first, first + count) to the sums. Pixel
+// (px, py) is at row py - oy, column px - ox of planes `ow` pixels wide.
+hipError_t launch_passes(const DevScene* scene, const u32* sobol, const PassPlanes& Q, u32 x0, u32 y0, u32 w, u32 h,
+                         u32 ox, u32 oy, u32 ow, u32 first, u32 count, u32 what, hipStream_t st);
+// The resolved albedo (3 per pixel) and alpha of n pixels of the pass film (either may be null).
+hipError_t launch_passes_resolve(const PassPlanes& Q, u32 n, float* albedo, float* alpha, hipStream_t st);
 hipError_t launch_energy(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_in, uint16_t* clear_out, float* albedo_avg,
                          float* schlick_avg, hipStream_t st);
 

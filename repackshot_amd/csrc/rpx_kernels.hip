@@ -1768,6 +1768,233 @@ __global__ void __launch_bounds__(128) k_trace_rays(const DevScene* __restrict__
   }
 }
 
+// ================================================================== exact and sampled passes
+namespace {
+constexpr u32 FIRST_BACKGROUND = 0, FIRST_FLOOR = 1, FIRST_SURFACE = 2;
+
+// What a camera ray meets first (integrator/mod.rs:911-990): the background (nothing, or a light
+// panel flagged `camera` in front of everything), the floor, or a surface with its face-forwarded
+// shading normal (bevel.rs:372-377 without a bevel) and base colour.
+struct First {
+  u32 kind, inst;
+  float t;
+  V3 normal;
+  C3 albedo;
+};
+
+template <int BLK, int LDSN>
+__device__ First first_hit(const DevScene& S, u32* lds_col, V3 o, V3 d) {
+  First f;
+  f.kind = FIRST_BACKGROUND;
+  f.inst = 0;
+  f.t = 0.0f;
+  f.normal = v3(0.0f, 0.0f, 0.0f);
+  f.albedo = splat(0.0f);
+  Geo g{S.nodes, S.tris, S.instances, S.tlas_root};
+  u32 hi = 0, ht = 0;
+  float tt = RPX_INF;
+  bool hit = trace_nearest<BLK, LDSN>(g, lds_col, o, d, RAY_EPSILON, RPX_INF, true, hi, ht, tt);
+  float t_geometry = hit ? tt : RPX_INF;
+  float t_floor = RPX_INF;
+  if (S.floor_mode == 1 && d.z != 0.0f) {
+    float tf = (S.floor_z - o.z) / d.z;
+    if (tf > RAY_EPSILON && tf < RPX_INF) t_floor = tf;
+  }
+  float t_nearest = fmin_r(t_geometry, t_floor);
+  if (t_nearest == RPX_INF) return f;
+  for (u32 a = 0; a < S.n_areas; ++a) {
+    const DevArea& L = S.areas[a];
+    float t, cos;
+    if (L.camera && area_intersect(L, o, d, t_nearest, t, cos)) return f;
+  }
+  if (t_floor < t_geometry) {
+    f.kind = FIRST_FLOOR;
+    f.t = t_floor;
+    f.normal = v3(0.0f, 0.0f, d.z < 0.0f ? 1.0f : -1.0f);
+    return f;
+  }
+  // Hit details (geometry/mod.rs:731-808): barycentrics from the scalar re-test, the shading
+  // normal facing the ray, the hit point.
+  const DevInstance& in = S.instances[hi];
+  const DevMesh& mesh = S.meshes[in.mesh];
+  bool ident = (in.flags & INST_IDENTITY) != 0;
+  const u32* idx = S.indices + mesh.index_offset + 3 * (size_t)ht;
+  u32 i0 = mesh.vertex_offset + idx[0], i1 = mesh.vertex_offset + idx[1], i2 = mesh.vertex_offset + idx[2];
+  const float* ps = S.positions;
+  V3 p0 = v3(ps[3 * (size_t)i0], ps[3 * (size_t)i0 + 1], ps[3 * (size_t)i0 + 2]);
+  V3 p1 = v3(ps[3 * (size_t)i1], ps[3 * (size_t)i1 + 1], ps[3 * (size_t)i1 + 2]);
+  V3 p2 = v3(ps[3 * (size_t)i2], ps[3 * (size_t)i2 + 1], ps[3 * (size_t)i2 + 2]);
+  V3 lo = ident ? o : xf_point(in.to_local, o), ld = ident ? d : xf_vector(in.to_local, d);
+  PRay r = prepare(lo, ld, RAY_EPSILON, RPX_INF);
+  float b0, b1, b2;
+  {
+    float u, v, w, az, bz, cz;
+    shear(r, p0 - lo, p1 - lo, p2 - lo, u, v, w, az, bz, cz);
+    bool negative = u < 0.0f || v < 0.0f || w < 0.0f;
+    bool positive = u > 0.0f || v > 0.0f || w > 0.0f;
+    float det = u + v + w;
+    bool ok = !(negative && positive) && det != 0.0f;
+    if (ok) {
+      float t = (u * az + v * bz + w * cz) / det;
+      ok = t >= r.t_min && t <= F32_MAX;
+    }
+    if (ok) {
+      b0 = u / det;
+      b1 = v / det;
+      b2 = w / det;
+    } else {
+      b0 = b1 = b2 = 1.0f / 3.0f;
+    }
+  }
+  V3 wind = cross(p1 - p0, p2 - p0);
+  V3 winding = normalize(ident ? wind : xf_transpose(in.to_local, wind));
+  V3 geometric = length_sq(winding) > 0.0f ? winding : -normalize(d);
+  V3 smooth = v3(0.0f, 0.0f, 0.0f);
+  if (mesh.has_normals) {
+    const float* ns = S.normals;
+    V3 n0 = v3(ns[3 * (size_t)i0], ns[3 * (size_t)i0 + 1], ns[3 * (size_t)i0 + 2]);
+    V3 n1 = v3(ns[3 * (size_t)i1], ns[3 * (size_t)i1 + 1], ns[3 * (size_t)i1 + 2]);
+    V3 n2 = v3(ns[3 * (size_t)i2], ns[3 * (size_t)i2 + 1], ns[3 * (size_t)i2 + 2]);
+    V3 interp = n0 * b0 + n1 * b1 + n2 * b2;
+    smooth = ident ? interp : xf_transpose(in.to_local, interp);
+  }
+  V3 normal = length_sq(smooth) > 1.0e-12f ? normalize(smooth) : geometric;
+  f.kind = FIRST_SURFACE;
+  f.inst = hi;
+  f.t = tt;
+  f.normal = dot(normal, d) > 0.0f ? -normal : normal;
+  u32 mat = in.material;
+  // An index outside the materials renders with the default material (base colour 0.8).
+  C3 base = mat < S.n_materials ? S.materials[mat].base : c3(0.8f, 0.8f, 0.8f);
+  if (mat == S.tex_material && mat < S.n_materials) {
+    V3 point_local = p0 * b0 + p1 * b1 + p2 * b2;
+    V3 hp = ident ? point_local : xf_point(in.to_world, point_local);
+    TexTap tap = texture_tap(S.tex_w, S.tex_h, S.tex_origin_x, S.tex_origin_y, S.tex_scale_x, S.tex_scale_y, hp.x, hp.y);
+    base = texture_value(S.texture, S.tex_w, tap);
+  }
+  f.albedo = base;
+  return f;
+}
+}  // namespace
+
+// Exact passes from the centre ray of each pixel (render.rs:500-556, passes.rs:177-206) and the
+// albedo and coverage of samples [first, first + count) added to their sums in sample-index order
+// (render.rs:558-640, passes.rs:209-235, 333-361), for the pixels of rectangle (x0, y0, w, h)
+// inside the image. Pixel (px, py) is written at row py - oy, column px - ox of planes `ow` wide.
+template <int BLK, int LDSN>
+__global__ void __launch_bounds__(BLK) k_passes(const DevScene* __restrict__ scene, const u32* __restrict__ cols,
+                                                PassPlanes Q, u32 x0, u32 y0, u32 w, u32 h, u32 ox, u32 oy, u32 ow,
+                                                u32 first, u32 count, u32 what) {
+  __shared__ u32 lds[LDSN * BLK];
+  u32* col = lds + threadIdx.x;
+  const DevScene& S = *scene;
+  u32 n = w * h;
+  u32 stride = gridDim.x * blockDim.x;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    u32 px = x0 + i % w, py = y0 + i / w;
+    if (px >= S.width || py >= S.height) continue;
+    size_t o_ix = (size_t)(py - oy) * ow + (px - ox);
+    if (what & PASSES_EXACT) {
+      // centre_ray: the pinhole ray through the pixel's centre and the lens centre.
+      float u = ((float)px + 0.5f) / (float)(S.width > 1 ? S.width : 1);
+      float v = ((float)py + 0.5f) / (float)(S.height > 1 ? S.height : 1);
+      V3 o, d;
+      if (S.ortho) {
+        float wdt = S.ortho_width;
+        o = S.cam_position + S.cam_right * ((u - 0.5f) * wdt) + S.cam_up * ((0.5f - v) * wdt / S.aspect);
+        d = S.cam_forward;
+      } else {
+        float x = (u - 0.5f) * S.sensor_w;
+        float y = (0.5f - v) * S.sensor_h;
+        V3 dir = normalize(S.cam_forward * S.focal_mm + S.cam_right * x + S.cam_up * y);
+        if (S.lens_radius == 0.0f) {
+          o = S.cam_position;
+          d = dir;
+        } else {
+          V3 focus_point = S.cam_position + dir * (S.focus_dist / dot(dir, S.cam_forward));
+          o = S.cam_position + (S.cam_right * 0.0f + S.cam_up * 0.0f) * S.lens_radius;
+          d = normalize(focus_point - o);
+        }
+      }
+      First f = first_hit<BLK, LDSN>(S, col, o, d);
+      V3 nrm = v3(0.0f, 0.0f, 0.0f);
+      float depth = RPX_INF, coc = 0.0f;
+      u32 part = 0, material = 0;
+      if (f.kind != FIRST_BACKGROUND) {
+        if (f.kind == FIRST_FLOOR) {
+          part = FLOOR_ID;
+          material = FLOOR_ID;
+        } else {
+          u32 m = S.instances[f.inst].material;
+          material = m < S.n_materials ? m + 1 : 0;
+          part = f.inst + 1;
+        }
+        nrm = f.normal;
+        depth = f.t * dot(d, S.cam_forward) + dot(o - S.cam_position, S.cam_forward);
+        // coc_diameter (passes.rs:144-153)
+        if (S.lens_radius > 0.0f && depth > 0.0f && is_finite(depth)) {
+          float aperture = S.focal_mm / S.cam_fstop;
+          float on_focus_plane = aperture * fabs_(depth - S.focus_dist) / depth;
+          u32 long_side = S.width > S.height ? S.width : S.height;
+          float pixel = S.cam_sensor_mm / (float)(long_side > 1 ? long_side : 1) * (S.focus_dist / S.focal_mm);
+          coc = on_focus_plane / pixel;
+        }
+      }
+      if (Q.normal) {
+        Q.normal[3 * o_ix] = nrm.x;
+        Q.normal[3 * o_ix + 1] = nrm.y;
+        Q.normal[3 * o_ix + 2] = nrm.z;
+      }
+      if (Q.depth) Q.depth[o_ix] = depth;
+      if (Q.part) Q.part[o_ix] = part;
+      if (Q.material) Q.material[o_ix] = material;
+      if (Q.coc) Q.coc[o_ix] = coc;
+    }
+    if ((what & PASSES_SAMPLED) && count > 0) {
+      C3 sum = c3(Q.albedo[3 * o_ix], Q.albedo[3 * o_ix + 1], Q.albedo[3 * o_ix + 2]);
+      float cover = Q.alpha[o_ix], wsum = Q.weight[o_ix];
+      u64 key = pixel_key(S.seed, px, py);
+      for (u32 k = 0; k < count; ++k) {
+        V3 o, d;
+        camera_ray(S, cols, key, px, py, first + k, o, d);
+        First f = first_hit<BLK, LDSN>(S, col, o, d);
+        C3 a = f.kind == FIRST_BACKGROUND ? splat(0.0f) : (f.kind == FIRST_FLOOR ? S.floor_color : f.albedo);
+        sum = c3(sum.r + a.r, sum.g + a.g, sum.b + a.b);
+        cover += f.kind == FIRST_SURFACE ? 1.0f : 0.0f;
+        wsum += 1.0f;
+      }
+      Q.albedo[3 * o_ix] = sum.r;
+      Q.albedo[3 * o_ix + 1] = sum.g;
+      Q.albedo[3 * o_ix + 2] = sum.b;
+      Q.alpha[o_ix] = cover;
+      Q.weight[o_ix] = wsum;
+    }
+  }
+}
+
+// Resolved passes (passes.rs:388-420): the albedo and coverage sums times the f32 reciprocal of
+// their sample count.
+__global__ void __launch_bounds__(256) k_passes_resolve(PassPlanes Q, u32 n, float* albedo_out, float* alpha_out) {
+  u32 stride = gridDim.x * blockDim.x;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    float w = Q.weight[i];
+    float r = 0.0f, g = 0.0f, b = 0.0f, a = 0.0f;
+    if (w > 0.0f) {
+      float inv = 1.0f / w;
+      r = Q.albedo[3 * (size_t)i] * inv;
+      g = Q.albedo[3 * (size_t)i + 1] * inv;
+      b = Q.albedo[3 * (size_t)i + 2] * inv;
+      a = Q.alpha[i] * (1.0f / w);
+    }
+    if (albedo_out) {
+      albedo_out[3 * (size_t)i] = r;
+      albedo_out[3 * (size_t)i + 1] = g;
+      albedo_out[3 * (size_t)i + 2] = b;
+    }
+    if (alpha_out) alpha_out[i] = a;
+  }
+}
+
 // ================================================================== energy tables (energy.rs recipe)
 namespace {
 __device__ double g2_over_g1(float a2, V3 wo, V3 wi) {
@@ -2004,6 +2231,16 @@ hipError_t launch_tile_out(const PathState& P, float* out, u32 n, u32 channels, 
 hipError_t launch_trace_rays(const DevScene* scene, const float* rays, u32 n, u32* inst, u32* tri, float* t,
                              uint8_t* any, hipStream_t st) {
   hipLaunchKernelGGL(k_trace_rays, dim3(grid_for(n, 128, 8192)), dim3(128), 0, st, scene, rays, n, inst, tri, t, any);
+  return hipGetLastError();
+}
+hipError_t launch_passes(const DevScene* scene, const u32* sobol, const PassPlanes& Q, u32 x0, u32 y0, u32 w, u32 h,
+                         u32 ox, u32 oy, u32 ow, u32 first, u32 count, u32 what, hipStream_t st) {
+  hipLaunchKernelGGL((k_passes<128, 16>), dim3(grid_for(w * h, 128, 1u << 16)), dim3(128), 0, st, scene, sobol, Q, x0,
+                     y0, w, h, ox, oy, ow, first, count, what);
+  return hipGetLastError();
+}
+hipError_t launch_passes_resolve(const PassPlanes& Q, u32 n, float* albedo, float* alpha, hipStream_t st) {
+  hipLaunchKernelGGL(k_passes_resolve, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, Q, n, albedo, alpha);
   return hipGetLastError();
 }
 hipError_t launch_energy(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_in, uint16_t* clear_out, float* albedo_avg,

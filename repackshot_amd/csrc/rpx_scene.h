@@ -94,6 +94,7 @@ struct DevScene {
   // camera
   V3 cam_position, cam_forward, cam_right, cam_up;
   float focal_mm, sensor_w, sensor_h, lens_radius, focus_dist, ortho_width, aspect, exterior;
+  float cam_fstop, cam_sensor_mm;  // for the circle of confusion of the passes
   u32 ortho;
   // render settings
   u32 width, height, seed, max_depth, pixel_filter, background;
@@ -101,6 +102,7 @@ struct DevScene {
   C3 background_color;
   u32 floor_mode;
   float floor_z;
+  C3 floor_color;  // the floor's albedo pass
   Bsdf floor_bsdf;
   Bsdf fallback_bsdf[2];
   // An RGB albedo texture (f32, row-major, bilinear, repeating) replacing the base colour of

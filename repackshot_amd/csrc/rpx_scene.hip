@@ -925,6 +925,8 @@ bool flatten_scene(const HostScene& s, const EnergyTables& tables, FlatScene& ou
   bool through_lens = !c.ortho && c.projection == 0;
   k.lens_radius = (c.fstop > 0.0f && c.focus_dist > 0.0f && through_lens) ? c.focal_mm / c.fstop * 0.5f : 0.0f;
   k.focus_dist = c.focus_dist;
+  k.cam_fstop = c.fstop;
+  k.cam_sensor_mm = c.sensor_mm;
   k.ortho = c.ortho;
   k.ortho_width = c.ortho_width;
   k.aspect = aspect;
@@ -941,6 +943,7 @@ bool flatten_scene(const HostScene& s, const EnergyTables& tables, FlatScene& ou
   k.clamp_indirect = s.clamp_indirect;
   k.floor_mode = s.floor_mode;
   k.floor_z = s.floor_z;
+  k.floor_color = s.floor_color;
   k.floor_bsdf = bsdf_lambert(s.floor_color);
   k.texture = nullptr;
   k.tex_material = NO_TEXTURE;

@@ -24,6 +24,7 @@ STATUS = {
     2: "empty or oversized tile",
     3: "null pointer",
     4: "bad scene",
+    5: "bad denoise input",
     16: "no GPU or GPU memory exhausted",
     17: "GPU kernel launch or execution failure",
 }
@@ -73,10 +74,48 @@ _SIGS = {
     "rpx_queue_free": (ctypes.c_uint32, [ctypes.c_uint32]),
     "rpx_render_adaptive": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_adaptive_counts": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32]),
+    "rt_render_tile_passes": (ctypes.c_uint32, [ctypes.c_uint32] * 6 + [ctypes.c_void_p]),
+    "rt_render_tile_albedo": (ctypes.c_uint32, [ctypes.c_uint32] * 6 + [ctypes.c_void_p]),
+    "rt_render_tile_alpha": (ctypes.c_uint32, [ctypes.c_uint32] * 6 + [ctypes.c_void_p]),
+    "rt_denoise": (ctypes.c_uint32, [ctypes.c_uint32] * 7 + [ctypes.c_void_p] * 2),
+    "rt_denoise_with": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_float] + [ctypes.c_uint32] * 7
+                        + [ctypes.c_void_p] * 2),
+    "rt_denoise_halo": (ctypes.c_uint32, [ctypes.c_uint32]),
+    "rpx_render_passes": (ctypes.c_uint32, [ctypes.c_uint32] * 3),
+    "rpx_passes_reset": (ctypes.c_uint32, [ctypes.c_uint32]),
+    "rpx_passes_read": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
+    "rpx_denoise_film": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32,
+                                           ctypes.c_void_p]),
+    "rpx_denoise_timing": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
+    "rpx_denoise_counted": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_float] + [ctypes.c_uint32] * 7
+                            + [ctypes.c_void_p] * 2),
     "rpx_device_count": (ctypes.c_uint32, []),
 }
 
 KEEP = 0xFFFFFFFF
+
+# The passes (rt-core/src/passes.rs:44-87): bit i of a mask is pass PASS_KEYS[i].
+PASS_KEYS = ["albedo", "normal", "depth", "partId", "materialId", "coc", "alpha"]
+PASS_CHANNELS = [3, 3, 1, 1, 1, 1, 1]
+PASS_ALBEDO, PASS_NORMAL, PASS_DEPTH, PASS_PART_ID, PASS_MATERIAL_ID, PASS_COC, PASS_ALPHA = (1 << i for i in range(7))
+PASS_ALL = (1 << 7) - 1
+PASS_SAMPLED = PASS_ALBEDO | PASS_ALPHA
+PASS_EXACT = PASS_ALL & ~PASS_SAMPLED
+FLOOR_ID = 0xFFFFFFFF
+
+
+def _planes_of(buf, mask, w, h):
+    """Splits little-endian words of the planes of `mask` (bit order) into (h, w[, c]) arrays."""
+    out, at = {}, 0
+    for i, (k, c) in enumerate(zip(PASS_KEYS, PASS_CHANNELS)):
+        if not mask & (1 << i):
+            continue
+        words = buf[at:at + w * h * c]
+        at += w * h * c
+        dt = np.uint32 if k in ("partId", "materialId") else np.float32
+        a = words.view(dt).copy()
+        out[k] = a.reshape((h, w, c) if c > 1 else (h, w))
+    return out
 FLAG_PER_PASS = 1  # one sample pass at a time (the pool drains after each pass)
 FLAG_GRAPH = 2     # groups of iterations launched as captured HIP graphs
 FLAG_NO_DRAIN = 4  # the iteration loop runs to the end (no drain launch for the last paths)
@@ -234,6 +273,62 @@ class Scene:
         _check(library().rt_render_block_rgba(self.handle, x, y, w, h, sample_index, iteration, _ptr(out)),
                "rt_render_block_rgba")
         return out
+
+    def render_tile_passes(self, x, y, w, h, mask=PASS_EXACT):
+        """rt_render_tile_passes: the exact passes of `mask` (normal, depth, partId, materialId, coc)
+        of the tile, from one ray through each pixel's centre; a dict of planes."""
+        mask &= PASS_EXACT
+        words = sum(c for i, c in enumerate(PASS_CHANNELS) if mask & (1 << i)) * w * h
+        buf = np.zeros(max(words, 1), np.uint32)
+        _check(library().rt_render_tile_passes(self.handle, x, y, w, h, mask, _ptr(buf)), "rt_render_tile_passes")
+        return _planes_of(buf, mask, w, h)
+
+    def render_tile_albedo(self, x, y, w, h, sample_index):
+        """rt_render_tile_albedo: (h, w, 3) base colour of what sample `sample_index`'s camera ray meets."""
+        out = np.zeros((h, w, 3), np.float32)
+        _check(library().rt_render_tile_albedo(self.handle, x, y, w, h, sample_index, _ptr(out)),
+               "rt_render_tile_albedo")
+        return out
+
+    def render_tile_alpha(self, x, y, w, h, sample_index):
+        """rt_render_tile_alpha: (h, w) coverage (1 where the sample's camera ray meets an instance)."""
+        out = np.zeros((h, w), np.float32)
+        _check(library().rt_render_tile_alpha(self.handle, x, y, w, h, sample_index, _ptr(out)),
+               "rt_render_tile_alpha")
+        return out
+
+    def render_passes(self, first, count):
+        """The exact passes of every pixel into the resident pass film, and the albedo and coverage of
+        samples [first, first + count) added to its sums in sample order."""
+        _check(library().rpx_render_passes(self.handle, first, count), "rpx_render_passes")
+
+    def reset_passes(self):
+        _check(library().rpx_passes_reset(self.handle), "rpx_passes_reset")
+
+    def passes(self, mask=PASS_ALL):
+        """The resolved planes of the resident pass film: a dict keyed as PASS_KEYS."""
+        W, H = self.width, self.height
+        words = sum(c for i, c in enumerate(PASS_CHANNELS) if mask & (1 << i)) * W * H
+        buf = np.zeros(max(words, 1), np.uint32)
+        _check(library().rpx_passes_read(self.handle, mask, _ptr(buf)), "rpx_passes_read")
+        return _planes_of(buf, mask, W, H)
+
+    def denoise(self, preset=0, strength=1.0, samples=0):
+        """The resident film denoised with the resident passes (preset 0 full, 1 viewport): (H, W, 4)
+        RGBA sums of `samples` samples (0: those rendered since the last film reset)."""
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        _check(library().rpx_denoise_film(self.handle, preset, strength, samples, _ptr(out)), "rpx_denoise_film")
+        return out
+
+    def denoise_on_device(self, preset=0, strength=1.0, samples=0):
+        """rpx_denoise_film with the result left in device memory (for timing)."""
+        _check(library().rpx_denoise_film(self.handle, preset, strength, samples, None), "rpx_denoise_film")
+
+    def denoise_timing(self):
+        out = np.zeros(8, np.float32)
+        _check(library().rpx_denoise_timing(self.handle, _ptr(out)), "rpx_denoise_timing")
+        keys = ["total_ms", "setup_ms", "firefly_ms", "variance_ms", "atrous_ms", "final_ms", "levels"]
+        return {k: float(v) for k, v in zip(keys, out)}
 
     def render_adaptive(self, cap=0):
         """Adaptive sampling in rounds into the resident film (cleared first): 8 x 8 blocks sampled
