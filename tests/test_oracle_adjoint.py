@@ -123,3 +123,61 @@ def test_textured_forward_render_reads_the_texture(setup):
     # the rounding of the bilinear weights' sum).
     np.testing.assert_allclose(img_flat, img_plain, rtol=1e-5, atol=1e-7)
     assert np.max(np.abs(img_tex - img_plain)) > 1e-3
+
+
+def conductor_scene(roughness, depth=1):
+    """A rough conductor on a Lambert floor under one rect light, no environment: at depth 1 the
+    image is next-event estimation plus BSDF-sampled hits of the light at the first vertex."""
+    w = SceneWriter()
+    w.camera((250, -700, 330), (0, 0, 90), focal_mm=35.0)
+    w.material(base_color=(0.639, 0.832, 0.648), metallic=1.0, roughness=roughness)
+    w.material(base_color=(0.5, 0.5, 0.5), ior=1.0, roughness=1.0)
+    p, n, i = uv_sphere((0, 0, 100), 100.0, 24, 48)
+    w.mesh(p, i, n, 0)
+    quad = np.array([[-900, -900, 0], [900, -900, 0], [900, 900, 0], [-900, 900, 0]], np.float32)
+    w.mesh(quad, [0, 1, 2, 0, 2, 3], np.tile([0, 0, 1], (4, 1)), 1)
+    w.instance(0, material=0)
+    w.instance(1, material=1)
+    w.light(position=(200, -300, 600), direction=(-0.2, 0.3, -1), size_x=300, size_y=300, power=6.0)
+    w.environment(kind=1, floor=0.0, horizon=0.0, zenith=0.0, strength=0.0)
+    w.render(48, 36, 64, 7, depth)
+    return w.bytes()
+
+
+def roughness_gradients(o, h, ranges=12, spp=64):
+    """The adjoint's and the central difference's d loss / d roughness of material 0 over
+    independent sample ranges: (adjoint mean, its standard error, difference mean, its error)."""
+    params = o.material_params()
+    weights = np.random.default_rng(5).uniform(0.5, 1.5, size=(o.height, o.width, 3)).astype(np.float32)
+
+    def loss(first):
+        film, _ = o.render_film(first, spp, threads=8)
+        img = film[0].astype(np.float64) / film[1].astype(np.float64)[..., None]
+        return float(np.sum(weights * img))
+    A, F = [], []
+    for k in range(ranges):
+        gm, _ = o.adjoint(1000 * k, spp, weights, threads=8)
+        A.append(gm[0, 0])
+        up, dn = params.copy(), params.copy()
+        up[0, 0] += h
+        dn[0, 0] -= h
+        o.set_material_params(up)
+        lu = loss(1000 * k)
+        o.set_material_params(dn)
+        ld = loss(1000 * k)
+        o.set_material_params(params)
+        F.append((lu - ld) / (2 * h))
+    A, F = np.array(A), np.array(F)
+    return A.mean(), A.std(ddof=1) / np.sqrt(ranges), F.mean(), F.std(ddof=1) / np.sqrt(ranges)
+
+
+def test_roughness_gradient_equals_finite_differences_inside_a_table_cell():
+    # The energy tables are linear in roughness between nodes k / 31: in the middle of a cell the
+    # image is smooth over the difference's step and the two estimates of d loss / d roughness must
+    # agree within their errors; across a node (0.514, 0.002 below 16/31) the difference averages
+    # two slopes and sits well above the derivative the adjoint gives there.
+    am, sa, fm, sf = roughness_gradients(orc.OracleScene(conductor_scene(16.5 / 31)), 1.0e-2)
+    assert abs(am - fm) <= 3.0 * np.hypot(sa, sf), (am, sa, fm, sf)
+    assert abs(am - fm) <= 0.03 * abs(fm)
+    am, sa, fm, sf = roughness_gradients(orc.OracleScene(conductor_scene(0.514)), 1.0e-2)
+    assert fm - am > 2.0 * np.hypot(sa, sf), (am, sa, fm, sf)
