@@ -164,7 +164,7 @@ def cell_centred(params, step):
     return out
 
 
-def adjoint_bench(native, seed=0, ranges=16, step=1.0e-2):
+def adjoint_bench(native, seed=0, ranges=32, step=1.0e-2):
     """The inverse-material fit's workload (512x512, 32 spp, depth 6, 200,000 triangles, 8 materials x
     (roughness, rgb albedo, IOR) + a 256x256 rgb floor texture = 196,648 parameters): gradients per
     second of the path-replay adjoint against central finite differences of the same GPU forward
@@ -234,6 +234,7 @@ def adjoint_bench(native, seed=0, ranges=16, step=1.0e-2):
                                        "speedup": round(32 * t_fwd / t_adj, 2)},
             "fd_check": {"params": len(FIT_PARAMS), "ranges": ranges, "fd_step": step,
                          "aggregate_rel_err": round(float(np.linalg.norm(am - fm) / np.linalg.norm(fm)), 4),
+                         "aggregate_rel_err_noise_floor": round(float(np.linalg.norm(sigma) / np.linalg.norm(fm)), 4),
                          "max_abs_diff_over_sigma": round(float(np.max(np.abs(am - fm) / sigma)), 2),
                          "resolved_params": int(resolved.sum()),
                          "max_rel_err_resolved": round(float(np.max(rel[resolved])) if resolved.any() else -1.0, 4),
