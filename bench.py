@@ -143,11 +143,12 @@ def mixed_queue(native, seed=0, jobs=64):
 FIT_PARAMS = [(m, 0) for m in range(6)] + [(m, 1) for m in range(8)] + [(4, 4), (5, 4)]
 
 # The BSDFs read their energy-compensation tables linearly in roughness between nodes k / 31
-# (bsdf/energy.rs, 32 nodes), and the Kulla-Conty lobe starts at alpha 0.09 (roughness 0.3): the
-# image is piecewise smooth in roughness, with a kink at every node. A central difference whose
-# step straddles a node measures the mean of two slopes, not the derivative, so the comparison is
-# made at roughness values in the middle of their table cells (and away from 0.3).
+# (bsdf/energy.rs, 32 nodes), and the compensation starts at alpha 2e-3 (energy.rs:58, roughness
+# 0.0447): the image is piecewise smooth in roughness, with a kink at every node. A central
+# difference whose step straddles a node measures the mean of two slopes, not the derivative, so the
+# comparison is made at roughness values in the middle of their table cells (and away from 0.0447).
 ROUGHNESS_NODES = 31
+COMPENSATION_START = float(np.sqrt(np.float32(2.0e-3)))
 
 
 def cell_centred(params, step):
@@ -158,8 +159,8 @@ def cell_centred(params, step):
             continue
         k = min(int(r * ROUGHNESS_NODES), ROUGHNESS_NODES - 1)
         c = (k + 0.5) / ROUGHNESS_NODES
-        if abs(c - 0.3) < 2.0 * step:
-            c += (1.0 if c > 0.3 else -1.0) / ROUGHNESS_NODES
+        if abs(c - COMPENSATION_START) < 2.0 * step:
+            c += 1.0 / ROUGHNESS_NODES
         out[m, 0] = np.float32(c)
     return out
 

@@ -89,8 +89,8 @@ def test_adjoint_agrees_with_finite_differences_on_the_fit_scene(gpu):
     s, tex = fit_setup(gpu)
     spp = s.info()["max_samples"]
     K, h = 6, 1.0e-2
-    # The image is piecewise smooth in roughness (energy-table nodes every 1/31, the Kulla-Conty
-    # lobe from roughness 0.3): the comparison is made in the middle of each roughness's cell.
+    # The image is piecewise smooth in roughness (energy-table nodes every 1/31, the compensation
+    # from roughness 0.0447): the comparison is made in the middle of each roughness's cell.
     params = bench.cell_centred(s.material_params(), h)
     s.set_material_params(params)
 
