@@ -84,6 +84,10 @@ def lib():
         L.orc_denoise.restype = ctypes.c_int
         L.orc_denoise.argtypes = [u32] * 9 + [f32] + [vp] * 9
         L.orc_denoise_synthetic.argtypes = [u32, u32, u32, f32, f32, f32] + [vp] * 7
+        L.orc_phase.restype = f32
+        L.orc_phase.argtypes = [f32, f32]
+        L.orc_sample_phase.restype = f32
+        L.orc_sample_phase.argtypes = [vp, f32, f32, f32, vp]
         os.makedirs(os.path.dirname(CACHE), exist_ok=True)
         L.orc_set_energy_cache(CACHE.encode())
         _lib = L
@@ -363,3 +367,16 @@ def denoise_synthetic(width, height, samples, left, right, spread):
                                 *(_p(planes[k]) for k in ("sums", "lum2", "albedo", "normal", "depth", "partId",
                                                           "materialId")))
     return planes
+
+
+def phase(cos, g):
+    """The Henyey-Greenstein phase function per steradian (integrator/medium.rs:143-150)."""
+    return float(lib().orc_phase(cos, g))
+
+
+def sample_phase(direction, g, u0, u1):
+    """A direction around `direction` drawn from the phase function, and its density."""
+    d = np.ascontiguousarray(direction, np.float32)
+    out = np.zeros(3, np.float32)
+    pdf = lib().orc_sample_phase(_p(d), g, u0, u1, _p(out))
+    return out, float(pdf)

@@ -549,6 +549,19 @@ int orc_denoise(uint32_t width, uint32_t height, uint32_t origin_y, uint32_t row
   return 0;
 }
 
+// The Henyey-Greenstein phase function and its sampling (integrator/medium.rs:143-169), for the
+// tests of its density.
+float orc_phase(float cos, float g);
+float orc_sample_phase(const float* dir, float g, float u0, float u1, float* out) {
+  float density;
+  Vec3 d = sample_phase_for_tests(Vec3(dir[0], dir[1], dir[2]), g, u0, u1, density);
+  out[0] = d.x;
+  out[1] = d.y;
+  out[2] = d.z;
+  return density;
+}
+float orc_phase(float cos, float g) { return phase_for_tests(cos, g); }
+
 // The reference tests' synthetic film (rt-post/src/denoise/tests.rs:22-73): the top row
 // background (a gradient), the left half part 1 around `left`, the right half part 2 around
 // `right`, exponential-like noise of relative `spread` from Pcg32(7, 11). Planes as orc_denoise.

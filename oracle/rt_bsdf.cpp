@@ -448,6 +448,14 @@ Bsdf Bsdf::lambert(Rgb albedo) {
   return b;
 }
 
+Bsdf Bsdf::medium_exit(float albedo) {
+  Bsdf b = lambert(BLACK);
+  b.exit = fmax_r(albedo, 0.0f);
+  b.diffuse = BLACK;
+  b.clear = 1.0f;
+  return b;
+}
+
 void Bsdf::layers() {
   float alpha = spec.ax;  // isotropic
   if (diffuse.r > 0.0f || diffuse.g > 0.0f || diffuse.b > 0.0f) {
@@ -546,6 +554,7 @@ LobeWeights Bsdf::lobe_weights(Vec3 wo) const {
 
 Rgb Bsdf::eval(Vec3 wo, Vec3 wi, Lobes lobes) const {
   if (wo.z <= 0.0f) return BLACK;
+  if (exit > 0.0f) return wi.z < 0.0f ? Rgb::splat(exit * FRAC_1_PI) : BLACK;
   if (wi.z <= 0.0f) {
     if (wi.z < 0.0f && lobes == Lobes::All && clear > 0.0f && !thin) return eval_refraction(wo, wi);
     return BLACK;
@@ -594,6 +603,7 @@ Rgb Bsdf::eval_refraction(Vec3 wo, Vec3 wi) const {
 
 float Bsdf::pdf(Vec3 wo, Vec3 wi) const {
   if (wo.z <= 0.0f) return 0.0f;
+  if (exit > 0.0f) return fmax_r(-wi.z, 0.0f) * FRAC_1_PI;
   if (wi.z <= 0.0f) {
     if (wi.z < 0.0f && clear > 0.0f && !thin) return refraction_pdf(lobe_weights(wo), wo, wi);
     return 0.0f;
@@ -630,6 +640,17 @@ static inline bool pdf_ok(float p) { return p > 0.0f && std::isfinite(p); }
 
 bool Bsdf::sample(Vec3 wo, float u0, float u1, BsdfSample& out) const {
   if (wo.z <= 0.0f) return false;
+  if (exit > 0.0f) {
+    // bsdf.rs:837-847: a cosine-distributed direction through the surface.
+    Vec3 d = cosine_hemisphere(u0, u1);
+    float pdf = d.z * FRAC_1_PI;
+    if (!(pdf > 0.0f)) return false;
+    out.wi = Vec3(d.x, d.y, -d.z);
+    out.weight = Rgb::splat(exit);
+    out.pdf = pdf;
+    out.straight = false;
+    return true;
+  }
   LobeWeights w = lobe_weights(wo);
   Vec3 wi;
   if (u0 < w.coat) {

@@ -137,8 +137,12 @@ struct Bsdf {
   float ms_clear_alpha = 0;
   energy::Layer base_layer, coat_layer;
   float diffuse_norm = 1.0f;
+  // Above zero, the surface is the exit of a scattering medium: a Lambertian transmission lobe of
+  // this albedo and nothing else (bsdf.rs:343-345, 549-556).
+  float exit = 0.0f;
 
   static Bsdf surface(const Material& m, bool entering) { return surface_at(m, entering, m.ior); }
+  static Bsdf medium_exit(float albedo);
   static Bsdf surface_at(const Material& m, bool entering, float ior);
   static Bsdf lambert(Rgb albedo);
 

@@ -354,6 +354,16 @@ inline Vec3 cosine_hemisphere(float u0, float u1) {
   return Vec3(x, y, z);
 }
 
+// warp.rs:38-45: 1 - cos uniform in [0, 1 - cos_max], the sine without cancellation
+inline Vec3 uniform_cone(float u0, float u1, float one_minus_cos_max) {
+  float one_minus_cos = u0 * one_minus_cos_max;
+  float cos = 1.0f - one_minus_cos;
+  float sin = std::sqrt(fmax_r(one_minus_cos * (2.0f - one_minus_cos), 0.0f));
+  float sp, cp;
+  sin_cos(TAU * u1, sp, cp);
+  return Vec3(sin * cp, sin * sp, cos);
+}
+
 // ---------------------------------------------------------------- sampler (sampler.rs)
 constexpr u32 SOBOL_DIMENSIONS = 64;
 constexpr unsigned char LAYOUT[64] = {
@@ -363,6 +373,14 @@ constexpr unsigned char LAYOUT[64] = {
 constexpr u32 DIM_CAMERA = 0, DIM_LENS = 2, DIM_FIRST_BOUNCE = 4, DIMS_PER_BOUNCE = 6;
 constexpr u32 OFFSET_LIGHT_SELECT = 0, OFFSET_LIGHT = 1, OFFSET_BSDF = 3, OFFSET_ROULETTE = 5;
 inline u32 bounce_dim(u32 b) { return DIM_FIRST_BOUNCE + b * DIMS_PER_BOUNCE; }
+// sampler.rs:137-164: the hashed dimensions of the scattering media's walks, beyond the Sobol
+// table and the cut-outs: 8 per event, 1024 events per walk, walks after vertex 127 share a block.
+constexpr u32 DIM_MEDIUM = 1u << 21, MEDIUM_DIMS_PER_EVENT = 8, MEDIUM_EVENTS = 1024, MEDIUM_BOUNCES = 128;
+inline u32 medium_dim(u32 bounce, u32 event) {
+  u32 b = bounce < MEDIUM_BOUNCES ? bounce : MEDIUM_BOUNCES - 1;
+  u32 e = event < MEDIUM_EVENTS ? event : MEDIUM_EVENTS - 1;
+  return DIM_MEDIUM + (b * MEDIUM_EVENTS + e) * MEDIUM_DIMS_PER_EVENT;
+}
 
 // The Joe–Kuo (new-joe-kuo-6.21201) degree s, coefficient a and initial direction numbers of
 // dimensions 2..64 (published data; sampler.rs:200-264 lists the same rows).

@@ -12,15 +12,7 @@ FirstHit first_hit(const Scene& scene, const Prepared& prep, const Ray& ray) {
   float t_geometry = has_hit ? hit.t : INF;
   float t_floor = prep.floor_t(ray, true);
   float t_nearest = fmin_r(t_geometry, t_floor);
-  bool panel = false;
-  for (const AreaLight& light : prep.lights.areas) {
-    float t, cos;
-    if (light.camera && light.intersect(ray, t_nearest, t, cos)) {
-      panel = true;
-      break;
-    }
-  }
-  if (t_nearest == INF || panel) return out;
+  if (t_nearest == INF || prep.lights.camera_sees_panel(ray, t_nearest)) return out;
   if (t_floor < t_geometry) {
     out.kind = FIRST_FLOOR;
     out.t = t_floor;

@@ -7,6 +7,16 @@ namespace orc {
 
 constexpr float MM2_TO_M2 = 1.0e-6f;
 constexpr float MM_TO_M = 1.0e-3f;
+// light.rs:118, 323-325: the closest a light's selection weight takes it; below this 1 - cos of
+// its cone a sphere light is a point.
+constexpr float MIN_DISTANCE2_M2 = 1.0e-6f;
+constexpr float POINT_CONE = 1.0e-7f;
+// medium.rs:69-87: events of a walk before its roulette, the largest anisotropy, and the offsets
+// inside one event's dimensions.
+constexpr u32 MEDIUM_ROULETTE_AFTER = 32;
+constexpr float MAX_G = 0.9f;
+constexpr u32 MEDIUM_OFFSET_DISTANCE = 0, MEDIUM_OFFSET_PHASE = 1, MEDIUM_OFFSET_ROULETTE = 3,
+              MEDIUM_OFFSET_FRESNEL = 4, MEDIUM_OFFSET_HERO = 5;
 constexpr size_t MAX_LOCAL_LIGHTS = 64;
 
 // ------------------------------------------------------------------ scene/mod.rs:160-357
@@ -252,13 +262,24 @@ bool scene_from_rsb(const uint8_t* bytes, size_t len, Scene& scene, std::string&
     if (m.anisotropy > 0.0f) { err = "anisotropic materials are not supported"; return false; }
     if (m.opacity < 1.0f) { err = "cut-outs are not supported"; return false; }
     if (m.flags & MATERIAL_FLAG_THIN) { err = "thin walls are not supported"; return false; }
-    if (m.scatter_mm > 0.0f) { err = "scattering media are not supported"; return false; }
     if (m.abbe > 0.0f) { err = "dispersion is not supported"; return false; }
     if (m.bevel_mm > 0.0f) { err = "edge bevels are not supported"; return false; }
     if (m.is_emissive() && (m.flags & MATERIAL_FLAG_LIGHT)) { err = "emissive mesh lights are not supported"; return false; }
   }
+  // A scene with a scattering medium and a thin wall (integrator/mod.rs:289-299: smooth, wholly clear
+  // glass without a medium of its own), which shadow rays from the medium's exits would cross, is
+  // outside the restated branches.
+  bool any_scatter = false, any_wall = false;
+  for (const Material& m : scene.materials) {
+    ScatterMedium sm;
+    bool scattering = scatter_medium_of(m, sm);
+    any_scatter = any_scatter || scattering;
+    any_wall = any_wall || (is_clear_solid(m) && !scattering && (m.kind == MATERIAL_GLASS || m.metallic <= 0.0f) &&
+                            m.transmission >= 1.0f && m.roughness <= 0.05f && m.coat <= 0.0f && m.opacity >= 1.0f);
+  }
+  if (any_scatter && any_wall) { err = "thin walls beside a scattering medium are not supported"; return false; }
   for (const Light& l : scene.lights)
-    if (l.kind != LIGHT_RECT && l.kind != LIGHT_DISK) { err = "only rect and disk lights are supported"; return false; }
+    if (l.kind == LIGHT_SUN || l.kind == LIGHT_IES) { err = "sun and IES lights are not supported"; return false; }
   std::vector<Mesh> built(meshes.size());
   for (size_t i = 0; i < meshes.size(); ++i) {
     if (meshes[i].material >= scene.materials.size()) { err = "material_id does not refer to a MATL section"; return false; }
@@ -385,6 +406,49 @@ float AreaLight::weight(Vec3 x) const {
   return base * orientation / (fmax_r(d2, area) * MM2_TO_M2);
 }
 
+// light.rs:364-420
+float SphereLight::mask(Vec3 x) const {
+  if (cos_outer < -1.0f) return 1.0f;
+  Vec3 d = x - centre;
+  float len2 = d.length_squared();
+  if (!positive(len2)) return 0.0f;
+  float c = d.dot(axis) / std::sqrt(len2);
+  if (c <= cos_outer) return 0.0f;
+  if (c >= cos_inner) return 1.0f;
+  float t = (c - cos_outer) / (cos_inner - cos_outer);
+  return t * t * (3.0f - 2.0f * t);
+}
+
+bool SphereLight::cone(float dist2, float& omc) const {
+  float r2 = radius * radius;
+  if (!positive(dist2 - r2)) return false;
+  float sin2 = r2 / dist2;
+  float v = sin2 / (1.0f + std::sqrt(fmax_r(1.0f - sin2, 0.0f)));
+  omc = v < POINT_CONE ? 0.0f : v;
+  return true;
+}
+
+bool SphereLight::intersect(const Ray& ray, float t_max, float& t_out) const {
+  if (!positive(radius)) return false;
+  Vec3 oc = ray.origin - centre;
+  float r2 = radius * radius;
+  if (oc.length_squared() <= r2) return false;
+  float b = oc.dot(ray.dir);
+  float disc = r2 - (oc - ray.dir * b).length_squared();
+  if (disc < 0.0f) return false;
+  float t = -b - std::sqrt(disc);
+  if (!(t > ray.t_min && t < t_max)) return false;
+  t_out = t;
+  return true;
+}
+
+float SphereLight::weight(Vec3 x) const {
+  float m = mask(x);
+  if (m <= 0.0f) return 0.0f;
+  float d2 = (x - centre).length_squared() * MM2_TO_M2;
+  return base * m / fmax_r(d2, near2);
+}
+
 static std::vector<float> build_cdf(const std::vector<float>& weights) {
   float total = -0.0f;
   for (float w : weights) total = total + w;
@@ -430,6 +494,7 @@ bool Lights::build(const Scene& scene, std::string& err) {
   };
   std::vector<float> weights;
   areas.clear();
+  spheres.clear();
   entries.clear();
   for (const Light& light : scene.lights) {
     bool glossy = (light.flags & LIGHT_FLAG_GLOSSY) != 0;
@@ -437,6 +502,46 @@ bool Lights::build(const Scene& scene, std::string& err) {
     bool shadow = (light.flags & LIGHT_FLAG_NO_SHADOW) == 0;
     float power = fmax_r(light.power, 0.0f);
     u32 entry = (u32)entries.size();
+    if (light.kind == LIGHT_POINT || light.kind == LIGHT_SPOT) {
+      // light.rs:793-863 (no IES profile)
+      Rgb color(fmax_r(light.color.r, 0.0f), fmax_r(light.color.g, 0.0f), fmax_r(light.color.b, 0.0f));
+      Rgb intensity = color * (power / (4.0f * PI));
+      float radius = 0.5f * std::fabs(light.size_x);
+      if (!std::isfinite(radius)) radius = 0.0f;
+      SphereLight sp;
+      if (light.kind == LIGHT_SPOT) {
+        Vec3 axis = light.direction.normalize();
+        if (axis.length_squared() < 0.5f) continue;
+        float half = 0.5f * clampf(light.spread_deg, 1.0f, 180.0f) * DEG_TO_RAD;
+        float soft = fmax_r(light.angle_deg, 0.0f) * DEG_TO_RAD;
+        float inner = fmax_r(half - soft, 0.0f);
+        sp.axis = axis;
+        sp.cos_outer = fcos(half);
+        sp.cos_inner = fcos(inner);
+      } else {
+        sp.axis = Vec3(0.0f, 0.0f, -1.0f);
+        sp.cos_outer = -2.0f;
+        sp.cos_inner = -2.0f;
+      }
+      float flux = color.luminance() * power;
+      if (!positive(flux)) continue;
+      float r_m = radius * MM_TO_M;
+      sp.radiance = radius > 0.0f ? intensity * (1.0f / (PI * r_m * r_m)) : Rgb(0.0f, 0.0f, 0.0f);
+      sp.centre = light.position;
+      sp.radius = radius;
+      sp.intensity = intensity;
+      sp.glossy = glossy;
+      sp.camera = camera;
+      sp.shadow = shadow;
+      sp.entry = entry;
+      sp.base = intensity.luminance();
+      sp.near2 = fmax_r(r_m * r_m, MIN_DISTANCE2_M2);
+      float toward = 0.05f + 0.95f * sp.mask(centre);
+      weights.push_back(flux * toward * reach(light.position, false, Vec3(0, 0, 0)));
+      entries.push_back(-2 - (int)spheres.size());
+      spheres.push_back(sp);
+      continue;
+    }
     if (light.kind != LIGHT_RECT && light.kind != LIGHT_DISK) { err = "unsupported light kind"; return false; }
     AreaLight a;
     a.rect = light.kind == LIGHT_RECT;
@@ -492,6 +597,7 @@ bool Lights::build(const Scene& scene, std::string& err) {
 
 float Lights::weight(size_t entry, Vec3 x) const {
   int a = entries[entry];
+  if (a <= -2) return spheres[-2 - a].weight(x);
   return a < 0 ? env_base : areas[a].weight(x);
 }
 
@@ -556,7 +662,39 @@ bool Lights::sample_selected(const Selection& from, float u_select, float u0, fl
   if (!positive(p)) return false;
   LightSample s;
   int a = entries[index];
-  if (a >= 0) {
+  if (a <= -2) {
+    // light.rs:1224-1268
+    const SphereLight& light = spheres[-2 - a];
+    float mask = light.mask(x);
+    Vec3 d = light.centre - x;
+    float dist2 = d.length_squared();
+    if (!(mask > 0.0f && dist2 > 0.0f)) return false;
+    float dist = std::sqrt(dist2);
+    Vec3 towards = d / dist;
+    float omc = 0.0f;
+    if (light.radius > 0.0f && !light.cone(dist2, omc)) return false;
+    if (omc == 0.0f) {
+      s.wi = towards;
+      s.distance = dist;
+      s.radiance = light.intensity * (mask / (dist2 * MM2_TO_M2));
+      s.pdf = p;
+      s.delta = true;
+    } else {
+      Vec3 local = uniform_cone(u0, u1, omc);
+      Vec3 wi = Frame::from_normal(towards).to_world(local).normalize();
+      float b = wi.dot(d);
+      float r2 = light.radius * light.radius;
+      float near = b - std::sqrt(fmax_r(r2 - (d - wi * b).length_squared(), 0.0f));
+      s.wi = wi;
+      s.distance = fmax_r(near, 0.0f);
+      s.radiance = light.radiance * mask;
+      s.pdf = p / (TAU * omc);
+      s.delta = false;
+    }
+    s.glossy = light.glossy;
+    s.photon = false;
+    s.shadow = light.shadow;
+  } else if (a >= 0) {
     const AreaLight& light = areas[a];
     float x0, y0;
     if (light.rect) {
@@ -617,7 +755,38 @@ Rgb Lights::crossed_from(const Ray& ray, float t_max, bool has_bsdf, float bsdf_
     }
     sum += light.radiance_at(cos) * weight;
   }
+  // light.rs:1386-1425: the spheres after the areas, seen from the vertex the ray left.
+  for (const SphereLight& light : spheres) {
+    bool visible = has_bsdf ? light.glossy : light.camera;
+    if (!visible) continue;
+    float t;
+    if (!light.intersect(ray, t_max, t)) continue;
+    Vec3 vertex = ray.origin - ray.dir * offset;
+    float mask = light.mask(vertex);
+    if (!positive(mask)) continue;
+    float weight = 1.0f;
+    if (has_bsdf) {
+      float omc;
+      if (light.cone((light.centre - vertex).length_squared(), omc) && omc > 0.0f)
+        weight = power_heuristic(bsdf_pdf, pmf(light.entry, *from) / (TAU * omc));
+      else
+        weight = 0.0f;
+    }
+    sum += light.radiance * (mask * weight);
+  }
   return sum;
+}
+
+bool Lights::camera_sees_panel(const Ray& ray, float t_max) const {
+  for (const AreaLight& light : areas) {
+    float t, cos;
+    if (light.camera && light.intersect(ray, t_max, t, cos)) return true;
+  }
+  for (const SphereLight& light : spheres) {
+    float t;
+    if (light.camera && light.intersect(ray, t_max, t) && light.mask(ray.origin) > 0.0f) return true;
+  }
+  return false;
 }
 
 Rgb Lights::escaped(Vec3 dir, bool has_bsdf, float bsdf_pdf, const Selection* from) const {
@@ -642,7 +811,19 @@ bool Prepared::build(const Scene& scene, std::string& err) {
   fallback = Material();
   has_medium.clear();
   media.clear();
+  has_scatter.clear();
+  scatter.clear();
   for (const Material& m : scene.materials) {
+    ScatterMedium sm{};
+    bool scattering = scatter_medium_of(m, sm);
+    has_scatter.push_back(scattering);
+    scatter.push_back(sm);
+    if (scattering) {
+      // integrator/mod.rs:419: a scattering medium folds the absorption into its walk.
+      has_medium.push_back(false);
+      media.push_back(Rgb());
+      continue;
+    }
     bool clear = m.transmission > 0.0f && m.metallic < 1.0f && (m.flags & MATERIAL_FLAG_THIN) == 0;
     bool on = false;
     Rgb sigma;
@@ -679,6 +860,166 @@ void Prepared::add(Rgb& sum, Rgb c, bool indirect) const {
   float pk = fmax_r(fmax_r(c.r, c.g), c.b);
   if (indirect && clamp_indirect > 0.0f && pk > clamp_indirect) sum += c * (clamp_indirect / pk);
   else sum += c;
+}
+
+// ------------------------------------------------------------------ integrator/medium.rs
+// medium.rs:99-110
+bool is_clear_solid(const Material& m) {
+  bool metal = m.kind != MATERIAL_GLASS && m.metallic >= 1.0f;
+  return m.transmission > 0.0f && !metal && (m.flags & MATERIAL_FLAG_THIN) == 0 && !m.is_emissive() &&
+         m.ior >= 1.0001f;
+}
+
+// medium.rs:113-141
+bool scatter_medium_of(const Material& m, ScatterMedium& out) {
+  if (!(m.scatter_mm > 0.0f && std::isfinite(m.scatter_mm) && is_clear_solid(m))) return false;
+  float sigma = 1.0f / m.scatter_mm;
+  Rgb albedo(clampf(m.scatter_color.r, 0.0f, 1.0f), clampf(m.scatter_color.g, 0.0f, 1.0f),
+             clampf(m.scatter_color.b, 0.0f, 1.0f));
+  Rgb sigma_a((1.0f - albedo.r) * sigma, (1.0f - albedo.g) * sigma, (1.0f - albedo.b) * sigma);
+  if (m.absorption_mm > 0.0f) {
+    Rgb c = m.base_color;
+    sigma_a += Rgb(-lnf_(clampf(c.r, 1.0e-4f, 1.0f)) / m.absorption_mm,
+                   -lnf_(clampf(c.g, 1.0e-4f, 1.0f)) / m.absorption_mm,
+                   -lnf_(clampf(c.b, 1.0e-4f, 1.0f)) / m.absorption_mm);
+  }
+  Rgb sigma_s = albedo * sigma;
+  out.sigma_s = sigma_s;
+  out.sigma_a = sigma_a;
+  out.sigma_t = sigma_s + sigma_a;
+  out.g = std::isfinite(m.scatter_g) ? clampf(m.scatter_g, -MAX_G, MAX_G) : 0.0f;
+  out.ior = m.ior;
+  return true;
+}
+
+namespace {
+// medium.rs:143-150
+float phase(float cos, float g) {
+  float d = 1.0f + g * g - 2.0f * g * cos;
+  return (1.0f - g * g) / (4.0f * PI * d * std::sqrt(fmax_r(d, 1.0e-12f)));
+}
+
+// medium.rs:152-169
+Vec3 sample_phase(Vec3 dir, float g, float u0, float u1, float& density) {
+  float cos;
+  if (std::fabs(g) < 1.0e-3f) {
+    cos = 1.0f - 2.0f * u0;
+  } else {
+    float s = (1.0f - g * g) / (1.0f - g + 2.0f * g * u0);
+    cos = clampf((1.0f + g * g - s * s) / (2.0f * g), -1.0f, 1.0f);
+  }
+  float sin = std::sqrt(fmax_r(1.0f - cos * cos, 0.0f));
+  float sp, cp;
+  sin_cos(TAU * u1, sp, cp);
+  Frame frame = Frame::from_normal(dir);
+  density = phase(cos, g);
+  return frame.to_world(Vec3(sin * cp, sin * sp, cos)).normalize();
+}
+
+inline float channel(Rgb c, u32 k) { return k == 0 ? c.r : (k == 1 ? c.g : c.b); }
+inline Rgb transmittance(Rgb sigma, float t) {
+  return Rgb(expf_(-sigma.r * t), expf_(-sigma.g * t), expf_(-sigma.b * t));
+}
+
+enum WalkEnd { WALK_CROSSED, WALK_EXIT, WALK_ESCAPED, WALK_ABSORBED };
+
+// medium.rs:226-360: the random walk inside `m` from `ray` until the path leaves the solid.
+WalkEnd walk(const Scene& scene, const PixelSampler& sampler, const ScatterMedium& m, Ray& ray, Rgb& throughput,
+             u32& path_events, u32 bounce, RenderStats* stats, RayHit& hit_out, float& eta2_out, float& pdf_out) {
+  const Geometry& geo = scene.geometry;
+  u32 first = medium_dim(bounce, 0);
+  float uh = sampler.get1(first + MEDIUM_OFFSET_HERO) * 3.0f;
+  u32 hero = (u32)std::min<u64>(f_to_usize(uh), 2);
+  float sigma_hero = channel(m.sigma_t, hero);
+  Rgb w = WHITE, ratio = WHITE;
+  u32 events = 0, steps = 0;
+  float pdf = 0.0f;
+  Rgb entry = throughput;
+  auto effective = [](Rgb w_, Rgb r_) { return w_ * (3.0f / (r_.r + r_.g + r_.b)); };
+  for (;;) {
+    if (stats) { stats->rays++; stats->nearest++; }
+    RayHit hit;
+    bool has_hit = geo.trace(ray, false, hit, stats ? &stats->nearest_trav : nullptr);
+    float t_hit = has_hit ? hit.t : INF;
+    bool capped = path_events >= MEDIUM_EVENTS || std::isnan(sigma_hero) || sigma_hero <= 0.0f;
+    u32 dims = medium_dim(bounce, steps);
+    steps += 1;
+    float t = INF;
+    if (!capped) {
+      float u = sampler.get1(dims + MEDIUM_OFFSET_DISTANCE);
+      t = -lnf_(1.0f - u) / sigma_hero;
+    }
+    if (t < t_hit) {
+      // A scattering event.
+      Rgb tr = transmittance(m.sigma_t, t);
+      Rgb p = m.sigma_t * tr;
+      Rgb f = m.sigma_s * tr;
+      float ph = channel(p, hero);
+      if (std::isnan(ph) || ph <= 0.0f) return WALK_ABSORBED;
+      w = w * (f / ph);
+      ratio = ratio * (p / ph);
+      float mx = peak(ratio);
+      if (mx > 0.0f) {
+        w = w / mx;
+        ratio = ratio / mx;
+      }
+      events += 1;
+      path_events += 1;
+      Rgb current = entry * effective(w, ratio);
+      if (!(peak(current) > 0.0f && std::isfinite(peak(current)))) return WALK_ABSORBED;
+      if (events > MEDIUM_ROULETTE_AFTER) {
+        float survive = clampf(peak(current) / peak(entry), 0.05f, 1.0f);
+        if (sampler.get1(dims + MEDIUM_OFFSET_ROULETTE) >= survive) return WALK_ABSORBED;
+        w = w / survive;
+      }
+      Vec3 x = ray.at(t);
+      float u0, u1;
+      sampler.get2(dims + MEDIUM_OFFSET_PHASE, u0, u1);
+      float density;
+      Vec3 dir = sample_phase(ray.dir, m.g, u0, u1, density);
+      pdf = density;
+      ray = Ray(x, dir);
+      continue;
+    }
+    // The boundary (or, past the last step, straight to it keeping all but absorption).
+    if (std::isfinite(t_hit)) {
+      if (capped) {
+        w = w * transmittance(m.sigma_a, t_hit);
+      } else {
+        Rgb p = transmittance(m.sigma_t, t_hit);
+        float ph = channel(p, hero);
+        if (std::isnan(ph) || ph <= 0.0f) return WALK_ABSORBED;
+        w = w * (p / ph);
+        ratio = ratio * (p / ph);
+      }
+    }
+    throughput = entry * effective(w, ratio);
+    if (!(peak(throughput) > 0.0f && std::isfinite(peak(throughput)))) return WALK_ABSORBED;
+    if (!has_hit) return WALK_ESCAPED;
+    hit_out = hit;
+    if (events == 0) return WALK_CROSSED;
+    Hit details;
+    if (!geo.hit_details(ray, hit, details)) return WALK_ABSORBED;
+    Vec3 ng = details.geometric_normal;
+    float cos = clampf(-ray.dir.dot(ng), 0.0f, 1.0f);
+    // medium.rs:218-224: leave with the probability of the Fresnel transmittance out.
+    bool leave = capped || sampler.get1(dims + MEDIUM_OFFSET_FRESNEL) >= fresnel_dielectric(cos, 1.0f / fmax_r(m.ior, 1.0001f));
+    if (leave) {
+      eta2_out = m.ior * m.ior;
+      pdf_out = pdf;
+      return WALK_EXIT;
+    }
+    // Reflected back in (a step of the path, for the cap).
+    path_events += 1;
+    Vec3 dir = (ray.dir + ng * (2.0f * cos)).normalize();
+    ray = Ray(offset_origin(details.point, ng), dir);
+  }
+}
+}  // namespace
+
+float phase_for_tests(float cos, float g) { return phase(cos, g); }
+Vec3 sample_phase_for_tests(Vec3 dir, float g, float u0, float u1, float& density) {
+  return sample_phase(dir, g, u0, u1, density);
 }
 
 static Ray shadow_ray(Vec3 origin, Vec3 dir, float distance) {
@@ -791,12 +1132,39 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
   float offset = 0.0f;
   bool in_medium = false;
   Rgb sigma;
+  // The scattering medium the ray is inside (a material index, -1 for none), the steps its walks
+  // took, and whether the last vertex was a medium's exit (integrator/mod.rs:1177-1185).
+  int scatter = -1;
+  u32 medium_events = 0;
+  bool exit_pdf = false;
   for (;;) {
-    if (stats) { stats->rays++; stats->nearest++; }
+    // After a walk that scattered: the exit lobe's albedo at the surface it reached.
+    bool exit = false;
+    float exit_eta2 = 0.0f;
     RayHit hit;
-    bool has_hit = geo.trace(ray, primary, hit, stats ? &stats->nearest_trav : nullptr);
+    bool has_hit;
+    if (scatter < 0) {
+      if (stats) { stats->rays++; stats->nearest++; }
+      has_hit = geo.trace(ray, primary, hit, stats ? &stats->nearest_trav : nullptr);
+    } else {
+      // integrator/mod.rs:1209-1242
+      float eta2 = 0.0f, pdf = 0.0f;
+      WalkEnd end = walk(scene, sampler, prep.scatter[scatter], ray, throughput, medium_events, bounce, stats, hit,
+                         eta2, pdf);
+      if (end == WALK_ABSORBED) break;
+      has_hit = end != WALK_ESCAPED;
+      if (end == WALK_EXIT) {
+        scatter = -1;
+        exit = true;
+        exit_eta2 = eta2;
+        has_pdf = true;
+        bsdf_pdf = pdf;
+        primary = false;
+        offset = 0.0f;
+      }
+    }
     float t_geometry = has_hit ? hit.t : INF;
-    float t_floor = prep.floor_t(ray, primary);
+    float t_floor = exit ? INF : prep.floor_t(ray, primary);
     float t_nearest = fmin_r(t_geometry, t_floor);
     if (adjoint && pending_target.on) {
       // The radiance gathered after the last vertex: the forward's total minus what is summed.
@@ -810,7 +1178,8 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
       Rgb bl(expf_(-sigma.r * t_nearest), expf_(-sigma.g * t_nearest), expf_(-sigma.b * t_nearest));
       throughput = throughput * bl;
     }
-    Rgb crossed = lights.crossed_from(ray, t_nearest, has_pdf, bsdf_pdf, &sampled_at, offset);
+    // After a walk, the segment to its exit vertex crosses no light.
+    Rgb crossed = exit ? BLACK : lights.crossed_from(ray, t_nearest, has_pdf, bsdf_pdf, &sampled_at, offset);
     prep.add(sum, throughput * crossed, bounce >= 2);
     if (t_nearest == INF) {
       if (primary) {
@@ -886,7 +1255,7 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
         target.tex = true;
       }
     }
-    Bsdf bsdf = material ? Bsdf::surface(shaded, s_outside) : prep.floor_bsdf;
+    Bsdf bsdf = exit ? Bsdf::medium_exit(exit_eta2) : (material ? Bsdf::surface(shaded, s_outside) : prep.floor_bsdf);
     Frame frame = Frame::from_normal(s_normal);
     Vec3 wo = frame.to_local(-ray.dir);
     Vec3 ng = s_geometric;
@@ -964,13 +1333,24 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
       sampled_at = selection;
       primary = false;
       offset = 0.0f;
+      // integrator/mod.rs:1822-1857: a BSDF sample transmitted into a solid right after an exit
+      // vertex is probed for a thin wall (one ray counted; the scenes with media have no walls).
+      bool next_exit = false;
+      if (exit) {
+        next_exit = true;
+      } else if (exit_pdf && s_has_instance && transmitted && s_outside) {
+        if (stats) stats->rays++;
+      }
+      exit_pdf = next_exit;
       if (transmitted) {
         if (s_outside && material && s_has_instance) {
           u32 id = geo.instances[s_instance].material_id;
           in_medium = id < prep.has_medium.size() && prep.has_medium[id];
           if (in_medium) sigma = prep.media[id];
+          scatter = id < prep.has_scatter.size() && prep.has_scatter[id] ? (int)id : -1;
         } else {
           in_medium = false;
+          scatter = -1;
         }
       }
     }

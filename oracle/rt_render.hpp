@@ -112,6 +112,23 @@ struct AreaLight {
   float weight(Vec3 x) const;
 };
 
+// A point light or a spot (kinds 3 and 4, light.rs:323-420): a sphere of uniform radiance (radius
+// 0: a point), a spot's masked by the direction from its centre.
+struct SphereLight {
+  Vec3 centre;
+  float radius;
+  Rgb intensity, radiance;
+  Vec3 axis;
+  float cos_outer, cos_inner;
+  bool glossy, camera, shadow;
+  u32 entry;
+  float base, near2;
+  float mask(Vec3 x) const;
+  bool cone(float dist2, float& omc) const;  // false from inside
+  bool intersect(const Ray& ray, float t_max, float& t_out) const;
+  float weight(Vec3 x) const;
+};
+
 struct EnvRamp {
   float floor, horizon, zenith, lower_mass, total_mass;
   float value(float z) const;
@@ -137,12 +154,13 @@ struct LightSample {
 
 struct Lights {
   std::vector<AreaLight> areas;
+  std::vector<SphereLight> spheres;
   EnvRamp env;
   bool has_env_entry = false;
   u32 env_entry = 0;
   float env_base = 0;
-  // entry -> (is_env, area index)
-  std::vector<int> entries;  // area index, or -1 for the environment
+  // entry -> area index (>= 0), the environment (-1), or sphere k (-2 - k)
+  std::vector<int> entries;
   std::vector<float> cdf, flux_pmf;
   bool per_point = false;
 
@@ -155,7 +173,21 @@ struct Lights {
   bool sample_selected(const Selection& sel, float u_select, float u0, float u1, LightSample& out) const;
   Rgb crossed_from(const Ray& ray, float t_max, bool has_bsdf, float bsdf_pdf, const Selection* from, float offset) const;
   Rgb escaped(Vec3 dir, bool has_bsdf, float bsdf_pdf, const Selection* from) const;
+  // An area light or a sphere flagged `camera` in front of everything before t_max (light.rs:1479-1488).
+  bool camera_sees_panel(const Ray& ray, float t_max) const;
 };
+
+// The homogeneous scattering medium inside a clear solid (integrator/medium.rs:91-140): per
+// channel scattering, absorption and extinction per mm, Henyey-Greenstein anisotropy, the solid's
+// index.
+struct ScatterMedium {
+  Rgb sigma_s, sigma_a, sigma_t;
+  float g, ior;
+};
+bool is_clear_solid(const Material& m);
+bool scatter_medium_of(const Material& m, ScatterMedium& out);
+float phase_for_tests(float cos, float g);
+Vec3 sample_phase_for_tests(Vec3 dir, float g, float u0, float u1, float& density);
 
 // ---------------------------------------------------------------- prepared scene + film
 struct Prepared {
@@ -170,6 +202,8 @@ struct Prepared {
   Material fallback;
   std::vector<bool> has_medium;
   std::vector<Rgb> media;  // per material: absorption coefficients per mm
+  std::vector<bool> has_scatter;
+  std::vector<ScatterMedium> scatter;  // per material: the scattering medium of its inside
 
   bool build(const Scene& scene, std::string& err);
   float floor_t(const Ray& ray, bool camera_ray) const;

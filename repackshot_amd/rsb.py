@@ -32,10 +32,10 @@ class SceneWriter:
 
     def material(self, kind=0, base_color=(0.8, 0.8, 0.8), metallic=0.0, roughness=0.5, ior=1.5, transmission=0.0,
                  coat=0.0, coat_roughness=0.0, emission=(0, 0, 0), emission_power=0.0, flags=0, opacity=1.0,
-                 anisotropy=0.0, absorption_mm=0.0):
+                 anisotropy=0.0, absorption_mm=0.0, scatter_mm=0.0, scatter_color=(1.0, 1.0, 1.0), scatter_g=0.0):
         payload = (_u(kind) + _f(*base_color, metallic, roughness, ior, transmission, coat, coat_roughness,
                                  *emission, emission_power) + _u(flags)
-                   + _f(opacity, anisotropy, absorption_mm, 0.0, 0.0, 1.0, 1.0, 1.0, 0.0, 0.0))
+                   + _f(opacity, anisotropy, absorption_mm, 0.0, scatter_mm, *scatter_color, scatter_g, 0.0))
         self.sections.append(_section(b"MATL", payload))
         return self
 
@@ -105,6 +105,48 @@ def with_adaptive(rsb: bytes, noise_threshold: float, width=None, height=None) -
             return bytes(out)
         i += 8 + n
     raise ValueError("the scene bytes have no RNDR section")
+
+
+def box_mesh(lo, hi):
+    """An axis-aligned box (positions, indices) wound outwards, without normals."""
+    c = np.array([[hi[0] if (k & 1) else lo[0], hi[1] if (k & 2) else lo[1], hi[2] if (k & 4) else lo[2]]
+                  for k in range(8)], np.float32)
+    quads = [[0, 2, 3, 1], [4, 5, 7, 6], [0, 1, 5, 4], [2, 6, 7, 3], [1, 3, 7, 5], [0, 4, 6, 2]]
+    idx = []
+    for a, b, cc, d in quads:
+        idx += [a, b, cc, a, cc, d]
+    return c, np.array(idx, np.uint32)
+
+
+def icosphere(center, radius, subdivisions=3):
+    """A geodesic sphere (positions, indices) with flat facets, wound outwards: the regular
+    icosahedron with each triangle split in four `subdivisions` times, new vertices pushed out to
+    the sphere."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    dirs = [np.array(v, np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdivisions):
+        mids = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mids:
+                m = dirs[a] + dirs[b]
+                dirs.append(m / np.linalg.norm(m))
+                mids[key] = len(dirs) - 1
+            return mids[key]
+
+        nxt = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nxt += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nxt
+    p = np.asarray(center, np.float64) + radius * np.array(dirs)
+    return p.astype(np.float32), np.array(faces, np.uint32).reshape(-1)
 
 
 def uv_sphere(center, radius, lat=16, lon=32):
