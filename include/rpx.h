@@ -142,7 +142,8 @@ uint32_t rpx_energy_tables(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_
  * returns the length, or 0 when cap is too small (call with cap 0 to size the buffer).
  * kind 0: the reference's example scene (example.rs:113-230); 1: icosphere on a floor
  * (256x256); 2: eight displaced level-7 icospheres (1024x1024); 4: the inverse-material fit's scene
- * (200,000 triangles, 512x512, floor material 8 takes the base-colour texture). */
+ * (200,000 triangles, 512x512, floor material 8 takes the base-colour texture); 5: a glass bottle
+ * holding a scattering medium, lit by a sphere light (200,000 triangles, 1920x1080). */
 uint64_t rpx_write_scene(uint32_t kind, uint32_t seed, uint8_t* buf, uint64_t cap);
 /* Job `job` of the seeded mixed render queue (this library's batched workload), as RSB v1 bytes
  * (same buffer protocol): a square image of 256..1024 px, 10k..500k triangles, 16/32/64 spp. */

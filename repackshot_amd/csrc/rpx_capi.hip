@@ -115,7 +115,8 @@ bool build_device_scene(Handle* h, std::string& err) {
       !upload(B, flat.indices, d.indices, st, err) || !upload(B, flat.bsdfs, d.bsdfs, st, err) ||
       !upload(B, flat.materials, d.materials, st, err) || !upload(B, flat.areas, d.areas, st, err) ||
       !upload(B, flat.cdf, d.cdf, st, err) || !upload(B, flat.flux_pmf, d.flux_pmf, st, err) ||
-      !upload(B, flat.entries, d.entries, st, err))
+      !upload(B, flat.entries, d.entries, st, err) || !upload(B, flat.spheres, d.spheres, st, err) ||
+      !upload(B, flat.media, d.media, st, err))
     return false;
   d.tables = tables->device_view();
   std::vector<DevScene> one(1, d);
@@ -261,6 +262,7 @@ void fill_timing(const Engine& e, float* out12) {
     out12[1 + k] = t.stage_ms[k];
     out12[5 + k] = (float)t.launches[k];
   }
+  out12[1] += t.stage_ms[5];  // the media walks are closest-hit work
   out12[9] = (float)t.iterations;
   out12[10] = (float)t.chunks;
   out12[11] = t.stage_ms[4];
@@ -389,6 +391,7 @@ static uint32_t render_tile_impl(uint32_t handle, uint32_t x, uint32_t y, uint32
     return RPX_GPU_MEMORY;
   }
   h->engine->textured = h->dev.tex_material != NO_TEXTURE;
+  h->engine->ext = h->dev.ext != 0;
   hipError_t e = h->engine->run(h->d_scene, std::vector<DevSegment>(1, s), false, err);
   if (e == hipSuccess) e = launch_tile_out(h->engine->P, d_out, cw * ch, channels, h->stream);
   std::vector<float> tmp((size_t)cw * ch * channels);
@@ -465,6 +468,7 @@ uint32_t rpx_render_adaptive(uint32_t handle, uint32_t cap, uint64_t* out8) {
   hipMemsetAsync(F.alpha, 0, px * 4, st);
   hipMemsetAsync(F.lum2, 0, px * 4, st);
   h->engine->textured = h->dev.tex_material != NO_TEXTURE;
+  h->engine->ext = h->dev.ext != 0;
   hipEvent_t ev[2];
   hipEventCreate(&ev[0]);
   hipEventCreate(&ev[1]);
@@ -624,6 +628,7 @@ uint32_t rpx_render_samples(uint32_t handle, uint32_t first, uint32_t count) {
     return RPX_GPU_MEMORY;
   }
   h->engine->textured = h->dev.tex_material != NO_TEXTURE;
+  h->engine->ext = h->dev.ext != 0;
   hipError_t e = h->engine->run(h->d_scene, std::vector<DevSegment>(1, whole_image(h.get(), 0, first, count)), true, err);
   if (e != hipSuccess) {
     set_error(err);
@@ -740,9 +745,9 @@ uint32_t rpx_scene_info(uint32_t handle, uint64_t* out8) {
 }
 
 static void stats_of(Engine& e, u64 samples, uint64_t* out4) {
-  u64 r[3] = {0, 0, 0};
+  u64 r[4] = {0, 0, 0, 0};
   e.ray_counts(r);
-  out4[0] = r[0] + r[1] + r[2];
+  out4[0] = r[0] + r[1] + r[2] + r[3];
   out4[1] = r[0];
   out4[2] = r[1] + r[2];
   out4[3] = samples;
@@ -795,6 +800,7 @@ uint32_t rpx_autotune(uint32_t handle, uint32_t samples, float* out60) {
   std::vector<float> ms;
   std::string err;
   h->engine->textured = h->dev.tex_material != NO_TEXTURE;
+  h->engine->ext = h->dev.ext != 0;
   hipError_t r = h->engine->autotune(h->d_scene, std::vector<DevSegment>(1, s), ms, err);
   return tune_report(*h->engine, r, ms, err, out60);
 }
@@ -872,8 +878,10 @@ uint32_t rpx_set_material_params(uint32_t handle, const float* params, uint32_t 
   }
   std::vector<Bsdf> bsdfs;
   std::vector<DevMaterial> mats;
+  std::vector<DevMedium> media;
+  std::vector<HostMaterial> updated = h->host.materials;
   for (u32 m = 0; m < n; ++m) {
-    HostMaterial& hm = h->host.materials[m];
+    HostMaterial& hm = updated[m];
     const float* p = params + N_MAT_PARAMS * m;
     hm.roughness = p[0];
     hm.base_color = c3(p[1], p[2], p[3]);
@@ -883,12 +891,23 @@ uint32_t rpx_set_material_params(uint32_t handle, const float* params, uint32_t 
     bsdfs.push_back(sides[0]);
     bsdfs.push_back(sides[1]);
     mats.push_back(dev_material(hm));
+    DevMedium md{};
+    if (scatter_medium_of(hm, md) && !h->dev.ext) {
+      // The scene's kernels were chosen without media (an index moved into a medium's range).
+      set_error("rpx_set_material_params: the parameters give a material a scattering medium it did not have");
+      return RPX_BAD_SCENE;
+    }
+    media.push_back(md);
   }
+  h->host.materials = updated;
   hipStreamSynchronize(h->stream);
   hipError_t e = hipMemcpy(const_cast<Bsdf*>(h->dev.bsdfs), bsdfs.data(), bsdfs.size() * sizeof(Bsdf),
                            hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy(const_cast<DevMaterial*>(h->dev.materials), mats.data(), mats.size() * sizeof(DevMaterial),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess && n)
+    e = hipMemcpy(const_cast<DevMedium*>(h->dev.media), media.data(), media.size() * sizeof(DevMedium),
                   hipMemcpyHostToDevice);
   return e == hipSuccess ? RPX_OK : RPX_GPU_LAUNCH;
 }
@@ -933,6 +952,10 @@ uint32_t rpx_adjoint(uint32_t handle, uint32_t first, uint32_t count, const floa
   if (!h) return RPX_BAD_HANDLE;
   if (count == 0) return RPX_OK;
   std::lock_guard<std::mutex> lock(h->mutex);
+  if (h->dev.ext) {
+    set_error("rpx_adjoint: the replay does not follow sphere lights or scattering media");
+    return RPX_BAD_SCENE;
+  }
   hipSetDevice(h->device);
   size_t px = (size_t)h->dev.width * h->dev.height;
   size_t n_mat = h->host.materials.size() * N_MAT_PARAMS;
@@ -955,6 +978,7 @@ uint32_t rpx_adjoint(uint32_t handle, uint32_t first, uint32_t count, const floa
     // The image is the mean of the samples: d image / d sample = f32(1 / count) (image.rs:158-169).
     s.adjoint_scale = 1.0f / (float)count;
     h->engine->textured = h->dev.tex_material != NO_TEXTURE;
+    h->engine->ext = false;
     e = h->engine->adjoint(h->d_scene, std::vector<DevSegment>(1, s), (float*)d_gm, (float*)d_gt, h->dev.tex_w,
                            h->dev.tex_h, err);
   }
@@ -1084,7 +1108,11 @@ uint32_t rpx_queue_run(uint32_t queue) {
     return RPX_GPU_LAUNCH;
   std::string err;
   q->engine->textured = false;
-  for (const DevScene& d : table) q->engine->textured = q->engine->textured || d.tex_material != NO_TEXTURE;
+  q->engine->ext = false;
+  for (const DevScene& d : table) {
+    q->engine->textured = q->engine->textured || d.tex_material != NO_TEXTURE;
+    q->engine->ext = q->engine->ext || d.ext != 0;
+  }
   hipError_t e = q->engine->run((const DevScene*)q->scenes.p, segs, true, err);
   if (e != hipSuccess) {
     set_error(err);
@@ -1172,7 +1200,11 @@ uint32_t rpx_queue_autotune(uint32_t queue, uint32_t paths, float* out60) {
   std::vector<float> ms;
   std::string err;
   q->engine->textured = false;
-  for (const DevScene& ds : table) q->engine->textured = q->engine->textured || ds.tex_material != NO_TEXTURE;
+  q->engine->ext = false;
+  for (const DevScene& ds : table) {
+    q->engine->textured = q->engine->textured || ds.tex_material != NO_TEXTURE;
+    q->engine->ext = q->engine->ext || ds.ext != 0;
+  }
   hipError_t r = q->engine->autotune((const DevScene*)d, probe, ms, err);
   hipFree(d);
   return tune_report(*q->engine, r, ms, err, out60);

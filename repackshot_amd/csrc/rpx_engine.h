@@ -48,8 +48,8 @@ struct EngineOptions {
 // GPU time of the last run, from events on the engine's stream (profiling on).
 struct Timing {
   float total_ms = 0;
-  float stage_ms[5] = {0, 0, 0, 0, 0};  // extend, shade, shadow, regen, drain
-  u32 launches[5] = {0, 0, 0, 0, 0};
+  float stage_ms[6] = {0, 0, 0, 0, 0, 0};  // extend, shade, shadow, regen, drain, walk
+  u32 launches[6] = {0, 0, 0, 0, 0, 0};
   u32 iterations = 0, chunks = 0;
 };
 
@@ -83,12 +83,14 @@ class Engine {
   // closest-hit candidates, then the shadow candidates).
   hipError_t autotune(const DevScene* d_scenes, const std::vector<DevSegment>& probe, std::vector<float>& ms,
                       std::string& err);
-  // Closest-hit rays, shadow rays counted without a trace, shadow rays traced (since init).
-  bool ray_counts(u64 out[3]);
+  // Closest-hit rays, shadow rays counted without a trace, shadow rays traced, thin-wall probes
+  // counted after media exits (since init).
+  bool ray_counts(u64 out[4]);
 
   EngineOptions opt;
   bool profiling = false;
   bool textured = false;  // a scene of the next runs has a texture (slots get a BSDF record)
+  bool ext = false;       // a scene of the next runs has sphere lights or scattering media
   Timing timing;
   PathState P{};
 

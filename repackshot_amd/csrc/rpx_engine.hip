@@ -184,8 +184,8 @@ hipEvent_t Engine::next_event() {
   return events_[events_used_++];
 }
 
-bool Engine::ray_counts(u64 out[3]) {
-  return hipMemcpyAsync(out, P.rays, 24, hipMemcpyDeviceToHost, stream_) == hipSuccess &&
+bool Engine::ray_counts(u64 out[4]) {
+  return hipMemcpyAsync(out, P.rays, 32, hipMemcpyDeviceToHost, stream_) == hipSuccess &&
          hipStreamSynchronize(stream_) == hipSuccess;
 }
 
@@ -200,8 +200,10 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegm
   L.pool = std::min(opt.pool, total);
   L.shape = opt.shape;
   L.shadow_shape = opt.shadow_shape;
-  // The drain runs the forward vertices only; the replay keeps the loop.
-  L.tail = adj ? 0u : opt.tail;
+  // The drain runs the forward vertices of the plain shade only; the replay and scenes with sphere
+  // lights or media keep the loop.
+  L.ext = ext ? 1u : 0u;
+  L.tail = (adj || ext) ? 0u : opt.tail;
   L.stream = stream_;
   e = launch_pool_init(P, L, total);
   if (e != hipSuccess) return e;
@@ -233,6 +235,12 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegm
       r = launch_extend(P, L, parity);
       done();
       if (r != hipSuccess) break;
+      if (L.ext) {
+        mark(5);
+        r = launch_walk(P, L, parity);
+        done();
+        if (r != hipSuccess) break;
+      }
       mark(1);
       r = launch_shade(P, L, parity, adj);
       done();

@@ -19,6 +19,10 @@
 //           (four conductors, two rough glass of IOR 1.5, two Lambert), a Lambert floor quad
 //           (material 8, the one a base-colour texture goes on), three rect lights and the studio
 //           dome; 512 x 512, 32 spp, depth 6
+//   kind 5  caustics and media: a 200,000-triangle bottle of smooth glass (IOR 1.5) holding a
+//           homogeneous scattering medium (sigma_s 0.5, sigma_a 0.05 per metre, g 0.7) on a Lambert
+//           floor, lit by one sphere light (radius 50 mm, radiance 500); 1920 x 1080, 64 spp,
+//           depth 16
 //   queue   job j of a seeded render queue (rpx_write_queue_job): image 256, 512, 768 or 1024 px
 //           square; 10k..500k triangles (log-uniform) in four value-noise displaced spheres
 //           (latitude / longitude grids) on a floor; materials drawn as in kind 2; one of 12
@@ -69,6 +73,9 @@ struct Mat {
   u32 kind = 0;
   C3 base{0.8f, 0.8f, 0.8f};
   float metallic = 0, roughness = 0.5f, ior = 1.5f, transmission = 0;
+  float scatter_mm = 0;
+  C3 scatter_color{1, 1, 1};
+  float scatter_g = 0;
 };
 struct MeshData {
   std::vector<V3> pos, nrm;
@@ -80,6 +87,7 @@ struct Cam {
   float focal = 50, sensor = 36, fstop = 0, focus = 1000, ortho_width = 1000;
 };
 struct Lgt {
+  u32 kind = 0;  // 0 rect, 3 point / sphere
   V3 position, direction, up{0, 0, 1};
   float sx, sy;
   C3 color{1, 1, 1};
@@ -122,9 +130,9 @@ void write_matl(Writer& w, const Mat& m) {
   w.f(0);  // anisotropy
   w.f(0);  // absorption
   w.f(0);  // bevel
-  w.f(0);  // scatter
-  w.rgb(c3(1, 1, 1));
-  w.f(0);  // g
+  w.f(m.scatter_mm);
+  w.rgb(m.scatter_color);
+  w.f(m.scatter_g);
   w.f(0);  // abbe
   w.end(s);
 }
@@ -148,7 +156,7 @@ void write_inst(Writer& w, const Inst& in) {
 }
 void write_lght(Writer& w, const Lgt& l) {
   size_t s = w.begin("LGHT");
-  w.u(0);
+  w.u(l.kind);
   w.vec(l.position);
   w.vec(l.direction);
   w.vec(l.up);
@@ -161,13 +169,13 @@ void write_lght(Writer& w, const Lgt& l) {
   w.u(l.flags);
   w.end(s);
 }
-void write_envr(Writer& w, u32 kind, float fl, float hz, float zn) {
+void write_envr(Writer& w, u32 kind, float fl, float hz, float zn, float strength = 1.0f) {
   size_t s = w.begin("ENVR");
   w.u(kind);
   w.f(fl);
   w.f(hz);
   w.f(zn);
-  w.f(1);  // strength
+  w.f(strength);
   w.f(0);  // rotation
   w.rgb(c3(0.05f, 0.05f, 0.05f));
   w.u(0);            // background: the environment
@@ -587,6 +595,111 @@ std::vector<uint8_t> config_fit(u32 seed) {
   return w.b;
 }
 
+// A closed surface of revolution about the z axis through rings of (height, radius): `segments`
+// around, smooth normals on the side, flat caps at both ends, wound outwards.
+MeshData revolution(const std::vector<float>& z, const std::vector<float>& r, u32 segments, u32 material) {
+  MeshData m;
+  m.material = material;
+  const u32 rings = (u32)z.size();
+  std::vector<float> cs(segments), sn(segments);
+  for (u32 j = 0; j < segments; ++j) sin_cos(TAU * (float)j / (float)segments, sn[j], cs[j]);
+  for (u32 i = 0; i < rings; ++i)
+    for (u32 j = 0; j < segments; ++j) m.pos.push_back(v3(r[i] * cs[j], r[i] * sn[j], z[i]));
+  for (u32 i = 0; i + 1 < rings; ++i)
+    for (u32 j = 0; j < segments; ++j) {
+      u32 jn = (j + 1) % segments;
+      u32 a = i * segments + j, b = i * segments + jn, c = (i + 1) * segments + jn, d = (i + 1) * segments + j;
+      m.idx.insert(m.idx.end(), {a, b, c, a, c, d});
+    }
+  m.nrm = vertex_normals(m.pos, m.idx);
+  // The caps: their own vertices (flat normals), a fan about the axis.
+  for (int end = 0; end < 2; ++end) {
+    u32 ring = end == 0 ? 0 : rings - 1;
+    float nz = end == 0 ? -1.0f : 1.0f;
+    u32 centre = (u32)m.pos.size();
+    m.pos.push_back(v3(0, 0, z[ring]));
+    m.nrm.push_back(v3(0, 0, nz));
+    for (u32 j = 0; j < segments; ++j) {
+      m.pos.push_back(v3(r[ring] * cs[j], r[ring] * sn[j], z[ring]));
+      m.nrm.push_back(v3(0, 0, nz));
+    }
+    for (u32 j = 0; j < segments; ++j) {
+      u32 a = centre + 1 + j, b = centre + 1 + (j + 1) % segments;
+      if (end == 0) m.idx.insert(m.idx.end(), {centre, b, a});
+      else m.idx.insert(m.idx.end(), {centre, a, b});
+    }
+  }
+  return m;
+}
+
+// Caustics-and-media workload: a bottle (surface of revolution, 300 mm tall, its radius profile
+// through 12 seeded control points, Catmull-Rom between them; 400 bands x 250 segments = 200,000
+// side triangles and two 250-triangle caps) of smooth glass (IOR 1.5) filled with a homogeneous
+// medium of sigma_s 0.5 and sigma_a 0.05 per metre (mean free path 1818.18 mm, single-scattering
+// albedo 0.5 / 0.55) and Henyey-Greenstein g 0.7, standing 1 mm above a Lambert floor (albedo
+// 0.8), lit by one sphere light of radius 50 mm and radiance 500 (power 500 pi r^2 4 pi W), no
+// dome; 1920 x 1080, 64 spp, depth 16.
+std::vector<uint8_t> config_bottle(u32 seed) {
+  Writer w;
+  w.b = {'R', 'S', 'B', '1', 0, 0, 0, 0};
+  Pcg32 rng = Pcg32::seeded((u64)seed, 5);
+  Cam cam;
+  cam.position = v3(0, -1150, 380);
+  cam.target = v3(0, 0, 150);
+  cam.focal = focal_for_fov(40.0f);
+  write_camr(w, cam);
+  Mat glass;
+  glass.kind = 1;
+  glass.base = c3(1, 1, 1);
+  glass.roughness = 0;
+  glass.ior = 1.5f;
+  glass.transmission = 1;
+  glass.scatter_mm = 1.0f / 0.55e-3f;
+  float albedo = 0.5f / 0.55f;
+  glass.scatter_color = c3(albedo, albedo, albedo);
+  glass.scatter_g = 0.7f;
+  Mat floor_m;
+  floor_m.base = c3(0.8f, 0.8f, 0.8f);
+  floor_m.ior = 1.0f;
+  floor_m.roughness = 1.0f;
+  write_matl(w, glass);
+  write_matl(w, floor_m);
+  // A wine-bottle outline: body, shoulder, neck, lip (mm), each point moved by up to 12 %.
+  const float shape[12] = {70, 74, 75, 75, 74, 72, 64, 44, 26, 20, 19, 21};
+  float ctrl[12];
+  for (int k = 0; k < 12; ++k) ctrl[k] = shape[k] * (1.0f + 0.12f * (2.0f * rng.next_f32() - 1.0f));
+  const u32 bands = 400, segments = 250;
+  const float height = 300.0f, base_z = 1.0f;
+  std::vector<float> zs, rs;
+  for (u32 i = 0; i <= bands; ++i) {
+    float t = (float)i / (float)bands * 11.0f;
+    int k = std::min(10, (int)t);
+    float u = t - (float)k;
+    float p0 = ctrl[k > 0 ? k - 1 : 0], p1 = ctrl[k], p2 = ctrl[k + 1], p3 = ctrl[k + 2 < 12 ? k + 2 : 11];
+    float u2 = u * u, u3 = u2 * u;
+    float radius = 0.5f * ((2.0f * p1) + (-p0 + p2) * u + (2.0f * p0 - 5.0f * p1 + 4.0f * p2 - p3) * u2 +
+                           (-p0 + 3.0f * p1 - 3.0f * p2 + p3) * u3);
+    zs.push_back(base_z + height * (float)i / (float)bands);
+    rs.push_back(fmax_r(radius, 5.0f));
+  }
+  write_mesh(w, revolution(zs, rs, segments, 0));
+  write_mesh(w, floor_quad(0, 3000, 1));
+  Lgt l;
+  l.kind = 3;
+  l.position = v3(350, -250, 650);
+  l.direction = v3(0, 0, -1);
+  l.up = v3(0, 1, 0);
+  l.sx = 100;  // diameter
+  l.sy = 100;
+  const float r_m = 0.05f;
+  l.power = 500.0f * PI * r_m * r_m * 4.0f * PI;
+  write_lght(w, l);
+  write_envr(w, 0, 0.1f, 0.3f, 0.6f, 0.0f);
+  write_flor(w);
+  write_rndr(w, 1920, 1080, 64, seed, 16);
+  return w.b;
+}
+
 std::vector<uint8_t> config_queue_job(u32 seed, u32 job) {
   QueueJob q = queue_job(seed, job);
   Writer w;
@@ -673,6 +786,7 @@ extern "C" uint64_t rpx_write_scene(uint32_t kind, uint32_t seed, uint8_t* buf, 
       case 1: bytes = rpx::config_plain(seed); break;
       case 2: bytes = rpx::config_large(seed); break;
       case 4: bytes = rpx::config_fit(seed); break;
+      case 5: bytes = rpx::config_bottle(seed); break;
       default: return 0;
     }
     last_kind = kind;

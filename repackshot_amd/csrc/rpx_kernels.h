@@ -16,8 +16,15 @@
 
 namespace rpx {
 
-constexpr u32 FLAG_PRIMARY = 1, FLAG_HAS_PDF = 2, FLAG_MEDIUM = 4, FLAG_BY_FLUX = 8;
-constexpr u32 BOUNCE_SHIFT = 16;  // flags bits 16..31: the bounce of the path's next vertex
+// Path flags. FLAG_SCATTER: the ray is inside a scattering medium (the material index in the
+// bits of med.x), so its walk runs before the next vertex; FLAG_EXIT_PDF: the vertex the ray left
+// was a medium's exit.
+constexpr u32 FLAG_PRIMARY = 1, FLAG_HAS_PDF = 2, FLAG_MEDIUM = 4, FLAG_BY_FLUX = 8, FLAG_SCATTER = 16,
+              FLAG_EXIT_PDF = 32;
+constexpr u32 EVENTS_SHIFT = 8, EVENTS_MASK = 0xfffu;  // flags bits 8..19: medium steps of the path so far
+constexpr u32 BOUNCE_SHIFT = 20;  // flags bits 20..31: the bounce of the path's next vertex
+// How a walk through a scattering medium ended (hit.w, written by the walk kernel; 0: no walk).
+constexpr u32 WALK_CROSSED = 1, WALK_EXIT = 2, WALK_ESCAPED = 3, WALK_ABSORBED = 4;
 constexpr u32 NO_WORK = 0xffffffffu;
 
 struct Film {
@@ -50,7 +57,7 @@ constexpr u32 N_COUNTS = 16;
 struct PathState {
   float4* ray_o;   // origin
   float4* ray_d;   // direction
-  uint4* hit;      // instance, mesh triangle, t bits (written by extend)
+  uint4* hit;      // instance, mesh triangle, t bits (written by extend), how a walk ended
   float4* thr;     // throughput rgb, density of the BSDF sample that made the ray
   float4* sum;     // radiance rgb, alpha
   float4* med;     // absorption coefficients of the solid the ray is in, flags (bits)
@@ -66,7 +73,8 @@ struct PathState {
   u32* shadow_queue;
   u32* done_queue;
   u32* counts;     // see C_*
-  u64* rays;       // [0] closest-hit rays traced, [1] shadow rays counted without a trace, [2] shadow rays traced
+  u64* rays;       // [0] closest-hit rays traced, [1] shadow rays counted without a trace, [2] shadow rays traced,
+                   // [3] thin-wall probes after a medium's exit (counted, never needed: no such walls)
   float4* results; // per work id of the current chunk: radiance rgb, alpha
   Bsdf* bsdf_tmp;  // per slot: the BSDF of a textured hit (allocated for scenes with a texture)
   const u32* sobol;  // 64 x 32 generator columns in logical-dimension order
@@ -103,6 +111,7 @@ struct Launch {
   int shape;         // closest-hit launch shape (index into SHAPES)
   int shadow_shape;  // any-hit launch shape
   u32 tail;          // once no new path is left, at most this many live paths are drained in one launch (0: never)
+  u32 ext;           // a scene has sphere lights or scattering media: the extended shade and the walk kernel
   hipStream_t stream;
 };
 
@@ -112,6 +121,9 @@ hipError_t launch_regen(const PathState& P, const Launch& L, u32 parity, bool ad
 hipError_t launch_extend(const PathState& P, const Launch& L, u32 parity);
 hipError_t launch_shade(const PathState& P, const Launch& L, u32 parity, bool adj = false);
 hipError_t launch_shadow(const PathState& P, const Launch& L, bool adj = false);
+// The walks of the live paths inside scattering media, from the closest hit extend found to where
+// each leaves, scatters out or is absorbed.
+hipError_t launch_walk(const PathState& P, const Launch& L, u32 parity);
 // The drain: when the last bookkeeping handed paths over (C_TAIL), takes each to its end.
 hipError_t launch_finish(const PathState& P, const Launch& L, u32 parity);
 // Adds the chunk's samples of segment `seg` (host copy `s`) to its film, in sample order.

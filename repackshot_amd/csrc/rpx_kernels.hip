@@ -988,8 +988,13 @@ __device__ __forceinline__ float area_weight(const DevArea& a, V3 x) {
   if (orientation <= 0.0f) return 0.0f;
   return a.base * orientation / (fmax_r(d2, a.area) * 1.0e-6f);
 }
+// EXT: entries may be sphere lights (-2 - k).
+template <bool EXT>
 __device__ __forceinline__ float entry_weight(const DevScene& S, u32 e, V3 x) {
   i32 a = S.entries[e];
+  if constexpr (EXT) {
+    if (a <= -2) return sphere_weight(S.spheres[-2 - a], x);
+  }
   return a < 0 ? S.env_base : area_weight(S.areas[a], x);
 }
 __device__ __forceinline__ C3 radiance_at(const DevArea& a, float cos) {
@@ -1018,14 +1023,37 @@ struct Sel {
   u32 by_flux;
   float w[KEPT_WEIGHTS];
 };
+template <bool EXT>
 __device__ __forceinline__ float sel_weight(const DevScene& S, const Sel& s, u32 e) {
-  return e < (u32)KEPT_WEIGHTS ? s.w[e] : entry_weight(S, e, s.point);
+  return e < (u32)KEPT_WEIGHTS ? s.w[e] : entry_weight<EXT>(S, e, s.point);
 }
+template <bool EXT>
 __device__ __forceinline__ float sel_pmf(const DevScene& S, const Sel& s, u32 e) {
   if (e >= S.n_entries) return 0.0f;
   if (s.by_flux) return S.flux_pmf[e];
   if (!(s.total > 0.0f && is_finite(s.total))) return 0.0f;
-  return sel_weight(S, s, e) / s.total;
+  return sel_weight<EXT>(S, s, e) / s.total;
+}
+
+// The exit lobe of a scattering medium (bsdf.rs:549-556, 697-701, 783-785, 837-847): Lambertian
+// transmission of albedo `exit` through the surface, nothing reflected.
+__device__ __forceinline__ C3 exit_eval(float exit, V3 wo, V3 wi) {
+  if (wo.z <= 0.0f) return splat(0.0f);
+  return wi.z < 0.0f ? splat(exit * FRAC_1_PI) : splat(0.0f);
+}
+__device__ __forceinline__ float exit_pdf(V3 wo, V3 wi) {
+  if (wo.z <= 0.0f) return 0.0f;
+  return fmax_r(-wi.z, 0.0f) * FRAC_1_PI;
+}
+__device__ __forceinline__ bool exit_sample(float exit, V3 wo, float u0, float u1, BsdfSample& out) {
+  if (wo.z <= 0.0f) return false;
+  V3 d = cosine_hemisphere(u0, u1);
+  float pdf = d.z * FRAC_1_PI;
+  if (!(pdf > 0.0f)) return false;
+  out.wi = v3(d.x, d.y, -d.z);
+  out.weight = splat(exit);
+  out.pdf = pdf;
+  return true;
 }
 
 __device__ __forceinline__ void add_to(C3& sum, C3 c, bool indirect, float clamp_ind) {
@@ -1074,7 +1102,9 @@ __device__ __forceinline__ float dot3(C3 a, C3 b) { return a.r * b.r + a.g * b.g
 // `g_prev` the term of the previous vertex's BSDF sample (it needs the radiance the path gathered
 // after it, the forward's total minus what is summed so far), `g_now` the next-event term of this
 // vertex when its light casts no shadow; a shadowed next-event term waits for the shadow kernel.
-template <bool ADJ>
+// EXT adds sphere, point and spot lights and the vertices after a walk through a scattering medium
+// (the walk kernel ran between extend and shade: hit.w says how it ended).
+template <bool ADJ, bool EXT>
 __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P, u32 i, bool& cont, bool& want_shadow,
                                            VertexGrad& g_prev, VertexGrad& g_now) {
   const EnergyTables& T = S.tables;
@@ -1097,6 +1127,15 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
   const u32 bounce = flags >> BOUNCE_SHIFT;
   const u32 dims = bounce_dim(bounce);
   const u32 sample_index = P.ident[i].w;
+  // After a walk that scattered (integrator/mod.rs:1209-1242): the exit lobe's albedo at the
+  // surface it reached; an absorbed walk ends the path.
+  bool exit = false;
+  float exit_eta2 = 0.0f;
+  if constexpr (EXT) {
+    if (h.w == WALK_ABSORBED) return;
+    exit = h.w == WALK_EXIT;
+    exit_eta2 = rd.w;
+  }
   C3 adj = splat(0.0f);
   if constexpr (ADJ) {
     float4 a = P.adj[i];
@@ -1130,7 +1169,7 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
   bool hit_ok = h.x != 0xffffffffu;
   float t_geometry = hit_ok ? u2f(h.z) : RPX_INF;
   float t_floor = RPX_INF;
-  if ((S.floor_mode == 1) && d.z != 0.0f) {
+  if ((S.floor_mode == 1) && d.z != 0.0f && !exit) {
     float tf = (S.floor_z - o.z) / d.z;
     if (tf > RAY_EPSILON && tf < RPX_INF) t_floor = tf;
   }
@@ -1139,9 +1178,10 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
     C3 sg = c3(md.x, md.y, md.z);
     thr = thr * c3(fexp(-sg.r * t_nearest), fexp(-sg.g * t_nearest), fexp(-sg.b * t_nearest));
   }
-  // Area lights crossed before the surface (light.rs:1354-1384).
+  // Area lights crossed before the surface (light.rs:1354-1384); after a walk, the segment to its
+  // exit vertex crosses none.
   C3 crossed = splat(0.0f);
-  for (u32 a = 0; a < S.n_areas; ++a) {
+  for (u32 a = 0; a < (exit ? 0u : S.n_areas); ++a) {
     const DevArea& L = S.areas[a];
     bool visible = has_pdf ? L.glossy : L.camera;
     if (!visible) continue;
@@ -1150,10 +1190,32 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
     float weight = 1.0f;
     if (has_pdf) {
       float dd = 0.0f + t;
-      float light_pdf = sel_pmf(S, from, L.entry) * dd * dd / (L.area * cos);
+      float light_pdf = sel_pmf<EXT>(S, from, L.entry) * dd * dd / (L.area * cos);
       weight = power_heuristic(prev_pdf, light_pdf);
     }
     crossed = crossed + radiance_at(L, cos) * weight;
+  }
+  if constexpr (EXT) {
+    // The spheres after the areas, seen from the vertex the ray left (light.rs:1386-1425).
+    for (u32 k = 0; k < (exit ? 0u : S.n_spheres); ++k) {
+      const DevSphere& L = S.spheres[k];
+      bool visible = has_pdf ? L.glossy : L.camera;
+      if (!visible) continue;
+      float t;
+      if (!sphere_intersect(L, o, d, t_nearest, t)) continue;
+      V3 vertex = o - d * 0.0f;
+      float mask = sphere_mask(L, vertex);
+      if (!(mask > 0.0f)) continue;
+      float weight = 1.0f;
+      if (has_pdf) {
+        float omc;
+        if (sphere_cone(L, length_sq(L.centre - vertex), omc) && omc > 0.0f)
+          weight = power_heuristic(prev_pdf, sel_pmf<EXT>(S, from, L.entry) / (TAU * omc));
+        else
+          weight = 0.0f;
+      }
+      crossed = crossed + L.radiance * (mask * weight);
+    }
   }
   add_to(sum, thr * crossed, bounce >= 2, S.clamp_indirect);
   bool done = false;
@@ -1171,7 +1233,7 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
     } else {
       C3 sky = splat(env_value(S, d.z));
       if (has_pdf && S.has_env_entry)
-        sky = sky * power_heuristic(prev_pdf, sel_pmf(S, from, S.env_entry) * env_pdf(S, d.z));
+        sky = sky * power_heuristic(prev_pdf, sel_pmf<EXT>(S, from, S.env_entry) * env_pdf(S, d.z));
       add_to(sum, thr * sky, bounce >= 2, S.clamp_indirect);
     }
     done = true;
@@ -1333,7 +1395,7 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
     if (S.per_point && ne > 0) {
       sel.by_flux = 0;
       for (u32 e = 0; e < ne; ++e) {
-        float w = entry_weight(S, e, s_point);
+        float w = entry_weight<EXT>(S, e, s_point);
         if (e < (u32)KEPT_WEIGHTS) sel.w[e] = w;
         sel.total += w;
       }
@@ -1346,7 +1408,7 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
     V3 ls_wi = v3(0, 0, 0);
     float ls_dist = 0.0f, ls_pdf = 0.0f;
     C3 ls_rad = splat(0.0f);
-    bool ls_glossy = true, ls_shadow = true;
+    bool ls_glossy = true, ls_shadow = true, ls_delta = false;
     if (ne > 0) {
       u32 index = 0;
       float p = 0.0f;
@@ -1355,14 +1417,14 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
         u32 k = 0;
         while (k < ne && S.cdf[k] <= u_sel) ++k;
         index = k < ne - 1 ? k : ne - 1;
-        p = sel_pmf(S, sel, index);
+        p = sel_pmf<EXT>(S, sel, index);
       } else if (!(sel.total > 0.0f && is_finite(sel.total))) {
         ok = false;
       } else {
         float acc = 0.0f, lo_c = 0.0f;
         u32 last = ne - 1;
         for (u32 e = 0; e <= last; ++e) {
-          float w = sel_weight(S, sel, e);
+          float w = sel_weight<EXT>(S, sel, e);
           acc += w;
           float hi = e == last ? 1.0f : acc / sel.total;
           if (hi > u_sel || e == last) {
@@ -1376,7 +1438,44 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
       }
       if (ok && p > 0.0f) {
         i32 a = S.entries[index];
-        if (a >= 0) {
+        bool sphere = false;
+        if constexpr (EXT) sphere = a <= -2;
+        if (sphere) {
+          // light.rs:1224-1268: a point (or a sphere too small to tell from one) by its
+          // intensity over d^2, else a direction in the cone the sphere subtends.
+          const DevSphere& L = S.spheres[-2 - a];
+          float mask = sphere_mask(L, s_point);
+          V3 dv = L.centre - s_point;
+          float dist2 = length_sq(dv);
+          if (mask > 0.0f && dist2 > 0.0f) {
+            float dist = sqrt_(dist2);
+            V3 towards = dv / dist;
+            float omc = 0.0f;
+            if (!(L.radius > 0.0f) || sphere_cone(L, dist2, omc)) {
+              if (omc == 0.0f) {
+                ls_wi = towards;
+                ls_dist = dist;
+                ls_rad = L.intensity * (mask / (dist2 * 1.0e-6f));
+                ls_pdf = p;
+                ls_delta = true;
+              } else {
+                V3 local = uniform_cone(ul0, ul1, omc);
+                V3 wi = normalize(to_world(frame_from_normal(towards), local));
+                float b = dot(wi, dv);
+                float r2 = L.radius * L.radius;
+                float near_t = b - sqrt_(fmax_r(r2 - length_sq(dv - wi * b), 0.0f));
+                ls_wi = wi;
+                ls_dist = fmax_r(near_t, 0.0f);
+                ls_rad = L.radiance * mask;
+                ls_pdf = p / (TAU * omc);
+                ls_delta = false;
+              }
+              ls_glossy = L.glossy != 0;
+              ls_shadow = L.shadow != 0;
+              have_ls = true;
+            }
+          }
+        } else if (a >= 0) {
           const DevArea& L = S.areas[a];
           float x0, y0;
           if (L.rect) {
@@ -1434,10 +1533,12 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
     }
     if (have_ls) {
       V3 wi = to_local(frame, ls_wi);
-      C3 f = bsdf_eval(T, B, wo, wi, !ls_glossy);
+      C3 f = exit ? exit_eval(exit_eta2, wo, wi) : bsdf_eval(T, B, wo, wi, !ls_glossy);
       if (peak(f) > 0.0f) {
         V3 from_p = wi.z < 0.0f ? offset_origin(s_hit_point, -ng) : origin;
-        float weight = !ls_glossy ? 1.0f : power_heuristic(ls_pdf, bsdf_pdf(T, B, wo, wi));
+        float weight = (ls_delta || !ls_glossy)
+                           ? 1.0f
+                           : power_heuristic(ls_pdf, exit ? exit_pdf(wo, wi) : bsdf_pdf(T, B, wo, wi));
         C3 c = thr * f * ls_rad * (fabs_(wi.z) * weight / ls_pdf);
         // What `add` would add (finite, clamped), queued behind the shadow ray.
         C3 add_c = splat(0.0f);
@@ -1492,7 +1593,7 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
     float ub0 = sample_at(P.sobol, key, sample_index, dims + OFFSET_BSDF);
     float ub1 = sample_at(P.sobol, key, sample_index, dims + OFFSET_BSDF + 1);
     BsdfSample bs;
-    if (bsdf_sample(T, B, wo, ub0, ub1, bs)) {
+    if (exit ? exit_sample(exit_eta2, wo, ub0, ub1, bs) : bsdf_sample(T, B, wo, ub0, ub1, bs)) {
       bool transmitted = bs.wi.z < 0.0f;
       V3 wi = normalize(to_world(frame, bs.wi));
       bool ok = transmitted ? !(dot(wi, ng) >= 0.0f) : !(dot(wi, ng) <= 0.0f);
@@ -1511,14 +1612,32 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
         u32 nflags = FLAG_HAS_PDF | (sel.by_flux ? FLAG_BY_FLUX : 0u) | ((bounce + 1) << BOUNCE_SHIFT);
         C3 sg = c3(md.x, md.y, md.z);
         bool med = in_medium;
+        bool scatter = false;
+        if constexpr (EXT) {
+          // The medium steps carry over; a sample transmitted into a solid right after an exit
+          // vertex is probed for a thin wall (integrator/mod.rs:1822-1857: one ray counted).
+          nflags |= flags & (EVENTS_MASK << EVENTS_SHIFT);
+          if (exit) nflags |= FLAG_EXIT_PDF;
+          else if ((flags & FLAG_EXIT_PDF) && s_instance && transmitted && s_outside)
+            atomicAdd((unsigned long long*)&P.rays[3], 1ull);
+          scatter = (flags & FLAG_SCATTER) != 0;
+        }
         if (transmitted) {
           med = false;
-          if (s_outside && s_instance && mat < S.n_materials && S.materials[mat].has_medium) {
-            med = true;
-            sg = S.materials[mat].sigma;
+          scatter = false;
+          if (s_outside && s_instance && mat < S.n_materials) {
+            u32 hm = S.materials[mat].has_medium;
+            if (EXT ? hm == MEDIUM_ABSORBING : hm != 0) {
+              med = true;
+              sg = S.materials[mat].sigma;
+            } else if (EXT && hm == MEDIUM_SCATTERING) {
+              scatter = true;
+              sg = c3(u2f(mat), 0.0f, 0.0f);
+            }
           }
         }
         if (med) nflags |= FLAG_MEDIUM;
+        if (scatter) nflags |= FLAG_SCATTER;
         P.ray_o[i] = make_float4(next_o.x, next_o.y, next_o.z, 0.0f);
         P.ray_d[i] = make_float4(wi.x, wi.y, wi.z, 0.0f);
         P.med[i] = make_float4(sg.r, sg.g, sg.b, u2f(nflags));
@@ -1552,7 +1671,7 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
 
 // Three waves per SIMD: the register budget (168) the shade fits without spilling; two waves leave
 // the loads of a path's records unhidden, four spill.
-template <bool ADJ>
+template <bool ADJ, bool EXT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 k_shade(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
   const u32* q = P.live[parity];
@@ -1581,7 +1700,7 @@ k_shade(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
     while (pending) {
       u32 cur = lowest_lane_value(pending, sid);
       if (todo && sid == cur) {
-        shade_path<ADJ>(scenes[cur], P, i, cont, want_shadow, g_prev, g_now);
+        shade_path<ADJ, EXT>(scenes[cur], P, i, cont, want_shadow, g_prev, g_now);
         todo = false;
       }
       pending = __ballot(todo);
@@ -1601,6 +1720,225 @@ k_shade(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
     if (done) P.done_queue[slots[1]] = i;
     if (want_shadow) P.shadow_queue[slots[2]] = i;
   }
+}
+
+// ================================================================== scattering media
+namespace {
+// integrator/medium.rs:143-150: the Henyey-Greenstein phase function per steradian.
+__device__ __forceinline__ float hg_phase(float cos, float g) {
+  float d = 1.0f + g * g - 2.0f * g * cos;
+  return (1.0f - g * g) / (4.0f * PI * d * sqrt_(fmax_r(d, 1.0e-12f)));
+}
+// integrator/medium.rs:152-169: a direction about `dir` from the phase function, and its density.
+__device__ __forceinline__ V3 hg_sample(V3 dir, float g, float u0, float u1, float& density) {
+  float cos;
+  if (fabs_(g) < 1.0e-3f) {
+    cos = 1.0f - 2.0f * u0;
+  } else {
+    float s = (1.0f - g * g) / (1.0f - g + 2.0f * g * u0);
+    cos = clampf((1.0f + g * g - s * s) / (2.0f * g), -1.0f, 1.0f);
+  }
+  float sin = sqrt_(fmax_r(1.0f - cos * cos, 0.0f));
+  float sp, cp;
+  sin_cos(TAU * u1, sp, cp);
+  Frame frame = frame_from_normal(dir);
+  density = hg_phase(cos, g);
+  return normalize(to_world(frame, v3(sin * cp, sin * sp, cos)));
+}
+__device__ __forceinline__ C3 transmittance(C3 sigma, float t) {
+  return c3(fexp(-sigma.r * t), fexp(-sigma.g * t), fexp(-sigma.b * t));
+}
+__device__ __forceinline__ float channel(C3 c, u32 k) { return k == 0 ? c.r : (k == 1 ? c.g : c.b); }
+// The hero channel's estimate of the throughput (spectral MIS over the three channels).
+__device__ __forceinline__ C3 effective(C3 w, C3 ratio) { return w * (3.0f / (ratio.r + ratio.g + ratio.b)); }
+
+// The hit point and the geometric normal facing the ray of triangle `tri` of instance `inst`
+// (geometry/mod.rs:731-808): what the walk needs at the solid's boundary.
+__device__ void boundary_of(const DevScene& S, u32 inst, u32 tri, V3 o, V3 d, V3& point, V3& ng) {
+  const DevInstance& in = S.instances[inst];
+  const DevMesh& mesh = S.meshes[in.mesh];
+  bool ident = (in.flags & INST_IDENTITY) != 0;
+  const u32* idx = S.indices + mesh.index_offset + 3 * (size_t)tri;
+  u32 i0 = mesh.vertex_offset + idx[0], i1 = mesh.vertex_offset + idx[1], i2 = mesh.vertex_offset + idx[2];
+  const float* ps = S.positions;
+  V3 p0 = v3(ps[3 * (size_t)i0], ps[3 * (size_t)i0 + 1], ps[3 * (size_t)i0 + 2]);
+  V3 p1 = v3(ps[3 * (size_t)i1], ps[3 * (size_t)i1 + 1], ps[3 * (size_t)i1 + 2]);
+  V3 p2 = v3(ps[3 * (size_t)i2], ps[3 * (size_t)i2 + 1], ps[3 * (size_t)i2 + 2]);
+  V3 lo = ident ? o : xf_point(in.to_local, o), ld = ident ? d : xf_vector(in.to_local, d);
+  PRay r = prepare(lo, ld, RAY_EPSILON, RPX_INF);
+  float b0, b1, b2;
+  {
+    float u, v, w, az, bz, cz;
+    shear(r, p0 - lo, p1 - lo, p2 - lo, u, v, w, az, bz, cz);
+    bool negative = u < 0.0f || v < 0.0f || w < 0.0f;
+    bool positive = u > 0.0f || v > 0.0f || w > 0.0f;
+    float det = u + v + w;
+    bool ok = !(negative && positive) && det != 0.0f;
+    if (ok) {
+      float t = (u * az + v * bz + w * cz) / det;
+      ok = t >= r.t_min && t <= F32_MAX;
+    }
+    if (ok) {
+      b0 = u / det;
+      b1 = v / det;
+      b2 = w / det;
+    } else {
+      b0 = b1 = b2 = 1.0f / 3.0f;
+    }
+  }
+  V3 point_local = p0 * b0 + p1 * b1 + p2 * b2;
+  V3 wind = cross(p1 - p0, p2 - p0);
+  V3 winding = normalize(ident ? wind : xf_transpose(in.to_local, wind));
+  V3 geometric = length_sq(winding) > 0.0f ? winding : -normalize(d);
+  ng = dot(geometric, d) > 0.0f ? -geometric : geometric;
+  point = ident ? point_local : xf_point(in.to_world, point_local);
+}
+}  // namespace
+
+// The random walk of every live path inside a scattering medium (integrator/medium.rs:226-360),
+// one lane per path, from the closest hit extend found for its ray to the walk's end: it crosses
+// without an event (the surface's own BSDF follows), leaves by the exit lobe after scattering,
+// escapes, or is absorbed. Distances are drawn in a hero channel picked per walk, the others
+// weighted by spectral MIS; after MEDIUM_EVENTS steps the path goes straight to the boundary.
+// Each trace after the first is a closest-hit ray of the reference's count.
+template <int BLK, int LDSN>
+__global__ void __launch_bounds__(BLK) k_walk(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
+  __shared__ u32 lds[LDSN * BLK];
+  u32* col = lds + threadIdx.x;
+  const u32* q = P.live[parity];
+  const u32 n = P.counts[C_LIVE0 + parity];
+  const u32 stride = gridDim.x * blockDim.x;
+  u32 traced = 0;
+  for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    const u32 i = q[j];
+    const float4 md = P.med[i];
+    const u32 flags = f2u(md.w);
+    if (!(flags & FLAG_SCATTER)) continue;
+    const uint4 id = P.ident[i];
+    const DevScene& S = scenes[id.y];
+    const DevMedium M = S.media[f2u(md.x)];
+    const u64 key = P.key[i];
+    const u32 index = id.w;
+    const u32 bounce = flags >> BOUNCE_SHIFT;
+    u32 path_events = (flags >> EVENTS_SHIFT) & EVENTS_MASK;
+    float4 ro = P.ray_o[i], rd = P.ray_d[i];
+    V3 o = v3(ro.x, ro.y, ro.z), d = v3(rd.x, rd.y, rd.z);
+    uint4 h = P.hit[i];
+    bool has_hit = h.x != 0xffffffffu;
+    u32 hit_inst = h.x, hit_tri = h.y;
+    float hit_t = u2f(h.z);
+    const float4 th = P.thr[i];
+    const C3 entry = c3(th.x, th.y, th.z);
+    float uh = sample_at(P.sobol, key, index, medium_dim(bounce, 0) + MEDIUM_OFFSET_HERO) * 3.0f;
+    u32 hero = f_to_u32(uh);
+    hero = hero < 2u ? hero : 2u;
+    const float sigma_hero = channel(M.sigma_t, hero);
+    C3 w = splat(1.0f), ratio = splat(1.0f), thr = entry;
+    u32 events = 0, steps = 0, end = WALK_ABSORBED;
+    float pdf = 0.0f;
+    const Geo g = geo_of(scenes, id.y);
+    for (;;) {
+      if (steps > 0) {
+        ++traced;
+        hit_inst = hit_tri = 0xffffffffu;
+        hit_t = RPX_INF;
+        has_hit = trace_nearest<BLK, LDSN>(g, col, o, d, RAY_EPSILON, RPX_INF, false, hit_inst, hit_tri, hit_t);
+      }
+      float t_hit = has_hit ? hit_t : RPX_INF;
+      bool capped = path_events >= MEDIUM_EVENTS || is_nan(sigma_hero) || sigma_hero <= 0.0f;
+      u32 dims = medium_dim(bounce, steps);
+      steps += 1;
+      float t = RPX_INF;
+      if (!capped) {
+        float u = sample_at(P.sobol, key, index, dims + MEDIUM_OFFSET_DISTANCE);
+        t = -fln(1.0f - u) / sigma_hero;
+      }
+      if (t < t_hit) {
+        // A scattering event.
+        C3 tr = transmittance(M.sigma_t, t);
+        C3 pe = M.sigma_t * tr;
+        C3 fe = M.sigma_s * tr;
+        float ph = channel(pe, hero);
+        if (is_nan(ph) || ph <= 0.0f) break;
+        w = w * (fe / ph);
+        ratio = ratio * (pe / ph);
+        float mx = peak(ratio);
+        if (mx > 0.0f) {
+          w = w / mx;
+          ratio = ratio / mx;
+        }
+        events += 1;
+        path_events += 1;
+        C3 current = entry * effective(w, ratio);
+        if (!(peak(current) > 0.0f && is_finite(peak(current)))) break;
+        if (events > 32) {
+          float survive = clampf(peak(current) / peak(entry), 0.05f, 1.0f);
+          if (sample_at(P.sobol, key, index, dims + MEDIUM_OFFSET_ROULETTE) >= survive) break;
+          w = w / survive;
+        }
+        V3 x = o + d * t;
+        float u0 = sample_at(P.sobol, key, index, dims + MEDIUM_OFFSET_PHASE);
+        float u1 = sample_at(P.sobol, key, index, dims + MEDIUM_OFFSET_PHASE + 1);
+        float density;
+        V3 dir = hg_sample(d, M.g, u0, u1, density);
+        pdf = density;
+        o = x;
+        d = dir;
+        continue;
+      }
+      // The boundary (or, past the last step, straight to it keeping all but the absorption).
+      if (is_finite(t_hit)) {
+        if (capped) {
+          w = w * transmittance(M.sigma_a, t_hit);
+        } else {
+          C3 pe = transmittance(M.sigma_t, t_hit);
+          float ph = channel(pe, hero);
+          if (is_nan(ph) || ph <= 0.0f) break;
+          w = w * (pe / ph);
+          ratio = ratio * (pe / ph);
+        }
+      }
+      thr = entry * effective(w, ratio);
+      if (!(peak(thr) > 0.0f && is_finite(peak(thr)))) break;
+      if (!has_hit) {
+        end = WALK_ESCAPED;
+        break;
+      }
+      if (events == 0) {
+        end = WALK_CROSSED;
+        break;
+      }
+      V3 point, ng;
+      boundary_of(S, hit_inst, hit_tri, o, d, point, ng);
+      float cos = clampf(-dot(d, ng), 0.0f, 1.0f);
+      // integrator/medium.rs:218-224: leave with the probability of the Fresnel transmittance out.
+      bool leave = capped || sample_at(P.sobol, key, index, dims + MEDIUM_OFFSET_FRESNEL) >=
+                                 fresnel_dielectric(cos, 1.0f / fmax_r(M.ior, 1.0001f));
+      if (leave) {
+        end = WALK_EXIT;
+        break;
+      }
+      // Reflected back in (a step of the path, for the cap).
+      path_events += 1;
+      V3 dir = normalize(d + ng * (2.0f * cos));
+      o = offset_origin(point, ng);
+      d = dir;
+    }
+    u32 nflags = (flags & ~(EVENTS_MASK << EVENTS_SHIFT)) | (path_events << EVENTS_SHIFT);
+    if (end == WALK_EXIT) nflags = (nflags & ~(FLAG_SCATTER | FLAG_PRIMARY)) | FLAG_HAS_PDF;
+    if (end != WALK_ABSORBED) {
+      P.ray_o[i] = make_float4(o.x, o.y, o.z, ro.w);
+      P.ray_d[i] = make_float4(d.x, d.y, d.z, end == WALK_EXIT ? M.ior * M.ior : 0.0f);
+      P.thr[i] = make_float4(thr.r, thr.g, thr.b, end == WALK_EXIT ? pdf : th.w);
+      P.med[i] = make_float4(md.x, md.y, md.z, u2f(nflags));
+    }
+    P.hit[i] = has_hit ? make_uint4(hit_inst, hit_tri, f2u(hit_t), end) : make_uint4(0xffffffffu, 0xffffffffu, f2u(RPX_INF), end);
+  }
+  // One atomic per wave.
+  u32 wn = traced;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) wn += __shfl_xor(wn, off);
+  if (__lane_id() == 0 && wn) atomicAdd((unsigned long long*)&P.rays[0], (unsigned long long)wn);
 }
 
 // ================================================================== drain
@@ -1654,7 +1992,7 @@ __global__ void __launch_bounds__(BLK) k_finish(const DevScene* __restrict__ sce
       while (pending) {
         u32 cur = lowest_lane_value(pending, sid);
         if (todo && sid == cur) {
-          shade_path<false>(scenes[cur], P, i, cont, want_shadow, g_prev, g_now);
+          shade_path<false, false>(scenes[cur], P, i, cont, want_shadow, g_prev, g_now);
           todo = false;
         }
         pending = __ballot(todo);
@@ -1806,6 +2144,11 @@ __device__ First first_hit(const DevScene& S, u32* lds_col, V3 o, V3 d) {
     const DevArea& L = S.areas[a];
     float t, cos;
     if (L.camera && area_intersect(L, o, d, t_nearest, t, cos)) return f;
+  }
+  for (u32 k = 0; k < S.n_spheres; ++k) {
+    const DevSphere& L = S.spheres[k];
+    float t;
+    if (L.camera && sphere_intersect(L, o, d, t_nearest, t) && sphere_mask(L, o) > 0.0f) return f;
   }
   if (t_floor < t_geometry) {
     f.kind = FIRST_FLOOR;
@@ -2183,10 +2526,21 @@ hipError_t launch_extend(const PathState& P, const Launch& L, u32 parity) {
   return hipGetLastError();
 }
 hipError_t launch_shade(const PathState& P, const Launch& L, u32 parity, bool adj) {
+  if (adj && L.ext) return hipErrorInvalidValue;  // the replay does not follow media or sphere lights
   if (adj)
-    hipLaunchKernelGGL(k_shade<true>, dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, P, parity);
+    hipLaunchKernelGGL((k_shade<true, false>), dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, P,
+                       parity);
+  else if (L.ext)
+    hipLaunchKernelGGL((k_shade<false, true>), dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, P,
+                       parity);
   else
-    hipLaunchKernelGGL(k_shade<false>, dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, P, parity);
+    hipLaunchKernelGGL((k_shade<false, false>), dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, P,
+                       parity);
+  return hipGetLastError();
+}
+hipError_t launch_walk(const PathState& P, const Launch& L, u32 parity) {
+  hipLaunchKernelGGL((k_walk<128, 24>), dim3(grid_for(L.pool, 128, 1u << 20)), dim3(128), 0, L.stream, L.scenes, P,
+                     parity);
   return hipGetLastError();
 }
 hipError_t launch_shadow(const PathState& P, const Launch& L, bool adj) {

@@ -285,10 +285,30 @@ RPX_HD V3 cosine_hemisphere(float u0, float u1) {
   return v3(x, y, z);
 }
 
+// warp.rs:38-45: 1 - cos uniform in [0, 1 - cos_max], the sine without cancellation.
+RPX_HD V3 uniform_cone(float u0, float u1, float one_minus_cos_max) {
+  float one_minus_cos = u0 * one_minus_cos_max;
+  float cos = 1.0f - one_minus_cos;
+  float sin = sqrt_(fmax_r(one_minus_cos * (2.0f - one_minus_cos), 0.0f));
+  float sp, cp;
+  sin_cos(TAU * u1, sp, cp);
+  return v3(sin * cp, sin * sp, cos);
+}
+
 // ---------------------------------------------------------------- sampler.rs
 constexpr u32 DIM_CAMERA = 0, DIM_LENS = 2, DIM_FIRST_BOUNCE = 4, DIMS_PER_BOUNCE = 6;
 constexpr u32 OFFSET_LIGHT_SELECT = 0, OFFSET_LIGHT = 1, OFFSET_BSDF = 3, OFFSET_ROULETTE = 5;
 RPX_HD u32 bounce_dim(u32 b) { return DIM_FIRST_BOUNCE + b * DIMS_PER_BOUNCE; }
+// sampler.rs:137-164: the hashed dimensions of a scattering medium's walk, 8 per event of bounce
+// b, beyond the Sobol table.
+constexpr u32 DIM_MEDIUM = 1u << 21, MEDIUM_DIMS_PER_EVENT = 8, MEDIUM_EVENTS = 1024, MEDIUM_BOUNCES = 128;
+constexpr u32 MEDIUM_OFFSET_DISTANCE = 0, MEDIUM_OFFSET_PHASE = 1, MEDIUM_OFFSET_ROULETTE = 3,
+              MEDIUM_OFFSET_FRESNEL = 4, MEDIUM_OFFSET_HERO = 5;
+RPX_HD u32 medium_dim(u32 bounce, u32 event) {
+  u32 b = bounce < MEDIUM_BOUNCES ? bounce : MEDIUM_BOUNCES - 1;
+  u32 e = event < MEDIUM_EVENTS ? event : MEDIUM_EVENTS - 1;
+  return DIM_MEDIUM + (b * MEDIUM_EVENTS + e) * MEDIUM_DIMS_PER_EVENT;
+}
 
 RPX_HD u32 reverse_bits(u32 v) { return __builtin_bitreverse32(v); }
 RPX_HD u32 owen_scramble(u32 x, u32 seed) {

@@ -47,6 +47,9 @@ struct DevMesh {
 
 // What the integrator reads of a material, and the parameters its BSDF is built from (for
 // textured materials, whose BSDF is built per hit, and for the adjoint's parameter derivatives).
+// `has_medium`: 0 none, MEDIUM_ABSORBING (Beer-Lambert with `sigma`), MEDIUM_SCATTERING (a random
+// walk in the scene's `media` entry of the material).
+constexpr u32 MEDIUM_ABSORBING = 1, MEDIUM_SCATTERING = 2;
 struct DevMaterial {
   u32 emissive, has_medium;
   C3 radiance;
@@ -69,6 +72,69 @@ struct DevArea {
   u32 entry;
 };
 
+// Point, sphere and spot lights (light.rs:364-420, 793-863): a sphere of `radius` (0: a point)
+// at `centre`, masked by a spot's cone about `axis` (cos_outer below -1: no cone).
+struct DevSphere {
+  V3 centre;
+  float radius;
+  V3 axis;
+  float cos_outer, cos_inner;
+  C3 radiance, intensity;
+  float base, near2;
+  u32 glossy, camera, shadow, entry;
+};
+
+// light.rs:364-373: the spot's mask at x (1 without a cone; the smoothstep across the soft edge).
+RPX_HD float sphere_mask(const DevSphere& L, V3 x) {
+  if (L.cos_outer < -1.0f) return 1.0f;
+  V3 d = x - L.centre;
+  float len2 = length_sq(d);
+  if (!(len2 > 0.0f)) return 0.0f;
+  float c = dot(d, L.axis) / sqrt_(len2);
+  if (c <= L.cos_outer) return 0.0f;
+  if (c >= L.cos_inner) return 1.0f;
+  float t = (c - L.cos_outer) / (L.cos_inner - L.cos_outer);
+  return t * t * (3.0f - 2.0f * t);
+}
+// light.rs:376-389: 1 - cos of the cone the sphere subtends from squared distance dist2 (0 when
+// it is indistinguishable from a point); false from inside it.
+RPX_HD bool sphere_cone(const DevSphere& L, float dist2, float& omc) {
+  float r2 = L.radius * L.radius;
+  if (!(dist2 - r2 > 0.0f)) return false;
+  float sin2 = r2 / dist2;
+  float v = sin2 / (1.0f + sqrt_(fmax_r(1.0f - sin2, 0.0f)));
+  omc = v < 1.0e-7f ? 0.0f : v;
+  return true;
+}
+// light.rs:392-408: the nearer crossing of the sphere in (RAY_EPSILON, t_max) from outside it.
+RPX_HD bool sphere_intersect(const DevSphere& L, V3 o, V3 d, float t_max, float& t_out) {
+  if (!(L.radius > 0.0f)) return false;
+  V3 oc = o - L.centre;
+  float r2 = L.radius * L.radius;
+  if (length_sq(oc) <= r2) return false;
+  float b = dot(oc, d);
+  float disc = r2 - length_sq(oc - d * b);
+  if (disc < 0.0f) return false;
+  float t = -b - sqrt_(disc);
+  if (!(t > RAY_EPSILON && t < t_max)) return false;
+  t_out = t;
+  return true;
+}
+// light.rs:411-420: the selection weight at x.
+RPX_HD float sphere_weight(const DevSphere& L, V3 x) {
+  float m = sphere_mask(L, x);
+  if (m <= 0.0f) return 0.0f;
+  float d2 = length_sq(x - L.centre) * 1.0e-6f;
+  return L.base * m / fmax_r(d2, L.near2);
+}
+
+// A homogeneous scattering medium filling a clear solid (integrator/medium.rs:113-141): per mm
+// coefficients, the Henyey-Greenstein anisotropy, and the solid's index.
+struct DevMedium {
+  C3 sigma_s, sigma_a, sigma_t;
+  float g, ior;
+};
+
 constexpr int MAX_ENTRIES = 64;
 constexpr int KEPT_WEIGHTS = 8;
 
@@ -86,7 +152,11 @@ struct DevScene {
   const DevArea* areas;
   const float* cdf;       // [n_entries]
   const float* flux_pmf;  // [n_entries]
-  const i32* entries;     // area index, -1 = environment
+  const i32* entries;     // area index, -1 = environment, -2 - k = sphere k
+  const DevSphere* spheres;
+  const DevMedium* media;  // per material (read for MEDIUM_SCATTERING materials)
+  u32 n_spheres;
+  u32 ext;  // the scene has sphere lights or scattering media (the kernels' extended variant)
   EnergyTables tables;
   u32 tlas_root, n_instances, n_materials, n_areas, n_entries, per_point, has_env_entry, env_entry;
   // environment ramp
@@ -246,6 +316,8 @@ struct FlatScene {
   std::vector<Bsdf> bsdfs;
   std::vector<DevMaterial> materials;
   std::vector<DevArea> areas;
+  std::vector<DevSphere> spheres;
+  std::vector<DevMedium> media;
   std::vector<float> cdf, flux_pmf;
   std::vector<i32> entries;
   DevScene consts;  // pointers filled at upload
@@ -258,6 +330,9 @@ bool flatten_scene(const HostScene& s, const EnergyTables& tables, FlatScene& ou
 
 // The device record and the two BSDF sides (outside, inside) of a material.
 DevMaterial dev_material(const HostMaterial& m);
+// The scattering medium of a material; false when it has none (a positive, finite scatter_mm in
+// a clear solid: integrator/medium.rs:99-141).
+bool scatter_medium_of(const HostMaterial& m, DevMedium& out);
 void material_sides(const EnergyTables& tables, const HostMaterial& m, Bsdf out[2]);
 
 }  // namespace rpx

@@ -179,7 +179,7 @@ bool parse_rsb(const uint8_t* bytes, size_t len, HostScene& s, std::string& err,
       l.flags = r.u();
       if (!r.ok) { err = "LGHT section too short"; return false; }
       if (l.kind > 5) { err = "unknown light kind"; return false; }
-      if (l.kind > 1) { err = "only rect and disk lights are supported by the GPU tracer"; return false; }
+      if (l.kind == 2 || l.kind == 5) { err = "sun and IES lights are not supported by the GPU tracer"; return false; }
       s.lights.push_back(l);
     } else if (t == "ENVR") {
       if (env) { err = "duplicate ENVR section"; return false; }
@@ -252,10 +252,27 @@ bool parse_rsb(const uint8_t* bytes, size_t len, HostScene& s, std::string& err,
     if (m.anisotropy > 0.0f) { err = "anisotropic materials are not supported by the GPU tracer"; return false; }
     if (m.opacity < 1.0f) { err = "cut-outs are not supported by the GPU tracer"; return false; }
     if (m.flags & MATERIAL_FLAG_THIN) { err = "thin walls are not supported by the GPU tracer"; return false; }
-    if (m.scatter_mm > 0.0f) { err = "scattering media are not supported by the GPU tracer"; return false; }
     if (m.abbe > 0.0f) { err = "dispersion is not supported by the GPU tracer"; return false; }
     if (m.bevel_mm > 0.0f) { err = "edge bevels are not supported by the GPU tracer"; return false; }
     if (m.kind == 2 && (m.flags & MATERIAL_FLAG_LIGHT)) { err = "emissive mesh lights are not supported by the GPU tracer"; return false; }
+  }
+  // A smooth, wholly clear solid without a medium of its own may be a thin wall, which the
+  // reference probes for after a scattering medium's exit (integrator/mod.rs:289-299, 1822-1857);
+  // scenes with both are outside what the GPU tracer restates.
+  bool any_scatter = false, any_wall = false;
+  for (const HostMaterial& m : s.materials) {
+    DevMedium unused;
+    bool scattering = scatter_medium_of(m, unused);
+    any_scatter = any_scatter || scattering;
+    bool metal = m.kind != MATERIAL_GLASS && m.metallic >= 1.0f;
+    bool clear_solid = m.transmission > 0.0f && !metal && (m.flags & MATERIAL_FLAG_THIN) == 0 && m.kind != 2 &&
+                       m.ior >= 1.0001f;
+    any_wall = any_wall || (clear_solid && !scattering && (m.kind == MATERIAL_GLASS || m.metallic <= 0.0f) &&
+                            m.transmission >= 1.0f && m.roughness <= 0.05f && m.coat <= 0.0f && m.opacity >= 1.0f);
+  }
+  if (any_scatter && any_wall) {
+    err = "thin walls beside a scattering medium are not supported by the GPU tracer";
+    return false;
   }
   for (const HostMesh& m : s.meshes)
     if (m.material >= s.materials.size()) { err = "material_id does not refer to a MATL section"; return false; }
@@ -583,21 +600,51 @@ Frame light_frame(V3 normal, V3 up) {
 }
 }  // namespace
 
+// integrator/medium.rs:99-141: a clear solid (transmissive, not a metal, not a thin wall, not an
+// emitter, of index at least 1.0001) with a positive, finite scatter_mm; the absorption of its
+// base colour over absorption_mm joins the walk's.
+bool scatter_medium_of(const HostMaterial& m, DevMedium& out) {
+  bool metal = m.kind != MATERIAL_GLASS && m.metallic >= 1.0f;
+  bool clear_solid = m.transmission > 0.0f && !metal && (m.flags & MATERIAL_FLAG_THIN) == 0 && m.kind != 2 &&
+                     m.ior >= 1.0001f;
+  if (!(m.scatter_mm > 0.0f && is_finite(m.scatter_mm) && clear_solid)) return false;
+  float sigma = 1.0f / m.scatter_mm;
+  C3 albedo = c3(clampf(m.scatter_color.r, 0.0f, 1.0f), clampf(m.scatter_color.g, 0.0f, 1.0f),
+                 clampf(m.scatter_color.b, 0.0f, 1.0f));
+  C3 sigma_a = c3((1.0f - albedo.r) * sigma, (1.0f - albedo.g) * sigma, (1.0f - albedo.b) * sigma);
+  if (m.absorption_mm > 0.0f) {
+    C3 c = m.base_color;
+    sigma_a = sigma_a + c3(-fln(clampf(c.r, 1.0e-4f, 1.0f)) / m.absorption_mm,
+                           -fln(clampf(c.g, 1.0e-4f, 1.0f)) / m.absorption_mm,
+                           -fln(clampf(c.b, 1.0e-4f, 1.0f)) / m.absorption_mm);
+  }
+  out.sigma_s = albedo * sigma;
+  out.sigma_a = sigma_a;
+  out.sigma_t = out.sigma_s + sigma_a;
+  out.g = is_finite(m.scatter_g) ? clampf(m.scatter_g, -0.9f, 0.9f) : 0.0f;
+  out.ior = m.ior;
+  return true;
+}
+
 DevMaterial dev_material(const HostMaterial& m) {
   DevMaterial dm;
   dm.emissive = m.kind == 2 ? 1u : 0u;
   C3 e = m.emission * m.emission_power;
   dm.radiance = dm.emissive ? c3(fmax_r(e.r, 0.0f), fmax_r(e.g, 0.0f), fmax_r(e.b, 0.0f)) : splat(0.0f);
-  // integrator/mod.rs:300-311 (medium_of)
+  // integrator/mod.rs:300-311 (medium_of); a scattering medium folds the absorption into its walk
+  // (:419).
   bool clear = m.transmission > 0.0f && m.metallic < 1.0f && (m.flags & MATERIAL_FLAG_THIN) == 0;
   dm.has_medium = 0;
   dm.sigma = splat(0.0f);
-  if (clear && m.absorption_mm > 0.0f && m.kind != 2) {
+  DevMedium scatter;
+  if (scatter_medium_of(m, scatter)) {
+    dm.has_medium = MEDIUM_SCATTERING;
+  } else if (clear && m.absorption_mm > 0.0f && m.kind != 2) {
     C3 c = m.base_color;
     C3 sg = c3(-fln(clampf(c.r, 1.0e-4f, 1.0f)) / m.absorption_mm, -fln(clampf(c.g, 1.0e-4f, 1.0f)) / m.absorption_mm,
                -fln(clampf(c.b, 1.0e-4f, 1.0f)) / m.absorption_mm);
     if (sg.r > 0.0f || sg.g > 0.0f || sg.b > 0.0f) {
-      dm.has_medium = 1;
+      dm.has_medium = MEDIUM_ABSORBING;
       dm.sigma = sg;
     }
   }
@@ -801,6 +848,9 @@ bool flatten_scene(const HostScene& s, const EnergyTables& tables, FlatScene& ou
     out.bsdfs.push_back(sides[0]);
     out.bsdfs.push_back(sides[1]);
     out.materials.push_back(dev_material(m));
+    DevMedium md{};
+    if (scatter_medium_of(m, md)) out.consts.ext = 1;
+    out.media.push_back(md);
   }
   // Lights (light.rs:656-994).
   V3 centre;
@@ -813,15 +863,58 @@ bool flatten_scene(const HostScene& s, const EnergyTables& tables, FlatScene& ou
     radius = fmax_r(0.5f * length(scene_bounds.hi - scene_bounds.lo), 1.0f);
   }
   float radius_m = radius * MM_TO_M;
-  auto reach = [&](V3 p, V3 n) -> float {
+  auto reach = [&](V3 p, const V3* n) -> float {
     V3 to_centre = centre - p;
     float d = length(to_centre) * MM_TO_M;
-    float facing = 0.05f + 0.95f * fmax_r(dot(n, normalize(to_centre)), 0.0f);
+    float facing = n ? 0.05f + 0.95f * fmax_r(dot(*n, normalize(to_centre)), 0.0f) : 1.0f;
     d = fmax_r(d, radius_m);
     return facing / (d * d);
   };
   std::vector<float> weights;
   for (const HostLight& l : s.lights) {
+    if (l.kind == 3 || l.kind == 4) {
+      // light.rs:793-863 (no IES profile): a point (diameter 0) or a sphere, a spot masked by its
+      // cone with a smoothstep over the soft edge.
+      float power = fmax_r(l.power, 0.0f);
+      C3 color = c3(fmax_r(l.color.r, 0.0f), fmax_r(l.color.g, 0.0f), fmax_r(l.color.b, 0.0f));
+      C3 intensity = color * (power / (4.0f * PI));
+      float rad = 0.5f * fabs_(l.size_x);
+      if (!is_finite(rad)) rad = 0.0f;
+      DevSphere sp;
+      if (l.kind == 4) {
+        V3 axis = normalize(l.direction);
+        if (length_sq(axis) < 0.5f) continue;
+        float half = 0.5f * clampf(l.spread_deg, 1.0f, 180.0f) * deg_to_rad();
+        float soft = fmax_r(l.angle_deg, 0.0f) * deg_to_rad();
+        float inner = fmax_r(half - soft, 0.0f);
+        sp.axis = axis;
+        sp.cos_outer = fcos(half);
+        sp.cos_inner = fcos(inner);
+      } else {
+        sp.axis = v3(0.0f, 0.0f, -1.0f);
+        sp.cos_outer = -2.0f;
+        sp.cos_inner = -2.0f;
+      }
+      float flux = luminance(color) * power;
+      if (!(flux > 0.0f)) continue;
+      float r_m = rad * MM_TO_M;
+      sp.radiance = rad > 0.0f ? intensity * (1.0f / (PI * r_m * r_m)) : splat(0.0f);
+      sp.centre = l.position;
+      sp.radius = rad;
+      sp.intensity = intensity;
+      sp.glossy = (l.flags & 1u) ? 1u : 0u;
+      sp.camera = (l.flags & 2u) ? 1u : 0u;
+      sp.shadow = (l.flags & 4u) ? 0u : 1u;
+      sp.entry = (u32)out.entries.size();
+      sp.base = luminance(intensity);
+      sp.near2 = fmax_r(r_m * r_m, 1.0e-6f);
+      float toward = 0.05f + 0.95f * sphere_mask(sp, centre);
+      weights.push_back(flux * toward * reach(l.position, nullptr));
+      out.entries.push_back(-2 - (i32)out.spheres.size());
+      out.spheres.push_back(sp);
+      out.consts.ext = 1;
+      continue;
+    }
     DevArea a;
     a.rect = l.kind == 0 ? 1u : 0u;
     a.glossy = (l.flags & 1u) ? 1u : 0u;
@@ -845,7 +938,7 @@ bool flatten_scene(const HostScene& s, const EnergyTables& tables, FlatScene& ou
     a.exponent = spread_exponent(l.spread_deg);
     float flux = luminance(color) * power * 2.0f / (a.exponent + 2.0f);
     if (!(flux > 0.0f)) continue;
-    weights.push_back(flux * reach(l.position, a.frame.z));
+    weights.push_back(flux * reach(l.position, &a.frame.z));
     a.entry = (u32)out.entries.size();
     out.entries.push_back((i32)out.areas.size());
     a.centre = l.position;
@@ -896,6 +989,7 @@ bool flatten_scene(const HostScene& s, const EnergyTables& tables, FlatScene& ou
   k.n_entries = (u32)out.entries.size();
   k.per_point = (out.entries.size() >= 2 && out.entries.size() <= (size_t)MAX_ENTRIES) ? 1u : 0u;
   k.n_areas = (u32)out.areas.size();
+  k.n_spheres = (u32)out.spheres.size();
   k.n_instances = (u32)out.instances.size();
   k.n_materials = (u32)out.materials.size();
   // Camera (scene/mod.rs:160-357): basis and the sensor of the longer side.

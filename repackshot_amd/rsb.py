@@ -107,6 +107,21 @@ def with_adaptive(rsb: bytes, noise_threshold: float, width=None, height=None) -
     raise ValueError("the scene bytes have no RNDR section")
 
 
+def with_size(rsb: bytes, width: int, height: int, max_depth=None) -> bytes:
+    """The scene bytes with another image size (and optionally path depth) in the RNDR section."""
+    out = bytearray(rsb)
+    i = 8
+    while i + 8 <= len(out):
+        tag, n = bytes(out[i:i + 4]), struct.unpack("<I", out[i + 4:i + 8])[0]
+        if tag == b"RNDR":
+            out[i + 8:i + 16] = _u(width, height)
+            if max_depth is not None:
+                out[i + 36:i + 40] = _u(max_depth)
+            return bytes(out)
+        i += 8 + n
+    raise ValueError("the scene bytes have no RNDR section")
+
+
 def box_mesh(lo, hi):
     """An axis-aligned box (positions, indices) wound outwards, without normals."""
     c = np.array([[hi[0] if (k & 1) else lo[0], hi[1] if (k & 2) else lo[1], hi[2] if (k & 4) else lo[2]]
