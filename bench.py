@@ -16,11 +16,14 @@ central finite differences of the same forward renderer.
 The denoise record times the guide passes and the reference's full denoising filter on the film of
 the timed render.
 
+The media record renders the caustics-and-media workload (a glass bottle holding a scattering
+medium, lit by a sphere light; 1920 x 1080, 64 spp, depth 16) with the walks in their own kernel.
+
 The adaptive record renders the same scene adaptively (8 x 8 blocks in rounds to a 3 % noise target
 or 64 samples) with the decisions on the GPU between rounds.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-queue] [--no-adjoint]
-                       [--no-adaptive] [--no-denoise]
+                       [--no-adaptive] [--no-denoise] [--no-media]
                        [--pool P --shape S --shadow-shape T]   (fixed launch options instead of the tuner)
 """
 import argparse
@@ -312,6 +315,63 @@ def adaptive_bench(native, rsb, tuned, target=0.03, cap=64):
     return rec
 
 
+def media_bench(native, tuned, seed=0, tiles=((944, 320), (944, 480), (912, 640), (960, 760), (640, 800),
+                                              (1280, 864))):
+    """Caustics and media: the bottle workload (a 200,000-triangle glass bottle holding a scattering
+    medium, one sphere light; 1920 x 1080, 64 spp, depth 16) rendered whole into the resident film
+    with the large render's launch options, the random walks in their own kernel between the
+    closest hits and the shade; and the GPU film against the oracle's on a few 32 x 32 tiles
+    through the bottle, whose render time is the CPU baseline of this workload."""
+    from oracle import oracle as orc
+
+    rsb = native.write_scene(5, seed)
+    scene = native.Scene(rsb)
+    info = scene.info()
+    W, H, spp = info["width"], info["height"], info["max_samples"]
+    scene.set_options(pool=tuned["pool"], shape=tuned["shape"], shadow_shape=tuned["shadow_shape"])
+    scene.render_samples(0, spp)  # warm
+    scene.reset_film()
+    s0 = scene.stats()
+    t0 = time.perf_counter()
+    scene.render_samples(0, spp)
+    wall = time.perf_counter() - t0
+    s1 = scene.stats()
+    scene.set_profiling(True)
+    scene.reset_film()
+    scene.render_samples(0, spp)
+    timing = scene.last_timing()
+    scene.set_profiling(False)
+    rgb, weight, alpha, lum2 = scene.film()
+    oracle = orc.OracleScene(rsb)
+    rects = [(x, y, 32, 32) for x, y in tiles]
+    threads = os.cpu_count() or 1
+    t1 = time.perf_counter()
+    ofilm, _ = oracle.render_tiles(rects, 0, spp, threads)
+    cpu_s = time.perf_counter() - t1
+    exact = all(np.array_equal(g[y:y + h, x:x + w].view(np.uint32), c[y:y + h, x:x + w].view(np.uint32))
+                for (x, y, w, h) in rects for g, c in ((rgb, ofilm[0]), (weight, ofilm[1]), (alpha, ofilm[2]),
+                                                      (lum2, ofilm[3])))
+    paths = W * H * spp
+    rec = {"workload": "caustics and media: 1920x1080, 64 spp, max depth 16, a 200,000-triangle glass bottle "
+                       "(IOR 1.5) holding a medium of sigma_s 0.5 / sigma_a 0.05 per metre and g 0.7, Lambert "
+                       "floor, one sphere light (radius 50 mm, radiance 500), no caustic photons (the reference's "
+                       "sphere lights send none)",
+           "width": W, "height": H, "spp": spp, "triangles": info["triangles"],
+           "mpaths_per_s": round(paths / wall / 1e6, 3), "ms_per_step": round(1000.0 * wall, 2),
+           "mrays_per_s": round((s1["rays"] - s0["rays"]) / wall / 1e6, 3),
+           "rays_per_path": round((s1["rays"] - s0["rays"]) / paths, 4),
+           "stage_ms_per_step": {k: round(timing[k], 2) for k in ("extend_ms", "shade_ms", "shadow_ms", "regen_ms",
+                                                                   "total_ms")},
+           "extend_ms_includes_walks": True,
+           "parity": {"tiles": len(rects), "film_bit_exact_vs_oracle": bool(exact)},
+           "cpu_baseline": {"value": round(len(rects) * 1024 * spp / cpu_s / 1e6, 4), "unit": "Mpaths/s",
+                            "cores": threads, "kind": "port",
+                            "sample": "%d tiles of 32 x 32 at %d spp through the bottle" % (len(rects), spp)}}
+    scene.close()
+    log("bench: media %s" % rec)
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -323,6 +383,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-adaptive", action="store_true")
     ap.add_argument("--no-denoise", action="store_true")
+    ap.add_argument("--no-media", action="store_true")
     ap.add_argument("--pool", type=int, default=0)
     ap.add_argument("--shape", type=int, default=-1)
     ap.add_argument("--shadow-shape", type=int, default=0)
@@ -461,6 +522,8 @@ def main():
         result["denoise"] = denoise_bench(scene, spp)
     if not args.no_adaptive:
         result["adaptive"] = adaptive_bench(native, rsb, tuned)
+    if not args.no_media:
+        result["media"] = media_bench(native, tuned, args.seed)
     if not args.no_queue:
         result["mixed_queue"] = mixed_queue(native, args.seed)
     if not args.no_adjoint:

@@ -107,8 +107,8 @@ class Engine {
 
   int device_ = 0;
   hipStream_t stream_ = nullptr;
-  u32 capacity_ = 0, results_cap_ = 0, segs_cap_ = 0, adjoint_cap_ = 0, bsdf_cap_ = 0;
-  std::vector<DevBuf> pool_bufs_, fixed_bufs_, adjoint_bufs_, bsdf_bufs_;
+  u32 capacity_ = 0, results_cap_ = 0, segs_cap_ = 0, adjoint_cap_ = 0, bsdf_cap_ = 0, walk_cap_ = 0;
+  std::vector<DevBuf> pool_bufs_, fixed_bufs_, adjoint_bufs_, bsdf_bufs_, walk_bufs_;
   DevBuf results_, segs_;
   u32* idle_host_ = nullptr;  // pinned
   std::vector<hipEvent_t> events_;

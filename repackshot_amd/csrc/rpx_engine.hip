@@ -81,6 +81,7 @@ Engine::~Engine() {
   dev_free_all(pool_bufs_);
   dev_free_all(adjoint_bufs_);
   dev_free_all(bsdf_bufs_);
+  dev_free_all(walk_bufs_);
   dev_free_all(fixed_bufs_);
   if (results_.p) hipFree(results_.p);
   if (segs_.p) hipFree(segs_.p);
@@ -120,6 +121,18 @@ bool Engine::ensure_pool(u32 n, std::string& err) {
       return false;
     }
     bsdf_cap_ = n;
+  }
+  if (ext && n > walk_cap_) {
+    hipStreamSynchronize(stream_);
+    dev_free_all(walk_bufs_);
+    walk_cap_ = 0;
+    size_t f4 = (size_t)n * sizeof(float4);
+    if (!dev_alloc(walk_bufs_, f4, (void**)&P.walk_w) || !dev_alloc(walk_bufs_, f4, (void**)&P.walk_r)) {
+      dev_free_all(walk_bufs_);
+      err = "device memory exhausted for the media walks' state";
+      return false;
+    }
+    walk_cap_ = n;
   }
   if (n <= capacity_) return true;
   hipStreamSynchronize(stream_);
@@ -200,10 +213,9 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegm
   L.pool = std::min(opt.pool, total);
   L.shape = opt.shape;
   L.shadow_shape = opt.shadow_shape;
-  // The drain runs the forward vertices of the plain shade only; the replay and scenes with sphere
-  // lights or media keep the loop.
+  // The drain runs the forward vertices only; the replay keeps the loop.
   L.ext = ext ? 1u : 0u;
-  L.tail = (adj || ext) ? 0u : opt.tail;
+  L.tail = adj ? 0u : opt.tail;
   L.stream = stream_;
   e = launch_pool_init(P, L, total);
   if (e != hipSuccess) return e;

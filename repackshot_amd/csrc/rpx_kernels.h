@@ -20,7 +20,9 @@ namespace rpx {
 // bits of med.x), so its walk runs before the next vertex; FLAG_EXIT_PDF: the vertex the ray left
 // was a medium's exit.
 constexpr u32 FLAG_PRIMARY = 1, FLAG_HAS_PDF = 2, FLAG_MEDIUM = 4, FLAG_BY_FLUX = 8, FLAG_SCATTER = 16,
-              FLAG_EXIT_PDF = 32;
+              FLAG_EXIT_PDF = 32, FLAG_WALKING = 64;
+// FLAG_WALKING: the ray is a step of a walk in progress (its state in walk_w / walk_r); the shade
+// passes the path on untouched and the next extend traces the step.
 constexpr u32 EVENTS_SHIFT = 8, EVENTS_MASK = 0xfffu;  // flags bits 8..19: medium steps of the path so far
 constexpr u32 BOUNCE_SHIFT = 20;  // flags bits 20..31: the bounce of the path's next vertex
 // How a walk through a scattering medium ended (hit.w, written by the walk kernel; 0: no walk).
@@ -77,6 +79,10 @@ struct PathState {
                    // [3] thin-wall probes after a medium's exit (counted, never needed: no such walls)
   float4* results; // per work id of the current chunk: radiance rgb, alpha
   Bsdf* bsdf_tmp;  // per slot: the BSDF of a textured hit (allocated for scenes with a texture)
+  // Per slot, a walk in progress (allocated for scenes with media): its throughput weight and the
+  // density of its last phase sample; its spectral-MIS ratio and (events | steps << 16) bits.
+  float4* walk_w;
+  float4* walk_r;
   const u32* sobol;  // 64 x 32 generator columns in logical-dimension order
   // Adjoint (path replay) state per slot: d loss / d radiance of the path, the forward's radiance of
   // the path, the pending d log f / d parameter (5 per slot) of the last BSDF sample, its target
@@ -121,8 +127,9 @@ hipError_t launch_regen(const PathState& P, const Launch& L, u32 parity, bool ad
 hipError_t launch_extend(const PathState& P, const Launch& L, u32 parity);
 hipError_t launch_shade(const PathState& P, const Launch& L, u32 parity, bool adj = false);
 hipError_t launch_shadow(const PathState& P, const Launch& L, bool adj = false);
-// The walks of the live paths inside scattering media, from the closest hit extend found to where
-// each leaves, scatters out or is absorbed.
+// One step of the walks of the live paths inside scattering media, at the closest hit extend found:
+// a scattering event or a reflection back in gives the next step's ray (traced by the next
+// extend); otherwise the walk ends (hit.w says how) and the shade takes the vertex.
 hipError_t launch_walk(const PathState& P, const Launch& L, u32 parity);
 // The drain: when the last bookkeeping handed paths over (C_TAIL), takes each to its end.
 hipError_t launch_finish(const PathState& P, const Launch& L, u32 parity);

@@ -1132,6 +1132,11 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
   bool exit = false;
   float exit_eta2 = 0.0f;
   if constexpr (EXT) {
+    // A walk in progress: its next step's ray goes to the next extend as it is.
+    if (flags & FLAG_WALKING) {
+      cont = true;
+      return;
+    }
     if (h.w == WALK_ABSORBED) return;
     exit = h.w == WALK_EXIT;
     exit_eta2 = rd.w;
@@ -1795,25 +1800,20 @@ __device__ void boundary_of(const DevScene& S, u32 inst, u32 tri, V3 o, V3 d, V3
 }
 }  // namespace
 
-// The random walk of every live path inside a scattering medium (integrator/medium.rs:226-360),
-// one lane per path, from the closest hit extend found for its ray to the walk's end: it crosses
-// without an event (the surface's own BSDF follows), leaves by the exit lobe after scattering,
-// escapes, or is absorbed. Distances are drawn in a hero channel picked per walk, the others
-// weighted by spectral MIS; after MEDIUM_EVENTS steps the path goes straight to the boundary.
-// Each trace after the first is a closest-hit ray of the reference's count.
-template <int BLK, int LDSN>
-__global__ void __launch_bounds__(BLK) k_walk(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
-  __shared__ u32 lds[LDSN * BLK];
-  u32* col = lds + threadIdx.x;
-  const u32* q = P.live[parity];
-  const u32 n = P.counts[C_LIVE0 + parity];
-  const u32 stride = gridDim.x * blockDim.x;
-  u32 traced = 0;
-  for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
-    const u32 i = q[j];
+// One step of the random walk (integrator/medium.rs:226-360) of the path in slot i when it is
+// inside a scattering medium, at the closest hit extend found for its ray: the loop body of the walk,
+// whose trace is the extend of the iteration. A scattering event, or a reflection back in at the
+// boundary, gives the next step's ray and keeps the path walking (the shade passes it on, the
+// next extend traces it, one closest-hit ray of the reference's count); otherwise the walk ends:
+// it crossed without an event (the surface's own BSDF follows), leaves by the exit lobe after
+// scattering, escaped, or was absorbed. Distances are drawn in a hero channel picked per walk, the
+// others weighted by spectral MIS; after MEDIUM_EVENTS steps of the path the walk goes straight
+// to the boundary. One lane per path, no traversal: every step's ray is traced at full width.
+__device__ void walk_step(const DevScene* __restrict__ scenes, const PathState& P, u32 i) {
+  {
     const float4 md = P.med[i];
     const u32 flags = f2u(md.w);
-    if (!(flags & FLAG_SCATTER)) continue;
+    if (!(flags & FLAG_SCATTER)) return;
     const uint4 id = P.ident[i];
     const DevScene& S = scenes[id.y];
     const DevMedium M = S.media[f2u(md.x)];
@@ -1821,45 +1821,47 @@ __global__ void __launch_bounds__(BLK) k_walk(const DevScene* __restrict__ scene
     const u32 index = id.w;
     const u32 bounce = flags >> BOUNCE_SHIFT;
     u32 path_events = (flags >> EVENTS_SHIFT) & EVENTS_MASK;
-    float4 ro = P.ray_o[i], rd = P.ray_d[i];
+    const float4 ro = P.ray_o[i], rd = P.ray_d[i];
     V3 o = v3(ro.x, ro.y, ro.z), d = v3(rd.x, rd.y, rd.z);
-    uint4 h = P.hit[i];
-    bool has_hit = h.x != 0xffffffffu;
-    u32 hit_inst = h.x, hit_tri = h.y;
-    float hit_t = u2f(h.z);
+    const uint4 h = P.hit[i];
+    const bool has_hit = h.x != 0xffffffffu;
     const float4 th = P.thr[i];
     const C3 entry = c3(th.x, th.y, th.z);
+    C3 w = splat(1.0f), ratio = splat(1.0f), thr = entry;
+    u32 events = 0, steps = 0;
+    float pdf = 0.0f;
+    if (flags & FLAG_WALKING) {
+      float4 a = P.walk_w[i], b = P.walk_r[i];
+      w = c3(a.x, a.y, a.z);
+      pdf = a.w;
+      ratio = c3(b.x, b.y, b.z);
+      u32 es = f2u(b.w);
+      events = es & 0xffffu;
+      steps = es >> 16;
+    }
     float uh = sample_at(P.sobol, key, index, medium_dim(bounce, 0) + MEDIUM_OFFSET_HERO) * 3.0f;
     u32 hero = f_to_u32(uh);
     hero = hero < 2u ? hero : 2u;
     const float sigma_hero = channel(M.sigma_t, hero);
-    C3 w = splat(1.0f), ratio = splat(1.0f), thr = entry;
-    u32 events = 0, steps = 0, end = WALK_ABSORBED;
-    float pdf = 0.0f;
-    const Geo g = geo_of(scenes, id.y);
-    for (;;) {
-      if (steps > 0) {
-        ++traced;
-        hit_inst = hit_tri = 0xffffffffu;
-        hit_t = RPX_INF;
-        has_hit = trace_nearest<BLK, LDSN>(g, col, o, d, RAY_EPSILON, RPX_INF, false, hit_inst, hit_tri, hit_t);
-      }
-      float t_hit = has_hit ? hit_t : RPX_INF;
-      bool capped = path_events >= MEDIUM_EVENTS || is_nan(sigma_hero) || sigma_hero <= 0.0f;
-      u32 dims = medium_dim(bounce, steps);
-      steps += 1;
-      float t = RPX_INF;
-      if (!capped) {
-        float u = sample_at(P.sobol, key, index, dims + MEDIUM_OFFSET_DISTANCE);
-        t = -fln(1.0f - u) / sigma_hero;
-      }
-      if (t < t_hit) {
-        // A scattering event.
-        C3 tr = transmittance(M.sigma_t, t);
-        C3 pe = M.sigma_t * tr;
-        C3 fe = M.sigma_s * tr;
-        float ph = channel(pe, hero);
-        if (is_nan(ph) || ph <= 0.0f) break;
+    u32 end = 0;  // 0: the walk goes on with the ray (o, d)
+    const float t_hit = has_hit ? u2f(h.z) : RPX_INF;
+    const bool capped = path_events >= MEDIUM_EVENTS || is_nan(sigma_hero) || sigma_hero <= 0.0f;
+    const u32 dims = medium_dim(bounce, steps);
+    steps += 1;
+    float t = RPX_INF;
+    if (!capped) {
+      float u = sample_at(P.sobol, key, index, dims + MEDIUM_OFFSET_DISTANCE);
+      t = -fln(1.0f - u) / sigma_hero;
+    }
+    if (t < t_hit) {
+      // A scattering event.
+      C3 tr = transmittance(M.sigma_t, t);
+      C3 pe = M.sigma_t * tr;
+      C3 fe = M.sigma_s * tr;
+      float ph = channel(pe, hero);
+      if (is_nan(ph) || ph <= 0.0f) {
+        end = WALK_ABSORBED;
+      } else {
         w = w * (fe / ph);
         ratio = ratio * (pe / ph);
         float mx = peak(ratio);
@@ -1870,22 +1872,25 @@ __global__ void __launch_bounds__(BLK) k_walk(const DevScene* __restrict__ scene
         events += 1;
         path_events += 1;
         C3 current = entry * effective(w, ratio);
-        if (!(peak(current) > 0.0f && is_finite(peak(current)))) break;
-        if (events > 32) {
+        if (!(peak(current) > 0.0f && is_finite(peak(current)))) {
+          end = WALK_ABSORBED;
+        } else if (events > 32) {
           float survive = clampf(peak(current) / peak(entry), 0.05f, 1.0f);
-          if (sample_at(P.sobol, key, index, dims + MEDIUM_OFFSET_ROULETTE) >= survive) break;
-          w = w / survive;
+          if (sample_at(P.sobol, key, index, dims + MEDIUM_OFFSET_ROULETTE) >= survive) end = WALK_ABSORBED;
+          else w = w / survive;
         }
-        V3 x = o + d * t;
-        float u0 = sample_at(P.sobol, key, index, dims + MEDIUM_OFFSET_PHASE);
-        float u1 = sample_at(P.sobol, key, index, dims + MEDIUM_OFFSET_PHASE + 1);
-        float density;
-        V3 dir = hg_sample(d, M.g, u0, u1, density);
-        pdf = density;
-        o = x;
-        d = dir;
-        continue;
+        if (end == 0) {
+          V3 x = o + d * t;
+          float u0 = sample_at(P.sobol, key, index, dims + MEDIUM_OFFSET_PHASE);
+          float u1 = sample_at(P.sobol, key, index, dims + MEDIUM_OFFSET_PHASE + 1);
+          float density;
+          V3 dir = hg_sample(d, M.g, u0, u1, density);
+          pdf = density;
+          o = x;
+          d = dir;
+        }
       }
+    } else {
       // The boundary (or, past the last step, straight to it keeping all but the absorption).
       if (is_finite(t_hit)) {
         if (capped) {
@@ -1893,52 +1898,68 @@ __global__ void __launch_bounds__(BLK) k_walk(const DevScene* __restrict__ scene
         } else {
           C3 pe = transmittance(M.sigma_t, t_hit);
           float ph = channel(pe, hero);
-          if (is_nan(ph) || ph <= 0.0f) break;
-          w = w * (pe / ph);
-          ratio = ratio * (pe / ph);
+          if (is_nan(ph) || ph <= 0.0f) {
+            end = WALK_ABSORBED;
+          } else {
+            w = w * (pe / ph);
+            ratio = ratio * (pe / ph);
+          }
         }
       }
-      thr = entry * effective(w, ratio);
-      if (!(peak(thr) > 0.0f && is_finite(peak(thr)))) break;
-      if (!has_hit) {
-        end = WALK_ESCAPED;
-        break;
+      if (end == 0) {
+        thr = entry * effective(w, ratio);
+        if (!(peak(thr) > 0.0f && is_finite(peak(thr)))) end = WALK_ABSORBED;
+        else if (!has_hit) end = WALK_ESCAPED;
+        else if (events == 0) end = WALK_CROSSED;
       }
-      if (events == 0) {
-        end = WALK_CROSSED;
-        break;
+      if (end == 0) {
+        V3 point, ng;
+        boundary_of(S, h.x, h.y, o, d, point, ng);
+        float cos = clampf(-dot(d, ng), 0.0f, 1.0f);
+        // integrator/medium.rs:218-224: leave with the probability of the Fresnel transmittance out.
+        bool leave = capped || sample_at(P.sobol, key, index, dims + MEDIUM_OFFSET_FRESNEL) >=
+                                   fresnel_dielectric(cos, 1.0f / fmax_r(M.ior, 1.0001f));
+        if (leave) {
+          end = WALK_EXIT;
+        } else {
+          // Reflected back in (a step of the path, for the cap).
+          path_events += 1;
+          V3 dir = normalize(d + ng * (2.0f * cos));
+          o = offset_origin(point, ng);
+          d = dir;
+        }
       }
-      V3 point, ng;
-      boundary_of(S, hit_inst, hit_tri, o, d, point, ng);
-      float cos = clampf(-dot(d, ng), 0.0f, 1.0f);
-      // integrator/medium.rs:218-224: leave with the probability of the Fresnel transmittance out.
-      bool leave = capped || sample_at(P.sobol, key, index, dims + MEDIUM_OFFSET_FRESNEL) >=
-                                 fresnel_dielectric(cos, 1.0f / fmax_r(M.ior, 1.0001f));
-      if (leave) {
-        end = WALK_EXIT;
-        break;
-      }
-      // Reflected back in (a step of the path, for the cap).
-      path_events += 1;
-      V3 dir = normalize(d + ng * (2.0f * cos));
-      o = offset_origin(point, ng);
-      d = dir;
     }
     u32 nflags = (flags & ~(EVENTS_MASK << EVENTS_SHIFT)) | (path_events << EVENTS_SHIFT);
+    if (end == 0) {
+      P.walk_w[i] = make_float4(w.r, w.g, w.b, pdf);
+      P.walk_r[i] = make_float4(ratio.r, ratio.g, ratio.b, u2f(events | (steps << 16)));
+      P.ray_o[i] = make_float4(o.x, o.y, o.z, ro.w);
+      P.ray_d[i] = make_float4(d.x, d.y, d.z, rd.w);
+      P.med[i] = make_float4(md.x, md.y, md.z, u2f(nflags | FLAG_WALKING));
+      return;
+    }
+    // The walk is over: the flags lose FLAG_WALKING whatever the end (an absorbed path must reach
+    // the shade's end of the path, not be passed on as a step).
+    nflags &= ~FLAG_WALKING;
     if (end == WALK_EXIT) nflags = (nflags & ~(FLAG_SCATTER | FLAG_PRIMARY)) | FLAG_HAS_PDF;
+    P.med[i] = make_float4(md.x, md.y, md.z, u2f(nflags));
     if (end != WALK_ABSORBED) {
+      // The walk's last ray and the throughput it leaves with (the exit's eta^2 for its lobe).
       P.ray_o[i] = make_float4(o.x, o.y, o.z, ro.w);
       P.ray_d[i] = make_float4(d.x, d.y, d.z, end == WALK_EXIT ? M.ior * M.ior : 0.0f);
       P.thr[i] = make_float4(thr.r, thr.g, thr.b, end == WALK_EXIT ? pdf : th.w);
-      P.med[i] = make_float4(md.x, md.y, md.z, u2f(nflags));
     }
-    P.hit[i] = has_hit ? make_uint4(hit_inst, hit_tri, f2u(hit_t), end) : make_uint4(0xffffffffu, 0xffffffffu, f2u(RPX_INF), end);
+    P.hit[i] = make_uint4(h.x, h.y, h.z, end);
   }
-  // One atomic per wave.
-  u32 wn = traced;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) wn += __shfl_xor(wn, off);
-  if (__lane_id() == 0 && wn) atomicAdd((unsigned long long*)&P.rays[0], (unsigned long long)wn);
+}
+
+// The walk steps of every live path (a step kernel between extend and shade).
+__global__ void __launch_bounds__(256) k_walk(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
+  const u32* q = P.live[parity];
+  const u32 n = P.counts[C_LIVE0 + parity];
+  const u32 stride = gridDim.x * blockDim.x;
+  for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) walk_step(scenes, P, q[j]);
 }
 
 // ================================================================== drain
@@ -1947,8 +1968,9 @@ __global__ void __launch_bounds__(BLK) k_walk(const DevScene* __restrict__ scene
 // of nodes). Here each lane takes its path to its end in one launch (closest hit, shade, shadow
 // ray, again), so a path waits only for its own rays and those of its wave. The vertices run in
 // the loop's order on the same per-slot state, so every path's radiance is the loop's, bit for
-// bit; the radiance goes to the results as the regen would put it there.
-template <int BLK, int LDSN>
+// bit; the radiance goes to the results as the regen would put it there. EXT: the extended shade,
+// and each closest hit inside a scattering medium is a walk step (a walking path traces again).
+template <int BLK, int LDSN, bool EXT>
 __global__ void __launch_bounds__(BLK) k_finish(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
   const u32 n = P.counts[C_TAIL];
   if (n == 0) return;
@@ -1982,6 +2004,7 @@ __global__ void __launch_bounds__(BLK) k_finish(const DevScene* __restrict__ sce
         P.hit[i] = make_uint4(inst, tri, f2u(t), 0u);
         if (!counted) ++nearest;
         counted = false;
+        if constexpr (EXT) walk_step(scenes, P, i);
       }
       bool cont = false, want_shadow = false;
       VertexGrad g_prev, g_now;
@@ -1992,7 +2015,7 @@ __global__ void __launch_bounds__(BLK) k_finish(const DevScene* __restrict__ sce
       while (pending) {
         u32 cur = lowest_lane_value(pending, sid);
         if (todo && sid == cur) {
-          shade_path<false, false>(scenes[cur], P, i, cont, want_shadow, g_prev, g_now);
+          shade_path<false, EXT>(scenes[cur], P, i, cont, want_shadow, g_prev, g_now);
           todo = false;
         }
         pending = __ballot(todo);
@@ -2483,8 +2506,12 @@ hipError_t launch_regen(const PathState& P, const Launch& L, u32 parity, bool ad
 }
 hipError_t launch_finish(const PathState& P, const Launch& L, u32 parity) {
   u32 lanes = L.pool < L.tail ? L.pool : L.tail;
-  hipLaunchKernelGGL((k_finish<64, 32>), dim3(grid_for(lanes, 64, 1u << 16)), dim3(64), 0, L.stream, L.scenes, P,
-                     parity);
+  if (L.ext)
+    hipLaunchKernelGGL((k_finish<64, 32, true>), dim3(grid_for(lanes, 64, 1u << 16)), dim3(64), 0, L.stream, L.scenes,
+                       P, parity);
+  else
+    hipLaunchKernelGGL((k_finish<64, 32, false>), dim3(grid_for(lanes, 64, 1u << 16)), dim3(64), 0, L.stream, L.scenes,
+                       P, parity);
   return hipGetLastError();
 }
 
@@ -2539,8 +2566,7 @@ hipError_t launch_shade(const PathState& P, const Launch& L, u32 parity, bool ad
   return hipGetLastError();
 }
 hipError_t launch_walk(const PathState& P, const Launch& L, u32 parity) {
-  hipLaunchKernelGGL((k_walk<128, 24>), dim3(grid_for(L.pool, 128, 1u << 20)), dim3(128), 0, L.stream, L.scenes, P,
-                     parity);
+  hipLaunchKernelGGL(k_walk, dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, P, parity);
   return hipGetLastError();
 }
 hipError_t launch_shadow(const PathState& P, const Launch& L, bool adj) {

@@ -22,17 +22,22 @@ def same_bits(a, b):
 
 def check_film(gpu, rsb, spp, split=None):
     """Renders samples [0, spp) on the GPU (in two calls at `split`) and compares film and rays
-    with the oracle's."""
+    with the oracle's: once with the drain (small images: every path walks and shades in the drain
+    kernel) and once with the iteration loop only (the walk kernel between extend and shade)."""
     s = gpu.Scene(rsb)
-    if split:
-        s.render_samples(0, split)
-        s.render_samples(split, spp - split)
-    else:
-        s.render_samples(0, spp)
-    rgb, w, a, l2 = s.film()
     (orgb, ow, oa, ol2), stats = orc.OracleScene(rsb).render_film(0, spp)
-    assert same_bits(rgb, orgb) and same_bits(w, ow) and same_bits(a, oa) and same_bits(l2, ol2)
-    assert s.stats()["rays"] == stats["rays"]
+    for drain in (True, False):
+        s.set_options(drain=drain)
+        s.reset_film()
+        before = s.stats()["rays"]
+        if split:
+            s.render_samples(0, split)
+            s.render_samples(split, spp - split)
+        else:
+            s.render_samples(0, spp)
+        rgb, w, a, l2 = s.film()
+        assert same_bits(rgb, orgb) and same_bits(w, ow) and same_bits(a, oa) and same_bits(l2, ol2), drain
+        assert s.stats()["rays"] - before == stats["rays"], drain
     return s, rgb
 
 
@@ -120,7 +125,8 @@ def test_options_and_queue_keep_media_films(gpu, example_rsb):
     rsb = furnace(3.0, -0.3, 0.3)
     (orgb, ow, oa, ol2), _ = orc.OracleScene(rsb).render_film(0, 6)
     s = gpu.Scene(rsb)
-    for opts in (dict(pool=1024), dict(per_pass=True), dict(graph=True, group=4), dict(shape=5, shadow_shape=6)):
+    for opts in (dict(pool=1024), dict(pool=1024, drain=False), dict(per_pass=True), dict(graph=True, group=4),
+                 dict(shape=5, shadow_shape=6), dict(tail=200)):
         s.set_options(**opts)
         s.reset_film()
         s.render_samples(0, 6)
