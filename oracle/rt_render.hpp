@@ -11,7 +11,14 @@
 //   render.rs:44-497         tile_grid, camera_sample_weighted, render_tile_traced
 //   image.rs:81-169          Film::splat_alpha, resolve
 //   pixel_filter.rs:103-120  offset (box, centre, tent)
+//   integrator/medium.rs:99-360  scattering media: the clear-solid test, coefficients, the
+//                            Henyey-Greenstein phase function and its sampling, the random walk
+//   light.rs:364-420, 793-863, 1224-1268, 1386-1425  sphere, point and spot lights
 // Scenes using anything outside that subset are refused at load with a message.
+// Pinning: the film hashes of the reference's tests pin everything but the media and the sphere
+// lights, which the reference pins by closed forms only (furnace, Chandrasekhar, Beer-Lambert,
+// inverse-square irradiance, the spot's smoothstep); the tests restate those, so these parts are
+// pinned statistically, their bit-level order of random dimensions is not.
 #pragma once
 #include <memory>
 #include <string>
