@@ -2338,6 +2338,59 @@ __global__ void __launch_bounds__(BLK) k_passes(const DevScene* __restrict__ sce
   }
 }
 
+// The sampled passes with one wave per pixel: lane k traces the camera ray of sample first + k (in
+// rounds of 64), and the samples' albedo and coverage are added to the pixel's sums in sample order
+// (render.rs:558-640, passes.rs:209-235, 333-361), the same f32 additions as one lane adding them
+// one by one. A pixel's samples are nearly one ray, so a wave walks the same nodes.
+template <int BLK, int LDSN>
+__global__ void __launch_bounds__(BLK) k_passes_sampled(const DevScene* __restrict__ scene,
+                                                        const u32* __restrict__ cols, PassPlanes Q, u32 x0, u32 y0,
+                                                        u32 w, u32 h, u32 ox, u32 oy, u32 ow, u32 first, u32 count) {
+  __shared__ u32 lds[LDSN * BLK];
+  u32* col = lds + threadIdx.x;
+  const DevScene& S = *scene;
+  const u32 lane = __lane_id();
+  const u32 n = w * h;
+  const u32 waves = gridDim.x * (BLK / 64);
+  for (u32 i = blockIdx.x * (BLK / 64) + threadIdx.x / 64; i < n; i += waves) {
+    u32 px = x0 + i % w, py = y0 + i / w;
+    if (px >= S.width || py >= S.height) continue;
+    size_t o_ix = (size_t)(py - oy) * ow + (px - ox);
+    float sr = Q.albedo[3 * o_ix], sg = Q.albedo[3 * o_ix + 1], sb = Q.albedo[3 * o_ix + 2];
+    float cover = Q.alpha[o_ix], wsum = Q.weight[o_ix];
+    u64 key = pixel_key(S.seed, px, py);
+    for (u32 base = 0; base < count; base += 64) {
+      u32 k = base + lane;
+      float ar = 0.0f, ag = 0.0f, ab = 0.0f, cv = 0.0f;
+      if (k < count) {
+        V3 o, d;
+        camera_ray(S, cols, key, px, py, first + k, o, d);
+        First f = first_hit<BLK, LDSN>(S, col, o, d);
+        C3 a = f.kind == FIRST_BACKGROUND ? splat(0.0f) : (f.kind == FIRST_FLOOR ? S.floor_color : f.albedo);
+        ar = a.r;
+        ag = a.g;
+        ab = a.b;
+        cv = f.kind == FIRST_SURFACE ? 1.0f : 0.0f;
+      }
+      u32 m = count - base < 64u ? count - base : 64u;
+      for (u32 j = 0; j < m; ++j) {
+        sr = sr + u2f(__builtin_amdgcn_readlane(f2u(ar), j));
+        sg = sg + u2f(__builtin_amdgcn_readlane(f2u(ag), j));
+        sb = sb + u2f(__builtin_amdgcn_readlane(f2u(ab), j));
+        cover = cover + u2f(__builtin_amdgcn_readlane(f2u(cv), j));
+        wsum += 1.0f;
+      }
+    }
+    if (lane == 0) {
+      Q.albedo[3 * o_ix] = sr;
+      Q.albedo[3 * o_ix + 1] = sg;
+      Q.albedo[3 * o_ix + 2] = sb;
+      Q.alpha[o_ix] = cover;
+      Q.weight[o_ix] = wsum;
+    }
+  }
+}
+
 // Resolved passes (passes.rs:388-420): the albedo and coverage sums times the f32 reciprocal of
 // their sample count.
 __global__ void __launch_bounds__(256) k_passes_resolve(PassPlanes Q, u32 n, float* albedo_out, float* alpha_out) {
@@ -2615,8 +2668,19 @@ hipError_t launch_trace_rays(const DevScene* scene, const float* rays, u32 n, u3
 }
 hipError_t launch_passes(const DevScene* scene, const u32* sobol, const PassPlanes& Q, u32 x0, u32 y0, u32 w, u32 h,
                          u32 ox, u32 oy, u32 ow, u32 first, u32 count, u32 what, hipStream_t st) {
-  hipLaunchKernelGGL((k_passes<128, 16>), dim3(grid_for(w * h, 128, 1u << 16)), dim3(128), 0, st, scene, sobol, Q, x0,
-                     y0, w, h, ox, oy, ow, first, count, what);
+  // Many samples per pixel: a wave per pixel for them (two pixels per block), the exact planes
+  // by a lane per pixel; a few (the tile entry points' one): a lane per pixel for everything.
+  const bool by_wave = (what & PASSES_SAMPLED) && count >= 16;
+  u32 per_lane = by_wave ? (what & PASSES_EXACT) : what;
+  if (per_lane) {
+    hipLaunchKernelGGL((k_passes<128, 16>), dim3(grid_for(w * h, 128, 1u << 16)), dim3(128), 0, st, scene, sobol, Q,
+                       x0, y0, w, h, ox, oy, ow, first, count, per_lane);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (by_wave)
+    hipLaunchKernelGGL((k_passes_sampled<128, 16>), dim3(grid_for(w * h, 2, 1u << 16)), dim3(128), 0, st, scene, sobol,
+                       Q, x0, y0, w, h, ox, oy, ow, first, count);
   return hipGetLastError();
 }
 hipError_t launch_passes_resolve(const PassPlanes& Q, u32 n, float* albedo, float* alpha, hipStream_t st) {

@@ -79,6 +79,24 @@ def test_feature_scene_passes_are_bit_exact(gpu, variant):
         assert (got["partId"] == gpu.FLOOR_ID).any()
 
 
+@pytest.mark.parametrize("variant,first,count,more", [(dict(), 0, 16, 20), (dict(fstop=4.0), 5, 70, 64),
+                                                       (dict(background_mode=1, floor=1, disk=True), 3, 130, 16)])
+def test_many_sample_passes_are_bit_exact(gpu, variant, first, count, more):
+    # From 16 samples a pixel's samples go one per lane of a wave, added in sample order after the
+    # wave (in rounds of 64: 70 and 130 are ragged); further calls add onto the sums.
+    rsb = feature_scene(**variant)
+    s = gpu.Scene(rsb)
+    o = orc.OracleScene(rsb)
+    s.render_passes(first, count)
+    got, want = s.passes(), o.render_passes(first, count)
+    for k in want:
+        assert same_bits(got[k], want[k]), (variant, k)
+    s.render_passes(first + count, more)
+    got, want = s.passes(), o.render_passes(first, count + more)
+    for k in ("albedo", "alpha"):
+        assert same_bits(got[k], want[k]), (variant, k)
+
+
 def oracle_planes(o, spp):
     film, _ = o.render_film(0, spp)
     rgb, _, alpha, lum2 = film
