@@ -7,6 +7,8 @@
  * engine/crates/rt-wasm/src:
  *   rt_version            ffi.rs:12-15
  *   rt_features           ffi.rs:16-25
+ *   rt_alloc              ffi.rs:27-33   (len zeroed bytes of host memory; never null)
+ *   rt_free               ffi.rs:35-48   (a buffer of rt_alloc with its length; null ignored)
  *   rt_scene_load         ffi.rs:52-75   (RSB v1 bytes, copied; 0 on error)
  *   rt_scene_update       ffi.rs:77-88   (keeps the meshes when the bytes carry no MESH)
  *   rt_scene_error_len    ffi.rs:90-93
@@ -71,6 +73,8 @@ extern "C" {
 
 uint32_t rt_version(void);
 uint32_t rt_features(void);
+uint8_t* rt_alloc(uint32_t len);
+void rt_free(uint8_t* ptr, uint32_t len);
 uint32_t rt_scene_load(const uint8_t* ptr, uint32_t len);
 uint32_t rt_scene_update(uint32_t handle, const uint8_t* ptr, uint32_t len);
 uint32_t rt_scene_error_len(void);

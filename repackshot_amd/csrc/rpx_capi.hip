@@ -2,6 +2,7 @@
 // device scene, path pool, resident film and HIP stream, and of render queues (several scenes'
 // jobs rendered by one pool).
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -276,6 +277,15 @@ extern "C" {
 
 uint32_t rt_version(void) { return 1; }
 uint32_t rt_features(void) { return 0; }
+
+// Host buffers for callers that write scene bytes or read results through the ABI's own memory
+// (the wasm hosts' pattern): zeroed, never null (a zero length gets one byte), freed with their
+// length.
+uint8_t* rt_alloc(uint32_t len) { return static_cast<uint8_t*>(std::calloc(len ? len : 1, 1)); }
+void rt_free(uint8_t* ptr, uint32_t len) {
+  (void)len;
+  std::free(ptr);
+}
 
 uint32_t rpx_device_count(void) {
   int n = 0;

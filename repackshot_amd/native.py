@@ -33,6 +33,8 @@ STATUS = {
 _SIGS = {
     "rt_version": (ctypes.c_uint32, []),
     "rt_features": (ctypes.c_uint32, []),
+    "rt_alloc": (ctypes.c_void_p, [ctypes.c_uint32]),
+    "rt_free": (None, [ctypes.c_void_p, ctypes.c_uint32]),
     "rt_scene_load": (ctypes.c_uint32, [ctypes.c_void_p, ctypes.c_uint32]),
     "rt_scene_update": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32]),
     "rt_scene_error_len": (ctypes.c_uint32, []),

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_functions():
     text = open(os.path.join(ROOT, "include", "rpx.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:uint32_t|uint64_t|void)\s+(r[tp]x?_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:uint32_t|uint64_t|void|uint8_t\s*\*)\s*(r[tp]x?_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol(rpx):
@@ -24,6 +24,20 @@ def test_library_exports_every_declared_symbol(rpx):
     for name in names:
         assert hasattr(lib, name), name
     assert sorted(rpx.exported_symbols()) == names
+
+
+def test_alloc_gives_zeroed_writable_buffers(rpx):
+    # ffi.rs:27-48: zeroed bytes, never null (also for a zero length), released with their length.
+    import ctypes
+    lib = rpx.library()
+    for n in (0, 1, 4096):
+        p = lib.rt_alloc(n)
+        assert p
+        buf = (ctypes.c_uint8 * max(n, 1)).from_address(p)
+        assert n == 0 or bytes(buf) == bytes(n)
+        buf[0] = 7
+        lib.rt_free(p, n)
+    lib.rt_free(None, 8)
 
 
 def test_status_codes_without_a_scene(rpx):
