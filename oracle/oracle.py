@@ -84,6 +84,10 @@ def lib():
         L.orc_denoise.restype = ctypes.c_int
         L.orc_denoise.argtypes = [u32] * 9 + [f32] + [vp] * 9
         L.orc_denoise_synthetic.argtypes = [u32, u32, u32, f32, f32, f32] + [vp] * 7
+        L.orc_bsdf_probe.restype = ctypes.c_int
+        L.orc_bsdf_probe.argtypes = [vp, ctypes.c_int, vp, vp, f32, f32, ctypes.c_int, ctypes.c_int, vp]
+        L.orc_fresnel_dielectric.restype = f32
+        L.orc_fresnel_dielectric.argtypes = [f32, f32]
         L.orc_phase.restype = f32
         L.orc_phase.argtypes = [f32, f32]
         L.orc_sample_phase.restype = f32
@@ -380,3 +384,37 @@ def sample_phase(direction, g, u0, u1):
     out = np.zeros(3, np.float32)
     pdf = lib().orc_sample_phase(_p(d), g, u0, u1, _p(out))
     return out, float(pdf)
+
+
+class Bsdf:
+    """The oracle's BSDF of a material seen from outside (`entering`) or inside, in the local
+    shading frame (z the normal): eval, pdf and sample (bsdf.rs:384-992)."""
+
+    def __init__(self, kind=0, base=(0.8, 0.8, 0.8), metallic=0.0, roughness=0.5, ior=1.5, transmission=0.0,
+                 coat=0.0, coat_roughness=0.0, entering=True):
+        self.mat = np.array([kind, *base, metallic, roughness, ior, transmission, coat, coat_roughness], np.float32)
+        self.entering = 1 if entering else 0
+
+    def eval(self, wo, wi, all_lobes=True):
+        out = np.zeros(3, np.float32)
+        lib().orc_bsdf_probe(_p(self.mat), self.entering, _p(np.asarray(wo, np.float32)),
+                             _p(np.asarray(wi, np.float32)), 0.0, 0.0, 0, 1 if all_lobes else 0, _p(out))
+        return out
+
+    def pdf(self, wo, wi):
+        out = np.zeros(1, np.float32)
+        lib().orc_bsdf_probe(_p(self.mat), self.entering, _p(np.asarray(wo, np.float32)),
+                             _p(np.asarray(wi, np.float32)), 0.0, 0.0, 1, 1, _p(out))
+        return float(out[0])
+
+    def sample(self, wo, u0, u1):
+        """(wi, weight, pdf) or None."""
+        out = np.zeros(7, np.float32)
+        z = np.zeros(3, np.float32)
+        ok = lib().orc_bsdf_probe(_p(self.mat), self.entering, _p(np.asarray(wo, np.float32)), _p(z), u0, u1, 2, 1,
+                                  _p(out))
+        return (out[:3].copy(), out[3:6].copy(), float(out[6])) if ok else None
+
+
+def fresnel_dielectric(cos_i, eta):
+    return float(lib().orc_fresnel_dielectric(cos_i, eta))

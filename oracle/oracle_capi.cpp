@@ -221,6 +221,47 @@ void orc_energy_tables(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_in, 
   std::memcpy(schlick_avg, t.schlick_avg.data(), t.schlick_avg.size() * 4);
 }
 
+// The BSDF of a material seen from one side, for its closed-form tests (bsdf.rs:384-992):
+// `mat` = kind, base r, g, b, metallic, roughness, ior, transmission, coat, coat roughness;
+// op 0 eval (out rgb; `all` 0 for the diffuse lobes only), 1 pdf (out[0]), 2 sample (out: wi xyz,
+// weight rgb, pdf; returns 0 when it gives none).
+int orc_bsdf_probe(const float* mat, int entering, const float* wo, const float* wi, float u0, float u1, int op,
+                   int all, float* out) {
+  Material m;
+  m.kind = (u32)mat[0];
+  m.base_color = Rgb(mat[1], mat[2], mat[3]);
+  m.metallic = mat[4];
+  m.roughness = mat[5];
+  m.ior = mat[6];
+  m.transmission = mat[7];
+  m.coat = mat[8];
+  m.coat_roughness = mat[9];
+  Bsdf b = Bsdf::surface(m, entering != 0);
+  Vec3 o(wo[0], wo[1], wo[2]);
+  if (op == 0) {
+    Rgb f = b.eval(o, Vec3(wi[0], wi[1], wi[2]), all ? Lobes::All : Lobes::Diffuse);
+    out[0] = f.r;
+    out[1] = f.g;
+    out[2] = f.b;
+    return 1;
+  }
+  if (op == 1) {
+    out[0] = b.pdf(o, Vec3(wi[0], wi[1], wi[2]));
+    return 1;
+  }
+  BsdfSample s;
+  if (!b.sample(o, u0, u1, s)) return 0;
+  out[0] = s.wi.x;
+  out[1] = s.wi.y;
+  out[2] = s.wi.z;
+  out[3] = s.weight.r;
+  out[4] = s.weight.g;
+  out[5] = s.weight.b;
+  out[6] = s.pdf;
+  return 1;
+}
+float orc_fresnel_dielectric(float cos_i, float eta) { return fresnel_dielectric(cos_i, eta); }
+
 // Scalar numerics for the golden-vector tests: 0 exp2, 1 log2, 2 powf, 3 sin, 4 cos, 5 atan2,
 // 6 acos.
 float orc_fmath(int which, float x, float y) {
