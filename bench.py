@@ -403,7 +403,8 @@ def main():
     t0 = time.perf_counter()
     scene = native.Scene(rsb)
     info = scene.info()
-    log("bench: scene loaded on the GPU in %.1f s: %s" % (time.perf_counter() - t0, info))
+    load_s = time.perf_counter() - t0
+    log("bench: scene loaded on the GPU in %.1f s: %s" % (load_s, info))
     spp = info["max_samples"]
     W, H = info["width"], info["height"]
     paths_per_step = W * H * spp
@@ -474,6 +475,8 @@ def main():
         "stage_ms_per_step": {k: round(timing[k], 2) for k in ("extend_ms", "shade_ms", "shadow_ms", "regen_ms",
                                                                 "drain_ms", "total_ms")},
         "iterations_per_step": timing["iterations"],
+        # RSB parse, host BVH build (binned SAH, 4-wide, all host threads) and upload, outside the steps.
+        "scene_load_s": round(load_s, 3),
     }
 
     if not args.no_cpu_baseline:
