@@ -403,7 +403,7 @@ static uint32_t render_tile_impl(uint32_t handle, uint32_t x, uint32_t y, uint32
   h->engine->textured = h->dev.tex_material != NO_TEXTURE;
   h->engine->ext = h->dev.ext != 0;
   hipError_t e = h->engine->run(h->d_scene, std::vector<DevSegment>(1, s), false, err);
-  if (e == hipSuccess) e = launch_tile_out(h->engine->P, d_out, cw * ch, channels, h->stream);
+  if (e == hipSuccess) e = launch_tile_out(h->engine->P, d_out, cw, ch, channels, h->stream);
   std::vector<float> tmp((size_t)cw * ch * channels);
   if (e == hipSuccess) e = hipMemcpyAsync(tmp.data(), d_out, tmp.size() * 4, hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);

@@ -137,8 +137,9 @@ hipError_t launch_finish(const PathState& P, const Launch& L, u32 parity);
 hipError_t launch_splat(const PathState& P, const DevSegment& s, hipStream_t st);
 // Splats every segment of a device table in one launch; the segments' pixels must not overlap.
 hipError_t launch_splat_segments(const PathState& P, const DevSegment* segs, u32 n_segs, hipStream_t st);
-// Copies the one-sample results of a w x h tile (work ids 0 ..) out as rgb or rgba floats.
-hipError_t launch_tile_out(const PathState& P, float* out, u32 n, u32 channels, hipStream_t st);
+// Copies the one-sample results of a w x h tile (work ids 0 .., pixels in the regen's tiled order)
+// out as row-major rgb or rgba floats.
+hipError_t launch_tile_out(const PathState& P, float* out, u32 w, u32 h, u32 channels, hipStream_t st);
 hipError_t launch_trace_rays(const DevScene* scene, const float* rays, u32 n, u32* inst, u32* tri, float* t,
                              uint8_t* any, hipStream_t st);
 // Planes of a pass film in device memory (rt-core/src/passes.rs): sums of the sampled passes (albedo

@@ -498,6 +498,28 @@ __device__ __forceinline__ u32 find_segment(const DevSegment* __restrict__ segs,
 }
 }  // namespace
 
+// Pixel p (0 .. w * h) of a w x h rectangle, in 8 x 8 tiles: bands of 8 rows, each in chunks of 8
+// columns (the last band and the last chunk narrower), row-major inside a chunk. Consecutive work
+// ids, and so the lanes of a wave, then cover a tile of the image rather than a strip of a row: its
+// camera rays, and the first bounces from them, walk much the same nodes. A bijection, so every
+// pixel's samples still reach its film in sample order.
+__device__ __forceinline__ void tiled_pixel(u32 w, u32 h, u32 p, u32& x, u32& y) {
+  const u32 band_px = 8u * w;
+  const u32 b = p / band_px;
+  const u32 rows = h - 8u * b < 8u ? h - 8u * b : 8u;
+  const u32 q = p - b * band_px, chunk_px = 8u * rows, full = w / 8u;
+  const u32 c = q / chunk_px;
+  if (c < full) {
+    const u32 j = q - c * chunk_px;
+    x = 8u * c + j % 8u;
+    y = 8u * b + j / 8u;
+  } else {
+    const u32 cw = w - 8u * full, j = q - full * chunk_px;
+    x = 8u * full + j % cw;
+    y = 8u * b + j / cw;
+  }
+}
+
 // Resets the pool: every slot empty and queued as finished, the work cursor at 0.
 __global__ void __launch_bounds__(256) k_pool_init(PathState P, u32 pool, u32 work_end) {
   u32 stride = gridDim.x * blockDim.x;
@@ -550,8 +572,10 @@ __global__ void __launch_bounds__(256) k_regen(const DevScene* __restrict__ scen
       u32 local = wid - sg.base, npix = sg.w * sg.h;
       u32 k = local / npix, p = local - k * npix;
       sid = sg.scene;
-      px = sg.x0 + p % sg.w;
-      py = sg.y0 + p / sg.w;
+      u32 lx, ly;
+      tiled_pixel(sg.w, sg.h, p, lx, ly);
+      px = sg.x0 + lx;
+      py = sg.y0 + ly;
       index = sg.first + k;
       if constexpr (ADJ) {
         const float* a = sg.adjoint + 3 * ((size_t)py * sg.film_w + px);
@@ -2047,7 +2071,9 @@ __global__ void __launch_bounds__(BLK) k_finish(const DevScene* __restrict__ sce
 // additions in the same order as adding sample by sample.
 __device__ __forceinline__ void splat_pixel(const PathState& P, const DevSegment& s, u32 p) {
   u32 n = s.w * s.h;
-  u32 px = s.x0 + p % s.w, py = s.y0 + p / s.w;
+  u32 lx, ly;
+  tiled_pixel(s.w, s.h, p, lx, ly);
+  u32 px = s.x0 + lx, py = s.y0 + ly;
   size_t f = (size_t)py * s.film_w + px;
   float r = s.film.rgb[3 * f], g = s.film.rgb[3 * f + 1], b = s.film.rgb[3 * f + 2];
   float wt = s.film.weight[f], al = s.film.alpha[f], l2 = s.film.lum2[f];
@@ -2087,16 +2113,21 @@ __global__ void __launch_bounds__(64) k_splat_segments(PathState P, const DevSeg
   }
 }
 
-__global__ void __launch_bounds__(256) k_tile_out(PathState P, float* out, u32 n, u32 channels) {
+// The one-sample results of a w x h tile (work ids in tiled pixel order) as row-major rgb or rgba.
+__global__ void __launch_bounds__(256) k_tile_out(PathState P, float* out, u32 w, u32 h, u32 channels) {
+  u32 n = w * h;
   u32 stride = gridDim.x * blockDim.x;
   for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     float4 s = P.results[i];
+    u32 x, y;
+    tiled_pixel(w, h, i, x, y);
+    size_t o = (size_t)y * w + x;
     // A one-sample film resolves to the sample times 1/1.
     float inv = 1.0f / 1.0f;
-    out[channels * (size_t)i] = s.x * inv;
-    out[channels * (size_t)i + 1] = s.y * inv;
-    out[channels * (size_t)i + 2] = s.z * inv;
-    if (channels == 4) out[channels * (size_t)i + 3] = s.w * inv;
+    out[channels * o] = s.x * inv;
+    out[channels * o + 1] = s.y * inv;
+    out[channels * o + 2] = s.z * inv;
+    if (channels == 4) out[channels * o + 3] = s.w * inv;
   }
 }
 
@@ -2648,8 +2679,8 @@ hipError_t launch_splat_segments(const PathState& P, const DevSegment* segs, u32
   hipLaunchKernelGGL(k_splat_segments, dim3(std::min<u32>(n_segs, 1u << 16)), dim3(64), 0, st, P, segs, n_segs);
   return hipGetLastError();
 }
-hipError_t launch_tile_out(const PathState& P, float* out, u32 n, u32 channels, hipStream_t st) {
-  hipLaunchKernelGGL(k_tile_out, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, P, out, n, channels);
+hipError_t launch_tile_out(const PathState& P, float* out, u32 w, u32 h, u32 channels, hipStream_t st) {
+  hipLaunchKernelGGL(k_tile_out, dim3(grid_for(w * h, 256, 8192)), dim3(256), 0, st, P, out, w, h, channels);
   return hipGetLastError();
 }
 hipError_t launch_trace_rays(const DevScene* scene, const float* rays, u32 n, u32* inst, u32* tri, float* t,
