@@ -2,7 +2,8 @@
 //
 // The tracer keeps a pool of path slots resident on the device. Work is a list of segments
 // (samples [first, first + count) of a pixel rectangle of one scene); every (segment, sample,
-// pixel) is one work id, numbered segment by segment, then sample-major. Each iteration runs
+// pixel) is one work id, numbered segment by segment, then pixel by pixel (8 x 8 tiles), the
+// samples of a pixel consecutive. Each iteration runs
 //   regen   finished slots write their radiance to `results[work id]` and take the next work ids
 //           from a device cursor (camera ray of the new path), appending to the live queue
 //   extend  closest hit of every live path
@@ -37,7 +38,8 @@ struct Film {
 };
 
 // Samples [first, first + count) of the pixels (x0 .. x0 + w, y0 .. y0 + h) of scene `scene`;
-// work ids base .. base + count * w * h.
+// work ids base .. base + count * w * h, sample-minor (base + p * count + k: sample first + k of
+// the p-th pixel, pixels in 8 x 8 tiles).
 struct DevSegment {
   u32 scene;
   u32 x0, y0, w, h;

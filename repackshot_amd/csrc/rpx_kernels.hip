@@ -569,8 +569,10 @@ __global__ void __launch_bounds__(256) k_regen(const DevScene* __restrict__ scen
     u32 sid = 0, px = 0, py = 0, index = 0;
     if (got) {
       const DevSegment& sg = segs[find_segment(segs, n_segs, wid)];
-      u32 local = wid - sg.base, npix = sg.w * sg.h;
-      u32 k = local / npix, p = local - k * npix;
+      // Sample-minor: consecutive work ids are the samples of one pixel (a wave's paths start as
+      // nearly one ray), pixels in tiled order.
+      u32 local = wid - sg.base;
+      u32 p = local / sg.count, k = local - p * sg.count;
       sid = sg.scene;
       u32 lx, ly;
       tiled_pixel(sg.w, sg.h, p, lx, ly);
@@ -2068,18 +2070,18 @@ __global__ void __launch_bounds__(BLK) k_finish(const DevScene* __restrict__ sce
 // ================================================================== splat / tile output
 // Adds samples first .. first + count of pixel p of segment s to its film, in sample order
 // (image.rs:81-94): the sums are carried in registers and stored once, which performs the same f32
-// additions in the same order as adding sample by sample.
+// additions in the same order as adding sample by sample. Work id base + p * count + k holds sample
+// first + k of the segment's p-th pixel in tiled order.
 __device__ __forceinline__ void splat_pixel(const PathState& P, const DevSegment& s, u32 p) {
-  u32 n = s.w * s.h;
   u32 lx, ly;
   tiled_pixel(s.w, s.h, p, lx, ly);
   u32 px = s.x0 + lx, py = s.y0 + ly;
   size_t f = (size_t)py * s.film_w + px;
   float r = s.film.rgb[3 * f], g = s.film.rgb[3 * f + 1], b = s.film.rgb[3 * f + 2];
   float wt = s.film.weight[f], al = s.film.alpha[f], l2 = s.film.lum2[f];
-  const float4* res = P.results + s.base + p;
+  const float4* res = P.results + s.base + (size_t)p * s.count;
   for (u32 k = 0; k < s.count; ++k) {
-    float4 v = res[(size_t)k * n];
+    float4 v = res[k];
     C3 c = c3(v.x, v.y, v.z) * splat(1.0f);
     r += c.r;
     g += c.g;
