@@ -2105,6 +2105,46 @@ __global__ void __launch_bounds__(256) k_splat(PathState P, DevSegment s) {
   for (u32 p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) splat_pixel(P, s, p);
 }
 
+// The same with one wave per pixel, for many samples: lane k loads sample k (rounds of 64, one
+// contiguous read per wave) and the wave adds them to the film in sample order (a readlane chain:
+// the same f32 additions as one lane adding them one by one).
+__global__ void __launch_bounds__(256) k_splat_wide(PathState P, DevSegment s) {
+  const u32 lane = __lane_id();
+  const u32 n = s.w * s.h;
+  const u32 waves = gridDim.x * 4;
+  for (u32 p = blockIdx.x * 4 + threadIdx.x / 64; p < n; p += waves) {
+    u32 lx, ly;
+    tiled_pixel(s.w, s.h, p, lx, ly);
+    size_t f = (size_t)(s.y0 + ly) * s.film_w + (s.x0 + lx);
+    float r = s.film.rgb[3 * f], g = s.film.rgb[3 * f + 1], b = s.film.rgb[3 * f + 2];
+    float wt = s.film.weight[f], al = s.film.alpha[f], l2 = s.film.lum2[f];
+    const float4* res = P.results + s.base + (size_t)p * s.count;
+    for (u32 base = 0; base < s.count; base += 64) {
+      float4 v = base + lane < s.count ? res[base + lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      C3 c = c3(v.x, v.y, v.z) * splat(1.0f);
+      float l = luminance(c);
+      float ll = l * l;
+      u32 m = s.count - base < 64u ? s.count - base : 64u;
+      for (u32 j = 0; j < m; ++j) {
+        r = r + u2f(__builtin_amdgcn_readlane(f2u(c.r), j));
+        g = g + u2f(__builtin_amdgcn_readlane(f2u(c.g), j));
+        b = b + u2f(__builtin_amdgcn_readlane(f2u(c.b), j));
+        wt += 1.0f;
+        al = al + u2f(__builtin_amdgcn_readlane(f2u(v.w), j));
+        l2 = l2 + u2f(__builtin_amdgcn_readlane(f2u(ll), j));
+      }
+    }
+    if (lane == 0) {
+      s.film.rgb[3 * f] = r;
+      s.film.rgb[3 * f + 1] = g;
+      s.film.rgb[3 * f + 2] = b;
+      s.film.weight[f] = wt;
+      s.film.alpha[f] = al;
+      s.film.lum2[f] = l2;
+    }
+  }
+}
+
 // Many small segments whose pixels are disjoint (the blocks of an adaptive round): one wave per
 // segment, in one launch.
 __global__ void __launch_bounds__(64) k_splat_segments(PathState P, const DevSegment* __restrict__ segs, u32 n_segs) {
@@ -2674,7 +2714,10 @@ hipError_t launch_shadow(const PathState& P, const Launch& L, bool adj) {
   return hipGetLastError();
 }
 hipError_t launch_splat(const PathState& P, const DevSegment& s, hipStream_t st) {
-  hipLaunchKernelGGL(k_splat, dim3(grid_for(s.w * s.h, 256, 8192)), dim3(256), 0, st, P, s);
+  if (s.count >= 16)
+    hipLaunchKernelGGL(k_splat_wide, dim3(grid_for(s.w * s.h, 4, 1u << 16)), dim3(256), 0, st, P, s);
+  else
+    hipLaunchKernelGGL(k_splat, dim3(grid_for(s.w * s.h, 256, 8192)), dim3(256), 0, st, P, s);
   return hipGetLastError();
 }
 hipError_t launch_splat_segments(const PathState& P, const DevSegment* segs, u32 n_segs, hipStream_t st) {

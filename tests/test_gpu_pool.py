@@ -119,3 +119,23 @@ def test_queue_graph_and_per_pass_agree(gpu):
         q.close()
     for f in films[1:]:
         assert films_equal(f[0], films[0][0]) and films_equal(f[1], films[0][1])
+
+
+@pytest.mark.parametrize("variant,splits", [(dict(), (16, 54)), (dict(fstop=4.0), (70,)),
+                                            (dict(background_mode=1, floor=1, disk=True), (5, 20, 45))])
+def test_many_samples_per_call_are_bit_exact(gpu, variant, splits):
+    # From 16 samples per call the film takes each pixel's samples with one wave (a contiguous read
+    # of its results, added in sample order), in rounds of 64 (70 is ragged); calls add onto the
+    # film. Every work order, chunking and option gives the oracle's sums.
+    rsb = feature_scene(**variant)
+    total = sum(splits)
+    want, _ = orc.OracleScene(rsb).render_film(0, total)
+    s = gpu.Scene(rsb)
+    for opts in (dict(), dict(pool=700), dict(chunk=3000), dict(per_pass=True)):
+        s.set_options(**opts)
+        s.reset_film()
+        first = 0
+        for n in splits:
+            s.render_samples(first, n)
+            first += n
+        assert films_equal(s.film(), want), (variant, splits, opts)
