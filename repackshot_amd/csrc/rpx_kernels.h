@@ -39,7 +39,7 @@ struct Film {
 
 // Samples [first, first + count) of the pixels (x0 .. x0 + w, y0 .. y0 + h) of scene `scene`;
 // work ids base .. base + count * w * h, sample-minor (base + p * count + k: sample first + k of
-// the p-th pixel, pixels in 8 x 8 tiles).
+// the p-th pixel, pixels in 8 x 8 tiles); its results slots sample-major (base + k * w * h + p).
 struct DevSegment {
   u32 scene;
   u32 x0, y0, w, h;
@@ -72,7 +72,7 @@ struct PathState {
   float4* sh_d;    // shadow ray direction
   float4* sh_c;    // contribution the shadow ray adds when unblocked
   u64* key;        // pixel key of the sampler
-  uint4* ident;    // work id (NO_WORK: empty slot), scene, x | y << 16, sample index
+  uint4* ident;    // results slot of the path's work id (NO_WORK: empty slot), scene, x | y << 16, sample index
   u32* live[2];    // live queues (ping-pong by iteration)
   u32* shadow_queue;
   u32* done_queue;

@@ -566,13 +566,15 @@ __global__ void __launch_bounds__(256) k_regen(const DevScene* __restrict__ scen
     }
     u32 wid = cursor + j;
     bool got = active && j < fresh;
-    u32 sid = 0, px = 0, py = 0, index = 0;
+    u32 sid = 0, px = 0, py = 0, index = 0, rslot = 0;
     if (got) {
       const DevSegment& sg = segs[find_segment(segs, n_segs, wid)];
       // Sample-minor: consecutive work ids are the samples of one pixel (a wave's paths start as
-      // nearly one ray), pixels in tiled order.
-      u32 local = wid - sg.base;
+      // nearly one ray), pixels in tiled order. The result is stored sample-major (results slot
+      // base + k * npix + p), so that the splat's lanes, one per pixel, read it contiguously.
+      u32 local = wid - sg.base, npix = sg.w * sg.h;
       u32 p = local / sg.count, k = local - p * sg.count;
+      rslot = sg.base + k * npix + p;
       sid = sg.scene;
       u32 lx, ly;
       tiled_pixel(sg.w, sg.h, p, lx, ly);
@@ -583,7 +585,7 @@ __global__ void __launch_bounds__(256) k_regen(const DevScene* __restrict__ scen
         const float* a = sg.adjoint + 3 * ((size_t)py * sg.film_w + px);
         float sc = sg.adjoint_scale;
         P.adj[slot] = make_float4(a[0] * sc, a[1] * sc, a[2] * sc, 0.0f);
-        P.ltot[slot] = P.results[wid];
+        P.ltot[slot] = P.results[rslot];
         P.tgt[slot] = make_uint4(NO_TARGET, NO_TARGET, 0u, 0u);
       }
     } else if (active) {
@@ -605,7 +607,7 @@ __global__ void __launch_bounds__(256) k_regen(const DevScene* __restrict__ scen
         P.sum[slot] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
         P.med[slot] = make_float4(0.0f, 0.0f, 0.0f, u2f(FLAG_PRIMARY));
         P.key[slot] = key;
-        P.ident[slot] = make_uint4(wid, sid, px | (py << 16), index);
+        P.ident[slot] = make_uint4(rslot, sid, px | (py << 16), index);
         todo = false;
       }
       pending = __ballot(todo);
@@ -2070,18 +2072,19 @@ __global__ void __launch_bounds__(BLK) k_finish(const DevScene* __restrict__ sce
 // ================================================================== splat / tile output
 // Adds samples first .. first + count of pixel p of segment s to its film, in sample order
 // (image.rs:81-94): the sums are carried in registers and stored once, which performs the same f32
-// additions in the same order as adding sample by sample. Work id base + p * count + k holds sample
-// first + k of the segment's p-th pixel in tiled order.
+// additions in the same order as adding sample by sample. Results slot base + k * (w * h) + p holds
+// sample first + k of the segment's p-th pixel in tiled order.
 __device__ __forceinline__ void splat_pixel(const PathState& P, const DevSegment& s, u32 p) {
+  u32 n = s.w * s.h;
   u32 lx, ly;
   tiled_pixel(s.w, s.h, p, lx, ly);
   u32 px = s.x0 + lx, py = s.y0 + ly;
   size_t f = (size_t)py * s.film_w + px;
   float r = s.film.rgb[3 * f], g = s.film.rgb[3 * f + 1], b = s.film.rgb[3 * f + 2];
   float wt = s.film.weight[f], al = s.film.alpha[f], l2 = s.film.lum2[f];
-  const float4* res = P.results + s.base + (size_t)p * s.count;
+  const float4* res = P.results + s.base + p;
   for (u32 k = 0; k < s.count; ++k) {
-    float4 v = res[k];
+    float4 v = res[(size_t)k * n];
     C3 c = c3(v.x, v.y, v.z) * splat(1.0f);
     r += c.r;
     g += c.g;
@@ -2103,46 +2106,6 @@ __global__ void __launch_bounds__(256) k_splat(PathState P, DevSegment s) {
   u32 n = s.w * s.h;
   u32 stride = gridDim.x * blockDim.x;
   for (u32 p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) splat_pixel(P, s, p);
-}
-
-// The same with one wave per pixel, for many samples: lane k loads sample k (rounds of 64, one
-// contiguous read per wave) and the wave adds them to the film in sample order (a readlane chain:
-// the same f32 additions as one lane adding them one by one).
-__global__ void __launch_bounds__(256) k_splat_wide(PathState P, DevSegment s) {
-  const u32 lane = __lane_id();
-  const u32 n = s.w * s.h;
-  const u32 waves = gridDim.x * 4;
-  for (u32 p = blockIdx.x * 4 + threadIdx.x / 64; p < n; p += waves) {
-    u32 lx, ly;
-    tiled_pixel(s.w, s.h, p, lx, ly);
-    size_t f = (size_t)(s.y0 + ly) * s.film_w + (s.x0 + lx);
-    float r = s.film.rgb[3 * f], g = s.film.rgb[3 * f + 1], b = s.film.rgb[3 * f + 2];
-    float wt = s.film.weight[f], al = s.film.alpha[f], l2 = s.film.lum2[f];
-    const float4* res = P.results + s.base + (size_t)p * s.count;
-    for (u32 base = 0; base < s.count; base += 64) {
-      float4 v = base + lane < s.count ? res[base + lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      C3 c = c3(v.x, v.y, v.z) * splat(1.0f);
-      float l = luminance(c);
-      float ll = l * l;
-      u32 m = s.count - base < 64u ? s.count - base : 64u;
-      for (u32 j = 0; j < m; ++j) {
-        r = r + u2f(__builtin_amdgcn_readlane(f2u(c.r), j));
-        g = g + u2f(__builtin_amdgcn_readlane(f2u(c.g), j));
-        b = b + u2f(__builtin_amdgcn_readlane(f2u(c.b), j));
-        wt += 1.0f;
-        al = al + u2f(__builtin_amdgcn_readlane(f2u(v.w), j));
-        l2 = l2 + u2f(__builtin_amdgcn_readlane(f2u(ll), j));
-      }
-    }
-    if (lane == 0) {
-      s.film.rgb[3 * f] = r;
-      s.film.rgb[3 * f + 1] = g;
-      s.film.rgb[3 * f + 2] = b;
-      s.film.weight[f] = wt;
-      s.film.alpha[f] = al;
-      s.film.lum2[f] = l2;
-    }
-  }
 }
 
 // Many small segments whose pixels are disjoint (the blocks of an adaptive round): one wave per
@@ -2714,10 +2677,7 @@ hipError_t launch_shadow(const PathState& P, const Launch& L, bool adj) {
   return hipGetLastError();
 }
 hipError_t launch_splat(const PathState& P, const DevSegment& s, hipStream_t st) {
-  if (s.count >= 16)
-    hipLaunchKernelGGL(k_splat_wide, dim3(grid_for(s.w * s.h, 4, 1u << 16)), dim3(256), 0, st, P, s);
-  else
-    hipLaunchKernelGGL(k_splat, dim3(grid_for(s.w * s.h, 256, 8192)), dim3(256), 0, st, P, s);
+  hipLaunchKernelGGL(k_splat, dim3(grid_for(s.w * s.h, 256, 8192)), dim3(256), 0, st, P, s);
   return hipGetLastError();
 }
 hipError_t launch_splat_segments(const PathState& P, const DevSegment* segs, u32 n_segs, hipStream_t st) {
