@@ -45,10 +45,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X datasheet: 8 TB/s HBM3E
 NODE_BYTES = 256
 PACKET_BYTES = 160
 # Memory-side bytes of k_extend per closest-hit ray, from rocprofv3 FETCH_SIZE and WRITE_SIZE runs
-# of this workload (pool 33,554,432, closest-hit shape 3, shadow shape 4; FETCH_SIZE scaled by 2.0,
-# WRITE_SIZE by 1.0, the factors a 256 MiB device copy gives): 2.832e9 bytes per dispatch over
-# 5,695,387 rays.
-TRAFFIC_BYTES_PER_RAY = 497.3
+# of this workload (pool 33,554,432, closest-hit shape 2, shadow shape 4, sample-minor work order;
+# FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0, the factors a 256 MiB device copy gives): 1.403e9
+# bytes per dispatch over 5,695,387 rays.
+TRAFFIC_BYTES_PER_RAY = 246.4
 
 
 def log(*a):
