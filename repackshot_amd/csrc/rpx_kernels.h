@@ -39,7 +39,7 @@ struct Film {
 
 // Samples [first, first + count) of the pixels (x0 .. x0 + w, y0 .. y0 + h) of scene `scene`;
 // work ids base .. base + count * w * h, sample-minor (base + p * count + k: sample first + k of
-// the p-th pixel, pixels in 8 x 8 tiles); its results slots sample-major (base + k * w * h + p).
+// the p-th pixel, pixels in 8 x 8 tiles), each path's result stored at its work id.
 struct DevSegment {
   u32 scene;
   u32 x0, y0, w, h;

@@ -570,11 +570,11 @@ __global__ void __launch_bounds__(256) k_regen(const DevScene* __restrict__ scen
     if (got) {
       const DevSegment& sg = segs[find_segment(segs, n_segs, wid)];
       // Sample-minor: consecutive work ids are the samples of one pixel (a wave's paths start as
-      // nearly one ray), pixels in tiled order. The result is stored sample-major (results slot
-      // base + k * npix + p), so that the splat's lanes, one per pixel, read it contiguously.
-      u32 local = wid - sg.base, npix = sg.w * sg.h;
+      // nearly one ray), pixels in tiled order. The result is stored at the work id, so a pixel's
+      // results are contiguous.
+      u32 local = wid - sg.base;
       u32 p = local / sg.count, k = local - p * sg.count;
-      rslot = sg.base + k * npix + p;
+      rslot = wid;
       sid = sg.scene;
       u32 lx, ly;
       tiled_pixel(sg.w, sg.h, p, lx, ly);
@@ -2074,47 +2074,75 @@ __global__ void __launch_bounds__(BLK) k_finish(const DevScene* __restrict__ sce
 // (image.rs:81-94): the sums are carried in registers and stored once, which performs the same f32
 // additions in the same order as adding sample by sample. Results slot base + k * (w * h) + p holds
 // sample first + k of the segment's p-th pixel in tiled order.
-__device__ __forceinline__ void splat_pixel(const PathState& P, const DevSegment& s, u32 p) {
-  u32 n = s.w * s.h;
-  u32 lx, ly;
-  tiled_pixel(s.w, s.h, p, lx, ly);
-  u32 px = s.x0 + lx, py = s.y0 + ly;
-  size_t f = (size_t)py * s.film_w + px;
-  float r = s.film.rgb[3 * f], g = s.film.rgb[3 * f + 1], b = s.film.rgb[3 * f + 2];
-  float wt = s.film.weight[f], al = s.film.alpha[f], l2 = s.film.lum2[f];
-  const float4* res = P.results + s.base + p;
-  for (u32 k = 0; k < s.count; ++k) {
-    float4 v = res[(size_t)k * n];
-    C3 c = c3(v.x, v.y, v.z) * splat(1.0f);
-    r += c.r;
-    g += c.g;
-    b += c.b;
-    wt += 1.0f;
-    al += v.w;
-    float l = luminance(c);
-    l2 += l * l;
+// Adds the samples of pixels p0 .. p0 + T - 1 of segment `s` to the film, each pixel's in sample
+// order (lane t: pixel p0 + t). A pixel's results are contiguous (work ids sample-minor); the block
+// stages SPLAT_CH samples of each of its pixels at a time in LDS with coalesced loads, then each
+// lane adds its own pixel's. Every lane of the block must call this with the same p0.
+constexpr u32 SPLAT_CH = 8;
+template <u32 T>
+__device__ __forceinline__ void splat_block(const PathState& P, const DevSegment& s, u32 p0,
+                                            float4 (*tile)[SPLAT_CH + 1]) {
+  const u32 n = s.w * s.h, count = s.count, t = threadIdx.x, p = p0 + t;
+  const bool mine = p < n;
+  size_t f = 0;
+  float r = 0.0f, g = 0.0f, b = 0.0f, wt = 0.0f, al = 0.0f, l2 = 0.0f;
+  if (mine) {
+    u32 lx, ly;
+    tiled_pixel(s.w, s.h, p, lx, ly);
+    f = (size_t)(s.y0 + ly) * s.film_w + (s.x0 + lx);
+    r = s.film.rgb[3 * f];
+    g = s.film.rgb[3 * f + 1];
+    b = s.film.rgb[3 * f + 2];
+    wt = s.film.weight[f];
+    al = s.film.alpha[f];
+    l2 = s.film.lum2[f];
   }
-  s.film.rgb[3 * f] = r;
-  s.film.rgb[3 * f + 1] = g;
-  s.film.rgb[3 * f + 2] = b;
-  s.film.weight[f] = wt;
-  s.film.alpha[f] = al;
-  s.film.lum2[f] = l2;
+  const float4* res = P.results + s.base;
+  for (u32 c0 = 0; c0 < count; c0 += SPLAT_CH) {
+    const u32 m = count - c0 < SPLAT_CH ? count - c0 : SPLAT_CH;
+    __syncthreads();
+    for (u32 e = t; e < T * SPLAT_CH; e += T) {
+      u32 q = e / SPLAT_CH, k = e - q * SPLAT_CH;
+      if (k < m && p0 + q < n) tile[q][k] = res[(size_t)(p0 + q) * count + c0 + k];
+    }
+    __syncthreads();
+    if (mine)
+      for (u32 k = 0; k < m; ++k) {
+        float4 v = tile[t][k];
+        C3 c = c3(v.x, v.y, v.z) * splat(1.0f);
+        r += c.r;
+        g += c.g;
+        b += c.b;
+        wt += 1.0f;
+        al += v.w;
+        float l = luminance(c);
+        l2 += l * l;
+      }
+  }
+  if (mine) {
+    s.film.rgb[3 * f] = r;
+    s.film.rgb[3 * f + 1] = g;
+    s.film.rgb[3 * f + 2] = b;
+    s.film.weight[f] = wt;
+    s.film.alpha[f] = al;
+    s.film.lum2[f] = l2;
+  }
 }
 
 __global__ void __launch_bounds__(256) k_splat(PathState P, DevSegment s) {
-  u32 n = s.w * s.h;
-  u32 stride = gridDim.x * blockDim.x;
-  for (u32 p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) splat_pixel(P, s, p);
+  __shared__ float4 tile[256][SPLAT_CH + 1];
+  const u32 n = s.w * s.h;
+  for (u32 p0 = blockIdx.x * 256; p0 < n; p0 += gridDim.x * 256) splat_block<256>(P, s, p0, tile);
 }
 
 // Many small segments whose pixels are disjoint (the blocks of an adaptive round): one wave per
 // segment, in one launch.
 __global__ void __launch_bounds__(64) k_splat_segments(PathState P, const DevSegment* __restrict__ segs, u32 n_segs) {
+  __shared__ float4 tile[64][SPLAT_CH + 1];
   for (u32 i = blockIdx.x; i < n_segs; i += gridDim.x) {
     DevSegment s = segs[i];
     u32 n = s.w * s.h;
-    for (u32 p = threadIdx.x; p < n; p += 64) splat_pixel(P, s, p);
+    for (u32 p0 = 0; p0 < n; p0 += 64) splat_block<64>(P, s, p0, tile);
   }
 }
 
