@@ -4,8 +4,8 @@ Lambert, three rect lights and the studio dome (repackshot_amd/csrc/rpx_generato
 
 A step renders the whole image at 64 samples per pixel into the film resident in HBM (the path
 pool's launch shape and size chosen first by the library's tuner, outside the timed region).
-Prints one JSON line: Mpaths/s of the timed steps, Mrays/s, the roofline of the closest-hit kernel
-(the dominant one), the CPU baseline (the oracle on this host's cores over a bounded sample of the
+Prints one JSON line: Mpaths/s of the timed steps, Mrays/s, the roofline of the traversal phase
+(the closest-hit kernel with the shadow kernel beside it on a second stream), the CPU baseline (the oracle on this host's cores over a bounded sample of the
 same workload), the parity of the GPU film against the oracle on that sample's tiles, and the
 mixed render queue (64 seeded jobs of 256..1024 px, 10k..500k triangles, 16..64 spp) rendered by
 one persistent pool beside the same jobs rendered one at a time in sample passes.
@@ -414,7 +414,7 @@ def main():
         tuned = {"pool": args.pool or (1 << 21), "shape": args.shape, "shadow_shape": args.shadow_shape,
                  "by": "fixed"}
     else:
-        t = scene.autotune(32)
+        t = scene.autotune(spp)
         tuned = {"pool": t["pool"], "shape": t["shape"], "shadow_shape": t["shadow_shape"], "by": "tuner",
                  "candidates_ms": t["ms"], "shadow_candidates_ms": t["shadow_ms"]}
         log("bench: tuner chose pool %d shapes %d / %d (%s %s)" % (t["pool"], t["shape"], t["shadow_shape"], t["ms"],
@@ -449,6 +449,7 @@ def main():
     timing = scene.last_timing()
     scene.set_profiling(False)
     nearest_prof = s_after["nearest"] - s_before["nearest"]
+    shadow_prof = s_after["shadow"] - s_before["shadow"]
     extend_avg_ms = timing["extend_ms"] / max(timing["extend_launches"], 1)
 
     result = {
@@ -502,11 +503,15 @@ def main():
         result["parity"] = {"tiles": len(rects), "pixels": int(sum(w * h for (_, _, w, h) in rects)),
                             "film_bit_exact_vs_oracle": exact, "max_rel_err": max_rel,
                             "image_rrmse": float(np.sqrt(num / max(den, 1e-30)))}
-        # Algorithmic bytes of one closest-hit ray: the reference traversal's BVH8 nodes and packets
-        # visited per query, counted by the oracle over the baseline's rays.
+        # Algorithmic bytes of one closest-hit and one any-hit ray: the reference traversal's BVH8
+        # nodes and packets visited per query, counted by the oracle over the baseline's rays. The
+        # measured span is one traversal phase: the closest-hit kernel and, beside it on a second
+        # stream, the previous iteration's shadow rays.
         b_ray = (NODE_BYTES * counts["nearest_nodes"] + PACKET_BYTES * counts["nearest_packets"]) / max(
             counts["nearest"], 1)
-        per_launch = b_ray * nearest_prof / max(timing["extend_launches"], 1)
+        b_any = (NODE_BYTES * counts["any_nodes"] + PACKET_BYTES * counts["any_packets"]) / max(counts["any"], 1)
+        launches = max(timing["extend_launches"], 1)
+        per_launch = (b_ray * nearest_prof + b_any * shadow_prof) / launches
         achieved = per_launch / (extend_avg_ms * 1e-3) / 1e9
         result["roofline"] = {
             "bound": "hbm",
@@ -514,10 +519,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": round(TRAFFIC_BYTES_PER_RAY * nearest_prof / max(timing["extend_launches"], 1)),
-            "kernel": "k_extend (closest hit), launch shape %d" % tuned["shape"],
+            "traffic": round(TRAFFIC_BYTES_PER_RAY * (nearest_prof + shadow_prof) / launches),
+            "kernel": "traversal phase: k_extend (closest hit, shape %d) with k_shadow (any hit, shape %d) "
+                      "beside it" % (tuned["shape"], tuned["shadow_shape"]),
             "bytes_per_ray": round(b_ray, 1),
-            "closest_hit_rays_per_launch": round(nearest_prof / max(timing["extend_launches"], 1), 1),
+            "bytes_per_shadow_ray": round(b_any, 1),
+            "closest_hit_rays_per_launch": round(nearest_prof / launches, 1),
+            "shadow_rays_per_launch": round(shadow_prof / launches, 1),
             "extend_avg_ms": round(extend_avg_ms, 4),
             "extend_launches_per_step": timing["extend_launches"],
         }

@@ -129,14 +129,16 @@ uint32_t rpx_scene_stats(uint32_t handle, uint64_t* out4);
 uint32_t rpx_set_options(uint32_t handle, uint32_t pool, uint32_t shape, uint32_t shadow_shape, uint32_t group,
                          uint32_t chunk, uint32_t flags, uint32_t tail);
 /* The launch tuner: renders samples [0, samples) of the image (default 32; the film is untouched)
- * with every closest-hit launch shape (8) at pool sizes 2^20 .. 2^25, then every shadow launch
+ * with every closest-hit launch shape (8) at pool sizes 2^21 .. 2^26, then every shadow launch
  * shape at the best of those, keeps the fastest as the handle's options, and reports: pool,
  * closest-hit shape, shadow shape, best ms, then the 48 + 8 candidates' ms. */
 uint32_t rpx_autotune(uint32_t handle, uint32_t samples, float* out60);
 /* Records per-stage GPU timing of the following renders (events on the handle's stream). */
 uint32_t rpx_set_profiling(uint32_t handle, uint32_t on);
 /* Timing of the last render: total ms, then ms of the closest-hit, shade, shadow and regen
- * kernels (summed; with profiling on), their launch counts, pool iterations and chunks. */
+ * stages (summed; with profiling on), their launch counts, pool iterations and chunks. The
+ * shadow kernel runs beside the closest hit on a second stream: the closest-hit stage spans
+ * both traversals, the shadow figure is the shadow kernel's own time within it. */
 uint32_t rpx_last_timing(uint32_t handle, float* out12);
 /* The GPU-generated energy-compensation tables of the current device (1024, 1024, 17408,
  * 17408 u16 and 32, 32 f32). */

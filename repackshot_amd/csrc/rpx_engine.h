@@ -57,7 +57,7 @@ struct Timing {
 constexpr u32 RESULTS_MAX = 1u << 27;
 constexpr int N_TUNE_POOLS = 6;
 constexpr int N_TUNE_CANDIDATES = N_SHAPES * N_TUNE_POOLS + N_SHAPES;
-constexpr u32 TUNE_POOLS[N_TUNE_POOLS] = {1u << 20, 1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25};
+constexpr u32 TUNE_POOLS[N_TUNE_POOLS] = {1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26};
 
 class Engine {
  public:
@@ -115,6 +115,9 @@ class Engine {
   size_t events_used_ = 0;
   std::vector<std::pair<int, size_t>> marks_;
   hipEvent_t ev_[2] = {nullptr, nullptr};
+  // The shadow kernel's stream: forked from stream_ after each regen, joined before the shade.
+  hipStream_t side_ = nullptr;
+  hipEvent_t fork_ = nullptr, join_ = nullptr;
 };
 
 }  // namespace rpx

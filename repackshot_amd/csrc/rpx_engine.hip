@@ -78,6 +78,12 @@ bool device_tables(int dev, DeviceTables*& out, std::string& err) {
 
 Engine::~Engine() {
   if (stream_) hipStreamSynchronize(stream_);
+  if (side_) {
+    hipStreamSynchronize(side_);
+    hipStreamDestroy(side_);
+  }
+  if (fork_) hipEventDestroy(fork_);
+  if (join_) hipEventDestroy(join_);
   dev_free_all(pool_bufs_);
   dev_free_all(adjoint_bufs_);
   dev_free_all(bsdf_bufs_);
@@ -104,6 +110,12 @@ bool Engine::init(int device, hipStream_t stream, std::string& err) {
   if (hipHostMalloc((void**)&idle_host_, 64, hipHostMallocDefault) != hipSuccess || hipEventCreate(&ev_[0]) != hipSuccess ||
       hipEventCreate(&ev_[1]) != hipSuccess) {
     err = "cannot create the pool's host flag or events";
+    return false;
+  }
+  if (hipStreamCreateWithFlags(&side_, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&fork_, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&join_, hipEventDisableTiming) != hipSuccess) {
+    err = "cannot create the shadow rays' stream or its events";
     return false;
   }
   hipMemsetAsync(P.counts, 0, N_COUNTS * 4, stream_);
@@ -147,7 +159,9 @@ bool Engine::ensure_pool(u32 n, std::string& err) {
             dev_alloc(B, f4, (void**)&P.sh_d) && dev_alloc(B, f4, (void**)&P.sh_c) &&
             dev_alloc(B, f4, (void**)&P.ident) && dev_alloc(B, (size_t)n * 8, (void**)&P.key) &&
             dev_alloc(B, (size_t)n * 4, (void**)&P.live[0]) && dev_alloc(B, (size_t)n * 4, (void**)&P.live[1]) &&
-            dev_alloc(B, (size_t)n * 4, (void**)&P.shadow_queue) && dev_alloc(B, (size_t)n * 4, (void**)&P.done_queue);
+            dev_alloc(B, (size_t)n * 4, (void**)&P.shadow_queue[0]) &&
+            dev_alloc(B, (size_t)n * 4, (void**)&P.shadow_queue[1]) &&
+            dev_alloc(B, (size_t)n * 4, (void**)&P.done_queue[0]) && dev_alloc(B, (size_t)n * 4, (void**)&P.done_queue[1]);
   if (!ok) {
     dev_free_all(pool_bufs_);
     err = "device memory exhausted for the path pool";
@@ -217,6 +231,7 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegm
   L.ext = ext ? 1u : 0u;
   L.tail = adj ? 0u : opt.tail;
   L.stream = stream_;
+  L.side = side_;
   e = launch_pool_init(P, L, total);
   if (e != hipSuccess) return e;
   u32 group = std::max<u32>(2, opt.group & ~1u);
@@ -229,6 +244,15 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegm
   auto done = [&]() {
     if (profiling && !opt.graph) hipEventRecord(next_event(), stream_);
   };
+  auto mark_side = [&](int stage) {
+    if (profiling && !opt.graph) {
+      marks_.push_back({stage, events_used_});
+      hipEventRecord(next_event(), side_);
+    }
+  };
+  auto done_side = [&]() {
+    if (profiling && !opt.graph) hipEventRecord(next_event(), side_);
+  };
   auto iterations = [&](u32 count) -> hipError_t {
     hipError_t r = hipSuccess;
     for (u32 k = 0; k < count && r == hipSuccess; ++k) {
@@ -237,14 +261,21 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegm
       r = launch_regen(P, L, parity, adj);
       done();
       if (r != hipSuccess) break;
-      if (L.tail) {
-        mark(4);
-        r = launch_finish(P, L, parity);
-        done();
-        if (r != hipSuccess) break;
-      }
+      // The last iteration's shadow rays, beside this iteration's extension rays.
+      r = hipEventRecord(fork_, stream_);
+      if (r == hipSuccess) r = hipStreamWaitEvent(side_, fork_, 0);
+      if (r != hipSuccess) break;
+      mark_side(2);
+      r = launch_shadow(P, L, parity ^ 1u, adj);
+      done_side();
+      if (r == hipSuccess) r = hipEventRecord(join_, side_);
+      if (r != hipSuccess) break;
+      // The extend stage's time spans both traversals (closest hit here, any hit beside it).
       mark(0);
       r = launch_extend(P, L, parity);
+      // Joined even after an error, so that no side work outlives the chunk unawaited.
+      hipError_t w = hipStreamWaitEvent(stream_, join_, 0);
+      if (r == hipSuccess) r = w;
       done();
       if (r != hipSuccess) break;
       if (L.ext) {
@@ -253,12 +284,14 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegm
         done();
         if (r != hipSuccess) break;
       }
+      if (L.tail) {
+        mark(4);
+        r = launch_finish(P, L, parity);
+        done();
+        if (r != hipSuccess) break;
+      }
       mark(1);
       r = launch_shade(P, L, parity, adj);
-      done();
-      if (r != hipSuccess) break;
-      mark(2);
-      r = launch_shadow(P, L, adj);
       done();
     }
     return r;

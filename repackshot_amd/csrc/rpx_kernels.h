@@ -4,13 +4,15 @@
 // (samples [first, first + count) of a pixel rectangle of one scene); every (segment, sample,
 // pixel) is one work id, numbered segment by segment, then pixel by pixel (8 x 8 tiles), the
 // samples of a pixel consecutive. Each iteration runs
-//   regen   finished slots write their radiance to `results[work id]` and take the next work ids
-//           from a device cursor (camera ray of the new path), appending to the live queue
-//   extend  closest hit of every live path
+//   regen   the slots finished two iterations before write their radiance to `results[work id]`
+//           and take the next work ids from a device cursor (camera ray of the new path),
+//           appending to the live queue
+//   extend  closest hit of every live path; beside it, on a second stream, the shadow kernel
+//           (any hit of the previous iteration's shadow rays; unblocked ones add their light)
 //   shade   surface, emission, NEE, BSDF sample, roulette: survivors to the other live queue,
-//           finished paths to the done queue, shadow rays to the shadow queue
-//   shadow  any hit of the shadow rays; unblocked ones add their contribution
-// until a regen leaves no live path. A splat then adds each pixel's samples to its film in
+//           finished paths to this parity's done queue, shadow rays to this parity's shadow queue
+// until a regen leaves no live path and nothing pending. A path's additions keep their order: the
+// shadow ray of a vertex adds its light before the shade of the next vertex runs. A splat then adds each pixel's samples to its film in
 // sample-index order, so the film does not depend on the pool size or on the order paths finish.
 #pragma once
 #include "rpx_scene.h"
@@ -54,8 +56,14 @@ struct DevSegment {
 
 // Counter slots (u32) of PathState::counts.
 // C_TAIL: paths handed to the drain kernel by the last bookkeeping (0: none).
-constexpr u32 C_LIVE0 = 0, C_LIVE1 = 1, C_SHADOW = 2, C_DONE = 3, C_CURSOR = 4, C_END = 5, C_IDLE = 6, C_TAIL = 7;
+// The shadow and finished queues are kept per iteration parity: the shadow rays of one iteration
+// are traced beside the next iteration's extension rays, and its finished slots are taken by the
+// regen of the iteration after that, once their shadow rays have added their light.
+constexpr u32 C_LIVE0 = 0, C_LIVE1 = 1, C_SHADOW0 = 2, C_DONE0 = 3, C_CURSOR = 4, C_END = 5, C_IDLE = 6, C_TAIL = 7,
+              C_SHADOW1 = 8, C_DONE1 = 9;
 constexpr u32 N_COUNTS = 16;
+__host__ __device__ inline u32 c_shadow(u32 parity) { return parity ? C_SHADOW1 : C_SHADOW0; }
+__host__ __device__ inline u32 c_done(u32 parity) { return parity ? C_DONE1 : C_DONE0; }
 
 // One entry per path slot.
 struct PathState {
@@ -74,8 +82,8 @@ struct PathState {
   u64* key;        // pixel key of the sampler
   uint4* ident;    // results slot of the path's work id (NO_WORK: empty slot), scene, x | y << 16, sample index
   u32* live[2];    // live queues (ping-pong by iteration)
-  u32* shadow_queue;
-  u32* done_queue;
+  u32* shadow_queue[2];  // shadow rays made by the shade of an iteration of that parity
+  u32* done_queue[2];    // slots finished by the shade of an iteration of that parity
   u32* counts;     // see C_*
   u64* rays;       // [0] closest-hit rays traced, [1] shadow rays counted without a trace, [2] shadow rays traced,
                    // [3] thin-wall probes after a medium's exit (counted, never needed: no such walls)
@@ -121,6 +129,7 @@ struct Launch {
   u32 tail;          // once no new path is left, at most this many live paths are drained in one launch (0: never)
   u32 ext;           // a scene has sphere lights or scattering media: the extended shade and the walk kernel
   hipStream_t stream;
+  hipStream_t side;  // the shadow rays' stream (default priority), forked from and joined to `stream`
 };
 
 hipError_t launch_pool_init(const PathState& P, const Launch& L, u32 work_end);
@@ -128,7 +137,8 @@ hipError_t launch_pool_init(const PathState& P, const Launch& L, u32 work_end);
 hipError_t launch_regen(const PathState& P, const Launch& L, u32 parity, bool adj = false);
 hipError_t launch_extend(const PathState& P, const Launch& L, u32 parity);
 hipError_t launch_shade(const PathState& P, const Launch& L, u32 parity, bool adj = false);
-hipError_t launch_shadow(const PathState& P, const Launch& L, bool adj = false);
+// Traces the shadow rays the shade of an iteration of parity `parity` made, on L.side.
+hipError_t launch_shadow(const PathState& P, const Launch& L, u32 parity, bool adj = false);
 // One step of the walks of the live paths inside scattering media, at the closest hit extend found:
 // a scattering event or a reflection back in gives the next step's ray (traced by the next
 // extend); otherwise the walk ends (hit.w says how) and the shade takes the vertex.

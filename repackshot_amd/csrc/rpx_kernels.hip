@@ -524,12 +524,12 @@ __device__ __forceinline__ void tiled_pixel(u32 w, u32 h, u32 p, u32& x, u32& y)
 __global__ void __launch_bounds__(256) k_pool_init(PathState P, u32 pool, u32 work_end) {
   u32 stride = gridDim.x * blockDim.x;
   for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < pool; i += stride) {
-    P.done_queue[i] = i;
+    P.done_queue[0][i] = i;
     P.ident[i] = make_uint4(NO_WORK, 0u, 0u, 0u);
   }
   if (blockIdx.x == 0 && threadIdx.x < N_COUNTS) {
     u32 c = threadIdx.x;
-    P.counts[c] = c == C_DONE ? pool : (c == C_END ? work_end : 0u);
+    P.counts[c] = c == C_DONE0 ? pool : (c == C_END ? work_end : 0u);
   }
 }
 
@@ -545,7 +545,7 @@ __device__ __forceinline__ u32 new_paths(u32 done, u32 cursor, u32 end) {
 template <bool ADJ>
 __global__ void __launch_bounds__(256) k_regen(const DevScene* __restrict__ scenes, const DevSegment* __restrict__ segs,
                                                u32 n_segs, PathState P, u32 parity) {
-  u32 n = P.counts[C_DONE];
+  u32 n = P.counts[c_done(parity)];
   u32 end = P.counts[C_END];
   u32 cursor = P.counts[C_CURSOR];
   u32 survivors = P.counts[C_LIVE0 + parity];
@@ -560,7 +560,7 @@ __global__ void __launch_bounds__(256) k_regen(const DevScene* __restrict__ scen
     bool active = j < n;
     u32 slot = 0;
     if (active) {
-      slot = P.done_queue[j];
+      slot = P.done_queue[parity][j];
       u32 done_work = P.ident[slot].x;
       if (!ADJ && done_work != NO_WORK) P.results[done_work] = P.sum[slot];
     }
@@ -617,24 +617,27 @@ __global__ void __launch_bounds__(256) k_regen(const DevScene* __restrict__ scen
 }
 
 // The counters after a regen: the live queue holds survivors + new paths, the cursor moves past
-// the new paths, and the queues this iteration's shade and shadow fill start empty. An empty live
-// queue marks the pool idle (every path of the chunk done). Once no new path is left and at most
-// `tail` paths are live, they go to the drain kernel instead (C_TAIL) and this iteration's extend,
-// shade and shadow find empty queues.
+// the new paths, and the queues this iteration's shade fills (this parity's finished and shadow
+// queues, both consumed by now) start empty; the other parity's keep what the last shade left
+// for this iteration's shadow kernel and the next regen. No live path and nothing left in those
+// marks the pool idle (every path of the chunk done). Once no new path is left and at most `tail`
+// paths are live, they go to the drain kernel instead (C_TAIL) and this iteration's extend and
+// shade find empty queues.
 __global__ void k_bookkeep(PathState P, u32 parity, u32 tail) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  u32 n = P.counts[C_DONE], end = P.counts[C_END], cursor = P.counts[C_CURSOR];
+  u32 n = P.counts[c_done(parity)], end = P.counts[C_END], cursor = P.counts[C_CURSOR];
   u32 fresh = new_paths(n, cursor, end);
   u32 live = P.counts[C_LIVE0 + parity] + fresh;
   P.counts[C_LIVE0 + parity] = live;
   P.counts[C_CURSOR] = cursor + fresh;
-  P.counts[C_DONE] = 0;
+  P.counts[c_done(parity)] = 0;
   P.counts[C_LIVE0 + (parity ^ 1)] = 0;
-  P.counts[C_SHADOW] = 0;
+  P.counts[c_shadow(parity)] = 0;
   P.counts[C_TAIL] = 0;
   P.rays[0] += live;
+  u32 pending = P.counts[c_done(parity ^ 1)] + P.counts[c_shadow(parity ^ 1)];
   if (live == 0) {
-    P.counts[C_IDLE] = 1;
+    if (pending == 0) P.counts[C_IDLE] = 1;
   } else if (live <= tail && cursor + fresh >= end) {
     P.counts[C_TAIL] = live;
     P.counts[C_LIVE0 + parity] = 0;
@@ -709,10 +712,11 @@ __device__ void commit_grad(const PathState& P, bool has, const VertexGrad& v) {
 }
 
 template <int BLK, int LDSN, int WPE, bool ADJ>
-__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) k_shadow(const DevScene* __restrict__ scenes, PathState P) {
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE))) k_shadow(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
   __shared__ u32 lds[LDSN * BLK];
   u32* col = lds + threadIdx.x;
-  u32 n = P.counts[C_SHADOW];
+  u32 n = P.counts[c_shadow(parity)];
+  const u32* q = P.shadow_queue[parity];
   if (blockIdx.x == 0 && threadIdx.x == 0) P.rays[2] += n;
   u32 stride = gridDim.x * blockDim.x;
   // Whole waves iterate together (the adjoint's gradient sums are wave operations).
@@ -723,7 +727,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
     bool visible = false;
     u32 i = 0;
     if (j < n) {
-      i = P.shadow_queue[j];
+      i = q[j];
       float4 o = P.sh_o[i], d = P.sh_d[i];
       Geo g = geo_of(scenes, P.ident[i].y);
       if (!trace_any<BLK, LDSN>(g, col, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), RAY_EPSILON, o.w)) {
@@ -867,9 +871,10 @@ k_extend_flat(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
 
 template <int BLK, int LDSN, int WPE>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
-k_shadow_flat(const DevScene* __restrict__ scenes, PathState P) {
+k_shadow_flat(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
   __shared__ u32 lds[LDSN * BLK];
-  const u32 n = P.counts[C_SHADOW];
+  const u32 n = P.counts[c_shadow(parity)];
+  const u32* q = P.shadow_queue[parity];
   if (blockIdx.x == 0 && threadIdx.x == 0) P.rays[2] += n;
   const u32 stride = gridDim.x * blockDim.x;
   u32 j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -885,7 +890,7 @@ k_shadow_flat(const DevScene* __restrict__ scenes, PathState P) {
   PRay ray = prepare(o, d, RAY_EPSILON, RPX_INF);
   for (;;) {
     while (!busy && j < n) {
-      slot = P.shadow_queue[j];
+      slot = q[j];
       float4 so = P.sh_o[slot], sd = P.sh_d[slot];
       o = v3(so.x, so.y, so.z);
       d = v3(sd.x, sd.y, sd.z);
@@ -1736,13 +1741,13 @@ k_shade(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
     // Compaction of the survivors, the finished paths and the shadow rays (one atomic per block
     // and queue).
     bool done = active && !cont;
-    u32* const counters[3] = {out_n, &P.counts[C_DONE], &P.counts[C_SHADOW]};
+    u32* const counters[3] = {out_n, &P.counts[c_done(parity)], &P.counts[c_shadow(parity)]};
     const bool want[3] = {cont, done, want_shadow};
     u32 slots[3];
     block_append<3>(counters, want, slots);
     if (cont) out_q[slots[0]] = i;
-    if (done) P.done_queue[slots[1]] = i;
-    if (want_shadow) P.shadow_queue[slots[2]] = i;
+    if (done) P.done_queue[parity][slots[1]] = i;
+    if (want_shadow) P.shadow_queue[parity][slots[2]] = i;
   }
 }
 
@@ -2677,16 +2682,17 @@ hipError_t launch_walk(const PathState& P, const Launch& L, u32 parity) {
   hipLaunchKernelGGL(k_walk, dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, P, parity);
   return hipGetLastError();
 }
-hipError_t launch_shadow(const PathState& P, const Launch& L, bool adj) {
+hipError_t launch_shadow(const PathState& P, const Launch& L, u32 parity, bool adj) {
   if (adj) {
-    hipLaunchKernelGGL((k_shadow<128, 24, 1, true>), dim3(grid_for(L.pool, 128, 1u << 20)), dim3(128), 0, L.stream,
-                       L.scenes, P);
+    hipLaunchKernelGGL((k_shadow<128, 24, 1, true>), dim3(grid_for(L.pool, 128, 1u << 20)), dim3(128), 0, L.side,
+                       L.scenes, P, parity);
     return hipGetLastError();
   }
   switch (L.shadow_shape) {
 #define X(I, B, N, W)                                                                                             \
   case I:                                                                                                        \
-    hipLaunchKernelGGL((k_shadow<B, N, W, false>), dim3(grid_for(L.pool, B, 1u << 20)), dim3(B), 0, L.stream, L.scenes, P); \
+    hipLaunchKernelGGL((k_shadow<B, N, W, false>), dim3(grid_for(L.pool, B, 1u << 20)), dim3(B), 0, L.side, L.scenes, \
+                       P, parity);                                                                               \
     break;
     RPX_TRAVERSAL_SHAPES(X)
 #undef X
@@ -2694,7 +2700,7 @@ hipError_t launch_shadow(const PathState& P, const Launch& L, bool adj) {
   case I: {                                                                                                 \
     static const u32 resident = resident_blocks(k_shadow_flat<B, N, W>, B);                                \
     u32 grid = grid_for(L.pool, B, resident);                                                               \
-    hipLaunchKernelGGL((k_shadow_flat<B, N, W>), dim3(grid), dim3(B), 0, L.stream, L.scenes, P);            \
+    hipLaunchKernelGGL((k_shadow_flat<B, N, W>), dim3(grid), dim3(B), 0, L.side, L.scenes, P, parity);      \
     break;                                                                                                  \
   }
     RPX_FLAT_SHAPES(X)
