@@ -44,11 +44,11 @@ HBM_PEAK_GBS = 8000.0  # MI355X datasheet: 8 TB/s HBM3E
 # geometry/triangle.rs:160-167: v [3][3][4] f32 + id [4] u32 = 160 B).
 NODE_BYTES = 256
 PACKET_BYTES = 160
-# Memory-side bytes of k_extend per closest-hit ray, from rocprofv3 FETCH_SIZE and WRITE_SIZE runs
-# of this workload (pool 33,554,432, closest-hit shape 2, shadow shape 4, sample-minor work order;
-# FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0, the factors a 256 MiB device copy gives): 1.403e9
-# bytes per dispatch over 5,695,387 rays.
-TRAFFIC_BYTES_PER_RAY = 246.4
+# Memory-side bytes of one traversal phase (k_extend + k_shadow) per traced ray, from rocprofv3
+# FETCH_SIZE and WRITE_SIZE runs of this workload (pool 67,108,864, closest-hit shape 2, shadow
+# shape 4; FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0, the factors a 256 MiB device copy gives):
+# 1.376e9 + 0.862e9 bytes per dispatch pair over 5,695,387 closest-hit and 938,280 shadow rays.
+TRAFFIC_BYTES_PER_RAY = 337.5
 
 
 def log(*a):
