@@ -52,7 +52,8 @@
  * hosts' progressive loop (rt-cli/src/lib.rs:266-366), with the film resident in HBM. A film
  * accumulates the samples of each call in sample-index order per pixel (image.rs:81-94), so any
  * split of [0, n) into consecutive calls gives the same film. Calls on one handle are
- * serialised; different handles run on their own streams.
+ * serialised; different handles run on their own streams (two per handle: the path pool's, and
+ * one its shadow rays are traced on beside the closest hit, joined within each call).
  */
 #ifndef RPX_H
 #define RPX_H
