@@ -92,6 +92,17 @@ def lib():
         L.orc_phase.argtypes = [f32, f32]
         L.orc_sample_phase.restype = f32
         L.orc_sample_phase.argtypes = [vp, f32, f32, f32, vp]
+        L.orc_lights_info.restype = u32
+        L.orc_lights_info.argtypes = [vp, vp, vp, vp, u32]
+        L.orc_lights_pmf.restype = f32
+        L.orc_lights_pmf.argtypes = [vp, vp, u32]
+        L.orc_lights_pick.argtypes = [vp, f32, vp, vp]
+        L.orc_lights_pick_at.argtypes = [vp, vp, f32, vp, vp, vp]
+        L.orc_set_techniques.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.orc_soup.restype = u32
+        L.orc_soup.argtypes = [vp, u32, vp]
+        L.orc_random_rays.restype = ctypes.c_int
+        L.orc_random_rays.argtypes = [vp, u32, u64, vp]
         os.makedirs(os.path.dirname(CACHE), exist_ok=True)
         L.orc_set_energy_cache(CACHE.encode())
         _lib = L
@@ -223,6 +234,46 @@ class OracleScene:
         gt = np.zeros((tex[1], tex[0], 3) if tex else (0,), np.float64)
         lib().orc_adjoint(self.h, first, count, threads, _p(d_image), _p(gm), _p(gt) if tex else None)
         return gm.reshape(n, 5), gt
+
+    def lights(self):
+        """(per_point, cdf, flux_pmf) of the prepared lights (light.rs:673-994)."""
+        cdf = np.zeros(256, np.float32)
+        pmf = np.zeros(256, np.float32)
+        per = np.zeros(1, np.uint32)
+        n = lib().orc_lights_info(self.h, _p(per), _p(cdf), _p(pmf), 256)
+        return bool(per[0]), cdf[:n].copy(), pmf[:n].copy()
+
+    def light_pmf(self, point, entry):
+        """Probability that next-event estimation at `point` picks `entry` (light.rs:1076-1088)."""
+        return float(lib().orc_lights_pmf(self.h, _p(np.asarray(point, np.float32)), entry))
+
+    def light_pick(self, u):
+        """pick() on the static cdf: (entry, rest) (light.rs:1059-1074)."""
+        i = np.zeros(1, np.uint32)
+        r = np.zeros(1, np.float32)
+        lib().orc_lights_pick(self.h, u, _p(i), _p(r))
+        return int(i[0]), float(r[0])
+
+    def light_pick_at(self, point, u):
+        """pick_at() on the selection at `point`: (entry, rest, probability) (light.rs:1118-1141)."""
+        i = np.zeros(1, np.uint32)
+        r = np.zeros(1, np.float32)
+        q = np.zeros(1, np.float32)
+        lib().orc_lights_pick_at(self.h, _p(np.asarray(point, np.float32)), u, _p(i), _p(r), _p(q))
+        return int(i[0]), float(r[0]), float(q[0])
+
+    def set_techniques(self, techniques, per_point=None):
+        """0 MIS, 1 lights only, 2 BSDF only (the reference tests' switch); per_point forces the
+        per-point selection on or off."""
+        lib().orc_set_techniques(self.h, techniques, -1 if per_point is None else int(bool(per_point)))
+
+    def random_rays(self, count, seed):
+        """The reference tests' random rays around this scene's geometry (geometry/tests.rs:92-140),
+        8 floats each: origin, direction, t_min, t_max."""
+        out = np.zeros((count, 8), np.float32)
+        if lib().orc_random_rays(self.h, count, seed, _p(out)) != 0:
+            raise ValueError("the scene has no instances")
+        return out
 
     def trace_rays(self, rays, brute=False):
         rays = np.ascontiguousarray(rays, np.float32)
@@ -395,10 +446,12 @@ class Bsdf:
         self.mat = np.array([kind, *base, metallic, roughness, ior, transmission, coat, coat_roughness], np.float32)
         self.entering = 1 if entering else 0
 
-    def eval(self, wo, wi, all_lobes=True):
+    def eval(self, wo, wi, all_lobes=True, single=False):
+        """f(wo, wi); `single` leaves out the energy-compensation (multiple-scattering) lobes."""
         out = np.zeros(3, np.float32)
+        mode = 2 if single else (1 if all_lobes else 0)
         lib().orc_bsdf_probe(_p(self.mat), self.entering, _p(np.asarray(wo, np.float32)),
-                             _p(np.asarray(wi, np.float32)), 0.0, 0.0, 0, 1 if all_lobes else 0, _p(out))
+                             _p(np.asarray(wi, np.float32)), 0.0, 0.0, 0, mode, _p(out))
         return out
 
     def pdf(self, wo, wi):
@@ -414,6 +467,17 @@ class Bsdf:
         ok = lib().orc_bsdf_probe(_p(self.mat), self.entering, _p(np.asarray(wo, np.float32)), _p(z), u0, u1, 2, 1,
                                   _p(out))
         return (out[:3].copy(), out[3:6].copy(), float(out[6])) if ok else None
+
+
+def soup():
+    """The reference tests' degenerate triangle soup (geometry/tests.rs:48-88): positions (n, 3)
+    and 15,000 indices."""
+    pos = np.zeros((20000, 3), np.float32)
+    idx = np.zeros(15000, np.uint32)
+    n = lib().orc_soup(_p(pos), 20000, _p(idx))
+    if n == 0:
+        raise RuntimeError("soup positions overflow")
+    return pos[:n].copy(), idx
 
 
 def fresnel_dielectric(cos_i, eta):

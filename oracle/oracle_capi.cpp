@@ -12,15 +12,11 @@
 #include "rt_adaptive.hpp"
 #include "rt_denoise.hpp"
 #include "rt_passes.hpp"
-#include "rt_render.hpp"
+#include "rt_handle.hpp"
 
 using namespace orc;
 
 namespace {
-struct Handle {
-  Scene scene;
-  Prepared prep;
-};
 void set_err(char* err, size_t cap, const std::string& msg) {
   if (!err || cap == 0) return;
   size_t n = std::min(cap - 1, msg.size());
@@ -223,8 +219,8 @@ void orc_energy_tables(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_in, 
 
 // The BSDF of a material seen from one side, for its closed-form tests (bsdf.rs:384-992):
 // `mat` = kind, base r, g, b, metallic, roughness, ior, transmission, coat, coat roughness;
-// op 0 eval (out rgb; `all` 0 for the diffuse lobes only), 1 pdf (out[0]), 2 sample (out: wi xyz,
-// weight rgb, pdf; returns 0 when it gives none).
+// op 0 eval (out rgb; `all` 0 for the diffuse lobes only, 2 without the energy-compensation
+// lobes), 1 pdf (out[0]), 2 sample (out: wi xyz, weight rgb, pdf; returns 0 when it gives none).
 int orc_bsdf_probe(const float* mat, int entering, const float* wo, const float* wi, float u0, float u1, int op,
                    int all, float* out) {
   Material m;
@@ -239,6 +235,11 @@ int orc_bsdf_probe(const float* mat, int entering, const float* wo, const float*
   Bsdf b = Bsdf::surface(m, entering != 0);
   Vec3 o(wo[0], wo[1], wo[2]);
   if (op == 0) {
+    if (all == 2) {
+      // The single-scattering lobes alone: the energy-compensation lobes switched off.
+      b.ms_spec_on = false;
+      b.ms_coat_on = false;
+    }
     Rgb f = b.eval(o, Vec3(wi[0], wi[1], wi[2]), all ? Lobes::All : Lobes::Diffuse);
     out[0] = f.r;
     out[1] = f.g;
@@ -261,6 +262,52 @@ int orc_bsdf_probe(const float* mat, int entering, const float* wo, const float*
   return 1;
 }
 float orc_fresnel_dielectric(float cos_i, float eta) { return fresnel_dielectric(cos_i, eta); }
+
+// The prepared lights of a scene, for the reference's selection tests (light.rs:1026-1141,
+// 1708-1750): entries, whether the selection is per point, the static cdf and per-entry pmf.
+uint32_t orc_lights_info(void* p, uint32_t* per_point, float* cdf, float* flux_pmf, uint32_t cap) {
+  const Lights& l = static_cast<Handle*>(p)->prep.lights;
+  *per_point = l.per_point ? 1u : 0u;
+  for (size_t k = 0; k < l.cdf.size() && k < cap; ++k) {
+    cdf[k] = l.cdf[k];
+    flux_pmf[k] = l.flux_pmf[k];
+  }
+  return (uint32_t)l.entries.size();
+}
+
+// The probability with which next-event estimation at `point` picks `entry` (Lights::pmf of
+// Lights::select there).
+float orc_lights_pmf(void* p, const float* point, uint32_t entry) {
+  const Lights& l = static_cast<Handle*>(p)->prep.lights;
+  Selection s = l.select(Vec3(point[0], point[1], point[2]));
+  return l.pmf(entry, s);
+}
+
+// pick() on the static cdf: the entry and u rescaled inside its bucket.
+void orc_lights_pick(void* p, float u, uint32_t* index, float* rest) {
+  const Lights& l = static_cast<Handle*>(p)->prep.lights;
+  size_t i = 0;
+  l.pick_static(u, i, *rest);
+  *index = (uint32_t)i;
+}
+
+// pick_at() on the selection at `point`: entry, rescaled u and probability.
+void orc_lights_pick_at(void* p, const float* point, float u, uint32_t* index, float* rest, float* prob) {
+  const Lights& l = static_cast<Handle*>(p)->prep.lights;
+  Selection s = l.select(Vec3(point[0], point[1], point[2]));
+  size_t i = 0;
+  l.pick_at(s, u, i, *rest, *prob);
+  *index = (uint32_t)i;
+}
+
+// The reference tests' switch (light.rs:550-558, integrator/tests.rs:2574-2583): techniques
+// 0 MIS, 1 lights only, 2 BSDF only; per_point 0 or 1 sets the per-point selection, other values
+// keep it.
+void orc_set_techniques(void* p, int techniques, int per_point) {
+  Lights& l = static_cast<Handle*>(p)->prep.lights;
+  l.techniques = techniques;
+  if (per_point == 0 || per_point == 1) l.per_point = per_point == 1 && !l.entries.empty();
+}
 
 // Scalar numerics for the golden-vector tests: 0 exp2, 1 log2, 2 powf, 3 sin, 4 cos, 5 atan2,
 // 6 acos.

@@ -620,6 +620,47 @@ float Lights::weight_at(size_t entry, const Selection& s) const {
   return entry < (size_t)KEPT_WEIGHTS ? s.weights[entry] : weight(entry, s.point);
 }
 
+// light.rs:1118-1141: pick() on the cumulative distribution of the weights at the selection's
+// vertex, built entry by entry up to the bucket that holds u: the entry, u rescaled inside its
+// bucket, and weight / total.
+void Lights::pick_at(const Selection& from, float u, size_t& index, float& rest, float& p) const {
+  float total = from.total;
+  size_t last = entries.size() - 1;
+  float acc = 0.0f, lo = 0.0f;
+  index = last;
+  rest = 0.0f;
+  p = 0.0f;
+  for (size_t e = 0; e <= last; ++e) {
+    float w = weight_at(e, from);
+    acc += w;
+    float hi = e == last ? 1.0f : acc / total;
+    if (hi > u || e == last) {
+      float width = hi - lo;
+      rest = width > 0.0f ? clampf((u - lo) / width, 0.0f, 0.99999994f) : 0.0f;
+      index = e;
+      p = w / total;
+      return;
+    }
+    lo = hi;
+  }
+}
+
+void Lights::pick_static(float u, size_t& index, float& rest) const { pick(cdf, u, index, rest); }
+
+// light.rs:1086-1111: the MIS weights of a light found by a BSDF-sampled ray and of a light
+// sample, with the reference tests' switch to one technique alone.
+float Lights::found_weight(float bsdf_pdf, float light_pdf) const {
+  if (techniques == TECH_LIGHTS_ONLY) return positive(light_pdf) ? 0.0f : 1.0f;
+  if (techniques == TECH_BSDF_ONLY) return 1.0f;
+  return power_heuristic(bsdf_pdf, light_pdf);
+}
+
+float Lights::sampled_weight(float light_pdf, float bsdf_pdf) const {
+  if (techniques == TECH_LIGHTS_ONLY) return 1.0f;
+  if (techniques == TECH_BSDF_ONLY) return 0.0f;
+  return power_heuristic(light_pdf, bsdf_pdf);
+}
+
 float Lights::pmf(u32 entry, const Selection& from) const {
   if (entry >= entries.size()) return 0.0f;
   if (from.by_flux) return flux_pmf[entry];
@@ -638,25 +679,7 @@ bool Lights::sample_selected(const Selection& from, float u_select, float u0, fl
     p = pmf((u32)index, from);
   } else {
     if (!usable_total(from.total)) return false;
-    float total = from.total;
-    size_t last = n - 1;
-    float acc = 0.0f, lo = 0.0f;
-    index = last;
-    rest = 0.0f;
-    p = 0.0f;
-    for (size_t e = 0; e <= last; ++e) {
-      float w = weight_at(e, from);
-      acc += w;
-      float hi = e == last ? 1.0f : acc / total;
-      if (hi > u_select || e == last) {
-        float width = hi - lo;
-        rest = width > 0.0f ? clampf((u_select - lo) / width, 0.0f, 0.99999994f) : 0.0f;
-        index = e;
-        p = w / total;
-        break;
-      }
-      lo = hi;
-    }
+    pick_at(from, u_select, index, rest, p);
   }
   (void)rest;
   if (!positive(p)) return false;
@@ -751,7 +774,7 @@ Rgb Lights::crossed_from(const Ray& ray, float t_max, bool has_bsdf, float bsdf_
     if (has_bsdf) {
       float d = offset + t;
       float light_pdf = pmf(light.entry, *from) * d * d / (light.area * cos);
-      weight = power_heuristic(bsdf_pdf, light_pdf);
+      weight = found_weight(bsdf_pdf, light_pdf);
     }
     sum += light.radiance_at(cos) * weight;
   }
@@ -768,7 +791,7 @@ Rgb Lights::crossed_from(const Ray& ray, float t_max, bool has_bsdf, float bsdf_
     if (has_bsdf) {
       float omc;
       if (light.cone((light.centre - vertex).length_squared(), omc) && omc > 0.0f)
-        weight = power_heuristic(bsdf_pdf, pmf(light.entry, *from) / (TAU * omc));
+        weight = found_weight(bsdf_pdf, pmf(light.entry, *from) / (TAU * omc));
       else
         weight = 0.0f;
     }
@@ -792,7 +815,7 @@ bool Lights::camera_sees_panel(const Ray& ray, float t_max) const {
 Rgb Lights::escaped(Vec3 dir, bool has_bsdf, float bsdf_pdf, const Selection* from) const {
   Rgb env_rad = Rgb::splat(env.value(dir.z));
   if (!has_bsdf) return env_rad;
-  if (has_env_entry) return env_rad * power_heuristic(bsdf_pdf, pmf(env_entry, *from) * env.pdf(dir));
+  if (has_env_entry) return env_rad * found_weight(bsdf_pdf, pmf(env_entry, *from) * env.pdf(dir));
   return env_rad;
 }
 
@@ -1280,7 +1303,7 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
           arrives = !blocked;
         }
         if (arrives) {
-          float weight = (ls.delta || !ls.glossy) ? 1.0f : power_heuristic(ls.pdf, bsdf.pdf(wo, wi));
+          float weight = (ls.delta || !ls.glossy) ? 1.0f : prep.lights.sampled_weight(ls.pdf, bsdf.pdf(wo, wi));
           Rgb c = throughput * f * ls.radiance * (std::fabs(wi.z) * weight / ls.pdf);
           if (adjoint && has_mat && std::isfinite(c.r) && std::isfinite(c.g) && std::isfinite(c.b)) {
             Rgb d[5];

@@ -159,6 +159,7 @@ struct LightSample {
   bool delta, glossy, photon, shadow;
 };
 
+constexpr int TECH_MIS = 0, TECH_LIGHTS_ONLY = 1, TECH_BSDF_ONLY = 2;
 struct Lights {
   std::vector<AreaLight> areas;
   std::vector<SphereLight> spheres;
@@ -170,6 +171,9 @@ struct Lights {
   std::vector<int> entries;
   std::vector<float> cdf, flux_pmf;
   bool per_point = false;
+  // Which techniques may find a light (the reference tests' switch, light.rs:550-558): both,
+  // weighted by MIS, as rendering always does, or one of them alone.
+  int techniques = TECH_MIS;
 
   bool build(const Scene& scene, std::string& err);
   float weight(size_t entry, Vec3 x) const;
@@ -177,6 +181,10 @@ struct Lights {
   Selection select_by_flux(Vec3 x) const { Selection s; s.point = x; s.by_flux = true; return s; }
   float weight_at(size_t entry, const Selection& s) const;
   float pmf(u32 entry, const Selection& from) const;
+  void pick_at(const Selection& from, float u, size_t& index, float& rest, float& p) const;
+  void pick_static(float u, size_t& index, float& rest) const;
+  float found_weight(float bsdf_pdf, float light_pdf) const;
+  float sampled_weight(float light_pdf, float bsdf_pdf) const;
   bool sample_selected(const Selection& sel, float u_select, float u0, float u1, LightSample& out) const;
   Rgb crossed_from(const Ray& ray, float t_max, bool has_bsdf, float bsdf_pdf, const Selection* from, float offset) const;
   Rgb escaped(Vec3 dir, bool has_bsdf, float bsdf_pdf, const Selection* from) const;

@@ -126,8 +126,11 @@ def test_example_at_four_samples_has_the_pinned_hash(example_rsb):
 
 
 def test_generated_example_equals_the_reference_example(example_rsb, rpx):
-    # The library's generator of the example (example.rs:20-230) gives the same film.
-    s = orc.OracleScene(rpx.write_scene(0))
+    # The library's generator of the example (example.rs:20-230) writes the reference's fixture
+    # byte for byte (tests/fixtures/example.rsb, 8,768 bytes) and so gives the same film.
+    generated = rpx.write_scene(0)
+    assert len(generated) == 8768 and generated == example_rsb
+    s = orc.OracleScene(generated)
     (rgb, w, _, _), _ = s.render_film(0, 4)
     assert native.fnv1a(native.resolve(rgb, w)) == PINNED_EXAMPLE_4SPP
 
