@@ -216,6 +216,17 @@ uint32_t rpx_denoise_timing(uint32_t handle, float* out8);
 uint32_t rpx_denoise_counted(uint32_t preset, float strength, uint32_t width, uint32_t height, uint32_t origin_y,
                              uint32_t rows, uint32_t samples, uint32_t y0, uint32_t y1, const uint8_t* in_ptr,
                              float* out_ptr);
+/* Allocation-free variant of rt_render_tile (no counterpart in the reference): for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same arguments, output (w*h*3 linear RGB floats of one sample, host memory)
+ * and film bits as rt_render_tile, for tiles of at most RPX_NOALLOC_MAX_PIXELS pixels (status 2
+ * beyond). The path state, results and staging live in one 16 MiB region of static device memory
+ * per GPU, held until the process ends; the call allocates nothing. Every call on a GPU shares
+ * that GPU's region, so calls that overlap on one GPU (from different threads or handles) race:
+ * serialise them. */
+#define RPX_NOALLOC_MAX_PIXELS 16384u
+uint32_t rpx_render_tile_noalloc(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t h,
+                                 uint32_t sample_index, float* out);
 /* Number of HIP devices visible (0 without a GPU). */
 uint32_t rpx_device_count(void);
 

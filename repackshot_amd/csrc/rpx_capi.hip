@@ -417,6 +417,74 @@ static uint32_t render_tile_impl(uint32_t handle, uint32_t x, uint32_t y, uint32
   return RPX_OK;
 }
 
+// ---------------------------------------------------------------- allocation-free tile render
+// The device memory of rpx_render_tile_noalloc: one region per GPU, held until the process ends.
+// Its host-side name is not a device address; the per-device address comes from
+// hipGetSymbolAddress on the entry point's first call on each device.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 16u << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mutex;
+static void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on device `dev` (current), looked up once per device.
+static void* scratch_of(int dev, std::string& err) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+    err = "device index beyond the allocation-free entry point's table";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      err = std::string("hipGetSymbolAddress of the scratch region: ") + hipGetErrorString(e);
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+
+uint32_t rpx_render_tile_noalloc(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t hh,
+                                 uint32_t sample_index, float* out) {
+  if (!out) return RPX_NULL_POINTER;
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  if (w == 0 || hh == 0 || (u64)w * hh > RPX_NOALLOC_MAX_PIXELS) {
+    set_error("rpx_render_tile_noalloc renders tiles of 1 to 16384 pixels");
+    return RPX_BAD_TILE;
+  }
+  std::lock_guard<std::mutex> lock(h->mutex);
+  hipSetDevice(h->device);
+  std::string err;
+  void* region = scratch_of(h->device, err);
+  if (!region) {
+    set_error(err);
+    return RPX_GPU_LAUNCH;
+  }
+  u32 W = h->dev.width, H = h->dev.height;
+  std::memset(out, 0, (size_t)w * hh * 3 * 4);
+  if (x >= W || y >= H) return RPX_OK;
+  DevSegment s{};
+  s.scene = 0;
+  s.x0 = x;
+  s.y0 = y;
+  s.w = std::min(w, W - x);
+  s.h = std::min(hh, H - y);
+  s.first = sample_index;
+  s.count = 1;
+  s.film_w = W;
+  h->engine->textured = h->dev.tex_material != NO_TEXTURE;
+  h->engine->ext = h->dev.ext != 0;
+  hipError_t e = h->engine->render_tile_in(h->d_scene, s, 3, region, NOALLOC_SCRATCH_BYTES, out, (size_t)w * 3, err);
+  if (e != hipSuccess) {
+    set_error(err);
+    return e == hipErrorInvalidValue ? RPX_BAD_TILE : RPX_GPU_LAUNCH;
+  }
+  return RPX_OK;
+}
+
 uint32_t rt_render_tile(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t h, uint32_t sample_index,
                         float* out) {
   return render_tile_impl(handle, x, y, w, h, sample_index, out, 3);

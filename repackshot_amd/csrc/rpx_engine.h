@@ -83,6 +83,14 @@ class Engine {
   // closest-hit candidates, then the shadow candidates).
   hipError_t autotune(const DevScene* d_scenes, const std::vector<DevSegment>& probe, std::vector<float>& ms,
                       std::string& err);
+  // One sample of a tile with every per-path buffer (pool, results, segment table, a staging
+  // tile) carved from `scratch`, `bytes` of device memory the caller owns: no allocation. The
+  // tile's channels-per-pixel radiance (3: rgb, 4: rgba) is copied to `out` (host memory, rows
+  // `out_row_floats` apart). Fails with hipErrorInvalidValue when the tile does not fit.
+  hipError_t render_tile_in(const DevScene* d_scenes, const DevSegment& seg, u32 channels, void* scratch, size_t bytes,
+                            float* out, size_t out_row_floats, std::string& err);
+  // The scratch bytes render_tile_in needs for a tile of `pixels` pixels.
+  static size_t tile_scratch_bytes(u32 pixels, bool textured, bool ext, u32 channels);
   // Closest-hit rays, shadow rays counted without a trace, shadow rays traced, thin-wall probes
   // counted after media exits (since init).
   bool ray_counts(u64 out[4]);
@@ -101,8 +109,9 @@ class Engine {
             std::string& err);
   bool ensure_results(u32 n, std::string& err);
   bool ensure_segs(u32 n, std::string& err);
-  hipError_t run_chunk(const DevScene* d_scenes, const std::vector<DevSegment>& chunk, u32 total, bool splat,
-                       bool adj = false, bool disjoint = false);
+  // `seg_table`: device memory for the chunk's segment table (null: the engine's own).
+  hipError_t run_chunk(const DevScene* d_scenes, const DevSegment* chunk, u32 n_chunk, u32 total, bool splat,
+                       bool adj = false, bool disjoint = false, DevSegment* seg_table = nullptr);
   hipEvent_t next_event();
 
   int device_ = 0;

@@ -216,14 +216,15 @@ bool Engine::ray_counts(u64 out[4]) {
          hipStreamSynchronize(stream_) == hipSuccess;
 }
 
-hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegment>& chunk, u32 total, bool splat,
-                             bool adj, bool disjoint) {
-  hipError_t e = hipMemcpyAsync(segs_.p, chunk.data(), chunk.size() * sizeof(DevSegment), hipMemcpyHostToDevice, stream_);
+hipError_t Engine::run_chunk(const DevScene* d_scenes, const DevSegment* chunk, u32 n_chunk, u32 total, bool splat,
+                             bool adj, bool disjoint, DevSegment* seg_table) {
+  DevSegment* table = seg_table ? seg_table : (DevSegment*)segs_.p;
+  hipError_t e = hipMemcpyAsync(table, chunk, n_chunk * sizeof(DevSegment), hipMemcpyHostToDevice, stream_);
   if (e != hipSuccess) return e;
   Launch L;
   L.scenes = d_scenes;
-  L.segs = (const DevSegment*)segs_.p;
-  L.n_segs = (u32)chunk.size();
+  L.segs = table;
+  L.n_segs = n_chunk;
   L.pool = std::min(opt.pool, total);
   L.shape = opt.shape;
   L.shadow_shape = opt.shadow_shape;
@@ -318,16 +319,117 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const std::vector<DevSegm
   }
   if (exec) hipGraphExecDestroy(exec);
   if (e != hipSuccess) return e;
-  if (splat && disjoint && chunk.size() > 1) {
+  if (splat && disjoint && n_chunk > 1) {
     e = launch_splat_segments(P, L.segs, L.n_segs, stream_);
     if (e != hipSuccess) return e;
   } else if (splat)
-    for (const DevSegment& s : chunk) {
-      e = launch_splat(P, s, stream_);
+    for (u32 k = 0; k < n_chunk; ++k) {
+      e = launch_splat(P, chunk[k], stream_);
       if (e != hipSuccess) return e;
     }
   timing.chunks++;
   return hipSuccess;
+}
+
+namespace {
+// Hands out 256-byte aligned pieces of one device region (nullptr base: only counts the bytes).
+struct Carver {
+  char* base;
+  size_t cap, used = 0;
+  bool ok = true;
+  template <class T>
+  T* take(size_t n) {
+    size_t bytes = (n * sizeof(T) + 255) & ~(size_t)255;
+    if (used + bytes > cap) {
+      ok = false;
+      return nullptr;
+    }
+    T* p = base ? reinterpret_cast<T*>(base + used) : nullptr;
+    used += bytes;
+    return p;
+  }
+};
+
+// The per-path buffers of an n-slot pool that runs n paths at once, its results, its segment
+// table and an n x channels staging tile, carved from `c` into Q.
+void carve_pool(Carver& c, PathState& Q, u32 n, bool textured, bool ext, u32 channels, DevSegment*& table,
+                float*& staging) {
+  Q.ray_o = c.take<float4>(n);
+  Q.ray_d = c.take<float4>(n);
+  Q.hit = c.take<uint4>(n);
+  Q.thr = c.take<float4>(n);
+  Q.sum = c.take<float4>(n);
+  Q.med = c.take<float4>(n);
+  Q.sel_p = c.take<float4>(n);
+  Q.sel_w0 = c.take<float4>(n);
+  Q.sel_w1 = c.take<float4>(n);
+  Q.sh_o = c.take<float4>(n);
+  Q.sh_d = c.take<float4>(n);
+  Q.sh_c = c.take<float4>(n);
+  Q.ident = c.take<uint4>(n);
+  Q.key = c.take<u64>(n);
+  for (int k = 0; k < 2; ++k) {
+    Q.live[k] = c.take<u32>(n);
+    Q.shadow_queue[k] = c.take<u32>(n);
+    Q.done_queue[k] = c.take<u32>(n);
+  }
+  Q.results = c.take<float4>(n);
+  Q.bsdf_tmp = textured ? c.take<Bsdf>(n) : nullptr;
+  Q.walk_w = ext ? c.take<float4>(n) : nullptr;
+  Q.walk_r = ext ? c.take<float4>(n) : nullptr;
+  table = c.take<DevSegment>(1);
+  staging = c.take<float>((size_t)n * channels);
+}
+}  // namespace
+
+size_t Engine::tile_scratch_bytes(u32 pixels, bool textured, bool ext, u32 channels) {
+  Carver c{nullptr, ~(size_t)0};
+  PathState Q{};
+  DevSegment* table;
+  float* staging;
+  carve_pool(c, Q, pixels, textured, ext, channels, table, staging);
+  return c.used;
+}
+
+hipError_t Engine::render_tile_in(const DevScene* d_scenes, const DevSegment& seg, u32 channels, void* scratch,
+                                  size_t bytes, float* out, size_t out_row_floats, std::string& err) {
+  u32 n = seg.w * seg.h;
+  if (n == 0 || seg.count != 1) {
+    err = "one sample of a non-empty tile";
+    return hipErrorInvalidValue;
+  }
+  Carver c{static_cast<char*>(scratch), bytes};
+  PathState keep = P;
+  DevSegment* table = nullptr;
+  float* staging = nullptr;
+  carve_pool(c, P, n, textured, ext, channels, table, staging);
+  if (!c.ok) {
+    P = keep;
+    err = "the tile's path state does not fit the scratch region";
+    return hipErrorInvalidValue;
+  }
+  EngineOptions keep_opt = opt;
+  bool keep_profiling = profiling;
+  // Every path in flight at once, launched eagerly (a captured graph would allocate).
+  opt.pool = n;
+  opt.graph = false;
+  opt.per_pass = false;
+  profiling = false;
+  timing = Timing{};
+  DevSegment s = seg;
+  s.base = 0;
+  hipError_t e = run_chunk(d_scenes, &s, 1, n, false, false, false, table);
+  if (e == hipSuccess) e = launch_tile_out(P, staging, seg.w, seg.h, channels, stream_);
+  if (e == hipSuccess)
+    e = hipMemcpy2DAsync(out, out_row_floats * 4, staging, (size_t)seg.w * channels * 4, (size_t)seg.w * channels * 4,
+                         seg.h, hipMemcpyDeviceToHost, stream_);
+  hipError_t w = hipStreamSynchronize(stream_);
+  if (e == hipSuccess) e = w;
+  P = keep;
+  opt = keep_opt;
+  profiling = keep_profiling;
+  if (e != hipSuccess) err = std::string("render: ") + hipGetErrorString(e);
+  return e;
 }
 
 bool Engine::plan(const std::vector<DevSegment>& segs, std::vector<std::vector<DevSegment>>& chunks,
@@ -396,7 +498,8 @@ hipError_t Engine::run(const DevScene* d_scenes, const std::vector<DevSegment>& 
     return hipErrorOutOfMemory;
   hipEventRecord(ev_[0], stream_);
   hipError_t e = hipSuccess;
-  for (size_t c = 0; c < chunks.size() && e == hipSuccess; ++c) e = run_chunk(d_scenes, chunks[c], totals[c], splat, false, disjoint);
+  for (size_t c = 0; c < chunks.size() && e == hipSuccess; ++c)
+    e = run_chunk(d_scenes, chunks[c].data(), (u32)chunks[c].size(), totals[c], splat, false, disjoint);
   hipEventRecord(ev_[1], stream_);
   if (e == hipSuccess) e = hipStreamSynchronize(stream_);
   if (e != hipSuccess) {
@@ -463,8 +566,10 @@ hipError_t Engine::adjoint(const DevScene* d_scenes, const std::vector<DevSegmen
   hipEventRecord(ev_[0], stream_);
   hipError_t e = hipSuccess;
   for (size_t c = 0; c < chunks.size() && e == hipSuccess; ++c) {
-    e = run_chunk(d_scenes, chunks[c], totals[c], false, false);  // the forward: radiance per path sample
-    if (e == hipSuccess) e = run_chunk(d_scenes, chunks[c], totals[c], false, true);  // the replay
+    // The forward (radiance per path sample), then the replay.
+    e = run_chunk(d_scenes, chunks[c].data(), (u32)chunks[c].size(), totals[c], false, false);
+    if (e == hipSuccess)
+      e = run_chunk(d_scenes, chunks[c].data(), (u32)chunks[c].size(), totals[c], false, true);
   }
   hipEventRecord(ev_[1], stream_);
   opt.graph = keep_graph;

@@ -91,6 +91,7 @@ _SIGS = {
     "rpx_denoise_timing": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_denoise_counted": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_float] + [ctypes.c_uint32] * 7
                             + [ctypes.c_void_p] * 2),
+    "rpx_render_tile_noalloc": (ctypes.c_uint32, [ctypes.c_uint32] * 6 + [ctypes.c_void_p]),
     "rpx_device_count": (ctypes.c_uint32, []),
 }
 
@@ -262,6 +263,13 @@ class Scene:
         """One sample of every pixel of the tile: (h, w, 3) float32 linear RGB."""
         out = np.zeros((h, w, 3), np.float32)
         _check(library().rt_render_tile(self.handle, x, y, w, h, sample_index, _ptr(out)), "rt_render_tile")
+        return out
+
+    def render_tile_noalloc(self, x, y, w, h, sample_index):
+        """render_tile through the allocation-free entry point (tiles of at most 16384 pixels)."""
+        out = np.zeros((h, w, 3), np.float32)
+        _check(library().rpx_render_tile_noalloc(self.handle, x, y, w, h, sample_index, _ptr(out)),
+               "rpx_render_tile_noalloc")
         return out
 
     def render_tile_rgba(self, x, y, w, h, sample_index):
