@@ -44,11 +44,12 @@ HBM_PEAK_GBS = 8000.0  # MI355X datasheet: 8 TB/s HBM3E
 # geometry/triangle.rs:160-167: v [3][3][4] f32 + id [4] u32 = 160 B).
 NODE_BYTES = 256
 PACKET_BYTES = 160
-# Memory-side bytes of one traversal phase (k_extend + k_shadow) per traced ray, from rocprofv3
-# FETCH_SIZE and WRITE_SIZE runs of this workload (pool 67,108,864, closest-hit shape 2, shadow
-# shape 4; FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0, the factors a 256 MiB device copy gives):
-# 1.376e9 + 0.862e9 bytes per dispatch pair over 5,695,387 closest-hit and 938,280 shadow rays.
-TRAFFIC_BYTES_PER_RAY = 337.5
+# Memory-side bytes of one k_extend dispatch per closest-hit ray: not measured in this run but taken
+# from rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this workload (FETCH_SIZE scaled by 2.0,
+# WRITE_SIZE by 1.0, the factors a 256 MiB device copy gives), recorded in TRAFFIC_SOURCE.
+EXTEND_TRAFFIC_BYTES_PER_RAY = 242.0
+TRAFFIC_SOURCE = ("profile-derived constant per closest-hit ray, not measured in this run: "
+                  "profiles/hbm_traffic_large_render_1024_64spp.md")
 
 
 def log(*a):
@@ -94,7 +95,10 @@ def cpu_baseline(rsb, spp, budget_s=15.0):
 
 def mixed_queue(native, seed=0, jobs=64):
     """The mixed render queue: every job loaded as its own scene; the persistent pool (one queue,
-    tuned) against the plain per-job tracer (each scene alone, one sample pass at a time)."""
+    tuned; and the same with captured graph launches) against the plain per-job tracer: each scene
+    rendered alone through the same library at the queue's tuned options (whole sample range, the
+    pool allocated by an untimed render first), and, as a second figure, alone in drained sample
+    passes. Every job's film from the queue is compared bit for bit with the same scene alone."""
     t0 = time.perf_counter()
     scenes, spps, paths = [], [], 0
     for j in range(jobs):
@@ -107,6 +111,7 @@ def mixed_queue(native, seed=0, jobs=64):
     q = native.Queue()
     ids = [q.add(sc, 0, spp) for sc, spp in zip(scenes, spps)]
     tune = q.autotune()
+    opts = dict(pool=tune["pool"], shape=tune["shape"], shadow_shape=tune["shadow_shape"])
     log("bench: queue tuner chose pool %d shapes %d / %d" % (tune["pool"], tune["shape"], tune["shadow_shape"]))
     q.run()  # warm
     q.reset()
@@ -116,30 +121,52 @@ def mixed_queue(native, seed=0, jobs=64):
     t_persistent = time.perf_counter() - t1
     rays = q.stats()["rays"] - st0["rays"]
     log("bench: persistent queue %.3f s" % t_persistent)
-    # The plain per-job tracer: each scene alone, sample passes drained one by one.
-    for sc in scenes:
-        sc.set_options(per_pass=True)
-    scenes[0].reset_film()
-    scenes[0].render_samples(0, 1)  # warm
-    t_plain = 0.0
-    for sc, spp in zip(scenes, spps):
+    films = [q.film(i) for i in ids]
+    q.set_options(graph=True, **opts)
+    q.reset()
+    q.run()  # warm (capture, instantiation)
+    q.reset()
+    t1 = time.perf_counter()
+    q.run()
+    t_graph = time.perf_counter() - t1
+    graph_same = all(all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(q.film(i), f))
+                     for i, f in zip(ids, films))
+    log("bench: persistent queue with graph launches %.3f s" % t_graph)
+    q.close()
+    # The plain per-job tracer: each scene alone, whole sample range, the queue's options; then the
+    # same scene in drained sample passes. Each scene is closed after its job (its pool is freed).
+    t_plain = t_pass = 0.0
+    identical = True
+    for sc, spp, film in zip(scenes, spps, films):
+        sc.set_options(**opts)
+        sc.reset_film()
+        sc.render_samples(0, spp)  # warm: the pool of this job's size
         sc.reset_film()
         t2 = time.perf_counter()
         sc.render_samples(0, spp)
         t_plain += time.perf_counter() - t2
-    log("bench: plain per-job %.3f s" % t_plain)
-    # Every job's film from the queue equals the same scene rendered alone, bit for bit.
-    identical = all(all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(q.film(i), sc.film()))
-                    for i, sc in zip(ids, scenes))
-    q.close()
-    for sc in scenes:
+        identical = identical and all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
+                                      for a, b in zip(film, sc.film()))
+        sc.set_options(per_pass=True)
+        sc.reset_film()
+        t2 = time.perf_counter()
+        sc.render_samples(0, spp)
+        t_pass += time.perf_counter() - t2
         sc.close()
+    log("bench: plain per-job %.3f s (in sample passes %.3f s)" % (t_plain, t_pass))
     return {"jobs": jobs, "paths": paths, "persistent_mpaths_per_s": round(paths / t_persistent / 1e6, 3),
             "persistent_mrays_per_s": round(rays / t_persistent / 1e6, 3),
             "plain_per_job_mpaths_per_s": round(paths / t_plain / 1e6, 3),
-            "speedup": round(t_plain / t_persistent, 3), "persistent_s": round(t_persistent, 3),
-            "plain_s": round(t_plain, 3), "tuned": {"pool": tune["pool"], "shape": tune["shape"],
-                                                      "shadow_shape": tune["shadow_shape"]},
+            "speedup": round(t_plain / t_persistent, 3),
+            "speedup_basis": "each job alone through the same library at the queue's tuned options, whole sample "
+                             "range per call, pool allocated beforehand",
+            "persistent_s": round(t_persistent, 3), "plain_s": round(t_plain, 3),
+            "graph_launches": {"persistent_s": round(t_graph, 3), "mpaths_per_s": round(paths / t_graph / 1e6, 3),
+                               "films_equal_eager": graph_same},
+            "per_pass_baseline": {"plain_s": round(t_pass, 3), "mpaths_per_s": round(paths / t_pass / 1e6, 3),
+                                  "speedup": round(t_pass / t_persistent, 3),
+                                  "what": "each job alone, one drained sample pass per call"},
+            "tuned": {"pool": tune["pool"], "shape": tune["shape"], "shadow_shape": tune["shadow_shape"]},
             "films_bit_exact_vs_plain": identical}
 
 
@@ -384,6 +411,7 @@ def main():
     ap.add_argument("--no-adaptive", action="store_true")
     ap.add_argument("--no-denoise", action="store_true")
     ap.add_argument("--no-media", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--pool", type=int, default=0)
     ap.add_argument("--shape", type=int, default=-1)
     ap.add_argument("--shadow-shape", type=int, default=0)
@@ -447,10 +475,31 @@ def main():
     scene.render_samples(0, spp)
     s_after = scene.stats()
     timing = scene.last_timing()
+    kernels = scene.kernel_times()
     scene.set_profiling(False)
     nearest_prof = s_after["nearest"] - s_before["nearest"]
     shadow_prof = s_after["shadow"] - s_before["shadow"]
-    extend_avg_ms = timing["extend_ms"] / max(timing["extend_launches"], 1)
+    phase_avg_ms = timing["extend_ms"] / max(timing["extend_launches"], 1)
+    extend_avg_ms = kernels["extend_kernel_ms"] / max(kernels["extend_launches"], 1)
+    shadow_avg_ms = kernels["shadow_kernel_ms"] / max(kernels["shadow_launches"], 1)
+
+    # The same step with each group of iterations launched as a captured HIP graph (same options).
+    graph_ms = None
+    if not args.no_graph:
+        scene.set_options(pool=tuned["pool"], shape=tuned["shape"], shadow_shape=tuned["shadow_shape"], graph=True)
+        scene.reset_film()
+        scene.render_samples(0, spp)  # warm (capture and instantiation)
+        gt = []
+        for _ in range(max(args.steps, 1)):
+            scene.reset_film()
+            t1 = time.perf_counter()
+            scene.render_samples(0, spp)
+            gt.append(time.perf_counter() - t1)
+        graph_ms = 1000.0 * sum(gt) / len(gt)
+        scene.set_options(pool=tuned["pool"], shape=tuned["shape"], shadow_shape=tuned["shadow_shape"])
+        scene.reset_film()
+        scene.render_samples(0, spp)
+        log("bench: graph launches %.3f ms per step" % graph_ms)
 
     result = {
         "metric": METRIC,
@@ -476,6 +525,9 @@ def main():
         "stage_ms_per_step": {k: round(timing[k], 2) for k in ("extend_ms", "shade_ms", "shadow_ms", "regen_ms",
                                                                 "drain_ms", "total_ms")},
         "iterations_per_step": timing["iterations"],
+        "graph_launches": None if graph_ms is None else {
+            "ms_per_step": round(graph_ms, 3), "mpaths_per_s": round(paths_per_step / (graph_ms * 1e-3) / 1e6, 3),
+            "note": "each group of 8 pool iterations launched as one captured HIP graph, same pool and shapes"},
         # RSB parse, host BVH build (binned SAH, 4-wide, all host threads) and upload, outside the steps.
         "scene_load_s": round(load_s, 3),
     }
@@ -510,24 +562,34 @@ def main():
         b_ray = (NODE_BYTES * counts["nearest_nodes"] + PACKET_BYTES * counts["nearest_packets"]) / max(
             counts["nearest"], 1)
         b_any = (NODE_BYTES * counts["any_nodes"] + PACKET_BYTES * counts["any_packets"]) / max(counts["any"], 1)
-        launches = max(timing["extend_launches"], 1)
-        per_launch = (b_ray * nearest_prof + b_any * shadow_prof) / launches
-        achieved = per_launch / (extend_avg_ms * 1e-3) / 1e9
+        launches = max(kernels["extend_launches"], 1)
+        ext_bytes = b_ray * nearest_prof / launches
+        shadow_bytes = b_any * shadow_prof / max(kernels["shadow_launches"], 1)
+        achieved = ext_bytes / (extend_avg_ms * 1e-3) / 1e9
+        shadow_achieved = shadow_bytes / (shadow_avg_ms * 1e-3) / 1e9
+        phase_achieved = (ext_bytes + shadow_bytes) / (phase_avg_ms * 1e-3) / 1e9
         result["roofline"] = {
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": round(TRAFFIC_BYTES_PER_RAY * (nearest_prof + shadow_prof) / launches),
-            "kernel": "traversal phase: k_extend (closest hit, shape %d) with k_shadow (any hit, shape %d) "
-                      "beside it" % (tuned["shape"], tuned["shadow_shape"]),
+            "traffic": round(EXTEND_TRAFFIC_BYTES_PER_RAY * nearest_prof / launches),
+            "traffic_source": TRAFFIC_SOURCE,
+            "kernel": "k_extend (closest hit, shape %d): algorithmic bytes of its rays per launch over its own "
+                      "average launch time (HIP events on its stream)" % tuned["shape"],
             "bytes_per_ray": round(b_ray, 1),
-            "bytes_per_shadow_ray": round(b_any, 1),
             "closest_hit_rays_per_launch": round(nearest_prof / launches, 1),
-            "shadow_rays_per_launch": round(shadow_prof / launches, 1),
             "extend_avg_ms": round(extend_avg_ms, 4),
-            "extend_launches_per_step": timing["extend_launches"],
+            "extend_launches_per_step": kernels["extend_launches"],
+            "other_kernels": {
+                "k_shadow": {"achieved": round(shadow_achieved, 1), "frac": round(shadow_achieved / HBM_PEAK_GBS, 4),
+                             "bytes_per_ray": round(b_any, 1),
+                             "rays_per_launch": round(shadow_prof / max(kernels["shadow_launches"], 1), 1),
+                             "avg_ms": round(shadow_avg_ms, 4), "stream": "second (beside k_extend)"},
+                "traversal_phase": {"achieved": round(phase_achieved, 1),
+                                    "frac": round(phase_achieved / HBM_PEAK_GBS, 4), "avg_ms": round(phase_avg_ms, 4),
+                                    "what": "k_extend until k_shadow beside it has joined; both kernels' bytes"}},
         }
     if not args.no_denoise:
         result["denoise"] = denoise_bench(scene, spp)

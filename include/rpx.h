@@ -141,6 +141,11 @@ uint32_t rpx_set_profiling(uint32_t handle, uint32_t on);
  * shadow kernel runs beside the closest hit on a second stream: the closest-hit stage spans
  * both traversals, the shadow figure is the shadow kernel's own time within it. */
 uint32_t rpx_last_timing(uint32_t handle, float* out12);
+/* Kernel times of the last render with profiling on (ms summed over its launches): the
+ * closest-hit kernel alone, its launches, the shadow kernel (on the second stream), its launches,
+ * the traversal phase (closest-hit kernel until the shadow kernel beside it has joined), the
+ * media walks, the drain, the shade. */
+uint32_t rpx_last_kernel_ms(uint32_t handle, float* out8);
 /* The GPU-generated energy-compensation tables of the current device (1024, 1024, 17408,
  * 17408 u16 and 32, 32 f32). */
 uint32_t rpx_energy_tables(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_in, uint16_t* clear_out,

@@ -28,16 +28,38 @@ def hipcc():
     return path if os.path.exists(path) else "hipcc"
 
 
-def build(verbose=False):
-    """Compiles librpx.so (always: the call is the build check)."""
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + FLAGS + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", tmp]
+OBJ_DIR = os.path.join(HERE, "_build")
+
+
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building librpx.so")
+    return res.returncode
+
+
+def build(verbose=False, jobs=None):
+    """Compiles librpx.so (always: the call is the build check): each source to an object in
+    parallel, then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    objs, cmds = [], []
+    for src in SOURCES:
+        obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
+        objs.append(obj)
+        cmds.append([hipcc()] + compile_flags + ["-c", os.path.join(CSRC, src), "-o", obj])
+    jobs = jobs or min(len(cmds), max(1, os.cpu_count() or 1), 8)
+    with ThreadPoolExecutor(jobs) as pool:
+        codes = list(pool.map(lambda c: _run(c, verbose), cmds))
+    if any(codes):
+        raise RuntimeError("hipcc failed compiling a source of librpx.so")
+    tmp = LIB + ".tmp"
+    if _run([hipcc(), "--offload-arch=gfx950", "-fPIC", "-shared"] + objs + ["-o", tmp], verbose):
+        raise RuntimeError("hipcc failed linking librpx.so")
     os.replace(tmp, LIB)
     return LIB
 

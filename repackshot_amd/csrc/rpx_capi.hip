@@ -900,6 +900,23 @@ uint32_t rpx_last_timing(uint32_t handle, float* out12) {
   return RPX_OK;
 }
 
+uint32_t rpx_last_kernel_ms(uint32_t handle, float* out8) {
+  if (!out8) return RPX_NULL_POINTER;
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  const Timing& t = h->engine->timing;
+  out8[0] = t.stage_ms[6];
+  out8[1] = (float)t.launches[6];
+  out8[2] = t.stage_ms[2];
+  out8[3] = (float)t.launches[2];
+  out8[4] = t.stage_ms[0];
+  out8[5] = t.stage_ms[5];
+  out8[6] = t.stage_ms[4];
+  out8[7] = t.stage_ms[1];
+  return RPX_OK;
+}
+
 uint32_t rpx_energy_tables(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_in, uint16_t* clear_out,
                            float* albedo_avg, float* schlick_avg) {
   if (rpx_device_count() == 0) return RPX_GPU_MEMORY;

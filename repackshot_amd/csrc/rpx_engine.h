@@ -48,8 +48,10 @@ struct EngineOptions {
 // GPU time of the last run, from events on the engine's stream (profiling on).
 struct Timing {
   float total_ms = 0;
-  float stage_ms[6] = {0, 0, 0, 0, 0, 0};  // extend, shade, shadow, regen, drain, walk
-  u32 launches[6] = {0, 0, 0, 0, 0, 0};
+  // extend (the closest-hit kernel until the shadow kernel beside it has joined), shade, shadow,
+  // regen, drain, walk, and the closest-hit kernel alone
+  float stage_ms[7] = {0, 0, 0, 0, 0, 0, 0};
+  u32 launches[7] = {0, 0, 0, 0, 0, 0, 0};
   u32 iterations = 0, chunks = 0;
 };
 
@@ -104,6 +106,7 @@ class Engine {
 
  private:
   bool ensure_pool(u32 n, std::string& err);
+  void release_pool();  // frees the path pool's buffers (the next run allocates them again)
   bool ensure_adjoint(u32 n, std::string& err);
   bool plan(const std::vector<DevSegment>& segs, std::vector<std::vector<DevSegment>>& chunks, std::vector<u32>& totals,
             std::string& err);
@@ -113,6 +116,7 @@ class Engine {
   hipError_t run_chunk(const DevScene* d_scenes, const DevSegment* chunk, u32 n_chunk, u32 total, bool splat,
                        bool adj = false, bool disjoint = false, DevSegment* seg_table = nullptr);
   hipEvent_t next_event();
+  void gather_marks();  // adds the profiled stages' event spans to `timing`
 
   int device_ = 0;
   hipStream_t stream_ = nullptr;

@@ -274,6 +274,7 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const DevSegment* chunk, 
       // The extend stage's time spans both traversals (closest hit here, any hit beside it).
       mark(0);
       r = launch_extend(P, L, parity);
+      done();  // the closest-hit kernel's own end (stage 0 spans three events: start, kernel, join)
       // Joined even after an error, so that no side work outlives the chunk unawaited.
       hipError_t w = hipStreamWaitEvent(stream_, join_, 0);
       if (r == hipSuccess) r = w;
@@ -507,13 +508,26 @@ hipError_t Engine::run(const DevScene* d_scenes, const std::vector<DevSegment>& 
     return e;
   }
   hipEventElapsedTime(&timing.total_ms, ev_[0], ev_[1]);
+  gather_marks();
+  return hipSuccess;
+}
+
+void Engine::gather_marks() {
   for (auto& m : marks_) {
     float ms = 0;
-    hipEventElapsedTime(&ms, events_[m.second], events_[m.second + 1]);
+    if (m.first == 0) {
+      // The closest-hit stage: its kernel alone, and the span until the shadow kernel beside it
+      // has joined.
+      hipEventElapsedTime(&ms, events_[m.second], events_[m.second + 1]);
+      timing.stage_ms[6] += ms;
+      timing.launches[6]++;
+      hipEventElapsedTime(&ms, events_[m.second], events_[m.second + 2]);
+    } else {
+      hipEventElapsedTime(&ms, events_[m.second], events_[m.second + 1]);
+    }
     timing.stage_ms[m.first] += ms;
     timing.launches[m.first]++;
   }
-  return hipSuccess;
 }
 
 bool Engine::ensure_adjoint(u32 n, std::string& err) {
@@ -581,12 +595,7 @@ hipError_t Engine::adjoint(const DevScene* d_scenes, const std::vector<DevSegmen
     return e;
   }
   hipEventElapsedTime(&timing.total_ms, ev_[0], ev_[1]);
-  for (auto& m : marks_) {
-    float ms = 0;
-    hipEventElapsedTime(&ms, events_[m.second], events_[m.second + 1]);
-    timing.stage_ms[m.first] += ms;
-    timing.launches[m.first]++;
-  }
+  gather_marks();
   return hipSuccess;
 }
 
@@ -597,14 +606,23 @@ hipError_t Engine::autotune(const DevScene* d_scenes, const std::vector<DevSegme
   profiling = false;
   opt.per_pass = false;
   opt.graph = false;
-  ms.assign(N_TUNE_CANDIDATES, 0.0f);
+  opt.chunk = RESULTS_MAX;  // the probe's results stay unsplatted: one chunk
+  ms.assign(N_TUNE_CANDIDATES, 0.0f);  // 0: the candidate did not fit in device memory
   float best = 0.0f;
   EngineOptions chosen = keep;
   hipError_t e = hipSuccess;
-  auto time_it = [&](float& t) {
+  // A candidate whose pool does not fit in device memory is skipped, not fatal.
+  auto time_it = [&](float& t) -> bool {
     e = run(d_scenes, probe, false, err);  // warm (allocations, caches)
     if (e == hipSuccess) e = run(d_scenes, probe, false, err);
+    if (e == hipErrorOutOfMemory) {
+      e = hipSuccess;
+      err.clear();
+      release_pool();
+      return false;
+    }
     t = timing.total_ms;
+    return e == hipSuccess;
   };
   opt.shadow_shape = 0;
   for (int sh = 0; sh < N_SHAPES && e == hipSuccess; ++sh)
@@ -612,8 +630,7 @@ hipError_t Engine::autotune(const DevScene* d_scenes, const std::vector<DevSegme
       opt.shape = sh;
       opt.pool = TUNE_POOLS[p];
       float t = 0.0f;
-      time_it(t);
-      if (e != hipSuccess) break;
+      if (!time_it(t)) continue;
       ms[sh * N_TUNE_POOLS + p] = t;
       if (best == 0.0f || t < best) {
         best = t;
@@ -622,13 +639,16 @@ hipError_t Engine::autotune(const DevScene* d_scenes, const std::vector<DevSegme
         chosen.shadow_shape = 0;
       }
     }
+  if (e == hipSuccess && best == 0.0f) {
+    err = "no tuner candidate fits in device memory";
+    e = hipErrorOutOfMemory;
+  }
   opt.shape = chosen.shape;
   opt.pool = chosen.pool;
   for (int sh = 0; sh < N_SHAPES && e == hipSuccess; ++sh) {
     opt.shadow_shape = sh;
     float t = 0.0f;
-    time_it(t);
-    if (e != hipSuccess) break;
+    if (!time_it(t)) continue;
     ms[N_SHAPES * N_TUNE_POOLS + sh] = t;
     if (t < best) {
       best = t;
@@ -637,11 +657,19 @@ hipError_t Engine::autotune(const DevScene* d_scenes, const std::vector<DevSegme
   }
   profiling = keep_profiling;
   opt = keep;
+  // The probes grew the pool to the largest candidate; keep no more than the chosen size.
+  if (capacity_ > chosen.pool) release_pool();
   if (e != hipSuccess) return e;
   opt.shape = chosen.shape;
   opt.shadow_shape = chosen.shadow_shape;
   opt.pool = chosen.pool;
   return hipSuccess;
+}
+
+void Engine::release_pool() {
+  hipStreamSynchronize(stream_);
+  dev_free_all(pool_bufs_);
+  capacity_ = 0;
 }
 
 }  // namespace rpx

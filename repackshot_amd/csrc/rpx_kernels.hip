@@ -2077,8 +2077,9 @@ __global__ void __launch_bounds__(BLK) k_finish(const DevScene* __restrict__ sce
 // ================================================================== splat / tile output
 // Adds samples first .. first + count of pixel p of segment s to its film, in sample order
 // (image.rs:81-94): the sums are carried in registers and stored once, which performs the same f32
-// additions in the same order as adding sample by sample. Results slot base + k * (w * h) + p holds
-// sample first + k of the segment's p-th pixel in tiled order.
+// additions in the same order as adding sample by sample. Results slot base + p * count + k holds
+// sample first + k of the segment's p-th pixel in tiled order (sample-minor: a pixel's samples are
+// contiguous).
 // Adds the samples of pixels p0 .. p0 + T - 1 of segment `s` to the film, each pixel's in sample
 // order (lane t: pixel p0 + t). A pixel's results are contiguous (work ids sample-minor); the block
 // stages SPLAT_CH samples of each of its pixels at a time in LDS with coalesced loads, then each

@@ -91,6 +91,7 @@ _SIGS = {
     "rpx_denoise_timing": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_denoise_counted": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_float] + [ctypes.c_uint32] * 7
                             + [ctypes.c_void_p] * 2),
+    "rpx_last_kernel_ms": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_render_tile_noalloc": (ctypes.c_uint32, [ctypes.c_uint32] * 6 + [ctypes.c_void_p]),
     "rpx_device_count": (ctypes.c_uint32, []),
 }
@@ -264,6 +265,14 @@ class Scene:
         out = np.zeros((h, w, 3), np.float32)
         _check(library().rt_render_tile(self.handle, x, y, w, h, sample_index, _ptr(out)), "rt_render_tile")
         return out
+
+    def kernel_times(self):
+        """Per-kernel GPU ms of the last profiled render (rpx_last_kernel_ms)."""
+        out = np.zeros(8, np.float32)
+        _check(library().rpx_last_kernel_ms(self.handle, _ptr(out)), "rpx_last_kernel_ms")
+        keys = ["extend_kernel_ms", "extend_launches", "shadow_kernel_ms", "shadow_launches", "phase_ms", "walk_ms",
+                "drain_ms", "shade_ms"]
+        return {k: (int(v) if k.endswith("launches") else float(v)) for k, v in zip(keys, out)}
 
     def render_tile_noalloc(self, x, y, w, h, sample_index):
         """render_tile through the allocation-free entry point (tiles of at most 16384 pixels)."""
