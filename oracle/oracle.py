@@ -99,6 +99,12 @@ def lib():
         L.orc_lights_pick.argtypes = [vp, f32, vp, vp]
         L.orc_lights_pick_at.argtypes = [vp, vp, f32, vp, vp, vp]
         L.orc_set_techniques.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.orc_lights_sample.restype = ctypes.c_int
+        L.orc_lights_sample.argtypes = [vp, vp, f32, f32, f32, vp]
+        L.orc_lights_mis_probe.argtypes = [vp, vp, u32, vp]
+        L.orc_lights_escaped.argtypes = [vp, vp, vp]
+        L.orc_lights_emitter_weight.restype = f32
+        L.orc_lights_emitter_weight.argtypes = [vp, vp, u32, f32, f32, f32]
         L.orc_soup.restype = u32
         L.orc_soup.argtypes = [vp, u32, vp]
         L.orc_random_rays.restype = ctypes.c_int
@@ -261,6 +267,34 @@ class OracleScene:
         q = np.zeros(1, np.float32)
         lib().orc_lights_pick_at(self.h, _p(np.asarray(point, np.float32)), u, _p(i), _p(r), _p(q))
         return int(i[0]), float(r[0]), float(q[0])
+
+    def light_sample(self, point, u_select, u0, u1):
+        """One light sample at `point` (light.rs:1145-1315), or None: a dict of wi, distance,
+        radiance, pdf, delta, glossy, entry, kind (0 area, 1 sun, 2 sphere, 3 mesh, 4 environment)."""
+        out = np.zeros(12, np.float32)
+        if not lib().orc_lights_sample(self.h, _p(np.asarray(point, np.float32)), u_select, u0, u1, _p(out)):
+            return None
+        return {"wi": out[0:3].copy(), "distance": float(out[3]), "radiance": out[4:7].copy(), "pdf": float(out[7]),
+                "delta": bool(out[8]), "glossy": bool(out[9]), "entry": int(out[10]), "kind": int(out[11])}
+
+    def light_escaped(self, direction):
+        """What a camera ray leaving along `direction` sees of the sky (light.rs:1502-1520)."""
+        out = np.zeros(3, np.float32)
+        lib().orc_lights_escaped(self.h, _p(np.asarray(direction, np.float32)), _p(out))
+        return out
+
+    def emitter_weight(self, point, instance, bsdf_pdf, t, cos):
+        """The MIS weight of an emitter found by a BSDF-sampled ray (light.rs:1537-1551)."""
+        return float(lib().orc_lights_emitter_weight(self.h, _p(np.asarray(point, np.float32)), instance, bsdf_pdf,
+                                                     t, cos))
+
+    def light_mis_probe(self, tuples):
+        """(kind, weight) per (point xyz, u_select, u0, u1): the MIS weight the BSDF side gives the
+        light a sample found, at the sample's own density (kind -1: skipped)."""
+        t = np.ascontiguousarray(tuples, np.float32).reshape(-1, 6)
+        out = np.zeros((t.shape[0], 2), np.float32)
+        lib().orc_lights_mis_probe(self.h, _p(t), t.shape[0], _p(out))
+        return out[:, 0].astype(np.int32), out[:, 1].copy()
 
     def set_techniques(self, techniques, per_point=None):
         """0 MIS, 1 lights only, 2 BSDF only (the reference tests' switch); per_point forces the

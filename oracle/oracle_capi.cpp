@@ -300,6 +300,99 @@ void orc_lights_pick_at(void* p, const float* point, float u, uint32_t* index, f
   *index = (uint32_t)i;
 }
 
+// One light sample at `point` with the selection there (Lights::sample, light.rs:1145-1315):
+// returns 0 when it gives none; out: wi xyz, distance, radiance rgb, pdf, delta, glossy, the entry
+// picked, its kind (0 area, 1 sun, 2 sphere, 3 mesh, 4 environment).
+int orc_lights_sample(void* p, const float* point, float u_select, float u0, float u1, float* out) {
+  const Lights& l = static_cast<Handle*>(p)->prep.lights;
+  Vec3 x(point[0], point[1], point[2]);
+  Selection sel = l.select(x);
+  LightSample s;
+  if (l.entries.empty() || !l.sample_selected(sel, u_select, u0, u1, s)) return 0;
+  size_t index = 0;
+  float rest, prob;
+  if (sel.by_flux) l.pick_static(u_select, index, rest);
+  else l.pick_at(sel, u_select, index, rest, prob);
+  float v[12] = {s.wi.x, s.wi.y, s.wi.z, s.distance, s.radiance.r, s.radiance.g, s.radiance.b, s.pdf,
+                 s.delta ? 1.0f : 0.0f, s.glossy ? 1.0f : 0.0f, (float)index, (float)l.entries[index].kind};
+  std::memcpy(out, v, sizeof(v));
+  return 1;
+}
+
+// Radiance a camera ray leaving the scene along `dir` sees of the sky (Lights::escaped without a
+// BSDF: the environment and the suns flagged camera, light.rs:1502-1520).
+void orc_lights_escaped(void* p, const float* dir, float* out3) {
+  const Lights& l = static_cast<Handle*>(p)->prep.lights;
+  Rgb r = l.escaped(Vec3(dir[0], dir[1], dir[2]), false, 0.0f, nullptr);
+  out3[0] = r.r;
+  out3[1] = r.g;
+  out3[2] = r.b;
+}
+
+// Lights::emitter_weight of `instance` for a BSDF-sampled ray of density bsdf_pdf that left
+// `point` (its selection there) and met the instance at distance t with `cos` (light.rs:1537-1551).
+float orc_lights_emitter_weight(void* p, const float* point, uint32_t instance, float bsdf_pdf, float t, float cos) {
+  const Lights& l = static_cast<Handle*>(p)->prep.lights;
+  Selection sel = l.select(Vec3(point[0], point[1], point[2]));
+  return l.emitter_weight(instance, bsdf_pdf, sel, t, cos);
+}
+
+// What the BSDF side gives a light that next-event estimation sampled (light.rs:2396-2470): for
+// each of n tuples (point xyz, u_select, u0, u1), a light sample at the point and, when it is not
+// a delta, the MIS weight of the same light found by a BSDF-sampled ray along the sample's
+// direction whose density is the sample's (crossed for areas and spheres crossed alone, the sky
+// beyond the environment for a sun, the environment's own, emitter_weight for the mesh of
+// instance 0). out per tuple: the entry's kind (-1: none, a delta or crossing several lights) and
+// the weight, which the power heuristic makes 1/2.
+void orc_lights_mis_probe(void* p, const float* tuples, uint32_t n, float* out) {
+  const Lights& l = static_cast<Handle*>(p)->prep.lights;
+  for (uint32_t k = 0; k < n; ++k) {
+    const float* t = tuples + 6 * (size_t)k;
+    out[2 * k] = -1.0f;
+    out[2 * k + 1] = 0.0f;
+    Vec3 x(t[0], t[1], t[2]);
+    Selection sel = l.select(x);
+    LightSample s;
+    if (l.entries.empty() || !l.sample_selected(sel, t[3], t[4], t[5], s) || s.delta) continue;
+    size_t index = 0;
+    float rest, prob;
+    if (sel.by_flux) l.pick_static(t[3], index, rest);
+    else l.pick_at(sel, t[3], index, rest, prob);
+    Ray ray(x, s.wi);
+    int crossings = 0;
+    for (const AreaLight& a : l.areas) {
+      float th, c;
+      if (a.intersect(ray, INF, th, c)) ++crossings;
+    }
+    for (const SphereLight& sp : l.spheres) {
+      float th;
+      if (sp.intersect(ray, INF, th)) ++crossings;
+    }
+    const LightRef ref = l.entries[index];
+    Rgb env = Rgb::splat(l.env.value(s.wi.z));
+    Rgb env_part = l.has_env_entry ? env * l.found_weight(s.pdf, l.pmf(l.env_entry, sel) * l.env.pdf(s.wi)) : env;
+    float w;
+    if ((ref.kind == REF_AREA || ref.kind == REF_SPHERE) && crossings != 1) continue;
+    if (ref.kind == REF_AREA) {
+      const AreaLight& a = l.areas[ref.index];
+      float cos = -s.wi.dot(a.frame.z);
+      w = l.crossed_from(ray, INF, true, s.pdf, &sel, 0.0f).g / a.radiance_at(cos).g;
+    } else if (ref.kind == REF_SPHERE) {
+      const SphereLight& sp = l.spheres[ref.index];
+      w = l.crossed_from(ray, INF, true, s.pdf, &sel, 0.0f).g / (sp.radiance.g * sp.mask(x));
+    } else if (ref.kind == REF_SUN) {
+      w = (l.escaped(s.wi, true, s.pdf, &sel).g - env_part.g) / l.suns[ref.index].radiance.g;
+    } else if (ref.kind == REF_ENV) {
+      w = env_part.g / env.g;
+    } else {
+      const MeshEmitter& m = l.meshes[ref.index];
+      w = l.emitter_weight(0, s.pdf, sel, s.distance, s.wi.dot(m.normals[0]));
+    }
+    out[2 * k] = (float)ref.kind;
+    out[2 * k + 1] = w;
+  }
+}
+
 // The reference tests' switch (light.rs:550-558, integrator/tests.rs:2574-2583): techniques
 // 0 MIS, 1 lights only, 2 BSDF only; per_point 0 or 1 sets the per-point selection, other values
 // keep it.

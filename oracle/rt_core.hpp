@@ -364,6 +364,16 @@ inline Vec3 uniform_cone(float u0, float u1, float one_minus_cos_max) {
   return Vec3(sin * cp, sin * sp, cos);
 }
 
+// warp.rs:57-62: barycentrics of a point uniform on a triangle
+inline void uniform_triangle(float u0, float u1, float w[3]) {
+  float su = std::sqrt(u0);
+  float b0 = 1.0f - su;
+  float b1 = u1 * su;
+  w[0] = b0;
+  w[1] = b1;
+  w[2] = 1.0f - b0 - b1;
+}
+
 // ---------------------------------------------------------------- sampler (sampler.rs)
 constexpr u32 SOBOL_DIMENSIONS = 64;
 constexpr unsigned char LAYOUT[64] = {

@@ -264,7 +264,6 @@ bool scene_from_rsb(const uint8_t* bytes, size_t len, Scene& scene, std::string&
     if (m.flags & MATERIAL_FLAG_THIN) { err = "thin walls are not supported"; return false; }
     if (m.abbe > 0.0f) { err = "dispersion is not supported"; return false; }
     if (m.bevel_mm > 0.0f) { err = "edge bevels are not supported"; return false; }
-    if (m.is_emissive() && (m.flags & MATERIAL_FLAG_LIGHT)) { err = "emissive mesh lights are not supported"; return false; }
   }
   // A scene with a scattering medium and a thin wall (integrator/mod.rs:289-299: smooth, wholly clear
   // glass without a medium of its own), which shadow rays from the medium's exits would cross, is
@@ -279,7 +278,7 @@ bool scene_from_rsb(const uint8_t* bytes, size_t len, Scene& scene, std::string&
   }
   if (any_scatter && any_wall) { err = "thin walls beside a scattering medium are not supported"; return false; }
   for (const Light& l : scene.lights)
-    if (l.kind == LIGHT_SUN || l.kind == LIGHT_IES) { err = "sun and IES lights are not supported"; return false; }
+    if (l.kind == LIGHT_IES) { err = "IES lights are not supported"; return false; }
   std::vector<Mesh> built(meshes.size());
   for (size_t i = 0; i < meshes.size(); ++i) {
     if (meshes[i].material >= scene.materials.size()) { err = "material_id does not refer to a MATL section"; return false; }
@@ -538,8 +537,35 @@ bool Lights::build(const Scene& scene, std::string& err) {
       sp.near2 = fmax_r(r_m * r_m, MIN_DISTANCE2_M2);
       float toward = 0.05f + 0.95f * sp.mask(centre);
       weights.push_back(flux * toward * reach(light.position, false, Vec3(0, 0, 0)));
-      entries.push_back(-2 - (int)spheres.size());
+      entries.push_back(LightRef{REF_SPHERE, (u32)spheres.size()});
       spheres.push_back(sp);
+      continue;
+    }
+    if (light.kind == LIGHT_SUN) {
+      // light.rs:755-791
+      Vec3 to_sun = -light.direction.normalize();
+      if (to_sun.length_squared() < 0.5f) continue;
+      float half = fmin_r(0.5f * fmax_r(light.angle_deg, 0.0f) * DEG_TO_RAD, FRAC_PI_2);
+      Rgb color(fmax_r(light.color.r, 0.0f), fmax_r(light.color.g, 0.0f), fmax_r(light.color.b, 0.0f));
+      Rgb irradiance = color * power;
+      if (!positive(irradiance.luminance())) continue;
+      SunLight sun;
+      sun.delta = half < 1.0e-5f;
+      float sh = fsin(0.5f * half);
+      sun.one_minus_cos_max = 2.0f * sh * sh;
+      float sin_max = fsin(half);
+      sun.radiance = sun.delta ? irradiance : irradiance * (1.0f / (PI * sin_max * sin_max));
+      weights.push_back(PI * irradiance.luminance());
+      entries.push_back(LightRef{REF_SUN, (u32)suns.size()});
+      sun.frame = Frame::from_normal(to_sun);
+      sun.cos_max = 1.0f - sun.one_minus_cos_max;
+      sun.solid_angle = TAU * sun.one_minus_cos_max;
+      sun.glossy = glossy;
+      sun.camera = camera;
+      sun.shadow = shadow;
+      sun.entry = entry;
+      sun.base = irradiance.luminance();
+      suns.push_back(sun);
       continue;
     }
     if (light.kind != LIGHT_RECT && light.kind != LIGHT_DISK) { err = "unsupported light kind"; return false; }
@@ -563,7 +589,7 @@ bool Lights::build(const Scene& scene, std::string& err) {
     float flux = color.luminance() * power * 2.0f / (a.exponent + 2.0f);
     if (!positive(flux)) continue;
     weights.push_back(flux * reach(light.position, true, a.frame.z));
-    entries.push_back((int)areas.size());
+    entries.push_back(LightRef{REF_AREA, (u32)areas.size()});
     a.centre = light.position;
     a.glossy = glossy;
     a.camera = camera;
@@ -571,6 +597,76 @@ bool Lights::build(const Scene& scene, std::string& err) {
     a.entry = entry;
     a.base = a.radiance.luminance() * a.area * MM2_TO_M2;
     areas.push_back(a);
+  }
+  // light.rs:866-953: emissive instances whose material is flagged a light.
+  const Geometry& geo = scene.geometry;
+  meshes.clear();
+  instance_emitter.assign(geo.instances.size(), -1);
+  for (size_t i = 0; i < geo.instances.size(); ++i) {
+    const Instance& inst = geo.instances[i];
+    if (inst.material_id >= scene.materials.size()) continue;
+    const Material& mat = scene.materials[inst.material_id];
+    Rgb radiance = mat.radiance();
+    if ((mat.flags & MATERIAL_FLAG_LIGHT) == 0 || !positive(radiance.luminance())) continue;
+    if (inst.mesh >= geo.meshes.size()) continue;
+    Transform inv;
+    if (!inst.transform.inverse(inv)) continue;
+    const Mesh& mesh = geo.meshes[inst.mesh];
+    MeshEmitter m;
+    std::vector<float> tri_areas;
+    Vec3 centroid(0.0f, 0.0f, 0.0f);
+    for (size_t t = 0; t < mesh.triangle_count(); ++t) {
+      Vec3 p0, p1, p2;
+      mesh.triangle(t, p0, p1, p2);
+      Vec3 w0 = inst.transform.point(p0), w1 = inst.transform.point(p1), w2 = inst.transform.point(p2);
+      Vec3 cross = (w1 - w0).cross(w2 - w0);
+      float area = 0.5f * cross.length();
+      if (!(area > 0.0f && std::isfinite(area))) continue;
+      centroid = centroid + (w0 + w1 + w2) * (area / 3.0f);
+      m.a.push_back(w0);
+      m.b.push_back(w1);
+      m.c.push_back(w2);
+      m.normals.push_back(cross.normalize());
+      tri_areas.push_back(area);
+    }
+    float area = -0.0f;
+    for (float x : tri_areas) area = area + x;
+    m.cdf = build_cdf(tri_areas);
+    if (m.cdf.empty()) continue;
+    m.two_sided = (inst.flags & INSTANCE_ONE_SIDED) == 0;
+    float sides = m.two_sided ? 2.0f : 1.0f;
+    float flux = radiance.luminance() * PI * area * MM2_TO_M2 * sides;
+    m.centroid = centroid / area;
+    float bound = 0.0f;
+    for (size_t t = 0; t < m.a.size(); ++t) {
+      bound = fmax_r(bound, (m.a[t] - m.centroid).length());
+      bound = fmax_r(bound, (m.b[t] - m.centroid).length());
+      bound = fmax_r(bound, (m.c[t] - m.centroid).length());
+    }
+    m.bound = bound;
+    Vec3 mean(0.0f, 0.0f, 0.0f);
+    for (size_t t = 0; t < m.normals.size(); ++t) mean = mean + m.normals[t] * tri_areas[t];
+    float length = mean.length();
+    if (m.two_sided || length <= 1.0e-3f * area) {
+      m.axis = Vec3(0.0f, 0.0f, 1.0f);
+      m.cos_o = -1.0f;
+      m.sin_o = 0.0f;
+    } else {
+      m.axis = mean / length;
+      float cos_o = 1.0f;
+      for (const Vec3& n : m.normals) cos_o = fmin_r(cos_o, n.dot(m.axis));
+      cos_o = clampf(cos_o, -1.0f, 1.0f);
+      m.cos_o = cos_o;
+      m.sin_o = std::sqrt(fmax_r(1.0f - cos_o * cos_o, 0.0f));
+    }
+    weights.push_back(flux * reach(m.centroid, false, Vec3(0, 0, 0)));
+    m.entry = (u32)entries.size();
+    entries.push_back(LightRef{REF_MESH, (u32)meshes.size()});
+    instance_emitter[i] = (int)meshes.size();
+    m.area = area;
+    m.radiance = radiance;
+    m.base = radiance.luminance() * area * MM2_TO_M2;
+    meshes.push_back(m);
   }
   const Environment& e = scene.environment;
   float strength = fmax_r(e.strength, 0.0f);
@@ -582,7 +678,7 @@ bool Lights::build(const Scene& scene, std::string& err) {
     weights.push_back(4.0f * PI * PI * mean);
     env_entry = (u32)entries.size();
     has_env_entry = true;
-    entries.push_back(-1);
+    entries.push_back(LightRef{REF_ENV, 0});
   }
   cdf = build_cdf(weights);
   if (cdf.empty()) {
@@ -596,9 +692,30 @@ bool Lights::build(const Scene& scene, std::string& err) {
 }
 
 float Lights::weight(size_t entry, Vec3 x) const {
-  int a = entries[entry];
-  if (a <= -2) return spheres[-2 - a].weight(x);
-  return a < 0 ? env_base : areas[a].weight(x);
+  const LightRef& r = entries[entry];
+  switch (r.kind) {
+    case REF_AREA: return areas[r.index].weight(x);
+    case REF_SUN: return suns[r.index].base;
+    case REF_SPHERE: return spheres[r.index].weight(x);
+    case REF_MESH: return meshes[r.index].weight(x);
+    default: return env_base;
+  }
+}
+
+// light.rs:465-481
+float MeshEmitter::weight(Vec3 x) const {
+  Vec3 v = x - centroid;
+  float d2 = v.length_squared();
+  float orientation = cos_o <= -1.0f ? 1.0f : orientation_bound(v, d2, axis, cos_o, sin_o, bound);
+  if (orientation <= 0.0f) return 0.0f;
+  return base * orientation / (fmax_r(d2, area) * MM2_TO_M2);
+}
+
+float Lights::emitter_weight(u32 instance, float bsdf_pdf, const Selection& from, float t, float cos) const {
+  if (!is_mesh_light(instance)) return 1.0f;
+  const MeshEmitter& m = meshes[instance_emitter[instance]];
+  float light_pdf = pmf(m.entry, from) * t * t / (m.area * fmax_r(std::fabs(cos), 1.0e-7f));
+  return found_weight(bsdf_pdf, light_pdf);
 }
 
 Selection Lights::select(Vec3 x) const {
@@ -681,13 +798,54 @@ bool Lights::sample_selected(const Selection& from, float u_select, float u0, fl
     if (!usable_total(from.total)) return false;
     pick_at(from, u_select, index, rest, p);
   }
-  (void)rest;
   if (!positive(p)) return false;
   LightSample s;
-  int a = entries[index];
-  if (a <= -2) {
+  const LightRef ref = entries[index];
+  if (ref.kind == REF_SUN) {
+    // light.rs:1197-1223
+    const SunLight& sun = suns[ref.index];
+    if (sun.delta) {
+      s.wi = sun.frame.z;
+      s.pdf = p;
+    } else {
+      Vec3 local = uniform_cone(u0, u1, sun.one_minus_cos_max);
+      s.wi = sun.frame.to_world(local).normalize();
+      s.pdf = p / sun.solid_angle;
+    }
+    s.distance = INF;
+    s.radiance = sun.radiance;
+    s.delta = sun.delta;
+    s.glossy = sun.glossy;
+    s.photon = sun.glossy;
+    s.shadow = sun.shadow;
+  } else if (ref.kind == REF_MESH) {
+    // light.rs:1270-1297: the triangle by area with the selection's remainder, a point on it.
+    const MeshEmitter& m = meshes[ref.index];
+    size_t t;
+    float unused;
+    pick(m.cdf, rest, t, unused);
+    float w[3];
+    uniform_triangle(u0, u1, w);
+    Vec3 pt = m.a[t] * w[0] + m.b[t] * w[1] + m.c[t] * w[2];
+    Vec3 d = pt - x;
+    float dist2 = d.length_squared();
+    float distance = std::sqrt(dist2);
+    Vec3 wi = d / distance;
+    float facing = -wi.dot(m.normals[t]);
+    if (!positive(distance) || (!m.two_sided && facing <= 0.0f)) return false;
+    float cos = std::fabs(facing);
+    if (!positive(cos)) return false;
+    s.wi = wi;
+    s.distance = distance;
+    s.radiance = m.radiance;
+    s.pdf = p * dist2 / (m.area * cos);
+    s.delta = false;
+    s.glossy = true;
+    s.photon = true;
+    s.shadow = true;
+  } else if (ref.kind == REF_SPHERE) {
     // light.rs:1224-1268
-    const SphereLight& light = spheres[-2 - a];
+    const SphereLight& light = spheres[ref.index];
     float mask = light.mask(x);
     Vec3 d = light.centre - x;
     float dist2 = d.length_squared();
@@ -717,8 +875,8 @@ bool Lights::sample_selected(const Selection& from, float u_select, float u0, fl
     s.glossy = light.glossy;
     s.photon = false;
     s.shadow = light.shadow;
-  } else if (a >= 0) {
-    const AreaLight& light = areas[a];
+  } else if (ref.kind == REF_AREA) {
+    const AreaLight& light = areas[ref.index];
     float x0, y0;
     if (light.rect) {
       x0 = (2.0f * u0 - 1.0f) * light.half_x;
@@ -812,11 +970,19 @@ bool Lights::camera_sees_panel(const Ray& ray, float t_max) const {
   return false;
 }
 
+// light.rs:1502-1520: the environment, then the sun discs (a camera ray sees the suns flagged
+// camera, a BSDF-sampled ray those flagged glossy, weighted against next-event estimation).
 Rgb Lights::escaped(Vec3 dir, bool has_bsdf, float bsdf_pdf, const Selection* from) const {
-  Rgb env_rad = Rgb::splat(env.value(dir.z));
-  if (!has_bsdf) return env_rad;
-  if (has_env_entry) return env_rad * found_weight(bsdf_pdf, pmf(env_entry, *from) * env.pdf(dir));
-  return env_rad;
+  Rgb sum = Rgb::splat(env.value(dir.z));
+  if (has_bsdf && has_env_entry) sum = sum * found_weight(bsdf_pdf, pmf(env_entry, *from) * env.pdf(dir));
+  for (const SunLight& sun : suns) {
+    bool visible = has_bsdf ? sun.glossy : sun.camera;
+    if (visible && sun.contains(dir)) {
+      float weight = has_bsdf ? found_weight(bsdf_pdf, pmf(sun.entry, *from) / sun.solid_angle) : 1.0f;
+      sum += sun.radiance * weight;
+    }
+  }
+  return sum;
 }
 
 // ------------------------------------------------------------------ integrator/mod.rs:237-864
@@ -1257,8 +1423,14 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
     if (material && material->is_emissive()) {
       bool one_sided = (geo.instances[s_instance].flags & INSTANCE_ONE_SIDED) != 0;
       if (s_front || !one_sided) {
-        // Emitters that are not sampled as lights have an MIS weight of one.
-        prep.add(sum, throughput * material->radiance() * 1.0f, bounce >= 2);
+        // integrator/mod.rs:1548-1568: a BSDF-sampled ray weighs an emitter sampled as a light
+        // against next-event estimation; others have a weight of one.
+        float weight = 1.0f;
+        if (has_pdf) {
+          float cos = s_geometric.dot(ray.dir);
+          weight = lights.emitter_weight(s_instance, bsdf_pdf, sampled_at, offset + t_geometry, cos);
+        }
+        prep.add(sum, throughput * material->radiance() * weight, bounce >= 2);
       }
       break;
     }

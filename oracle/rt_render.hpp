@@ -14,6 +14,8 @@
 //   integrator/medium.rs:99-360  scattering media: the clear-solid test, coefficients, the
 //                            Henyey-Greenstein phase function and its sampling, the random walk
 //   light.rs:364-420, 793-863, 1224-1268, 1386-1425  sphere, point and spot lights
+//   light.rs:298-321, 440-481, 755-791, 866-953, 1197-1223, 1270-1297, 1502-1551  suns and
+//                            emissive meshes sampled as lights
 // Scenes using anything outside that subset are refused at load with a message.
 // Pinning: the film hashes of the reference's tests pin everything but the media and the sphere
 // lights, which the reference pins by closed forms only (furnace, Chandrasekhar, Beer-Lambert,
@@ -136,6 +138,37 @@ struct SphereLight {
   float weight(Vec3 x) const;
 };
 
+// A sun (kind 2, light.rs:298-321, 755-791): a disc of angular diameter angle_deg around the
+// direction towards it (frame.z), sampled uniformly inside its cone; below 1e-5 rad of half angle
+// a delta direction whose radiance is the irradiance.
+struct SunLight {
+  Frame frame;
+  float cos_max, one_minus_cos_max, solid_angle;
+  Rgb radiance;
+  bool delta, glossy, camera, shadow;
+  u32 entry;
+  float base;
+  bool contains(Vec3 dir) const { return !delta && dir.dot(frame.z) >= cos_max; }
+};
+
+// An emissive instance sampled as an area light (light.rs:440-481, 866-953): its world-space
+// triangles, picked by area, a point uniform on the triangle.
+struct MeshEmitter {
+  std::vector<Vec3> a, b, c;  // corners of each triangle
+  std::vector<Vec3> normals;  // unit normal of each (the winding's front)
+  std::vector<float> cdf;     // cumulative area fractions
+  float area;
+  Rgb radiance;
+  bool two_sided;
+  u32 entry;
+  Vec3 centroid;
+  float bound;
+  Vec3 axis;
+  float cos_o, sin_o;
+  float base;
+  float weight(Vec3 x) const;
+};
+
 struct EnvRamp {
   float floor, horizon, zenith, lower_mass, total_mass;
   float value(float z) const;
@@ -160,15 +193,23 @@ struct LightSample {
 };
 
 constexpr int TECH_MIS = 0, TECH_LIGHTS_ONLY = 1, TECH_BSDF_ONLY = 2;
+enum RefKind : u32 { REF_AREA, REF_SUN, REF_SPHERE, REF_MESH, REF_ENV };
+struct LightRef {
+  RefKind kind;
+  u32 index;
+};
+
 struct Lights {
   std::vector<AreaLight> areas;
+  std::vector<SunLight> suns;
   std::vector<SphereLight> spheres;
+  std::vector<MeshEmitter> meshes;
+  std::vector<int> instance_emitter;  // per instance: its mesh emitter, or -1
   EnvRamp env;
   bool has_env_entry = false;
   u32 env_entry = 0;
   float env_base = 0;
-  // entry -> area index (>= 0), the environment (-1), or sphere k (-2 - k)
-  std::vector<int> entries;
+  std::vector<LightRef> entries;
   std::vector<float> cdf, flux_pmf;
   bool per_point = false;
   // Which techniques may find a light (the reference tests' switch, light.rs:550-558): both,
@@ -188,6 +229,13 @@ struct Lights {
   bool sample_selected(const Selection& sel, float u_select, float u0, float u1, LightSample& out) const;
   Rgb crossed_from(const Ray& ray, float t_max, bool has_bsdf, float bsdf_pdf, const Selection* from, float offset) const;
   Rgb escaped(Vec3 dir, bool has_bsdf, float bsdf_pdf, const Selection* from) const;
+  // The MIS weight of the emission a BSDF-sampled ray finds on `instance` at distance t with `cos`
+  // between the ray and the surface normal; 1 when the instance is not a sampled emitter
+  // (light.rs:1537-1551).
+  float emitter_weight(u32 instance, float bsdf_pdf, const Selection& from, float t, float cos) const;
+  bool is_mesh_light(u32 instance) const {
+    return instance < instance_emitter.size() && instance_emitter[instance] >= 0;
+  }
   // An area light or a sphere flagged `camera` in front of everything before t_max (light.rs:1479-1488).
   bool camera_sees_panel(const Ray& ray, float t_max) const;
 };

@@ -117,7 +117,9 @@ bool build_device_scene(Handle* h, std::string& err) {
       !upload(B, flat.materials, d.materials, st, err) || !upload(B, flat.areas, d.areas, st, err) ||
       !upload(B, flat.cdf, d.cdf, st, err) || !upload(B, flat.flux_pmf, d.flux_pmf, st, err) ||
       !upload(B, flat.entries, d.entries, st, err) || !upload(B, flat.spheres, d.spheres, st, err) ||
-      !upload(B, flat.media, d.media, st, err))
+      !upload(B, flat.media, d.media, st, err) || !upload(B, flat.suns, d.suns, st, err) ||
+      !upload(B, flat.emitters, d.emitters, st, err) || !upload(B, flat.emit_tris, d.emit_tris, st, err) ||
+      !upload(B, flat.emit_cdf, d.emit_cdf, st, err) || !upload(B, flat.instance_emitter, d.instance_emitter, st, err))
     return false;
   d.tables = tables->device_view();
   std::vector<DevScene> one(1, d);

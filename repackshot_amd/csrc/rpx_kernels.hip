@@ -991,9 +991,8 @@ __device__ __forceinline__ float invert_linear(float a, float b, float u) {
   if (denom > 0.0f) return clampf(2.0f * c / denom, 0.0f, 1.0f);
   return 0.0f;
 }
-__device__ __forceinline__ float orientation_bound(V3 v, float d2, V3 axis, float bound) {
+__device__ __forceinline__ float orientation_bound(V3 v, float d2, V3 axis, float cos_o, float sin_o, float bound) {
   if (d2 <= bound * bound) return 1.0f;
-  const float cos_o = 1.0f, sin_o = 0.0f;
   float inv_d = 1.0f / sqrt_(d2);
   float sin_u = bound * inv_d;
   float cos_u = sqrt_(fmax_r(1.0f - sin_u * sin_u, 0.0f));
@@ -1008,16 +1007,31 @@ __device__ __forceinline__ float orientation_bound(V3 v, float d2, V3 axis, floa
 __device__ __forceinline__ float area_weight(const DevArea& a, V3 x) {
   V3 v = x - a.centre;
   float d2 = length_sq(v);
-  float orientation = orientation_bound(v, d2, a.frame.z, a.bound);
+  float orientation = orientation_bound(v, d2, a.frame.z, 1.0f, 0.0f, a.bound);
   if (orientation <= 0.0f) return 0.0f;
   return a.base * orientation / (fmax_r(d2, a.area) * 1.0e-6f);
 }
-// EXT: entries may be sphere lights (-2 - k).
+// light.rs:465-481: an emissive mesh's selection weight from its centroid, with the cone of its
+// normals.
+__device__ __forceinline__ float emitter_weight_at(const DevEmitter& m, V3 x) {
+  V3 v = x - m.centroid;
+  float d2 = length_sq(v);
+  float orientation = m.cos_o <= -1.0f ? 1.0f : orientation_bound(v, d2, m.axis, m.cos_o, m.sin_o, m.bound);
+  if (orientation <= 0.0f) return 0.0f;
+  return m.base * orientation / (fmax_r(d2, m.area) * 1.0e-6f);
+}
+// EXT: entries may also be sphere lights, suns and emissive meshes (entry_kind).
 template <bool EXT>
 __device__ __forceinline__ float entry_weight(const DevScene& S, u32 e, V3 x) {
   i32 a = S.entries[e];
   if constexpr (EXT) {
-    if (a <= -2) return sphere_weight(S.spheres[-2 - a], x);
+    if (a < -1) {
+      u32 k;
+      EntryKind kind = entry_kind(a, k);
+      if (kind == E_SPHERE) return sphere_weight(S.spheres[k], x);
+      if (kind == E_SUN) return S.suns[k].base;
+      return emitter_weight_at(S.emitters[k], x);
+    }
   }
   return a < 0 ? S.env_base : area_weight(S.areas[a], x);
 }
@@ -1252,6 +1266,13 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
     if (primary) {
       if (S.background == 0) {
         C3 seen = splat(env_value(S, d.z));
+        if constexpr (EXT) {
+          // The sun discs flagged camera (light.rs:1502-1520).
+          for (u32 k = 0; k < S.n_suns; ++k) {
+            const DevSun& L = S.suns[k];
+            if (L.camera && sun_contains(L, d)) seen = seen + L.radiance * 1.0f;
+          }
+        }
         if (S.exterior != 1.0f) seen = seen * S.exterior;
         sum = sum + thr * seen;
       } else if (S.background == 1) {
@@ -1263,6 +1284,17 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
       C3 sky = splat(env_value(S, d.z));
       if (has_pdf && S.has_env_entry)
         sky = sky * power_heuristic(prev_pdf, sel_pmf<EXT>(S, from, S.env_entry) * env_pdf(S, d.z));
+      if constexpr (EXT) {
+        // The sun discs a BSDF-sampled ray finds, weighted against next-event estimation.
+        for (u32 k = 0; k < S.n_suns; ++k) {
+          const DevSun& L = S.suns[k];
+          bool visible = has_pdf ? L.glossy : L.camera;
+          if (visible && sun_contains(L, d)) {
+            float weight = has_pdf ? power_heuristic(prev_pdf, sel_pmf<EXT>(S, from, L.entry) / L.solid_angle) : 1.0f;
+            sky = sky + L.radiance * weight;
+          }
+        }
+      }
       add_to(sum, thr * sky, bounce >= 2, S.clamp_indirect);
     }
     done = true;
@@ -1373,7 +1405,22 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
       const DevMaterial& M = S.materials[mat];
       if (M.emissive) {
         bool one_sided = (S.instances[inst].flags & INST_ONE_SIDED) != 0;
-        if (s_front || !one_sided) add_to(sum, thr * M.radiance * 1.0f, bounce >= 2, S.clamp_indirect);
+        if (s_front || !one_sided) {
+          // A BSDF-sampled ray weighs an emitter sampled as a light against next-event estimation
+          // (integrator/mod.rs:1548-1568, light.rs:1537-1551); others count at weight one.
+          float weight = 1.0f;
+          if constexpr (EXT) {
+            i32 em = S.instance_emitter[inst];
+            if (has_pdf && em >= 0) {
+              const DevEmitter& E = S.emitters[em];
+              float cos = dot(s_geometric, d);
+              float light_pdf = sel_pmf<EXT>(S, from, E.entry) * t_geometry * t_geometry /
+                                (E.area * fmax_r(fabs_(cos), 1.0e-7f));
+              weight = power_heuristic(prev_pdf, light_pdf);
+            }
+          }
+          add_to(sum, thr * M.radiance * weight, bounce >= 2, S.clamp_indirect);
+        }
         done = true;
       }
     }
@@ -1440,12 +1487,20 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
     bool ls_glossy = true, ls_shadow = true, ls_delta = false;
     if (ne > 0) {
       u32 index = 0;
-      float p = 0.0f;
+      float p = 0.0f, rest = 0.0f;
       bool ok = true;
       if (sel.by_flux) {
-        u32 k = 0;
-        while (k < ne && S.cdf[k] <= u_sel) ++k;
-        index = k < ne - 1 ? k : ne - 1;
+        // pick (light.rs:1059-1074): the first bucket whose end passes u, by bisection.
+        u32 lo = 0, hi = ne;
+        while (lo < hi) {
+          u32 mid = (lo + hi) >> 1;
+          if (S.cdf[mid] <= u_sel) lo = mid + 1;
+          else hi = mid;
+        }
+        index = lo < ne - 1 ? lo : ne - 1;
+        float lo_c = index == 0 ? 0.0f : S.cdf[index - 1];
+        float width = S.cdf[index] - lo_c;
+        rest = width > 0.0f ? clampf((u_sel - lo_c) / width, 0.0f, 0.99999994f) : 0.0f;
         p = sel_pmf<EXT>(S, sel, index);
       } else if (!(sel.total > 0.0f && is_finite(sel.total))) {
         ok = false;
@@ -1457,22 +1512,74 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
           acc += w;
           float hi = e == last ? 1.0f : acc / sel.total;
           if (hi > u_sel || e == last) {
+            float width = hi - lo_c;
+            rest = width > 0.0f ? clampf((u_sel - lo_c) / width, 0.0f, 0.99999994f) : 0.0f;
             index = e;
             p = w / sel.total;
             break;
           }
           lo_c = hi;
         }
-        (void)lo_c;
       }
       if (ok && p > 0.0f) {
         i32 a = S.entries[index];
-        bool sphere = false;
-        if constexpr (EXT) sphere = a <= -2;
-        if (sphere) {
+        u32 ek = 0;
+        EntryKind kind = a >= 0 ? E_AREA : (a == -1 ? E_ENV : E_SPHERE);
+        if constexpr (EXT) kind = entry_kind(a, ek);
+        if (EXT && kind == E_SUN) {
+          // light.rs:1197-1223: the delta direction, or a direction in the disc's cone.
+          const DevSun& L = S.suns[ek];
+          if (L.delta) {
+            ls_wi = L.frame.z;
+            ls_pdf = p;
+          } else {
+            V3 local = uniform_cone(ul0, ul1, L.one_minus_cos_max);
+            ls_wi = normalize(to_world(L.frame, local));
+            ls_pdf = p / L.solid_angle;
+          }
+          ls_dist = RPX_INF;
+          ls_rad = L.radiance;
+          ls_delta = L.delta != 0;
+          ls_glossy = L.glossy != 0;
+          ls_shadow = L.shadow != 0;
+          have_ls = true;
+        } else if (EXT && kind == E_MESH) {
+          // light.rs:1270-1297: a triangle by area with the selection's remainder, a point
+          // uniform on it.
+          const DevEmitter& E = S.emitters[ek];
+          const float* cdf = S.emit_cdf + E.first;
+          u32 lo = 0, hi = E.count;
+          while (lo < hi) {
+            u32 mid = (lo + hi) >> 1;
+            if (cdf[mid] <= rest) lo = mid + 1;
+            else hi = mid;
+          }
+          u32 t = lo < E.count - 1 ? lo : E.count - 1;
+          const float4* tp = S.emit_tris + 4 * (size_t)(E.first + t);
+          float4 ta = tp[0], tb = tp[1], tc = tp[2], tn = tp[3];
+          float su = sqrt_(ul0);
+          float w0 = 1.0f - su, w1 = ul1 * su;
+          float w2 = 1.0f - w0 - w1;
+          V3 pt = (v3(ta.x, ta.y, ta.z) * w0 + v3(tb.x, tb.y, tb.z) * w1) + v3(tc.x, tc.y, tc.z) * w2;
+          V3 dv = pt - s_point;
+          float dist2 = length_sq(dv);
+          float distance = sqrt_(dist2);
+          V3 wi = dv / distance;
+          float facing = -dot(wi, v3(tn.x, tn.y, tn.z));
+          float cos = fabs_(facing);
+          if (distance > 0.0f && (E.two_sided || facing > 0.0f) && cos > 0.0f) {
+            ls_wi = wi;
+            ls_dist = distance;
+            ls_rad = E.radiance;
+            ls_pdf = p * dist2 / (E.area * cos);
+            ls_glossy = true;
+            ls_shadow = true;
+            have_ls = true;
+          }
+        } else if (EXT && kind == E_SPHERE) {
           // light.rs:1224-1268: a point (or a sphere too small to tell from one) by its
           // intensity over d^2, else a direction in the cone the sphere subtends.
-          const DevSphere& L = S.spheres[-2 - a];
+          const DevSphere& L = S.spheres[ek];
           float mask = sphere_mask(L, s_point);
           V3 dv = L.centre - s_point;
           float dist2 = length_sq(dv);
@@ -1504,7 +1611,7 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
               have_ls = true;
             }
           }
-        } else if (a >= 0) {
+        } else if (kind == E_AREA) {
           const DevArea& L = S.areas[a];
           float x0, y0;
           if (L.rect) {

@@ -106,6 +106,11 @@ RPX_HD void sin_cos(float x, float& s_out, float& c_out) {
     default: s_out = -c; c_out = s; break;
   }
 }
+RPX_HD float fsin(float x) {
+  float s, c;
+  sin_cos(x, s, c);
+  return s;
+}
 RPX_HD float fcos(float x) {
   float s, c;
   sin_cos(x, s, c);

@@ -128,6 +128,52 @@ RPX_HD float sphere_weight(const DevSphere& L, V3 x) {
   return L.base * m / fmax_r(d2, L.near2);
 }
 
+// A sun (light.rs:298-321, 755-791): a disc of half angle acos(cos_max) about frame.z (towards
+// the sun), sampled uniformly in its cone; a delta direction below 1e-5 rad of half angle, whose
+// radiance is then the irradiance.
+struct DevSun {
+  Frame frame;
+  float cos_max, one_minus_cos_max, solid_angle, base;
+  C3 radiance;
+  u32 delta, glossy, camera, shadow, entry;
+};
+RPX_HD bool sun_contains(const DevSun& L, V3 dir) { return !L.delta && dot(dir, L.frame.z) >= L.cos_max; }
+
+// An emissive instance sampled as an area light (light.rs:440-481, 866-953): triangles
+// [first, first + count) of the scene's emitter triangle table (world corners and unit normal, 4 x
+// float4 each) with their cumulative area fractions; the cone of its normals (cos_o <= -1: any).
+struct DevEmitter {
+  u32 first, count, two_sided, entry;
+  float area, bound, cos_o, sin_o, base;
+  C3 radiance;
+  V3 centroid, axis;
+};
+
+// Entries of the light selection (i32): an area light k >= 0, the environment -1, a sphere light
+// -2 - k, a sun ENTRY_SUN - k, an emissive mesh ENTRY_MESH - k.
+constexpr i32 ENTRY_SUN = -(1 << 20), ENTRY_MESH = -(2 << 20);
+enum EntryKind : u32 { E_AREA, E_ENV, E_SPHERE, E_SUN, E_MESH };
+RPX_HD EntryKind entry_kind(i32 a, u32& k) {
+  if (a >= 0) {
+    k = (u32)a;
+    return E_AREA;
+  }
+  if (a == -1) {
+    k = 0;
+    return E_ENV;
+  }
+  if (a > ENTRY_SUN) {
+    k = (u32)(-2 - a);
+    return E_SPHERE;
+  }
+  if (a > ENTRY_MESH) {
+    k = (u32)(ENTRY_SUN - a);
+    return E_SUN;
+  }
+  k = (u32)(ENTRY_MESH - a);
+  return E_MESH;
+}
+
 // A homogeneous scattering medium filling a clear solid (integrator/medium.rs:113-141): per mm
 // coefficients, the Henyey-Greenstein anisotropy, and the solid's index.
 struct DevMedium {
@@ -152,11 +198,18 @@ struct DevScene {
   const DevArea* areas;
   const float* cdf;       // [n_entries]
   const float* flux_pmf;  // [n_entries]
-  const i32* entries;     // area index, -1 = environment, -2 - k = sphere k
+  const i32* entries;     // see entry_kind
   const DevSphere* spheres;
   const DevMedium* media;  // per material (read for MEDIUM_SCATTERING materials)
-  u32 n_spheres;
-  u32 ext;  // the scene has sphere lights or scattering media (the kernels' extended variant)
+  const DevSun* suns;
+  const DevEmitter* emitters;
+  const float4* emit_tris;        // 4 per emitter triangle: corners a, b, c, unit normal
+  const float* emit_cdf;          // per emitter triangle: cumulative area fraction within its emitter
+  const i32* instance_emitter;    // per instance: its emitter, or -1
+  u32 n_spheres, n_suns, n_emitters;
+  // The scene has sphere lights, suns, emissive meshes sampled as lights or scattering media (the
+  // kernels' extended variant).
+  u32 ext;
   EnergyTables tables;
   u32 tlas_root, n_instances, n_materials, n_areas, n_entries, per_point, has_env_entry, env_entry;
   // environment ramp
@@ -317,6 +370,11 @@ struct FlatScene {
   std::vector<DevMaterial> materials;
   std::vector<DevArea> areas;
   std::vector<DevSphere> spheres;
+  std::vector<DevSun> suns;
+  std::vector<DevEmitter> emitters;
+  std::vector<float4> emit_tris;
+  std::vector<float> emit_cdf;
+  std::vector<i32> instance_emitter;
   std::vector<DevMedium> media;
   std::vector<float> cdf, flux_pmf;
   std::vector<i32> entries;

@@ -179,7 +179,7 @@ bool parse_rsb(const uint8_t* bytes, size_t len, HostScene& s, std::string& err,
       l.flags = r.u();
       if (!r.ok) { err = "LGHT section too short"; return false; }
       if (l.kind > 5) { err = "unknown light kind"; return false; }
-      if (l.kind == 2 || l.kind == 5) { err = "sun and IES lights are not supported by the GPU tracer"; return false; }
+      if (l.kind == 5) { err = "IES lights are not supported by the GPU tracer"; return false; }
       s.lights.push_back(l);
     } else if (t == "ENVR") {
       if (env) { err = "duplicate ENVR section"; return false; }
@@ -254,7 +254,6 @@ bool parse_rsb(const uint8_t* bytes, size_t len, HostScene& s, std::string& err,
     if (m.flags & MATERIAL_FLAG_THIN) { err = "thin walls are not supported by the GPU tracer"; return false; }
     if (m.abbe > 0.0f) { err = "dispersion is not supported by the GPU tracer"; return false; }
     if (m.bevel_mm > 0.0f) { err = "edge bevels are not supported by the GPU tracer"; return false; }
-    if (m.kind == 2 && (m.flags & MATERIAL_FLAG_LIGHT)) { err = "emissive mesh lights are not supported by the GPU tracer"; return false; }
   }
   // A smooth, wholly clear solid without a medium of its own may be a thin wall, which the
   // reference probes for after a scattering medium's exit (integrator/mod.rs:289-299, 1822-1857);
@@ -872,6 +871,34 @@ bool flatten_scene(const HostScene& s, const EnergyTables& tables, FlatScene& ou
   };
   std::vector<float> weights;
   for (const HostLight& l : s.lights) {
+    if (l.kind == 2) {
+      // A sun (light.rs:755-791).
+      V3 to_sun = -normalize(l.direction);
+      if (length_sq(to_sun) < 0.5f) continue;
+      float half = fmin_r(0.5f * fmax_r(l.angle_deg, 0.0f) * deg_to_rad(), 0.5f * PI);
+      C3 color = c3(fmax_r(l.color.r, 0.0f), fmax_r(l.color.g, 0.0f), fmax_r(l.color.b, 0.0f));
+      C3 irradiance = color * fmax_r(l.power, 0.0f);
+      if (!(luminance(irradiance) > 0.0f)) continue;
+      DevSun sun;
+      sun.delta = half < 1.0e-5f ? 1u : 0u;
+      float sh = fsin(0.5f * half);
+      sun.one_minus_cos_max = 2.0f * sh * sh;
+      float sin_max = fsin(half);
+      sun.radiance = sun.delta ? irradiance : irradiance * (1.0f / (PI * sin_max * sin_max));
+      weights.push_back(PI * luminance(irradiance));
+      sun.entry = (u32)out.entries.size();
+      out.entries.push_back(ENTRY_SUN - (i32)out.suns.size());
+      sun.frame = frame_from_normal(to_sun);
+      sun.cos_max = 1.0f - sun.one_minus_cos_max;
+      sun.solid_angle = TAU * sun.one_minus_cos_max;
+      sun.glossy = (l.flags & 1u) ? 1u : 0u;
+      sun.camera = (l.flags & 2u) ? 1u : 0u;
+      sun.shadow = (l.flags & 4u) ? 0u : 1u;
+      sun.base = luminance(irradiance);
+      out.suns.push_back(sun);
+      out.consts.ext = 1;
+      continue;
+    }
     if (l.kind == 3 || l.kind == 4) {
       // light.rs:793-863 (no IES profile): a point (diameter 0) or a sphere, a spot masked by its
       // cone with a smoothstep over the soft edge.
@@ -945,6 +972,102 @@ bool flatten_scene(const HostScene& s, const EnergyTables& tables, FlatScene& ou
     a.base = luminance(a.radiance) * a.area * MM2_TO_M2;
     out.areas.push_back(a);
   }
+  // Emissive instances whose material is flagged a light, sampled as area lights
+  // (light.rs:866-953): world-space triangles of positive area, picked by area.
+  out.instance_emitter.assign(s.instances.size(), -1);
+  for (size_t i = 0; i < s.instances.size(); ++i) {
+    const HostInstance& in = s.instances[i];
+    if (in.material >= s.materials.size()) continue;
+    const HostMaterial& mat = s.materials[in.material];
+    C3 radiance = mat.kind == 2 ? c3(fmax_r(mat.emission.r * mat.emission_power, 0.0f),
+                                     fmax_r(mat.emission.g * mat.emission_power, 0.0f),
+                                     fmax_r(mat.emission.b * mat.emission_power, 0.0f))
+                                : splat(0.0f);
+    if (!(mat.flags & MATERIAL_FLAG_LIGHT) || !(luminance(radiance) > 0.0f)) continue;
+    if (in.mesh >= meshes.size()) continue;
+    Xf x = xf_from(in.rows), inv;
+    if (!xf_inverse(x, inv)) continue;
+    const HostMesh& m = meshes[in.mesh];
+    DevEmitter em;
+    em.first = (u32)(out.emit_tris.size() / 4);
+    std::vector<float> tri_areas;
+    std::vector<V3> corners;
+    std::vector<V3> normals;
+    V3 centroid = v3(0.0f, 0.0f, 0.0f);
+    for (size_t t = 0; t + 2 < m.indices.size(); t += 3) {
+      V3 w[3];
+      for (int c = 0; c < 3; ++c) {
+        const float* p = &m.positions[3 * (size_t)m.indices[t + c]];
+        w[c] = xf_point(x, v3(p[0], p[1], p[2]));
+      }
+      V3 cr = cross(w[1] - w[0], w[2] - w[0]);
+      float area = 0.5f * length(cr);
+      if (!(area > 0.0f && is_finite(area))) continue;
+      centroid = centroid + ((w[0] + w[1]) + w[2]) * (area / 3.0f);
+      V3 n = normalize(cr);
+      out.emit_tris.push_back(make_float4(w[0].x, w[0].y, w[0].z, 0.0f));
+      out.emit_tris.push_back(make_float4(w[1].x, w[1].y, w[1].z, 0.0f));
+      out.emit_tris.push_back(make_float4(w[2].x, w[2].y, w[2].z, 0.0f));
+      out.emit_tris.push_back(make_float4(n.x, n.y, n.z, 0.0f));
+      corners.push_back(w[0]);
+      corners.push_back(w[1]);
+      corners.push_back(w[2]);
+      normals.push_back(n);
+      tri_areas.push_back(area);
+    }
+    float area = -0.0f;
+    for (float a : tri_areas) area = area + a;
+    std::vector<float> cdf;
+    {
+      float total = -0.0f;
+      for (float a : tri_areas) total = total + a;
+      if (total > 0.0f && is_finite(total)) {
+        float acc = 0.0f;
+        for (float a : tri_areas) {
+          acc += a;
+          cdf.push_back(acc / total);
+        }
+        cdf.back() = 1.0f;
+      }
+    }
+    if (cdf.empty()) {
+      out.emit_tris.resize((size_t)em.first * 4);
+      continue;
+    }
+    out.emit_cdf.insert(out.emit_cdf.end(), cdf.begin(), cdf.end());
+    em.count = (u32)cdf.size();
+    em.two_sided = (in.flags & 1u) ? 0u : 1u;
+    float sides = em.two_sided ? 2.0f : 1.0f;
+    float flux = luminance(radiance) * PI * area * MM2_TO_M2 * sides;
+    em.centroid = centroid / area;
+    float bound = 0.0f;
+    for (const V3& c : corners) bound = fmax_r(bound, length(c - em.centroid));
+    em.bound = bound;
+    V3 mean = v3(0.0f, 0.0f, 0.0f);
+    for (size_t t = 0; t < normals.size(); ++t) mean = mean + normals[t] * tri_areas[t];
+    float mlen = length(mean);
+    if (em.two_sided || mlen <= 1.0e-3f * area) {
+      em.axis = v3(0.0f, 0.0f, 1.0f);
+      em.cos_o = -1.0f;
+      em.sin_o = 0.0f;
+    } else {
+      em.axis = mean / mlen;
+      float cos_o = 1.0f;
+      for (const V3& n : normals) cos_o = fmin_r(cos_o, dot(n, em.axis));
+      cos_o = clampf(cos_o, -1.0f, 1.0f);
+      em.cos_o = cos_o;
+      em.sin_o = sqrt_(fmax_r(1.0f - cos_o * cos_o, 0.0f));
+    }
+    weights.push_back(flux * reach(em.centroid, nullptr));
+    em.entry = (u32)out.entries.size();
+    out.entries.push_back(ENTRY_MESH - (i32)out.emitters.size());
+    out.instance_emitter[i] = (i32)out.emitters.size();
+    em.area = area;
+    em.radiance = radiance;
+    em.base = luminance(radiance) * area * MM2_TO_M2;
+    out.emitters.push_back(em);
+    out.consts.ext = 1;
+  }
   float strength = fmax_r(s.env_strength, 0.0f);
   float ef = s.env_kind == 1 ? s.env_horizon : s.env_floor, eh = s.env_horizon, ez = s.env_kind == 1 ? s.env_horizon : s.env_zenith;
   DevScene& k = out.consts;
@@ -982,14 +1105,12 @@ bool flatten_scene(const HostScene& s, const EnergyTables& tables, FlatScene& ou
     k.has_env_entry = 0;
   }
   for (size_t e = 0; e < out.cdf.size(); ++e) out.flux_pmf.push_back(out.cdf[e] - (e == 0 ? 0.0f : out.cdf[e - 1]));
-  if (out.entries.size() > (size_t)MAX_ENTRIES) {
-    err = "more than 64 lights are not supported by the GPU tracer";
-    return false;
-  }
   k.n_entries = (u32)out.entries.size();
   k.per_point = (out.entries.size() >= 2 && out.entries.size() <= (size_t)MAX_ENTRIES) ? 1u : 0u;
   k.n_areas = (u32)out.areas.size();
   k.n_spheres = (u32)out.spheres.size();
+  k.n_suns = (u32)out.suns.size();
+  k.n_emitters = (u32)out.emitters.size();
   k.n_instances = (u32)out.instances.size();
   k.n_materials = (u32)out.materials.size();
   // Camera (scene/mod.rs:160-357): basis and the sensor of the longer side.
