@@ -103,6 +103,14 @@ def lib():
         L.orc_lights_sample.argtypes = [vp, vp, f32, f32, f32, vp]
         L.orc_lights_mis_probe.argtypes = [vp, vp, u32, vp]
         L.orc_lights_escaped.argtypes = [vp, vp, vp]
+        L.orc_caustics_info.argtypes = [vp, vp]
+        L.orc_caustics_radius.restype = f32
+        L.orc_caustics_radius.argtypes = [vp, u32]
+        L.orc_photons.restype = u32
+        L.orc_photons.argtypes = [vp, u32, u32, u32, vp, u32, vp]
+        L.orc_photon_gather.argtypes = [vp, u32, vp, u32, vp, u32, vp]
+        L.orc_is_caster.restype = ctypes.c_int
+        L.orc_is_caster.argtypes = [vp]
         L.orc_lights_emitter_weight.restype = f32
         L.orc_lights_emitter_weight.argtypes = [vp, vp, u32, f32, f32, f32]
         L.orc_soup.restype = u32
@@ -276,6 +284,36 @@ class OracleScene:
             return None
         return {"wi": out[0:3].copy(), "distance": float(out[3]), "radiance": out[4:7].copy(), "pdf": float(out[7]),
                 "delta": bool(out[8]), "glossy": bool(out[9]), "entry": int(out[10]), "kind": int(out[11])}
+
+    def caustics(self):
+        """The caustic setup (caustics.rs:229-345): has, photons, first radius, sources, caster
+        spheres, most casters a photon crosses."""
+        out = np.zeros(6, np.float32)
+        lib().orc_caustics_info(self.h, _p(out))
+        return {"on": bool(out[0]), "photons": int(out[1]), "radius": float(out[2]), "sources": int(out[3]),
+                "spheres": int(out[4]), "max_bounces": int(out[5])}
+
+    def caustic_radius(self, iteration):
+        return float(lib().orc_caustics_radius(self.h, iteration))
+
+    def photons(self, iteration, first=0, last=None):
+        """The stored photons of photons [first, last) of pass `iteration`: an (n, 13) array of
+        position, wi, power, normal, casters crossed, and the rays cast."""
+        if last is None:
+            last = self.caustics()["photons"]
+        rays = np.zeros(1, np.uint64)
+        n = lib().orc_photons(self.h, iteration, first, last, None, 0, _p(rays))
+        out = np.zeros((max(n, 1), 13), np.float32)
+        lib().orc_photons(self.h, iteration, first, last, _p(out), n, _p(rays))
+        return out[:n], int(rays[0])
+
+    def photon_gather(self, iteration, photons, queries):
+        """Gathers (BSDF one) at queries (point, normal, most casters) from a map of `photons`."""
+        ph = np.ascontiguousarray(photons, np.float32).reshape(-1, 13)
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 7)
+        out = np.zeros((q.shape[0], 3), np.float32)
+        lib().orc_photon_gather(self.h, iteration, _p(ph), ph.shape[0], _p(q), q.shape[0], _p(out))
+        return out
 
     def light_escaped(self, direction):
         """What a camera ray leaving along `direction` sees of the sky (light.rs:1502-1520)."""
@@ -512,6 +550,13 @@ def soup():
     if n == 0:
         raise RuntimeError("soup positions overflow")
     return pos[:n].copy(), idx
+
+
+def is_caster(kind=0, base=(0.8, 0.8, 0.8), metallic=0.0, roughness=0.5, ior=1.5, transmission=0.0, coat=0.0,
+              coat_roughness=0.0):
+    """caustics.rs:104-119: glass or metal sharp enough to focus light."""
+    m = np.array([kind, *base, metallic, roughness, ior, transmission, coat, coat_roughness], np.float32)
+    return bool(lib().orc_is_caster(_p(m)))
 
 
 def fresnel_dielectric(cos_i, eta):

@@ -12,6 +12,7 @@
 #include "rt_adaptive.hpp"
 #include "rt_denoise.hpp"
 #include "rt_passes.hpp"
+#include "rt_caustics.hpp"
 #include "rt_handle.hpp"
 
 using namespace orc;
@@ -75,6 +76,9 @@ int orc_render_tile(void* p, uint32_t x, uint32_t y, uint32_t w, uint32_t h, uin
   auto* hd = static_cast<Handle*>(p);
   const Scene& s = hd->scene;
   RenderStats st;
+  PhotonMap map;
+  if (hd->prep.has_caustics) map = trace_photon_map(s, hd->prep, sample_index, (int)std::thread::hardware_concurrency());
+  const PhotonMap* photons = hd->prep.has_caustics ? &map : nullptr;
   for (uint32_t j = 0; j < h; ++j)
     for (uint32_t i = 0; i < w; ++i) {
       uint32_t px = x + i, py = y + j;
@@ -89,7 +93,7 @@ int orc_render_tile(void* p, uint32_t x, uint32_t y, uint32_t w, uint32_t h, uin
       camera_sample(s, sampler, px, py, ray);
       Rgb c;
       float a;
-      radiance(s, hd->prep, sampler, ray, c, a, &st);
+      radiance(s, hd->prep, sampler, ray, c, a, &st, nullptr, photons);
       rgb[3 * k] = c.r;
       rgb[3 * k + 1] = c.g;
       rgb[3 * k + 2] = c.b;
@@ -128,6 +132,10 @@ static int render_tiles(Handle* hd, const std::vector<TileRect>& tiles, uint32_t
                         float* rgb, float* weight, float* alpha, float* lum2, uint64_t* stats_out) {
   const Scene& s = hd->scene;
   uint32_t W = s.render.width;
+  // The photon maps of the samples (one per sample index), traced before the tiles.
+  std::vector<PhotonMap> maps;
+  if (hd->prep.has_caustics)
+    for (uint32_t k = 0; k < count; ++k) maps.push_back(trace_photon_map(s, hd->prep, first + k, threads));
   std::atomic<size_t> next{0};
   std::vector<RenderStats> per(std::max(threads, 1));
   auto worker = [&](int tid) {
@@ -145,7 +153,8 @@ static int render_tiles(Handle* hd, const std::vector<TileRect>& tiles, uint32_t
           film.alpha[dst] = alpha[src];
           film.lum2[dst] = lum2[src];
         }
-      for (uint32_t k = 0; k < count; ++k) render_tile(s, hd->prep, tile, first + k, film, &per[tid]);
+      for (uint32_t k = 0; k < count; ++k)
+        render_tile(s, hd->prep, tile, first + k, film, &per[tid], maps.empty() ? nullptr : &maps[k]);
       for (uint32_t j = 0; j < tile.h; ++j)
         for (uint32_t i = 0; i < tile.w; ++i) {
           size_t dst = (size_t)(tile.y + j) * W + tile.x + i, src = (size_t)j * tile.w + i;
@@ -335,6 +344,77 @@ float orc_lights_emitter_weight(void* p, const float* point, uint32_t instance, 
   const Lights& l = static_cast<Handle*>(p)->prep.lights;
   Selection sel = l.select(Vec3(point[0], point[1], point[2]));
   return l.emitter_weight(instance, bsdf_pdf, sel, t, cos);
+}
+
+// ---------------------------------------------------------------- caustics (caustics.rs)
+// has caustics, photons per sample, first radius (mm), sources, caster spheres, most casters a
+// photon crosses.
+void orc_caustics_info(void* p, float* out6) {
+  const Prepared& pr = static_cast<Handle*>(p)->prep;
+  const CausticSetup& c = pr.caustics;
+  out6[0] = pr.has_caustics ? 1.0f : 0.0f;
+  out6[1] = (float)c.photons;
+  out6[2] = c.radius;
+  out6[3] = (float)c.sources.size();
+  out6[4] = (float)c.sphere_c.size();
+  out6[5] = (float)c.max_bounces;
+}
+float orc_caustics_radius(void* p, uint32_t iteration) {
+  return static_cast<Handle*>(p)->prep.caustics.radius_of(iteration);
+}
+
+// The stored photons of photons [first, last) of pass `iteration`, 13 floats each (position, wi,
+// power, normal, casters crossed), up to `cap`; returns how many were stored (even beyond cap).
+uint32_t orc_photons(void* p, uint32_t iteration, uint32_t first, uint32_t last, float* out, uint32_t cap,
+                     uint64_t* rays) {
+  auto* hd = static_cast<Handle*>(p);
+  std::vector<Photon> ph;
+  u64 r = 0;
+  if (hd->prep.has_caustics) trace_photons(hd->scene, hd->prep, hd->prep.caustics, iteration, first, last, ph, r);
+  for (size_t i = 0; i < ph.size() && i < cap; ++i) {
+    const Photon& q = ph[i];
+    float v[13] = {q.position.x, q.position.y, q.position.z, q.wi.x, q.wi.y, q.wi.z, q.power.r, q.power.g,
+                   q.power.b, q.normal.x, q.normal.y, q.normal.z, (float)q.bounces};
+    std::memcpy(out + 13 * i, v, sizeof(v));
+  }
+  if (rays) *rays = r;
+  return (uint32_t)ph.size();
+}
+
+// The map of pass `iteration` built from n given photons (13 floats each), gathered with a BSDF
+// of one at nq queries (point xyz, normal xyz, most casters): rgb per query.
+void orc_photon_gather(void* p, uint32_t iteration, const float* photons, uint32_t n, const float* queries, uint32_t nq,
+                       float* out3) {
+  auto* hd = static_cast<Handle*>(p);
+  std::vector<Photon> ph(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const float* v = photons + 13 * (size_t)i;
+    ph[i] = Photon{Vec3(v[0], v[1], v[2]), Vec3(v[3], v[4], v[5]), Rgb(v[6], v[7], v[8]), Vec3(v[9], v[10], v[11]),
+                   (u32)v[12]};
+  }
+  PhotonMap m = build_photon_map(hd->prep.caustics, iteration, std::move(ph), 0);
+  for (uint32_t k = 0; k < nq; ++k) {
+    const float* q = queries + 7 * (size_t)k;
+    Rgb c = m.gather(Vec3(q[0], q[1], q[2]), Vec3(q[3], q[4], q[5]), (u32)q[6], [](Vec3) { return Rgb(1, 1, 1); });
+    out3[3 * k] = c.r;
+    out3[3 * k + 1] = c.g;
+    out3[3 * k + 2] = c.b;
+  }
+}
+
+// caustics.rs:104-119 for a material given as orc_bsdf_probe's `mat` (kind, base rgb, metallic,
+// roughness, ior, transmission, coat, coat roughness).
+int orc_is_caster(const float* mat) {
+  Material m;
+  m.kind = (u32)mat[0];
+  m.base_color = Rgb(mat[1], mat[2], mat[3]);
+  m.metallic = mat[4];
+  m.roughness = mat[5];
+  m.ior = mat[6];
+  m.transmission = mat[7];
+  m.coat = mat[8];
+  m.coat_roughness = mat[9];
+  return is_caster(m) ? 1 : 0;
 }
 
 // What the BSDF side gives a light that next-event estimation sampled (light.rs:2396-2470): for
@@ -540,26 +620,44 @@ int orc_render_film_f64(void* p, uint32_t first, uint32_t count, int threads, do
   auto* hd = static_cast<Handle*>(p);
   const Scene& s = hd->scene;
   u32 W = s.render.width, H = s.render.height;
-  std::atomic<u32> next_row{0};
-  auto worker = [&]() {
+  // Scenes with caustics go sample by sample (each sample index has its photon map); every
+  // pixel's samples are summed in index order either way.
+  bool by_sample = hd->prep.has_caustics;
+  std::vector<double> acc(by_sample ? (size_t)W * H * 4 : 0, 0.0);
+  PhotonMap map;
+  u32 k0 = 0, k1 = count;
+  auto worker = [&](std::atomic<u32>* next_row) {
     for (;;) {
-      u32 y = next_row.fetch_add(1);
+      u32 y = next_row->fetch_add(1);
       if (y >= H) return;
       for (u32 x = 0; x < W; ++x) {
+        size_t i = (size_t)y * W + x;
         double r = 0, g = 0, b = 0, a = 0;
-        for (u32 k = 0; k < count; ++k) {
+        if (by_sample) {
+          r = acc[4 * i];
+          g = acc[4 * i + 1];
+          b = acc[4 * i + 2];
+          a = acc[4 * i + 3];
+        }
+        for (u32 k = k0; k < k1; ++k) {
           PixelSampler sampler = PixelSampler::make(s.render.seed, x, y, first + k);
           Ray ray;
           camera_sample(s, sampler, x, y, ray);
           Rgb c;
           float al;
-          radiance(s, hd->prep, sampler, ray, c, al, nullptr);
+          radiance(s, hd->prep, sampler, ray, c, al, nullptr, nullptr, by_sample ? &map : nullptr);
           r += c.r;
           g += c.g;
           b += c.b;
           a += al;
         }
-        size_t i = (size_t)y * W + x;
+        if (by_sample && k1 < count) {
+          acc[4 * i] = r;
+          acc[4 * i + 1] = g;
+          acc[4 * i + 2] = b;
+          acc[4 * i + 3] = a;
+          continue;
+        }
         rgb[3 * i] += r;
         rgb[3 * i + 1] += g;
         rgb[3 * i + 2] += b;
@@ -567,9 +665,22 @@ int orc_render_film_f64(void* p, uint32_t first, uint32_t count, int threads, do
       }
     }
   };
-  std::vector<std::thread> pool;
-  for (int t = 0; t < std::max(threads, 1); ++t) pool.emplace_back(worker);
-  for (auto& t : pool) t.join();
+  auto run = [&]() {
+    std::atomic<u32> next_row{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < std::max(threads, 1); ++t) pool.emplace_back(worker, &next_row);
+    for (auto& t : pool) t.join();
+  };
+  if (!by_sample) {
+    run();
+    return 0;
+  }
+  for (u32 k = 0; k < count; ++k) {
+    map = trace_photon_map(s, hd->prep, first + k, threads);
+    k0 = k;
+    k1 = k + 1;
+    run();
+  }
   return 0;
 }
 

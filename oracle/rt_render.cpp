@@ -1,5 +1,6 @@
 // ORACLE — test infrastructure only (see rt_core.hpp). Citations in rt_render.hpp.
 #include "rt_render.hpp"
+#include "rt_caustics.hpp"
 #include <algorithm>
 #include <cstring>
 
@@ -231,15 +232,16 @@ bool scene_from_rsb(const uint8_t* bytes, size_t len, Scene& scene, std::string&
       s.clamp_indirect = r.opt_f32(0);
       (void)r.opt_u32(0);   // passes
       (void)r.opt_u32(0);   // denoise
-      u32 caustic_photons = r.opt_u32(0);
-      (void)r.opt_f32(0);   // caustic radius
+      s.caustic_photons = r.opt_u32(0);
+      s.caustic_radius = r.opt_f32(0);
       (void)r.opt_f32(1);   // denoise strength
       s.pixel_filter = r.opt_u32(0);
       s.edge_supersample = r.opt_u32(1);
       // E10 (rsb.rs:1105-1110): the adaptive bit asks the host for rounds; the renderer reads it not.
       s.adaptive = r.opt_u32(0);
       s.noise_threshold = r.opt_f32(0);
-      if (caustic_photons != 0) { err = "caustics are not supported"; return false; }
+      if (s.caustic_photons > MAX_CAUSTIC_PHOTONS) { err = "caustic_photons must be at most 4194304"; return false; }
+      if (s.caustic_radius < 0.0f) { err = "caustic_radius must not be negative"; return false; }
       if (!(s.noise_threshold >= 0.0f && s.noise_threshold <= 1.0f)) {
         err = "noise_threshold must be between 0 and 1";
         return false;
@@ -921,9 +923,11 @@ bool Lights::sample_selected(const Selection& from, float u_select, float u0, fl
 }
 
 Rgb Lights::crossed_from(const Ray& ray, float t_max, bool has_bsdf, float bsdf_pdf, const Selection* from,
-                         float offset) const {
+                         float offset, bool with_areas) const {
   Rgb sum = BLACK;
+  // Without the areas (their light is the photon map's): light.rs:1344-1352.
   for (const AreaLight& light : areas) {
+    if (!with_areas) break;
     bool visible = has_bsdf ? light.glossy : light.camera;
     if (!visible) continue;
     float t, cos;
@@ -972,9 +976,10 @@ bool Lights::camera_sees_panel(const Ray& ray, float t_max) const {
 
 // light.rs:1502-1520: the environment, then the sun discs (a camera ray sees the suns flagged
 // camera, a BSDF-sampled ray those flagged glossy, weighted against next-event estimation).
-Rgb Lights::escaped(Vec3 dir, bool has_bsdf, float bsdf_pdf, const Selection* from) const {
+Rgb Lights::escaped(Vec3 dir, bool has_bsdf, float bsdf_pdf, const Selection* from, bool with_suns) const {
   Rgb sum = Rgb::splat(env.value(dir.z));
   if (has_bsdf && has_env_entry) sum = sum * found_weight(bsdf_pdf, pmf(env_entry, *from) * env.pdf(dir));
+  if (!with_suns) return sum;  // light.rs:1522-1533: the environment alone
   for (const SunLight& sun : suns) {
     bool visible = has_bsdf ? sun.glossy : sun.camera;
     if (visible && sun.contains(dir)) {
@@ -1026,6 +1031,7 @@ bool Prepared::build(const Scene& scene, std::string& err) {
     has_medium.push_back(on);
     media.push_back(sigma);
   }
+  has_caustics = caustic_setup(scene, lights, caustics);
   return true;
 }
 
@@ -1303,7 +1309,12 @@ void commit(const Scene& scene, const AdjointPath& a, const GradTarget& t, const
 }  // namespace
 
 void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& sampler, Ray camera_ray, Rgb& out_sum,
-              float& out_alpha, RenderStats* stats, const AdjointPath* adjoint) {
+              float& out_alpha, RenderStats* stats, const AdjointPath* adjoint, const PhotonMap* photons) {
+  // Caustics (integrator/mod.rs:1084-1113): where the path is in the split between the photon map
+  // and the tracer: before its first receiver, after one, or after one and casters since.
+  enum { EYE, AFTER_RECEIVER, AFTER_RECEIVER_CASTERS };
+  const PhotonMap* map = prep.has_caustics ? photons : nullptr;
+  int cstate = EYE;
   // The adjoint's pending term: d log f / d parameter of the last BSDF sample, and its target.
   GradTarget pending_target;
   Rgb pending_dl[5];
@@ -1367,8 +1378,11 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
       Rgb bl(expf_(-sigma.r * t_nearest), expf_(-sigma.g * t_nearest), expf_(-sigma.b * t_nearest));
       throughput = throughput * bl;
     }
+    // Light of photon lights arriving now is the photon map's (integrator/mod.rs:1261-1272): the
+    // area lights crossed are left out, the spheres stay.
+    bool dropped = map && cstate == AFTER_RECEIVER_CASTERS;
     // After a walk, the segment to its exit vertex crosses no light.
-    Rgb crossed = exit ? BLACK : lights.crossed_from(ray, t_nearest, has_pdf, bsdf_pdf, &sampled_at, offset);
+    Rgb crossed = exit ? BLACK : lights.crossed_from(ray, t_nearest, has_pdf, bsdf_pdf, &sampled_at, offset, !dropped);
     prep.add(sum, throughput * crossed, bounce >= 2);
     if (t_nearest == INF) {
       if (primary) {
@@ -1382,7 +1396,7 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
           alpha = peak(crossed) > 0.0f ? 1.0f : 0.0f;
         }
       } else {
-        Rgb sky = lights.escaped(ray.dir, has_pdf, bsdf_pdf, &sampled_at);
+        Rgb sky = lights.escaped(ray.dir, has_pdf, bsdf_pdf, &sampled_at, !(dropped && has_pdf));
         prep.add(sum, throughput * sky, bounce >= 2);
       }
       break;
@@ -1422,7 +1436,7 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
     const Material* material = s_has_instance ? &prep.material(scene, s_instance) : nullptr;
     if (material && material->is_emissive()) {
       bool one_sided = (geo.instances[s_instance].flags & INSTANCE_ONE_SIDED) != 0;
-      if (s_front || !one_sided) {
+      if ((s_front || !one_sided) && !(dropped && lights.is_mesh_light(s_instance))) {
         // integrator/mod.rs:1548-1568: a BSDF-sampled ray weighs an emitter sampled as a light
         // against next-event estimation; others have a weight of one.
         float weight = 1.0f;
@@ -1456,11 +1470,30 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
     Vec3 ng = s_geometric;
     Vec3 origin = offset_origin(s_point, ng);
     u32 dims = bounce_dim(bounce);
+    // Caustics (integrator/mod.rs:1672-1694): every receiver gathers the photons (the first one's
+    // light direct, a later one's indirect); the state after this vertex says whether its light
+    // of photon lights is the map's.
+    int next_cstate = cstate;
+    if (map) {
+      bool caster = s_has_instance && prep.caustics.is_caster_instance(s_instance);
+      if (!caster) {
+        u32 budget = prep.max_depth > bounce + 1 ? prep.max_depth - (bounce + 1) : 0u;
+        Vec3 side = exit ? -s_geometric : s_geometric;
+        Rgb c = map->gather(s_hit_point, side, budget,
+                            [&](Vec3 wi) { return bsdf.eval(wo, frame.to_local(wi), Lobes::All); });
+        prep.add(sum, throughput * c, cstate != EYE);
+        next_cstate = AFTER_RECEIVER;
+      } else if (cstate != EYE) {
+        next_cstate = AFTER_RECEIVER_CASTERS;
+      }
+    }
+    bool drop_nee = map && next_cstate == AFTER_RECEIVER_CASTERS;
     Selection selection = lights.select(s_point);
     LightSample ls;
     float ul0, ul1;
     sampler.get2(dims + OFFSET_LIGHT, ul0, ul1);
-    if (lights.sample_selected(selection, sampler.get1(dims + OFFSET_LIGHT_SELECT), ul0, ul1, ls)) {
+    if (lights.sample_selected(selection, sampler.get1(dims + OFFSET_LIGHT_SELECT), ul0, ul1, ls) &&
+        !(drop_nee && ls.photon)) {
       Vec3 wi = frame.to_local(ls.wi);
       Rgb f = bsdf.eval(wo, wi, ls.glossy ? Lobes::All : Lobes::Diffuse);
       if (peak(f) > 0.0f) {
@@ -1528,6 +1561,7 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
       sampled_at = selection;
       primary = false;
       offset = 0.0f;
+      cstate = next_cstate;
       // integrator/mod.rs:1822-1857: a BSDF sample transmitted into a solid right after an exit
       // vertex is probed for a thin wall (one ray counted; the scenes with media have no walls).
       bool next_exit = false;
@@ -1635,7 +1669,7 @@ std::vector<float> Film::resolve() const {
 }
 
 void render_tile(const Scene& scene, const Prepared& prep, TileRect tile, u32 sample_index, Film& film,
-                 RenderStats* stats) {
+                 RenderStats* stats, const PhotonMap* photons) {
   u32 width = scene.render.width, height = scene.render.height;
   u32 x_end = std::min<u32>(tile.x + tile.w, width);
   u32 y_end = std::min<u32>(tile.y + tile.h, height);
@@ -1646,7 +1680,7 @@ void render_tile(const Scene& scene, const Prepared& prep, TileRect tile, u32 sa
       camera_sample(scene, sampler, x, y, ray);
       Rgb c;
       float a;
-      radiance(scene, prep, sampler, ray, c, a, stats);
+      radiance(scene, prep, sampler, ray, c, a, stats, nullptr, photons);
       film.splat_alpha(x, y, c * WHITE, a);
     }
   }

@@ -75,6 +75,8 @@ struct RenderSettings {
   u32 pixel_filter = 0, edge_supersample = 1;
   u32 adaptive = 0;
   float noise_threshold = 0;
+  u32 caustic_photons = 0;
+  float caustic_radius = 0;
   u32 depth() const { return max_depth == 0 ? 8 : std::min<u32>(max_depth, 1024); }
 };
 
@@ -227,8 +229,9 @@ struct Lights {
   float found_weight(float bsdf_pdf, float light_pdf) const;
   float sampled_weight(float light_pdf, float bsdf_pdf) const;
   bool sample_selected(const Selection& sel, float u_select, float u0, float u1, LightSample& out) const;
-  Rgb crossed_from(const Ray& ray, float t_max, bool has_bsdf, float bsdf_pdf, const Selection* from, float offset) const;
-  Rgb escaped(Vec3 dir, bool has_bsdf, float bsdf_pdf, const Selection* from) const;
+  Rgb crossed_from(const Ray& ray, float t_max, bool has_bsdf, float bsdf_pdf, const Selection* from, float offset,
+                   bool with_areas = true) const;
+  Rgb escaped(Vec3 dir, bool has_bsdf, float bsdf_pdf, const Selection* from, bool with_suns = true) const;
   // The MIS weight of the emission a BSDF-sampled ray finds on `instance` at distance t with `cos`
   // between the ray and the surface normal; 1 when the instance is not a sampled emitter
   // (light.rs:1537-1551).
@@ -252,6 +255,39 @@ bool scatter_medium_of(const Material& m, ScatterMedium& out);
 float phase_for_tests(float cos, float g);
 Vec3 sample_phase_for_tests(Vec3 dir, float g, float u0, float u1, float& density);
 
+// ---------------------------------------------------------------- caustics (rt_caustics.*)
+constexpr float CAUSTIC_MAX_ROUGHNESS = 0.25f;
+constexpr u32 CAUSTIC_MAX_SPHERES = 16;
+constexpr float CAUSTIC_RADIUS_ALPHA = 2.0f / 3.0f;
+constexpr float CAUSTIC_DEFAULT_RADIUS_PX = 3.0f;
+constexpr u32 MAX_CAUSTIC_PHOTONS = 1u << 22;
+
+// What tracing caustic photons needs from a scene (caustics.rs:121-200): photons per sample
+// index, the first gather radius (mm), the casters' bounding spheres, which instances cast, the
+// photon sources and their selection cdf, the scene's bounding sphere, the seed and the most
+// casters a photon crosses.
+struct CausticSetup {
+  u32 photons = 0;
+  float radius = 0;  // first gather radius (mm)
+  std::vector<Vec3> sphere_c;
+  std::vector<float> sphere_r;
+  std::vector<char> casters;  // per instance
+  struct Source {
+    u32 kind;  // 0 area light, 1 sun, 2 emissive mesh
+    u32 index;
+  };
+  std::vector<Source> sources;
+  std::vector<float> cdf;
+  Vec3 scene_centre;
+  float scene_radius = 1;
+  u32 seed = 0;
+  u32 max_bounces = 0;
+  bool is_caster_instance(u32 i) const { return i < casters.size() && casters[i]; }
+  float radius_of(u32 iteration) const;
+};
+
+struct PhotonMap;
+
 // ---------------------------------------------------------------- prepared scene + film
 struct Prepared {
   Lights lights;
@@ -267,6 +303,8 @@ struct Prepared {
   std::vector<Rgb> media;  // per material: absorption coefficients per mm
   std::vector<bool> has_scatter;
   std::vector<ScatterMedium> scatter;  // per material: the scattering medium of its inside
+  bool has_caustics = false;
+  CausticSetup caustics;
 
   bool build(const Scene& scene, std::string& err);
   float floor_t(const Ray& ray, bool camera_ray) const;
@@ -292,8 +330,10 @@ struct AdjointPath {
 constexpr float PARAM_STEP = 1.0e-3f;
 
 // One sample of one pixel: radiance and alpha. `stats` and `adjoint` may be null.
+// `photons`: the photon map of this sample index when the scene has caustics.
 void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& sampler, Ray camera_ray,
-              Rgb& out_sum, float& out_alpha, RenderStats* stats, const AdjointPath* adjoint = nullptr);
+              Rgb& out_sum, float& out_alpha, RenderStats* stats, const AdjointPath* adjoint = nullptr,
+              const PhotonMap* photons = nullptr);
 void camera_sample(const Scene& scene, const PixelSampler& sampler, u32 x, u32 y, Ray& ray);
 
 struct TileRect { u32 x, y, w, h; };
@@ -308,6 +348,6 @@ struct Film {
 };
 
 void render_tile(const Scene& scene, const Prepared& prep, TileRect tile, u32 sample_index, Film& film,
-                 RenderStats* stats);
+                 RenderStats* stats, const PhotonMap* photons = nullptr);
 
 }  // namespace orc

@@ -70,9 +70,10 @@ class SceneWriter:
         self.sections.append(_section(b"FLOR", _u(mode) + _f(z, *color)))
         return self
 
-    def render(self, width, height, max_samples=16, seed=0, max_depth=8, clamp_indirect=0.0, pixel_filter=0):
+    def render(self, width, height, max_samples=16, seed=0, max_depth=8, clamp_indirect=0.0, pixel_filter=0,
+               caustic_photons=0, caustic_radius=0.0):
         payload = (_u(width, height, max_samples, seed) + _f(0.0, 0.0) + _u(1, max_depth) + _f(clamp_indirect)
-                   + _u(0, 0, 0) + _f(0.0, 1.0) + _u(pixel_filter, 1))
+                   + _u(0, 0, caustic_photons) + _f(caustic_radius, 1.0) + _u(pixel_filter, 1))
         self.sections.append(_section(b"RNDR", payload))
         return self
 
