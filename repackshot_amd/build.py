@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librpx.so")
 SOURCES = ["rpx_kernels.hip", "rpx_engine.hip", "rpx_scene.hip", "rpx_capi.hip", "rpx_sobol.hip", "rpx_generators.hip",
-           "rpx_adaptive.hip", "rpx_denoise.hip"]
+           "rpx_adaptive.hip", "rpx_denoise.hip", "rpx_caustics.hip"]
 
 FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",

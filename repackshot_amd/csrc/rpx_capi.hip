@@ -10,6 +10,7 @@
 #include <thread>
 #include "../../include/rpx.h"
 #include "rpx_adaptive.h"
+#include "rpx_caustics.h"
 #include "rpx_denoise.h"
 #include "rpx_engine.h"
 
@@ -29,6 +30,7 @@ struct Handle {
   std::vector<DevBuf> scene_bufs;
   u64 triangles = 0, nodes = 0;
   std::unique_ptr<Engine> engine;
+  std::unique_ptr<PhotonPasses> photons;  // a scene with caustics: its photon maps' memory
   // resident film
   Film film{};
   std::vector<DevBuf> film_bufs;
@@ -55,6 +57,7 @@ struct Handle {
     denoiser.reset();
     dev_free_all(pass_bufs);
     engine.reset();
+    photons.reset();
     dev_free_all(scene_bufs);
     dev_free_all(film_bufs);
     dev_free_all(tex_bufs);
@@ -106,6 +109,8 @@ bool build_device_scene(Handle* h, std::string& err) {
     return false;
   }
   hipStreamSynchronize(h->stream);
+  if (h->engine) h->engine->photons = nullptr;
+  h->photons.reset();
   dev_free_all(h->scene_bufs);
   DevScene d = flat.consts;
   auto& B = h->scene_bufs;
@@ -121,6 +126,16 @@ bool build_device_scene(Handle* h, std::string& err) {
       !upload(B, flat.emitters, d.emitters, st, err) || !upload(B, flat.emit_tris, d.emit_tris, st, err) ||
       !upload(B, flat.emit_cdf, d.emit_cdf, st, err) || !upload(B, flat.instance_emitter, d.instance_emitter, st, err))
     return false;
+  d.pmap = nullptr;
+  if (d.caustics_on) {
+    if (!upload(B, flat.caustic_sources, d.caustics.sources, st, err) ||
+        !upload(B, flat.caustic_cdf, d.caustics.source_cdf, st, err))
+      return false;
+    auto maps = std::make_unique<PhotonPasses>();
+    if (!maps->init(d.caustics.photons, d.caustics.radius, st, err)) return false;
+    d.pmap = maps->header();
+    h->photons = std::move(maps);
+  }
   d.tables = tables->device_view();
   std::vector<DevScene> one(1, d);
   const DevScene* ds = nullptr;
@@ -135,6 +150,15 @@ bool build_device_scene(Handle* h, std::string& err) {
   h->triangles = flat.triangles;
   h->nodes = flat.nodes.size();
   return true;
+}
+
+// Points the handle's engine at what its scene needs (BSDF records per slot, the extended
+// shade, the photon maps of a scene with caustics, by sample index).
+void prepare_engine(Handle* h) {
+  h->engine->textured = h->dev.tex_material != NO_TEXTURE;
+  h->engine->ext = h->dev.ext != 0;
+  h->engine->photons = h->photons.get();
+  h->engine->photon_pass = -1;
 }
 
 // Copies the handle's scene constants to its one-entry device scene table.
@@ -374,8 +398,9 @@ uint32_t rt_scene_free(uint32_t handle) {
   return RPX_OK;
 }
 
+// `photon_pass` >= 0: the photon map's pass (else the sample index's).
 static uint32_t render_tile_impl(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t hh,
-                                 uint32_t sample_index, float* out, u32 channels) {
+                                 uint32_t sample_index, float* out, u32 channels, long long photon_pass = -1) {
   if (!out) return RPX_NULL_POINTER;
   auto h = find(handle);
   if (!h) return RPX_BAD_HANDLE;
@@ -402,8 +427,8 @@ static uint32_t render_tile_impl(uint32_t handle, uint32_t x, uint32_t y, uint32
     set_error("device memory exhausted for the tile");
     return RPX_GPU_MEMORY;
   }
-  h->engine->textured = h->dev.tex_material != NO_TEXTURE;
-  h->engine->ext = h->dev.ext != 0;
+  prepare_engine(h.get());
+  h->engine->photon_pass = photon_pass;
   hipError_t e = h->engine->run(h->d_scene, std::vector<DevSegment>(1, s), false, err);
   if (e == hipSuccess) e = launch_tile_out(h->engine->P, d_out, cw, ch, channels, h->stream);
   std::vector<float> tmp((size_t)cw * ch * channels);
@@ -477,8 +502,7 @@ uint32_t rpx_render_tile_noalloc(uint32_t handle, uint32_t x, uint32_t y, uint32
   s.first = sample_index;
   s.count = 1;
   s.film_w = W;
-  h->engine->textured = h->dev.tex_material != NO_TEXTURE;
-  h->engine->ext = h->dev.ext != 0;
+  prepare_engine(h.get());
   hipError_t e = h->engine->render_tile_in(h->d_scene, s, 3, region, NOALLOC_SCRATCH_BYTES, out, (size_t)w * 3, err);
   if (e != hipSuccess) {
     set_error(err);
@@ -497,10 +521,9 @@ uint32_t rt_render_tile_rgba(uint32_t handle, uint32_t x, uint32_t y, uint32_t w
 }
 uint32_t rt_render_block_rgba(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t h, uint32_t sample_index,
                               uint32_t iteration, float* out) {
-  // Scenes with photon maps or edge supersampling are refused at load, so a block of a round is a
-  // tile and `iteration` (the photon map's) selects nothing.
-  (void)iteration;
-  return render_tile_impl(handle, x, y, w, h, sample_index, out, 4);
+  // Scenes with edge supersampling are refused at load, so a block of a round is a tile;
+  // `iteration` selects the photon map of a scene with caustics.
+  return render_tile_impl(handle, x, y, w, h, sample_index, out, 4, (long long)iteration);
 }
 
 // ---------------------------------------------------------------- adaptive sampling
@@ -532,6 +555,10 @@ uint32_t rpx_render_adaptive(uint32_t handle, uint32_t cap, uint64_t* out8) {
     set_error("rpx_render_adaptive: neither a cap nor the scene's max_samples bounds the rounds");
     return RPX_BAD_TILE;
   }
+  if (h->photons) {
+    set_error("rpx_render_adaptive: a scene with caustics renders its samples pass by pass (rpx_render_samples)");
+    return RPX_BAD_SCENE;
+  }
   std::string err;
   u32 W = h->dev.width, H = h->dev.height;
   adaptive::State ad;
@@ -547,8 +574,7 @@ uint32_t rpx_render_adaptive(uint32_t handle, uint32_t cap, uint64_t* out8) {
   hipMemsetAsync(F.weight, 0, px * 4, st);
   hipMemsetAsync(F.alpha, 0, px * 4, st);
   hipMemsetAsync(F.lum2, 0, px * 4, st);
-  h->engine->textured = h->dev.tex_material != NO_TEXTURE;
-  h->engine->ext = h->dev.ext != 0;
+  prepare_engine(h.get());
   hipEvent_t ev[2];
   hipEventCreate(&ev[0]);
   hipEventCreate(&ev[1]);
@@ -707,8 +733,7 @@ uint32_t rpx_render_samples(uint32_t handle, uint32_t first, uint32_t count) {
     set_error(err);
     return RPX_GPU_MEMORY;
   }
-  h->engine->textured = h->dev.tex_material != NO_TEXTURE;
-  h->engine->ext = h->dev.ext != 0;
+  prepare_engine(h.get());
   hipError_t e = h->engine->run(h->d_scene, std::vector<DevSegment>(1, whole_image(h.get(), 0, first, count)), true, err);
   if (e != hipSuccess) {
     set_error(err);
@@ -879,8 +904,7 @@ uint32_t rpx_autotune(uint32_t handle, uint32_t samples, float* out60) {
   DevSegment s = whole_image(h.get(), 0, 0, count);
   std::vector<float> ms;
   std::string err;
-  h->engine->textured = h->dev.tex_material != NO_TEXTURE;
-  h->engine->ext = h->dev.ext != 0;
+  prepare_engine(h.get());
   hipError_t r = h->engine->autotune(h->d_scene, std::vector<DevSegment>(1, s), ms, err);
   return tune_report(*h->engine, r, ms, err, out60);
 }
@@ -1050,7 +1074,8 @@ uint32_t rpx_adjoint(uint32_t handle, uint32_t first, uint32_t count, const floa
   if (count == 0) return RPX_OK;
   std::lock_guard<std::mutex> lock(h->mutex);
   if (h->dev.ext) {
-    set_error("rpx_adjoint: the replay does not follow sphere lights or scattering media");
+    set_error("rpx_adjoint: the replay follows area lights and the environment only (no sphere lights, suns, "
+              "emissive meshes sampled as lights, caustics or scattering media)");
     return RPX_BAD_SCENE;
   }
   hipSetDevice(h->device);
@@ -1133,6 +1158,10 @@ uint32_t rpx_queue_add(uint32_t queue, uint32_t scene, uint32_t first, uint32_t 
   j.count = count;
   {
     std::lock_guard<std::mutex> hl(h->mutex);
+    if (h->photons) {
+      set_error("rpx_queue_add: a scene with caustics renders its samples pass by pass, not in a queue");
+      return RPX_BAD_SCENE;
+    }
     j.width = h->dev.width;
     j.height = h->dev.height;
   }

@@ -1,0 +1,220 @@
+// The photon map of a pass on the GPU (caustics.rs:737-860): the traced photons get the bucket of
+// their grid cell, a stable radix sort by bucket puts them in bucket order with trace (index)
+// order kept inside a bucket, and each bucket's start is found by bisection of the sorted keys.
+#include "rpx_caustics.h"
+
+#include <hipcub/hipcub.hpp>
+#include <cmath>
+
+namespace rpx {
+namespace {
+
+constexpr u32 MAX_BUCKETS = 1u << 24;
+constexpr u32 KEY_BITS = 25;  // buckets below 2^24 and PHOTON_NONE = 2^24
+constexpr float MM_TO_M = 1.0e-3f;
+constexpr float RADIUS_ALPHA = 2.0f / 3.0f;
+
+__global__ void k_map_reset(u32* stored) { *stored = 0; }
+
+__global__ void __launch_bounds__(256) k_map_iota(u32* index, u32 n) {
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) index[i] = i;
+}
+
+// caustics.rs:807-818: a table of a power of two of at least twice the stored photons' buckets
+// (at most 2^24), and the pass's constants.
+__global__ void k_map_header(DevPhotonMap* M, const u32* stored, u32 iteration, float radius, float inv_r2, float norm,
+                             float inv_cell) {
+  u32 n = *stored;
+  u32 want = (n > 1u ? n : 1u) * 2u;
+  u32 buckets = 1;
+  while (buckets < want && buckets < MAX_BUCKETS) buckets <<= 1;
+  M->count = n;
+  M->mask = buckets - 1;
+  M->iteration = iteration;
+  M->radius = radius;
+  M->inv_r2 = inv_r2;
+  M->norm = norm;
+  M->inv_cell = inv_cell;
+}
+
+// Each stored photon's bucket (caustics.rs:820-830).
+__global__ void __launch_bounds__(256) k_map_keys(PhotonBuffers B, const DevPhotonMap* M, u32 n) {
+  const u32 mask = M->mask;
+  const float inv_cell = M->inv_cell;
+  for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (B.key[i] != PHOTON_STORED) continue;
+    float4 p = B.position[i];
+    i32 c[3];
+    photon_cell(v3(p.x, p.y, p.z), inv_cell, c);
+    B.key[i] = photon_bucket(c, mask);
+  }
+}
+
+// The stored photons' records in bucket order.
+__global__ void __launch_bounds__(256) k_map_permute(const DevPhotonMap* M, const u32* order, PhotonBuffers B,
+                                                    float4* pos, float4* wi, float4* power, float4* normal) {
+  const u32 n = M->count;
+  for (u32 k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    u32 s = order[k];
+    pos[k] = B.position[s];
+    wi[k] = B.wi[s];
+    power[k] = B.power[s];
+    normal[k] = B.normal[s];
+  }
+}
+
+// starts[b] = the first sorted photon whose bucket is at least b, for b in [0, mask + 1] (the last
+// is the stored count: every unstored key is above every bucket).
+__global__ void __launch_bounds__(256) k_map_starts(const DevPhotonMap* M, const u32* keys, u32 n, u32* starts,
+                                                   u32 cap) {
+  const u32 last = M->mask + 1;
+  for (u32 b = blockIdx.x * blockDim.x + threadIdx.x; b <= last && b <= cap; b += gridDim.x * blockDim.x) {
+    u32 lo = 0, hi = n;
+    while (lo < hi) {
+      u32 mid = (lo + hi) >> 1;
+      if (keys[mid] < b) lo = mid + 1;
+      else hi = mid;
+    }
+    starts[b] = lo;
+  }
+}
+
+inline u32 grid_of(u32 n, u32 block, u32 cap) {
+  u32 g = (n + block - 1) / block;
+  return g < 1 ? 1 : (g > cap ? cap : g);
+}
+
+inline size_t round_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+float PhotonPasses::radius_of(float first, u32 iteration) {
+  float r2 = first * first;
+  for (u32 k = 1; k <= iteration; ++k) r2 *= ((float)k + RADIUS_ALPHA) / ((float)k + 1.0f);
+  return std::sqrt(r2);
+}
+
+PhotonPasses::~PhotonPasses() {
+  if (block_) (void)hipFree(block_);
+}
+
+bool PhotonPasses::init(u32 photons, float first_radius, hipStream_t st, std::string& err) {
+  if (photons == 0) {
+    err = "a scene with caustics traces at least one photon per pass";
+    return false;
+  }
+  photons_ = photons;
+  first_radius_ = first_radius;
+  u32 buckets = 1;
+  while (buckets < 2u * photons && buckets < MAX_BUCKETS) buckets <<= 1;
+  buckets_cap_ = buckets;
+  size_t n = photons;
+  sort_bytes_ = 0;
+  if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes_, (u32*)nullptr, (u32*)nullptr, (u32*)nullptr,
+                                         (u32*)nullptr, (int)photons, 0, (int)KEY_BITS, st) != hipSuccess) {
+    err = "photon map: the sort's workspace query failed";
+    return false;
+  }
+  // Layout of the one block: header, raw records (4 x float4) and keys, bucket-ordered records,
+  // sorted keys, index, sorted index, starts, the sort's workspace, the stored counter.
+  size_t sizes[] = {round_up(sizeof(DevPhotonMap)),
+                    round_up(16 * n), round_up(16 * n), round_up(16 * n), round_up(16 * n), round_up(4 * n),
+                    round_up(16 * n), round_up(16 * n), round_up(16 * n), round_up(16 * n),
+                    round_up(4 * n), round_up(4 * n), round_up(4 * n), round_up(4 * ((size_t)buckets + 1)),
+                    round_up(sort_bytes_ ? sort_bytes_ : 1), round_up(4)};
+  size_t total = 0;
+  for (size_t s : sizes) total += s;
+  if (hipMalloc(&block_, total) != hipSuccess) {
+    block_ = nullptr;
+    err = "photon map: out of device memory";
+    return false;
+  }
+  char* p = (char*)block_;
+  auto take = [&](int k) {
+    char* q = p;
+    p += sizes[k];
+    return (void*)q;
+  };
+  header_ = (DevPhotonMap*)take(0);
+  raw_.position = (float4*)take(1);
+  raw_.wi = (float4*)take(2);
+  raw_.power = (float4*)take(3);
+  raw_.normal = (float4*)take(4);
+  raw_.key = (u32*)take(5);
+  pos_ = (float4*)take(6);
+  wi_ = (float4*)take(7);
+  power_ = (float4*)take(8);
+  normal_ = (float4*)take(9);
+  keys_sorted_ = (u32*)take(10);
+  index_ = (u32*)take(11);
+  index_sorted_ = (u32*)take(12);
+  starts_ = (u32*)take(13);
+  sort_tmp_ = take(14);
+  raw_.stored = (u32*)take(15);
+  // An empty map until the first pass: two buckets, both empty.
+  DevPhotonMap h{};
+  h.count = 0;
+  h.mask = 1;
+  h.radius = first_radius;
+  h.inv_r2 = 1.0f / (first_radius * first_radius);
+  h.inv_cell = 1.0f / (2.0f * first_radius);
+  float r_m = first_radius * MM_TO_M;
+  h.norm = 2.0f / (PI * r_m * r_m);
+  h.position = pos_;
+  h.wi = wi_;
+  h.power = power_;
+  h.normal = normal_;
+  h.starts = starts_;
+  u32 zeros[3] = {0, 0, 0};
+  if (hipMemcpyAsync(header_, &h, sizeof(h), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(starts_, zeros, sizeof(zeros), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    err = "photon map: upload failed";
+    return false;
+  }
+  hipLaunchKernelGGL(k_map_iota, dim3(grid_of(photons, 256, 4096)), dim3(256), 0, st, index_, photons);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+    err = "photon map: initialisation failed";
+    return false;
+  }
+  return true;
+}
+
+hipError_t PhotonPasses::build(const DevScene* d_scene, u32 iteration, hipStream_t st, std::string& err) {
+  float radius = radius_of(first_radius_, iteration);
+  float r_m = radius * MM_TO_M;
+  float inv_cell = 1.0f / (2.0f * radius);
+  float inv_r2 = 1.0f / (radius * radius);
+  float norm = 2.0f / (PI * r_m * r_m);
+  hipLaunchKernelGGL(k_map_reset, dim3(1), dim3(1), 0, st, raw_.stored);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = launch_photons(d_scene, photons_, raw_, iteration, st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_map_header, dim3(1), dim3(1), 0, st, header_, raw_.stored, iteration, radius, inv_r2, norm,
+                       inv_cell);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_map_keys, dim3(grid_of(photons_, 256, 8192)), dim3(256), 0, st, raw_, header_, photons_);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    size_t bytes = sort_bytes_;
+    e = hipcub::DeviceRadixSort::SortPairs(sort_tmp_, bytes, raw_.key, keys_sorted_, index_, index_sorted_,
+                                           (int)photons_, 0, (int)KEY_BITS, st);
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_map_permute, dim3(grid_of(photons_, 256, 8192)), dim3(256), 0, st, header_, index_sorted_,
+                       raw_, pos_, wi_, power_, normal_);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_map_starts, dim3(grid_of(buckets_cap_ + 1, 256, 8192)), dim3(256), 0, st, header_,
+                       keys_sorted_, photons_, starts_, buckets_cap_);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) err = std::string("photon map: ") + hipGetErrorString(e);
+  return e;
+}
+
+}  // namespace rpx

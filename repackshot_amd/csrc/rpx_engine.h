@@ -7,6 +7,8 @@
 
 namespace rpx {
 
+class PhotonPasses;
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
@@ -98,6 +100,11 @@ class Engine {
   bool ray_counts(u64 out[4]);
 
   EngineOptions opt;
+  // A scene with caustics: the photon map of each sample pass is traced and built on the engine's
+  // stream before the pass, and the passes run one at a time. `photon_pass` >= 0 names the map's
+  // pass for every sample instead of the sample index.
+  PhotonPasses* photons = nullptr;
+  long long photon_pass = -1;
   bool profiling = false;
   bool textured = false;  // a scene of the next runs has a texture (slots get a BSDF record)
   bool ext = false;       // a scene of the next runs has sphere lights or scattering media

@@ -10,6 +10,7 @@
 #include <map>
 #include <mutex>
 #include "rpx_engine.h"
+#include "rpx_caustics.h"
 
 namespace rpx {
 
@@ -419,7 +420,10 @@ hipError_t Engine::render_tile_in(const DevScene* d_scenes, const DevSegment& se
   timing = Timing{};
   DevSegment s = seg;
   s.base = 0;
-  hipError_t e = run_chunk(d_scenes, &s, 1, n, false, false, false, table);
+  hipError_t e = hipSuccess;
+  // The photon map's memory is the scene's, set aside when it was loaded.
+  if (photons) e = photons->build(d_scenes + s.scene, photon_pass >= 0 ? (u32)photon_pass : s.first, stream_, err);
+  if (e == hipSuccess) e = run_chunk(d_scenes, &s, 1, n, false, false, false, table);
   if (e == hipSuccess) e = launch_tile_out(P, staging, seg.w, seg.h, channels, stream_);
   if (e == hipSuccess)
     e = hipMemcpy2DAsync(out, out_row_floats * 4, staging, (size_t)seg.w * channels * 4, (size_t)seg.w * channels * 4,
@@ -485,7 +489,11 @@ hipError_t Engine::run(const DevScene* d_scenes, const std::vector<DevSegment>& 
   marks_.clear();
   std::vector<std::vector<DevSegment>> chunks;
   std::vector<u32> totals;
-  if (!plan(segs, chunks, totals, err)) return hipErrorInvalidValue;
+  const bool keep_per_pass = opt.per_pass;
+  if (photons) opt.per_pass = true;  // each pass has its own photon map
+  bool planned = plan(segs, chunks, totals, err);
+  opt.per_pass = keep_per_pass;
+  if (!planned) return hipErrorInvalidValue;
   if (!splat && chunks.size() > 1) {
     err = "unsplatted results must fit one chunk";
     return hipErrorInvalidValue;
@@ -499,8 +507,14 @@ hipError_t Engine::run(const DevScene* d_scenes, const std::vector<DevSegment>& 
     return hipErrorOutOfMemory;
   hipEventRecord(ev_[0], stream_);
   hipError_t e = hipSuccess;
-  for (size_t c = 0; c < chunks.size() && e == hipSuccess; ++c)
+  for (size_t c = 0; c < chunks.size() && e == hipSuccess; ++c) {
+    if (photons) {
+      const DevSegment& s = chunks[c][0];
+      e = photons->build(d_scenes + s.scene, photon_pass >= 0 ? (u32)photon_pass : s.first, stream_, err);
+      if (e != hipSuccess) break;
+    }
     e = run_chunk(d_scenes, chunks[c].data(), (u32)chunks[c].size(), totals[c], splat, false, disjoint);
+  }
   hipEventRecord(ev_[1], stream_);
   if (e == hipSuccess) e = hipStreamSynchronize(stream_);
   if (e != hipSuccess) {
@@ -607,6 +621,19 @@ hipError_t Engine::autotune(const DevScene* d_scenes, const std::vector<DevSegme
   opt.per_pass = false;
   opt.graph = false;
   opt.chunk = RESULTS_MAX;  // the probe's results stay unsplatted: one chunk
+  // A scene with caustics: the candidates render the probe against one photon map, built first.
+  PhotonPasses* keep_photons = photons;
+  if (photons && !probe.empty()) {
+    hipError_t b = photons->build(d_scenes + probe[0].scene, photon_pass >= 0 ? (u32)photon_pass : probe[0].first,
+                                  stream_, err);
+    if (b == hipSuccess) b = hipStreamSynchronize(stream_);
+    if (b != hipSuccess) {
+      opt = keep;
+      profiling = keep_profiling;
+      return b;
+    }
+  }
+  photons = nullptr;
   ms.assign(N_TUNE_CANDIDATES, 0.0f);  // 0: the candidate did not fit in device memory
   float best = 0.0f;
   EngineOptions chosen = keep;
@@ -657,6 +684,7 @@ hipError_t Engine::autotune(const DevScene* d_scenes, const std::vector<DevSegme
   }
   profiling = keep_profiling;
   opt = keep;
+  photons = keep_photons;
   // The probes grew the pool to the largest candidate; keep no more than the chosen size.
   if (capacity_ > chosen.pool) release_pool();
   if (e != hipSuccess) return e;

@@ -24,10 +24,14 @@ namespace rpx {
 // was a medium's exit.
 constexpr u32 FLAG_PRIMARY = 1, FLAG_HAS_PDF = 2, FLAG_MEDIUM = 4, FLAG_BY_FLUX = 8, FLAG_SCATTER = 16,
               FLAG_EXIT_PDF = 32, FLAG_WALKING = 64;
+// Where a path of a scene with caustics is in the split between the photon map and the paths
+// (integrator/mod.rs:1084-1113): FLAG_CAU_RECV after a receiver (a surface that is not a caster),
+// with FLAG_CAU_CASTERS once a caster followed it; neither: the eye state.
+constexpr u32 FLAG_CAU_RECV = 128, FLAG_CAU_CASTERS = 1u << 20;
 // FLAG_WALKING: the ray is a step of a walk in progress (its state in walk_w / walk_r); the shade
 // passes the path on untouched and the next extend traces the step.
 constexpr u32 EVENTS_SHIFT = 8, EVENTS_MASK = 0xfffu;  // flags bits 8..19: medium steps of the path so far
-constexpr u32 BOUNCE_SHIFT = 20;  // flags bits 20..31: the bounce of the path's next vertex
+constexpr u32 BOUNCE_SHIFT = 21;  // flags bits 21..31: the bounce of the path's next vertex (at most 1025)
 // How a walk through a scattering medium ended (hit.w, written by the walk kernel; 0: no walk).
 constexpr u32 WALK_CROSSED = 1, WALK_EXIT = 2, WALK_ESCAPED = 3, WALK_ABSORBED = 4;
 constexpr u32 NO_WORK = 0xffffffffu;
@@ -178,6 +182,20 @@ hipError_t launch_passes(const DevScene* scene, const u32* sobol, const PassPlan
 hipError_t launch_passes_resolve(const PassPlanes& Q, u32 n, float* albedo, float* alpha, hipStream_t st);
 hipError_t launch_energy(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_in, uint16_t* clear_out, float* albedo_avg,
                          float* schlick_avg, hipStream_t st);
+
+// The photons of one pass by photon index, before the map is built: where each was stored
+// (xyz, casters crossed in the bits of w), the direction it came from, its power (W), the normal
+// of the surface it was stored on, and its key (PHOTON_STORED or PHOTON_NONE; the map builder
+// turns a stored one's key into its bucket). `stored` counts the stored photons.
+struct PhotonBuffers {
+  float4 *position, *wi, *power, *normal;
+  u32* key;
+  u32* stored;
+};
+constexpr u32 PHOTON_STORED = 0, PHOTON_NONE = 1u << 24;  // above every bucket (at most 2^24)
+// Traces photons [0, scene.caustics.photons) of pass `iteration` (caustics.rs:581-735). The
+// caller zeroes `stored` first.
+hipError_t launch_photons(const DevScene* scene, u32 photons, const PhotonBuffers& B, u32 iteration, hipStream_t st);
 
 // The Sobol generator matrices (Joe–Kuo new-joe-kuo-6.21201, 64 dimensions) in the reference
 // sampler's logical-dimension layout: column k of logical dimension d at [d * 32 + k].

@@ -1140,6 +1140,331 @@ __device__ __forceinline__ float dot3(C3 a, C3 b) { return a.r * b.r + a.g * b.g
 // `g_prev` the term of the previous vertex's BSDF sample (it needs the radiance the path gathered
 // after it, the forward's total minus what is summed so far), `g_now` the next-event term of this
 // vertex when its light casts no shadow; a shadowed next-event term waits for the shadow kernel.
+// The surface a ray (o, d) met at instance hit h (instance, mesh triangle): hit details
+// (geometry/mod.rs:731-808), the shading normal without a bevel (bevel.rs:232-252) and the
+// terminator lift (geometry/mod.rs:682-728).
+struct SurfaceHit {
+  V3 point, hit_point, normal, geometric;
+  bool front, outside;
+  u32 inst;
+};
+__device__ __forceinline__ SurfaceHit surface_hit(const DevScene& S, V3 o, V3 d, uint4 h) {
+  V3 s_point, s_hit_point, s_normal, s_geometric;
+  bool s_instance, s_front, s_outside;
+  u32 inst;
+  // Hit details (geometry/mod.rs:731-808) and the terminator lift (:682-728).
+  inst = h.x;
+  u32 tri = h.y;
+  const DevInstance& in = S.instances[inst];
+  const DevMesh& mesh = S.meshes[in.mesh];
+  bool ident = (in.flags & INST_IDENTITY) != 0;
+  const u32* idx = S.indices + mesh.index_offset + 3 * (size_t)tri;
+  u32 i0 = mesh.vertex_offset + idx[0], i1 = mesh.vertex_offset + idx[1], i2 = mesh.vertex_offset + idx[2];
+  const float* ps = S.positions;
+  V3 p0 = v3(ps[3 * (size_t)i0], ps[3 * (size_t)i0 + 1], ps[3 * (size_t)i0 + 2]);
+  V3 p1 = v3(ps[3 * (size_t)i1], ps[3 * (size_t)i1 + 1], ps[3 * (size_t)i1 + 2]);
+  V3 p2 = v3(ps[3 * (size_t)i2], ps[3 * (size_t)i2 + 1], ps[3 * (size_t)i2 + 2]);
+  V3 lo = ident ? o : xf_point(in.to_local, o), ld = ident ? d : xf_vector(in.to_local, d);
+  PRay r = prepare(lo, ld, RAY_EPSILON, RPX_INF);
+  float b0, b1, b2;
+  {
+    float u, v, w, az, bz, cz;
+    shear(r, p0 - lo, p1 - lo, p2 - lo, u, v, w, az, bz, cz);
+    bool negative = u < 0.0f || v < 0.0f || w < 0.0f;
+    bool positive = u > 0.0f || v > 0.0f || w > 0.0f;
+    float det = u + v + w;
+    bool ok = !(negative && positive) && det != 0.0f;
+    if (ok) {
+      float t = (u * az + v * bz + w * cz) / det;
+      ok = t >= r.t_min && t <= F32_MAX;
+    }
+    if (ok) {
+      b0 = u / det;
+      b1 = v / det;
+      b2 = w / det;
+    } else {
+      b0 = b1 = b2 = 1.0f / 3.0f;
+    }
+  }
+  V3 point_local = p0 * b0 + p1 * b1 + p2 * b2;
+  V3 wind = cross(p1 - p0, p2 - p0);
+  V3 winding = normalize(ident ? wind : xf_transpose(in.to_local, wind));
+  V3 incoming = normalize(d);
+  V3 geometric = length_sq(winding) > 0.0f ? winding : -incoming;
+  bool front_face = dot(winding, d) < 0.0f;
+  V3 smooth = v3(0.0f, 0.0f, 0.0f);
+  V3 n0, n1, n2;
+  if (mesh.has_normals) {
+    const float* ns = S.normals;
+    n0 = v3(ns[3 * (size_t)i0], ns[3 * (size_t)i0 + 1], ns[3 * (size_t)i0 + 2]);
+    n1 = v3(ns[3 * (size_t)i1], ns[3 * (size_t)i1 + 1], ns[3 * (size_t)i1 + 2]);
+    n2 = v3(ns[3 * (size_t)i2], ns[3 * (size_t)i2 + 1], ns[3 * (size_t)i2 + 2]);
+    V3 interp = n0 * b0 + n1 * b1 + n2 * b2;
+    smooth = ident ? interp : xf_transpose(in.to_local, interp);
+  }
+  V3 normal = length_sq(smooth) > 1.0e-12f ? normalize(smooth) : geometric;
+  V3 f_normal = dot(normal, d) > 0.0f ? -normal : normal;
+  V3 ng = dot(geometric, d) > 0.0f ? -geometric : geometric;
+  V3 hit_point = ident ? point_local : xf_point(in.to_world, point_local);
+  // bevel::shading_point without a bevel
+  V3 sn = f_normal;
+  if (dot(sn, ng) < 0.0f) sn = -sn;
+  if (dot(sn, d) >= -1.0e-4f) sn = ng;
+  // terminator_point
+  V3 tp = hit_point;
+  if (mesh.has_normals) {
+    V3 pt = p0 * b0 + p1 * b1 + p2 * b2;
+    V3 interp = n0 * b0 + n1 * b1 + n2 * b2;
+    V3 world_normal = ident ? interp : xf_transpose(in.to_local, interp);
+    float side = dot(world_normal, ng) < 0.0f ? -1.0f : 1.0f;
+    V3 lifted = pt;
+    V3 cs[3] = {p0, p1, p2};
+    V3 nsv[3] = {n0, n1, n2};
+    float ws[3] = {b0, b1, b2};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      V3 nk = normalize(nsv[k]) * side;
+      float below = dot(pt - cs[k], nk);
+      if (below < 0.0f) lifted = lifted - nk * (below * ws[k]);
+    }
+    if (finite3(lifted)) tp = ident ? lifted : xf_point(in.to_world, lifted);
+  }
+  s_point = tp;
+  s_hit_point = hit_point;
+  s_normal = sn;
+  s_geometric = ng;
+  s_instance = true;
+  s_front = front_face;
+  s_outside = front_face != (mesh.inside_out != 0);
+  (void)s_instance;
+  return SurfaceHit{s_point, s_hit_point, s_normal, s_geometric, s_front, s_outside, inst};
+}
+
+// ================================================================== caustic photons (caustics.rs)
+// pick (light.rs:1059-1074) on a short cdf: the first bucket whose end passes u.
+__device__ __forceinline__ u32 pick_bucket(const float* cdf, u32 n, float u) {
+  u32 k = 0;
+  while (k < n && cdf[k] <= u) ++k;
+  return k < n - 1 ? k : n - 1;
+}
+
+// caustics.rs:171-178
+__device__ __forceinline__ float cone_omc(float r, float d) {
+  float sin2 = (r * r) / (d * d);
+  float cos = sqrt_(fmax_r(1.0f - sin2, 0.0f));
+  return sin2 / (1.0f + cos);
+}
+
+// caustics.rs:495-559: a direction from x into the casters' cones and its density under the
+// mixture of every cone (`one_sided`: the emitting side n of a one-sided light).
+__device__ bool photon_towards(const DevCaustics& C, V3 x, bool one_sided, V3 n, float u_pick, float u0, float u1,
+                               V3& dir, float& pdf_out) {
+  u32 count = C.n_spheres;
+  V3 axes[CAUSTIC_MAX_SPHERES];
+  float omcs[CAUSTIC_MAX_SPHERES], cdf[CAUSTIC_MAX_SPHERES];
+  float total = -0.0f;
+  for (u32 i = 0; i < count; ++i) {
+    V3 d = C.sphere_c[i] - x;
+    float dist = length(d);
+    V3 axis = dist > 0.0f ? d / dist : v3(0.0f, 0.0f, 1.0f);
+    float omc, omega;
+    if (dist <= C.sphere_r[i]) {
+      omc = 0.0f;
+      omega = PI;
+    } else {
+      omc = cone_omc(C.sphere_r[i], dist);
+      omega = TAU * omc;
+    }
+    float facing = one_sided ? 0.05f + fmax_r(dot(n, axis), 0.0f) : 1.0f;
+    axes[i] = axis;
+    omcs[i] = omc;
+    cdf[i] = omega * facing;  // the weight, turned into the cdf below
+    total = total + cdf[i];
+  }
+  if (!(total > 0.0f && is_finite(total))) return false;
+  float acc = 0.0f;
+  for (u32 i = 0; i < count; ++i) {
+    acc += cdf[i];
+    cdf[i] = acc / total;
+  }
+  cdf[count - 1] = 1.0f;
+  u32 k = pick_bucket(cdf, count, u_pick);
+  if (omcs[k] > 0.0f) {
+    dir = normalize(to_world(frame_from_normal(axes[k]), uniform_cone(u0, u1, omcs[k])));
+  } else if (one_sided) {
+    dir = normalize(to_world(frame_from_normal(n), cosine_hemisphere(u0, u1)));
+  } else {
+    float z = 1.0f - 2.0f * u0;
+    float r = sqrt_(fmax_r(1.0f - z * z, 0.0f));
+    float sn, cs;
+    sin_cos(TAU * u1, sn, cs);
+    dir = v3(r * cs, r * sn, z);
+  }
+  float pdf = 0.0f;
+  for (u32 i = 0; i < count; ++i) {
+    float p = cdf[i] - (i == 0 ? 0.0f : cdf[i - 1]);
+    if (!(p > 0.0f)) continue;
+    if (omcs[i] > 0.0f) {
+      if (1.0f - dot(dir, axes[i]) <= omcs[i]) pdf += p / (TAU * omcs[i]);
+    } else {
+      pdf += p * (one_sided ? fmax_r(dot(dir, n), 0.0f) * FRAC_1_PI : 0.25f * FRAC_1_PI);
+    }
+  }
+  if (!(pdf > 0.0f && is_finite(pdf))) return false;
+  pdf_out = pdf;
+  return true;
+}
+
+// caustics.rs:369-491: where a photon leaves its light, its direction and its flux (W).
+__device__ bool photon_emit(const DevScene& S, Pcg32& rng, V3& o, V3& d, C3& flux) {
+  const DevCaustics& C = S.caustics;
+  float n = (float)C.photons;
+  u32 j = pick_bucket(C.source_cdf, C.n_sources, rng.next_f32());
+  float p_light = C.source_cdf[j] - (j == 0 ? 0.0f : C.source_cdf[j - 1]);
+  if (!(p_light > 0.0f)) return false;
+  float up0 = rng.next_f32();
+  float up1 = rng.next_f32();
+  float u_pick = rng.next_f32();
+  float ud0 = rng.next_f32();
+  float ud1 = rng.next_f32();
+  u32 src = C.sources[j], kind = src >> 24, idx = src & 0xffffffu;
+  if (kind == 0) {
+    const DevArea& L = S.areas[idx];
+    float x0, y0;
+    if (L.rect) {
+      x0 = (2.0f * up0 - 1.0f) * L.half_x;
+      y0 = (2.0f * up1 - 1.0f) * L.half_y;
+    } else {
+      float dx, dy;
+      concentric_disk(up0, up1, dx, dy);
+      x0 = dx * L.radius;
+      y0 = dy * L.radius;
+    }
+    V3 x = L.centre + L.frame.x * x0 + L.frame.y * y0;
+    float pdf_dir;
+    if (!photon_towards(C, x, true, L.frame.z, u_pick, ud0, ud1, d, pdf_dir)) return false;
+    float cos = dot(d, L.frame.z);
+    if (!(cos > 0.0f)) return false;
+    float pdf_area = 1.0f / (L.area * 1.0e-6f);
+    C3 rad = L.exponent == 0.0f ? L.radiance : L.radiance * fpow(cos, L.exponent);
+    flux = rad * (cos / (pdf_area * pdf_dir * p_light * n));
+    o = x;
+    return true;
+  }
+  if (kind == 2) {
+    const DevEmitter& E = S.emitters[idx];
+    const float* cdf = S.emit_cdf + E.first;
+    u32 t = pick_bucket(cdf, E.count, up0);
+    float lo = t == 0 ? 0.0f : cdf[t - 1];
+    float width = cdf[t] - lo;
+    float rest = width > 0.0f ? clampf((up0 - lo) / width, 0.0f, 0.99999994f) : 0.0f;
+    const float4* tp = S.emit_tris + 4 * (size_t)(E.first + t);
+    float su = sqrt_(rest);
+    float w0 = 1.0f - su, w1 = up1 * su;
+    float w2 = 1.0f - w0 - w1;
+    V3 x = (v3(tp[0].x, tp[0].y, tp[0].z) * w0 + v3(tp[1].x, tp[1].y, tp[1].z) * w1) + v3(tp[2].x, tp[2].y, tp[2].z) * w2;
+    V3 normal = v3(tp[3].x, tp[3].y, tp[3].z);
+    float pdf_dir;
+    if (!photon_towards(C, x, !E.two_sided, normal, u_pick, ud0, ud1, d, pdf_dir)) return false;
+    float facing = dot(d, normal);
+    if (!E.two_sided && facing <= 0.0f) return false;
+    float cos = fabs_(facing);
+    if (!(cos > 0.0f)) return false;
+    float pdf_area = 1.0f / (E.area * 1.0e-6f);
+    flux = E.radiance * (cos / (pdf_area * pdf_dir * p_light * n));
+    o = offset_origin(x, facing > 0.0f ? normal : -normal);
+    return true;
+  }
+  const DevSun& L = S.suns[idx];
+  V3 to_sun = L.frame.z;
+  float cos, pdf_dir;
+  if (L.delta) {
+    d = -to_sun;
+    cos = 1.0f;
+    pdf_dir = 1.0f;
+  } else {
+    V3 local = uniform_cone(ud0, ud1, L.one_minus_cos_max);
+    V3 dd = normalize(to_world(L.frame, local));
+    d = -dd;
+    cos = local.z;
+    pdf_dir = 1.0f / L.solid_angle;
+  }
+  V3 plane = C.scene_centre + to_sun * (2.0f * C.scene_radius);
+  float tilt = 0.0f;
+  if (!L.delta) {
+    float s2 = 2.0f * L.one_minus_cos_max - L.one_minus_cos_max * L.one_minus_cos_max;
+    tilt = sqrt_(fmax_r(s2, 0.0f)) * 4.0f * C.scene_radius;
+  }
+  float cdf[CAUSTIC_MAX_SPHERES];
+  float total = -0.0f;
+  for (u32 i = 0; i < C.n_spheres; ++i) {
+    cdf[i] = PI * (C.sphere_r[i] + tilt) * (C.sphere_r[i] + tilt);
+    total = total + cdf[i];
+  }
+  if (!(total > 0.0f && is_finite(total))) return false;
+  float acc = 0.0f;
+  for (u32 i = 0; i < C.n_spheres; ++i) {
+    acc += cdf[i];
+    cdf[i] = acc / total;
+  }
+  cdf[C.n_spheres - 1] = 1.0f;
+  u32 k = pick_bucket(cdf, C.n_spheres, u_pick);
+  Frame frame = frame_from_normal(to_sun);
+  float dx, dy;
+  concentric_disk(up0, up1, dx, dy);
+  float rk = C.sphere_r[k] + tilt;
+  V3 ck = C.sphere_c[k];
+  V3 x = ((ck - to_sun * dot(ck - plane, to_sun)) + frame.x * (dx * rk)) + frame.y * (dy * rk);
+  float pdf_area = 0.0f;
+  for (u32 i = 0; i < C.n_spheres; ++i) {
+    float ri = C.sphere_r[i] + tilt;
+    V3 ci = C.sphere_c[i];
+    if (length_sq(x - (ci - to_sun * dot(ci - plane, to_sun))) <= ri * ri)
+      pdf_area += (cdf[i] - (i == 0 ? 0.0f : cdf[i - 1])) / (PI * ri * ri * 1.0e-6f);
+  }
+  if (!(pdf_area > 0.0f)) return false;
+  flux = L.radiance * (cos / (pdf_area * pdf_dir * p_light * n));
+  o = x;
+  return true;
+}
+
+// caustics.rs:902-950: the radiance the photons around `point` send towards the camera, f giving
+// the BSDF of the visible point for a photon's direction (world space).
+template <class F>
+__device__ C3 photon_gather(const DevPhotonMap& M, V3 point, V3 normal, u32 max_bounces, F f) {
+  C3 sum = splat(0.0f);
+  if (M.count == 0) return sum;
+  float r = M.radius;
+  i32 lo[3], hi[3];
+  photon_cell(point - v3(r, r, r), M.inv_cell, lo);
+  photon_cell(point + v3(r, r, r), M.inv_cell, hi);
+  for (i32 cz = lo[2]; cz <= min(hi[2], lo[2] + 1); ++cz)
+    for (i32 cy = lo[1]; cy <= min(hi[1], lo[1] + 1); ++cy)
+      for (i32 cx = lo[0]; cx <= min(hi[0], lo[0] + 1); ++cx) {
+        i32 cell[3] = {cx, cy, cz};
+        u32 b = photon_bucket(cell, M.mask);
+        u32 end = M.starts[b + 1];
+        for (u32 k = M.starts[b]; k < end; ++k) {
+          float4 pp = M.position[k];
+          float4 pn = M.normal[k];
+          if (f2u(pp.w) > max_bounces || dot(v3(pn.x, pn.y, pn.z), normal) < 0.5f) continue;
+          V3 pos = v3(pp.x, pp.y, pp.z);
+          float d2 = length_sq(pos - point) * M.inv_r2;
+          i32 pc[3];
+          photon_cell(pos, M.inv_cell, pc);
+          if (d2 >= 1.0f || pc[0] != cx || pc[1] != cy || pc[2] != cz) continue;
+          float kern = M.norm * (1.0f - d2);
+          float4 pw = M.wi[k];
+          C3 fv = f(v3(pw.x, pw.y, pw.z));
+          if (peak(fv) > 0.0f) {
+            float4 po = M.power[k];
+            sum = sum + (fv * c3(po.x, po.y, po.z)) * kern;
+          }
+        }
+      }
+  return sum;
+}
+
 // EXT adds sphere, point and spot lights and the vertices after a walk through a scattering medium
 // (the walk kernel ran between extend and shade: hit.w says how it ended).
 template <bool ADJ, bool EXT>
@@ -1165,6 +1490,11 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
   const u32 bounce = flags >> BOUNCE_SHIFT;
   const u32 dims = bounce_dim(bounce);
   const u32 sample_index = P.ident[i].w;
+  // Caustics (integrator/mod.rs:1084-1113, 1261-1272): once a caster follows a receiver, the light
+  // of photon lights reaching the path is the photon map's.
+  bool caustics = false;
+  if constexpr (EXT) caustics = S.caustics_on != 0;
+  const bool dropped = caustics && (flags & FLAG_CAU_CASTERS) != 0;
   // After a walk that scattered (integrator/mod.rs:1209-1242): the exit lobe's albedo at the
   // surface it reached; an absorbed walk ends the path.
   bool exit = false;
@@ -1222,9 +1552,9 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
     thr = thr * c3(fexp(-sg.r * t_nearest), fexp(-sg.g * t_nearest), fexp(-sg.b * t_nearest));
   }
   // Area lights crossed before the surface (light.rs:1354-1384); after a walk, the segment to its
-  // exit vertex crosses none.
+  // exit vertex crosses none, and none count once their light is the photon map's.
   C3 crossed = splat(0.0f);
-  for (u32 a = 0; a < (exit ? 0u : S.n_areas); ++a) {
+  for (u32 a = 0; a < (exit || dropped ? 0u : S.n_areas); ++a) {
     const DevArea& L = S.areas[a];
     bool visible = has_pdf ? L.glossy : L.camera;
     if (!visible) continue;
@@ -1285,8 +1615,9 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
       if (has_pdf && S.has_env_entry)
         sky = sky * power_heuristic(prev_pdf, sel_pmf<EXT>(S, from, S.env_entry) * env_pdf(S, d.z));
       if constexpr (EXT) {
-        // The sun discs a BSDF-sampled ray finds, weighted against next-event estimation.
-        for (u32 k = 0; k < S.n_suns; ++k) {
+        // The sun discs a BSDF-sampled ray finds, weighted against next-event estimation (none
+        // once their light is the photon map's: light.rs:1522-1533).
+        for (u32 k = 0; k < (dropped && has_pdf ? 0u : S.n_suns); ++k) {
           const DevSun& L = S.suns[k];
           bool visible = has_pdf ? L.glossy : L.camera;
           if (visible && sun_contains(L, d)) {
@@ -1312,90 +1643,15 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
       s_normal = nn;
       s_geometric = nn;
     } else {
-      // Hit details (geometry/mod.rs:731-808) and the terminator lift (:682-728).
-      inst = h.x;
-      u32 tri = h.y;
-      const DevInstance& in = S.instances[inst];
-      const DevMesh& mesh = S.meshes[in.mesh];
-      bool ident = (in.flags & INST_IDENTITY) != 0;
-      const u32* idx = S.indices + mesh.index_offset + 3 * (size_t)tri;
-      u32 i0 = mesh.vertex_offset + idx[0], i1 = mesh.vertex_offset + idx[1], i2 = mesh.vertex_offset + idx[2];
-      const float* ps = S.positions;
-      V3 p0 = v3(ps[3 * (size_t)i0], ps[3 * (size_t)i0 + 1], ps[3 * (size_t)i0 + 2]);
-      V3 p1 = v3(ps[3 * (size_t)i1], ps[3 * (size_t)i1 + 1], ps[3 * (size_t)i1 + 2]);
-      V3 p2 = v3(ps[3 * (size_t)i2], ps[3 * (size_t)i2 + 1], ps[3 * (size_t)i2 + 2]);
-      V3 lo = ident ? o : xf_point(in.to_local, o), ld = ident ? d : xf_vector(in.to_local, d);
-      PRay r = prepare(lo, ld, RAY_EPSILON, RPX_INF);
-      float b0, b1, b2;
-      {
-        float u, v, w, az, bz, cz;
-        shear(r, p0 - lo, p1 - lo, p2 - lo, u, v, w, az, bz, cz);
-        bool negative = u < 0.0f || v < 0.0f || w < 0.0f;
-        bool positive = u > 0.0f || v > 0.0f || w > 0.0f;
-        float det = u + v + w;
-        bool ok = !(negative && positive) && det != 0.0f;
-        if (ok) {
-          float t = (u * az + v * bz + w * cz) / det;
-          ok = t >= r.t_min && t <= F32_MAX;
-        }
-        if (ok) {
-          b0 = u / det;
-          b1 = v / det;
-          b2 = w / det;
-        } else {
-          b0 = b1 = b2 = 1.0f / 3.0f;
-        }
-      }
-      V3 point_local = p0 * b0 + p1 * b1 + p2 * b2;
-      V3 wind = cross(p1 - p0, p2 - p0);
-      V3 winding = normalize(ident ? wind : xf_transpose(in.to_local, wind));
-      V3 incoming = normalize(d);
-      V3 geometric = length_sq(winding) > 0.0f ? winding : -incoming;
-      bool front_face = dot(winding, d) < 0.0f;
-      V3 smooth = v3(0.0f, 0.0f, 0.0f);
-      V3 n0, n1, n2;
-      if (mesh.has_normals) {
-        const float* ns = S.normals;
-        n0 = v3(ns[3 * (size_t)i0], ns[3 * (size_t)i0 + 1], ns[3 * (size_t)i0 + 2]);
-        n1 = v3(ns[3 * (size_t)i1], ns[3 * (size_t)i1 + 1], ns[3 * (size_t)i1 + 2]);
-        n2 = v3(ns[3 * (size_t)i2], ns[3 * (size_t)i2 + 1], ns[3 * (size_t)i2 + 2]);
-        V3 interp = n0 * b0 + n1 * b1 + n2 * b2;
-        smooth = ident ? interp : xf_transpose(in.to_local, interp);
-      }
-      V3 normal = length_sq(smooth) > 1.0e-12f ? normalize(smooth) : geometric;
-      V3 f_normal = dot(normal, d) > 0.0f ? -normal : normal;
-      V3 ng = dot(geometric, d) > 0.0f ? -geometric : geometric;
-      V3 hit_point = ident ? point_local : xf_point(in.to_world, point_local);
-      // bevel::shading_point without a bevel
-      V3 sn = f_normal;
-      if (dot(sn, ng) < 0.0f) sn = -sn;
-      if (dot(sn, d) >= -1.0e-4f) sn = ng;
-      // terminator_point
-      V3 tp = hit_point;
-      if (mesh.has_normals) {
-        V3 pt = p0 * b0 + p1 * b1 + p2 * b2;
-        V3 interp = n0 * b0 + n1 * b1 + n2 * b2;
-        V3 world_normal = ident ? interp : xf_transpose(in.to_local, interp);
-        float side = dot(world_normal, ng) < 0.0f ? -1.0f : 1.0f;
-        V3 lifted = pt;
-        V3 cs[3] = {p0, p1, p2};
-        V3 nsv[3] = {n0, n1, n2};
-        float ws[3] = {b0, b1, b2};
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-          V3 nk = normalize(nsv[k]) * side;
-          float below = dot(pt - cs[k], nk);
-          if (below < 0.0f) lifted = lifted - nk * (below * ws[k]);
-        }
-        if (finite3(lifted)) tp = ident ? lifted : xf_point(in.to_world, lifted);
-      }
-      s_point = tp;
-      s_hit_point = hit_point;
-      s_normal = sn;
-      s_geometric = ng;
+      SurfaceHit sh = surface_hit(S, o, d, h);
+      s_point = sh.point;
+      s_hit_point = sh.hit_point;
+      s_normal = sh.normal;
+      s_geometric = sh.geometric;
       s_instance = true;
-      s_front = front_face;
-      s_outside = front_face != (mesh.inside_out != 0);
+      s_front = sh.front;
+      s_outside = sh.outside;
+      inst = sh.inst;
     }
   }
   u32 mat = 0xffffffffu;
@@ -1405,7 +1661,9 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
       const DevMaterial& M = S.materials[mat];
       if (M.emissive) {
         bool one_sided = (S.instances[inst].flags & INST_ONE_SIDED) != 0;
-        if (s_front || !one_sided) {
+        bool mapped = false;  // an emissive mesh sampled as a photon light, its light the map's
+        if constexpr (EXT) mapped = dropped && S.instance_emitter[inst] >= 0;
+        if ((s_front || !one_sided) && !mapped) {
           // A BSDF-sampled ray weighs an emitter sampled as a light against next-event estimation
           // (integrator/mod.rs:1548-1568, light.rs:1537-1551); others count at weight one.
           float weight = 1.0f;
@@ -1461,6 +1719,28 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
     V3 wo = to_local(frame, -d);
     V3 ng = s_geometric;
     V3 origin = offset_origin(s_point, ng);
+    // Caustics (integrator/mod.rs:1672-1694): every receiver gathers the photons (the first one's
+    // light direct, a later one's indirect); the state after this vertex says whether its light of
+    // photon lights is the map's.
+    u32 next_cflags = flags & (FLAG_CAU_RECV | FLAG_CAU_CASTERS);
+    if constexpr (EXT) {
+      if (caustics) {
+        bool caster = s_instance && (S.instances[inst].flags & INST_CASTER) != 0;
+        if (!caster) {
+          u32 budget = S.max_depth > bounce + 1 ? S.max_depth - (bounce + 1) : 0u;
+          V3 side = exit ? -s_geometric : s_geometric;
+          C3 c = photon_gather(*S.pmap, s_hit_point, side, budget, [&](V3 w) {
+            V3 wl = to_local(frame, w);
+            return exit ? exit_eval(exit_eta2, wo, wl) : bsdf_eval(T, B, wo, wl, false);
+          });
+          add_to(sum, thr * c, (flags & FLAG_CAU_RECV) != 0, S.clamp_indirect);
+          next_cflags = FLAG_CAU_RECV;
+        } else if (flags & FLAG_CAU_RECV) {
+          next_cflags = FLAG_CAU_RECV | FLAG_CAU_CASTERS;
+        }
+      }
+    }
+    const bool drop_nee = next_cflags == (FLAG_CAU_RECV | FLAG_CAU_CASTERS);
     // Light selection at this vertex (light.rs:1026-1046).
     Sel sel;
     sel.point = s_point;
@@ -1485,6 +1765,7 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
     float ls_dist = 0.0f, ls_pdf = 0.0f;
     C3 ls_rad = splat(0.0f);
     bool ls_glossy = true, ls_shadow = true, ls_delta = false;
+    bool ls_photon = false;  // a light whose photons the map holds
     if (ne > 0) {
       u32 index = 0;
       float p = 0.0f, rest = 0.0f;
@@ -1542,6 +1823,7 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
           ls_delta = L.delta != 0;
           ls_glossy = L.glossy != 0;
           ls_shadow = L.shadow != 0;
+          ls_photon = L.glossy != 0;
           have_ls = true;
         } else if (EXT && kind == E_MESH) {
           // light.rs:1270-1297: a triangle by area with the selection's remainder, a point
@@ -1574,6 +1856,7 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
             ls_pdf = p * dist2 / (E.area * cos);
             ls_glossy = true;
             ls_shadow = true;
+            ls_photon = true;
             have_ls = true;
           }
         } else if (EXT && kind == E_SPHERE) {
@@ -1636,6 +1919,7 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
             ls_pdf = p * dist2 / (L.area * cos);
             ls_glossy = L.glossy != 0;
             ls_shadow = L.shadow != 0;
+            ls_photon = L.glossy != 0;
             have_ls = true;
           }
         } else if (S.env_total_mass > 0.0f) {
@@ -1665,6 +1949,7 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
           }
         }
         if (have_ls && !(ls_pdf > 0.0f && is_finite(ls_pdf) && luminance(ls_rad) > 0.0f)) have_ls = false;
+        if (drop_nee && ls_photon) have_ls = false;
       }
     }
     if (have_ls) {
@@ -1745,7 +2030,7 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
       }
       if (ok) {
         V3 next_o = transmitted ? offset_origin(s_hit_point, -ng) : origin;
-        u32 nflags = FLAG_HAS_PDF | (sel.by_flux ? FLAG_BY_FLUX : 0u) | ((bounce + 1) << BOUNCE_SHIFT);
+        u32 nflags = FLAG_HAS_PDF | (sel.by_flux ? FLAG_BY_FLUX : 0u) | next_cflags | ((bounce + 1) << BOUNCE_SHIFT);
         C3 sg = c3(md.x, md.y, md.z);
         bool med = in_medium;
         bool scatter = false;
@@ -2297,6 +2582,128 @@ __global__ void __launch_bounds__(128) k_trace_rays(const DevScene* __restrict__
   }
 }
 
+// ================================================================== caustic photons: tracing
+namespace {
+// caustics.rs:581-735: photon `index` of pass `iteration` from its light through the casters to
+// the first receiver, where it is stored unless it crossed no caster. True when it was stored.
+template <int BLK, int LDSN>
+__device__ bool trace_photon(const DevScene& S, u32* col, const PhotonBuffers& B, u32 iteration, u32 index) {
+  const DevCaustics& C = S.caustics;
+  u64 key = ((u64)C.seed << 32) | (u64)iteration;
+  Pcg32 rng = Pcg32::seeded(mix64(key ^ 0x0e5b0001c0decafeull), (u64)index);
+  V3 o, d;
+  C3 power;
+  if (!photon_emit(S, rng, o, d, power)) return false;
+  if (!(peak(power) > 0.0f && is_finite(peak(power)))) return false;
+  Geo g{S.nodes, S.tris, S.instances, S.tlas_root};
+  float throughput = 1.0f;
+  u32 bounces = 0;
+  bool in_medium = false;
+  C3 sigma = splat(0.0f);
+  V3 at = v3(0.0f, 0.0f, 0.0f), normal_at = v3(0.0f, 0.0f, 1.0f);
+  bool stored = false;
+  for (u32 step = 0; step < C.max_bounces + 128; ++step) {
+    u32 hi = 0xffffffffu, ht = 0xffffffffu;
+    float th = RPX_INF;
+    bool has_hit = trace_nearest<BLK, LDSN>(g, col, o, d, RAY_EPSILON, RPX_INF, false, hi, ht, th);
+    float t_geometry = has_hit ? th : RPX_INF;
+    float t_floor = RPX_INF;
+    if (S.floor_mode == 1 && d.z != 0.0f) {
+      float tf = (S.floor_z - o.z) / d.z;
+      if (tf > RAY_EPSILON && tf < RPX_INF) t_floor = tf;
+    }
+    float t_nearest = fmin_r(t_geometry, t_floor);
+    if (in_medium && is_finite(t_nearest))
+      power = power * c3(fexp(-sigma.r * t_nearest), fexp(-sigma.g * t_nearest), fexp(-sigma.b * t_nearest));
+    if (t_nearest == RPX_INF) break;
+    if (t_floor < t_geometry) {
+      if (bounces > 0) {
+        V3 p = o + d * t_floor;
+        at = v3(p.x, p.y, S.floor_z);
+        normal_at = v3(0.0f, 0.0f, d.z < 0.0f ? 1.0f : -1.0f);
+        stored = true;
+      }
+      break;
+    }
+    SurfaceHit sh = surface_hit(S, o, d, make_uint4(hi, ht, f2u(th), 0u));
+    u32 mat = S.instances[sh.inst].material;
+    bool has_mat = mat < S.n_materials;
+    if (has_mat && S.materials[mat].emissive) break;
+    V3 ng = sh.geometric;
+    if (!(S.instances[sh.inst].flags & INST_CASTER)) {
+      // A receiver ends the photon (no thin walls in the supported scenes).
+      if (bounces > 0) {
+        at = sh.hit_point;
+        normal_at = ng;
+        stored = true;
+      }
+      break;
+    }
+    if (bounces >= C.max_bounces) break;
+    Frame frame = frame_from_normal(sh.normal);
+    V3 wo_world = -d;
+    V3 wo = to_local(frame, wo_world);
+    float u0 = rng.next_f32();
+    float u1 = rng.next_f32();
+    const Bsdf& Bs = has_mat ? S.bsdfs[2 * mat + (sh.outside ? 0 : 1)] : S.fallback_bsdf[sh.outside ? 0 : 1];
+    BsdfSample bs;
+    if (!bsdf_sample(S.tables, Bs, wo, u0, u1, bs)) break;
+    bool transmitted = bs.wi.z < 0.0f;
+    V3 wi = normalize(to_world(frame, bs.wi));
+    if (transmitted ? dot(wi, ng) >= 0.0f : dot(wi, ng) <= 0.0f) break;
+    C3 weight = bs.weight;
+    if (transmitted) weight = weight * (Bs.eta * Bs.eta);
+    // The adjoint BSDF's shading-normal correction (Veach), bounded.
+    float num = fabs_(dot(wo_world, sh.normal)) * fabs_(dot(wi, ng));
+    float den = fabs_(dot(wo_world, ng)) * fabs_(dot(wi, sh.normal));
+    if (den > 0.0f) weight = weight * fmin_r(num / den, 10.0f);
+    ++bounces;
+    power = power * weight;
+    throughput *= peak(weight);
+    if (bounces >= 3) {
+      float survive = clampf(throughput, 0.05f, 1.0f);
+      if (rng.next_f32() >= survive) break;
+      power = power * (1.0f / survive);
+      throughput /= survive;
+    }
+    if (!(peak(power) > 0.0f && is_finite(peak(power)))) break;
+    V3 next_o = transmitted ? offset_origin(sh.hit_point, -ng) : offset_origin(sh.point, ng);
+    if (transmitted) {
+      in_medium = false;
+      if (sh.outside && has_mat && S.materials[mat].has_medium == MEDIUM_ABSORBING) {
+        in_medium = true;
+        sigma = S.materials[mat].sigma;
+      }
+    }
+    o = next_o;
+    d = wi;
+  }
+  if (!stored) return false;
+  B.position[index] = make_float4(at.x, at.y, at.z, u2f(bounces));
+  B.wi[index] = make_float4(-d.x, -d.y, -d.z, 0.0f);
+  B.power[index] = make_float4(power.r, power.g, power.b, 0.0f);
+  B.normal[index] = make_float4(normal_at.x, normal_at.y, normal_at.z, 0.0f);
+  return true;
+}
+}  // namespace
+
+// One lane per photon; the stored ones are counted with one atomic per wave.
+template <int BLK, int LDSN>
+__global__ void __launch_bounds__(BLK) k_photons(const DevScene* __restrict__ scene, u32 photons, PhotonBuffers B,
+                                                 u32 iteration) {
+  __shared__ u32 lds[LDSN * BLK];
+  u32* col = lds + threadIdx.x;
+  const DevScene& S = *scene;
+  u32 stride = gridDim.x * blockDim.x;
+  for (u32 base = blockIdx.x * blockDim.x; base < photons; base += stride) {
+    u32 index = base + threadIdx.x;
+    bool stored = index < photons && trace_photon<BLK, LDSN>(S, col, B, iteration, index);
+    if (index < photons) B.key[index] = stored ? PHOTON_STORED : PHOTON_NONE;
+    u64 m = __ballot(stored);
+    if (m != 0 && __lane_id() == (u32)(__ffsll((long long)m) - 1)) atomicAdd(B.stored, (u32)__popcll(m));
+  }
+}
+
 // ================================================================== exact and sampled passes
 namespace {
 constexpr u32 FIRST_BACKGROUND = 0, FIRST_FLOOR = 1, FIRST_SURFACE = 2;
@@ -2833,6 +3240,12 @@ hipError_t launch_tile_out(const PathState& P, float* out, u32 w, u32 h, u32 cha
 hipError_t launch_trace_rays(const DevScene* scene, const float* rays, u32 n, u32* inst, u32* tri, float* t,
                              uint8_t* any, hipStream_t st) {
   hipLaunchKernelGGL(k_trace_rays, dim3(grid_for(n, 128, 8192)), dim3(128), 0, st, scene, rays, n, inst, tri, t, any);
+  return hipGetLastError();
+}
+hipError_t launch_photons(const DevScene* scene, u32 photons, const PhotonBuffers& B, u32 iteration, hipStream_t st) {
+  if (photons == 0) return hipSuccess;
+  hipLaunchKernelGGL((k_photons<128, 16>), dim3(grid_for(photons, 128, 1u << 16)), dim3(128), 0, st, scene, photons, B,
+                     iteration);
   return hipGetLastError();
 }
 hipError_t launch_passes(const DevScene* scene, const u32* sobol, const PassPlanes& Q, u32 x0, u32 y0, u32 w, u32 h,

@@ -33,7 +33,8 @@ struct alignas(16) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "Node4 is two cache lines");
 
-constexpr u32 INST_IDENTITY = 1, INST_ONE_SIDED = 2, INST_CAMERA_HIDDEN = 4, INST_ACTIVE = 8;
+// INST_CASTER: a caster of caustic photons (glass or metal that focuses light).
+constexpr u32 INST_IDENTITY = 1, INST_ONE_SIDED = 2, INST_CAMERA_HIDDEN = 4, INST_ACTIVE = 8, INST_CASTER = 16;
 
 struct alignas(16) DevInstance {
   float to_local[12];  // world -> object rows
@@ -174,6 +175,45 @@ RPX_HD EntryKind entry_kind(i32 a, u32& k) {
   return E_MESH;
 }
 
+// Caustics (caustics.rs:121-345): photons per sample index, the first gather radius (mm), the
+// casters' bounding spheres, the photon sources (kind << 24 | index; kinds 0 area light, 1 sun,
+// 2 emissive mesh) with their selection cdf, the scene's bounding sphere, the seed and the most
+// casters a photon crosses.
+constexpr u32 CAUSTIC_MAX_SPHERES = 16;
+struct DevCaustics {
+  u32 photons, n_spheres, n_sources, seed, max_bounces;
+  float radius, scene_radius;
+  V3 scene_centre;
+  V3 sphere_c[CAUSTIC_MAX_SPHERES];
+  float sphere_r[CAUSTIC_MAX_SPHERES];
+  const u32* sources;
+  const float* source_cdf;
+};
+
+// The photon map of one pass (caustics.rs:737-860): its stored photons in bucket order (trace
+// order within a bucket) and where each bucket starts (mask + 2 entries), in device memory.
+struct DevPhotonMap {
+  u32 count, mask, iteration, pad;
+  float radius, inv_r2, norm, inv_cell;
+  const float4* position;  // xyz, casters crossed (bits)
+  const float4* wi;
+  const float4* power;
+  const float4* normal;
+  const u32* starts;
+};
+
+// caustics.rs:760-790: the grid cell of a point (cells one diameter wide) and its bucket.
+RPX_HD void photon_cell(V3 p, float inv_cell, i32 c[3]) {
+  float v[3] = {p.x, p.y, p.z};
+  for (int k = 0; k < 3; ++k) {
+    float f = floor_(v[k] * inv_cell);
+    c[k] = is_finite(f) ? (i32)clampf(f, -1.0e9f, 1.0e9f) : 0;
+  }
+}
+RPX_HD u32 photon_bucket(const i32 c[3], u32 mask) {
+  return (((u32)c[0] * 73856093u) ^ ((u32)c[1] * 19349663u) ^ ((u32)c[2] * 83492791u)) & mask;
+}
+
 // A homogeneous scattering medium filling a clear solid (integrator/medium.rs:113-141): per mm
 // coefficients, the Henyey-Greenstein anisotropy, and the solid's index.
 struct DevMedium {
@@ -207,6 +247,11 @@ struct DevScene {
   const float* emit_cdf;          // per emitter triangle: cumulative area fraction within its emitter
   const i32* instance_emitter;    // per instance: its emitter, or -1
   u32 n_spheres, n_suns, n_emitters;
+  // Caustics: on when the scene asks for photons and has casters and photon sources; the map of
+  // the pass being rendered.
+  u32 caustics_on;
+  DevCaustics caustics;
+  const DevPhotonMap* pmap;
   // The scene has sphere lights, suns, emissive meshes sampled as lights or scattering media (the
   // kernels' extended variant).
   u32 ext;
@@ -351,6 +396,8 @@ struct HostScene {
   u32 adaptive = 0;
   float noise_threshold = 0;
   float clamp_indirect = 0;
+  u32 caustic_photons = 0;
+  float caustic_radius = 0;
 };
 
 // Parses the RSB subset the tracer follows; false with a message for anything else. With
@@ -375,6 +422,8 @@ struct FlatScene {
   std::vector<float4> emit_tris;
   std::vector<float> emit_cdf;
   std::vector<i32> instance_emitter;
+  std::vector<u32> caustic_sources;
+  std::vector<float> caustic_cdf;
   std::vector<DevMedium> media;
   std::vector<float> cdf, flux_pmf;
   std::vector<i32> entries;

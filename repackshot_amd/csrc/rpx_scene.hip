@@ -222,15 +222,16 @@ bool parse_rsb(const uint8_t* bytes, size_t len, HostScene& s, std::string& err,
       s.clamp_indirect = r.opt_f(0);
       (void)r.opt_u(0);
       (void)r.opt_u(0);
-      u32 photons = r.opt_u(0);
-      (void)r.opt_f(0);
+      s.caustic_photons = r.opt_u(0);
+      s.caustic_radius = r.opt_f(0);
       (void)r.opt_f(1);
       s.pixel_filter = r.opt_u(0);
       u32 edge = r.opt_u(1);
       // E10 (rsb.rs:1105-1110): the adaptive bit asks the host for rounds (rpx_render_adaptive).
       s.adaptive = r.opt_u(0);
       s.noise_threshold = r.opt_f(0);
-      if (photons) { err = "caustics are not supported by the GPU tracer"; return false; }
+      if (s.caustic_photons > (1u << 22)) { err = "caustic_photons must be at most 4194304"; return false; }
+      if (s.caustic_radius < 0.0f) { err = "caustic_radius must not be negative"; return false; }
       if (!(s.noise_threshold >= 0.0f && s.noise_threshold <= 1.0f)) {
         err = "noise_threshold must be between 0 and 1";
         return false;
@@ -666,6 +667,177 @@ void material_sides(const EnergyTables& tables, const HostMaterial& m, Bsdf out[
                              m.coat_roughness, m.flags, m.absorption_mm, side == 0);
 }
 
+namespace {
+// caustics.rs:104-119: glass or metal of roughness at most 0.25, not an emitter, not a scattering
+// medium (anisotropic materials are refused at load).
+bool is_caster(const HostMaterial& m) {
+  float r = m.roughness;
+  if (m.kind == 2 || r != r || r > 0.25f) return false;
+  DevMedium unused;
+  if (scatter_medium_of(m, unused)) return false;
+  bool glass = m.kind == MATERIAL_GLASS || (m.transmission >= 0.5f && m.metallic < 0.5f);
+  bool metal = m.kind != MATERIAL_GLASS && m.metallic >= 0.5f;
+  return glass || metal;
+}
+
+// The world box of an instance: its mesh box's eight corners transformed (caustics.rs:122-143).
+Box caustic_instance_box(const HostMesh& m, const float* rows) {
+  Box out;
+  if (m.bmin[0] > m.bmax[0] || m.bmin[1] > m.bmax[1] || m.bmin[2] > m.bmax[2]) return out;
+  Xf x = xf_from(rows);
+  for (u32 k = 0; k < 8; ++k) {
+    V3 c = v3((k & 1) == 0 ? m.bmin[0] : m.bmax[0], (k & 2) == 0 ? m.bmin[1] : m.bmax[1],
+              (k & 4) == 0 ? m.bmin[2] : m.bmax[2]);
+    out.grow(xf_point(x, c));
+  }
+  return out;
+}
+
+void sphere_of_box(const Box& b, V3& c, float& r) {
+  c = b.centroid();
+  r = fmax_r(0.5f * length(b.hi - b.lo), 1.0e-3f);
+}
+
+// caustics.rs:171-178
+float cone_one_minus_cos_host(float r, float d) {
+  float sin2 = (r * r) / (d * d);
+  float cos = sqrt_(fmax_r(1.0f - sin2, 0.0f));
+  return sin2 / (1.0f + cos);
+}
+
+std::vector<float> cdf_of_weights(const std::vector<float>& weights) {
+  std::vector<float> cdf;
+  float total = -0.0f;
+  for (float w : weights) total = total + w;
+  if (!(total > 0.0f && is_finite(total))) return cdf;
+  float acc = 0.0f;
+  for (float w : weights) {
+    acc += w;
+    cdf.push_back(acc / total);
+  }
+  cdf.back() = 1.0f;
+  return cdf;
+}
+
+void caustic_setup(const HostScene& s, const std::vector<HostMesh>& meshes, const Box& scene_bounds,
+                   FlatScene& out) {
+  DevScene& k = out.consts;
+  k.caustics_on = 0;
+  k.pmap = nullptr;
+  if (s.caustic_photons == 0) return;
+  DevCaustics& c = k.caustics;
+  std::vector<size_t> parts;
+  std::vector<Box> boxes(s.instances.size());
+  for (size_t i = 0; i < s.instances.size(); ++i) {
+    const HostInstance& in = s.instances[i];
+    bool caster = in.material < s.materials.size() && is_caster(s.materials[in.material]);
+    if (in.mesh < meshes.size()) boxes[i] = caustic_instance_box(meshes[in.mesh], in.rows);
+    if (caster) out.instances[i].flags |= INST_CASTER;
+    if (caster && !boxes[i].empty()) parts.push_back(i);
+  }
+  if (parts.empty()) return;
+  c.n_spheres = 0;
+  if (parts.size() > CAUSTIC_MAX_SPHERES) {
+    Box all;
+    for (size_t i : parts) all.grow(boxes[i]);
+    sphere_of_box(all, c.sphere_c[0], c.sphere_r[0]);
+    c.n_spheres = 1;
+  } else {
+    for (size_t i : parts) {
+      // caustics.rs:145-169: the box's centre and a hair beyond the farthest vertex.
+      const HostInstance& in = s.instances[i];
+      V3 centre = boxes[i].centroid();
+      const HostMesh& m = meshes[in.mesh];
+      Xf x = xf_from(in.rows);
+      float r2 = 0.0f;
+      for (size_t v = 0; v + 2 < m.positions.size(); v += 3)
+        r2 = fmax_r(r2, length_sq(xf_point(x, v3(m.positions[v], m.positions[v + 1], m.positions[v + 2])) - centre));
+      float r = sqrt_(r2) * 1.0001f + 1.0e-3f;
+      V3 cc = centre;
+      if (!(r > 0.0f && is_finite(r))) sphere_of_box(boxes[i], cc, r);
+      c.sphere_c[c.n_spheres] = cc;
+      c.sphere_r[c.n_spheres] = r;
+      ++c.n_spheres;
+    }
+  }
+  auto towards = [&](V3 x, const V3* normal) {
+    float sum = -0.0f;
+    for (u32 i = 0; i < c.n_spheres; ++i) {
+      V3 d = c.sphere_c[i] - x;
+      float dist = length(d);
+      float facing = normal ? 0.05f + fmax_r(dot(*normal, d / fmax_r(dist, 1.0e-6f)), 0.0f) : 1.0f;
+      float omega = dist <= c.sphere_r[i] ? PI : TAU * cone_one_minus_cos_host(c.sphere_r[i], dist);
+      sum = sum + omega * facing;
+    }
+    return sum;
+  };
+  std::vector<float> weights;
+  out.caustic_sources.clear();
+  for (size_t i = 0; i < out.areas.size(); ++i) {
+    const DevArea& a = out.areas[i];
+    if (!a.glossy) continue;
+    float w = luminance(a.radiance) * a.area * 1.0e-6f * towards(a.centre, &a.frame.z);
+    if (w > 0.0f && is_finite(w)) {
+      out.caustic_sources.push_back((0u << 24) | (u32)i);
+      weights.push_back(w);
+    }
+  }
+  for (size_t i = 0; i < out.suns.size(); ++i) {
+    const DevSun& sn = out.suns[i];
+    if (!sn.glossy) continue;
+    C3 irradiance = sn.delta ? sn.radiance : sn.radiance * (sn.solid_angle * (1.0f - 0.5f * sn.one_minus_cos_max));
+    float area = -0.0f;
+    for (u32 j = 0; j < c.n_spheres; ++j) area = area + PI * c.sphere_r[j] * c.sphere_r[j] * 1.0e-6f;
+    float w = luminance(irradiance) * area;
+    if (w > 0.0f && is_finite(w)) {
+      out.caustic_sources.push_back((1u << 24) | (u32)i);
+      weights.push_back(w);
+    }
+  }
+  for (size_t i = 0; i < out.emitters.size(); ++i) {
+    const DevEmitter& m = out.emitters[i];
+    V3 acc = v3(0.0f, 0.0f, 0.0f);
+    for (u32 t = 0; t < m.count; ++t) {
+      const float4* tp = &out.emit_tris[4 * (size_t)(m.first + t)];
+      acc = acc + ((v3(tp[0].x, tp[0].y, tp[0].z) + v3(tp[1].x, tp[1].y, tp[1].z)) + v3(tp[2].x, tp[2].y, tp[2].z)) *
+                      (1.0f / 3.0f);
+    }
+    V3 centroid = acc * (1.0f / (float)std::max<u32>(m.count, 1));
+    float w = luminance(m.radiance) * m.area * 1.0e-6f * 0.5f * towards(centroid, nullptr);
+    if (w > 0.0f && is_finite(w)) {
+      out.caustic_sources.push_back((2u << 24) | (u32)i);
+      weights.push_back(w);
+    }
+  }
+  out.caustic_cdf = cdf_of_weights(weights);
+  if (out.caustic_cdf.empty()) {
+    out.caustic_sources.clear();
+    return;
+  }
+  c.n_sources = (u32)out.caustic_sources.size();
+  if (scene_bounds.empty()) {
+    c.scene_centre = v3(0.0f, 0.0f, 0.0f);
+    c.scene_radius = 1.0f;
+  } else {
+    sphere_of_box(scene_bounds, c.scene_centre, c.scene_radius);
+  }
+  // caustics.rs:180-197: the pixel footprint at the camera's target distance.
+  const HostCamera& cam = s.camera;
+  u32 w = std::max<u32>(s.width, 1), h = std::max<u32>(s.height, 1);
+  float footprint;
+  if (cam.ortho) footprint = fabs_(cam.ortho_width) / (float)w;
+  else footprint = length(cam.target - cam.position) * cam.sensor_mm / (fmax_r(cam.focal_mm, 1.0e-3f) * (float)std::max(w, h));
+  float radius_px = s.caustic_radius > 0.0f ? s.caustic_radius : 3.0f;
+  c.radius = fmax_r(radius_px * footprint, 1.0e-4f);
+  c.photons = s.caustic_photons;
+  c.seed = s.seed;
+  u32 depth = s.max_depth == 0 ? 8u : std::min<u32>(s.max_depth, 1024u);
+  c.max_bounces = depth > 0 ? depth - 1 : 0;
+  k.caustics_on = 1;
+  k.ext = 1;
+}
+}  // namespace
+
 bool flatten_scene(const HostScene& s, const EnergyTables& tables, FlatScene& out, std::string& err, int threads) {
   out = FlatScene();
   const size_t nm = s.meshes.size();
@@ -1068,6 +1240,9 @@ bool flatten_scene(const HostScene& s, const EnergyTables& tables, FlatScene& ou
     out.emitters.push_back(em);
     out.consts.ext = 1;
   }
+  // Caustics (caustics.rs:104-345): casters, their bounding spheres, the photon sources by the
+  // flux they send at the spheres, the first radius from the pixel footprint.
+  caustic_setup(s, meshes, scene_bounds, out);
   float strength = fmax_r(s.env_strength, 0.0f);
   float ef = s.env_kind == 1 ? s.env_horizon : s.env_floor, eh = s.env_horizon, ez = s.env_kind == 1 ? s.env_horizon : s.env_zenith;
   DevScene& k = out.consts;
