@@ -17,8 +17,15 @@
  *   rt_render_tile        ffi.rs:118-158 (one sample of a tile: w*h*3 linear RGB floats)
  *   rt_render_tile_rgba   ffi.rs:160-196 (w*h*4 floats, RGB premultiplied, alpha)
  *   rt_render_block_rgba  ffi.rs:196-234 (a block of an adaptive round: one sample of the
- *                         rectangle, as rt_render_tile_rgba; `iteration` picks a photon map,
- *                         and scenes with photon maps or edge supersampling are refused at load)
+ *                         rectangle, as rt_render_tile_rgba; `iteration` picks the photon map
+ *                         of a scene with caustics; scenes with edge supersampling are refused at
+ *                         load)
+ *   rt_scene_caustics     ffi.rs:420-425 (caustic photons per sample: 0 without caustics, none
+ *                         asked for, nothing that focuses light or no light that sends photons;
+ *                         UINT32_MAX for a bad handle). A scene with caustics traces the photon
+ *                         map of each sample index on the GPU before rendering it and keeps the
+ *                         last 16 maps (rt-wasm/src/lib.rs:58,103-121), so tiles of one sample
+ *                         rendered one by one trace its photons once.
  *   rt_adaptive_decide    ffi.rs:237-296 (the adaptive decision of a width x height image from its
  *                         blocks' sample counts, RGBA sums (4 floats per pixel) and squared-
  *                         luminance sums: one byte per block (1: in the next round) and six words:
@@ -81,6 +88,7 @@ uint32_t rt_scene_update(uint32_t handle, const uint8_t* ptr, uint32_t len);
 uint32_t rt_scene_error_len(void);
 void rt_scene_error_copy(uint8_t* ptr);
 uint32_t rt_scene_free(uint32_t handle);
+uint32_t rt_scene_caustics(uint32_t handle);
 uint32_t rt_render_tile(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t h, uint32_t sample_index,
                         float* out);
 uint32_t rt_render_tile_rgba(uint32_t handle, uint32_t x, uint32_t y, uint32_t w, uint32_t h,

@@ -163,6 +163,7 @@ void prepare_engine(Handle* h) {
 
 // Copies the handle's scene constants to its one-entry device scene table.
 bool upload_scene_table(Handle* h) {
+  if (h->photons) h->photons->invalidate();
   return hipMemcpyAsync(h->d_scene, &h->dev, sizeof(DevScene), hipMemcpyHostToDevice, h->stream) == hipSuccess &&
          hipStreamSynchronize(h->stream) == hipSuccess;
 }
@@ -380,6 +381,13 @@ uint32_t rt_scene_update(uint32_t handle, const uint8_t* ptr, uint32_t len) {
 }
 
 uint32_t rt_scene_error_len(void) { return (uint32_t)g_error.size(); }
+
+uint32_t rt_scene_caustics(uint32_t handle) {
+  auto h = find(handle);
+  if (!h) return 0xffffffffu;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  return h->photons ? h->photons->photons() : 0u;
+}
 void rt_scene_error_copy(uint8_t* ptr) {
   if (ptr) std::memcpy(ptr, g_error.data(), g_error.size());
 }
@@ -1021,6 +1029,15 @@ uint32_t rpx_set_material_params(uint32_t handle, const float* params, uint32_t 
     media.push_back(md);
   }
   h->host.materials = updated;
+  if (h->host.caustic_photons > 0) {
+    // Which instances cast caustics, and the photons' paths, follow the materials: the scene is
+    // built again (its kept photon maps dropped with it).
+    if (!build_device_scene(h.get(), err) || !upload_scene_table(h.get())) {
+      set_error("rpx_set_material_params: " + err);
+      return RPX_GPU_LAUNCH;
+    }
+    return RPX_OK;
+  }
   hipStreamSynchronize(h->stream);
   hipError_t e = hipMemcpy(const_cast<Bsdf*>(h->dev.bsdfs), bsdfs.data(), bsdfs.size() * sizeof(Bsdf),
                            hipMemcpyHostToDevice);

@@ -95,6 +95,8 @@ float PhotonPasses::radius_of(float first, u32 iteration) {
 }
 
 PhotonPasses::~PhotonPasses() {
+  for (Kept& k : kept_)
+    if (k.block) (void)hipFree(k.block);
   if (block_) (void)hipFree(block_);
 }
 
@@ -115,15 +117,13 @@ bool PhotonPasses::init(u32 photons, float first_radius, hipStream_t st, std::st
     err = "photon map: the sort's workspace query failed";
     return false;
   }
-  // Layout of the one block: header, raw records (4 x float4) and keys, bucket-ordered records,
-  // sorted keys, index, sorted index, starts, the sort's workspace, the stored counter.
-  size_t sizes[] = {round_up(sizeof(DevPhotonMap)),
-                    round_up(16 * n), round_up(16 * n), round_up(16 * n), round_up(16 * n), round_up(4 * n),
-                    round_up(16 * n), round_up(16 * n), round_up(16 * n), round_up(16 * n),
-                    round_up(4 * n), round_up(4 * n), round_up(4 * n), round_up(4 * ((size_t)buckets + 1)),
+  // One block: the current header, the traced records (4 x float4) and keys, the sorted keys,
+  // the index, the sorted index, the sort's workspace, the stored counter.
+  size_t sizes[] = {round_up(sizeof(DevPhotonMap)), round_up(16 * n), round_up(16 * n), round_up(16 * n),
+                    round_up(16 * n), round_up(4 * n), round_up(4 * n), round_up(4 * n), round_up(4 * n),
                     round_up(sort_bytes_ ? sort_bytes_ : 1), round_up(4)};
   size_t total = 0;
-  for (size_t s : sizes) total += s;
+  for (size_t b : sizes) total += b;
   if (hipMalloc(&block_, total) != hipSuccess) {
     block_ = nullptr;
     err = "photon map: out of device memory";
@@ -141,34 +141,14 @@ bool PhotonPasses::init(u32 photons, float first_radius, hipStream_t st, std::st
   raw_.power = (float4*)take(3);
   raw_.normal = (float4*)take(4);
   raw_.key = (u32*)take(5);
-  pos_ = (float4*)take(6);
-  wi_ = (float4*)take(7);
-  power_ = (float4*)take(8);
-  normal_ = (float4*)take(9);
-  keys_sorted_ = (u32*)take(10);
-  index_ = (u32*)take(11);
-  index_sorted_ = (u32*)take(12);
-  starts_ = (u32*)take(13);
-  sort_tmp_ = take(14);
-  raw_.stored = (u32*)take(15);
-  // An empty map until the first pass: two buckets, both empty.
-  DevPhotonMap h{};
-  h.count = 0;
-  h.mask = 1;
-  h.radius = first_radius;
-  h.inv_r2 = 1.0f / (first_radius * first_radius);
-  h.inv_cell = 1.0f / (2.0f * first_radius);
-  float r_m = first_radius * MM_TO_M;
-  h.norm = 2.0f / (PI * r_m * r_m);
-  h.position = pos_;
-  h.wi = wi_;
-  h.power = power_;
-  h.normal = normal_;
-  h.starts = starts_;
-  u32 zeros[3] = {0, 0, 0};
-  if (hipMemcpyAsync(header_, &h, sizeof(h), hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(starts_, zeros, sizeof(zeros), hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipStreamSynchronize(st) != hipSuccess) {
+  keys_sorted_ = (u32*)take(6);
+  index_ = (u32*)take(7);
+  index_sorted_ = (u32*)take(8);
+  sort_tmp_ = take(9);
+  raw_.stored = (u32*)take(10);
+  if (!add_slot(st, err)) return false;
+  // Until the first pass, the current map is slot 0's empty one.
+  if (hipMemcpyAsync(header_, kept_[0].header, sizeof(DevPhotonMap), hipMemcpyDeviceToDevice, st) != hipSuccess) {
     err = "photon map: upload failed";
     return false;
   }
@@ -180,39 +160,125 @@ bool PhotonPasses::init(u32 photons, float first_radius, hipStream_t st, std::st
   return true;
 }
 
-hipError_t PhotonPasses::build(const DevScene* d_scene, u32 iteration, hipStream_t st, std::string& err) {
-  float radius = radius_of(first_radius_, iteration);
-  float r_m = radius * MM_TO_M;
-  float inv_cell = 1.0f / (2.0f * radius);
-  float inv_r2 = 1.0f / (radius * radius);
-  float norm = 2.0f / (PI * r_m * r_m);
-  hipLaunchKernelGGL(k_map_reset, dim3(1), dim3(1), 0, st, raw_.stored);
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = launch_photons(d_scene, photons_, raw_, iteration, st);
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_map_header, dim3(1), dim3(1), 0, st, header_, raw_.stored, iteration, radius, inv_r2, norm,
-                       inv_cell);
-    e = hipGetLastError();
+// A kept map's own memory: its header, its records in bucket order and its bucket starts, the
+// header holding an empty map (two empty buckets) until the slot is built.
+bool PhotonPasses::add_slot(hipStream_t st, std::string& err) {
+  size_t n = photons_;
+  size_t sizes[] = {round_up(sizeof(DevPhotonMap)), round_up(16 * n), round_up(16 * n), round_up(16 * n),
+                    round_up(16 * n), round_up(4 * ((size_t)buckets_cap_ + 1))};
+  size_t total = 0;
+  for (size_t b : sizes) total += b;
+  Kept k;
+  if (hipMalloc(&k.block, total) != hipSuccess) {
+    err = "photon map: out of device memory";
+    return false;
   }
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_map_keys, dim3(grid_of(photons_, 256, 8192)), dim3(256), 0, st, raw_, header_, photons_);
-    e = hipGetLastError();
+  char* p = (char*)k.block;
+  auto take = [&](int i) {
+    char* q = p;
+    p += sizes[i];
+    return (void*)q;
+  };
+  k.header = (DevPhotonMap*)take(0);
+  k.pos = (float4*)take(1);
+  k.wi = (float4*)take(2);
+  k.power = (float4*)take(3);
+  k.normal = (float4*)take(4);
+  k.starts = (u32*)take(5);
+  DevPhotonMap h{};
+  h.count = 0;
+  h.mask = 1;
+  h.radius = first_radius_;
+  h.inv_r2 = 1.0f / (first_radius_ * first_radius_);
+  h.inv_cell = 1.0f / (2.0f * first_radius_);
+  float r_m = first_radius_ * MM_TO_M;
+  h.norm = 2.0f / (PI * r_m * r_m);
+  h.position = k.pos;
+  h.wi = k.wi;
+  h.power = k.power;
+  h.normal = k.normal;
+  h.starts = k.starts;
+  u32 zeros[3] = {0, 0, 0};
+  if (hipMemcpyAsync(k.header, &h, sizeof(h), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(k.starts, zeros, sizeof(zeros), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    (void)hipFree(k.block);
+    err = "photon map: upload failed";
+    return false;
   }
-  if (e == hipSuccess) {
-    size_t bytes = sort_bytes_;
-    e = hipcub::DeviceRadixSort::SortPairs(sort_tmp_, bytes, raw_.key, keys_sorted_, index_, index_sorted_,
-                                           (int)photons_, 0, (int)KEY_BITS, st);
+  kept_.push_back(k);
+  return true;
+}
+
+void PhotonPasses::invalidate() {
+  for (Kept& k : kept_) k.valid = false;
+}
+
+hipError_t PhotonPasses::build(const DevScene* d_scene, u32 iteration, hipStream_t st, std::string& err,
+                                bool may_allocate) {
+  Kept* slot = nullptr;
+  for (Kept& k : kept_)
+    if (k.valid && k.iteration == iteration) slot = &k;
+  if (!slot) {
+    // A free slot, a new one, or the oldest map's.
+    for (Kept& k : kept_)
+      if (!k.valid) {
+        slot = &k;
+        break;
+      }
+    if (!slot && may_allocate && kept_.size() < KEPT_PHOTON_MAPS) {
+      std::string e2;
+      if (add_slot(st, e2)) slot = &kept_.back();
+    }
+    if (!slot) {
+      slot = &kept_[0];
+      for (Kept& k : kept_)
+        if (k.age < slot->age) slot = &k;
+    }
+    slot->valid = false;
+    float radius = radius_of(first_radius_, iteration);
+    float r_m = radius * MM_TO_M;
+    float inv_cell = 1.0f / (2.0f * radius);
+    float inv_r2 = 1.0f / (radius * radius);
+    float norm = 2.0f / (PI * r_m * r_m);
+    hipLaunchKernelGGL(k_map_reset, dim3(1), dim3(1), 0, st, raw_.stored);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = launch_photons(d_scene, photons_, raw_, iteration, st);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_map_header, dim3(1), dim3(1), 0, st, slot->header, raw_.stored, iteration, radius, inv_r2,
+                         norm, inv_cell);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_map_keys, dim3(grid_of(photons_, 256, 8192)), dim3(256), 0, st, raw_, slot->header,
+                         photons_);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+      size_t bytes = sort_bytes_;
+      e = hipcub::DeviceRadixSort::SortPairs(sort_tmp_, bytes, raw_.key, keys_sorted_, index_, index_sorted_,
+                                             (int)photons_, 0, (int)KEY_BITS, st);
+    }
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_map_permute, dim3(grid_of(photons_, 256, 8192)), dim3(256), 0, st, slot->header,
+                         index_sorted_, raw_, slot->pos, slot->wi, slot->power, slot->normal);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_map_starts, dim3(grid_of(buckets_cap_ + 1, 256, 8192)), dim3(256), 0, st, slot->header,
+                         keys_sorted_, photons_, slot->starts, buckets_cap_);
+      e = hipGetLastError();
+    }
+    if (e != hipSuccess) {
+      err = std::string("photon map: ") + hipGetErrorString(e);
+      return e;
+    }
+    slot->valid = true;
+    slot->iteration = iteration;
+    slot->age = ++clock_;
+    ++traced_;
   }
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_map_permute, dim3(grid_of(photons_, 256, 8192)), dim3(256), 0, st, header_, index_sorted_,
-                       raw_, pos_, wi_, power_, normal_);
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_map_starts, dim3(grid_of(buckets_cap_ + 1, 256, 8192)), dim3(256), 0, st, header_,
-                       keys_sorted_, photons_, starts_, buckets_cap_);
-    e = hipGetLastError();
-  }
+  hipError_t e = hipMemcpyAsync(header_, slot->header, sizeof(DevPhotonMap), hipMemcpyDeviceToDevice, st);
   if (e != hipSuccess) err = std::string("photon map: ") + hipGetErrorString(e);
   return e;
 }

@@ -422,7 +422,8 @@ hipError_t Engine::render_tile_in(const DevScene* d_scenes, const DevSegment& se
   s.base = 0;
   hipError_t e = hipSuccess;
   // The photon map's memory is the scene's, set aside when it was loaded.
-  if (photons) e = photons->build(d_scenes + s.scene, photon_pass >= 0 ? (u32)photon_pass : s.first, stream_, err);
+  if (photons)
+    e = photons->build(d_scenes + s.scene, photon_pass >= 0 ? (u32)photon_pass : s.first, stream_, err, false);
   if (e == hipSuccess) e = run_chunk(d_scenes, &s, 1, n, false, false, false, table);
   if (e == hipSuccess) e = launch_tile_out(P, staging, seg.w, seg.h, channels, stream_);
   if (e == hipSuccess)

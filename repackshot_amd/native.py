@@ -40,6 +40,7 @@ _SIGS = {
     "rt_scene_error_len": (ctypes.c_uint32, []),
     "rt_scene_error_copy": (None, [ctypes.c_void_p]),
     "rt_scene_free": (ctypes.c_uint32, [ctypes.c_uint32]),
+    "rt_scene_caustics": (ctypes.c_uint32, [ctypes.c_uint32]),
     "rt_render_tile": (ctypes.c_uint32, [ctypes.c_uint32] * 6 + [ctypes.c_void_p]),
     "rt_render_tile_rgba": (ctypes.c_uint32, [ctypes.c_uint32] * 6 + [ctypes.c_void_p]),
     "rt_render_block_rgba": (ctypes.c_uint32, [ctypes.c_uint32] * 7 + [ctypes.c_void_p]),
@@ -259,6 +260,10 @@ class Scene:
         _check(library().rpx_scene_info(self.handle, _ptr(out)), "rpx_scene_info")
         keys = ["width", "height", "max_samples", "seed", "max_depth", "triangles", "nodes", "instances"]
         return {k: int(v) for k, v in zip(keys, out)}
+
+    def caustics(self):
+        """Caustic photons per sample (0: the scene has no caustics)."""
+        return int(library().rt_scene_caustics(self.handle))
 
     def render_tile(self, x, y, w, h, sample_index):
         """One sample of every pixel of the tile: (h, w, 3) float32 linear RGB."""

@@ -83,6 +83,17 @@ def test_caustic_films_equal_the_oracle(gpu, name):
     s.close()
 
 
+def test_a_quarter_million_photons_per_pass_equal_the_oracle(gpu):
+    # 2^18 photons a pass: a table of 2^19 buckets, sorted and gathered at scale.
+    rsb = sphere_scene(30.0, disc_light(size=60.0), 1 << 18, (48, 36))
+    want, _ = orc.OracleScene(rsb).render_film(3, 2)
+    s = gpu.Scene(rsb)
+    s.render_samples(3, 2)
+    got = s.film()
+    assert all(bits_equal(a, b) for a, b in zip(got, want))
+    s.close()
+
+
 def test_caustic_tiles_blocks_and_the_allocation_free_tile_equal_the_oracle(gpu):
     rsb = SCENES["glass under a disc"]()
     o = orc.OracleScene(rsb)
@@ -98,6 +109,43 @@ def test_caustic_tiles_blocks_and_the_allocation_free_tile_equal_the_oracle(gpu)
         assert bits_equal(s.render_block_rgba(8, 6, 24, 18, k, iteration=k), rgba), k
         assert not bits_equal(s.render_block_rgba(8, 6, 24, 18, k, iteration=k + 7), rgba), k
         assert bits_equal(s.render_tile_noalloc(8, 6, 24, 18, k), want_rgb), k
+    s.close()
+
+
+def test_tiles_of_passes_out_of_order_reuse_their_kept_maps(gpu):
+    # rt-wasm/src/lib.rs:1311-1351: tiles of samples 0, 1, 0, 2 each equal the engine's tile with
+    # that sample's map; an update to no photons leaves a scene without caustics.
+    rsb = SCENES["overlapping casters"]()
+    o = orc.OracleScene(rsb)
+    s = gpu.Scene(rsb)
+    assert s.caustics() == 3000
+    assert native.library().rt_scene_caustics(999) == 0xFFFFFFFF
+    for k in (0, 1, 0, 2, 1):
+        assert bits_equal(s.render_tile(12, 4, 20, 16, k), o.render_tile(12, 4, 20, 16, k)[0]), k
+    plain = sphere_scene(30.0, disc_light(), 0, (40, 30),
+                         spheres=[((0.0, 0.0, 30.0), 30.0, 4), ((0.0, -40.0, 72.5), 12.0, 3)])
+    s.update(plain)
+    assert s.caustics() == 0
+    assert bits_equal(s.render_tile(12, 4, 20, 16, 0), orc.OracleScene(plain).render_tile(12, 4, 20, 16, 0)[0])
+    s.close()
+
+
+def test_material_changes_rebuild_the_caster_set(gpu):
+    # Roughening the glass past 0.25 leaves nothing that focuses light: the films follow the oracle
+    # of the changed scene either way.
+    rsb = SCENES["glass under a disc"]()
+    s = gpu.Scene(rsb)
+    params = s.material_params()
+    s.render_tile(0, 0, 40, 30, 1)
+    rough = params.copy()
+    rough[0, 0] = 0.4
+    s.set_material_params(rough)
+    glass = dict(kind=1, base_color=(1, 1, 1), roughness=0.4, ior=1.5, transmission=1.0)
+    changed = sphere_scene(30.0, disc_light(), 3000, (40, 30), material=glass)
+    assert not orc.OracleScene(changed).caustics()["on"]
+    assert bits_equal(s.render_tile(0, 0, 40, 30, 1), orc.OracleScene(changed).render_tile(0, 0, 40, 30, 1)[0])
+    s.set_material_params(params)
+    assert bits_equal(s.render_tile(0, 0, 40, 30, 1), orc.OracleScene(rsb).render_tile(0, 0, 40, 30, 1)[0])
     s.close()
 
 
