@@ -152,8 +152,9 @@ uint32_t rpx_last_timing(uint32_t handle, float* out12);
 /* Kernel times of the last render with profiling on (ms summed over its launches): the
  * closest-hit kernel alone, its launches, the shadow kernel (on the second stream), its launches,
  * the traversal phase (closest-hit kernel until the shadow kernel beside it has joined), the
- * media walks, the drain, the shade. */
-uint32_t rpx_last_kernel_ms(uint32_t handle, float* out8);
+ * media walks, the drain, the shade, the photon maps of a scene with caustics (tracing and
+ * building), and the photon maps the scene has traced since it was loaded. */
+uint32_t rpx_last_kernel_ms(uint32_t handle, float* out10);
 /* The GPU-generated energy-compensation tables of the current device (1024, 1024, 17408,
  * 17408 u16 and 32, 32 f32). */
 uint32_t rpx_energy_tables(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_in, uint16_t* clear_out,
@@ -163,7 +164,8 @@ uint32_t rpx_energy_tables(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_
  * kind 0: the reference's example scene (example.rs:113-230); 1: icosphere on a floor
  * (256x256); 2: eight displaced level-7 icospheres (1024x1024); 4: the inverse-material fit's scene
  * (200,000 triangles, 512x512, floor material 8 takes the base-colour texture); 5: a glass bottle
- * holding a scattering medium, lit by a sphere light (200,000 triangles, 1920x1080). */
+ * holding a scattering medium, lit by a sphere light (200,000 triangles, 1920x1080); 6: that bottle
+ * of clear glass under a glossy disc light, 4,194,304 caustic photons per sample. */
 uint64_t rpx_write_scene(uint32_t kind, uint32_t seed, uint8_t* buf, uint64_t cap);
 /* Job `job` of the seeded mixed render queue (this library's batched workload), as RSB v1 bytes
  * (same buffer protocol): a square image of 256..1024 px, 10k..500k triangles, 16/32/64 spp. */

@@ -126,14 +126,14 @@ bool build_device_scene(Handle* h, std::string& err) {
       !upload(B, flat.emitters, d.emitters, st, err) || !upload(B, flat.emit_tris, d.emit_tris, st, err) ||
       !upload(B, flat.emit_cdf, d.emit_cdf, st, err) || !upload(B, flat.instance_emitter, d.instance_emitter, st, err))
     return false;
-  d.pmap = nullptr;
+  d.pmaps = nullptr;
   if (d.caustics_on) {
     if (!upload(B, flat.caustic_sources, d.caustics.sources, st, err) ||
         !upload(B, flat.caustic_cdf, d.caustics.source_cdf, st, err))
       return false;
     auto maps = std::make_unique<PhotonPasses>();
     if (!maps->init(d.caustics.photons, d.caustics.radius, st, err)) return false;
-    d.pmap = maps->header();
+    d.pmaps = maps->table();
     h->photons = std::move(maps);
   }
   d.tables = tables->device_view();
@@ -934,12 +934,13 @@ uint32_t rpx_last_timing(uint32_t handle, float* out12) {
   return RPX_OK;
 }
 
-uint32_t rpx_last_kernel_ms(uint32_t handle, float* out8) {
-  if (!out8) return RPX_NULL_POINTER;
+uint32_t rpx_last_kernel_ms(uint32_t handle, float* out10) {
+  if (!out10) return RPX_NULL_POINTER;
   auto h = find(handle);
   if (!h) return RPX_BAD_HANDLE;
   std::lock_guard<std::mutex> lock(h->mutex);
   const Timing& t = h->engine->timing;
+  float* out8 = out10;
   out8[0] = t.stage_ms[6];
   out8[1] = (float)t.launches[6];
   out8[2] = t.stage_ms[2];
@@ -948,6 +949,8 @@ uint32_t rpx_last_kernel_ms(uint32_t handle, float* out8) {
   out8[5] = t.stage_ms[5];
   out8[6] = t.stage_ms[4];
   out8[7] = t.stage_ms[1];
+  out10[8] = t.stage_ms[7];
+  out10[9] = (float)(h->photons ? h->photons->traced() : 0u);
   return RPX_OK;
 }
 

@@ -79,6 +79,19 @@ __global__ void __launch_bounds__(256) k_map_starts(const DevPhotonMap* M, const
   }
 }
 
+// The table of current maps: pass first + i reads a copy of kept map heads.p[i]'s header.
+struct MapHeads {
+  const DevPhotonMap* p[MAX_PASS_MAPS];
+};
+__global__ void k_map_table(DevPhotonMaps* T, MapHeads heads, u32 first, u32 n) {
+  u32 i = threadIdx.x;
+  if (i == 0) {
+    T->first = first;
+    T->count = n;
+  }
+  if (i < n) T->map[i] = *heads.p[i];
+}
+
 inline u32 grid_of(u32 n, u32 block, u32 cap) {
   u32 g = (n + block - 1) / block;
   return g < 1 ? 1 : (g > cap ? cap : g);
@@ -117,9 +130,9 @@ bool PhotonPasses::init(u32 photons, float first_radius, hipStream_t st, std::st
     err = "photon map: the sort's workspace query failed";
     return false;
   }
-  // One block: the current header, the traced records (4 x float4) and keys, the sorted keys,
-  // the index, the sorted index, the sort's workspace, the stored counter.
-  size_t sizes[] = {round_up(sizeof(DevPhotonMap)), round_up(16 * n), round_up(16 * n), round_up(16 * n),
+  // One block: the table of current maps, the traced records (4 x float4) and keys, the sorted
+  // keys, the index, the sorted index, the sort's workspace, the stored counter.
+  size_t sizes[] = {round_up(sizeof(DevPhotonMaps)), round_up(16 * n), round_up(16 * n), round_up(16 * n),
                     round_up(16 * n), round_up(4 * n), round_up(4 * n), round_up(4 * n), round_up(4 * n),
                     round_up(sort_bytes_ ? sort_bytes_ : 1), round_up(4)};
   size_t total = 0;
@@ -135,7 +148,7 @@ bool PhotonPasses::init(u32 photons, float first_radius, hipStream_t st, std::st
     p += sizes[k];
     return (void*)q;
   };
-  header_ = (DevPhotonMap*)take(0);
+  table_ = (DevPhotonMaps*)take(0);
   raw_.position = (float4*)take(1);
   raw_.wi = (float4*)take(2);
   raw_.power = (float4*)take(3);
@@ -147,8 +160,9 @@ bool PhotonPasses::init(u32 photons, float first_radius, hipStream_t st, std::st
   sort_tmp_ = take(9);
   raw_.stored = (u32*)take(10);
   if (!add_slot(st, err)) return false;
-  // Until the first pass, the current map is slot 0's empty one.
-  if (hipMemcpyAsync(header_, kept_[0].header, sizeof(DevPhotonMap), hipMemcpyDeviceToDevice, st) != hipSuccess) {
+  // Until the first render, no pass has a map.
+  DevPhotonMaps none{};
+  if (hipMemcpyAsync(table_, &none, sizeof(none), hipMemcpyHostToDevice, st) != hipSuccess) {
     err = "photon map: upload failed";
     return false;
   }
@@ -214,71 +228,93 @@ void PhotonPasses::invalidate() {
   for (Kept& k : kept_) k.valid = false;
 }
 
-hipError_t PhotonPasses::build(const DevScene* d_scene, u32 iteration, hipStream_t st, std::string& err,
-                                bool may_allocate) {
-  Kept* slot = nullptr;
-  for (Kept& k : kept_)
-    if (k.valid && k.iteration == iteration) slot = &k;
-  if (!slot) {
-    // A free slot, a new one, or the oldest map's.
-    for (Kept& k : kept_)
-      if (!k.valid) {
-        slot = &k;
-        break;
-      }
-    if (!slot && may_allocate && kept_.size() < KEPT_PHOTON_MAPS) {
-      std::string e2;
-      if (add_slot(st, e2)) slot = &kept_.back();
-    }
-    if (!slot) {
-      slot = &kept_[0];
-      for (Kept& k : kept_)
-        if (k.age < slot->age) slot = &k;
-    }
-    slot->valid = false;
-    float radius = radius_of(first_radius_, iteration);
-    float r_m = radius * MM_TO_M;
-    float inv_cell = 1.0f / (2.0f * radius);
-    float inv_r2 = 1.0f / (radius * radius);
-    float norm = 2.0f / (PI * r_m * r_m);
-    hipLaunchKernelGGL(k_map_reset, dim3(1), dim3(1), 0, st, raw_.stored);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = launch_photons(d_scene, photons_, raw_, iteration, st);
-    if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_map_header, dim3(1), dim3(1), 0, st, slot->header, raw_.stored, iteration, radius, inv_r2,
-                         norm, inv_cell);
-      e = hipGetLastError();
-    }
-    if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_map_keys, dim3(grid_of(photons_, 256, 8192)), dim3(256), 0, st, raw_, slot->header,
-                         photons_);
-      e = hipGetLastError();
-    }
-    if (e == hipSuccess) {
-      size_t bytes = sort_bytes_;
-      e = hipcub::DeviceRadixSort::SortPairs(sort_tmp_, bytes, raw_.key, keys_sorted_, index_, index_sorted_,
-                                             (int)photons_, 0, (int)KEY_BITS, st);
-    }
-    if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_map_permute, dim3(grid_of(photons_, 256, 8192)), dim3(256), 0, st, slot->header,
-                         index_sorted_, raw_, slot->pos, slot->wi, slot->power, slot->normal);
-      e = hipGetLastError();
-    }
-    if (e == hipSuccess) {
-      hipLaunchKernelGGL(k_map_starts, dim3(grid_of(buckets_cap_ + 1, 256, 8192)), dim3(256), 0, st, slot->header,
-                         keys_sorted_, photons_, slot->starts, buckets_cap_);
-      e = hipGetLastError();
-    }
-    if (e != hipSuccess) {
-      err = std::string("photon map: ") + hipGetErrorString(e);
-      return e;
-    }
-    slot->valid = true;
-    slot->iteration = iteration;
-    slot->age = ++clock_;
-    ++traced_;
+int PhotonPasses::find(u32 iteration) const {
+  for (size_t k = 0; k < kept_.size(); ++k)
+    if (kept_[k].valid && kept_[k].iteration == iteration) return (int)k;
+  return -1;
+}
+
+hipError_t PhotonPasses::trace(const DevScene* d_scene, u32 iteration, hipStream_t st, u32 avoid, int& slot_out,
+                               std::string& err, bool may_allocate) {
+  int slot = -1;
+  for (size_t k = 0; k < kept_.size() && slot < 0; ++k)
+    if (!kept_[k].valid && !((avoid >> k) & 1u)) slot = (int)k;
+  if (slot < 0 && may_allocate && kept_.size() < KEPT_PHOTON_MAPS) {
+    std::string unused;
+    if (add_slot(st, unused)) slot = (int)kept_.size() - 1;
   }
-  hipError_t e = hipMemcpyAsync(header_, slot->header, sizeof(DevPhotonMap), hipMemcpyDeviceToDevice, st);
+  if (slot < 0)
+    for (size_t k = 0; k < kept_.size(); ++k)
+      if (!((avoid >> k) & 1u) && (slot < 0 || kept_[k].age < kept_[slot].age)) slot = (int)k;
+  if (slot < 0) {
+    err = "photon map: no kept map to trace into";
+    return hipErrorInvalidValue;
+  }
+  Kept& K = kept_[slot];
+  K.valid = false;
+  float radius = radius_of(first_radius_, iteration);
+  float r_m = radius * MM_TO_M;
+  float inv_cell = 1.0f / (2.0f * radius);
+  float inv_r2 = 1.0f / (radius * radius);
+  float norm = 2.0f / (PI * r_m * r_m);
+  hipLaunchKernelGGL(k_map_reset, dim3(1), dim3(1), 0, st, raw_.stored);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = launch_photons(d_scene, photons_, raw_, iteration, st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_map_header, dim3(1), dim3(1), 0, st, K.header, raw_.stored, iteration, radius, inv_r2, norm,
+                       inv_cell);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_map_keys, dim3(grid_of(photons_, 256, 8192)), dim3(256), 0, st, raw_, K.header, photons_);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    size_t bytes = sort_bytes_;
+    e = hipcub::DeviceRadixSort::SortPairs(sort_tmp_, bytes, raw_.key, keys_sorted_, index_, index_sorted_,
+                                           (int)photons_, 0, (int)KEY_BITS, st);
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_map_permute, dim3(grid_of(photons_, 256, 8192)), dim3(256), 0, st, K.header, index_sorted_,
+                       raw_, K.pos, K.wi, K.power, K.normal);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_map_starts, dim3(grid_of(buckets_cap_ + 1, 256, 8192)), dim3(256), 0, st, K.header,
+                       keys_sorted_, photons_, K.starts, buckets_cap_);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) {
+    err = std::string("photon map: ") + hipGetErrorString(e);
+    return e;
+  }
+  K.valid = true;
+  K.iteration = iteration;
+  K.age = ++clock_;
+  ++traced_;
+  slot_out = slot;
+  return hipSuccess;
+}
+
+hipError_t PhotonPasses::prepare(const DevScene* d_scene, u32 first, const u32* iterations, u32 n, hipStream_t st,
+                                  std::string& err, bool may_allocate) {
+  if (n == 0 || n > MAX_PASS_MAPS) {
+    err = "photon map: a render runs 1 to 16 sample passes with their maps";
+    return hipErrorInvalidValue;
+  }
+  MapHeads heads{};
+  u32 used = 0;  // kept maps this call reads
+  for (u32 i = 0; i < n; ++i) {
+    int slot = find(iterations[i]);
+    if (slot < 0) {
+      hipError_t e = trace(d_scene, iterations[i], st, used, slot, err, may_allocate);
+      if (e != hipSuccess) return e;
+    }
+    used |= 1u << slot;
+    heads.p[i] = kept_[slot].header;
+  }
+  hipLaunchKernelGGL(k_map_table, dim3(1), dim3(64), 0, st, table_, heads, first, n);
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) err = std::string("photon map: ") + hipGetErrorString(e);
   return e;
 }

@@ -51,9 +51,10 @@ struct EngineOptions {
 struct Timing {
   float total_ms = 0;
   // extend (the closest-hit kernel until the shadow kernel beside it has joined), shade, shadow,
-  // regen, drain, walk, and the closest-hit kernel alone
-  float stage_ms[7] = {0, 0, 0, 0, 0, 0, 0};
-  u32 launches[7] = {0, 0, 0, 0, 0, 0, 0};
+  // regen, drain, walk, the closest-hit kernel alone, and the photon maps (tracing and building;
+  // launches: maps made current)
+  float stage_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  u32 launches[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   u32 iterations = 0, chunks = 0;
 };
 
@@ -101,8 +102,9 @@ class Engine {
 
   EngineOptions opt;
   // A scene with caustics: the photon map of each sample pass is traced and built on the engine's
-  // stream before the pass, and the passes run one at a time. `photon_pass` >= 0 names the map's
-  // pass for every sample instead of the sample index.
+  // stream before the chunk that runs the pass; a chunk holds one segment's samples, at most
+  // MAX_PASS_MAPS of them. `photon_pass` >= 0 names the map's pass for every sample instead of the
+  // sample index.
   PhotonPasses* photons = nullptr;
   long long photon_pass = -1;
   bool profiling = false;
@@ -119,6 +121,8 @@ class Engine {
             std::string& err);
   bool ensure_results(u32 n, std::string& err);
   bool ensure_segs(u32 n, std::string& err);
+  // The photon maps of a chunk's sample passes made current (profiled as stage 7).
+  hipError_t prepare_photons(const DevScene* d_scenes, const DevSegment& piece, bool may_allocate, std::string& err);
   // `seg_table`: device memory for the chunk's segment table (null: the engine's own).
   hipError_t run_chunk(const DevScene* d_scenes, const DevSegment* chunk, u32 n_chunk, u32 total, bool splat,
                        bool adj = false, bool disjoint = false, DevSegment* seg_table = nullptr);
@@ -138,6 +142,7 @@ class Engine {
   // The shadow kernel's stream: forked from stream_ after each regen, joined before the shade.
   hipStream_t side_ = nullptr;
   hipEvent_t fork_ = nullptr, join_ = nullptr;
+
 };
 
 }  // namespace rpx

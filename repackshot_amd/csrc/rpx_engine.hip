@@ -85,6 +85,7 @@ Engine::~Engine() {
   }
   if (fork_) hipEventDestroy(fork_);
   if (join_) hipEventDestroy(join_);
+
   dev_free_all(pool_bufs_);
   dev_free_all(adjoint_bufs_);
   dev_free_all(bsdf_bufs_);
@@ -119,6 +120,7 @@ bool Engine::init(int device, hipStream_t stream, std::string& err) {
     err = "cannot create the shadow rays' stream or its events";
     return false;
   }
+
   hipMemsetAsync(P.counts, 0, N_COUNTS * 4, stream_);
   hipMemsetAsync(P.rays, 0, 64, stream_);
   return hipStreamSynchronize(stream_) == hipSuccess;
@@ -421,9 +423,8 @@ hipError_t Engine::render_tile_in(const DevScene* d_scenes, const DevSegment& se
   DevSegment s = seg;
   s.base = 0;
   hipError_t e = hipSuccess;
-  // The photon map's memory is the scene's, set aside when it was loaded.
-  if (photons)
-    e = photons->build(d_scenes + s.scene, photon_pass >= 0 ? (u32)photon_pass : s.first, stream_, err, false);
+  // The photon maps' memory is the scene's, set aside when it was loaded.
+  if (photons) e = prepare_photons(d_scenes, s, false, err);
   if (e == hipSuccess) e = run_chunk(d_scenes, &s, 1, n, false, false, false, table);
   if (e == hipSuccess) e = launch_tile_out(P, staging, seg.w, seg.h, channels, stream_);
   if (e == hipSuccess)
@@ -464,6 +465,7 @@ bool Engine::plan(const std::vector<DevSegment>& segs, std::vector<std::vector<D
     while (k0 < s.count) {
       u32 room = (limit - base) / npix;
       if (opt.per_pass) room = std::min<u32>(room, 1);
+      if (photons) room = std::min<u32>(room, MAX_PASS_MAPS);  // the passes' maps are kept together
       if (room == 0) {
         flush();
         continue;
@@ -476,11 +478,25 @@ bool Engine::plan(const std::vector<DevSegment>& segs, std::vector<std::vector<D
       cur.push_back(piece);
       base += take * npix;
       k0 += take;
-      if (opt.per_pass) flush();
+      if (opt.per_pass || photons) flush();  // a chunk of a scene with caustics: one segment's passes
     }
   }
   flush();
   return true;
+}
+
+hipError_t Engine::prepare_photons(const DevScene* d_scenes, const DevSegment& piece, bool may_allocate,
+                                   std::string& err) {
+  u32 passes[MAX_PASS_MAPS];
+  for (u32 i = 0; i < piece.count && i < MAX_PASS_MAPS; ++i)
+    passes[i] = photon_pass >= 0 ? (u32)photon_pass : piece.first + i;
+  if (profiling) {
+    marks_.push_back({7, events_used_});
+    hipEventRecord(next_event(), stream_);
+  }
+  hipError_t e = photons->prepare(d_scenes + piece.scene, piece.first, passes, piece.count, stream_, err, may_allocate);
+  if (profiling) hipEventRecord(next_event(), stream_);
+  return e;
 }
 
 hipError_t Engine::run(const DevScene* d_scenes, const std::vector<DevSegment>& segs, bool splat, std::string& err,
@@ -490,11 +506,7 @@ hipError_t Engine::run(const DevScene* d_scenes, const std::vector<DevSegment>& 
   marks_.clear();
   std::vector<std::vector<DevSegment>> chunks;
   std::vector<u32> totals;
-  const bool keep_per_pass = opt.per_pass;
-  if (photons) opt.per_pass = true;  // each pass has its own photon map
-  bool planned = plan(segs, chunks, totals, err);
-  opt.per_pass = keep_per_pass;
-  if (!planned) return hipErrorInvalidValue;
+  if (!plan(segs, chunks, totals, err)) return hipErrorInvalidValue;
   if (!splat && chunks.size() > 1) {
     err = "unsplatted results must fit one chunk";
     return hipErrorInvalidValue;
@@ -509,12 +521,9 @@ hipError_t Engine::run(const DevScene* d_scenes, const std::vector<DevSegment>& 
   hipEventRecord(ev_[0], stream_);
   hipError_t e = hipSuccess;
   for (size_t c = 0; c < chunks.size() && e == hipSuccess; ++c) {
-    if (photons) {
-      const DevSegment& s = chunks[c][0];
-      e = photons->build(d_scenes + s.scene, photon_pass >= 0 ? (u32)photon_pass : s.first, stream_, err);
-      if (e != hipSuccess) break;
-    }
-    e = run_chunk(d_scenes, chunks[c].data(), (u32)chunks[c].size(), totals[c], splat, false, disjoint);
+    if (photons) e = prepare_photons(d_scenes, chunks[c][0], true, err);
+    if (e == hipSuccess)
+      e = run_chunk(d_scenes, chunks[c].data(), (u32)chunks[c].size(), totals[c], splat, false, disjoint);
   }
   hipEventRecord(ev_[1], stream_);
   if (e == hipSuccess) e = hipStreamSynchronize(stream_);
@@ -622,11 +631,14 @@ hipError_t Engine::autotune(const DevScene* d_scenes, const std::vector<DevSegme
   opt.per_pass = false;
   opt.graph = false;
   opt.chunk = RESULTS_MAX;  // the probe's results stay unsplatted: one chunk
-  // A scene with caustics: the candidates render the probe against one photon map, built first.
+  // A scene with caustics: the candidates render the probe's first MAX_PASS_MAPS passes against
+  // their photon maps, made first.
   PhotonPasses* keep_photons = photons;
-  if (photons && !probe.empty()) {
-    hipError_t b = photons->build(d_scenes + probe[0].scene, photon_pass >= 0 ? (u32)photon_pass : probe[0].first,
-                                  stream_, err);
+  std::vector<DevSegment> probe_run = probe;
+  if (photons && !probe_run.empty()) {
+    probe_run.resize(1);
+    probe_run[0].count = std::min<u32>(probe_run[0].count, MAX_PASS_MAPS);
+    hipError_t b = prepare_photons(d_scenes, probe_run[0], true, err);
     if (b == hipSuccess) b = hipStreamSynchronize(stream_);
     if (b != hipSuccess) {
       opt = keep;
@@ -641,8 +653,8 @@ hipError_t Engine::autotune(const DevScene* d_scenes, const std::vector<DevSegme
   hipError_t e = hipSuccess;
   // A candidate whose pool does not fit in device memory is skipped, not fatal.
   auto time_it = [&](float& t) -> bool {
-    e = run(d_scenes, probe, false, err);  // warm (allocations, caches)
-    if (e == hipSuccess) e = run(d_scenes, probe, false, err);
+    e = run(d_scenes, probe_run, false, err);  // warm (allocations, caches)
+    if (e == hipSuccess) e = run(d_scenes, probe_run, false, err);
     if (e == hipErrorOutOfMemory) {
       e = hipSuccess;
       err.clear();

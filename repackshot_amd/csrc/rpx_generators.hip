@@ -23,6 +23,8 @@
 //           homogeneous scattering medium (sigma_s 0.5, sigma_a 0.05 per metre, g 0.7) on a Lambert
 //           floor, lit by one sphere light (radius 50 mm, radiance 500); 1920 x 1080, 64 spp,
 //           depth 16
+//   kind 6  caustics: kind 5's bottle of clear glass under a glossy disc light (100 mm, radiance
+//           500) aimed at it, 4,194,304 caustic photons per sample; 1920 x 1080, 64 spp, depth 16
 //   queue   job j of a seeded render queue (rpx_write_queue_job): image 256, 512, 768 or 1024 px
 //           square; 10k..500k triangles (log-uniform) in four value-noise displaced spheres
 //           (latitude / longitude grids) on a floor; materials drawn as in kind 2; one of 12
@@ -199,7 +201,7 @@ void write_flor(Writer& w) {
   w.rgb(c3(0.5f, 0.5f, 0.5f));
   w.end(s);
 }
-void write_rndr(Writer& w, u32 width, u32 height, u32 spp, u32 seed, u32 depth) {
+void write_rndr(Writer& w, u32 width, u32 height, u32 spp, u32 seed, u32 depth, u32 photons = 0) {
   size_t s = w.begin("RNDR");
   w.u(width);
   w.u(height);
@@ -212,8 +214,8 @@ void write_rndr(Writer& w, u32 width, u32 height, u32 spp, u32 seed, u32 depth) 
   w.f(0);  // clamp
   w.u(0);  // passes
   w.u(0);  // denoise
-  w.u(0);  // caustic photons
-  w.f(0);  // caustic radius
+  w.u(photons);  // caustic photons
+  w.f(0);        // caustic radius (0: three pixels)
   w.f(1);  // denoise strength
   w.u(0);  // pixel filter
   w.u(1);  // edge supersample
@@ -638,8 +640,10 @@ MeshData revolution(const std::vector<float>& z, const std::vector<float>& r, u3
 // medium of sigma_s 0.5 and sigma_a 0.05 per metre (mean free path 1818.18 mm, single-scattering
 // albedo 0.5 / 0.55) and Henyey-Greenstein g 0.7, standing 1 mm above a Lambert floor (albedo
 // 0.8), lit by one sphere light of radius 50 mm and radiance 500 (power 500 pi r^2 4 pi W), no
-// dome; 1920 x 1080, 64 spp, depth 16.
-std::vector<uint8_t> config_bottle(u32 seed) {
+// dome; 1920 x 1080, 64 spp, depth 16. With `caustics`: the bottle of clear glass (no medium)
+// under a glossy disc light of the same diameter and radiance at the sphere light's place, aimed
+// at the bottle, 4,194,304 caustic photons per sample (the most a scene may ask for).
+std::vector<uint8_t> config_bottle(u32 seed, bool caustics = false) {
   Writer w;
   w.b = {'R', 'S', 'B', '1', 0, 0, 0, 0};
   Pcg32 rng = Pcg32::seeded((u64)seed, 5);
@@ -654,10 +658,12 @@ std::vector<uint8_t> config_bottle(u32 seed) {
   glass.roughness = 0;
   glass.ior = 1.5f;
   glass.transmission = 1;
-  glass.scatter_mm = 1.0f / 0.55e-3f;
-  float albedo = 0.5f / 0.55f;
-  glass.scatter_color = c3(albedo, albedo, albedo);
-  glass.scatter_g = 0.7f;
+  if (!caustics) {
+    glass.scatter_mm = 1.0f / 0.55e-3f;
+    float albedo = 0.5f / 0.55f;
+    glass.scatter_color = c3(albedo, albedo, albedo);
+    glass.scatter_g = 0.7f;
+  }
   Mat floor_m;
   floor_m.base = c3(0.8f, 0.8f, 0.8f);
   floor_m.ior = 1.0f;
@@ -693,10 +699,17 @@ std::vector<uint8_t> config_bottle(u32 seed) {
   l.sy = 100;
   const float r_m = 0.05f;
   l.power = 500.0f * PI * r_m * r_m * 4.0f * PI;
+  if (caustics) {
+    // A disc light's power is its radiance times pi times its area (m^2).
+    l.kind = 1;
+    l.direction = normalize(v3(0, 0, 150) - l.position);
+    l.up = v3(0, 0, 1);
+    l.power = 500.0f * PI * (PI * r_m * r_m);
+  }
   write_lght(w, l);
   write_envr(w, 0, 0.1f, 0.3f, 0.6f, 0.0f);
   write_flor(w);
-  write_rndr(w, 1920, 1080, 64, seed, 16);
+  write_rndr(w, 1920, 1080, 64, seed, 16, caustics ? (1u << 22) : 0u);
   return w.b;
 }
 
@@ -787,6 +800,7 @@ extern "C" uint64_t rpx_write_scene(uint32_t kind, uint32_t seed, uint8_t* buf, 
       case 2: bytes = rpx::config_large(seed); break;
       case 4: bytes = rpx::config_fit(seed); break;
       case 5: bytes = rpx::config_bottle(seed); break;
+      case 6: bytes = rpx::config_bottle(seed, true); break;
       default: return 0;
     }
     last_kind = kind;

@@ -1255,42 +1255,54 @@ __device__ __forceinline__ float cone_omc(float r, float d) {
   return sin2 / (1.0f + cos);
 }
 
+// The cone from x to caster sphere i: its axis, 1 - cos of its half angle (0: x is inside the
+// sphere, which then takes the hemisphere or the sphere) and its weight in the mixture.
+__device__ __forceinline__ void caster_cone(const DevCaustics& C, u32 i, V3 x, bool one_sided, V3 n, V3& axis,
+                                            float& omc, float& weight) {
+  V3 d = C.sphere_c[i] - x;
+  float dist = length(d);
+  axis = dist > 0.0f ? d / dist : v3(0.0f, 0.0f, 1.0f);
+  float omega;
+  if (dist <= C.sphere_r[i]) {
+    omc = 0.0f;
+    omega = PI;
+  } else {
+    omc = cone_omc(C.sphere_r[i], dist);
+    omega = TAU * omc;
+  }
+  float facing = one_sided ? 0.05f + fmax_r(dot(n, axis), 0.0f) : 1.0f;
+  weight = omega * facing;
+}
+
 // caustics.rs:495-559: a direction from x into the casters' cones and its density under the
-// mixture of every cone (`one_sided`: the emitting side n of a one-sided light).
+// mixture of every cone (`one_sided`: the emitting side n of a one-sided light). The cones'
+// cdf (light.rs:636-654) is walked again rather than kept, so nothing lands in scratch memory.
 __device__ bool photon_towards(const DevCaustics& C, V3 x, bool one_sided, V3 n, float u_pick, float u0, float u1,
                                V3& dir, float& pdf_out) {
-  u32 count = C.n_spheres;
-  V3 axes[CAUSTIC_MAX_SPHERES];
-  float omcs[CAUSTIC_MAX_SPHERES], cdf[CAUSTIC_MAX_SPHERES];
+  const u32 count = C.n_spheres;
+  V3 axis;
+  float omc, w;
   float total = -0.0f;
   for (u32 i = 0; i < count; ++i) {
-    V3 d = C.sphere_c[i] - x;
-    float dist = length(d);
-    V3 axis = dist > 0.0f ? d / dist : v3(0.0f, 0.0f, 1.0f);
-    float omc, omega;
-    if (dist <= C.sphere_r[i]) {
-      omc = 0.0f;
-      omega = PI;
-    } else {
-      omc = cone_omc(C.sphere_r[i], dist);
-      omega = TAU * omc;
-    }
-    float facing = one_sided ? 0.05f + fmax_r(dot(n, axis), 0.0f) : 1.0f;
-    axes[i] = axis;
-    omcs[i] = omc;
-    cdf[i] = omega * facing;  // the weight, turned into the cdf below
-    total = total + cdf[i];
+    caster_cone(C, i, x, one_sided, n, axis, omc, w);
+    total = total + w;
   }
   if (!(total > 0.0f && is_finite(total))) return false;
+  // pick (light.rs:1059-1074): the first cone whose cdf end passes u_pick.
+  u32 k = count - 1;
   float acc = 0.0f;
   for (u32 i = 0; i < count; ++i) {
-    acc += cdf[i];
-    cdf[i] = acc / total;
+    caster_cone(C, i, x, one_sided, n, axis, omc, w);
+    acc += w;
+    float end = i == count - 1 ? 1.0f : acc / total;
+    if (!(end <= u_pick)) {
+      k = i;
+      break;
+    }
   }
-  cdf[count - 1] = 1.0f;
-  u32 k = pick_bucket(cdf, count, u_pick);
-  if (omcs[k] > 0.0f) {
-    dir = normalize(to_world(frame_from_normal(axes[k]), uniform_cone(u0, u1, omcs[k])));
+  caster_cone(C, k, x, one_sided, n, axis, omc, w);
+  if (omc > 0.0f) {
+    dir = normalize(to_world(frame_from_normal(axis), uniform_cone(u0, u1, omc)));
   } else if (one_sided) {
     dir = normalize(to_world(frame_from_normal(n), cosine_hemisphere(u0, u1)));
   } else {
@@ -1301,11 +1313,17 @@ __device__ bool photon_towards(const DevCaustics& C, V3 x, bool one_sided, V3 n,
     dir = v3(r * cs, r * sn, z);
   }
   float pdf = 0.0f;
+  acc = 0.0f;
+  float prev = 0.0f;
   for (u32 i = 0; i < count; ++i) {
-    float p = cdf[i] - (i == 0 ? 0.0f : cdf[i - 1]);
+    caster_cone(C, i, x, one_sided, n, axis, omc, w);
+    acc += w;
+    float end = i == count - 1 ? 1.0f : acc / total;
+    float p = end - prev;
+    prev = end;
     if (!(p > 0.0f)) continue;
-    if (omcs[i] > 0.0f) {
-      if (1.0f - dot(dir, axes[i]) <= omcs[i]) pdf += p / (TAU * omcs[i]);
+    if (omc > 0.0f) {
+      if (1.0f - dot(dir, axis) <= omc) pdf += p / (TAU * omc);
     } else {
       pdf += p * (one_sided ? fmax_r(dot(dir, n), 0.0f) * FRAC_1_PI : 0.25f * FRAC_1_PI);
     }
@@ -1354,7 +1372,13 @@ __device__ bool photon_emit(const DevScene& S, Pcg32& rng, V3& o, V3& d, C3& flu
   if (kind == 2) {
     const DevEmitter& E = S.emitters[idx];
     const float* cdf = S.emit_cdf + E.first;
-    u32 t = pick_bucket(cdf, E.count, up0);
+    u32 lo_i = 0, hi_i = E.count;  // the first triangle whose cdf end passes up0, by bisection
+    while (lo_i < hi_i) {
+      u32 mid = (lo_i + hi_i) >> 1;
+      if (cdf[mid] <= up0) lo_i = mid + 1;
+      else hi_i = mid;
+    }
+    u32 t = lo_i < E.count - 1 ? lo_i : E.count - 1;
     float lo = t == 0 ? 0.0f : cdf[t - 1];
     float width = cdf[t] - lo;
     float rest = width > 0.0f ? clampf((up0 - lo) / width, 0.0f, 0.99999994f) : 0.0f;
@@ -1395,20 +1419,22 @@ __device__ bool photon_emit(const DevScene& S, Pcg32& rng, V3& o, V3& d, C3& flu
     float s2 = 2.0f * L.one_minus_cos_max - L.one_minus_cos_max * L.one_minus_cos_max;
     tilt = sqrt_(fmax_r(s2, 0.0f)) * 4.0f * C.scene_radius;
   }
-  float cdf[CAUSTIC_MAX_SPHERES];
+  // The sphere discs projected on the plane, by area (light.rs:636-654), walked again rather
+  // than kept.
+  const u32 count = C.n_spheres;
   float total = -0.0f;
-  for (u32 i = 0; i < C.n_spheres; ++i) {
-    cdf[i] = PI * (C.sphere_r[i] + tilt) * (C.sphere_r[i] + tilt);
-    total = total + cdf[i];
-  }
+  for (u32 i = 0; i < count; ++i) total = total + PI * (C.sphere_r[i] + tilt) * (C.sphere_r[i] + tilt);
   if (!(total > 0.0f && is_finite(total))) return false;
+  u32 k = count - 1;
   float acc = 0.0f;
-  for (u32 i = 0; i < C.n_spheres; ++i) {
-    acc += cdf[i];
-    cdf[i] = acc / total;
+  for (u32 i = 0; i < count; ++i) {
+    acc += PI * (C.sphere_r[i] + tilt) * (C.sphere_r[i] + tilt);
+    float end = i == count - 1 ? 1.0f : acc / total;
+    if (!(end <= u_pick)) {
+      k = i;
+      break;
+    }
   }
-  cdf[C.n_spheres - 1] = 1.0f;
-  u32 k = pick_bucket(cdf, C.n_spheres, u_pick);
   Frame frame = frame_from_normal(to_sun);
   float dx, dy;
   concentric_disk(up0, up1, dx, dy);
@@ -1416,11 +1442,16 @@ __device__ bool photon_emit(const DevScene& S, Pcg32& rng, V3& o, V3& d, C3& flu
   V3 ck = C.sphere_c[k];
   V3 x = ((ck - to_sun * dot(ck - plane, to_sun)) + frame.x * (dx * rk)) + frame.y * (dy * rk);
   float pdf_area = 0.0f;
-  for (u32 i = 0; i < C.n_spheres; ++i) {
+  acc = 0.0f;
+  float prev = 0.0f;
+  for (u32 i = 0; i < count; ++i) {
     float ri = C.sphere_r[i] + tilt;
+    acc += PI * ri * ri;
+    float end = i == count - 1 ? 1.0f : acc / total;
+    float p = end - prev;
+    prev = end;
     V3 ci = C.sphere_c[i];
-    if (length_sq(x - (ci - to_sun * dot(ci - plane, to_sun))) <= ri * ri)
-      pdf_area += (cdf[i] - (i == 0 ? 0.0f : cdf[i - 1])) / (PI * ri * ri * 1.0e-6f);
+    if (length_sq(x - (ci - to_sun * dot(ci - plane, to_sun))) <= ri * ri) pdf_area += p / (PI * ri * ri * 1.0e-6f);
   }
   if (!(pdf_area > 0.0f)) return false;
   flux = L.radiance * (cos / (pdf_area * pdf_dir * p_light * n));
@@ -1729,10 +1760,15 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
         if (!caster) {
           u32 budget = S.max_depth > bounce + 1 ? S.max_depth - (bounce + 1) : 0u;
           V3 side = exit ? -s_geometric : s_geometric;
-          C3 c = photon_gather(*S.pmap, s_hit_point, side, budget, [&](V3 w) {
-            V3 wl = to_local(frame, w);
-            return exit ? exit_eval(exit_eta2, wo, wl) : bsdf_eval(T, B, wo, wl, false);
-          });
+          // The map of this path's sample pass (the engine runs at most MAX_PASS_MAPS at once).
+          const DevPhotonMaps& maps = *S.pmaps;
+          u32 slot = sample_index - maps.first;
+          C3 c = splat(0.0f);
+          if (slot < maps.count)
+            c = photon_gather(maps.map[slot], s_hit_point, side, budget, [&](V3 w) {
+              V3 wl = to_local(frame, w);
+              return exit ? exit_eval(exit_eta2, wo, wl) : bsdf_eval(T, B, wo, wl, false);
+            });
           add_to(sum, thr * c, (flags & FLAG_CAU_RECV) != 0, S.clamp_indirect);
           next_cflags = FLAG_CAU_RECV;
         } else if (flags & FLAG_CAU_RECV) {
@@ -2689,7 +2725,7 @@ __device__ bool trace_photon(const DevScene& S, u32* col, const PhotonBuffers& B
 
 // One lane per photon; the stored ones are counted with one atomic per wave.
 template <int BLK, int LDSN>
-__global__ void __launch_bounds__(BLK) k_photons(const DevScene* __restrict__ scene, u32 photons, PhotonBuffers B,
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k_photons(const DevScene* __restrict__ scene, u32 photons, PhotonBuffers B,
                                                  u32 iteration) {
   __shared__ u32 lds[LDSN * BLK];
   u32* col = lds + threadIdx.x;
@@ -3244,7 +3280,7 @@ hipError_t launch_trace_rays(const DevScene* scene, const float* rays, u32 n, u3
 }
 hipError_t launch_photons(const DevScene* scene, u32 photons, const PhotonBuffers& B, u32 iteration, hipStream_t st) {
   if (photons == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_photons<128, 16>), dim3(grid_for(photons, 128, 1u << 16)), dim3(128), 0, st, scene, photons, B,
+  hipLaunchKernelGGL((k_photons<128, 32>), dim3(grid_for(photons, 128, 1u << 16)), dim3(128), 0, st, scene, photons, B,
                      iteration);
   return hipGetLastError();
 }

@@ -202,6 +202,14 @@ struct DevPhotonMap {
   const u32* starts;
 };
 
+// The photon maps of the sample passes a render is running: pass `first + i` gathers from
+// map[i], i < count.
+constexpr u32 MAX_PASS_MAPS = 16;
+struct DevPhotonMaps {
+  u32 first, count, pad0, pad1;
+  DevPhotonMap map[MAX_PASS_MAPS];
+};
+
 // caustics.rs:760-790: the grid cell of a point (cells one diameter wide) and its bucket.
 RPX_HD void photon_cell(V3 p, float inv_cell, i32 c[3]) {
   float v[3] = {p.x, p.y, p.z};
@@ -251,7 +259,7 @@ struct DevScene {
   // the pass being rendered.
   u32 caustics_on;
   DevCaustics caustics;
-  const DevPhotonMap* pmap;
+  const DevPhotonMaps* pmaps;
   // The scene has sphere lights, suns, emissive meshes sampled as lights or scattering media (the
   // kernels' extended variant).
   u32 ext;

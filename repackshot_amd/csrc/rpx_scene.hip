@@ -723,7 +723,7 @@ void caustic_setup(const HostScene& s, const std::vector<HostMesh>& meshes, cons
                    FlatScene& out) {
   DevScene& k = out.consts;
   k.caustics_on = 0;
-  k.pmap = nullptr;
+  k.pmaps = nullptr;
   if (s.caustic_photons == 0) return;
   DevCaustics& c = k.caustics;
   std::vector<size_t> parts;

@@ -273,11 +273,11 @@ class Scene:
 
     def kernel_times(self):
         """Per-kernel GPU ms of the last profiled render (rpx_last_kernel_ms)."""
-        out = np.zeros(8, np.float32)
+        out = np.zeros(10, np.float32)
         _check(library().rpx_last_kernel_ms(self.handle, _ptr(out)), "rpx_last_kernel_ms")
         keys = ["extend_kernel_ms", "extend_launches", "shadow_kernel_ms", "shadow_launches", "phase_ms", "walk_ms",
-                "drain_ms", "shade_ms"]
-        return {k: (int(v) if k.endswith("launches") else float(v)) for k, v in zip(keys, out)}
+                "drain_ms", "shade_ms", "photon_map_ms", "photon_maps_traced"]
+        return {k: (int(v) if k.endswith("launches") or k.endswith("traced") else float(v)) for k, v in zip(keys, out)}
 
     def render_tile_noalloc(self, x, y, w, h, sample_index):
         """render_tile through the allocation-free entry point (tiles of at most 16384 pixels)."""
