@@ -19,11 +19,14 @@ the timed render.
 The media record renders the caustics-and-media workload (a glass bottle holding a scattering
 medium, lit by a sphere light; 1920 x 1080, 64 spp, depth 16) with the walks in their own kernel.
 
+The caustics record renders the bottle of clear glass under a glossy disc light with 4,194,304
+caustic photons per sample pass, the photon maps traced and built on the GPU.
+
 The adaptive record renders the same scene adaptively (8 x 8 blocks in rounds to a 3 % noise target
 or 64 samples) with the decisions on the GPU between rounds.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-queue] [--no-adjoint]
-                       [--no-adaptive] [--no-denoise] [--no-media]
+                       [--no-adaptive] [--no-denoise] [--no-media] [--no-caustics]
                        [--pool P --shape S --shadow-shape T]   (fixed launch options instead of the tuner)
 """
 import argparse
@@ -399,6 +402,74 @@ def media_bench(native, tuned, seed=0, tiles=((944, 320), (944, 480), (912, 640)
     return rec
 
 
+def caustics_bench(native, tuned, seed=0, tiles=((864, 800), (832, 800), (800, 832), (768, 768))):
+    """Caustics: the bottle of clear glass under a glossy disc light (rpx_write_scene(6); 1920 x 1080,
+    64 spp, depth 16): before each chunk of up to 16 sample passes the GPU traces 4,194,304 photons
+    per pass and builds each pass's hashed-grid map (radix sort by cell bucket); every receiver the
+    camera paths meet gathers from its pass's map. Timed whole into the resident film with the
+    large render's launch options. Parity: the GPU film of samples [0, 2) against the oracle's on
+    32 x 32 tiles of the caustic under the bottle (the oracle traces the same two photon maps on
+    this host's cores), whose time is the CPU baseline."""
+    from oracle import oracle as orc
+
+    rsb = native.write_scene(6, seed)
+    scene = native.Scene(rsb)
+    info = scene.info()
+    W, H, spp = info["width"], info["height"], info["max_samples"]
+    photons = scene.caustics()
+    scene.set_options(pool=tuned["pool"], shape=tuned["shape"], shadow_shape=tuned["shadow_shape"])
+    scene.render_samples(spp, 16)  # warm: the path pool and the kept maps' memory
+    scene.reset_film()
+    s0 = scene.stats()
+    t0 = time.perf_counter()
+    scene.render_samples(0, spp)
+    wall = time.perf_counter() - t0
+    s1 = scene.stats()
+    scene.set_profiling(True)
+    scene.reset_film()
+    scene.render_samples(2 * spp, spp)  # passes whose maps are not kept
+    timing = scene.last_timing()
+    kernels = scene.kernel_times()
+    scene.set_profiling(False)
+    scene.reset_film()
+    scene.render_samples(0, 2)
+    rgb, weight, alpha, lum2 = scene.film()
+    oracle = orc.OracleScene(rsb)
+    rects = [(x, y, 32, 32) for x, y in tiles]
+    threads = os.cpu_count() or 1
+    t1 = time.perf_counter()
+    ofilm, _ = oracle.render_tiles(rects, 0, 2, threads)
+    cpu_s = time.perf_counter() - t1
+    exact = all(np.array_equal(g[y:y + h, x:x + w].view(np.uint32), c[y:y + h, x:x + w].view(np.uint32))
+                for (x, y, w, h) in rects for g, c in ((rgb, ofilm[0]), (weight, ofilm[1]), (alpha, ofilm[2]),
+                                                      (lum2, ofilm[3])))
+    lit = float(np.mean([rgb[y:y + h, x:x + w].mean() for (x, y, w, h) in rects]))
+    paths = W * H * spp
+    map_ms = kernels["photon_map_ms"]
+    rec = {"workload": "caustics: 1920x1080, 64 spp, max depth 16, a 200,000-triangle bottle of clear glass "
+                       "(IOR 1.5) on a Lambert floor under a glossy disc light (100 mm, radiance 500), %d caustic "
+                       "photons per sample pass (the most a scene may ask for), first gather radius 3 pixels"
+                       % photons,
+           "width": W, "height": H, "spp": spp, "triangles": info["triangles"], "photons_per_pass": photons,
+           "mpaths_per_s": round(paths / wall / 1e6, 3), "ms_per_step": round(1000.0 * wall, 2),
+           "mrays_per_s": round((s1["rays"] - s0["rays"]) / wall / 1e6, 3),
+           "photon_maps_ms_per_step": round(map_ms, 2),
+           "mphotons_per_s": round(photons * spp / (map_ms * 1e-3) / 1e6, 1) if map_ms > 0 else None,
+           "photon_map_share": round(map_ms / timing["total_ms"], 3) if timing["total_ms"] > 0 else None,
+           "stage_ms_per_step": {k: round(timing[k], 2) for k in ("extend_ms", "shade_ms", "shadow_ms", "regen_ms",
+                                                                   "drain_ms", "total_ms")},
+           "shade_includes_gather": True,
+           "parity": {"tiles": len(rects), "samples": 2, "film_bit_exact_vs_oracle": bool(exact),
+                      "tiles_mean_rgb": round(lit, 4)},
+           "cpu_baseline": {"value": round(len(rects) * 1024 * 2 / cpu_s / 1e6, 5), "unit": "Mpaths/s",
+                            "cores": threads, "kind": "port",
+                            "sample": "%d tiles of 32 x 32 at 2 spp under the bottle, the two passes' photon maps "
+                                      "(%d photons each) traced first" % (len(rects), photons)}}
+    scene.close()
+    log("bench: caustics %s" % rec)
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -411,6 +482,7 @@ def main():
     ap.add_argument("--no-adaptive", action="store_true")
     ap.add_argument("--no-denoise", action="store_true")
     ap.add_argument("--no-media", action="store_true")
+    ap.add_argument("--no-caustics", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--pool", type=int, default=0)
     ap.add_argument("--shape", type=int, default=-1)
@@ -597,6 +669,8 @@ def main():
         result["adaptive"] = adaptive_bench(native, rsb, tuned)
     if not args.no_media:
         result["media"] = media_bench(native, tuned, args.seed)
+    if not args.no_caustics:
+        result["caustics"] = caustics_bench(native, tuned, args.seed)
     if not args.no_queue:
         result["mixed_queue"] = mixed_queue(native, args.seed)
     if not args.no_adjoint:

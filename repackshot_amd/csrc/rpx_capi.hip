@@ -132,7 +132,7 @@ bool build_device_scene(Handle* h, std::string& err) {
         !upload(B, flat.caustic_cdf, d.caustics.source_cdf, st, err))
       return false;
     auto maps = std::make_unique<PhotonPasses>();
-    if (!maps->init(d.caustics.photons, d.caustics.radius, st, err)) return false;
+    if (!maps->init(d.caustics.photons, d.caustics.max_bounces, d.caustics.radius, st, err)) return false;
     d.pmaps = maps->table();
     h->photons = std::move(maps);
   }

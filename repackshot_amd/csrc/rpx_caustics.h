@@ -22,7 +22,7 @@ class PhotonPasses {
   PhotonPasses& operator=(const PhotonPasses&) = delete;
   ~PhotonPasses();
   // Device memory for tracing `photons` photons per pass; the header starts as an empty map.
-  bool init(u32 photons, float first_radius, hipStream_t st, std::string& err);
+  bool init(u32 photons, u32 max_bounces, float first_radius, hipStream_t st, std::string& err);
   // Makes the maps of passes iterations[0 .. n) current (n <= MAX_PASS_MAPS): sample pass
   // first + i gathers from the map of iterations[i]. Kept maps are reused, the others traced and
   // built from the scene (`d_scene`: its device record, whose pmaps is table()), all on `st`.
@@ -62,6 +62,9 @@ class PhotonPasses {
   float first_radius_ = 1.0f;
   DevPhotonMaps* table_ = nullptr;  // the current maps, read by the shade
   PhotonBuffers raw_{};
+  PhotonPaths paths_{};
+  u32 max_steps_ = 0;
+  u32* host_count_ = nullptr;  // pinned
   u32 *keys_sorted_ = nullptr, *index_ = nullptr, *index_sorted_ = nullptr;
   void* sort_tmp_ = nullptr;
   size_t sort_bytes_ = 0;

@@ -14,8 +14,6 @@ constexpr u32 KEY_BITS = 25;  // buckets below 2^24 and PHOTON_NONE = 2^24
 constexpr float MM_TO_M = 1.0e-3f;
 constexpr float RADIUS_ALPHA = 2.0f / 3.0f;
 
-__global__ void k_map_reset(u32* stored) { *stored = 0; }
-
 __global__ void __launch_bounds__(256) k_map_iota(u32* index, u32 n) {
   for (u32 i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) index[i] = i;
 }
@@ -111,15 +109,22 @@ PhotonPasses::~PhotonPasses() {
   for (Kept& k : kept_)
     if (k.block) (void)hipFree(k.block);
   if (block_) (void)hipFree(block_);
+  if (host_count_) (void)hipHostFree(host_count_);
 }
 
-bool PhotonPasses::init(u32 photons, float first_radius, hipStream_t st, std::string& err) {
+bool PhotonPasses::init(u32 photons, u32 max_bounces, float first_radius, hipStream_t st, std::string& err) {
   if (photons == 0) {
     err = "a scene with caustics traces at least one photon per pass";
     return false;
   }
   photons_ = photons;
   first_radius_ = first_radius;
+  max_steps_ = max_bounces + 128;  // caustics.rs:609: the most closest hits a photon takes
+  if (hipHostMalloc((void**)&host_count_, 64, hipHostMallocDefault) != hipSuccess) {
+    host_count_ = nullptr;
+    err = "photon map: cannot allocate the pinned step counter";
+    return false;
+  }
   u32 buckets = 1;
   while (buckets < 2u * photons && buckets < MAX_BUCKETS) buckets <<= 1;
   buckets_cap_ = buckets;
@@ -131,10 +136,13 @@ bool PhotonPasses::init(u32 photons, float first_radius, hipStream_t st, std::st
     return false;
   }
   // One block: the table of current maps, the traced records (4 x float4) and keys, the sorted
-  // keys, the index, the sorted index, the sort's workspace, the stored counter.
+  // keys, the index, the sorted index, the sort's workspace, the stored counter, then the photons
+  // in flight (4 x float4, the PCG32 states, the hits, two queues, their counts).
   size_t sizes[] = {round_up(sizeof(DevPhotonMaps)), round_up(16 * n), round_up(16 * n), round_up(16 * n),
                     round_up(16 * n), round_up(4 * n), round_up(4 * n), round_up(4 * n), round_up(4 * n),
-                    round_up(sort_bytes_ ? sort_bytes_ : 1), round_up(4)};
+                    round_up(sort_bytes_ ? sort_bytes_ : 1), round_up(4),
+                    round_up(16 * n), round_up(16 * n), round_up(16 * n), round_up(16 * n), round_up(8 * n),
+                    round_up(16 * n), round_up(4 * n), round_up(4 * n), round_up(8)};
   size_t total = 0;
   for (size_t b : sizes) total += b;
   if (hipMalloc(&block_, total) != hipSuccess) {
@@ -159,6 +167,15 @@ bool PhotonPasses::init(u32 photons, float first_radius, hipStream_t st, std::st
   index_sorted_ = (u32*)take(8);
   sort_tmp_ = take(9);
   raw_.stored = (u32*)take(10);
+  paths_.o = (float4*)take(11);
+  paths_.d = (float4*)take(12);
+  paths_.power = (float4*)take(13);
+  paths_.sigma = (float4*)take(14);
+  paths_.rng = (u64*)take(15);
+  paths_.hit = (uint4*)take(16);
+  paths_.live[0] = (u32*)take(17);
+  paths_.live[1] = (u32*)take(18);
+  paths_.counts = (u32*)take(19);
   if (!add_slot(st, err)) return false;
   // Until the first render, no pass has a map.
   DevPhotonMaps none{};
@@ -257,9 +274,7 @@ hipError_t PhotonPasses::trace(const DevScene* d_scene, u32 iteration, hipStream
   float inv_cell = 1.0f / (2.0f * radius);
   float inv_r2 = 1.0f / (radius * radius);
   float norm = 2.0f / (PI * r_m * r_m);
-  hipLaunchKernelGGL(k_map_reset, dim3(1), dim3(1), 0, st, raw_.stored);
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = launch_photons(d_scene, photons_, raw_, iteration, st);
+  hipError_t e = launch_photons(d_scene, photons_, max_steps_, raw_, paths_, iteration, host_count_, st);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_map_header, dim3(1), dim3(1), 0, st, K.header, raw_.stored, iteration, radius, inv_r2, norm,
                        inv_cell);

@@ -193,9 +193,22 @@ struct PhotonBuffers {
   u32* stored;
 };
 constexpr u32 PHOTON_STORED = 0, PHOTON_NONE = 1u << 24;  // above every bucket (at most 2^24)
-// Traces photons [0, scene.caustics.photons) of pass `iteration` (caustics.rs:581-735). The
-// caller zeroes `stored` first.
-hipError_t launch_photons(const DevScene* scene, u32 photons, const PhotonBuffers& B, u32 iteration, hipStream_t st);
+// The photons in flight, by photon index: ray origin (w: the roulette's throughput), direction
+// (w: casters crossed), power (w: inside an absorbing solid), that solid's coefficients, the
+// state of the photon's PCG32 stream, its closest hit; two queues of photon indices (a step reads
+// one and appends the survivors to the other) and their counts.
+struct PhotonPaths {
+  float4 *o, *d, *power, *sigma;
+  u64* rng;
+  uint4* hit;
+  u32* live[2];
+  u32* counts;  // 2
+};
+// Traces photons [0, photons) of pass `iteration` (caustics.rs:581-735) for at most `max_steps`
+// closest hits each, storing them in B; `host_count` (pinned) receives the queue counts the
+// launcher checks between steps.
+hipError_t launch_photons(const DevScene* scene, u32 photons, u32 max_steps, const PhotonBuffers& B,
+                          const PhotonPaths& Q, u32 iteration, u32* host_count, hipStream_t st);
 
 // The Sobol generator matrices (Joe–Kuo new-joe-kuo-6.21201, 64 dimensions) in the reference
 // sampler's logical-dimension layout: column k of logical dimension d at [d * 32 + k].

@@ -2619,124 +2619,205 @@ __global__ void __launch_bounds__(128) k_trace_rays(const DevScene* __restrict__
 }
 
 // ================================================================== caustic photons: tracing
+// caustics.rs:581-735 as a wavefront: photons leave their lights (k_photon_emit), then each step
+// traces the closest hit of every photon still in flight (k_photon_trace, the lean traversal
+// kernel) and lets it scatter (k_photon_scatter: stored on the first receiver unless it crossed no
+// caster, through a caster by BSDF sampling, roulette from three casters); the survivors go to the
+// other queue. A photon's random numbers come from its own PCG32 stream, so the order in which
+// photons are processed changes nothing.
 namespace {
-// caustics.rs:581-735: photon `index` of pass `iteration` from its light through the casters to
-// the first receiver, where it is stored unless it crossed no caster. True when it was stored.
-template <int BLK, int LDSN>
-__device__ bool trace_photon(const DevScene& S, u32* col, const PhotonBuffers& B, u32 iteration, u32 index) {
-  const DevCaustics& C = S.caustics;
+// Appends `value` to `queue` for the lanes that want it: one atomic per wave.
+__device__ __forceinline__ void wave_append(u32* queue, u32* count, bool want, u32 value) {
+  u64 m = __ballot(want);
+  if (m == 0) return;
+  u32 lane = __lane_id();
+  u32 leader = (u32)(__ffsll((long long)m) - 1);
+  u32 base = 0;
+  if (lane == leader) base = atomicAdd(count, (u32)__popcll(m));
+  base = (u32)__shfl((int)base, (int)leader);
+  if (want) queue[base + (u32)__popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+__device__ __forceinline__ void wave_count(u32* count, bool want) {
+  u64 m = __ballot(want);
+  if (m != 0 && __lane_id() == (u32)(__ffsll((long long)m) - 1)) atomicAdd(count, (u32)__popcll(m));
+}
+__global__ void k_clear_u32(u32* p, u32 n) {
+  for (u32 i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+__device__ __forceinline__ u64 photon_seed(const DevCaustics& C, u32 iteration) {
   u64 key = ((u64)C.seed << 32) | (u64)iteration;
-  Pcg32 rng = Pcg32::seeded(mix64(key ^ 0x0e5b0001c0decafeull), (u64)index);
-  V3 o, d;
-  C3 power;
-  if (!photon_emit(S, rng, o, d, power)) return false;
-  if (!(peak(power) > 0.0f && is_finite(peak(power)))) return false;
-  Geo g{S.nodes, S.tris, S.instances, S.tlas_root};
-  float throughput = 1.0f;
-  u32 bounces = 0;
-  bool in_medium = false;
-  C3 sigma = splat(0.0f);
-  V3 at = v3(0.0f, 0.0f, 0.0f), normal_at = v3(0.0f, 0.0f, 1.0f);
-  bool stored = false;
-  for (u32 step = 0; step < C.max_bounces + 128; ++step) {
-    u32 hi = 0xffffffffu, ht = 0xffffffffu;
-    float th = RPX_INF;
-    bool has_hit = trace_nearest<BLK, LDSN>(g, col, o, d, RAY_EPSILON, RPX_INF, false, hi, ht, th);
-    float t_geometry = has_hit ? th : RPX_INF;
-    float t_floor = RPX_INF;
-    if (S.floor_mode == 1 && d.z != 0.0f) {
-      float tf = (S.floor_z - o.z) / d.z;
-      if (tf > RAY_EPSILON && tf < RPX_INF) t_floor = tf;
-    }
-    float t_nearest = fmin_r(t_geometry, t_floor);
-    if (in_medium && is_finite(t_nearest))
-      power = power * c3(fexp(-sigma.r * t_nearest), fexp(-sigma.g * t_nearest), fexp(-sigma.b * t_nearest));
-    if (t_nearest == RPX_INF) break;
-    if (t_floor < t_geometry) {
-      if (bounces > 0) {
-        V3 p = o + d * t_floor;
-        at = v3(p.x, p.y, S.floor_z);
-        normal_at = v3(0.0f, 0.0f, d.z < 0.0f ? 1.0f : -1.0f);
-        stored = true;
-      }
-      break;
-    }
-    SurfaceHit sh = surface_hit(S, o, d, make_uint4(hi, ht, f2u(th), 0u));
-    u32 mat = S.instances[sh.inst].material;
-    bool has_mat = mat < S.n_materials;
-    if (has_mat && S.materials[mat].emissive) break;
-    V3 ng = sh.geometric;
-    if (!(S.instances[sh.inst].flags & INST_CASTER)) {
-      // A receiver ends the photon (no thin walls in the supported scenes).
-      if (bounces > 0) {
-        at = sh.hit_point;
-        normal_at = ng;
-        stored = true;
-      }
-      break;
-    }
-    if (bounces >= C.max_bounces) break;
-    Frame frame = frame_from_normal(sh.normal);
-    V3 wo_world = -d;
-    V3 wo = to_local(frame, wo_world);
-    float u0 = rng.next_f32();
-    float u1 = rng.next_f32();
-    const Bsdf& Bs = has_mat ? S.bsdfs[2 * mat + (sh.outside ? 0 : 1)] : S.fallback_bsdf[sh.outside ? 0 : 1];
-    BsdfSample bs;
-    if (!bsdf_sample(S.tables, Bs, wo, u0, u1, bs)) break;
-    bool transmitted = bs.wi.z < 0.0f;
-    V3 wi = normalize(to_world(frame, bs.wi));
-    if (transmitted ? dot(wi, ng) >= 0.0f : dot(wi, ng) <= 0.0f) break;
-    C3 weight = bs.weight;
-    if (transmitted) weight = weight * (Bs.eta * Bs.eta);
-    // The adjoint BSDF's shading-normal correction (Veach), bounded.
-    float num = fabs_(dot(wo_world, sh.normal)) * fabs_(dot(wi, ng));
-    float den = fabs_(dot(wo_world, ng)) * fabs_(dot(wi, sh.normal));
-    if (den > 0.0f) weight = weight * fmin_r(num / den, 10.0f);
-    ++bounces;
-    power = power * weight;
-    throughput *= peak(weight);
-    if (bounces >= 3) {
-      float survive = clampf(throughput, 0.05f, 1.0f);
-      if (rng.next_f32() >= survive) break;
-      power = power * (1.0f / survive);
-      throughput /= survive;
-    }
-    if (!(peak(power) > 0.0f && is_finite(peak(power)))) break;
-    V3 next_o = transmitted ? offset_origin(sh.hit_point, -ng) : offset_origin(sh.point, ng);
-    if (transmitted) {
-      in_medium = false;
-      if (sh.outside && has_mat && S.materials[mat].has_medium == MEDIUM_ABSORBING) {
-        in_medium = true;
-        sigma = S.materials[mat].sigma;
-      }
-    }
-    o = next_o;
-    d = wi;
-  }
-  if (!stored) return false;
-  B.position[index] = make_float4(at.x, at.y, at.z, u2f(bounces));
-  B.wi[index] = make_float4(-d.x, -d.y, -d.z, 0.0f);
-  B.power[index] = make_float4(power.r, power.g, power.b, 0.0f);
-  B.normal[index] = make_float4(normal_at.x, normal_at.y, normal_at.z, 0.0f);
-  return true;
+  return mix64(key ^ 0x0e5b0001c0decafeull);
 }
 }  // namespace
 
-// One lane per photon; the stored ones are counted with one atomic per wave.
-template <int BLK, int LDSN>
-__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k_photons(const DevScene* __restrict__ scene, u32 photons, PhotonBuffers B,
-                                                 u32 iteration) {
-  __shared__ u32 lds[LDSN * BLK];
-  u32* col = lds + threadIdx.x;
+__global__ void __launch_bounds__(256) k_photon_emit(const DevScene* __restrict__ scene, u32 photons, PhotonBuffers B,
+                                                     PhotonPaths Q, u32 iteration) {
   const DevScene& S = *scene;
+  const u64 seed = photon_seed(S.caustics, iteration);
   u32 stride = gridDim.x * blockDim.x;
   for (u32 base = blockIdx.x * blockDim.x; base < photons; base += stride) {
     u32 index = base + threadIdx.x;
-    bool stored = index < photons && trace_photon<BLK, LDSN>(S, col, B, iteration, index);
-    if (index < photons) B.key[index] = stored ? PHOTON_STORED : PHOTON_NONE;
-    u64 m = __ballot(stored);
-    if (m != 0 && __lane_id() == (u32)(__ffsll((long long)m) - 1)) atomicAdd(B.stored, (u32)__popcll(m));
+    bool live = false;
+    if (index < photons) {
+      B.key[index] = PHOTON_NONE;
+      Pcg32 rng = Pcg32::seeded(seed, (u64)index);
+      V3 o, d;
+      C3 power;
+      if (photon_emit(S, rng, o, d, power) && peak(power) > 0.0f && is_finite(peak(power))) {
+        Q.o[index] = make_float4(o.x, o.y, o.z, 1.0f);  // w: the roulette's throughput
+        Q.d[index] = make_float4(d.x, d.y, d.z, u2f(0u));  // w: casters crossed
+        Q.power[index] = make_float4(power.r, power.g, power.b, u2f(0u));  // w: inside an absorbing solid
+        Q.rng[index] = rng.state;
+        live = true;
+      }
+    }
+    wave_append(Q.live[0], Q.counts + 0, live, index);
+  }
+}
+
+template <int BLK, int LDSN>
+__global__ void __launch_bounds__(BLK) k_photon_trace(const DevScene* __restrict__ scene, PhotonPaths Q, u32 parity) {
+  __shared__ u32 lds[LDSN * BLK];
+  u32* col = lds + threadIdx.x;
+  const DevScene& S = *scene;
+  // The other queue was last read by the previous step's scatter: it starts empty for this one's.
+  if (blockIdx.x == 0 && threadIdx.x == 0) Q.counts[parity ^ 1u] = 0;
+  const u32* q = Q.live[parity];
+  u32 n = Q.counts[parity];
+  Geo g{S.nodes, S.tris, S.instances, S.tlas_root};
+  u32 stride = gridDim.x * blockDim.x;
+  for (u32 j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+    u32 i = q[j];
+    float4 o = Q.o[i], d = Q.d[i];
+    u32 hi = 0xffffffffu, ht = 0xffffffffu;
+    float t = RPX_INF;
+    trace_nearest<BLK, LDSN>(g, col, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), RAY_EPSILON, RPX_INF, false, hi, ht, t);
+    Q.hit[i] = make_uint4(hi, ht, f2u(t), 0u);
+  }
+}
+
+// One step of caustics.rs:600-735 after the closest hit; `last`: no step may follow this one.
+__global__ void __launch_bounds__(256) k_photon_scatter(const DevScene* __restrict__ scene, PhotonBuffers B,
+                                                        PhotonPaths Q, u32 parity, u32 iteration, u32 last) {
+  const DevScene& S = *scene;
+  const DevCaustics& C = S.caustics;
+  const u32* q = Q.live[parity];
+  u32 n = Q.counts[parity];
+  u32 stride = gridDim.x * blockDim.x;
+  for (u32 base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    u32 j = base + threadIdx.x;
+    bool more = false, stored = false;
+    u32 i = 0;
+    if (j < n) {
+      i = q[j];
+      float4 o4 = Q.o[i], d4 = Q.d[i], p4 = Q.power[i];
+      uint4 h = Q.hit[i];
+      V3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
+      float throughput = o4.w;
+      u32 bounces = f2u(d4.w);
+      C3 power = c3(p4.x, p4.y, p4.z);
+      bool in_medium = f2u(p4.w) != 0u;
+      bool has_hit = h.x != 0xffffffffu;
+      float t_geometry = has_hit ? u2f(h.z) : RPX_INF;
+      float t_floor = RPX_INF;
+      if (S.floor_mode == 1 && d.z != 0.0f) {
+        float tf = (S.floor_z - o.z) / d.z;
+        if (tf > RAY_EPSILON && tf < RPX_INF) t_floor = tf;
+      }
+      float t_nearest = fmin_r(t_geometry, t_floor);
+      if (in_medium && is_finite(t_nearest)) {
+        float4 sg = Q.sigma[i];
+        power = power * c3(fexp(-sg.x * t_nearest), fexp(-sg.y * t_nearest), fexp(-sg.z * t_nearest));
+      }
+      V3 at = v3(0.0f, 0.0f, 0.0f), normal_at = v3(0.0f, 0.0f, 1.0f);
+      if (t_nearest == RPX_INF) {
+        // escaped
+      } else if (t_floor < t_geometry) {
+        if (bounces > 0) {
+          V3 p = o + d * t_floor;
+          at = v3(p.x, p.y, S.floor_z);
+          normal_at = v3(0.0f, 0.0f, d.z < 0.0f ? 1.0f : -1.0f);
+          stored = true;
+        }
+      } else {
+        SurfaceHit sh = surface_hit(S, o, d, h);
+        u32 mat = S.instances[sh.inst].material;
+        bool has_mat = mat < S.n_materials;
+        V3 ng = sh.geometric;
+        if (has_mat && S.materials[mat].emissive) {
+          // absorbed by a light
+        } else if (!(S.instances[sh.inst].flags & INST_CASTER)) {
+          // A receiver ends the photon (no thin walls in the supported scenes).
+          if (bounces > 0) {
+            at = sh.hit_point;
+            normal_at = ng;
+            stored = true;
+          }
+        } else if (bounces < C.max_bounces) {
+          Pcg32 rng;
+          rng.state = Q.rng[i];
+          rng.inc = ((u64)i << 1) | 1ull;
+          Frame frame = frame_from_normal(sh.normal);
+          V3 wo_world = -d;
+          V3 wo = to_local(frame, wo_world);
+          float u0 = rng.next_f32();
+          float u1 = rng.next_f32();
+          const Bsdf& Bs = has_mat ? S.bsdfs[2 * mat + (sh.outside ? 0 : 1)] : S.fallback_bsdf[sh.outside ? 0 : 1];
+          BsdfSample bs;
+          if (bsdf_sample(S.tables, Bs, wo, u0, u1, bs)) {
+            bool transmitted = bs.wi.z < 0.0f;
+            V3 wi = normalize(to_world(frame, bs.wi));
+            if (!(transmitted ? dot(wi, ng) >= 0.0f : dot(wi, ng) <= 0.0f)) {
+              C3 weight = bs.weight;
+              if (transmitted) weight = weight * (Bs.eta * Bs.eta);
+              // The adjoint BSDF's shading-normal correction (Veach), bounded.
+              float num = fabs_(dot(wo_world, sh.normal)) * fabs_(dot(wi, ng));
+              float den = fabs_(dot(wo_world, ng)) * fabs_(dot(wi, sh.normal));
+              if (den > 0.0f) weight = weight * fmin_r(num / den, 10.0f);
+              ++bounces;
+              power = power * weight;
+              throughput *= peak(weight);
+              bool alive = true;
+              if (bounces >= 3) {
+                float survive = clampf(throughput, 0.05f, 1.0f);
+                if (rng.next_f32() >= survive) {
+                  alive = false;
+                } else {
+                  power = power * (1.0f / survive);
+                  throughput /= survive;
+                }
+              }
+              if (alive && peak(power) > 0.0f && is_finite(peak(power))) {
+                V3 next_o = transmitted ? offset_origin(sh.hit_point, -ng) : offset_origin(sh.point, ng);
+                if (transmitted) {
+                  in_medium = false;
+                  if (sh.outside && has_mat && S.materials[mat].has_medium == MEDIUM_ABSORBING) {
+                    in_medium = true;
+                    C3 sg = S.materials[mat].sigma;
+                    Q.sigma[i] = make_float4(sg.r, sg.g, sg.b, 0.0f);
+                  }
+                }
+                Q.o[i] = make_float4(next_o.x, next_o.y, next_o.z, throughput);
+                Q.d[i] = make_float4(wi.x, wi.y, wi.z, u2f(bounces));
+                Q.power[i] = make_float4(power.r, power.g, power.b, u2f(in_medium ? 1u : 0u));
+                Q.rng[i] = rng.state;
+                more = !last;
+              }
+            }
+          }
+        }
+      }
+      if (stored) {
+        B.position[i] = make_float4(at.x, at.y, at.z, u2f(bounces));
+        B.wi[i] = make_float4(-d.x, -d.y, -d.z, 0.0f);
+        B.power[i] = make_float4(power.r, power.g, power.b, 0.0f);
+        B.normal[i] = make_float4(normal_at.x, normal_at.y, normal_at.z, 0.0f);
+        B.key[i] = PHOTON_STORED;
+      }
+    }
+    (void)iteration;
+    wave_append(Q.live[parity ^ 1u], Q.counts + (parity ^ 1u), more, i);
+    wave_count(B.stored, stored);
   }
 }
 
@@ -3278,11 +3359,31 @@ hipError_t launch_trace_rays(const DevScene* scene, const float* rays, u32 n, u3
   hipLaunchKernelGGL(k_trace_rays, dim3(grid_for(n, 128, 8192)), dim3(128), 0, st, scene, rays, n, inst, tri, t, any);
   return hipGetLastError();
 }
-hipError_t launch_photons(const DevScene* scene, u32 photons, const PhotonBuffers& B, u32 iteration, hipStream_t st) {
+hipError_t launch_photons(const DevScene* scene, u32 photons, u32 max_steps, const PhotonBuffers& B,
+                          const PhotonPaths& Q, u32 iteration, u32* host_count, hipStream_t st) {
   if (photons == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_photons<128, 32>), dim3(grid_for(photons, 128, 1u << 16)), dim3(128), 0, st, scene, photons, B,
+  // The stored counter and the first queue start empty (fill kernels, not memsets).
+  hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, B.stored, 1u);
+  hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, st, Q.counts, 2u);
+  hipLaunchKernelGGL(k_photon_emit, dim3(grid_for(photons, 256, 1u << 14)), dim3(256), 0, st, scene, photons, B, Q,
                      iteration);
-  return hipGetLastError();
+  hipError_t e = hipGetLastError();
+  const u32 grid_t = grid_for(photons, 128, 8192), grid_s = grid_for(photons, 256, 4096);
+  u32 parity = 0;
+  for (u32 step = 0; step < max_steps && e == hipSuccess; ++step) {
+    hipLaunchKernelGGL((k_photon_trace<128, 24>), dim3(grid_t), dim3(128), 0, st, scene, Q, parity);
+    hipLaunchKernelGGL(k_photon_scatter, dim3(grid_s), dim3(256), 0, st, scene, B, Q, parity, iteration,
+                       step + 1 >= max_steps ? 1u : 0u);
+    e = hipGetLastError();
+    parity ^= 1u;
+    // Every few steps: stop once no photon is in flight.
+    if (e == hipSuccess && (step % 4 == 3 || step < 2)) {
+      e = hipMemcpyAsync(host_count, Q.counts + parity, 4, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e == hipSuccess && *host_count == 0) break;
+    }
+  }
+  return e;
 }
 hipError_t launch_passes(const DevScene* scene, const u32* sobol, const PassPlanes& Q, u32 x0, u32 y0, u32 w, u32 h,
                          u32 ox, u32 oy, u32 ow, u32 first, u32 count, u32 what, hipStream_t st) {
