@@ -103,7 +103,8 @@ std::vector<float> filter_band(const Input& in, Preset preset) {
   const float inv = 1.0f / (float)samples;
   std::vector<float> out(in.sums, in.sums + n * 4);
 
-  // Guides, the albedo divisor, the illumination and its variance.
+  // Per pixel: the edge-stopping guides, the divisor that takes the albedo out, the demodulated
+  // light and its variance.
   std::vector<Guide> guides(n);
   for (size_t p = 0; p < n; ++p) {
     Guide& g = guides[p];

@@ -1176,7 +1176,8 @@ WalkEnd walk(const Scene& scene, const PixelSampler& sampler, const ScatterMediu
       ray = Ray(x, dir);
       continue;
     }
-    // The boundary (or, past the last step, straight to it keeping all but absorption).
+    // No scattering before the surface (or the step cap is reached: the path then goes to the
+    // surface with only the absorption applied).
     if (std::isfinite(t_hit)) {
       if (capped) {
         w = w * transmittance(m.sigma_a, t_hit);
@@ -1204,7 +1205,7 @@ WalkEnd walk(const Scene& scene, const PixelSampler& sampler, const ScatterMediu
       pdf_out = pdf;
       return WALK_EXIT;
     }
-    // Reflected back in (a step of the path, for the cap).
+    // Fresnel sent it inside again: this counts against the walk's step cap like a scattering event.
     path_events += 1;
     Vec3 dir = (ray.dir + ng * (2.0f * cos)).normalize();
     ray = Ray(offset_origin(details.point, ng), dir);
@@ -1338,7 +1339,7 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
   u32 medium_events = 0;
   bool exit_pdf = false;
   for (;;) {
-    // After a walk that scattered: the exit lobe's albedo at the surface it reached.
+    // A scattering walk that reached its solid's surface leaves by the exit lobe, whose albedo is carried here.
     bool exit = false;
     float exit_eta2 = 0.0f;
     RayHit hit;
@@ -1381,7 +1382,7 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
     // Light of photon lights arriving now is the photon map's (integrator/mod.rs:1261-1272): the
     // area lights crossed are left out, the spheres stay.
     bool dropped = map && cstate == AFTER_RECEIVER_CASTERS;
-    // After a walk, the segment to its exit vertex crosses no light.
+    // The walk's last segment ends at its exit vertex inside the solid, so no light panel lies on it.
     Rgb crossed = exit ? BLACK : lights.crossed_from(ray, t_nearest, has_pdf, bsdf_pdf, &sampled_at, offset, !dropped);
     prep.add(sum, throughput * crossed, bounce >= 2);
     if (t_nearest == INF) {

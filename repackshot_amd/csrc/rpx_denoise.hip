@@ -275,7 +275,8 @@ __global__ void __launch_bounds__(256) k_dn_atrous(u32 W, u32 R, int step, const
         s2 += wgt * o.z;
       }
     }
-    // The centre tap always counts (weight 9/64), so `sw` is positive.
+    // `sw` cannot be zero: the pixel's own tap carries 9/64 of the kernel's weight whatever the
+    // guides say.
     float inv_w = 1.0f / sw;
     l = make_float4(s0 * inv_w, s1 * inv_w, s2 * inv_w, sv * inv_w * inv_w);
   }

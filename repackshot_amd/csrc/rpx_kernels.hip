@@ -1526,8 +1526,8 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
   bool caustics = false;
   if constexpr (EXT) caustics = S.caustics_on != 0;
   const bool dropped = caustics && (flags & FLAG_CAU_CASTERS) != 0;
-  // After a walk that scattered (integrator/mod.rs:1209-1242): the exit lobe's albedo at the
-  // surface it reached; an absorbed walk ends the path.
+  // A scattering walk that reached its solid's surface (integrator/mod.rs:1209-1242) leaves by
+  // the exit lobe, whose albedo the walk kernel carried here; an absorbed walk ends the path.
   bool exit = false;
   float exit_eta2 = 0.0f;
   if constexpr (EXT) {
@@ -2343,7 +2343,8 @@ __device__ void walk_step(const DevScene* __restrict__ scenes, const PathState& 
         }
       }
     } else {
-      // The boundary (or, past the last step, straight to it keeping all but the absorption).
+      // No scattering before the surface (or the step cap is reached: the path then goes to the
+      // surface with only the absorption applied).
       if (is_finite(t_hit)) {
         if (capped) {
           w = w * transmittance(M.sigma_a, t_hit);
@@ -2374,7 +2375,7 @@ __device__ void walk_step(const DevScene* __restrict__ scenes, const PathState& 
         if (leave) {
           end = WALK_EXIT;
         } else {
-          // Reflected back in (a step of the path, for the cap).
+          // Fresnel sent it inside again: this counts against the walk's step cap like a scattering event.
           path_events += 1;
           V3 dir = normalize(d + ng * (2.0f * cos));
           o = offset_origin(point, ng);

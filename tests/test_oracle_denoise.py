@@ -61,7 +61,7 @@ def test_denoised_example_has_the_pinned_hash(example, spp, preset, pin):
     st, out = orc.denoise(W, H, 0, H, spp, 0, H, planes, preset)
     assert st == 0
     assert orc.fnv1a(means(out, spp)) == pin
-    # Bands of any height give the same image.
+    # Splitting the rows into bands, tall or short, changes no pixel.
     halo = HALO if preset == orc.FULL else VIEWPORT_HALO
     parts = []
     for k in range(5):
@@ -124,7 +124,7 @@ def test_it_never_mixes_two_parts_and_leaves_the_background_alone():
     _, other = run(brighter, 8)
     right = s["partId"] == 2
     assert np.array_equal(other[right].view(np.uint32), out[right].view(np.uint32))
-    # Another material on the same part is another surface too.
+    # Two materials sharing one part id are still told apart.
     two = synthetic(40, 32, 8, 3.0, 3.0, 1.5)
     fg = two["partId"] != 0
     two["partId"][fg] = 1
@@ -147,7 +147,7 @@ def test_a_band_of_rows_is_exactly_those_rows_of_the_whole_image(preset, halo):
             st, rows = run(band(s, lo, hi), spp, preset, origin=lo, rows=hi - lo, y0=y0, y1=y1, height=200)
             assert st == 0
             assert np.array_equal(rows.view(np.uint32), whole[y0:y1].view(np.uint32)), (y0, y1)
-        # A band without its halo is refused, not computed wrong.
+        # Missing halo rows make the call fail instead of filtering with too little context.
         lo = 100 - halo + 1
         st, _ = run(band(s, lo, 200), spp, preset, origin=lo, rows=200 - lo, y0=100, y1=110, height=200)
         assert st == 2
