@@ -1,0 +1,49 @@
+"""The inverse-material fit end to end (the fit workload's scene, rpx_write_scene(4)): starting from
+all 40 material parameters and the floor's texels perturbed by up to +-30 %, Adam on the L2 image
+loss through the path-replay adjoint brings the base colours, the conductors' roughnesses, the
+texels and the loss closer to the truth within 80 steps (the full fit is a bench record)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+METALS = (0, 1, 2, 3)
+
+
+def test_a_short_fit_recovers_what_the_image_shows(gpu):
+    from repackshot_amd import fit as fitting
+
+    s = gpu.Scene(gpu.write_scene(4, 0))
+    origin, scale = (-4000.0, -4000.0), (1.0 / 8000.0, 1.0 / 8000.0)
+    yy, xx = np.mgrid[0:256, 0:256] / 255.0
+    tex_true = np.stack([0.5 + 0.3 * np.sin(6 * xx), 0.5 + 0.3 * np.cos(5 * yy), 0.4 + 0.2 * np.sin(4 * (xx + yy))],
+                        -1).astype(np.float32)
+    p_true = s.material_params().copy()
+    s.set_texture(8, tex_true, origin, scale)
+    s.reset_film()
+    s.render_samples(1_000_000, 64)
+    target = s.resolve().astype(np.float64)
+    rng = np.random.default_rng(11)
+    p0 = np.clip(p_true * rng.uniform(0.7, 1.3, size=p_true.shape).astype(np.float32), fitting.PARAM_LO,
+                 fitting.PARAM_HI)
+    p0[8] = p_true[8]
+    tex0 = np.clip(tex_true * rng.uniform(0.7, 1.3, size=tex_true.shape).astype(np.float32), 0.01, 0.99)
+    mask = fitting.identifiable(p_true, METALS)
+    base = np.zeros(mask.shape, bool)
+    base[:8, 1:4] = True
+
+    def err(p, sel):
+        return float(np.linalg.norm((p - p_true)[sel]) / np.linalg.norm(p_true[sel]))
+
+    print("fit: 80 steps at 8 spp", flush=True)
+    p, tex, losses = fitting.fit(s, target, p0, tex0, 8, origin, scale, steps=80, spp=8, frozen=(8,))
+    s.close()
+    metal_roughness = np.zeros(mask.shape, bool)
+    metal_roughness[list(METALS), 0] = True
+    assert err(p, base) < 0.75 * err(p0, base), (err(p0, base), err(p, base))
+    assert err(p, metal_roughness) < 0.75 * err(p0, metal_roughness), (err(p0, metal_roughness),
+                                                                        err(p, metal_roughness))
+    # The loss of an 8-spp image sits on its own noise (about 0.003 here); the fit removes most of the rest.
+    assert np.mean(losses[-10:]) < 0.9 * np.mean(losses[:3]), (losses[:3], losses[-10:])
+    assert np.all(p >= fitting.PARAM_LO) and np.all(p <= fitting.PARAM_HI)
+    assert np.linalg.norm(tex - tex_true) < np.linalg.norm(tex0 - tex_true)
