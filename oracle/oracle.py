@@ -532,6 +532,14 @@ class Bsdf:
                              _p(np.asarray(wi, np.float32)), 0.0, 0.0, 1, 1, _p(out))
         return float(out[0])
 
+    def param_derivs(self, wo, wi, all_lobes=True):
+        """d f(wo, wi) / d (roughness, base r, g, b, IOR) as (5, 3): the BSDF rebuilt with each
+        parameter as a dual number (the adjoint's per-vertex derivatives)."""
+        out = np.zeros(15, np.float32)
+        lib().orc_bsdf_probe(_p(self.mat), self.entering, _p(np.asarray(wo, np.float32)),
+                             _p(np.asarray(wi, np.float32)), 0.0, 0.0, 3, 1 if all_lobes else 0, _p(out))
+        return out.reshape(5, 3)
+
     def sample(self, wo, u0, u1):
         """(wi, weight, pdf) or None."""
         out = np.zeros(7, np.float32)

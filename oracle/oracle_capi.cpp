@@ -10,6 +10,7 @@
 #include <cstring>
 #include <thread>
 #include "rt_adaptive.hpp"
+#include "rt_bsdf_grad.hpp"
 #include "rt_denoise.hpp"
 #include "rt_passes.hpp"
 #include "rt_caustics.hpp"
@@ -229,7 +230,8 @@ void orc_energy_tables(uint16_t* albedo, uint16_t* schlick, uint16_t* clear_in, 
 // The BSDF of a material seen from one side, for its closed-form tests (bsdf.rs:384-992):
 // `mat` = kind, base r, g, b, metallic, roughness, ior, transmission, coat, coat roughness;
 // op 0 eval (out rgb; `all` 0 for the diffuse lobes only, 2 without the energy-compensation
-// lobes), 1 pdf (out[0]), 2 sample (out: wi xyz, weight rgb, pdf; returns 0 when it gives none).
+// lobes), 1 pdf (out[0]), 2 sample (out: wi xyz, weight rgb, pdf; returns 0 when it gives none),
+// 3 d eval / d (roughness, base r, g, b, ior) (out: 5 rgb triples, `all` as for op 0 but 0 or 1).
 int orc_bsdf_probe(const float* mat, int entering, const float* wo, const float* wi, float u0, float u1, int op,
                    int all, float* out) {
   Material m;
@@ -241,8 +243,18 @@ int orc_bsdf_probe(const float* mat, int entering, const float* wo, const float*
   m.transmission = mat[7];
   m.coat = mat[8];
   m.coat_roughness = mat[9];
-  Bsdf b = Bsdf::surface(m, entering != 0);
   Vec3 o(wo[0], wo[1], wo[2]);
+  if (op == 3) {
+    Rgb d[5];
+    bsdf_param_derivs_exact(m, entering != 0, o, Vec3(wi[0], wi[1], wi[2]), all ? Lobes::All : Lobes::Diffuse, d);
+    for (int k = 0; k < 5; ++k) {
+      out[3 * k] = d[k].r;
+      out[3 * k + 1] = d[k].g;
+      out[3 * k + 2] = d[k].b;
+    }
+    return 1;
+  }
+  Bsdf b = Bsdf::surface(m, entering != 0);
   if (op == 0) {
     if (all == 2) {
       // The single-scattering lobes alone: the energy-compensation lobes switched off.

@@ -1,6 +1,7 @@
 // ORACLE — test infrastructure only (see rt_core.hpp). Citations in rt_render.hpp.
 #include "rt_render.hpp"
 #include "rt_caustics.hpp"
+#include "rt_bsdf_grad.hpp"
 #include <algorithm>
 #include <cstring>
 
@@ -1263,26 +1264,6 @@ Rgb Texture::value(float x, float y) const {
 }
 
 namespace {
-// d f / d parameter at fixed directions by central differences of the BSDF rebuilt with each of
-// the five parameters moved by +-PARAM_STEP.
-void bsdf_param_derivs(const Material& m, bool entering, Vec3 wo, Vec3 wi, Lobes lobes, Rgb out[5]) {
-  for (int k = 0; k < 5; ++k) {
-    Rgb d(0.0f, 0.0f, 0.0f);
-    for (int side = 0; side < 2; ++side) {
-      float step = side == 0 ? PARAM_STEP : -PARAM_STEP;
-      Material p = m;
-      if (k == 0) p.roughness += step;
-      else if (k == 1) p.base_color.r += step;
-      else if (k == 2) p.base_color.g += step;
-      else if (k == 3) p.base_color.b += step;
-      else p.ior += step;
-      Rgb f = Bsdf::surface(p, entering).eval(wo, wi, lobes);
-      d = d + f * (side == 0 ? 1.0f : -1.0f);
-    }
-    out[k] = d * (0.5f / PARAM_STEP);
-  }
-}
-
 float dot3(Rgb a, Rgb b) { return a.r * b.r + a.g * b.g + a.b * b.b; }
 
 // Where a vertex's gradient goes: material `mat` (or none) and, for a textured one, its texels.
@@ -1513,7 +1494,7 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
           Rgb c = throughput * f * ls.radiance * (std::fabs(wi.z) * weight / ls.pdf);
           if (adjoint && has_mat && std::isfinite(c.r) && std::isfinite(c.g) && std::isfinite(c.b)) {
             Rgb d[5];
-            bsdf_param_derivs(shaded, s_outside, wo, wi, ls.glossy ? Lobes::All : Lobes::Diffuse, d);
+            bsdf_param_derivs_exact(shaded, s_outside, wo, wi, ls.glossy ? Lobes::All : Lobes::Diffuse, d);
             float pk = fmax_r(fmax_r(c.r, c.g), c.b);
             float scale = (bounce >= 1 && prep.clamp_indirect > 0.0f && pk > prep.clamp_indirect)
                               ? prep.clamp_indirect / pk : 1.0f;
@@ -1547,7 +1528,7 @@ void radiance(const Scene& scene, const Prepared& prep, const PixelSampler& samp
     if (adjoint && has_mat) {
       Rgb f_s = bsdf.eval(wo, bs.wi, Lobes::All);
       Rgb d[5];
-      bsdf_param_derivs(shaded, s_outside, wo, bs.wi, Lobes::All, d);
+      bsdf_param_derivs_exact(shaded, s_outside, wo, bs.wi, Lobes::All, d);
       for (int k = 0; k < 5; ++k)
         pending_dl[k] = Rgb(f_s.r > 0.0f ? d[k].r / f_s.r : 0.0f, f_s.g > 0.0f ? d[k].g / f_s.g : 0.0f,
                             f_s.b > 0.0f ? d[k].b / f_s.b : 0.0f);

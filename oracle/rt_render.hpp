@@ -327,7 +327,6 @@ struct AdjointPath {
   double* grad_mat;
   double* grad_tex;
 };
-constexpr float PARAM_STEP = 1.0e-3f;
 
 // One sample of one pixel: radiance and alpha. `stats` and `adjoint` may be null.
 // `photons`: the photon map of this sample index when the scene has caustics.

@@ -27,6 +27,7 @@
 // scene/mod.rs:294-357 (camera), image.rs:81-94 (film).
 #include <algorithm>
 #include "rpx_kernels.h"
+#include "rpx_bsdf_grad.h"
 
 namespace rpx {
 
@@ -1101,34 +1102,21 @@ __device__ __forceinline__ void add_to(C3& sum, C3 c, bool indirect, float clamp
   else sum = sum + c;
 }
 
-// d f / d parameter of the BSDF at (wo, wi_a) (diffuse-only flag `diffuse_a`) and at (wo, wi_b), by
-// central differences of the BSDF rebuilt with each parameter moved by +-PARAM_STEP (the
-// directions stay fixed: the adjoint differentiates the integrand in path space, its sampling
-// densities held constant).
-constexpr float PARAM_STEP = 1.0e-3f;
+// d f / d parameter of the BSDF at (wo, wi_a) (diffuse-only flag `diffuse_a`) and at (wo, wi_b):
+// forward-mode derivatives (rpx_bsdf_grad.h), one parameter at a time, of the BSDF built from the
+// material with that parameter as the dual variable (the directions stay fixed: the adjoint
+// differentiates the integrand in path space, its sampling densities held constant).
 __device__ void bsdf_param_derivs(const EnergyTables& T, const DevMaterial& M, C3 base, bool entering, V3 wo,
                                   bool want_a, V3 wi_a, bool diffuse_a, bool want_b, V3 wi_b, C3 da[N_MAT_PARAMS],
                                   C3 db[N_MAT_PARAMS]) {
 #pragma unroll 1
   for (int k = 0; k < N_MAT_PARAMS; ++k) {
-    C3 d_a = splat(0.0f), d_b = splat(0.0f);
-#pragma unroll 1
-    for (int side = 0; side < 2; ++side) {
-      float step = side == 0 ? PARAM_STEP : -PARAM_STEP;
-      float r = M.roughness, ior = M.ior;
-      C3 bc = base;
-      if (k == 0) r += step;
-      else if (k == 1) bc.r += step;
-      else if (k == 2) bc.g += step;
-      else if (k == 3) bc.b += step;
-      else ior += step;
-      Bsdf b = material_bsdf(T, M, bc, r, ior, entering);
-      float sgn = side == 0 ? 1.0f : -1.0f;
-      if (want_a) d_a = d_a + bsdf_eval(T, b, wo, wi_a, diffuse_a) * sgn;
-      if (want_b) d_b = d_b + bsdf_eval(T, b, wo, wi_b, false) * sgn;
-    }
-    da[k] = d_a * (0.5f / PARAM_STEP);
-    db[k] = d_b * (0.5f / PARAM_STEP);
+    Dual r{M.roughness, k == 0 ? 1.0f : 0.0f}, ior{M.ior, k == 4 ? 1.0f : 0.0f};
+    DC3 bc{Dual{base.r, k == 1 ? 1.0f : 0.0f}, Dual{base.g, k == 2 ? 1.0f : 0.0f}, Dual{base.b, k == 3 ? 1.0f : 0.0f}};
+    DBsdf b = dbsdf_surface(T, M.kind, bc, M.metallic, r, ior, M.transmission, M.coat, M.coat_roughness, M.flags,
+                            M.absorption_mm, entering);
+    da[k] = want_a ? dderiv(dbsdf_eval(T, b, wo, wi_a, diffuse_a)) : splat(0.0f);
+    db[k] = want_b ? dderiv(dbsdf_eval(T, b, wo, wi_b, false)) : splat(0.0f);
   }
 }
 

@@ -6,7 +6,9 @@ depend on the albedo, so with the same random numbers the render is a smooth (po
 function of the base colours and texels: the adjoint must then equal the finite difference of a
 linear loss up to f32 rounding. This pins the replay bookkeeping (remaining radiance, next-event
 terms, pending BSDF-sample terms), the per-vertex parameter derivatives and the texture's
-bilinear distribution, independently of any statistical argument.
+bilinear distribution, independently of any statistical argument. The per-vertex derivatives
+themselves (the BSDF rebuilt with one dual-number parameter) are pinned against extrapolated
+central differences of the BSDF value at fixed directions, material by material.
 """
 import numpy as np
 import pytest
@@ -181,3 +183,109 @@ def test_roughness_gradient_equals_finite_differences_inside_a_table_cell():
     assert abs(am - fm) <= 0.03 * abs(fm)
     am, sa, fm, sf = roughness_gradients(orc.OracleScene(conductor_scene(0.514)), 1.0e-2)
     assert fm - am > 2.0 * np.hypot(sa, sf), (am, sa, fm, sf)
+
+
+# Materials for the per-vertex derivative pin: roughness at the middle of an energy-table cell
+# (nodes k / 31) and glass IOR at the middle of a clear-table cell (x = (ior - 1) / (ior + 1) at
+# nodes 0.5 k / 16), so that the difference stencils below stay inside one cell.
+DERIV_MATERIALS = {
+    "plastic": dict(kind=0, base=(0.7, 0.4, 0.2), roughness=12.5 / 31, ior=1.45),
+    "metal": dict(kind=0, base=(0.9, 0.6, 0.3), metallic=1.0, roughness=9.5 / 31, ior=1.5),
+    "coated": dict(kind=0, base=(0.2, 0.5, 0.6), roughness=17.5 / 31, ior=1.6, coat=0.7, coat_roughness=0.1),
+    "mixed": dict(kind=0, base=(0.6, 0.6, 0.5), metallic=0.4, roughness=6.5 / 31, ior=1.5, transmission=0.3),
+    "glass_entering": dict(kind=1, base=(0.9, 0.95, 0.8), roughness=7.5 / 31, ior=1.415, transmission=1.0),
+    "glass_leaving": dict(kind=1, base=(0.9, 0.95, 0.8), roughness=7.5 / 31, ior=1.415, transmission=1.0,
+                          entering=False),
+}
+DERIV_STEPS = (4e-3, 4e-3, 4e-3, 4e-3, 1e-2)
+
+
+def moved(kw, k, delta):
+    kw = dict(kw)
+    if k == 0:
+        kw["roughness"] += delta
+    elif k == 4:
+        kw["ior"] += delta
+    else:
+        base = list(kw["base"])
+        base[k - 1] += delta
+        kw["base"] = tuple(base)
+    return kw
+
+
+def richardson(kw, k, wo, wi, all_lobes, h):
+    """(4 D(h / 2) - D(h)) / 3 of the central differences D of the oracle's BSDF value: error
+    O(h^4) where the value is smooth over [-h, h]."""
+    def ev(delta):
+        return orc.Bsdf(**moved(kw, k, delta)).eval(wo, wi, all_lobes).astype(np.float64)
+
+    def central(step):
+        return (ev(step) - ev(-step)) / (2.0 * step)
+    return (4.0 * central(h / 2) - central(h)) / 3.0
+
+
+def smooth_over(kw, k, wo, wi, all_lobes, h):
+    """Whether the BSDF takes one branch over the whole stencil: the value stays non-zero (the
+    refraction half-vector stays valid) and, for a clear material's reflection, the Fresnel term
+    stays on one side of total internal reflection."""
+    vals = [orc.Bsdf(**moved(kw, k, d)).eval(wo, wi, all_lobes) for d in (-h, -h / 2, 0.0, h / 2, h)]
+    if any(not np.any(v != 0) for v in vals) and any(np.any(v != 0) for v in vals):
+        return False
+    if kw.get("transmission", 0.0) > 0.0 and wi[2] > 0.0:
+        hv = (wo + wi) / np.linalg.norm(wo + wi)
+        c = float(np.dot(wo, hv))
+        sides = set()
+        for d in (-h, h):
+            ior = moved(kw, k, d)["ior"]
+            eta = ior if kw.get("entering", True) else 1.0 / ior
+            sides.add((1.0 - c * c) / (eta * eta) >= 1.0)
+        if len(sides) > 1:
+            return False
+    return True
+
+
+def unit_dirs(rng, n, up):
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    v[:, 2] = np.abs(v[:, 2]) * (1.0 if up else -1.0)
+    return v.astype(np.float32)
+
+
+@pytest.mark.parametrize("name", sorted(DERIV_MATERIALS))
+def test_per_vertex_derivatives_equal_richardson_differences(name):
+    # The adjoint's d f / d (roughness, base r, g, b, IOR) at fixed directions, from the BSDF built
+    # with one dual-number parameter, against extrapolated central differences of the BSDF value:
+    # a dropped or mis-signed term in any lobe, layer, Fresnel or energy-table slope shows here.
+    kw = DERIV_MATERIALS[name]
+    b = orc.Bsdf(**kw)
+    rng = np.random.default_rng(3)
+    checked = 0
+    for trial in range(40):
+        wo = unit_dirs(rng, 1, True)[0]
+        wi = unit_dirs(rng, 1, not (kw["kind"] == 1 and trial % 2 == 1))[0]
+        for all_lobes in (True, False):
+            f = b.eval(wo, wi, all_lobes)
+            exact = b.param_derivs(wo, wi, all_lobes)
+            for k in range(5):
+                if not smooth_over(kw, k, wo, wi, all_lobes, DERIV_STEPS[k]):
+                    continue
+                fd = richardson(kw, k, wo, wi, all_lobes, DERIV_STEPS[k])
+                scale = max(np.abs(exact[k]).max(), 0.05 * np.abs(f).max(), 1e-7)
+                assert np.abs(exact[k] - fd).max() <= 5e-3 * scale, (trial, all_lobes, k, exact[k], fd, f)
+                checked += 1
+    assert checked >= 300
+
+
+def test_per_vertex_derivatives_vanish_where_the_value_cannot_move():
+    # A surface of IOR 1 without metal has no specular lobe: f0 = ((ior - 1) / (ior + 1))^2 has zero
+    # slope there, and a pure conductor's value does not depend on its IOR; a base colour moves only
+    # its own channel.
+    wo, wi = np.array([0.3, 0.1, 0.95], np.float32), np.array([-0.4, 0.2, 0.89], np.float32)
+    wo /= np.linalg.norm(wo)
+    wi /= np.linalg.norm(wi)
+    d = orc.Bsdf(base=(0.5, 0.4, 0.3), roughness=0.6, ior=1.0).param_derivs(wo, wi)
+    assert np.all(d[4] == 0.0) and np.all(d[0] == 0.0)
+    d = orc.Bsdf(base=(0.9, 0.6, 0.3), metallic=1.0, roughness=0.3, ior=1.7).param_derivs(wo, wi)
+    assert np.all(d[4] == 0.0)
+    for k in (1, 2, 3):
+        assert np.all(np.delete(d[k], k - 1) == 0.0) and d[k][k - 1] != 0.0
