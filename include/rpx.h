@@ -123,6 +123,11 @@ uint32_t rpx_film_device(uint32_t handle, void** ptrs4);
  * memory: instance, mesh triangle (0xffffffff on a miss), t (inf on a miss), any (0/1). */
 uint32_t rpx_trace_rays(uint32_t handle, const float* rays, uint32_t n, uint32_t* inst, uint32_t* tri, float* t,
                         uint8_t* any);
+/* The same queries traced one wavefront per ray (the traversal the drain gives the last paths of a
+ * scene with media; no counterpart in the reference): 16 BVH entries per round over a shared stack
+ * of stack_entries (1..1024) before a ray falls back to one lane. Results equal rpx_trace_rays'. */
+uint32_t rpx_trace_rays_wave(uint32_t handle, const float* rays, uint32_t n, uint32_t stack_entries, uint32_t* inst,
+                             uint32_t* tri, float* t, uint8_t* any);
 /* width, height, max_samples, seed, max_depth, triangles, BVH nodes, instances. */
 uint32_t rpx_scene_info(uint32_t handle, uint64_t* out8);
 /* Since the handle was loaded: rays cast (camera + extension + shadow, as the reference counts

@@ -814,8 +814,8 @@ uint32_t rpx_film_device(uint32_t handle, void** ptrs4) {
   return RPX_OK;
 }
 
-uint32_t rpx_trace_rays(uint32_t handle, const float* rays, uint32_t n, uint32_t* inst, uint32_t* tri, float* t,
-                        uint8_t* any) {
+static uint32_t trace_rays_impl(uint32_t handle, const float* rays, uint32_t n, uint32_t* inst, uint32_t* tri, float* t,
+                                uint8_t* any, int wave_cap) {
   if (!rays || !inst || !tri || !t || !any) return RPX_NULL_POINTER;
   auto h = find(handle);
   if (!h) return RPX_BAD_HANDLE;
@@ -832,7 +832,7 @@ uint32_t rpx_trace_rays(uint32_t handle, const float* rays, uint32_t n, uint32_t
   hipError_t e = hipMemcpyAsync(d_rays, rays, (size_t)n * 32, hipMemcpyHostToDevice, h->stream);
   if (e == hipSuccess)
     e = launch_trace_rays(h->d_scene, (const float*)d_rays, n, (u32*)d_inst, (u32*)d_tri, (float*)d_t, (uint8_t*)d_any,
-                          h->stream);
+                          h->stream, wave_cap);
   if (e == hipSuccess) e = hipMemcpyAsync(inst, d_inst, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(tri, d_tri, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(t, d_t, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream);
@@ -840,6 +840,20 @@ uint32_t rpx_trace_rays(uint32_t handle, const float* rays, uint32_t n, uint32_t
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   dev_free_all(tmp);
   return status_of(e);
+}
+
+uint32_t rpx_trace_rays(uint32_t handle, const float* rays, uint32_t n, uint32_t* inst, uint32_t* tri, float* t,
+                        uint8_t* any) {
+  return trace_rays_impl(handle, rays, n, inst, tri, t, any, 0);
+}
+
+uint32_t rpx_trace_rays_wave(uint32_t handle, const float* rays, uint32_t n, uint32_t stack_entries, uint32_t* inst,
+                             uint32_t* tri, float* t, uint8_t* any) {
+  if (stack_entries == 0 || stack_entries > 1024) {
+    set_error("rpx_trace_rays_wave: stack_entries must be 1 .. 1024");
+    return RPX_BAD_TILE;
+  }
+  return trace_rays_impl(handle, rays, n, inst, tri, t, any, (int)stack_entries);
 }
 
 uint32_t rpx_scene_info(uint32_t handle, uint64_t* out8) {

@@ -45,6 +45,12 @@ struct EngineOptions {
   // Once no new path is left, at most this many live paths finish in one drain launch (0: the
   // iteration loop runs to the end).
   u32 tail = 1u << 16;
+  // Scenes with walks: lanes per wave the drain gives paths to, and how many paths may be left
+  // when the drain hands them on to one wave each (at most PARK_MAX). On the bottle workload
+  // (1920 x 1080 x 64 paths, tail 2^16) the drain took 72 ms with no hand-on, 50 ms from 1,024
+  // paths left, 43 ms from 16,384 and 59 ms from all of them; fewer lanes per wave were slower.
+  u32 drain_lanes = 64;
+  u32 park_at = 16384;
 };
 
 // GPU time of the last run, from events on the engine's stream (profiling on).

@@ -142,7 +142,8 @@ bool Engine::ensure_pool(u32 n, std::string& err) {
     dev_free_all(walk_bufs_);
     walk_cap_ = 0;
     size_t f4 = (size_t)n * sizeof(float4);
-    if (!dev_alloc(walk_bufs_, f4, (void**)&P.walk_w) || !dev_alloc(walk_bufs_, f4, (void**)&P.walk_r)) {
+    if (!dev_alloc(walk_bufs_, f4, (void**)&P.walk_w) || !dev_alloc(walk_bufs_, f4, (void**)&P.walk_r) ||
+        !dev_alloc(walk_bufs_, (size_t)PARK_MAX * 4, (void**)&P.parked)) {
       dev_free_all(walk_bufs_);
       err = "device memory exhausted for the media walks' state";
       return false;
@@ -234,6 +235,8 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const DevSegment* chunk, 
   // The drain runs the forward vertices only; the replay keeps the loop.
   L.ext = ext ? 1u : 0u;
   L.tail = adj ? 0u : opt.tail;
+  L.drain_lanes = std::min(std::max(opt.drain_lanes, 1u), 64u);
+  L.park_at = std::min(opt.park_at, PARK_MAX);
   L.stream = stream_;
   L.side = side_;
   e = launch_pool_init(P, L, total);
@@ -381,6 +384,7 @@ void carve_pool(Carver& c, PathState& Q, u32 n, bool textured, bool ext, u32 cha
   Q.bsdf_tmp = textured ? c.take<Bsdf>(n) : nullptr;
   Q.walk_w = ext ? c.take<float4>(n) : nullptr;
   Q.walk_r = ext ? c.take<float4>(n) : nullptr;
+  Q.parked = ext ? c.take<u32>(PARK_MAX) : nullptr;
   table = c.take<DevSegment>(1);
   staging = c.take<float>((size_t)n * channels);
 }

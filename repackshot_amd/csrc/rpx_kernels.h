@@ -63,8 +63,11 @@ struct DevSegment {
 // The shadow and finished queues are kept per iteration parity: the shadow rays of one iteration
 // are traced beside the next iteration's extension rays, and its finished slots are taken by the
 // regen of the iteration after that, once their shadow rays have added their light.
+// C_DRAIN_LEFT: paths of the drain not finished yet; C_PARKED: paths the drain handed on to the
+// one-wave-per-path stage (when no more than Launch::park_at of them were left).
 constexpr u32 C_LIVE0 = 0, C_LIVE1 = 1, C_SHADOW0 = 2, C_DONE0 = 3, C_CURSOR = 4, C_END = 5, C_IDLE = 6, C_TAIL = 7,
-              C_SHADOW1 = 8, C_DONE1 = 9;
+              C_SHADOW1 = 8, C_DONE1 = 9, C_DRAIN_LEFT = 10, C_PARKED = 11;
+constexpr u32 PARK_MAX = 1u << 16;
 constexpr u32 N_COUNTS = 16;
 __host__ __device__ inline u32 c_shadow(u32 parity) { return parity ? C_SHADOW1 : C_SHADOW0; }
 __host__ __device__ inline u32 c_done(u32 parity) { return parity ? C_DONE1 : C_DONE0; }
@@ -88,6 +91,7 @@ struct PathState {
   u32* live[2];    // live queues (ping-pong by iteration)
   u32* shadow_queue[2];  // shadow rays made by the shade of an iteration of that parity
   u32* done_queue[2];    // slots finished by the shade of an iteration of that parity
+  u32* parked;           // PARK_MAX slots the drain hands to one wave each (scenes with media)
   u32* counts;     // see C_*
   u64* rays;       // [0] closest-hit rays traced, [1] shadow rays counted without a trace, [2] shadow rays traced,
                    // [3] thin-wall probes after a medium's exit (counted, never needed: no such walls)
@@ -132,6 +136,8 @@ struct Launch {
   int shadow_shape;  // any-hit launch shape
   u32 tail;          // once no new path is left, at most this many live paths are drained in one launch (0: never)
   u32 ext;           // a scene has sphere lights or scattering media: the extended shade and the walk kernel
+  u32 drain_lanes;   // with ext: lanes per wave of the drain (1..64)
+  u32 park_at;       // with ext: paths left when the drain hands them to one wave each (0..PARK_MAX)
   hipStream_t stream;
   hipStream_t side;  // the shadow rays' stream (default priority), forked from and joined to `stream`
 };
@@ -156,8 +162,10 @@ hipError_t launch_splat_segments(const PathState& P, const DevSegment* segs, u32
 // Copies the one-sample results of a w x h tile (work ids 0 .., pixels in the regen's tiled order)
 // out as row-major rgb or rgba floats.
 hipError_t launch_tile_out(const PathState& P, float* out, u32 w, u32 h, u32 channels, hipStream_t st);
+// Nearest and any hit of n rays; with wave_cap > 0, one wave per ray (the drain's traversal) with a
+// shared stack of wave_cap entries (at most 1024) before it falls back to one lane.
 hipError_t launch_trace_rays(const DevScene* scene, const float* rays, u32 n, u32* inst, u32* tri, float* t,
-                             uint8_t* any, hipStream_t st);
+                             uint8_t* any, hipStream_t st, int wave_cap = 0);
 // Planes of a pass film in device memory (rt-core/src/passes.rs): sums of the sampled passes (albedo
 // rgb, coverage) with their sample count, and the exact passes of the pixels' centre rays. Planes
 // a launch does not write may be null.

@@ -370,6 +370,171 @@ __device__ bool trace_any(const Geo& g, u32* lds_col, V3 o, V3 d, float t_min, f
   return false;
 }
 
+// ------------------------------------------------------------------ one ray per wave
+// For a handful of rays left alone on the GPU (the last steps of the longest walks), one lane per
+// ray leaves the traversal a chain of dependent fetches: one node, then one triangle at a time.
+// Here the whole wave traverses one ray: each round takes up to 16 entries off a shared LDS stack,
+// four lanes per entry (a child slot of a node, or two triangles of a leaf), so a round costs one
+// fetch latency for up to 16 nodes. The closest hit is the lexicographic minimum of (t, instance,
+// triangle) over the triangles hit within the best distance so far, and the slab test is
+// conservative, so the order of the visits does not change it: the result is that of trace_nearest,
+// bit for bit. Every lane of the wave must call these with the same ray.
+constexpr u32 WAVE_WORLD = 0xffffffffu;  // an entry of the instance tree
+constexpr int WAVE_STACK = 1024;         // entries of the shared stack
+constexpr u32 WAVE_ENTRIES = 16;         // entries per round (four lanes each)
+struct WaveStack {
+  u32 ref[WAVE_STACK];
+  u32 space[WAVE_STACK];  // instance whose object space the entry is in, or WAVE_WORLD
+};
+enum class WaveEnd { done, hit, overflow };
+
+template <bool ANY>
+__device__ WaveEnd wave_traverse(const Geo& g, WaveStack& S, int cap, V3 o, V3 d, float t_min, float t_max,
+                                 bool camera, float& best_t, u32& best_inst, u32& best_tri) {
+  const u32 lane = __lane_id(), e = lane >> 2, c = lane & 3u;
+  const PRay world = prepare(o, d, t_min, t_max);
+  best_t = world.t_max;
+  best_inst = best_tri = 0xffffffffu;
+  if (!usable_ray(o, d) || !(world.t_min <= world.t_max)) return WaveEnd::done;
+  if (lane == 0) {
+    S.ref[0] = g.root;
+    S.space[0] = WAVE_WORLD;
+  }
+  int sp = 1;
+  __syncthreads();
+  while (sp > 0) {
+    const int n = sp < (int)WAVE_ENTRIES ? sp : (int)WAVE_ENTRIES;
+    const int base = sp - n;
+    const bool mine = (int)e < n;
+    u32 ref = REF_EMPTY, space = WAVE_WORLD;
+    if (mine) {
+      ref = S.ref[base + e];
+      space = S.space[base + e];
+    }
+    __syncthreads();
+    sp = base;
+    PRay ray = world;
+    bool cull = false;
+    if (mine && space != WAVE_WORLD) {
+      const DevInstance& in = g.instances[space];
+      cull = (in.flags & INST_ONE_SIDED) != 0;
+      if (!(in.flags & INST_IDENTITY)) ray = prepare(xf_point(in.to_local, o), xf_vector(in.to_local, d), t_min, t_max);
+    }
+    bool push = false, hit = false;
+    u32 push_ref = REF_EMPTY, push_space = WAVE_WORLD, h_tri = 0xffffffffu;
+    float h_t = RPX_INF;
+    if (mine && ref != REF_EMPTY) {
+      if (!(ref & REF_LEAF)) {
+        const Node4* nd = g.nodes + ref;
+        float near;
+        u32 ch = nd->child[c];
+        if (ch != REF_EMPTY && slab(ray, nd->lo_x[c], nd->hi_x[c], nd->lo_y[c], nd->hi_y[c], nd->lo_z[c], nd->hi_z[c],
+                                    ANY ? ray.t_max : best_t, near)) {
+          push = true;
+          push_ref = ch;
+          push_space = space;
+        }
+      } else if (space == WAVE_WORLD) {
+        // An instance leaf: its mesh's tree, in the instance's object space.
+        if (c == 0) {
+          u32 i = (ref & ~REF_LEAF) >> LEAF_COUNT_BITS;
+          const DevInstance& in = g.instances[i];
+          u32 fl = in.flags;
+          if ((fl & INST_ACTIVE) && (ANY || !(camera && (fl & INST_CAMERA_HIDDEN))) && in.blas_root != REF_EMPTY) {
+            push = true;
+            push_ref = in.blas_root;
+            push_space = i;
+          }
+        }
+      } else {
+        u32 first = (ref & ~REF_LEAF) >> LEAF_COUNT_BITS, count = (ref & 7u) + 1;
+        for (u32 k = c; k < count; k += 4) {
+          const float4* tp = g.tris + 3 * (size_t)(first + k);
+          float4 va = tp[0], vb = tp[1], vc = tp[2];
+          float t;
+          if (watertight(ray, va, vb, vc, ANY ? ray.t_max : best_t, cull, t)) {
+            u32 tri = f2u(va.w);
+            if (!hit || t < h_t || (t == h_t && tri < h_tri)) {
+              h_t = t;
+              h_tri = tri;
+            }
+            hit = true;
+          }
+        }
+      }
+    }
+    if (ANY) {
+      if (__ballot(hit)) return WaveEnd::hit;
+    } else if (__ballot(hit)) {
+      // The round's lexicographic minimum over the wave, then against the best so far.
+      float t = hit ? h_t : RPX_INF;
+      u32 a = hit ? space : 0xffffffffu, b = hit ? h_tri : 0xffffffffu;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        float t2 = __shfl_xor(t, off);
+        u32 a2 = (u32)__shfl_xor((int)a, off), b2 = (u32)__shfl_xor((int)b, off);
+        if (t2 < t || (t2 == t && (a2 < a || (a2 == a && b2 < b)))) {
+          t = t2;
+          a = a2;
+          b = b2;
+        }
+      }
+      if (t < best_t || (t == best_t && (a < best_inst || (a == best_inst && b < best_tri)))) {
+        best_t = t;
+        best_inst = a;
+        best_tri = b;
+      }
+    }
+    const u64 m = __ballot(push);
+    const int np = (int)__popcll(m);
+    if (sp + np > cap) return WaveEnd::overflow;
+    if (push) {
+      int pos = sp + (int)__builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
+      S.ref[pos] = push_ref;
+      S.space[pos] = push_space;
+    }
+    sp += np;
+    __syncthreads();
+  }
+  return WaveEnd::done;
+}
+
+// Nearest hit of one ray by the whole wave: trace_nearest's result. A stack deeper than `cap`
+// entries hands the ray to trace_nearest in lane 0 (its stack's first 16 entries in the shared
+// stack's memory, one every 64 words, as in a 64-lane block's LDS columns).
+__device__ bool trace_nearest_wave(const Geo& g, WaveStack& S, int cap, V3 o, V3 d, float t_min, float t_max,
+                                   bool camera, u32& hit_inst, u32& hit_tri, float& hit_t) {
+  float bt;
+  u32 bi, bn;
+  WaveEnd end = wave_traverse<false>(g, S, cap, o, d, t_min, t_max, camera, bt, bi, bn);
+  if (end == WaveEnd::overflow) {
+    bool found = false;
+    if (__lane_id() == 0) found = trace_nearest<64, 16>(g, S.ref, o, d, t_min, t_max, camera, bi, bn, bt);
+    found = __shfl((int)found, 0) != 0;
+    bi = (u32)__shfl((int)bi, 0);
+    bn = (u32)__shfl((int)bn, 0);
+    bt = __shfl(bt, 0);
+    if (!found) return false;
+  } else if (bi == 0xffffffffu) {
+    return false;
+  }
+  hit_inst = bi;
+  hit_tri = bn;
+  hit_t = bt;
+  return true;
+}
+
+// Any hit of one ray by the whole wave: trace_any's answer.
+__device__ bool trace_any_wave(const Geo& g, WaveStack& S, int cap, V3 o, V3 d, float t_min, float t_max) {
+  float bt;
+  u32 bi, bn;
+  WaveEnd end = wave_traverse<true>(g, S, cap, o, d, t_min, t_max, false, bt, bi, bn);
+  if (end != WaveEnd::overflow) return end == WaveEnd::hit;
+  bool blocked = false;
+  if (__lane_id() == 0) blocked = trace_any<64, 16>(g, S.ref, o, d, t_min, t_max);
+  return __shfl((int)blocked, 0) != 0;
+}
+
 // ================================================================== queue compaction
 
 // Appends for a whole block to NC queues at once: per-wave ballots, a prefix over the block's waves
@@ -635,12 +800,14 @@ __global__ void k_bookkeep(PathState P, u32 parity, u32 tail) {
   P.counts[C_LIVE0 + (parity ^ 1)] = 0;
   P.counts[c_shadow(parity)] = 0;
   P.counts[C_TAIL] = 0;
+  P.counts[C_PARKED] = 0;
   P.rays[0] += live;
   u32 pending = P.counts[c_done(parity ^ 1)] + P.counts[c_shadow(parity ^ 1)];
   if (live == 0) {
     if (pending == 0) P.counts[C_IDLE] = 1;
   } else if (live <= tail && cursor + fresh >= end) {
     P.counts[C_TAIL] = live;
+    P.counts[C_DRAIN_LEFT] = live;
     P.counts[C_LIVE0 + parity] = 0;
   }
 }
@@ -2412,20 +2579,26 @@ __global__ void __launch_bounds__(256) k_walk(const DevScene* __restrict__ scene
 // bit; the radiance goes to the results as the regen would put it there. EXT: the extended shade,
 // and each closest hit inside a scattering medium is a walk step (a walking path traces again).
 template <int BLK, int LDSN, bool EXT>
-__global__ void __launch_bounds__(BLK) k_finish(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
+__global__ void __launch_bounds__(BLK) k_finish(const DevScene* __restrict__ scenes, PathState P, u32 parity,
+                                                u32 lanes, u32 park_at) {
   const u32 n = P.counts[C_TAIL];
   if (n == 0) return;
   __shared__ u32 lds[LDSN * BLK];
   u32* col = lds + threadIdx.x;
   const u32* q = P.live[parity];
-  u32 stride = gridDim.x * blockDim.x;
+  // The first `lanes` lanes of each wave take paths (fewer lanes: a wave's step waits for fewer
+  // traversals).
+  const u32 wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u, lane_id = __lane_id();
+  const u32 stride = (gridDim.x * blockDim.x / 64u) * lanes;
   // Whole waves iterate together (the shade's scene passes are wave operations).
   u32 n_iter = (n + stride - 1) / stride;
-  u32 gid = blockIdx.x * blockDim.x + threadIdx.x;
   u32 nearest = 0, shadow = 0;
+  // With walks, once no more than park_at paths of the drain are left anywhere, the waves hand
+  // theirs on to k_finish_wave (one wave per path) at the end of a vertex.
+  const bool can_park = EXT && n_iter == 1 && P.parked != nullptr && park_at > 0;
   for (u32 it = 0; it < n_iter; ++it) {
-    u32 j = it * stride + gid;
-    bool active = j < n;
+    u32 j = it * stride + wave * lanes + lane_id;
+    bool active = lane_id < lanes && j < n;
     u32 i = 0, sid = 0;
     if (active) {
       i = q[j];
@@ -2471,10 +2644,35 @@ __global__ void __launch_bounds__(BLK) k_finish(const DevScene* __restrict__ sce
           P.sum[i] = make_float4(s.x + c.x, s.y + c.y, s.z + c.z, s.w);
         }
       }
-      if (active && !cont) {
+      bool finished = active && !cont;
+      if (finished) {
         P.results[P.ident[i].x] = P.sum[i];
         P.ident[i] = make_uint4(NO_WORK, 0u, 0u, 0u);
         active = false;
+      }
+      if (can_park) {
+        const u64 fin = __ballot(finished);
+        u32 left = 0;
+        if (__lane_id() == 0) {
+          u32 k = (u32)__popcll(fin);
+          left = k ? atomicSub(&P.counts[C_DRAIN_LEFT], k) - k : atomicAdd(&P.counts[C_DRAIN_LEFT], 0u);
+        }
+        left = (u32)__shfl((int)left, 0);
+        const u64 m = __ballot(active);
+        if (m != 0 && left <= park_at) {
+          u32 base = 0;
+          if (__lane_id() == 0) base = atomicAdd(&P.counts[C_PARKED], (u32)__popcll(m));
+          base = (u32)__shfl((int)base, 0);
+          if (active) {
+            u32 pos = base + __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
+            // At most park_at paths were left when any wave parked, so pos < park_at; a path that
+            // found no room would stay here.
+            if (pos < park_at) {
+              P.parked[pos] = i;
+              active = false;
+            }
+          }
+        }
       }
     }
   }
@@ -2488,6 +2686,73 @@ __global__ void __launch_bounds__(BLK) k_finish(const DevScene* __restrict__ sce
   if (__lane_id() == 0) {
     if (wn) atomicAdd((unsigned long long*)&P.rays[0], (unsigned long long)wn);
     if (ws) atomicAdd((unsigned long long*)&P.rays[2], (unsigned long long)ws);
+  }
+}
+
+// The paths the drain parked, one wave each (block b takes parked path b): the loop of k_finish for
+// one path, its closest-hit and shadow rays traced by the whole wave, everything else in lane 0.
+// Lane 0 alone reads and writes the path's state; the other lanes get the rays from its registers.
+__global__ void __launch_bounds__(64) k_finish_wave(const DevScene* __restrict__ scenes, PathState P, u32 park_at) {
+  const u32 n = P.counts[C_PARKED];
+  if (blockIdx.x >= (n < park_at ? n : park_at)) return;
+  __shared__ WaveStack stack;
+  const u32 lane = __lane_id();
+  const u32 i = P.parked[blockIdx.x];
+  const u32 sid = P.ident[i].y;
+  const Geo g = geo_of(scenes, sid);
+  u32 nearest = 0, shadow = 0;
+  for (;;) {
+    float4 o = make_float4(0, 0, 0, 0), d = make_float4(0, 0, 0, 0);
+    u32 flags = 0;
+    if (lane == 0) {
+      o = P.ray_o[i];
+      d = P.ray_d[i];
+      flags = f2u(P.med[i].w);
+    }
+    V3 wo = v3(__shfl(o.x, 0), __shfl(o.y, 0), __shfl(o.z, 0));
+    V3 wd = v3(__shfl(d.x, 0), __shfl(d.y, 0), __shfl(d.z, 0));
+    bool primary = ((u32)__shfl((int)flags, 0) & FLAG_PRIMARY) != 0;
+    u32 inst = 0xffffffffu, tri = 0xffffffffu;
+    float t = RPX_INF;
+    trace_nearest_wave(g, stack, WAVE_STACK, wo, wd, RAY_EPSILON, RPX_INF, primary, inst, tri, t);
+    bool cont = false, want_shadow = false;
+    float4 so = make_float4(0, 0, 0, 0), sd = make_float4(0, 0, 0, 0);
+    if (lane == 0) {
+      P.hit[i] = make_uint4(inst, tri, f2u(t), 0u);
+      ++nearest;
+      walk_step(scenes, P, i);
+      VertexGrad g_prev, g_now;
+      no_grad(g_prev);
+      no_grad(g_now);
+      shade_path<false, true>(scenes[sid], P, i, cont, want_shadow, g_prev, g_now);
+      if (want_shadow) {
+        so = P.sh_o[i];
+        sd = P.sh_d[i];
+      }
+    }
+    cont = __shfl((int)cont, 0) != 0;
+    if (__shfl((int)want_shadow, 0)) {
+      ++shadow;
+      V3 po = v3(__shfl(so.x, 0), __shfl(so.y, 0), __shfl(so.z, 0));
+      V3 pd = v3(__shfl(sd.x, 0), __shfl(sd.y, 0), __shfl(sd.z, 0));
+      bool blocked = trace_any_wave(g, stack, WAVE_STACK, po, pd, RAY_EPSILON, __shfl(so.w, 0));
+      if (!blocked && lane == 0) {
+        float4 c = P.sh_c[i];
+        float4 s = P.sum[i];
+        P.sum[i] = make_float4(s.x + c.x, s.y + c.y, s.z + c.z, s.w);
+      }
+    }
+    if (!cont) {
+      if (lane == 0) {
+        P.results[P.ident[i].x] = P.sum[i];
+        P.ident[i] = make_uint4(NO_WORK, 0u, 0u, 0u);
+      }
+      break;
+    }
+  }
+  if (lane == 0) {
+    atomicAdd((unsigned long long*)&P.rays[0], (unsigned long long)nearest);
+    if (shadow) atomicAdd((unsigned long long*)&P.rays[2], (unsigned long long)shadow);
   }
 }
 
@@ -2604,6 +2869,28 @@ __global__ void __launch_bounds__(128) k_trace_rays(const DevScene* __restrict__
     tri[i] = ht;
     t[i] = tt;
     any[i] = trace_any<128, 32>(g, col, o, d, r[6], r[7]) ? 1 : 0;
+  }
+}
+
+// The same queries, one wave per ray (the drain's traversal of its last paths); `cap` entries of
+// the wave's stack (at most WAVE_STACK) before it falls back to one lane.
+__global__ void __launch_bounds__(64) k_trace_rays_wave(const DevScene* __restrict__ scene, const float* rays, u32 n,
+                                                        int cap, u32* inst, u32* tri, float* t, uint8_t* any) {
+  __shared__ WaveStack stack;
+  Geo g = geo_of(scene, 0);
+  for (u32 i = blockIdx.x; i < n; i += gridDim.x) {
+    const float* r = rays + 8 * (size_t)i;
+    V3 o = v3(r[0], r[1], r[2]), d = v3(r[3], r[4], r[5]);
+    u32 hi = 0xffffffffu, ht = 0xffffffffu;
+    float tt = RPX_INF;
+    trace_nearest_wave(g, stack, cap, o, d, r[6], r[7], false, hi, ht, tt);
+    bool a = trace_any_wave(g, stack, cap, o, d, r[6], r[7]);
+    if (__lane_id() == 0) {
+      inst[i] = hi;
+      tri[i] = ht;
+      t[i] = tt;
+      any[i] = a ? 1 : 0;
+    }
   }
 }
 
@@ -3240,12 +3527,16 @@ hipError_t launch_regen(const PathState& P, const Launch& L, u32 parity, bool ad
 }
 hipError_t launch_finish(const PathState& P, const Launch& L, u32 parity) {
   u32 lanes = L.pool < L.tail ? L.pool : L.tail;
-  if (L.ext)
-    hipLaunchKernelGGL((k_finish<64, 32, true>), dim3(grid_for(lanes, 64, 1u << 16)), dim3(64), 0, L.stream, L.scenes,
-                       P, parity);
-  else
+  if (L.ext) {
+    u32 waves = lanes ? (lanes + L.drain_lanes - 1) / L.drain_lanes : 1u;
+    hipLaunchKernelGGL((k_finish<64, 32, true>), dim3(waves < (1u << 16) ? waves : (1u << 16)), dim3(64), 0, L.stream,
+                       L.scenes, P, parity, L.drain_lanes, L.park_at);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || L.park_at == 0) return e;
+    hipLaunchKernelGGL(k_finish_wave, dim3(L.park_at), dim3(64), 0, L.stream, L.scenes, P, L.park_at);
+  } else
     hipLaunchKernelGGL((k_finish<64, 32, false>), dim3(grid_for(lanes, 64, 1u << 16)), dim3(64), 0, L.stream, L.scenes,
-                       P, parity);
+                       P, parity, 64u, 0u);
   return hipGetLastError();
 }
 
@@ -3344,8 +3635,14 @@ hipError_t launch_tile_out(const PathState& P, float* out, u32 w, u32 h, u32 cha
   return hipGetLastError();
 }
 hipError_t launch_trace_rays(const DevScene* scene, const float* rays, u32 n, u32* inst, u32* tri, float* t,
-                             uint8_t* any, hipStream_t st) {
-  hipLaunchKernelGGL(k_trace_rays, dim3(grid_for(n, 128, 8192)), dim3(128), 0, st, scene, rays, n, inst, tri, t, any);
+                             uint8_t* any, hipStream_t st, int wave_cap) {
+  if (wave_cap > 0) {
+    int cap = wave_cap < WAVE_STACK ? wave_cap : WAVE_STACK;
+    hipLaunchKernelGGL(k_trace_rays_wave, dim3(n < 65536u ? n : 65536u), dim3(64), 0, st, scene, rays, n, cap, inst, tri,
+                       t, any);
+  } else {
+    hipLaunchKernelGGL(k_trace_rays, dim3(grid_for(n, 128, 8192)), dim3(128), 0, st, scene, rays, n, inst, tri, t, any);
+  }
   return hipGetLastError();
 }
 hipError_t launch_photons(const DevScene* scene, u32 photons, u32 max_steps, const PhotonBuffers& B,

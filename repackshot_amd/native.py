@@ -51,6 +51,8 @@ _SIGS = {
     "rpx_film_read": (ctypes.c_uint32, [ctypes.c_uint32] + [ctypes.c_void_p] * 4),
     "rpx_film_device": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_trace_rays": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 4),
+    "rpx_trace_rays_wave": (ctypes.c_uint32,
+                            [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32] + [ctypes.c_void_p] * 4),
     "rpx_scene_info": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_scene_stats": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_set_options": (ctypes.c_uint32, [ctypes.c_uint32] * 8),
@@ -400,16 +402,22 @@ class Scene:
         rgb, weight, _, _ = self.film()
         return resolve(rgb, weight)
 
-    def trace_rays(self, rays):
-        """Nearest and any hit of rays (n, 8) float32: origin, direction, t_min, t_max."""
+    def trace_rays(self, rays, wave_stack=0):
+        """Nearest and any hit of rays (n, 8) float32: origin, direction, t_min, t_max. With
+        `wave_stack` (1..1024), each ray is traced by a whole wave over a shared stack of that many
+        entries (rpx_trace_rays_wave)."""
         rays = np.ascontiguousarray(rays, np.float32)
         n = rays.shape[0]
         inst = np.zeros(n, np.uint32)
         tri = np.zeros(n, np.uint32)
         t = np.zeros(n, np.float32)
         anyhit = np.zeros(n, np.uint8)
-        _check(library().rpx_trace_rays(self.handle, _ptr(rays), n, _ptr(inst), _ptr(tri), _ptr(t), _ptr(anyhit)),
-               "rpx_trace_rays")
+        if wave_stack:
+            _check(library().rpx_trace_rays_wave(self.handle, _ptr(rays), n, wave_stack, _ptr(inst), _ptr(tri), _ptr(t),
+                                                 _ptr(anyhit)), "rpx_trace_rays_wave")
+        else:
+            _check(library().rpx_trace_rays(self.handle, _ptr(rays), n, _ptr(inst), _ptr(tri), _ptr(t), _ptr(anyhit)),
+                   "rpx_trace_rays")
         return inst, tri, t, anyhit
 
     def stats(self):
