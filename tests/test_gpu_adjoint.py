@@ -163,5 +163,5 @@ def test_torch_function_gradients_and_a_fit_step(gpu):
         with torch.no_grad():
             params[:, 1:4].clamp_(0.02, 0.98)
             texture.clamp_(0.02, 0.98)
-        losses.append(float(loss))
+        losses.append(float(loss.detach()))
     assert losses[-1] < 0.8 * losses[0], losses
