@@ -409,26 +409,41 @@ def denoise_bench(scene, spp, repeats=5):
 def adaptive_bench(native, rsb, tuned, target=0.03, cap=64):
     """Adaptive sampling on the same render: 8 x 8 blocks in rounds to a 3 % noise target (the
     reference's E10 example target) or 64 samples, the film resident in HBM and the decisions
-    (per-pixel errors, smoothing, block maxima) on the GPU between rounds."""
+    (per-pixel errors, smoothing, block maxima) on the GPU between rounds. Timed with candidate
+    blocks' samples up to the cap rendered ahead in one run and kept for their rounds (the
+    default), and with every round rendering only its own samples; both give the same film."""
     from repackshot_amd.rsb import with_adaptive
 
     scene = native.Scene(with_adaptive(rsb, target))
-    scene.set_options(pool=tuned["pool"], shape=tuned["shape"], shadow_shape=tuned["shadow_shape"])
     W, H = scene.width, scene.height
-    scene.render_adaptive(cap)  # warm-up (pool and planes allocated)
-    t0 = time.perf_counter()
-    st = scene.render_adaptive(cap)
-    wall = time.perf_counter() - t0
+    modes = {}
+    films = {}
+    for mode, ahead in (("rounds_only", False), ("ahead", True)):
+        scene.set_options(pool=tuned["pool"], shape=tuned["shape"], shadow_shape=tuned["shadow_shape"],
+                          adaptive_ahead=ahead)
+        scene.render_adaptive(cap)  # warm-up (pool, planes and kept samples allocated)
+        t0 = time.perf_counter()
+        st = scene.render_adaptive(cap)
+        wall = time.perf_counter() - t0
+        work = scene.adaptive_work()
+        films[mode] = [f.copy() for f in scene.film()]
+        modes[mode] = {"wall_s": round(wall, 4), "gpu_ms": round(st["gpu_ms"], 2),
+                       "mpaths_per_s": round(st["paths"] / wall / 1e6, 3), "rendered_path_samples": work["rendered"],
+                       "rendered_ahead": work["rendered_ahead"], "runs_ahead": work["runs_ahead"],
+                       "unused_path_samples": work["unused"]}
     counts = scene.adaptive_counts()
     scene.close()
     uniform = W * H * cap
+    same = all(np.array_equal(a.view(np.uint32), b.view(np.uint32))
+               for a, b in zip(films["ahead"], films["rounds_only"]))
     rec = {"noise_target": target, "cap": cap, "decisions": st["decisions"], "rounds": st["rounds"],
            "path_samples": st["paths"], "uniform_path_samples": uniform,
            "fraction_of_uniform": round(st["paths"] / uniform, 4),
            "block_samples_min": st["min_samples"], "block_samples_max": st["max_samples"],
            "blocks_at_cap": int((counts >= cap).sum()), "blocks": int(counts.size),
-           "last_max_error": round(st["max_error"], 5), "wall_s": round(wall, 4), "gpu_ms": round(st["gpu_ms"], 2),
-           "mpaths_per_s": round(st["paths"] / wall / 1e6, 3)}
+           "last_max_error": round(st["max_error"], 5), "wall_s": modes["ahead"]["wall_s"],
+           "gpu_ms": modes["ahead"]["gpu_ms"], "mpaths_per_s": modes["ahead"]["mpaths_per_s"],
+           "modes": modes, "films_equal_across_modes": same}
     log("bench: adaptive %s" % rec)
     return rec
 

@@ -137,7 +137,8 @@ uint32_t rpx_scene_stats(uint32_t handle, uint64_t* out4);
  * launch shapes (index 0..7; 0xffffffff keeps), iterations per completion check (0 keeps), most
  * path samples per chunk (0 keeps; at most 2^27), flags: bit 0 one sample pass at a time (the
  * pool drains after each pass), bit 1 each group of iterations launched as a captured HIP graph,
- * bit 2 no drain; tail: once no new path is left, at most this many live paths are finished in
+ * bit 2 no drain, bit 3 adaptive rounds render only their own samples (default: a block's samples
+ * up to the cap are rendered ahead in one run once it is a candidate, and kept); tail: once no new path is left, at most this many live paths are finished in
  * one drain launch, each lane taking its path to its end (0 keeps; default 2^16). Films do not
  * depend on any of them. */
 uint32_t rpx_set_options(uint32_t handle, uint32_t pool, uint32_t shape, uint32_t shadow_shape, uint32_t group,
@@ -217,6 +218,10 @@ uint32_t rpx_queue_free(uint32_t queue);
  * row; status 2 when `cap` words do not hold them). */
 uint32_t rpx_render_adaptive(uint32_t handle, uint32_t cap, uint64_t* out8);
 uint32_t rpx_adaptive_counts(uint32_t handle, uint32_t* counts, uint32_t cap);
+/* The last adaptive render's work (no counterpart in the reference): path samples rendered, of
+ * them rendered ahead of their rounds, runs that rendered ahead, and samples rendered ahead that no
+ * round used. */
+uint32_t rpx_adaptive_work(uint32_t handle, uint64_t* out4);
 /* Passes and the denoiser with everything resident in HBM. rpx_render_passes writes the exact
  * passes of every pixel (centre rays) into the handle's pass film and adds the albedo and coverage
  * of samples [first, first + count) to its sums in sample order (rt-core/src/render.rs:500-640);

@@ -44,6 +44,12 @@ struct Handle {
   size_t adaptive_px = 0, adaptive_blocks = 0;
   u32* d_block_counts = nullptr;
   float *d_pixel_error = nullptr, *d_block_error = nullptr;
+  KeptSplat* d_kept_table = nullptr;  // one entry per block
+  // samples rendered ahead of the adaptive rounds (float4 radiance + alpha each), and the last
+  // adaptive render's work: path samples rendered, of them rendered ahead, runs ahead, unused
+  DevBuf kept{};
+  size_t kept_cap = 0;
+  u64 adaptive_work[4] = {0, 0, 0, 0};
   // resident pass film (sums of the sampled passes, exact planes), the resolved albedo the
   // denoiser reads, the denoiser's work buffers and its output (RGBA sums)
   PassPlanes passes{};
@@ -62,6 +68,7 @@ struct Handle {
     dev_free_all(film_bufs);
     dev_free_all(tex_bufs);
     dev_free_all(adaptive_bufs);
+    if (kept.p) hipFree(kept.p);
     if (stream) hipStreamDestroy(stream);
   }
 };
@@ -281,6 +288,7 @@ void apply_options(Engine& e, u32 pool, u32 shape, u32 shadow_shape, u32 group, 
   if (group) e.opt.group = std::max<u32>(2, group & ~1u);
   e.opt.per_pass = (flags & 1u) != 0;
   e.opt.graph = (flags & 2u) != 0;
+  e.opt.adaptive_ahead = (flags & 8u) == 0;
 }
 
 void fill_timing(const Engine& e, float* out12) {
@@ -542,7 +550,8 @@ static bool ensure_adaptive(Handle* h, size_t px, size_t blocks, std::string& er
   h->adaptive_px = h->adaptive_blocks = 0;
   auto& B = h->adaptive_bufs;
   if (!dev_alloc(B, blocks * 4, (void**)&h->d_block_counts) || !dev_alloc(B, px * 4, (void**)&h->d_pixel_error) ||
-      !dev_alloc(B, blocks * 4, (void**)&h->d_block_error)) {
+      !dev_alloc(B, blocks * 4, (void**)&h->d_block_error) ||
+      !dev_alloc(B, blocks * sizeof(KeptSplat), (void**)&h->d_kept_table)) {
     dev_free_all(h->adaptive_bufs);
     err = "device memory exhausted for the adaptive sampling's error planes";
     return false;
@@ -592,6 +601,95 @@ uint32_t rpx_render_adaptive(uint32_t handle, uint32_t cap, uint64_t* out8) {
   std::vector<float> block_error(blocks);
   std::vector<DevSegment> segs;
   hipError_t e = hipSuccess;
+  // Rendering ahead: a block's k-th sample is the same whichever round renders it, so once a block
+  // may take part in the rounds (a candidate of a decision), all its samples up to the cap are
+  // rendered in one run and kept; each round then adds the block's next samples from what was
+  // kept, in sample order, and only blocks chosen without kept samples need a run of their own.
+  // The decisions and the film are those of rendering each round's samples alone.
+  const bool ahead = h->engine->opt.adaptive_ahead;
+  struct Kept {
+    size_t base = 0;  // pixel p's kept samples start at kept + base + p * count
+    u32 first = 0, count = 0;
+  };
+  std::vector<Kept> kept(blocks);
+  size_t kept_used = 0;
+  u64 rendered = 0, rendered_ahead = 0, runs_ahead = 0;
+  std::vector<KeptSplat> table;
+  auto rect_of = [&](u32 b, DevSegment& s) {
+    s = DevSegment{};
+    s.scene = 0;
+    s.x0 = (b % ad.cols) * adaptive::BLOCK;
+    s.y0 = (b / ad.cols) * adaptive::BLOCK;
+    s.w = std::min(adaptive::BLOCK, W - s.x0);
+    s.h = std::min(adaptive::BLOCK, H - s.y0);
+    s.film = F;
+    s.film_w = W;
+  };
+  // Renders blocks `need` from their current count to the cap and keeps the results; one run per
+  // group of blocks whose results fit one chunk.
+  auto render_ahead = [&](const std::vector<u32>& need) -> hipError_t {
+    const u32 limit = std::min(std::max<u32>(h->engine->opt.chunk, 1), RESULTS_MAX);
+    size_t total = 0;
+    for (u32 b : need) {
+      DevSegment s;
+      rect_of(b, s);
+      total += (size_t)s.w * s.h * (ad.cap - ad.counts[b]);
+    }
+    if (kept_used + total > h->kept_cap) {
+      size_t cap_new = std::max(kept_used + total + total / 4, 2 * h->kept_cap);
+      void* p = nullptr;
+      if (hipMalloc(&p, cap_new * sizeof(float4)) != hipSuccess) {
+        err = "device memory exhausted for the samples rendered ahead of the adaptive rounds";
+        return hipErrorOutOfMemory;
+      }
+      hipError_t r = hipSuccess;
+      if (kept_used) r = hipMemcpyAsync(p, h->kept.p, kept_used * sizeof(float4), hipMemcpyDeviceToDevice, st);
+      if (r == hipSuccess) r = hipStreamSynchronize(st);
+      if (r != hipSuccess) {
+        hipFree(p);
+        return r;
+      }
+      if (h->kept.p) hipFree(h->kept.p);
+      h->kept.p = p;
+      h->kept.bytes = cap_new * sizeof(float4);
+      h->kept_cap = cap_new;
+    }
+    const bool per_pass = h->engine->opt.per_pass;
+    h->engine->opt.per_pass = false;  // the run's results must stay in one chunk
+    hipError_t r = hipSuccess;
+    size_t k = 0;
+    while (k < need.size() && r == hipSuccess) {
+      std::vector<DevSegment> run_segs;
+      std::vector<u32> run_blocks;
+      u32 base = 0;
+      for (; k < need.size(); ++k) {
+        u32 b = need[k];
+        DevSegment s;
+        rect_of(b, s);
+        s.first = ad.counts[b];
+        s.count = ad.cap - ad.counts[b];
+        u32 n = s.w * s.h * s.count;
+        if (!run_segs.empty() && (u64)base + n > limit) break;
+        s.base = base;
+        base += n;
+        run_segs.push_back(s);
+        run_blocks.push_back(b);
+      }
+      r = h->engine->run(h->d_scene, run_segs, false, err);
+      if (r == hipSuccess)
+        r = hipMemcpyAsync((float4*)h->kept.p + kept_used, h->engine->P.results, (size_t)base * sizeof(float4),
+                           hipMemcpyDeviceToDevice, st);
+      if (r != hipSuccess) break;
+      for (size_t i = 0; i < run_blocks.size(); ++i)
+        kept[run_blocks[i]] = Kept{kept_used + run_segs[i].base, run_segs[i].first, run_segs[i].count};
+      kept_used += base;
+      rendered += base;
+      rendered_ahead += base;
+      ++runs_ahead;
+    }
+    h->engine->opt.per_pass = per_pass;
+    return r;
+  };
   for (;;) {
     if (ad.left() == 0) {
       // adaptive.rs:517-539: the round's samples are counted, then the next round is chosen.
@@ -612,30 +710,84 @@ uint32_t rpx_render_adaptive(uint32_t handle, uint32_t cap, uint64_t* out8) {
       if (ad.round == 0) break;
     }
     u32 count = ad.left();
-    // Every block with a quota renders its next samples; the blocks' pixels are disjoint.
-    segs.clear();
-    for (u32 b = 0; b < blocks; ++b) {
-      u32 q = ad.quota(b);
-      if (q == 0) continue;
-      u32 bx = b % ad.cols, by = b / ad.cols;
-      DevSegment s{};
-      s.scene = 0;
-      s.x0 = bx * adaptive::BLOCK;
-      s.y0 = by * adaptive::BLOCK;
-      s.w = std::min(adaptive::BLOCK, W - s.x0);
-      s.h = std::min(adaptive::BLOCK, H - s.y0);
-      s.first = ad.counts[b];
-      s.count = std::min(q, count);
-      s.film = F;
-      s.film_w = W;
-      segs.push_back(s);
-      paths += (u64)s.w * s.h * s.count;
+    if (!ahead || decisions == 0) {
+      // Every block with a quota renders its next samples; the blocks' pixels are disjoint.
+      segs.clear();
+      for (u32 b = 0; b < blocks; ++b) {
+        u32 q = ad.quota(b);
+        if (q == 0) continue;
+        DevSegment s;
+        rect_of(b, s);
+        s.first = ad.counts[b];
+        s.count = std::min(q, count);
+        segs.push_back(s);
+        paths += (u64)s.w * s.h * s.count;
+      }
+      e = h->engine->run(h->d_scene, segs, true, err, true);
+      if (e != hipSuccess) break;
+      for (const DevSegment& s : segs) rendered += (u64)s.w * s.h * s.count;
+    } else {
+      // Chosen blocks without their next samples kept, and (with them, in the same run) every
+      // candidate of the decision not yet rendered ahead; at the first decision, every candidate.
+      const float floor_target = ad.target > 0.0f ? ad.target : 0.0f;
+      std::vector<u32> need;
+      bool chosen_missing = false;
+      for (u32 b = 0; b < blocks; ++b) {
+        u32 q = ad.quota(b);
+        if (q == 0) continue;
+        const Kept& k = kept[b];
+        u32 n = std::min(q, count);
+        if (!(k.count && k.first <= ad.counts[b] && ad.counts[b] + n <= k.first + k.count)) chosen_missing = true;
+      }
+      if (chosen_missing || decisions == 1)
+        for (u32 b = 0; b < blocks; ++b) {
+          bool candidate = block_error[b] > floor_target && ad.counts[b] < ad.cap;
+          bool chosen = ad.quota(b) > 0;
+          const Kept& k = kept[b];
+          bool has = k.count && k.first <= ad.counts[b] && ad.counts[b] < k.first + k.count;
+          if ((candidate || chosen) && !has) need.push_back(b);
+        }
+      if (!need.empty()) {
+        e = render_ahead(need);
+        if (e != hipSuccess) break;
+      }
+      table.clear();
+      for (u32 b = 0; b < blocks; ++b) {
+        u32 q = ad.quota(b);
+        if (q == 0) continue;
+        const Kept& k = kept[b];
+        KeptSplat t{};
+        rect_of(b, t.seg);
+        t.seg.first = ad.counts[b];
+        t.seg.count = std::min(q, count);
+        t.res = (const float4*)h->kept.p + k.base;
+        t.stride = k.count;
+        t.skip = ad.counts[b] - k.first;
+        table.push_back(t);
+        paths += (u64)t.seg.w * t.seg.h * t.seg.count;
+      }
+      e = hipMemcpyAsync(h->d_kept_table, table.data(), table.size() * sizeof(KeptSplat), hipMemcpyHostToDevice, st);
+      if (e == hipSuccess) e = launch_splat_kept(h->d_kept_table, (u32)table.size(), st);
+      if (e != hipSuccess) break;
     }
-    e = h->engine->run(h->d_scene, segs, true, err, true);
-    if (e != hipSuccess) break;
     ad.done = std::min(ad.done + count, ad.round);
     ++rounds;
   }
+  // Samples rendered ahead that no round used.
+  u64 unused = 0;
+  if (e == hipSuccess)
+    for (u32 b = 0; b < blocks; ++b) {
+      const Kept& k = kept[b];
+      if (k.count && ad.counts[b] < k.first + k.count) {
+        DevSegment s;
+        rect_of(b, s);
+        unused += (u64)s.w * s.h * (k.first + k.count - std::max(ad.counts[b], k.first));
+      }
+    }
+  h->adaptive_work[0] = rendered;
+  h->adaptive_work[1] = rendered_ahead;
+  h->adaptive_work[2] = runs_ahead;
+  h->adaptive_work[3] = unused;
   hipEventRecord(ev[1], st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   float ms = 0;
@@ -675,6 +827,15 @@ uint32_t rpx_adaptive_counts(uint32_t handle, uint32_t* counts, uint32_t cap) {
   std::lock_guard<std::mutex> lock(h->mutex);
   if (cap < h->block_counts.size()) return RPX_BAD_TILE;
   std::memcpy(counts, h->block_counts.data(), h->block_counts.size() * 4);
+  return RPX_OK;
+}
+
+uint32_t rpx_adaptive_work(uint32_t handle, uint64_t* out4) {
+  if (!out4) return RPX_NULL_POINTER;
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  for (int k = 0; k < 4; ++k) out4[k] = h->adaptive_work[k];
   return RPX_OK;
 }
 

@@ -51,6 +51,9 @@ struct EngineOptions {
   // paths left, 43 ms from 16,384 and 59 ms from all of them; fewer lanes per wave were slower.
   u32 drain_lanes = 64;
   u32 park_at = 16384;
+  // Adaptive rounds: a block's samples up to the cap are rendered in one run once it is a
+  // candidate, and kept for the rounds (false: each round renders just its own samples).
+  bool adaptive_ahead = true;
 };
 
 // GPU time of the last run, from events on the engine's stream (profiling on).

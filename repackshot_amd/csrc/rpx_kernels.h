@@ -159,6 +159,15 @@ hipError_t launch_finish(const PathState& P, const Launch& L, u32 parity);
 hipError_t launch_splat(const PathState& P, const DevSegment& s, hipStream_t st);
 // Splats every segment of a device table in one launch; the segments' pixels must not overlap.
 hipError_t launch_splat_segments(const PathState& P, const DevSegment* segs, u32 n_segs, hipStream_t st);
+// Samples kept from an earlier run: `seg` names the pixels, the film, and how many samples each
+// pixel gets (seg.count); pixel p's (tiled order) are res[p * stride + skip + k], k < seg.count.
+struct KeptSplat {
+  DevSegment seg;
+  const float4* res;
+  u32 stride, skip;
+};
+// Splats a device table of kept samples in one launch (pixels disjoint across entries).
+hipError_t launch_splat_kept(const KeptSplat* kept, u32 n, hipStream_t st);
 // Copies the one-sample results of a w x h tile (work ids 0 .., pixels in the regen's tiled order)
 // out as row-major rgb or rgba floats.
 hipError_t launch_tile_out(const PathState& P, float* out, u32 w, u32 h, u32 channels, hipStream_t st);

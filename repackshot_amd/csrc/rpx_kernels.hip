@@ -2766,9 +2766,12 @@ __global__ void __launch_bounds__(64) k_finish_wave(const DevScene* __restrict__
 // order (lane t: pixel p0 + t). A pixel's results are contiguous (work ids sample-minor); the block
 // stages SPLAT_CH samples of each of its pixels at a time in LDS with coalesced loads, then each
 // lane adds its own pixel's. Every lane of the block must call this with the same p0.
+// `res`, `stride` and `skip`: pixel p's samples start at res + p * stride + skip (a segment's own
+// results: stride count, skip 0; results kept from an earlier run may hold more samples per pixel
+// than the segment adds).
 constexpr u32 SPLAT_CH = 8;
 template <u32 T>
-__device__ __forceinline__ void splat_block(const PathState& P, const DevSegment& s, u32 p0,
+__device__ __forceinline__ void splat_block(const float4* res, u32 stride, u32 skip, const DevSegment& s, u32 p0,
                                             float4 (*tile)[SPLAT_CH + 1]) {
   const u32 n = s.w * s.h, count = s.count, t = threadIdx.x, p = p0 + t;
   const bool mine = p < n;
@@ -2785,13 +2788,12 @@ __device__ __forceinline__ void splat_block(const PathState& P, const DevSegment
     al = s.film.alpha[f];
     l2 = s.film.lum2[f];
   }
-  const float4* res = P.results + s.base;
   for (u32 c0 = 0; c0 < count; c0 += SPLAT_CH) {
     const u32 m = count - c0 < SPLAT_CH ? count - c0 : SPLAT_CH;
     __syncthreads();
     for (u32 e = t; e < T * SPLAT_CH; e += T) {
       u32 q = e / SPLAT_CH, k = e - q * SPLAT_CH;
-      if (k < m && p0 + q < n) tile[q][k] = res[(size_t)(p0 + q) * count + c0 + k];
+      if (k < m && p0 + q < n) tile[q][k] = res[(size_t)(p0 + q) * stride + skip + c0 + k];
     }
     __syncthreads();
     if (mine)
@@ -2820,7 +2822,8 @@ __device__ __forceinline__ void splat_block(const PathState& P, const DevSegment
 __global__ void __launch_bounds__(256) k_splat(PathState P, DevSegment s) {
   __shared__ float4 tile[256][SPLAT_CH + 1];
   const u32 n = s.w * s.h;
-  for (u32 p0 = blockIdx.x * 256; p0 < n; p0 += gridDim.x * 256) splat_block<256>(P, s, p0, tile);
+  for (u32 p0 = blockIdx.x * 256; p0 < n; p0 += gridDim.x * 256)
+    splat_block<256>(P.results + s.base, s.count, 0u, s, p0, tile);
 }
 
 // Many small segments whose pixels are disjoint (the blocks of an adaptive round): one wave per
@@ -2830,7 +2833,19 @@ __global__ void __launch_bounds__(64) k_splat_segments(PathState P, const DevSeg
   for (u32 i = blockIdx.x; i < n_segs; i += gridDim.x) {
     DevSegment s = segs[i];
     u32 n = s.w * s.h;
-    for (u32 p0 = 0; p0 < n; p0 += 64) splat_block<64>(P, s, p0, tile);
+    for (u32 p0 = 0; p0 < n; p0 += 64) splat_block<64>(P.results + s.base, s.count, 0u, s, p0, tile);
+  }
+}
+
+// Segments whose samples were rendered ahead by an earlier run (the adaptive rounds' kept
+// results): entry i adds `seg.count` samples per pixel from its own results, one wave per entry,
+// the entries' pixels disjoint.
+__global__ void __launch_bounds__(64) k_splat_kept(const KeptSplat* __restrict__ kept, u32 n) {
+  __shared__ float4 tile[64][SPLAT_CH + 1];
+  for (u32 i = blockIdx.x; i < n; i += gridDim.x) {
+    KeptSplat k = kept[i];
+    u32 npx = k.seg.w * k.seg.h;
+    for (u32 p0 = 0; p0 < npx; p0 += 64) splat_block<64>(k.res, k.stride, k.skip, k.seg, p0, tile);
   }
 }
 
@@ -3628,6 +3643,11 @@ hipError_t launch_splat(const PathState& P, const DevSegment& s, hipStream_t st)
 }
 hipError_t launch_splat_segments(const PathState& P, const DevSegment* segs, u32 n_segs, hipStream_t st) {
   hipLaunchKernelGGL(k_splat_segments, dim3(std::min<u32>(n_segs, 1u << 16)), dim3(64), 0, st, P, segs, n_segs);
+  return hipGetLastError();
+}
+hipError_t launch_splat_kept(const KeptSplat* kept, u32 n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_splat_kept, dim3(std::min<u32>(n, 1u << 16)), dim3(64), 0, st, kept, n);
   return hipGetLastError();
 }
 hipError_t launch_tile_out(const PathState& P, float* out, u32 w, u32 h, u32 channels, hipStream_t st) {

@@ -79,6 +79,7 @@ _SIGS = {
     "rpx_queue_free": (ctypes.c_uint32, [ctypes.c_uint32]),
     "rpx_render_adaptive": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_adaptive_counts": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32]),
+    "rpx_adaptive_work": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rt_render_tile_passes": (ctypes.c_uint32, [ctypes.c_uint32] * 6 + [ctypes.c_void_p]),
     "rt_render_tile_albedo": (ctypes.c_uint32, [ctypes.c_uint32] * 6 + [ctypes.c_void_p]),
     "rt_render_tile_alpha": (ctypes.c_uint32, [ctypes.c_uint32] * 6 + [ctypes.c_void_p]),
@@ -126,6 +127,7 @@ def _planes_of(buf, mask, w, h):
 FLAG_PER_PASS = 1  # one sample pass at a time (the pool drains after each pass)
 FLAG_GRAPH = 2     # groups of iterations launched as captured HIP graphs
 FLAG_NO_DRAIN = 4  # the iteration loop runs to the end (no drain launch for the last paths)
+FLAG_ROUND_ONLY = 8  # adaptive rounds render only their own samples (nothing rendered ahead)
 
 _lib = None
 
@@ -368,6 +370,14 @@ class Scene:
                 "max_samples": int(out[4]), "max_error": float(bits[0]), "remaining": float(bits[1]),
                 "gpu_ms": int(out[7]) / 1.0e6}
 
+    def adaptive_work(self):
+        """The last adaptive render's work: path samples rendered, of them rendered ahead of their
+        rounds, runs that rendered ahead, and samples rendered ahead that no round used."""
+        out = np.zeros(4, np.uint64)
+        _check(library().rpx_adaptive_work(self.handle, _ptr(out)), "rpx_adaptive_work")
+        return {"rendered": int(out[0]), "rendered_ahead": int(out[1]), "runs_ahead": int(out[2]),
+                "unused": int(out[3])}
+
     def adaptive_counts(self):
         """Samples of every 8 x 8 block of the last adaptive render, (rows, cols) uint32."""
         cols, rows = (self.width + 7) // 8, (self.height + 7) // 8
@@ -426,10 +436,12 @@ class Scene:
         return {"rays": int(out[0]), "nearest": int(out[1]), "shadow": int(out[2]), "samples": int(out[3])}
 
     def set_options(self, pool=0, shape=KEEP, shadow_shape=KEEP, group=0, chunk=0, per_pass=False, graph=False,
-                    drain=True, tail=0):
-        """Path-pool options (0 / KEEP keep the current value; the flags per_pass, graph and drain
-        are set by every call); the film does not depend on them."""
-        flags = _flags(per_pass, graph, drain)
+                    drain=True, tail=0, adaptive_ahead=True):
+        """Path-pool options (0 / KEEP keep the current value; the flags per_pass, graph, drain and
+        adaptive_ahead are set by every call); the film does not depend on them. adaptive_ahead:
+        the adaptive rounds render a candidate block's samples up to the cap in one run and keep
+        them (False: each round renders only its own samples)."""
+        flags = _flags(per_pass, graph, drain, adaptive_ahead)
         _check(library().rpx_set_options(self.handle, pool, shape, shadow_shape, group, chunk, flags, tail),
                "rpx_set_options")
 
@@ -567,8 +579,9 @@ class Adaptive:
         return out, active
 
 
-def _flags(per_pass, graph, drain):
-    return (FLAG_PER_PASS if per_pass else 0) | (FLAG_GRAPH if graph else 0) | (0 if drain else FLAG_NO_DRAIN)
+def _flags(per_pass, graph, drain, ahead=True):
+    return ((FLAG_PER_PASS if per_pass else 0) | (FLAG_GRAPH if graph else 0) | (0 if drain else FLAG_NO_DRAIN)
+            | (0 if ahead else FLAG_ROUND_ONLY))
 
 
 def _tune(out):

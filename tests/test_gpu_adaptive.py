@@ -101,3 +101,30 @@ def test_adaptive_render_twice_is_the_same(gpu):
     assert {k: a[k] for k in a if k != "gpu_ms"} == {k: b[k] for k in b if k != "gpu_ms"}
     for x, y in zip(fa, fb):
         assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+
+
+@pytest.mark.parametrize("cap,chunk", [(64, 0), (24, 0), (64, 40000)])
+def test_samples_rendered_ahead_give_the_rounds_film(gpu, cap, chunk):
+    """A candidate block's samples up to the cap rendered in one run and kept for its rounds (the
+    default) against every round rendering only its own samples: the same decisions, block counts
+    and film bit for bit, also when the kept samples need several runs (a small chunk)."""
+    rsb = adaptive_scene()
+    s = gpu.Scene(rsb)
+    s.set_options(chunk=chunk, adaptive_ahead=False)
+    a = s.render_adaptive(cap)
+    wa = s.adaptive_work()
+    fa, ca = s.film(), s.adaptive_counts()
+    s.set_options(chunk=chunk)
+    b = s.render_adaptive(cap)
+    wb = s.adaptive_work()
+    fb, cb = s.film(), s.adaptive_counts()
+    assert {k: a[k] for k in a if k != "gpu_ms"} == {k: b[k] for k in b if k != "gpu_ms"}
+    assert np.array_equal(ca, cb)
+    for x, y in zip(fa, fb):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+    # Without rendering ahead every rendered sample is used; with it, the used ones are the same.
+    assert wa["rendered"] == a["paths"] and wa["rendered_ahead"] == 0 and wa["unused"] == 0
+    assert wb["rendered"] - wb["unused"] == b["paths"]
+    assert wb["rendered_ahead"] > 0 and wb["runs_ahead"] >= 1
+    if chunk:
+        assert wb["runs_ahead"] >= 2
