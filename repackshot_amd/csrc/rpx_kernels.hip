@@ -1900,7 +1900,13 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
         }
       }
     }
-    C3 d_nee[N_MAT_PARAMS];
+    // The next-event term's derivative inputs (ADJ), and where its gradient goes: 1 this vertex's
+    // g_now, 2 the shadow ray's record.
+    bool nee_d = false, nee_diffuse = false;
+    V3 nee_wi = v3(0.0f, 0.0f, 0.0f);
+    C3 nee_thr = splat(0.0f), nee_rad = splat(0.0f);
+    float nee_scale = 0.0f;
+    u32 nee_to = 0;
     Frame frame = frame_from_normal(s_normal);
     V3 wo = to_local(frame, -d);
     V3 ng = s_geometric;
@@ -2160,20 +2166,23 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
           add_c = (bounce >= 1 && S.clamp_indirect > 0.0f && pk > S.clamp_indirect) ? c * (S.clamp_indirect / pk) : c;
         }
         if constexpr (ADJ) {
-          // d(added contribution)/d parameter with the light sample and the MIS weight held.
+          // d(added contribution)/d parameter with the light sample and the MIS weight held; the
+          // derivative is taken below, with the sampled direction's, from the same BSDF builds.
           if (adds && has_mat) {
-            C3 unused[N_MAT_PARAMS];
-            bsdf_param_derivs(T, S.materials[mat], base, s_outside, wo, true, wi, !ls_glossy, false, wi, d_nee, unused);
             float pk = fmax_r(fmax_r(c.r, c.g), c.b);
             float scale = (bounce >= 1 && S.clamp_indirect > 0.0f && pk > S.clamp_indirect) ? S.clamp_indirect / pk : 1.0f;
-            for (int k = 0; k < N_MAT_PARAMS; ++k)
-              target.g[k] = dot3(adj, thr * d_nee[k] * ls_rad * (fabs_(wi.z) * weight / ls_pdf * scale));
+            nee_d = true;
+            nee_wi = wi;
+            nee_diffuse = !ls_glossy;
+            nee_thr = thr;
+            nee_rad = ls_rad;
+            nee_scale = fabs_(wi.z) * weight / ls_pdf * scale;
           }
         }
         if (!ls_shadow) {
           if (adds) sum = sum + add_c;
           if constexpr (ADJ) {
-            if (adds && has_mat) g_now = target;
+            if (adds && has_mat) nee_to = 1;
           }
         } else {
           float t_max = is_finite(ls_dist) ? ls_dist * (1.0f - 1.0e-4f) : RPX_INF;
@@ -2189,11 +2198,7 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
             P.sh_o[i] = make_float4(from_p.x, from_p.y, from_p.z, t_max);
             P.sh_d[i] = make_float4(ls_wi.x, ls_wi.y, ls_wi.z, 0.0f);
             P.sh_c[i] = make_float4(add_c.r, add_c.g, add_c.b, 0.0f);
-            if constexpr (ADJ) {
-              P.gn[2 * (size_t)i] = make_float4(target.g[0], target.g[1], target.g[2], target.g[3]);
-              P.gn[2 * (size_t)i + 1] = make_float4(target.g[4], 0.0f, 0.0f, 0.0f);
-              P.tgt[i] = make_uint4(target.mat, target.tex, f2u(target.ax), f2u(target.ay));
-            }
+            if constexpr (ADJ) nee_to = 2;
           } else {
             // A shadow ray is still cast (and counted) for a contribution that adds nothing.
             atomicAdd((unsigned long long*)&P.rays[1], 1ull);
@@ -2205,6 +2210,7 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
     float ub0 = sample_at(P.sobol, key, sample_index, dims + OFFSET_BSDF);
     float ub1 = sample_at(P.sobol, key, sample_index, dims + OFFSET_BSDF + 1);
     BsdfSample bs;
+    bool sampled_d = false;  // ADJ: the path goes on along bs.wi from a surface with a material
     if (exit ? exit_sample(exit_eta2, wo, ub0, ub1, bs) : bsdf_sample(T, B, wo, ub0, ub1, bs)) {
       bool transmitted = bs.wi.z < 0.0f;
       V3 wi = normalize(to_world(frame, bs.wi));
@@ -2258,22 +2264,35 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
         P.sel_w0[i] = make_float4(sel.w[0], sel.w[1], sel.w[2], sel.w[3]);
         P.sel_w1[i] = make_float4(sel.w[4], sel.w[5], sel.w[6], sel.w[7]);
         cont = true;
-        if constexpr (ADJ) {
-          if (has_mat) {
-            // d log f / d parameter along the sampled direction, applied once the radiance
-            // gathered after this vertex is known (at the next vertex).
-            C3 f_s = bsdf_eval(T, B, wo, bs.wi, false);
-            C3 unused[N_MAT_PARAMS], d_s[N_MAT_PARAMS];
-            bsdf_param_derivs(T, S.materials[mat], base, s_outside, wo, false, bs.wi, false, true, bs.wi, unused, d_s);
-            for (int k = 0; k < N_MAT_PARAMS; ++k) {
-              C3 q = c3(f_s.r > 0.0f ? d_s[k].r / f_s.r : 0.0f, f_s.g > 0.0f ? d_s[k].g / f_s.g : 0.0f,
-                        f_s.b > 0.0f ? d_s[k].b / f_s.b : 0.0f);
-              P.dl[(size_t)N_MAT_PARAMS * i + k] = make_float4(q.r, q.g, q.b, 0.0f);
-            }
-            P.tgt[i] = make_uint4(target.mat, target.tex, f2u(target.ax), f2u(target.ay));
+        if constexpr (ADJ) sampled_d = has_mat;
+      }
+    }
+    if constexpr (ADJ) {
+      // The BSDF's derivatives at the next-event direction and along the sampled one, each
+      // parameter's dual BSDF built once for both.
+      if (nee_d || sampled_d) {
+        C3 d_nee[N_MAT_PARAMS], d_s[N_MAT_PARAMS];
+        bsdf_param_derivs(T, S.materials[mat], base, s_outside, wo, nee_d, nee_wi, nee_diffuse, sampled_d,
+                          sampled_d ? bs.wi : nee_wi, d_nee, d_s);
+        if (nee_d)
+          for (int k = 0; k < N_MAT_PARAMS; ++k) target.g[k] = dot3(adj, nee_thr * d_nee[k] * nee_rad * nee_scale);
+        if (sampled_d) {
+          // d log f / d parameter along the sampled direction, applied once the radiance
+          // gathered after this vertex is known (at the next vertex).
+          C3 f_s = bsdf_eval(T, B, wo, bs.wi, false);
+          for (int k = 0; k < N_MAT_PARAMS; ++k) {
+            C3 q = c3(f_s.r > 0.0f ? d_s[k].r / f_s.r : 0.0f, f_s.g > 0.0f ? d_s[k].g / f_s.g : 0.0f,
+                      f_s.b > 0.0f ? d_s[k].b / f_s.b : 0.0f);
+            P.dl[(size_t)N_MAT_PARAMS * i + k] = make_float4(q.r, q.g, q.b, 0.0f);
           }
         }
       }
+      if (nee_to == 1) g_now = target;
+      if (nee_to == 2) {
+        P.gn[2 * (size_t)i] = make_float4(target.g[0], target.g[1], target.g[2], target.g[3]);
+        P.gn[2 * (size_t)i + 1] = make_float4(target.g[4], 0.0f, 0.0f, 0.0f);
+      }
+      if (nee_to == 2 || sampled_d) P.tgt[i] = make_uint4(target.mat, target.tex, f2u(target.ax), f2u(target.ay));
     }
   }
   P.sum[i] = make_float4(sum.r, sum.g, sum.b, alpha);
@@ -2282,7 +2301,9 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
 }  // namespace
 
 // Three waves per SIMD: the register budget (168) the shade fits without spilling; two waves leave
-// the loads of a path's records unhidden, four spill.
+// the loads of a path's records unhidden, four spill. The media and adjoint variants spill at this
+// budget (164 and 83 registers) and not at two waves (222 and 256 registers), but two waves made the
+// caustics render slower (its gather wants the waves) and left the adjoint's time as it was.
 template <bool ADJ, bool EXT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 k_shade(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
