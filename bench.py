@@ -755,6 +755,12 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": round(EXTEND_TRAFFIC_BYTES_PER_RAY * nearest_prof / launches),
             "traffic_source": TRAFFIC_SOURCE,
+            # What the kernel moves from memory per launch over its time: the L2 and the Infinity
+            # Cache serve most node fetches, so the kernel is held by divergence and dependent-fetch
+            # latency, well under the HBM roof it is priced against.
+            "traffic_frac": round(EXTEND_TRAFFIC_BYTES_PER_RAY * nearest_prof / launches / (extend_avg_ms * 1e-3) / 1e9
+                                  / HBM_PEAK_GBS, 4),
+            "limiter": "divergence and dependent-fetch latency (17 of 64 lanes active, VALU busy 72 %)",
             "kernel": "k_extend (closest hit, shape %d): algorithmic bytes of its rays per launch over its own "
                       "average launch time (HIP events on its stream)" % tuned["shape"],
             "bytes_per_ray": round(b_ray, 1),

@@ -125,7 +125,8 @@ bool build_device_scene(Handle* h, std::string& err) {
   if (!upload(B, flat.nodes, d.nodes, st, err) || !upload(B, flat.tris, d.tris, st, err) ||
       !upload(B, flat.instances, d.instances, st, err) || !upload(B, flat.meshes, d.meshes, st, err) ||
       !upload(B, flat.positions, d.positions, st, err) || !upload(B, flat.normals, d.normals, st, err) ||
-      !upload(B, flat.indices, d.indices, st, err) || !upload(B, flat.bsdfs, d.bsdfs, st, err) ||
+      !upload(B, flat.indices, d.indices, st, err) || !upload(B, flat.corners, d.corners, st, err) ||
+      !upload(B, flat.bsdfs, d.bsdfs, st, err) ||
       !upload(B, flat.materials, d.materials, st, err) || !upload(B, flat.areas, d.areas, st, err) ||
       !upload(B, flat.cdf, d.cdf, st, err) || !upload(B, flat.flux_pmf, d.flux_pmf, st, err) ||
       !upload(B, flat.entries, d.entries, st, err) || !upload(B, flat.spheres, d.spheres, st, err) ||

@@ -1298,6 +1298,29 @@ __device__ __forceinline__ float dot3(C3 a, C3 b) { return a.r * b.r + a.g * b.g
 // The surface a ray (o, d) met at instance hit h (instance, mesh triangle): hit details
 // (geometry/mod.rs:731-808), the shading normal without a bevel (bevel.rs:232-252) and the
 // terminator lift (geometry/mod.rs:682-728).
+// A mesh triangle's corner positions and normals from its packed record (one dependent fetch after
+// the mesh, two cache lines), the values the mesh's vertex arrays hold.
+__device__ __forceinline__ void corners_of(const DevScene& S, const DevMesh& mesh, u32 tri, V3& p0, V3& p1, V3& p2,
+                                           V3& n0, V3& n1, V3& n2) {
+  const float4* c = S.corners + CORNER_FLOAT4S * ((size_t)(mesh.index_offset / 3) + tri);
+  float4 a = c[0], b = c[1], e = c[2], f = c[3], g = c[4];
+  p0 = v3(a.x, a.y, a.z);
+  p1 = v3(a.w, b.x, b.y);
+  p2 = v3(b.z, b.w, e.x);
+  n0 = v3(e.y, e.z, e.w);
+  n1 = v3(f.x, f.y, f.z);
+  n2 = v3(f.w, g.x, g.y);
+}
+
+__device__ __forceinline__ void corner_positions_of(const DevScene& S, const DevMesh& mesh, u32 tri, V3& p0, V3& p1,
+                                                    V3& p2) {
+  const float4* c = S.corners + CORNER_FLOAT4S * ((size_t)(mesh.index_offset / 3) + tri);
+  float4 a = c[0], b = c[1], e = c[2];
+  p0 = v3(a.x, a.y, a.z);
+  p1 = v3(a.w, b.x, b.y);
+  p2 = v3(b.z, b.w, e.x);
+}
+
 struct SurfaceHit {
   V3 point, hit_point, normal, geometric;
   bool front, outside;
@@ -1313,12 +1336,8 @@ __device__ __forceinline__ SurfaceHit surface_hit(const DevScene& S, V3 o, V3 d,
   const DevInstance& in = S.instances[inst];
   const DevMesh& mesh = S.meshes[in.mesh];
   bool ident = (in.flags & INST_IDENTITY) != 0;
-  const u32* idx = S.indices + mesh.index_offset + 3 * (size_t)tri;
-  u32 i0 = mesh.vertex_offset + idx[0], i1 = mesh.vertex_offset + idx[1], i2 = mesh.vertex_offset + idx[2];
-  const float* ps = S.positions;
-  V3 p0 = v3(ps[3 * (size_t)i0], ps[3 * (size_t)i0 + 1], ps[3 * (size_t)i0 + 2]);
-  V3 p1 = v3(ps[3 * (size_t)i1], ps[3 * (size_t)i1 + 1], ps[3 * (size_t)i1 + 2]);
-  V3 p2 = v3(ps[3 * (size_t)i2], ps[3 * (size_t)i2 + 1], ps[3 * (size_t)i2 + 2]);
+  V3 p0, p1, p2, n0, n1, n2;
+  corners_of(S, mesh, tri, p0, p1, p2, n0, n1, n2);
   V3 lo = ident ? o : xf_point(in.to_local, o), ld = ident ? d : xf_vector(in.to_local, d);
   PRay r = prepare(lo, ld, RAY_EPSILON, RPX_INF);
   float b0, b1, b2;
@@ -1348,12 +1367,7 @@ __device__ __forceinline__ SurfaceHit surface_hit(const DevScene& S, V3 o, V3 d,
   V3 geometric = length_sq(winding) > 0.0f ? winding : -incoming;
   bool front_face = dot(winding, d) < 0.0f;
   V3 smooth = v3(0.0f, 0.0f, 0.0f);
-  V3 n0, n1, n2;
   if (mesh.has_normals) {
-    const float* ns = S.normals;
-    n0 = v3(ns[3 * (size_t)i0], ns[3 * (size_t)i0 + 1], ns[3 * (size_t)i0 + 2]);
-    n1 = v3(ns[3 * (size_t)i1], ns[3 * (size_t)i1 + 1], ns[3 * (size_t)i1 + 2]);
-    n2 = v3(ns[3 * (size_t)i2], ns[3 * (size_t)i2 + 1], ns[3 * (size_t)i2 + 2]);
     V3 interp = n0 * b0 + n1 * b1 + n2 * b2;
     smooth = ident ? interp : xf_transpose(in.to_local, interp);
   }
@@ -2391,12 +2405,8 @@ __device__ void boundary_of(const DevScene& S, u32 inst, u32 tri, V3 o, V3 d, V3
   const DevInstance& in = S.instances[inst];
   const DevMesh& mesh = S.meshes[in.mesh];
   bool ident = (in.flags & INST_IDENTITY) != 0;
-  const u32* idx = S.indices + mesh.index_offset + 3 * (size_t)tri;
-  u32 i0 = mesh.vertex_offset + idx[0], i1 = mesh.vertex_offset + idx[1], i2 = mesh.vertex_offset + idx[2];
-  const float* ps = S.positions;
-  V3 p0 = v3(ps[3 * (size_t)i0], ps[3 * (size_t)i0 + 1], ps[3 * (size_t)i0 + 2]);
-  V3 p1 = v3(ps[3 * (size_t)i1], ps[3 * (size_t)i1 + 1], ps[3 * (size_t)i1 + 2]);
-  V3 p2 = v3(ps[3 * (size_t)i2], ps[3 * (size_t)i2 + 1], ps[3 * (size_t)i2 + 2]);
+  V3 p0, p1, p2;
+  corner_positions_of(S, mesh, tri, p0, p1, p2);
   V3 lo = ident ? o : xf_point(in.to_local, o), ld = ident ? d : xf_vector(in.to_local, d);
   PRay r = prepare(lo, ld, RAY_EPSILON, RPX_INF);
   float b0, b1, b2;
@@ -3188,12 +3198,8 @@ __device__ First first_hit(const DevScene& S, u32* lds_col, V3 o, V3 d) {
   const DevInstance& in = S.instances[hi];
   const DevMesh& mesh = S.meshes[in.mesh];
   bool ident = (in.flags & INST_IDENTITY) != 0;
-  const u32* idx = S.indices + mesh.index_offset + 3 * (size_t)ht;
-  u32 i0 = mesh.vertex_offset + idx[0], i1 = mesh.vertex_offset + idx[1], i2 = mesh.vertex_offset + idx[2];
-  const float* ps = S.positions;
-  V3 p0 = v3(ps[3 * (size_t)i0], ps[3 * (size_t)i0 + 1], ps[3 * (size_t)i0 + 2]);
-  V3 p1 = v3(ps[3 * (size_t)i1], ps[3 * (size_t)i1 + 1], ps[3 * (size_t)i1 + 2]);
-  V3 p2 = v3(ps[3 * (size_t)i2], ps[3 * (size_t)i2 + 1], ps[3 * (size_t)i2 + 2]);
+  V3 p0, p1, p2, n0, n1, n2;
+  corners_of(S, mesh, ht, p0, p1, p2, n0, n1, n2);
   V3 lo = ident ? o : xf_point(in.to_local, o), ld = ident ? d : xf_vector(in.to_local, d);
   PRay r = prepare(lo, ld, RAY_EPSILON, RPX_INF);
   float b0, b1, b2;
@@ -3221,10 +3227,6 @@ __device__ First first_hit(const DevScene& S, u32* lds_col, V3 o, V3 d) {
   V3 geometric = length_sq(winding) > 0.0f ? winding : -normalize(d);
   V3 smooth = v3(0.0f, 0.0f, 0.0f);
   if (mesh.has_normals) {
-    const float* ns = S.normals;
-    V3 n0 = v3(ns[3 * (size_t)i0], ns[3 * (size_t)i0 + 1], ns[3 * (size_t)i0 + 2]);
-    V3 n1 = v3(ns[3 * (size_t)i1], ns[3 * (size_t)i1 + 1], ns[3 * (size_t)i1 + 2]);
-    V3 n2 = v3(ns[3 * (size_t)i2], ns[3 * (size_t)i2 + 1], ns[3 * (size_t)i2 + 2]);
     V3 interp = n0 * b0 + n1 * b1 + n2 * b2;
     smooth = ident ? interp : xf_transpose(in.to_local, interp);
   }

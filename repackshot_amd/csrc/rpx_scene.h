@@ -42,6 +42,8 @@ struct alignas(16) DevInstance {
   u32 mesh, blas_root, material, flags;
 };
 
+constexpr u32 CORNER_FLOAT4S = 5;
+
 struct DevMesh {
   u32 vertex_offset, index_offset, has_normals, inside_out;
 };
@@ -241,6 +243,10 @@ struct DevScene {
   const float* positions;
   const float* normals;
   const u32* indices;
+  // Per mesh triangle (mesh index_offset / 3 + triangle), CORNER_FLOAT4S float4s: the corners'
+  // positions, then their normals (zero for a mesh without normals), packed, so that the hit
+  // details read one record instead of the indices and then six scattered vertices.
+  const float4* corners;
   const Bsdf* bsdfs;
   const DevMaterial* materials;
   const DevArea* areas;
@@ -421,6 +427,7 @@ struct FlatScene {
   std::vector<DevMesh> meshes;
   std::vector<float> positions, normals;
   std::vector<u32> indices;
+  std::vector<float4> corners;
   std::vector<Bsdf> bsdfs;
   std::vector<DevMaterial> materials;
   std::vector<DevArea> areas;

@@ -950,6 +950,17 @@ bool flatten_scene(const HostScene& s, const EnergyTables& tables, FlatScene& ou
     if (m.normals.empty()) out.normals.insert(out.normals.end(), m.positions.size(), 0.0f);
     else out.normals.insert(out.normals.end(), m.normals.begin(), m.normals.end());
     out.indices.insert(out.indices.end(), m.indices.begin(), m.indices.end());
+    for (size_t t = 0; t + 2 < m.indices.size(); t += 3) {
+      float c[20] = {};
+      for (int k = 0; k < 3; ++k) {
+        size_t v = 3 * (size_t)m.indices[t + k];
+        for (int a = 0; a < 3; ++a) {
+          c[3 * k + a] = m.positions[v + a];
+          c[9 + 3 * k + a] = m.normals.empty() ? 0.0f : m.normals[v + a];
+        }
+      }
+      for (u32 q = 0; q < CORNER_FLOAT4S; ++q) out.corners.push_back(make_float4(c[4 * q], c[4 * q + 1], c[4 * q + 2], c[4 * q + 3]));
+    }
     out.max_blas_depth = std::max(out.max_blas_depth, bl.depth);
   }
   out.triangles = total_tris;
