@@ -234,6 +234,7 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const DevSegment* chunk, 
   L.shadow_shape = opt.shadow_shape;
   // The drain runs the forward vertices only; the replay keeps the loop.
   L.ext = ext ? 1u : 0u;
+  L.cau = photons ? 1u : 0u;
   L.tail = adj ? 0u : opt.tail;
   L.drain_lanes = std::min(std::max(opt.drain_lanes, 1u), 64u);
   L.park_at = std::min(opt.park_at, PARK_MAX);

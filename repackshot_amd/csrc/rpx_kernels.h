@@ -136,6 +136,7 @@ struct Launch {
   int shadow_shape;  // any-hit launch shape
   u32 tail;          // once no new path is left, at most this many live paths are drained in one launch (0: never)
   u32 ext;           // a scene has sphere lights or scattering media: the extended shade and the walk kernel
+  u32 cau;           // with ext: the scene has caustic photon maps (the shade gathers)
   u32 drain_lanes;   // with ext: lanes per wave of the drain (1..64)
   u32 park_at;       // with ext: paths left when the drain hands them to one wave each (0..PARK_MAX)
   hipStream_t stream;

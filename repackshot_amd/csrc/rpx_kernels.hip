@@ -2315,11 +2315,13 @@ __device__ __forceinline__ void shade_path(const DevScene& S, const PathState& P
 }  // namespace
 
 // Three waves per SIMD: the register budget (168) the shade fits without spilling; two waves leave
-// the loads of a path's records unhidden, four spill. The media and adjoint variants spill at this
-// budget (164 and 83 registers) and not at two waves (222 and 256 registers), but two waves made the
-// caustics render slower (its gather wants the waves) and left the adjoint's time as it was.
-template <bool ADJ, bool EXT>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+// the loads of a path's records unhidden, four spill. The extended shade spills 164 registers at
+// three waves and none at two (222 registers): two waves are faster for scenes with media (the
+// bottle's step 185 -> 180 ms) and slower with caustics, whose gather loops want the waves (1357 ->
+// 1386 ms), so the extended shade has an instance of each (WAVES). The adjoint's spills (83 at
+// three waves, 12 at two) cost it nothing measurable.
+template <bool ADJ, bool EXT, int WAVES = 3>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)))
 k_shade(const DevScene* __restrict__ scenes, PathState P, u32 parity) {
   const u32* q = P.live[parity];
   u32 n = P.counts[C_LIVE0 + parity];
@@ -3620,9 +3622,12 @@ hipError_t launch_shade(const PathState& P, const Launch& L, u32 parity, bool ad
   if (adj)
     hipLaunchKernelGGL((k_shade<true, false>), dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, P,
                        parity);
-  else if (L.ext)
+  else if (L.ext && L.cau)
     hipLaunchKernelGGL((k_shade<false, true>), dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, P,
                        parity);
+  else if (L.ext)
+    hipLaunchKernelGGL((k_shade<false, true, 2>), dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes,
+                       P, parity);
   else
     hipLaunchKernelGGL((k_shade<false, false>), dim3(grid_for(L.pool, 256, 8192)), dim3(256), 0, L.stream, L.scenes, P,
                        parity);
