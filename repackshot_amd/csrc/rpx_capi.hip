@@ -45,6 +45,8 @@ struct Handle {
   u32* d_block_counts = nullptr;
   float *d_pixel_error = nullptr, *d_block_error = nullptr;
   KeptSplat* d_kept_table = nullptr;  // one entry per block
+  // pinned staging of a decision's transfers: block counts up, block errors down, kept-splat table up
+  void* adaptive_pinned = nullptr;
   // samples rendered ahead of the adaptive rounds (float4 radiance + alpha each), and the last
   // adaptive render's work: path samples rendered, of them rendered ahead, runs ahead, unused
   DevBuf kept{};
@@ -68,6 +70,7 @@ struct Handle {
     dev_free_all(film_bufs);
     dev_free_all(tex_bufs);
     dev_free_all(adaptive_bufs);
+    if (adaptive_pinned) hipHostFree(adaptive_pinned);
     if (kept.p) hipFree(kept.p);
     if (stream) hipStreamDestroy(stream);
   }
@@ -548,12 +551,16 @@ static bool ensure_adaptive(Handle* h, size_t px, size_t blocks, std::string& er
   if (px <= h->adaptive_px && blocks <= h->adaptive_blocks) return true;
   hipStreamSynchronize(h->stream);
   dev_free_all(h->adaptive_bufs);
+  if (h->adaptive_pinned) hipHostFree(h->adaptive_pinned);
+  h->adaptive_pinned = nullptr;
   h->adaptive_px = h->adaptive_blocks = 0;
   auto& B = h->adaptive_bufs;
   if (!dev_alloc(B, blocks * 4, (void**)&h->d_block_counts) || !dev_alloc(B, px * 4, (void**)&h->d_pixel_error) ||
       !dev_alloc(B, blocks * 4, (void**)&h->d_block_error) ||
-      !dev_alloc(B, blocks * sizeof(KeptSplat), (void**)&h->d_kept_table)) {
+      !dev_alloc(B, blocks * sizeof(KeptSplat), (void**)&h->d_kept_table) ||
+      hipHostMalloc(&h->adaptive_pinned, blocks * (8 + sizeof(KeptSplat)), hipHostMallocDefault) != hipSuccess) {
     dev_free_all(h->adaptive_bufs);
+    h->adaptive_pinned = nullptr;
     err = "device memory exhausted for the adaptive sampling's error planes";
     return false;
   }
@@ -608,6 +615,9 @@ uint32_t rpx_render_adaptive(uint32_t handle, uint32_t cap, uint64_t* out8) {
   // kept, in sample order, and only blocks chosen without kept samples need a run of their own.
   // The decisions and the film are those of rendering each round's samples alone.
   const bool ahead = h->engine->opt.adaptive_ahead;
+  u32* pinned_counts = (u32*)h->adaptive_pinned;
+  float* pinned_errors = (float*)(pinned_counts + blocks);
+  KeptSplat* pinned_table = (KeptSplat*)(pinned_errors + blocks);
   struct Kept {
     size_t base = 0;  // pixel p's kept samples start at kept + base + p * count
     u32 first = 0, count = 0;
@@ -697,14 +707,16 @@ uint32_t rpx_render_adaptive(uint32_t handle, uint32_t cap, uint64_t* out8) {
       for (u32 b = 0; b < blocks; ++b) ad.counts[b] += ad.quota(b);
       ad.iteration += ad.round;
       ad.done = 0;
-      e = hipMemcpyAsync(h->d_block_counts, ad.counts.data(), blocks * 4, hipMemcpyHostToDevice, st);
+      // (The pinned staging is free again: the previous decision's transfers were waited for.)
+      std::memcpy(pinned_counts, ad.counts.data(), blocks * 4);
+      e = hipMemcpyAsync(h->d_block_counts, pinned_counts, blocks * 4, hipMemcpyHostToDevice, st);
       if (e == hipSuccess)
         e = adaptive::block_errors(W, H, F.rgb, 3, F.lum2, h->d_block_counts, ad.floor, h->d_pixel_error,
                                    h->d_block_error, st);
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(block_error.data(), h->d_block_error, blocks * 4, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipMemcpyAsync(pinned_errors, h->d_block_error, blocks * 4, hipMemcpyDeviceToHost, st);
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       if (e != hipSuccess) break;
+      std::memcpy(block_error.data(), pinned_errors, blocks * 4);
       last = adaptive::choose(W, H, ad.counts, block_error, ad.target, ad.cap, ad.active);
       ad.round = last.round;
       ++decisions;
@@ -767,7 +779,8 @@ uint32_t rpx_render_adaptive(uint32_t handle, uint32_t cap, uint64_t* out8) {
         table.push_back(t);
         paths += (u64)t.seg.w * t.seg.h * t.seg.count;
       }
-      e = hipMemcpyAsync(h->d_kept_table, table.data(), table.size() * sizeof(KeptSplat), hipMemcpyHostToDevice, st);
+      std::memcpy(pinned_table, table.data(), table.size() * sizeof(KeptSplat));
+      e = hipMemcpyAsync(h->d_kept_table, pinned_table, table.size() * sizeof(KeptSplat), hipMemcpyHostToDevice, st);
       if (e == hipSuccess) e = launch_splat_kept(h->d_kept_table, (u32)table.size(), st);
       if (e != hipSuccess) break;
     }
