@@ -39,20 +39,28 @@ __global__ void __launch_bounds__(256) k_pixel_error(u32 width, u32 height, u32 
 }
 
 // adaptive.rs:173-186: the 3 x 3 mean (inside the image, rows then columns) and its largest value
-// per block. The smoothed errors are finite and non-negative, so the largest float is the largest
-// bit pattern.
-__global__ void __launch_bounds__(256) k_block_error(u32 width, u32 height, u32 cols, const float* error,
-                                                     u32* block_error_bits) {
-  u32 n = width * height;
-  for (u32 p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-    u32 x = p % width, y = p / width;
-    u32 y0 = y > 0 ? y - 1 : 0, y1 = y + 2 < height ? y + 2 : height;
-    u32 x0 = x > 0 ? x - 1 : 0, x1 = x + 2 < width ? x + 2 : width;
-    float sum = 0.0f;
-    for (u32 ny = y0; ny < y1; ++ny)
-      for (u32 nx = x0; nx < x1; ++nx) sum += error[(size_t)ny * width + nx];
-    float s = sum / (float)((y1 - y0) * (x1 - x0));
-    atomicMax(block_error_bits + (y / BLOCK) * cols + x / BLOCK, f2u(s));
+// per block. One wave per 8 x 8 block, lane t its pixel (t % 8, t / 8); the smoothed errors are
+// finite and non-negative, so the largest float is the largest bit pattern, taken across the wave
+// by shuffles (lanes outside the image hold 0).
+__global__ void __launch_bounds__(64) k_block_error(u32 width, u32 height, u32 cols, u32 rows, const float* error,
+                                                    u32* block_error_bits) {
+  for (u32 blk = blockIdx.x; blk < cols * rows; blk += gridDim.x) {
+    u32 x = (blk % cols) * BLOCK + threadIdx.x % BLOCK, y = (blk / cols) * BLOCK + threadIdx.x / BLOCK;
+    u32 bits = 0;
+    if (x < width && y < height) {
+      u32 y0 = y > 0 ? y - 1 : 0, y1 = y + 2 < height ? y + 2 : height;
+      u32 x0 = x > 0 ? x - 1 : 0, x1 = x + 2 < width ? x + 2 : width;
+      float sum = 0.0f;
+      for (u32 ny = y0; ny < y1; ++ny)
+        for (u32 nx = x0; nx < x1; ++nx) sum += error[(size_t)ny * width + nx];
+      bits = f2u(sum / (float)((y1 - y0) * (x1 - x0)));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      u32 o = (u32)__shfl_xor((int)bits, off);
+      bits = o > bits ? o : bits;
+    }
+    if (threadIdx.x == 0) block_error_bits[blk] = bits;
   }
 }
 
@@ -61,13 +69,12 @@ hipError_t block_errors(u32 width, u32 height, const float* sums, u32 stride, co
   u32 cols = (width + BLOCK - 1) / BLOCK, rows = (height + BLOCK - 1) / BLOCK;
   u32 n = width * height;
   u32 grid = std::min<u32>((n + 255) / 256, 8192);
-  hipError_t e = hipMemsetAsync(block_error, 0, (size_t)cols * rows * 4, st);
-  if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_pixel_error, dim3(grid), dim3(256), 0, st, width, height, cols, sums, stride, lum2, counts,
                      floor, error);
-  e = hipGetLastError();
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_block_error, dim3(grid), dim3(256), 0, st, width, height, cols, error, (u32*)block_error);
+  hipLaunchKernelGGL(k_block_error, dim3(std::min<u32>(cols * rows, 1u << 16)), dim3(64), 0, st, width, height, cols,
+                     rows, error, (u32*)block_error);
   return hipGetLastError();
 }
 
