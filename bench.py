@@ -115,7 +115,8 @@ def mixed_queue(native, seed=0, jobs=64):
     ids = [q.add(sc, 0, spp) for sc, spp in zip(scenes, spps)]
     tune = q.autotune()
     opts = dict(pool=tune["pool"], shape=tune["shape"], shadow_shape=tune["shadow_shape"])
-    log("bench: queue tuner chose pool %d shapes %d / %d" % (tune["pool"], tune["shape"], tune["shadow_shape"]))
+    log("bench: queue tuner chose pool %d shapes %d / %d trip budget %d"
+        % (tune["pool"], tune["shape"], tune["shadow_shape"], tune["trip_budget"]))
     q.run()  # warm
     q.reset()
     st0 = q.stats()
@@ -142,6 +143,7 @@ def mixed_queue(native, seed=0, jobs=64):
     identical = True
     for sc, spp, film in zip(scenes, spps, films):
         sc.set_options(**opts)
+        sc.set_trip_budget(tune["trip_budget"])
         sc.reset_film()
         sc.render_samples(0, spp)  # warm: the pool of this job's size
         sc.reset_film()
@@ -169,7 +171,8 @@ def mixed_queue(native, seed=0, jobs=64):
             "per_pass_baseline": {"plain_s": round(t_pass, 3), "mpaths_per_s": round(paths / t_pass / 1e6, 3),
                                   "speedup": round(t_pass / t_persistent, 3),
                                   "what": "each job alone, one drained sample pass per call"},
-            "tuned": {"pool": tune["pool"], "shape": tune["shape"], "shadow_shape": tune["shadow_shape"]},
+            "tuned": {"pool": tune["pool"], "shape": tune["shape"], "shadow_shape": tune["shadow_shape"],
+                      "trip_budget": tune["trip_budget"]},
             "films_bit_exact_vs_plain": identical}
 
 
@@ -425,6 +428,7 @@ def adaptive_bench(native, rsb, tuned, target=0.03, cap=64):
     for mode, ahead in (("rounds_only", False), ("ahead", True)):
         scene.set_options(pool=tuned["pool"], shape=tuned["shape"], shadow_shape=tuned["shadow_shape"],
                           adaptive_ahead=ahead)
+        scene.set_trip_budget(tuned["trip_budget"])
         scene.render_adaptive(cap)  # warm-up (pool, planes and kept samples allocated)
         t0 = time.perf_counter()
         st = scene.render_adaptive(cap)
@@ -466,6 +470,7 @@ def media_bench(native, tuned, seed=0, tiles=((944, 320), (944, 480), (912, 640)
     info = scene.info()
     W, H, spp = info["width"], info["height"], info["max_samples"]
     scene.set_options(pool=tuned["pool"], shape=tuned["shape"], shadow_shape=tuned["shadow_shape"])
+    budget = scene.tune_trip_budget()
     scene.render_samples(0, spp)  # warm
     scene.reset_film()
     s0 = scene.stats()
@@ -500,6 +505,7 @@ def media_bench(native, tuned, seed=0, tiles=((944, 320), (944, 480), (912, 640)
            "stage_ms_per_step": {k: round(timing[k], 2) for k in ("extend_ms", "shade_ms", "shadow_ms", "regen_ms",
                                                                    "total_ms")},
            "extend_ms_includes_walks": True,
+           "trip_budget": budget,
            "parity": {"tiles": len(rects), "film_bit_exact_vs_oracle": bool(exact)},
            "cpu_baseline": {"value": round(len(rects) * 1024 * spp / cpu_s / 1e6, 4), "unit": "Mpaths/s",
                             "cores": threads, "kind": "port",
@@ -525,6 +531,7 @@ def caustics_bench(native, tuned, seed=0, tiles=((864, 800), (832, 800), (800, 8
     W, H, spp = info["width"], info["height"], info["max_samples"]
     photons = scene.caustics()
     scene.set_options(pool=tuned["pool"], shape=tuned["shape"], shadow_shape=tuned["shadow_shape"])
+    budget = scene.tune_trip_budget()
     scene.render_samples(spp, 16)  # warm: the path pool and the kept maps' memory
     scene.reset_film()
     s0 = scene.stats()
@@ -566,6 +573,7 @@ def caustics_bench(native, tuned, seed=0, tiles=((864, 800), (832, 800), (800, 8
            "stage_ms_per_step": {k: round(timing[k], 2) for k in ("extend_ms", "shade_ms", "shadow_ms", "regen_ms",
                                                                    "drain_ms", "total_ms")},
            "shade_includes_gather": True,
+           "trip_budget": budget,
            "parity": {"tiles": len(rects), "samples": 2, "film_bit_exact_vs_oracle": bool(exact),
                       "tiles_mean_rgb": round(lit, 4)},
            "cpu_baseline": {"value": round(len(rects) * 1024 * 2 / cpu_s / 1e6, 5), "unit": "Mpaths/s",
@@ -619,13 +627,14 @@ def main():
     if args.shape >= 0:
         scene.set_options(pool=args.pool or (1 << 21), shape=args.shape, shadow_shape=args.shadow_shape)
         tuned = {"pool": args.pool or (1 << 21), "shape": args.shape, "shadow_shape": args.shadow_shape,
-                 "by": "fixed"}
+                 "trip_budget": scene.trip_budget(), "by": "fixed"}
     else:
         t = scene.autotune(spp)
-        tuned = {"pool": t["pool"], "shape": t["shape"], "shadow_shape": t["shadow_shape"], "by": "tuner",
-                 "candidates_ms": t["ms"], "shadow_candidates_ms": t["shadow_ms"]}
-        log("bench: tuner chose pool %d shapes %d / %d (%s %s)" % (t["pool"], t["shape"], t["shadow_shape"], t["ms"],
-                                                                  t["shadow_ms"]))
+        tuned = {"pool": t["pool"], "shape": t["shape"], "shadow_shape": t["shadow_shape"],
+                 "trip_budget": t["trip_budget"], "by": "tuner", "candidates_ms": t["ms"],
+                 "shadow_candidates_ms": t["shadow_ms"]}
+        log("bench: tuner chose pool %d shapes %d / %d trip budget %d (%s %s)"
+            % (t["pool"], t["shape"], t["shadow_shape"], t["trip_budget"], t["ms"], t["shadow_ms"]))
     scene.set_profiling(False)
     for i in range(args.warmup):
         scene.reset_film()
@@ -761,8 +770,11 @@ def main():
             "traffic_frac": round(EXTEND_TRAFFIC_BYTES_PER_RAY * nearest_prof / launches / (extend_avg_ms * 1e-3) / 1e9
                                   / HBM_PEAK_GBS, 4),
             "limiter": "divergence and dependent-fetch latency (17 of 64 lanes active, VALU busy 72 %)",
-            "kernel": "k_extend (closest hit, shape %d): algorithmic bytes of its rays per launch over its own "
-                      "average launch time (HIP events on its stream)" % tuned["shape"],
+            "kernel": ("k_extend (closest hit, shape %d): algorithmic bytes of its rays per launch over its own "
+                       "average launch time (HIP events on its stream)" % tuned["shape"]) if not tuned["trip_budget"]
+                      else ("k_extend_budget + k_extend_resume (the closest hit, shape %d, walks past %d trips "
+                            "finished by the second kernel): algorithmic bytes of its rays per launch pair over the "
+                            "pair's average time (HIP events on its stream)" % (tuned["shape"], tuned["trip_budget"])),
             "bytes_per_ray": round(b_ray, 1),
             "closest_hit_rays_per_launch": round(nearest_prof / launches, 1),
             "extend_avg_ms": round(extend_avg_ms, 4),

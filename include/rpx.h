@@ -145,9 +145,22 @@ uint32_t rpx_set_options(uint32_t handle, uint32_t pool, uint32_t shape, uint32_
                          uint32_t chunk, uint32_t flags, uint32_t tail);
 /* The launch tuner: renders samples [0, samples) of the image (default 32; the film is untouched)
  * with every closest-hit launch shape (8) at pool sizes 2^21 .. 2^26, then every shadow launch
- * shape at the best of those, keeps the fastest as the handle's options, and reports: pool,
- * closest-hit shape, shadow shape, best ms, then the 48 + 8 candidates' ms. */
+ * shape at the best of those, then the closest-hit trip budgets 0 and 48 at the best of those,
+ * keeps the fastest as the handle's options, and reports: pool, closest-hit shape, shadow shape,
+ * best ms of the shape stages, then the 48 + 8 shape candidates' ms (rpx_trip_budget reads the
+ * chosen budget). */
 uint32_t rpx_autotune(uint32_t handle, uint32_t samples, float* out60);
+/* The closest-hit trip budget (no counterpart in the reference, whose CPU walks are not split): a
+ * closest-hit walk that has made `trips` loop trips (node visits and leaves) is set aside, with its
+ * stack, for a second launch that takes such walks to their ends, so that the first launch's
+ * waves are not held by a few long walks. 0: one launch. The hits, and so the films, do not depend
+ * on it (the closest hit is the least (t, instance, triangle), whatever the order of visits).
+ * Default 48; the resident-lane launch shapes (5 .. 7) always run unsplit. rpx_tune_trip_budget
+ * renders samples [0, samples) (default 32) at the handle's launch options with budgets 0 and 48,
+ * keeps the faster and reports it, then both candidates' ms. */
+uint32_t rpx_set_trip_budget(uint32_t handle, uint32_t trips);
+uint32_t rpx_trip_budget(uint32_t handle, uint32_t* out);
+uint32_t rpx_tune_trip_budget(uint32_t handle, uint32_t samples, float* out3);
 /* Records per-stage GPU timing of the following renders (events on the handle's stream). */
 uint32_t rpx_set_profiling(uint32_t handle, uint32_t on);
 /* Timing of the last render: total ms, then ms of the closest-hit, shade, shadow and regen
@@ -203,6 +216,8 @@ uint32_t rpx_queue_set_options(uint32_t queue, uint32_t pool, uint32_t shape, ui
                                uint32_t group, uint32_t chunk, uint32_t flags, uint32_t tail);
 /* The tuner over the queue's jobs' first samples (up to `paths` path samples; 0: 2^25). */
 uint32_t rpx_queue_autotune(uint32_t queue, uint32_t paths, float* out60);
+/* The closest-hit trip budget the queue tuner chose for the queue (as rpx_set_trip_budget; default 48). */
+uint32_t rpx_queue_trip_budget(uint32_t queue, uint32_t* out);
 uint32_t rpx_queue_set_profiling(uint32_t queue, uint32_t on);
 uint32_t rpx_queue_last_timing(uint32_t queue, float* out12);
 uint32_t rpx_queue_stats(uint32_t queue, uint64_t* out4);

@@ -1065,6 +1065,45 @@ uint32_t rpx_scene_stats(uint32_t handle, uint64_t* out4) {
   return RPX_OK;
 }
 
+uint32_t rpx_set_trip_budget(uint32_t handle, uint32_t trips) {
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  h->engine->opt.trip_budget = trips;
+  return RPX_OK;
+}
+
+uint32_t rpx_trip_budget(uint32_t handle, uint32_t* out) {
+  if (!out) return RPX_NULL_POINTER;
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  *out = h->engine->opt.trip_budget;
+  return RPX_OK;
+}
+
+uint32_t rpx_tune_trip_budget(uint32_t handle, uint32_t samples, float* out3) {
+  if (!out3) return RPX_NULL_POINTER;
+  auto h = find(handle);
+  if (!h) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(h->mutex);
+  hipSetDevice(h->device);
+  u32 npix = h->dev.width * h->dev.height;
+  u32 count = std::max<u32>(1, std::min<u32>(samples ? samples : 32, RESULTS_MAX / std::max<u32>(npix, 1)));
+  DevSegment s = whole_image(h.get(), 0, 0, count);
+  std::vector<float> ms;
+  std::string err;
+  prepare_engine(h.get());
+  hipError_t r = h->engine->autotune(h->d_scene, std::vector<DevSegment>(1, s), ms, err, true);
+  if (r != hipSuccess) {
+    set_error(err);
+    return r == hipErrorOutOfMemory ? RPX_GPU_MEMORY : RPX_GPU_LAUNCH;
+  }
+  out3[0] = (float)h->engine->opt.trip_budget;
+  for (int b = 0; b < N_TUNE_BUDGETS; ++b) out3[1 + b] = ms[b];
+  return RPX_OK;
+}
+
 uint32_t rpx_set_options(uint32_t handle, uint32_t pool, uint32_t shape, uint32_t shadow_shape, uint32_t group,
                          uint32_t chunk, uint32_t flags, uint32_t tail) {
   auto h = find(handle);
@@ -1083,7 +1122,7 @@ static uint32_t tune_report(Engine& e, hipError_t r, const std::vector<float>& m
   out[1] = (float)e.opt.shape;
   out[2] = (float)e.opt.shadow_shape;
   out[3] = 0.0f;
-  for (size_t k = 0; k < ms.size(); ++k) {
+  for (size_t k = 0; k < ms.size() && k < (size_t)N_TUNE_CANDIDATES; ++k) {
     out[4 + k] = ms[k];
     if (out[3] == 0.0f || (ms[k] > 0.0f && ms[k] < out[3])) out[3] = ms[k];
   }
@@ -1543,6 +1582,15 @@ uint32_t rpx_queue_autotune(uint32_t queue, uint32_t paths, float* out60) {
   hipError_t r = q->engine->autotune((const DevScene*)d, probe, ms, err);
   hipFree(d);
   return tune_report(*q->engine, r, ms, err, out60);
+}
+
+uint32_t rpx_queue_trip_budget(uint32_t queue, uint32_t* out) {
+  if (!out) return RPX_NULL_POINTER;
+  auto q = find_queue(queue);
+  if (!q) return RPX_BAD_HANDLE;
+  std::lock_guard<std::mutex> lock(q->mutex);
+  *out = q->engine->opt.trip_budget;
+  return RPX_OK;
 }
 
 uint32_t rpx_queue_set_profiling(uint32_t queue, uint32_t on) {

@@ -51,6 +51,8 @@ struct EngineOptions {
   // paths left, 43 ms from 16,384 and 59 ms from all of them; fewer lanes per wave were slower.
   u32 drain_lanes = 64;
   u32 park_at = 16384;
+  // Closest-hit loop trips before a walk is set aside for a second launch (0: one launch).
+  u32 trip_budget = 48;
   // Adaptive rounds: a block's samples up to the cap are rendered in one run once it is a
   // candidate, and kept for the rounds (false: each round renders just its own samples).
   bool adaptive_ahead = true;
@@ -72,6 +74,9 @@ constexpr u32 RESULTS_MAX = 1u << 27;
 constexpr int N_TUNE_POOLS = 6;
 constexpr int N_TUNE_CANDIDATES = N_SHAPES * N_TUNE_POOLS + N_SHAPES;
 constexpr u32 TUNE_POOLS[N_TUNE_POOLS] = {1u << 21, 1u << 22, 1u << 23, 1u << 24, 1u << 25, 1u << 26};
+// Closest-hit trip budgets the tuner tries last, at the chosen launch shapes.
+constexpr int N_TUNE_BUDGETS = 2;
+constexpr u32 TUNE_BUDGETS[N_TUNE_BUDGETS] = {0u, 48u};
 
 class Engine {
  public:
@@ -93,10 +98,12 @@ class Engine {
                      u32 tex_w, u32 tex_h, std::string& err);
   // The tuner: renders `probe` (whose results are discarded; one chunk) with every closest-hit
   // launch shape at each of TUNE_POOLS pool sizes, then every shadow launch shape at the best of
-  // those, keeps the fastest in `opt`, and returns every candidate's GPU time (ms: shape-major
-  // closest-hit candidates, then the shadow candidates).
+  // those, then each of TUNE_BUDGETS at the best of those, keeps the fastest in `opt`, and returns
+  // every candidate's GPU time (ms: shape-major closest-hit candidates, the shadow candidates,
+  // then the budgets). `budget_only`: the current shapes and pool are kept and only the trip
+  // budget is tuned (ms: the budgets' alone).
   hipError_t autotune(const DevScene* d_scenes, const std::vector<DevSegment>& probe, std::vector<float>& ms,
-                      std::string& err);
+                      std::string& err, bool budget_only = false);
   // One sample of a tile with every per-path buffer (pool, results, segment table, a staging
   // tile) carved from `scratch`, `bytes` of device memory the caller owns: no allocation. The
   // tile's channels-per-pixel radiance (3: rgb, 4: rgba) is copied to `out` (host memory, rows
@@ -129,6 +136,7 @@ class Engine {
   bool plan(const std::vector<DevSegment>& segs, std::vector<std::vector<DevSegment>>& chunks, std::vector<u32>& totals,
             std::string& err);
   bool ensure_results(u32 n, std::string& err);
+  bool ensure_cont(u32 n, std::string& err);  // continuation slots for a pool of n (trip budget on)
   bool ensure_segs(u32 n, std::string& err);
   // The photon maps of a chunk's sample passes made current (profiled as stage 7).
   hipError_t prepare_photons(const DevScene* d_scenes, const DevSegment& piece, bool may_allocate, std::string& err);
@@ -141,7 +149,8 @@ class Engine {
   int device_ = 0;
   hipStream_t stream_ = nullptr;
   u32 capacity_ = 0, results_cap_ = 0, segs_cap_ = 0, adjoint_cap_ = 0, bsdf_cap_ = 0, walk_cap_ = 0;
-  std::vector<DevBuf> pool_bufs_, fixed_bufs_, adjoint_bufs_, bsdf_bufs_, walk_bufs_;
+  std::vector<DevBuf> pool_bufs_, fixed_bufs_, adjoint_bufs_, bsdf_bufs_, walk_bufs_, cont_bufs_;
+  u32 cont_cap_ = 0;
   DevBuf results_, segs_;
   u32* idle_host_ = nullptr;  // pinned
   std::vector<hipEvent_t> events_;

@@ -90,6 +90,7 @@ Engine::~Engine() {
   dev_free_all(adjoint_bufs_);
   dev_free_all(bsdf_bufs_);
   dev_free_all(walk_bufs_);
+  dev_free_all(cont_bufs_);
   dev_free_all(fixed_bufs_);
   if (results_.p) hipFree(results_.p);
   if (segs_.p) hipFree(segs_.p);
@@ -175,6 +176,26 @@ bool Engine::ensure_pool(u32 n, std::string& err) {
   return true;
 }
 
+bool Engine::ensure_cont(u32 n, std::string& err) {
+  u32 cap = opt.trip_budget ? std::max<u32>(n / 4, 1024u) : 0u;
+  if (cap <= cont_cap_) return true;
+  hipStreamSynchronize(stream_);
+  dev_free_all(cont_bufs_);
+  P.cont = nullptr;
+  P.cont_stack = nullptr;
+  P.cont_cap = cont_cap_ = 0;
+  if (!dev_alloc(cont_bufs_, (size_t)cap * 2 * sizeof(uint4), (void**)&P.cont) ||
+      !dev_alloc(cont_bufs_, (size_t)cap * CONT_STACK * 4, (void**)&P.cont_stack)) {
+    dev_free_all(cont_bufs_);
+    P.cont = nullptr;
+    P.cont_stack = nullptr;
+    err = "device memory exhausted for the closest-hit continuations";
+    return false;
+  }
+  P.cont_cap = cont_cap_ = cap;
+  return true;
+}
+
 bool Engine::ensure_results(u32 n, std::string& err) {
   if (n <= results_cap_) return true;
   hipStreamSynchronize(stream_);
@@ -238,6 +259,7 @@ hipError_t Engine::run_chunk(const DevScene* d_scenes, const DevSegment* chunk, 
   L.tail = adj ? 0u : opt.tail;
   L.drain_lanes = std::min(std::max(opt.drain_lanes, 1u), 64u);
   L.park_at = std::min(opt.park_at, PARK_MAX);
+  L.budget = (opt.trip_budget && P.cont) ? opt.trip_budget : 0u;
   L.stream = stream_;
   L.side = side_;
   e = launch_pool_init(P, L, total);
@@ -521,7 +543,8 @@ hipError_t Engine::run(const DevScene* d_scenes, const std::vector<DevSegment>& 
     max_total = std::max(max_total, totals[c]);
     max_segs = std::max<u32>(max_segs, (u32)chunks[c].size());
   }
-  if (!ensure_pool(std::min(opt.pool, max_total), err) || !ensure_results(max_total, err) || !ensure_segs(max_segs, err))
+  if (!ensure_pool(std::min(opt.pool, max_total), err) || !ensure_results(max_total, err) ||
+      !ensure_segs(max_segs, err) || !ensure_cont(std::min(opt.pool, max_total), err))
     return hipErrorOutOfMemory;
   hipEventRecord(ev_[0], stream_);
   hipError_t e = hipSuccess;
@@ -629,7 +652,7 @@ hipError_t Engine::adjoint(const DevScene* d_scenes, const std::vector<DevSegmen
 }
 
 hipError_t Engine::autotune(const DevScene* d_scenes, const std::vector<DevSegment>& probe, std::vector<float>& ms,
-                            std::string& err) {
+                            std::string& err, bool budget_only) {
   EngineOptions keep = opt;
   bool keep_profiling = profiling;
   profiling = false;
@@ -652,7 +675,9 @@ hipError_t Engine::autotune(const DevScene* d_scenes, const std::vector<DevSegme
     }
   }
   photons = nullptr;
-  ms.assign(N_TUNE_CANDIDATES, 0.0f);  // 0: the candidate did not fit in device memory
+  // 0: the candidate did not fit in device memory
+  ms.assign(budget_only ? N_TUNE_BUDGETS : N_TUNE_CANDIDATES + N_TUNE_BUDGETS, 0.0f);
+  float* budget_ms = ms.data() + (budget_only ? 0 : N_TUNE_CANDIDATES);
   float best = 0.0f;
   EngineOptions chosen = keep;
   hipError_t e = hipSuccess;
@@ -669,8 +694,8 @@ hipError_t Engine::autotune(const DevScene* d_scenes, const std::vector<DevSegme
     t = timing.total_ms;
     return e == hipSuccess;
   };
-  opt.shadow_shape = 0;
-  for (int sh = 0; sh < N_SHAPES && e == hipSuccess; ++sh)
+  opt.shadow_shape = budget_only ? keep.shadow_shape : 0;
+  for (int sh = 0; sh < N_SHAPES && e == hipSuccess && !budget_only; ++sh)
     for (int p = 0; p < N_TUNE_POOLS && e == hipSuccess; ++p) {
       opt.shape = sh;
       opt.pool = TUNE_POOLS[p];
@@ -684,13 +709,13 @@ hipError_t Engine::autotune(const DevScene* d_scenes, const std::vector<DevSegme
         chosen.shadow_shape = 0;
       }
     }
-  if (e == hipSuccess && best == 0.0f) {
+  if (e == hipSuccess && best == 0.0f && !budget_only) {
     err = "no tuner candidate fits in device memory";
     e = hipErrorOutOfMemory;
   }
   opt.shape = chosen.shape;
   opt.pool = chosen.pool;
-  for (int sh = 0; sh < N_SHAPES && e == hipSuccess; ++sh) {
+  for (int sh = 0; sh < N_SHAPES && e == hipSuccess && !budget_only; ++sh) {
     opt.shadow_shape = sh;
     float t = 0.0f;
     if (!time_it(t)) continue;
@@ -699,6 +724,23 @@ hipError_t Engine::autotune(const DevScene* d_scenes, const std::vector<DevSegme
       best = t;
       chosen.shadow_shape = sh;
     }
+  }
+  // The trip budgets, each timed afresh (the stages above ran at the handle's own budget).
+  opt.shadow_shape = chosen.shadow_shape;
+  float best_budget = 0.0f;
+  for (int b = 0; b < N_TUNE_BUDGETS && e == hipSuccess; ++b) {
+    opt.trip_budget = TUNE_BUDGETS[b];
+    float t = 0.0f;
+    if (!time_it(t)) continue;
+    budget_ms[b] = t;
+    if (best_budget == 0.0f || t < best_budget) {
+      best_budget = t;
+      chosen.trip_budget = TUNE_BUDGETS[b];
+    }
+  }
+  if (e == hipSuccess && best_budget == 0.0f) {
+    err = "no tuner candidate fits in device memory";
+    e = hipErrorOutOfMemory;
   }
   profiling = keep_profiling;
   opt = keep;
@@ -709,6 +751,7 @@ hipError_t Engine::autotune(const DevScene* d_scenes, const std::vector<DevSegme
   opt.shape = chosen.shape;
   opt.shadow_shape = chosen.shadow_shape;
   opt.pool = chosen.pool;
+  opt.trip_budget = chosen.trip_budget;
   return hipSuccess;
 }
 

@@ -66,9 +66,11 @@ struct DevSegment {
 // C_DRAIN_LEFT: paths of the drain not finished yet; C_PARKED: paths the drain handed on to the
 // one-wave-per-path stage (when no more than Launch::park_at of them were left).
 constexpr u32 C_LIVE0 = 0, C_LIVE1 = 1, C_SHADOW0 = 2, C_DONE0 = 3, C_CURSOR = 4, C_END = 5, C_IDLE = 6, C_TAIL = 7,
-              C_SHADOW1 = 8, C_DONE1 = 9, C_DRAIN_LEFT = 10, C_PARKED = 11;
+              C_SHADOW1 = 8, C_DONE1 = 9, C_DRAIN_LEFT = 10, C_PARKED = 11, C_CONT = 12;
 constexpr u32 PARK_MAX = 1u << 16;
 constexpr u32 N_COUNTS = 16;
+// Stack entries a closest-hit walk stopped by its trip budget keeps (a deeper one goes on).
+constexpr int CONT_STACK = 32;
 __host__ __device__ inline u32 c_shadow(u32 parity) { return parity ? C_SHADOW1 : C_SHADOW0; }
 __host__ __device__ inline u32 c_done(u32 parity) { return parity ? C_DONE1 : C_DONE0; }
 
@@ -92,6 +94,12 @@ struct PathState {
   u32* shadow_queue[2];  // shadow rays made by the shade of an iteration of that parity
   u32* done_queue[2];    // slots finished by the shade of an iteration of that parity
   u32* parked;           // PARK_MAX slots the drain hands to one wave each (scenes with media)
+  // Closest-hit walks stopped by their trip budget (C_CONT of them, at most cont_cap): per walk two
+  // uint4 (slot, next entry, best instance, best triangle; best t bits, stack height, instance
+  // walk's height, instance | cull << 30 | object space << 31) and 32 stack entries.
+  uint4* cont;
+  u32* cont_stack;
+  u32 cont_cap;
   u32* counts;     // see C_*
   u64* rays;       // [0] closest-hit rays traced, [1] shadow rays counted without a trace, [2] shadow rays traced,
                    // [3] thin-wall probes after a medium's exit (counted, never needed: no such walls)
@@ -139,6 +147,7 @@ struct Launch {
   u32 cau;           // with ext: the scene has caustic photon maps (the shade gathers)
   u32 drain_lanes;   // with ext: lanes per wave of the drain (1..64)
   u32 park_at;       // with ext: paths left when the drain hands them to one wave each (0..PARK_MAX)
+  u32 budget;        // closest-hit loop trips before a walk is handed to the resume launch (0: one launch)
   hipStream_t stream;
   hipStream_t side;  // the shadow rays' stream (default priority), forked from and joined to `stream`
 };

@@ -185,6 +185,7 @@ struct Stack {
     --sp;
     return sp < LDSN ? lds[sp * BLK] : tail[sp - LDSN];
   }
+  __device__ __forceinline__ u32 at(int e) const { return e < LDSN ? lds[e * BLK] : tail[e - LDSN]; }
 };
 
 __device__ __forceinline__ void order2(u64& a, u64& b) {
@@ -308,6 +309,78 @@ __device__ bool trace_nearest(const Geo& g, u32* lds_col, V3 o, V3 d, float t_mi
   hit_tri = best_tri;
   hit_t = best_t;
   return true;
+}
+
+// A closest-hit walk that may stop after a number of loop trips and go on later (trace_nearest's
+// loop with its state in `w`): the entry about to be visited, the best hit so far, the instance
+// being walked and its stack height, the object-space flag. The hit is the lexicographic minimum of
+// (t, instance, triangle) over the triangles hit, which no order of visits changes, so a walk cut
+// in two gives trace_nearest's result.
+struct WalkState {
+  u32 cur, best_inst, best_tri, inst;
+  float best_t;
+  int tlas_sp;
+  bool local, cull;
+};
+// Runs the walk until it ends (true) or `budget` trips have been made (false, `w` and `st` hold
+// where it stands). `ray` is the ray in the space of w (world, or the instance's when w.local).
+template <int BLK, int LDSN>
+__device__ bool walk_nearest(const Geo& g, Stack<BLK, LDSN>& st, V3 o, V3 d, float t_min, float t_max, bool camera,
+                             PRay& ray, WalkState& w, u32 budget) {
+  for (u32 trips = 0;; ++trips) {
+    if (trips == budget) return false;
+    u32 cur = w.cur;
+    if (cur != REF_EMPTY) {
+      if (!(cur & REF_LEAF)) {
+        float4 near;
+        uint4 ch;
+        u32 bits = node_test(g.nodes + cur, ray, w.best_t, near, ch);
+        if (bits) {
+          w.cur = push_ordered(bits, near, ch, st);
+          continue;
+        }
+      } else if (w.tlas_sp < 0) {
+        u32 i = (cur & ~REF_LEAF) >> LEAF_COUNT_BITS;
+        const DevInstance& in = g.instances[i];
+        u32 fl = in.flags;
+        if ((fl & INST_ACTIVE) && !(camera && (fl & INST_CAMERA_HIDDEN))) {
+          w.inst = i;
+          w.cull = (fl & INST_ONE_SIDED) != 0;
+          if (!(fl & INST_IDENTITY)) {
+            V3 lo = xf_point(in.to_local, o), ld = xf_vector(in.to_local, d);
+            ray = prepare(lo, ld, t_min, t_max);
+            w.local = true;
+          }
+          w.tlas_sp = st.sp;
+          w.cur = in.blas_root;
+          continue;
+        }
+      } else {
+        u32 first = (cur & ~REF_LEAF) >> LEAF_COUNT_BITS, count = (cur & 7u) + 1;
+        const float4* tp = g.tris + 3 * (size_t)first;
+        for (u32 k = 0; k < count; ++k, tp += 3) {
+          float4 va = tp[0], vb = tp[1], vc = tp[2];
+          float t;
+          if (watertight(ray, va, vb, vc, w.best_t, w.cull, t)) {
+            u32 tri = f2u(va.w);
+            if (t < w.best_t ||
+                (t == w.best_t && (w.inst < w.best_inst || (w.inst == w.best_inst && tri < w.best_tri)))) {
+              w.best_t = t;
+              w.best_inst = w.inst;
+              w.best_tri = tri;
+            }
+          }
+        }
+      }
+    }
+    if (w.tlas_sp >= 0 && st.sp == w.tlas_sp) {
+      w.tlas_sp = -1;
+      if (w.local) ray = prepare(o, d, t_min, t_max);
+      w.local = false;
+    }
+    if (st.sp == 0) return true;
+    w.cur = st.pop();
+  }
 }
 
 // Any hit in [t_min, t_max] (traverse.rs:742-802).
@@ -801,6 +874,7 @@ __global__ void k_bookkeep(PathState P, u32 parity, u32 tail) {
   P.counts[c_shadow(parity)] = 0;
   P.counts[C_TAIL] = 0;
   P.counts[C_PARKED] = 0;
+  P.counts[C_CONT] = 0;
   P.rays[0] += live;
   u32 pending = P.counts[c_done(parity ^ 1)] + P.counts[c_shadow(parity ^ 1)];
   if (live == 0) {
@@ -829,6 +903,114 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
     float t = RPX_INF;
     trace_nearest<BLK, LDSN>(g, col, v3(o.x, o.y, o.z), v3(d.x, d.y, d.z), RAY_EPSILON, RPX_INF, primary, inst, tri, t);
     P.hit[i] = make_uint4(inst, tri, f2u(t), 0u);
+  }
+}
+
+// The closest hit in two launches: every walk runs at most `budget` loop trips; the walks still
+// going then take continuation slots (their state and stack saved, one atomic per wave) and
+// k_extend_resume finishes them in waves of their own, so a wave of the first launch is not held
+// by its longest walks. A walk that finds no slot (or a stack deeper than CONT_STACK) goes on where
+// it is. The hits are trace_nearest's.
+template <int BLK, int LDSN, int WPE>
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_extend_budget(const DevScene* __restrict__ scenes, PathState P, u32 parity, u32 budget) {
+  __shared__ u32 lds[LDSN * BLK];
+  u32* col = lds + threadIdx.x;
+  const u32* q = P.live[parity];
+  u32 n = P.counts[C_LIVE0 + parity];
+  u32 stride = gridDim.x * blockDim.x;
+  // Whole waves iterate together (the continuation slots are taken per wave).
+  u32 n_iter = (n + stride - 1) / stride;
+  u32 gid = blockIdx.x * blockDim.x + threadIdx.x;
+  for (u32 it = 0; it < n_iter; ++it) {
+    u32 j = it * stride + gid;
+    bool active = j < n, stopped = false;
+    u32 i = 0;
+    Stack<BLK, LDSN> st;
+    st.lds = col;
+    st.sp = 0;
+    WalkState w{REF_EMPTY, 0xffffffffu, 0xffffffffu, 0u, RPX_INF, -1, false, false};
+    V3 o = v3(0.0f, 0.0f, 0.0f), d = o;
+    bool primary = false;
+    Geo g{nullptr, nullptr, nullptr, REF_EMPTY};
+    PRay ray;
+    if (active) {
+      i = q[j];
+      float4 o4 = P.ray_o[i], d4 = P.ray_d[i];
+      o = v3(o4.x, o4.y, o4.z);
+      d = v3(d4.x, d4.y, d4.z);
+      primary = (f2u(P.med[i].w) & FLAG_PRIMARY) != 0;
+      g = geo_of(scenes, P.ident[i].y);
+      if (usable_ray(o, d)) {
+        ray = prepare(o, d, RAY_EPSILON, RPX_INF);
+        if (ray.t_min <= ray.t_max) {
+          w.cur = g.root;
+          w.best_t = ray.t_max;
+          stopped = !walk_nearest<BLK, LDSN>(g, st, o, d, RAY_EPSILON, RPX_INF, primary, ray, w, budget);
+        }
+      }
+    }
+    const u64 m = __ballot(stopped && st.sp <= CONT_STACK);
+    u32 base = 0;
+    if (m) {
+      const int leader = (int)__ffsll((long long)m) - 1;
+      if ((int)__lane_id() == leader) base = atomicAdd(&P.counts[C_CONT], (u32)__popcll(m));
+      base = (u32)__shfl((int)base, leader);
+    }
+    if (stopped) {
+      const u32 k = base + __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
+      if (st.sp <= CONT_STACK && k < P.cont_cap) {
+        P.cont[2 * (size_t)k] = make_uint4(i, w.cur, w.best_inst, w.best_tri);
+        P.cont[2 * (size_t)k + 1] = make_uint4(f2u(w.best_t), (u32)st.sp, (u32)w.tlas_sp,
+                                               w.inst | (w.cull ? 1u << 30 : 0u) | (w.local ? 1u << 31 : 0u));
+        u32* sv = P.cont_stack + (size_t)CONT_STACK * k;
+        for (int e = 0; e < st.sp; ++e) sv[e] = st.at(e);
+      } else {
+        walk_nearest<BLK, LDSN>(g, st, o, d, RAY_EPSILON, RPX_INF, primary, ray, w, 0xffffffffu);
+        stopped = false;
+      }
+    }
+    if (active && !stopped) {
+      bool hit = w.best_inst != 0xffffffffu;
+      P.hit[i] = make_uint4(hit ? w.best_inst : 0xffffffffu, hit ? w.best_tri : 0xffffffffu,
+                            f2u(hit ? w.best_t : RPX_INF), 0u);
+    }
+  }
+}
+
+// The walks k_extend_budget stopped, each to its end.
+template <int BLK, int LDSN, int WPE>
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_extend_resume(const DevScene* __restrict__ scenes, PathState P) {
+  __shared__ u32 lds[LDSN * BLK];
+  u32* col = lds + threadIdx.x;
+  const u32 count = P.counts[C_CONT];
+  const u32 n = count < P.cont_cap ? count : P.cont_cap;
+  u32 stride = gridDim.x * blockDim.x;
+  for (u32 k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    uint4 a = P.cont[2 * (size_t)k], b = P.cont[2 * (size_t)k + 1];
+    const u32 i = a.x;
+    float4 o4 = P.ray_o[i], d4 = P.ray_d[i];
+    V3 o = v3(o4.x, o4.y, o4.z), d = v3(d4.x, d4.y, d4.z);
+    const bool primary = (f2u(P.med[i].w) & FLAG_PRIMARY) != 0;
+    Geo g = geo_of(scenes, P.ident[i].y);
+    WalkState w{a.y, a.z, a.w, b.w & 0x3fffffffu, u2f(b.x), (int)b.z, (b.w >> 31) != 0, ((b.w >> 30) & 1u) != 0};
+    Stack<BLK, LDSN> st;
+    st.lds = col;
+    st.sp = 0;
+    const u32* sv = P.cont_stack + (size_t)CONT_STACK * k;
+    for (u32 e = 0; e < b.y; ++e) st.push(sv[e]);
+    PRay ray;
+    if (w.local) {
+      const DevInstance& in = g.instances[w.inst];
+      ray = prepare(xf_point(in.to_local, o), xf_vector(in.to_local, d), RAY_EPSILON, RPX_INF);
+    } else {
+      ray = prepare(o, d, RAY_EPSILON, RPX_INF);
+    }
+    walk_nearest<BLK, LDSN>(g, st, o, d, RAY_EPSILON, RPX_INF, primary, ray, w, 0xffffffffu);
+    bool hit = w.best_inst != 0xffffffffu;
+    P.hit[i] = make_uint4(hit ? w.best_inst : 0xffffffffu, hit ? w.best_tri : 0xffffffffu,
+                          f2u(hit ? w.best_t : RPX_INF), 0u);
   }
 }
 
@@ -3595,6 +3777,21 @@ static u32 resident_blocks(K kernel, int block) {
 }
 
 hipError_t launch_extend(const PathState& P, const Launch& L, u32 parity) {
+  if (L.budget && P.cont) {
+    switch (L.shape) {
+#define X(I, B, N, W)                                                                                             \
+  case I:                                                                                                        \
+    hipLaunchKernelGGL((k_extend_budget<B, N, W>), dim3(grid_for(L.pool, B, 1u << 20)), dim3(B), 0, L.stream,     \
+                       L.scenes, P, parity, L.budget);                                                          \
+    hipLaunchKernelGGL((k_extend_resume<B, N, W>), dim3(grid_for(P.cont_cap, B, 1u << 16)), dim3(B), 0, L.stream, \
+                       L.scenes, P);                                                                             \
+    return hipGetLastError();
+      RPX_TRAVERSAL_SHAPES(X)
+#undef X
+      default:
+        break;  // the resident-lane shapes run unsplit
+    }
+  }
   switch (L.shape) {
 #define X(I, B, N, W)                                                                                            \
   case I:                                                                                                       \

@@ -57,6 +57,10 @@ _SIGS = {
     "rpx_scene_stats": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_set_options": (ctypes.c_uint32, [ctypes.c_uint32] * 8),
     "rpx_autotune": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
+    "rpx_set_trip_budget": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
+    "rpx_trip_budget": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
+    "rpx_tune_trip_budget": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
+    "rpx_queue_trip_budget": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_set_profiling": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
     "rpx_last_timing": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_void_p]),
     "rpx_energy_tables": (ctypes.c_uint32, [ctypes.c_void_p] * 6),
@@ -484,10 +488,30 @@ class Scene:
         return gm, gt
 
     def autotune(self, samples=32):
-        """Runs the launch tuner; returns the chosen pool, shape and every candidate's ms."""
+        """Runs the launch tuner; returns the chosen pool, shapes, trip budget and every shape
+        candidate's ms."""
         out = np.zeros(60, np.float32)
         _check(library().rpx_autotune(self.handle, samples, _ptr(out)), "rpx_autotune")
-        return _tune(out)
+        t = _tune(out)
+        t["trip_budget"] = self.trip_budget()
+        return t
+
+    def set_trip_budget(self, trips):
+        """Closest-hit loop trips before a walk is set aside for a second launch (0: one launch);
+        the film does not depend on it."""
+        _check(library().rpx_set_trip_budget(self.handle, trips), "rpx_set_trip_budget")
+
+    def trip_budget(self):
+        out = ctypes.c_uint32()
+        _check(library().rpx_trip_budget(self.handle, ctypes.byref(out)), "rpx_trip_budget")
+        return int(out.value)
+
+    def tune_trip_budget(self, samples=32):
+        """Times samples [0, samples) at the current launch options with each trip budget the
+        tuner knows, keeps the faster; returns it and each candidate's ms."""
+        out = np.zeros(3, np.float32)
+        _check(library().rpx_tune_trip_budget(self.handle, samples, _ptr(out)), "rpx_tune_trip_budget")
+        return {"trip_budget": int(out[0]), "ms": {0: round(float(out[1]), 3), 48: round(float(out[2]), 3)}}
 
     def set_profiling(self, on=True):
         _check(library().rpx_set_profiling(self.handle, 1 if on else 0), "rpx_set_profiling")
@@ -640,7 +664,11 @@ class Queue:
     def autotune(self, paths=0):
         out = np.zeros(60, np.float32)
         _check(library().rpx_queue_autotune(self.handle, paths, _ptr(out)), "rpx_queue_autotune")
-        return _tune(out)
+        t = _tune(out)
+        b = ctypes.c_uint32()
+        _check(library().rpx_queue_trip_budget(self.handle, ctypes.byref(b)), "rpx_queue_trip_budget")
+        t["trip_budget"] = int(b.value)
+        return t
 
     def set_profiling(self, on=True):
         _check(library().rpx_queue_set_profiling(self.handle, 1 if on else 0), "rpx_queue_set_profiling")

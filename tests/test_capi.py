@@ -48,6 +48,11 @@ def test_status_codes_without_a_scene(rpx):
     assert lib.rt_scene_free(12345) == 1
     assert lib.rt_scene_load(None, 10) == 0
     assert "null pointer" in rpx.last_error()
+    b = np.zeros(3, np.float32)
+    assert lib.rpx_set_trip_budget(12345, 48) == 1                              # bad handle
+    assert lib.rpx_trip_budget(12345, None) == 3                                # null pointer
+    assert lib.rpx_tune_trip_budget(12345, 8, b.ctypes.data) == 1
+    assert lib.rpx_queue_trip_budget(12345, b.ctypes.data) == 1
 
 
 def test_load_without_a_gpu_fails_loudly(rpx, example_rsb):
