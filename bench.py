@@ -47,10 +47,11 @@ HBM_PEAK_GBS = 8000.0  # MI355X datasheet: 8 TB/s HBM3E
 # geometry/triangle.rs:160-167: v [3][3][4] f32 + id [4] u32 = 160 B).
 NODE_BYTES = 256
 PACKET_BYTES = 160
-# Memory-side bytes of one k_extend dispatch per closest-hit ray: not measured in this run but taken
-# from rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this workload (FETCH_SIZE scaled by 2.0,
-# WRITE_SIZE by 1.0, the factors a 256 MiB device copy gives), recorded in TRAFFIC_SOURCE.
-EXTEND_TRAFFIC_BYTES_PER_RAY = 242.0
+# Memory-side bytes of one closest-hit launch per closest-hit ray, by trip budget (0: k_extend
+# alone; 48: k_extend_budget + k_extend_resume): not measured in this run but taken from rocprofv3
+# FETCH_SIZE and WRITE_SIZE runs of this workload (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0, the
+# factors a 256 MiB device copy gives), recorded in TRAFFIC_SOURCE.
+EXTEND_TRAFFIC_BYTES_PER_RAY = {0: 241.7, 48: 259.3}
 TRAFFIC_SOURCE = ("profile-derived constant per closest-hit ray, not measured in this run: "
                   "profiles/hbm_traffic_large_render_1024_64spp.md")
 
@@ -751,6 +752,7 @@ def main():
             counts["nearest"], 1)
         b_any = (NODE_BYTES * counts["any_nodes"] + PACKET_BYTES * counts["any_packets"]) / max(counts["any"], 1)
         launches = max(kernels["extend_launches"], 1)
+        traffic_per_ray = EXTEND_TRAFFIC_BYTES_PER_RAY.get(tuned["trip_budget"])
         ext_bytes = b_ray * nearest_prof / launches
         shadow_bytes = b_any * shadow_prof / max(kernels["shadow_launches"], 1)
         achieved = ext_bytes / (extend_avg_ms * 1e-3) / 1e9
@@ -762,13 +764,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": round(EXTEND_TRAFFIC_BYTES_PER_RAY * nearest_prof / launches),
+            "traffic": None if traffic_per_ray is None else round(traffic_per_ray * nearest_prof / launches),
             "traffic_source": TRAFFIC_SOURCE,
             # What the kernel moves from memory per launch over its time: the L2 and the Infinity
             # Cache serve most node fetches, so the kernel is held by divergence and dependent-fetch
             # latency, well under the HBM roof it is priced against.
-            "traffic_frac": round(EXTEND_TRAFFIC_BYTES_PER_RAY * nearest_prof / launches / (extend_avg_ms * 1e-3) / 1e9
-                                  / HBM_PEAK_GBS, 4),
+            "traffic_frac": None if traffic_per_ray is None else round(
+                traffic_per_ray * nearest_prof / launches / (extend_avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "limiter": "divergence and dependent-fetch latency (17 of 64 lanes active, VALU busy 72 %)",
             "kernel": ("k_extend (closest hit, shape %d): algorithmic bytes of its rays per launch over its own "
                        "average launch time (HIP events on its stream)" % tuned["shape"]) if not tuned["trip_budget"]
