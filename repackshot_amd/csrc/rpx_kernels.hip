@@ -3762,7 +3762,7 @@ hipError_t launch_finish(const PathState& P, const Launch& L, u32 parity) {
   return hipGetLastError();
 }
 
-#define RPX_TRAVERSAL_SHAPES(X) X(0, 128, 24, 1) X(1, 128, 32, 1) X(2, 128, 16, 1) X(3, 256, 16, 1) X(4, 128, 16, 8)
+#define RPX_TRAVERSAL_SHAPES(X) X(0, 128, 24, 1) X(1, 128, 32, 1) X(2, 128, 16, 7) X(3, 256, 16, 1) X(4, 128, 16, 8)
 #define RPX_FLAT_SHAPES(X) X(5, 128, 24, 1) X(6, 128, 16, 1) X(7, 256, 16, 1)
 
 // Blocks of `kernel` the GPU keeps resident at once (all CUs), for the resident-lane kernels.
