@@ -51,7 +51,7 @@ PACKET_BYTES = 160
 # alone; 48: k_extend_budget + k_extend_resume): not measured in this run but taken from rocprofv3
 # FETCH_SIZE and WRITE_SIZE runs of this workload (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0, the
 # factors a 256 MiB device copy gives), recorded in TRAFFIC_SOURCE.
-EXTEND_TRAFFIC_BYTES_PER_RAY = {0: 241.7, 48: 259.3}
+EXTEND_TRAFFIC_BYTES_PER_RAY = {0: 245.7, 48: 271.6}
 TRAFFIC_SOURCE = ("profile-derived constant per closest-hit ray, not measured in this run: "
                   "profiles/hbm_traffic_large_render_1024_64spp.md")
 
@@ -608,6 +608,7 @@ def main():
     ap.add_argument("--pool", type=int, default=0)
     ap.add_argument("--shape", type=int, default=-1)
     ap.add_argument("--shadow-shape", type=int, default=0)
+    ap.add_argument("--trip-budget", type=int, default=48, help="closest-hit trip budget with --shape")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this tracer runs on one GPU; --gpus %d is not supported (one process, one GPU)"
@@ -632,6 +633,7 @@ def main():
 
     if args.shape >= 0:
         scene.set_options(pool=args.pool or (1 << 21), shape=args.shape, shadow_shape=args.shadow_shape)
+        scene.set_trip_budget(args.trip_budget)
         tuned = {"pool": args.pool or (1 << 21), "shape": args.shape, "shadow_shape": args.shadow_shape,
                  "trip_budget": scene.trip_budget(), "by": "fixed"}
     else:
@@ -797,6 +799,10 @@ def main():
         }
     if not args.no_denoise:
         result["denoise"] = denoise_bench(scene, spp)
+    # Each later workload loads its own scene after this one is closed (its pool, results and
+    # continuation buffers freed): a scene loaded while they were held measured its adaptive render
+    # 11 % slower for its whole life (61.5 against 55.2 ms), a placement effect in device memory.
+    scene.close()
     if not args.no_adaptive:
         result["adaptive"] = adaptive_bench(native, rsb, tuned)
     if not args.no_media:
