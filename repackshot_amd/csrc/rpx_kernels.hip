@@ -3730,6 +3730,12 @@ __global__ void k_energy_avg(const uint16_t* albedo, const uint16_t* schlick, fl
 }
 
 // ================================================================== launch wrappers
+// Most blocks of a traversal launch: its grid strides over the live queue, so 2^17 blocks (16.8M
+// lanes at 128) cover any pool while an iteration with few rays left does not pay for dispatching
+// a block per slot of a 2^26 pool (about 110 us per launch; large render 49.5 -> 48.0 ms per step,
+// 48.2 ms at 2^15 blocks and 48.8 at 2^14).
+constexpr u32 TRAVERSAL_BLOCKS = 1u << 17;
+
 static inline u32 grid_for(u32 n, u32 block, u32 cap) {
   u32 g = (n + block - 1) / block;
   if (g < 1) g = 1;
@@ -3786,7 +3792,7 @@ hipError_t launch_extend(const PathState& P, const Launch& L, u32 parity) {
     switch (L.shape) {
 #define X(I, B, N, W)                                                                                             \
   case I:                                                                                                        \
-    hipLaunchKernelGGL((k_extend_budget<B, N, W>), dim3(grid_for(L.pool, B, 1u << 20)), dim3(B), 0, L.stream,     \
+    hipLaunchKernelGGL((k_extend_budget<B, N, W>), dim3(grid_for(L.pool, B, TRAVERSAL_BLOCKS)), dim3(B), 0, L.stream,     \
                        L.scenes, P, parity, L.budget);                                                          \
     hipLaunchKernelGGL((k_extend_resume<B, N, W>), dim3(grid_for(P.cont_cap, B, 1u << 16)), dim3(B), 0, L.stream, \
                        L.scenes, P);                                                                             \
@@ -3800,7 +3806,7 @@ hipError_t launch_extend(const PathState& P, const Launch& L, u32 parity) {
   switch (L.shape) {
 #define X(I, B, N, W)                                                                                            \
   case I:                                                                                                       \
-    hipLaunchKernelGGL((k_extend<B, N, W>), dim3(grid_for(L.pool, B, 1u << 20)), dim3(B), 0, L.stream, L.scenes, P, \
+    hipLaunchKernelGGL((k_extend<B, N, W>), dim3(grid_for(L.pool, B, TRAVERSAL_BLOCKS)), dim3(B), 0, L.stream, L.scenes, P, \
                        parity);                                                                                 \
     break;
     RPX_TRAVERSAL_SHAPES(X)
@@ -3841,14 +3847,14 @@ hipError_t launch_walk(const PathState& P, const Launch& L, u32 parity) {
 }
 hipError_t launch_shadow(const PathState& P, const Launch& L, u32 parity, bool adj) {
   if (adj) {
-    hipLaunchKernelGGL((k_shadow<128, 24, 1, true>), dim3(grid_for(L.pool, 128, 1u << 20)), dim3(128), 0, L.side,
+    hipLaunchKernelGGL((k_shadow<128, 24, 1, true>), dim3(grid_for(L.pool, 128, TRAVERSAL_BLOCKS)), dim3(128), 0, L.side,
                        L.scenes, P, parity);
     return hipGetLastError();
   }
   switch (L.shadow_shape) {
 #define X(I, B, N, W)                                                                                             \
   case I:                                                                                                        \
-    hipLaunchKernelGGL((k_shadow<B, N, W, false>), dim3(grid_for(L.pool, B, 1u << 20)), dim3(B), 0, L.side, L.scenes, \
+    hipLaunchKernelGGL((k_shadow<B, N, W, false>), dim3(grid_for(L.pool, B, TRAVERSAL_BLOCKS)), dim3(B), 0, L.side, L.scenes, \
                        P, parity);                                                                               \
     break;
     RPX_TRAVERSAL_SHAPES(X)
