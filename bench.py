@@ -51,7 +51,7 @@ PACKET_BYTES = 160
 # alone; 48: k_extend_budget + k_extend_resume): not measured in this run but taken from rocprofv3
 # FETCH_SIZE and WRITE_SIZE runs of this workload (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0, the
 # factors a 256 MiB device copy gives), recorded in TRAFFIC_SOURCE.
-EXTEND_TRAFFIC_BYTES_PER_RAY = {0: 245.7, 48: 271.6}
+EXTEND_TRAFFIC_BYTES_PER_RAY = {0: 245.9, 48: 266.6}
 TRAFFIC_SOURCE = ("profile-derived constant per closest-hit ray, not measured in this run: "
                   "profiles/hbm_traffic_large_render_1024_64spp.md")
 
